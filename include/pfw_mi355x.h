@@ -1,0 +1,117 @@
+/*
+ * pfw_mi355x.h — C ABI of the MI355X (gfx950) force-evaluation hot path of
+ * PotentialFrameWork.jl: triclinic multi-image cell lists, SmoothPair and
+ * Lennard-Jones energy/force/virial, and the ACE site pipeline.
+ *
+ * Conventions (the reference's, src/submodules/PairPotentials/evaluation.jl and
+ * README.md "conventions"): lengths in Å, energies in eV, forces in eV/Å, virial
+ * in eV (configurational, W = -sum d (x) g). Positions are 3 x N column-major
+ * doubles in ORIGINAL atom order; the cell H is 3 x 3 column-major with the
+ * lattice vectors in COLUMNS; pbc is three 0/1 bytes. Forces come back in
+ * original order. Every entry point returns a pfw_status; pfw_last_error()
+ * gives the message of the last failure on that context.
+ *
+ * Ownership and threading: the caller owns every host buffer; the library owns
+ * device memory. A context is NOT safe for concurrent calls (the reference's
+ * Workspace rule, README.md:217-219). Results are written before return.
+ */
+#ifndef PFW_MI355X_H
+#define PFW_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  PFW_OK = 0,
+  PFW_INVALID_ARGUMENT = 1, /* Julia ArgumentError */
+  PFW_DIMENSION = 2,        /* Julia DimensionMismatch */
+  PFW_DOMAIN = 3,           /* Julia DomainError: singular or nonfinite model output */
+  PFW_STALE = 4,            /* Julia ErrorException: stale topology / model */
+  PFW_CAPACITY = 5,         /* a caller-supplied capacity is too small */
+  PFW_DEVICE = 6            /* HIP runtime failure */
+} pfw_status;
+
+/* Request mask, as AtomsCalculators Energy()/Forces()/Virial() tuples
+ * (src/submodules/NeighbourList/execution.jl:4-12). Any subset, including 0. */
+enum { PFW_ENERGY = 1, PFW_FORCES = 2, PFW_VIRIAL = 4 };
+
+/* Arithmetic of the pair and ACE kernels. FP64: plain double everywhere.
+ * MIXED: float per-pair / per-edge arithmetic, double force, energy and virial
+ * accumulation; neighbour membership is still decided bit-exactly in double. */
+enum { PFW_FP64 = 0, PFW_MIXED = 1 };
+
+/* Pair model kinds. */
+enum { PFW_PAIR_SMOOTH = 0, PFW_PAIR_LJ = 1 };
+
+typedef struct pfw_ctx pfw_ctx;
+
+/* Replaces constructing NeighbourList.Workspace / GPUWorkspace
+ * (src/submodules/NeighbourList/storage.jl:59-65, gpu.jl:82-116). */
+int pfw_ctx_create(int device, pfw_ctx** out);
+void pfw_ctx_destroy(pfw_ctx* ctx);
+const char* pfw_last_error(const pfw_ctx* ctx);
+
+/* Geometry upload and validation: the extraction half of prepare!
+ * (src/submodules/NeighbourList/storage.jl:130-185, periodic_basis :83-108).
+ * species are channel ids 0..nz-1, already mapped from ChemicalSpecies by the
+ * host (bind_species, src/submodules/PairPotentials/models.jl:50-58). The cell
+ * grid is fixed here for the given cutoff; positions may later be replaced
+ * with pfw_update_positions without changing the cell. */
+int pfw_set_system(pfw_ctx* ctx, int64_t n, const double* X, const int32_t* species,
+                   const double* H, const uint8_t* pbc, double cutoff);
+/* New positions for the same atoms and cell (host buffer). */
+int pfw_update_positions(pfw_ctx* ctx, const double* X);
+
+/* Pair model (NI.prepare_potential for SmoothPair / RadialPotential{LennardJonesRadial},
+ * src/submodules/PairPotentials/models.jl:145-176, lennard_jones.jl:5-22;
+ * GPU certification prepare_gpu_potential, gpu.jl:15-36). params:
+ *   PFW_PAIR_SMOOTH: nz*nz symmetric amplitude table, column-major (eV)
+ *   PFW_PAIR_LJ:     {epsilon (eV), sigma (Å)}; nz ignored.
+ * The cutoff must equal the one given to pfw_set_system. */
+int pfw_set_pair_model(pfw_ctx* ctx, int kind, int nz, const double* params, double cutoff);
+
+/* One complete step on the device: fresh cell build, pair E/F/V, FP64
+ * reductions and the nonfinite status check; outputs stay on the device.
+ * Replaces gpu_execute! (src/submodules/NeighbourList/gpu.jl:363-402) and the
+ * CPU execute! (src/submodules/NeighbourList/execution.jl:223-238). */
+int pfw_pair_step(pfw_ctx* ctx, uint32_t request_mask, int precision);
+/* Copy the last step's outputs to the host (any pointer may be NULL).
+ * E: 1 double; F: 3 x N column-major, original order; W: 3 x 3 column-major. */
+int pfw_read_results(pfw_ctx* ctx, double* E, double* F, double* W);
+/* pfw_pair_step + pfw_read_results: the evaluate! boundary
+ * (src/submodules/PairPotentials/evaluation.jl:31-52; gpu.jl:45-55). */
+int pfw_pair_evaluate(pfw_ctx* ctx, uint32_t request_mask, int precision, double* E,
+                      double* F, double* W);
+
+/* Canonical directed neighbour set of the current system, in original order:
+ * for every directed image (i -> j, S) with R = x_j - x_i + H S strictly inside
+ * the cutoff, excluding (i, i, 0). Replaces neighbour_list / materialize /
+ * visit_directed_images! (src/submodules/NeighbourList/interface.jl:95-171).
+ * i, j are 0-based; S is 3 ints per pair; R is 3 doubles per pair (either may
+ * be NULL). If cap is too small, *count receives the size and PFW_CAPACITY is
+ * returned. Pairs are ordered by (i, then traversal). */
+int pfw_export_pairs(pfw_ctx* ctx, int32_t* i, int32_t* j, int32_t* S, double* R, int64_t cap,
+                     int64_t* count);
+
+/* Diagnostics: number of cells and the cell grid (dims[3]) of the current
+ * system, and whether the minimum-image path is active (storage.jl:140). */
+int pfw_grid_info(pfw_ctx* ctx, int64_t* ncells, int32_t* dims, int32_t* minimum_image);
+
+/* Per-stage device times of the last step in ms, measured with HIP events on
+ * the context's stream when timing is enabled (the reference's w.timings
+ * dictionary, storage.jl:185-236). stages: 0 bin, 1 scan, 2 scatter+sort,
+ * 3 interaction, 4 reduction. Returns the number written. */
+int pfw_set_timing(pfw_ctx* ctx, int enabled);
+int pfw_get_timings(pfw_ctx* ctx, double* ms, int n);
+
+/* Synchronise the context's stream. */
+int pfw_synchronize(pfw_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PFW_MI355X_H */

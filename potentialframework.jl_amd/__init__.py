@@ -1,0 +1,47 @@
+"""MI355X-native force-evaluation hot path of PotentialFrameWork.jl.
+
+The product is ``libpfw_mi355x.so`` (HIP kernels for gfx950 behind the C ABI
+in include/pfw_mi355x.h); this package is the host-side mirror of the
+reference's ``Workspace`` / ``evaluate!`` / AtomsCalculators API over ctypes.
+
+Because the directory name contains a dot, import it by path, e.g.::
+
+    import importlib.util
+    spec = importlib.util.spec_from_file_location(
+        "pfw_amd", "potentialframework.jl_amd/__init__.py",
+        submodule_search_locations=["potentialframework.jl_amd"])
+"""
+from ._capi import (  # noqa: F401
+    ArgumentError,
+    CapacityError,
+    DeviceError,
+    DimensionMismatch,
+    DomainError,
+    StaleError,
+    load as load_library,
+    LIB_PATH,
+)
+from .system import System  # noqa: F401
+from . import workloads  # noqa: F401
+from .neighbourlist import Workspace, neighbour_list, neighbours, NeighbourSnapshot  # noqa: F401
+from .pair import (  # noqa: F401
+    SmoothPair,
+    RadialPotential,
+    LennardJonesRadial,
+    lennard_jones,
+    EnergyOnly,
+    EnergyForcesVirial,
+    evaluate_,
+    evaluate,
+)
+from .calculators import (  # noqa: F401
+    potential_energy,
+    forces,
+    virial,
+    energy_forces,
+    energy_forces_virial,
+    forces_,
+    energy_forces_,
+    energy_forces_virial_,
+    calculate,
+)

@@ -1,0 +1,228 @@
+// extern "C" entry points of libpfw_mi355x (declared in include/pfw_mi355x.h).
+// Host-side validation mirrors the reference's error contract:
+// ArgumentError for invalid geometry/parameters (storage.jl:87-94,134,162;
+// models.jl:34-43), DimensionMismatch for shapes, DomainError for nonfinite
+// model output (gpu.jl:390-395).
+#include <cmath>
+#include <cstring>
+
+#include "pfw_device.h"
+
+using pfw::Error;
+
+namespace {
+
+template <typename F>
+int run(pfw_ctx* ctx, F&& f) {
+  if (!ctx) return PFW_INVALID_ARGUMENT;
+  try {
+    PFW_HIP(hipSetDevice(ctx->device));
+    f();
+    ctx->err.clear();
+    return PFW_OK;
+  } catch (const Error& e) {
+    ctx->err = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    ctx->err = e.what();
+    return PFW_DEVICE;
+  }
+}
+
+void check_positions(int64_t n, const double* X) {
+  if (n > 0 && !X) throw Error(PFW_INVALID_ARGUMENT, "positions pointer is null");
+  for (int64_t k = 0; k < 3 * n; ++k)
+    if (!std::isfinite(X[k])) throw Error(PFW_INVALID_ARGUMENT, "positions must be finite");
+}
+
+}  // namespace
+
+extern "C" {
+
+int pfw_ctx_create(int device, pfw_ctx** out) {
+  if (!out) return PFW_INVALID_ARGUMENT;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return PFW_DEVICE;
+  if (device < 0 || device >= ndev) return PFW_INVALID_ARGUMENT;
+  auto* ctx = new pfw_ctx;
+  ctx->device = device;
+  int rc = run(ctx, [&] {
+    PFW_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    PFW_HIP(hipMalloc(&ctx->d_out, 16 * sizeof(double)));
+    PFW_HIP(hipHostMalloc(&ctx->h_out, 16 * sizeof(double), hipHostMallocDefault));
+    for (auto& e : ctx->ev) PFW_HIP(hipEventCreate(&e));
+  });
+  if (rc != PFW_OK) {
+    pfw_ctx_destroy(ctx);
+    return rc;
+  }
+  *out = ctx;
+  return PFW_OK;
+}
+
+void pfw_ctx_destroy(pfw_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  pfw::nl_free(ctx);
+  if (ctx->d_F) (void)hipFree(ctx->d_F);
+  if (ctx->d_partials) (void)hipFree(ctx->d_partials);
+  if (ctx->d_out) (void)hipFree(ctx->d_out);
+  if (ctx->h_out) (void)hipHostFree(ctx->h_out);
+  if (ctx->pair.d_params) (void)hipFree(ctx->pair.d_params);
+  for (auto& e : ctx->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char* pfw_last_error(const pfw_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int pfw_set_system(pfw_ctx* ctx, int64_t n, const double* X, const int32_t* species, const double* H,
+                   const uint8_t* pbc, double cutoff) {
+  return run(ctx, [&] {
+    if (n < 0 || n > ((int64_t)1 << 31) - 1) throw Error(PFW_DIMENSION, "atom count out of range");
+    if (!H || !pbc || (n > 0 && !species)) throw Error(PFW_INVALID_ARGUMENT, "null geometry pointer");
+    if (!(std::isfinite(cutoff) && cutoff > 0)) throw Error(PFW_INVALID_ARGUMENT, "cutoff must be finite and positive (Å)");
+    check_positions(n, X);
+    for (int64_t k = 0; k < n; ++k)
+      if (species[k] < 0) throw Error(PFW_INVALID_ARGUMENT, "species channels must be nonnegative");
+    ctx->have_system = false;
+    ctx->have_results = false;
+    pfw::nl_free(ctx);
+    for (int k = 0; k < 9; ++k) ctx->host_H[k] = H[k];
+    for (int a = 0; a < 3; ++a) ctx->host_pb[a] = pbc[a] ? 1 : 0;
+    // open cell vectors may be infinite or zero (storage.jl:80-82); periodic ones are checked by the basis
+    ctx->sys.n = n;
+    PFW_HIP(hipMalloc(&ctx->sys.X, std::max<int64_t>(3 * n, 1) * sizeof(double)));
+    PFW_HIP(hipMalloc(&ctx->sys.species, std::max<int64_t>(n, 1) * sizeof(int32_t)));
+    if (n > 0) {
+      PFW_HIP(hipMemcpyAsync(ctx->sys.X, X, 3 * n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+      PFW_HIP(hipMemcpyAsync(ctx->sys.species, species, n * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+    }
+    pfw::nl_configure(ctx, cutoff);
+    if (ctx->d_F) (void)hipFree(ctx->d_F);
+    ctx->d_F = nullptr;
+    PFW_HIP(hipMalloc(&ctx->d_F, std::max<int64_t>(3 * n, 1) * sizeof(double)));
+    if (ctx->d_partials) (void)hipFree(ctx->d_partials);
+    ctx->d_partials = nullptr;
+    PFW_HIP(hipMalloc(&ctx->d_partials, (size_t)ctx->grid.ncells * 7 * sizeof(double)));
+    PFW_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->have_system = true;
+  });
+}
+
+int pfw_update_positions(pfw_ctx* ctx, const double* X) {
+  return run(ctx, [&] {
+    if (!ctx->have_system) throw Error(PFW_STALE, "no system set");
+    check_positions(ctx->sys.n, X);
+    if (ctx->sys.n > 0)
+      PFW_HIP(hipMemcpyAsync(ctx->sys.X, X, 3 * ctx->sys.n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    PFW_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->have_results = false;
+  });
+}
+
+int pfw_set_pair_model(pfw_ctx* ctx, int kind, int nz, const double* params, double cutoff) {
+  return run(ctx, [&] {
+    if (!params) throw Error(PFW_INVALID_ARGUMENT, "null parameters");
+    if (!(std::isfinite(cutoff) && cutoff > 0)) throw Error(PFW_INVALID_ARGUMENT, "cutoff must be finite and positive");
+    std::vector<double> p;
+    if (kind == PFW_PAIR_SMOOTH) {
+      if (nz <= 0) throw Error(PFW_INVALID_ARGUMENT, "SmoothPair needs at least one species");
+      p.assign(params, params + (size_t)nz * nz);
+      for (int a = 0; a < nz; ++a)
+        for (int b = 0; b < nz; ++b) {
+          double v = p[a + nz * b];
+          if (!std::isfinite(v) || v != p[b + nz * a])
+            throw Error(PFW_INVALID_ARGUMENT, "amplitude table must be finite and symmetric");
+        }
+    } else if (kind == PFW_PAIR_LJ) {
+      p.assign(params, params + 2);
+      if (!(std::isfinite(p[0]) && std::isfinite(p[1]) && p[0] > 0 && p[1] > 0))
+        throw Error(PFW_INVALID_ARGUMENT, "LJ parameters must be positive and finite");
+      nz = 1;
+    } else {
+      throw Error(PFW_INVALID_ARGUMENT, "unknown pair model kind");
+    }
+    if (ctx->pair.d_params) (void)hipFree(ctx->pair.d_params);
+    ctx->pair.d_params = nullptr;
+    PFW_HIP(hipMalloc(&ctx->pair.d_params, p.size() * sizeof(double)));
+    PFW_HIP(hipMemcpyAsync(ctx->pair.d_params, p.data(), p.size() * sizeof(double), hipMemcpyHostToDevice,
+                           ctx->stream));
+    PFW_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->pair.kind = kind;
+    ctx->pair.nz = nz;
+    ctx->pair.cutoff = cutoff;
+    ctx->pair.params = p;
+  });
+}
+
+int pfw_pair_step(pfw_ctx* ctx, uint32_t mask, int precision) {
+  return run(ctx, [&] {
+    if (!ctx->have_system) throw Error(PFW_STALE, "no system set");
+    if (precision != PFW_FP64 && precision != PFW_MIXED) throw Error(PFW_INVALID_ARGUMENT, "unknown precision");
+    if (mask & ~7u) throw Error(PFW_INVALID_ARGUMENT, "unknown request bits");
+    ctx->have_results = false;
+    pfw::pair_run(ctx, mask, precision);
+  });
+}
+
+int pfw_read_results(pfw_ctx* ctx, double* E, double* F, double* W) {
+  return run(ctx, [&] {
+    if (!ctx->have_results) throw Error(PFW_STALE, "no results: run a step first");
+    if (E) *E = (ctx->last_mask & PFW_ENERGY) ? ctx->h_out[0] : 0.0;
+    if (W)
+      for (int k = 0; k < 9; ++k) W[k] = (ctx->last_mask & PFW_VIRIAL) ? ctx->h_out[1 + k] : 0.0;
+    if (F && ctx->sys.n > 0) {
+      if (ctx->last_mask & PFW_FORCES) {
+        PFW_HIP(hipMemcpyAsync(F, ctx->d_F, 3 * ctx->sys.n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        PFW_HIP(hipStreamSynchronize(ctx->stream));
+      } else {
+        std::memset(F, 0, 3 * ctx->sys.n * sizeof(double));
+      }
+    }
+  });
+}
+
+int pfw_pair_evaluate(pfw_ctx* ctx, uint32_t mask, int precision, double* E, double* F, double* W) {
+  int rc = pfw_pair_step(ctx, mask, precision);
+  if (rc != PFW_OK) return rc;
+  return pfw_read_results(ctx, E, F, W);
+}
+
+int pfw_export_pairs(pfw_ctx* ctx, int32_t* i, int32_t* j, int32_t* S, double* R, int64_t cap, int64_t* count) {
+  return run(ctx, [&] {
+    if (!ctx->have_system) throw Error(PFW_STALE, "no system set");
+    if (!count) throw Error(PFW_INVALID_ARGUMENT, "null count");
+    pfw::nl_export_pairs(ctx, i, j, S, R, cap, count);
+  });
+}
+
+int pfw_grid_info(pfw_ctx* ctx, int64_t* ncells, int32_t* dims, int32_t* minimum_image) {
+  return run(ctx, [&] {
+    if (!ctx->have_system) throw Error(PFW_STALE, "no system set");
+    if (ncells) *ncells = ctx->grid.ncells;
+    if (dims)
+      for (int a = 0; a < 3; ++a) dims[a] = ctx->grid.dims[a];
+    if (minimum_image) *minimum_image = ctx->grid.minimum_image;
+  });
+}
+
+int pfw_set_timing(pfw_ctx* ctx, int enabled) {
+  return run(ctx, [&] { ctx->timing = enabled != 0; });
+}
+
+int pfw_get_timings(pfw_ctx* ctx, double* ms, int n) {
+  if (!ctx || !ms) return 0;
+  int m = std::min(n, pfw::kStages);
+  for (int k = 0; k < m; ++k) ms[k] = ctx->stage_ms[k];
+  return m;
+}
+
+int pfw_synchronize(pfw_ctx* ctx) {
+  return run(ctx, [&] { PFW_HIP(hipStreamSynchronize(ctx->stream)); });
+}
+
+}  // extern "C"

@@ -1,0 +1,588 @@
+// Cell-list build on gfx950: FP64 wrap and binning, counting sort by cell
+// (histogram, exclusive scan, scatter, per-cell ordering by original index),
+// and the canonical directed-pair export used for parity.
+//
+// Reference semantics: src/submodules/NeighbourList/storage.jl:83-283 (wrap,
+// reach, cell dims, image stencil with shift fld(key+off, dims)) and
+// interface.jl:128-171 (directed pairs with shifts relative to the original
+// coordinates). The grid may be coarser than the reference's (it is capped for
+// huge or open systems) because the pair set does not depend on the grid, only
+// on the exact membership test r^2 < c^2 evaluated below in FP64.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "pfw_device.h"
+
+namespace pfw {
+
+// ---------------------------------------------------------------- host setup
+
+static void host_periodic_basis(const double* H, const int* pb, double* B) {
+  double orth[3][3];
+  int no = 0;
+  for (int i = 0; i < 9; ++i) B[i] = 0;
+  auto dot = [](const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; };
+  for (int a = 0; a < 3; ++a) {
+    if (!pb[a]) continue;
+    double v[3] = {H[3 * a], H[3 * a + 1], H[3 * a + 2]};
+    for (double t : v)
+      if (!std::isfinite(t)) throw Error(PFW_INVALID_ARGUMENT, "periodic cell vectors must be finite");
+    for (int k = 0; k < 3; ++k) B[3 * a + k] = v[k];
+    for (int q = 0; q < no; ++q) {
+      double s = dot(orth[q], v);
+      for (int k = 0; k < 3; ++k) v[k] -= s * orth[q][k];
+    }
+    double nrm = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    if (!(std::isfinite(nrm) && nrm > 0))
+      throw Error(PFW_INVALID_ARGUMENT, "periodic cell vectors must be linearly independent");
+    for (int k = 0; k < 3; ++k) orth[no][k] = v[k] / nrm;
+    ++no;
+  }
+  for (int a = 0; a < 3; ++a) {
+    if (pb[a]) continue;
+    double best[3] = {0, 0, 0}, bestn = -1;
+    for (int k = 0; k < 3; ++k) {
+      double v[3] = {0, 0, 0};
+      v[k] = 1;
+      for (int q = 0; q < no; ++q) {
+        double s = dot(orth[q], v);
+        for (int m = 0; m < 3; ++m) v[m] -= s * orth[q][m];
+      }
+      double n2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+      if (n2 > bestn) {
+        bestn = n2;
+        for (int m = 0; m < 3; ++m) best[m] = v[m];
+      }
+    }
+    double nrm = std::sqrt(best[0] * best[0] + best[1] * best[1] + best[2] * best[2]);
+    for (int m = 0; m < 3; ++m) {
+      best[m] /= nrm;
+      B[3 * a + m] = best[m];
+      orth[no][m] = best[m];
+    }
+    ++no;
+  }
+}
+
+// 3x3 inverse by cross products (the StaticArrays form used by storage.jl:148).
+static void host_inv3(const double* A, double* out) {
+  auto cross = [](const double* a, const double* b, double* o) {
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+  };
+  double x0[3] = {A[0], A[1], A[2]}, x1[3] = {A[3], A[4], A[5]}, x2[3] = {A[6], A[7], A[8]};
+  double y0[3], y1[3], y2[3];
+  cross(x1, x2, y0);
+  double d = x0[0] * y0[0] + x0[1] * y0[1] + x0[2] * y0[2];
+  for (int k = 0; k < 3; ++k) {
+    x0[k] /= d;
+    y0[k] /= d;
+  }
+  cross(x2, x0, y1);
+  cross(x0, x1, y2);
+  for (int col = 0; col < 3; ++col) {
+    out[3 * col + 0] = y0[col];
+    out[3 * col + 1] = y1[col];
+    out[3 * col + 2] = y2[col];
+  }
+}
+
+template <typename T>
+static void dmalloc(T** p, size_t count) {
+  if (*p) {
+    (void)hipFree(*p);
+    *p = nullptr;
+  }
+  PFW_HIP(hipMalloc((void**)p, std::max<size_t>(count, 1) * sizeof(T)));
+}
+
+void nl_free(pfw_ctx* ctx) {
+  SystemBuffers& s = ctx->sys;
+  void* ptrs[] = {s.X, s.species, s.xw, s.u, s.cell_of, s.count, s.start, s.cursor, s.blocksum,
+                  s.perm, s.xs, s.sps, s.bounds, s.grid};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  s = SystemBuffers();
+}
+
+// Grid for the uploaded geometry: mirrors prepare!'s choice of wrap path,
+// basis, reach and dims (storage.jl:130-193), with the cell count capped.
+void nl_configure(pfw_ctx* ctx, double c) {
+  Grid g{};
+  const double* H = ctx->host_H;
+  const int* pb = ctx->host_pb;
+  bool axisaligned = true;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      if (i != j && H[i + 3 * j] != 0) axisaligned = false;
+  double L[3] = {H[0], H[4], H[8]};
+  bool anyp = pb[0] || pb[1] || pb[2];
+  bool mi = !anyp;
+  if (anyp && axisaligned) {
+    mi = true;
+    for (int a = 0; a < 3; ++a)
+      if (pb[a] && !(std::isfinite(L[a]) && L[a] > 2 * c)) mi = false;
+  }
+  g.minimum_image = mi;
+  for (int a = 0; a < 3; ++a) {
+    g.pb[a] = pb[a];
+    g.L[a] = L[a];
+  }
+  if (mi) {
+    for (int k = 0; k < 9; ++k) g.basis[k] = g.recip[k] = (k % 4 == 0) ? 1.0 : 0.0;
+  } else {
+    host_periodic_basis(H, pb, g.basis);
+    host_inv3(g.basis, g.recip);
+    for (double v : g.recip)
+      if (!std::isfinite(v)) throw Error(PFW_INVALID_ARGUMENT, "periodic cell vectors must be linearly independent");
+  }
+  g.c = c;
+  g.c2 = c * c;
+  double extent[3];
+  int nper = 0, nopen = 0;
+  for (int a = 0; a < 3; ++a) {
+    if (mi) {
+      g.reach[a] = c;
+      extent[a] = L[a];
+    } else {
+      double r0 = g.recip[a], r1 = g.recip[a + 3], r2 = g.recip[a + 6];
+      g.reach[a] = c * std::sqrt(r0 * r0 + r1 * r1 + r2 * r2);
+      extent[a] = 1.0;
+    }
+    if (pb[a])
+      ++nper;
+    else
+      ++nopen;
+  }
+  int64_t n = ctx->sys.n;
+  double limit = std::max<double>(2.0 * (double)n, 4096.0);
+  // periodic axes: reach-sized cells (storage.jl:192), capped in count
+  double dimsd[3] = {1, 1, 1};
+  for (int a = 0; a < 3; ++a)
+    if (pb[a]) dimsd[a] = std::max(1.0, std::floor(extent[a] / g.reach[a]));
+  double prod = dimsd[0] * dimsd[1] * dimsd[2];
+  if (prod > limit) {
+    double f = std::pow(limit / prod, 1.0 / std::max(nper, 1));
+    for (int a = 0; a < 3; ++a)
+      if (pb[a]) dimsd[a] = std::max(1.0, std::floor(dimsd[a] * f));
+  }
+  double perprod = dimsd[0] * dimsd[1] * dimsd[2];
+  int opencap = nopen ? (int)std::min(1024.0, std::max(1.0, std::ceil(std::pow(limit / perprod, 1.0 / nopen))))
+                      : 1;
+  for (int a = 0; a < 3; ++a) {
+    if (pb[a]) {
+      g.dims[a] = (int)dimsd[a];
+      g.sizecell[a] = extent[a] / g.dims[a];
+      g.radius[a] = mi ? 1 : std::max(1, (int)std::ceil(g.reach[a] / g.sizecell[a]));
+      g.used[a] = g.dims[a];
+      g.origin[a] = 0;
+    } else {
+      g.dims[a] = opencap;
+      g.radius[a] = 1;
+      g.used[a] = opencap;
+      g.sizecell[a] = g.reach[a];
+      g.origin[a] = 0;
+    }
+  }
+  g.open_any = nopen > 0;
+  int64_t nc = (int64_t)g.dims[0] * g.dims[1] * g.dims[2];
+  if (nc > (int64_t)1 << 30) throw Error(PFW_INVALID_ARGUMENT, "cell grid too large");
+  g.ncells = (int)nc;
+  g.nstencil = (2 * g.radius[0] + 1) * (2 * g.radius[1] + 1) * (2 * g.radius[2] + 1);
+  if (g.nstencil > 4096)
+    throw Error(PFW_INVALID_ARGUMENT, "cutoff spans too many periodic images for the cell stencil");
+  ctx->grid = g;
+
+  SystemBuffers& s = ctx->sys;
+  dmalloc(&s.xw, 3 * n);
+  dmalloc(&s.u, 3 * n);
+  dmalloc(&s.cell_of, n);
+  dmalloc(&s.perm, n);
+  dmalloc(&s.xs, 3 * n);
+  dmalloc(&s.sps, n);
+  dmalloc(&s.count, g.ncells);
+  dmalloc(&s.start, g.ncells + 1);
+  dmalloc(&s.cursor, g.ncells);
+  dmalloc(&s.blocksum, (g.ncells + 1023) / 1024 + 1);
+  dmalloc(&s.bounds, 6);
+  dmalloc(&s.grid, 1);
+  PFW_HIP(hipMemcpyAsync(s.grid, &ctx->grid, sizeof(Grid), hipMemcpyHostToDevice, ctx->stream));
+}
+
+// ---------------------------------------------------------------- kernels
+
+__device__ inline void wrap_atom(const Grid& g, const double* x, double* w, double* u) {
+  if (g.minimum_image) {
+    for (int a = 0; a < 3; ++a) {
+      w[a] = g.pb[a] ? jl_mod(x[a], g.L[a]) : x[a];
+      u[a] = w[a];
+    }
+  } else {
+    double f[3];
+    exact_matvec(g.recip, x, f);
+    for (int a = 0; a < 3; ++a)
+      if (g.pb[a]) f[a] = jl_mod(f[a], 1.0);
+    exact_matvec(g.basis, f, w);
+    for (int a = 0; a < 3; ++a) u[a] = f[a];
+  }
+}
+
+__device__ inline int bin_atom(const Grid& g, const double* u) {
+  int k[3];
+  for (int a = 0; a < 3; ++a) {
+    double t = g.pb[a] ? u[a] / g.sizecell[a] : (u[a] - g.origin[a]) / g.sizecell[a];
+    int lim = g.pb[a] ? g.dims[a] : g.used[a];
+    double fl = floor(t);
+    int v = fl < 0 ? 0 : (fl > (double)(lim - 1) ? lim - 1 : (int)fl);
+    k[a] = v;
+  }
+  return cell_index(g, k);
+}
+
+// Wrap every atom; on fully periodic grids also bin and count in the same pass.
+__global__ void k_wrap(const Grid* __restrict__ gp, int64_t n, const double* __restrict__ X,
+                       double* __restrict__ xw, double* __restrict__ u, int32_t* __restrict__ cell_of,
+                       int32_t* __restrict__ count, unsigned long long* __restrict__ bounds) {
+  const Grid& g = *gp;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  if (i < n) {
+    double x[3] = {X[3 * i], X[3 * i + 1], X[3 * i + 2]}, w[3], uu[3];
+    wrap_atom(g, x, w, uu);
+    for (int a = 0; a < 3; ++a) {
+      xw[a * n + i] = w[a];
+      u[a * n + i] = uu[a];
+    }
+    if (!g.open_any) {
+      int c = bin_atom(g, uu);
+      cell_of[i] = c;
+      atomicAdd(&count[c], 1);
+    } else {
+      for (int a = 0; a < 3; ++a) lo[a] = hi[a] = uu[a];
+    }
+  }
+  if (g.open_any) {
+    for (int a = 0; a < 3; ++a) {
+      if (g.pb[a]) continue;
+      double l = lo[a], h = hi[a];
+      for (int off = kWave / 2; off > 0; off >>= 1) {
+        l = fmin(l, __shfl_xor(l, off, kWave));
+        h = fmax(h, __shfl_xor(h, off, kWave));
+      }
+      if ((threadIdx.x % kWave) == 0 && isfinite(l)) {
+        atomicMin(&bounds[2 * a], ordered_bits(l));
+        atomicMax(&bounds[2 * a + 1], ordered_bits(h));
+      }
+    }
+  }
+}
+
+// Open axes: grid origin and cell size from the atoms' extent (sparse-vacuum
+// handling of storage.jl:187-197 with a bounded dense grid).
+__global__ void k_open_grid(Grid* __restrict__ gp, const unsigned long long* __restrict__ bounds, int64_t n) {
+  Grid& g = *gp;
+  for (int a = 0; a < 3; ++a) {
+    if (g.pb[a]) continue;
+    if (n == 0) {
+      g.origin[a] = 0;
+      g.sizecell[a] = g.reach[a];
+      g.used[a] = 1;
+      continue;
+    }
+    double lo = from_ordered_bits(bounds[2 * a]), hi = from_ordered_bits(bounds[2 * a + 1]);
+    double span = hi - lo;
+    double sc = g.reach[a];
+    double nominal = floor(span / sc) + 1.0;
+    if (nominal > (double)g.dims[a]) sc = fmax(sc, span / (double)g.dims[a] * (1.0 + 1e-12));
+    int used = (int)fmin((double)g.dims[a], floor(span / sc) + 1.0);
+    g.origin[a] = lo;
+    g.sizecell[a] = sc;
+    g.used[a] = used < 1 ? 1 : used;
+  }
+}
+
+__global__ void k_bin_open(const Grid* __restrict__ gp, int64_t n, const double* __restrict__ u,
+                           int32_t* __restrict__ cell_of, int32_t* __restrict__ count) {
+  const Grid& g = *gp;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double uu[3] = {u[i], u[n + i], u[2 * n + i]};
+  int c = bin_atom(g, uu);
+  cell_of[i] = c;
+  atomicAdd(&count[c], 1);
+}
+
+// Exclusive scan of cell counts: per-block totals, a one-block scan of the
+// totals, then per-block rescans with offsets. 256 threads x 4 cells per block.
+constexpr int kScanItems = 4;
+constexpr int kScanThreads = 256;
+constexpr int kScanBlock = kScanItems * kScanThreads;
+
+__device__ inline int block_exclusive_scan(int v, int* lds, int* total) {
+  // lds: kScanThreads/64 ints
+  int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+  int incl = v;
+  for (int off = 1; off < kWave; off <<= 1) {
+    int t = __shfl_up(incl, off, kWave);
+    if (lane >= off) incl += t;
+  }
+  if (lane == kWave - 1) lds[wid] = incl;
+  __syncthreads();
+  int woff = 0, tot = 0;
+  for (int w = 0; w < (int)(blockDim.x / kWave); ++w) {
+    if (w < wid) woff += lds[w];
+    tot += lds[w];
+  }
+  __syncthreads();
+  *total = tot;
+  return woff + incl - v;
+}
+
+__global__ void k_scan_blocks(const int32_t* __restrict__ count, int ncells, int32_t* __restrict__ blocksum) {
+  __shared__ int lds[kScanThreads / kWave];
+  int base = blockIdx.x * kScanBlock + threadIdx.x * kScanItems;
+  int s = 0;
+  for (int k = 0; k < kScanItems; ++k)
+    if (base + k < ncells) s += count[base + k];
+  int total;
+  block_exclusive_scan(s, lds, &total);
+  if (threadIdx.x == 0) blocksum[blockIdx.x] = total;
+}
+
+__global__ void k_scan_top(int32_t* __restrict__ blocksum, int nb) {
+  __shared__ int lds[kScanThreads / kWave];
+  int carry = 0;
+  for (int base = 0; base < nb; base += kScanThreads) {
+    int i = base + threadIdx.x;
+    int v = i < nb ? blocksum[i] : 0;
+    int total;
+    int ex = block_exclusive_scan(v, lds, &total);
+    if (i < nb) blocksum[i] = carry + ex;
+    carry += total;
+    __syncthreads();
+  }
+}
+
+__global__ void k_scan_final(const int32_t* __restrict__ count, int ncells, const int32_t* __restrict__ blocksum,
+                             int32_t* __restrict__ start, int32_t* __restrict__ cursor, int64_t n) {
+  __shared__ int lds[kScanThreads / kWave];
+  int base = blockIdx.x * kScanBlock + threadIdx.x * kScanItems;
+  int v[kScanItems], s = 0;
+  for (int k = 0; k < kScanItems; ++k) {
+    v[k] = base + k < ncells ? count[base + k] : 0;
+    s += v[k];
+  }
+  int total;
+  int ex = block_exclusive_scan(s, lds, &total) + blocksum[blockIdx.x];
+  for (int k = 0; k < kScanItems; ++k) {
+    if (base + k < ncells) {
+      start[base + k] = ex;
+      cursor[base + k] = 0;
+    }
+    ex += v[k];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) start[ncells] = (int32_t)n;
+}
+
+__global__ void k_scatter(int64_t n, const int32_t* __restrict__ cell_of, const int32_t* __restrict__ start,
+                          int32_t* __restrict__ cursor, int32_t* __restrict__ perm) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int c = cell_of[i];
+  int pos = start[c] + atomicAdd(&cursor[c], 1);
+  perm[pos] = (int32_t)i;
+}
+
+// Deterministic order inside each cell: ascending original index (the stable
+// sortperm of storage.jl:199).
+__global__ void k_cell_sort(int ncells, const int32_t* __restrict__ start, int32_t* __restrict__ perm) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  int b = start[c], e = start[c + 1];
+  for (int k = b + 1; k < e; ++k) {
+    int v = perm[k];
+    int m = k - 1;
+    while (m >= b && perm[m] > v) {
+      perm[m + 1] = perm[m];
+      --m;
+    }
+    perm[m + 1] = v;
+  }
+}
+
+__global__ void k_gather(int64_t n, const int32_t* __restrict__ perm, const double* __restrict__ xw,
+                         const int32_t* __restrict__ species, double* __restrict__ xs, int32_t* __restrict__ sps) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  int i = perm[k];
+  xs[k] = xw[i];
+  xs[n + k] = xw[n + i];
+  xs[2 * n + k] = xw[2 * n + i];
+  sps[k] = species[i];
+}
+
+static inline unsigned nblocks(int64_t n, int t) { return (unsigned)std::max<int64_t>(1, (n + t - 1) / t); }
+
+void nl_build(pfw_ctx* ctx) {
+  SystemBuffers& s = ctx->sys;
+  const Grid& g = ctx->grid;
+  hipStream_t st = ctx->stream;
+  int64_t n = s.n;
+  record_stage(ctx, 0);
+  PFW_HIP(hipMemsetAsync(s.count, 0, sizeof(int32_t) * g.ncells, st));
+  if (g.open_any) {
+    unsigned long long init[6];
+    for (int a = 0; a < 3; ++a) {
+      init[2 * a] = ~0ull;
+      init[2 * a + 1] = 0ull;
+    }
+    PFW_HIP(hipMemcpyAsync(s.bounds, init, sizeof(init), hipMemcpyHostToDevice, st));
+  }
+  if (n > 0)
+    hipLaunchKernelGGL(k_wrap, dim3(nblocks(n, 256)), dim3(256), 0, st, s.grid, n, s.X, s.xw, s.u, s.cell_of,
+                       s.count, (unsigned long long*)s.bounds);
+  if (g.open_any) {
+    hipLaunchKernelGGL(k_open_grid, dim3(1), dim3(1), 0, st, s.grid, (const unsigned long long*)s.bounds, n);
+    if (n > 0)
+      hipLaunchKernelGGL(k_bin_open, dim3(nblocks(n, 256)), dim3(256), 0, st, s.grid, n, s.u, s.cell_of, s.count);
+  }
+  PFW_HIP(hipGetLastError());
+  record_stage(ctx, 1);
+  int nb = (int)nblocks(g.ncells, kScanBlock);
+  hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(kScanThreads), 0, st, s.count, g.ncells, s.blocksum);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, st, s.blocksum, nb);
+  hipLaunchKernelGGL(k_scan_final, dim3(nb), dim3(kScanThreads), 0, st, s.count, g.ncells, s.blocksum, s.start,
+                     s.cursor, n);
+  PFW_HIP(hipGetLastError());
+  record_stage(ctx, 2);
+  if (n > 0) {
+    hipLaunchKernelGGL(k_scatter, dim3(nblocks(n, 256)), dim3(256), 0, st, n, s.cell_of, s.start, s.cursor, s.perm);
+    hipLaunchKernelGGL(k_cell_sort, dim3(nblocks(g.ncells, 256)), dim3(256), 0, st, g.ncells, s.start, s.perm);
+    hipLaunchKernelGGL(k_gather, dim3(nblocks(n, 256)), dim3(256), 0, st, n, s.perm, s.xw, s.species, s.xs, s.sps);
+  }
+  PFW_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------- pair export
+
+__device__ inline void winding(const Grid& g, const double* X, int i, int* w) {
+  double x[3] = {X[3 * i], X[3 * i + 1], X[3 * i + 2]};
+  if (g.minimum_image) {
+    for (int a = 0; a < 3; ++a) w[a] = g.pb[a] ? (int)floor(x[a] / g.L[a]) : 0;
+  } else {
+    double f[3];
+    exact_matvec(g.recip, x, f);
+    for (int a = 0; a < 3; ++a) w[a] = g.pb[a] ? (int)floor(f[a]) : 0;
+  }
+}
+
+// One thread per sorted atom: walk the image stencil and emit every directed
+// pair strictly inside the cutoff (interface.jl:144-171). Pass 0 counts, pass 1 writes.
+__global__ void k_export(const Grid* __restrict__ gp, int64_t n, const double* __restrict__ X,
+                         const double* __restrict__ xs, const int32_t* __restrict__ start,
+                         const int32_t* __restrict__ cell_of, const int32_t* __restrict__ perm,
+                         int32_t* __restrict__ counts, const int64_t* __restrict__ offsets, int pass,
+                         int32_t* __restrict__ oi, int32_t* __restrict__ oj, int32_t* __restrict__ oS,
+                         double* __restrict__ oR) {
+  const Grid& g = *gp;
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  int i = perm[k];
+  int cell = cell_of[i];
+  int key[3];
+  cell_key(g, cell, key);
+  double xi = xs[k], yi = xs[n + k], zi = xs[2 * n + k];
+  int wi[3];
+  if (pass) winding(g, X, i, wi);
+  int64_t pos = pass ? offsets[k] : 0;
+  int cnt = 0;
+  for (int e = 0; e < g.nstencil; ++e) {
+    int lat[3];
+    bool zero;
+    int nc = stencil_entry(g, key, e, lat, &zero);
+    if (nc < 0) continue;
+    double sh[3];
+    exact_shift(g, lat, sh);
+    for (int m = start[nc]; m < start[nc + 1]; ++m) {
+      if (zero && m == k) continue;
+      double d[3];
+      int l[3];
+      if (!exact_member(g, xi, yi, zi, xs[m], xs[n + m], xs[2 * n + m], lat, sh, d, l)) continue;
+      if (pass) {
+        int j = perm[m];
+        int wj[3];
+        winding(g, X, j, wj);
+        oi[pos] = i;
+        oj[pos] = j;
+        if (oS)
+          for (int a = 0; a < 3; ++a) oS[3 * pos + a] = l[a] + wi[a] - wj[a];
+        if (oR)
+          for (int a = 0; a < 3; ++a) oR[3 * pos + a] = d[a];
+        ++pos;
+      }
+      ++cnt;
+    }
+  }
+  if (!pass) counts[k] = cnt;
+}
+
+void nl_export_pairs(pfw_ctx* ctx, int32_t* hi, int32_t* hj, int32_t* hS, double* hR, int64_t cap,
+                     int64_t* count) {
+  SystemBuffers& s = ctx->sys;
+  hipStream_t st = ctx->stream;
+  int64_t n = s.n;
+  nl_build(ctx);
+  if (n == 0) {
+    *count = 0;
+    return;
+  }
+  int32_t* d_counts = nullptr;
+  int64_t* d_off = nullptr;
+  PFW_HIP(hipMalloc(&d_counts, n * sizeof(int32_t)));
+  PFW_HIP(hipMalloc(&d_off, n * sizeof(int64_t)));
+  hipLaunchKernelGGL(k_export, dim3(nblocks(n, 128)), dim3(128), 0, st, s.grid, n, s.X, s.xs, s.start, s.cell_of,
+                     s.perm, d_counts, (const int64_t*)nullptr, 0, nullptr, nullptr, nullptr, nullptr);
+  PFW_HIP(hipGetLastError());
+  std::vector<int32_t> counts(n);
+  PFW_HIP(hipMemcpyAsync(counts.data(), d_counts, n * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  PFW_HIP(hipStreamSynchronize(st));
+  std::vector<int64_t> off(n);
+  int64_t total = 0;
+  for (int64_t k = 0; k < n; ++k) {
+    off[k] = total;
+    total += counts[k];
+  }
+  *count = total;
+  if (total > cap) {
+    (void)hipFree(d_counts);
+    (void)hipFree(d_off);
+    throw Error(PFW_CAPACITY, "pair export capacity too small");
+  }
+  int32_t *di = nullptr, *dj = nullptr, *dS = nullptr;
+  double* dR = nullptr;
+  PFW_HIP(hipMemcpyAsync(d_off, off.data(), n * sizeof(int64_t), hipMemcpyHostToDevice, st));
+  PFW_HIP(hipMalloc(&di, std::max<int64_t>(total, 1) * sizeof(int32_t)));
+  PFW_HIP(hipMalloc(&dj, std::max<int64_t>(total, 1) * sizeof(int32_t)));
+  PFW_HIP(hipMalloc(&dS, std::max<int64_t>(total, 1) * 3 * sizeof(int32_t)));
+  PFW_HIP(hipMalloc(&dR, std::max<int64_t>(total, 1) * 3 * sizeof(double)));
+  hipLaunchKernelGGL(k_export, dim3(nblocks(n, 128)), dim3(128), 0, st, s.grid, n, s.X, s.xs, s.start, s.cell_of,
+                     s.perm, d_counts, (const int64_t*)d_off, 1, di, dj, dS, dR);
+  PFW_HIP(hipGetLastError());
+  if (total > 0) {
+    if (hi) PFW_HIP(hipMemcpyAsync(hi, di, total * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    if (hj) PFW_HIP(hipMemcpyAsync(hj, dj, total * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    if (hS) PFW_HIP(hipMemcpyAsync(hS, dS, total * 3 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    if (hR) PFW_HIP(hipMemcpyAsync(hR, dR, total * 3 * sizeof(double), hipMemcpyDeviceToHost, st));
+  }
+  PFW_HIP(hipStreamSynchronize(st));
+  (void)hipFree(d_counts);
+  (void)hipFree(d_off);
+  (void)hipFree(di);
+  (void)hipFree(dj);
+  (void)hipFree(dS);
+  (void)hipFree(dR);
+}
+
+}  // namespace pfw

@@ -1,0 +1,338 @@
+// Pair energy / force / virial on gfx950 (SmoothPair and energy-shifted
+// Lennard-Jones), full directed list without atomics.
+//
+// Reference: the half-list executor src/submodules/NeighbourList/execution.jl:173-217
+// with the callbacks src/submodules/PairPotentials/models.jl:153-167 and
+// lennard_jones.jl:5-17; the GPU boundary src/submodules/NeighbourList/gpu.jl:236-402.
+// Every directed pair (i -> j, S) contributes g = (dE/dr / r) d to F_i, half its
+// energy to E and -1/2 d (x) g to W, so each unordered pair image is counted once.
+//
+// Layout: one 64-lane wavefront per cell. The neighbour cells' atoms are staged
+// in LDS as coordinates relative to the cell's first atom (computed in FP64,
+// stored in T). Lanes are (atom, slice) pairs: a cell of n_i atoms uses
+// S = 64 / n_i slices, each lane testing every S-th candidate. Hits are
+// compacted per lane with a 32-bit mask before the model body runs. Membership
+// inside a thin band around the cutoff is re-decided with the exact FP64
+// formula of the oracle, so the pair set is bit-exact in both precisions.
+#include <algorithm>
+
+#include "pfw_device.h"
+
+namespace pfw {
+
+constexpr int kCand = 256;   // candidates per LDS chunk
+constexpr int kEnt = 64;     // stencil entries per entry chunk
+constexpr int kMaxZ = 16;    // species table size limit
+
+struct LJConsts {
+  double c4e, c24e, sig2, eshift;
+};
+
+template <typename T>
+struct Band;
+template <>
+struct Band<float> {
+  static constexpr double rel = 1e-4;
+};
+template <>
+struct Band<double> {
+  static constexpr double rel = 1e-11;
+};
+
+// Block index -> cell, giving each of the 8 XCDs (dispatched round-robin) a
+// contiguous range of cells so neighbouring cells share an L2.
+__device__ inline int xcd_remap(int b, int nb) {
+  int q = nb / 8, r = nb % 8;
+  int x = b % 8, idx = b / 8;
+  return x < r ? x * (q + 1) + idx : r * (q + 1) + (x - r) * q + idx;
+}
+
+template <typename T, int KIND>
+__global__ __launch_bounds__(64) void k_pair(const Grid* __restrict__ gp, int64_t n, const int32_t* __restrict__ start,
+                                             const double* __restrict__ xs, const int32_t* __restrict__ sps,
+                                             const int32_t* __restrict__ perm, const double* __restrict__ params,
+                                             int nz, LJConsts lj, uint32_t mask, double* __restrict__ F,
+                                             double* __restrict__ partials, int ncells) {
+  __shared__ T cx[kCand], cy[kCand], cz[kCand];
+  __shared__ int cj[kCand], cm[kCand];
+  __shared__ int e_pref[kEnt + 1], e_start[kEnt], e_lat[kEnt][3];
+  __shared__ double e_sh[kEnt][3];
+  __shared__ T sA[kMaxZ * kMaxZ], sA6[kMaxZ * kMaxZ];
+  __shared__ double red[kWave][3];
+  __shared__ int e_zero;
+
+  const Grid& g = *gp;
+  const int lane = threadIdx.x;
+  const int cell = xcd_remap(blockIdx.x, gridDim.x);
+  const T c2 = (T)g.c2;
+  const T c2lo = (T)(g.c2 * (1.0 - Band<T>::rel)), c2hi = (T)(g.c2 * (1.0 + Band<T>::rel));
+  const T invc2 = (T)(1.0 / g.c2);
+  if (KIND == PFW_PAIR_SMOOTH) {
+    for (int k = lane; k < nz * nz; k += kWave) {
+      double a = params[k];
+      sA[k] = (T)a;
+      sA6[k] = (T)(-6.0 * a / g.c2);
+    }
+  }
+  const T c4e = (T)lj.c4e, c24e = (T)lj.c24e, sig2 = (T)lj.sig2, eshift = (T)lj.eshift;
+  double eacc = 0;
+  T w0 = 0, w1 = 0, w2 = 0, w3 = 0, w4 = 0, w5 = 0;
+  __syncthreads();
+
+  const int b = start[cell], ncnt = start[cell + 1] - b;
+  if (ncnt > 0) {
+    int key[3];
+    cell_key(g, cell, key);
+    const double rx = xs[b], ry = xs[n + b], rz = xs[2 * n + b];
+    for (int ib = 0; ib < ncnt; ib += kWave) {
+      const int nb = min(kWave, ncnt - ib);
+      const int S = kWave / nb;
+      const int il = lane % nb, split = lane / nb;
+      const bool on = split < S;
+      const int is = b + ib + il;
+      double xid = xs[is], yid = xs[n + is], zid = xs[2 * n + is];
+      const T xi = (T)(xid - rx), yi = (T)(yid - ry), zi = (T)(zid - rz);
+      const int zsp = sps[is];
+      double fx = 0, fy = 0, fz = 0;
+      for (int e0 = 0; e0 < g.nstencil; e0 += kEnt) {
+        int cntE = 0;
+        const int e = e0 + lane;
+        if (lane == 0) e_zero = -1;
+        __syncthreads();
+        if (e < g.nstencil) {
+          int lat[3];
+          bool zero;
+          int nc = stencil_entry(g, key, e, lat, &zero);
+          if (nc >= 0) {
+            int s0 = start[nc];
+            cntE = start[nc + 1] - s0;
+            e_start[lane] = s0;
+            double sh[3];
+            exact_shift(g, lat, sh);
+            for (int a = 0; a < 3; ++a) {
+              e_sh[lane][a] = sh[a];
+              e_lat[lane][a] = lat[a];
+            }
+            if (zero) e_zero = lane;
+          }
+        }
+        int incl = cntE;
+        for (int off = 1; off < kWave; off <<= 1) {
+          int t = __shfl_up(incl, off, kWave);
+          if (lane >= off) incl += t;
+        }
+        e_pref[lane + 1] = incl;
+        if (lane == 0) e_pref[0] = 0;
+        const int total = __shfl(incl, kWave - 1, kWave);
+        __syncthreads();
+        const int ez = e_zero;
+        for (int t0 = 0; t0 < total; t0 += kCand) {
+          const int tn = min(kCand, total - t0);
+          for (int t = lane; t < tn; t += kWave) {
+            const int tt = t0 + t;
+            int lo = 0, hi = kEnt;
+            while (hi - lo > 1) {
+              int mid = (lo + hi) >> 1;
+              if (e_pref[mid] <= tt)
+                lo = mid;
+              else
+                hi = mid;
+            }
+            const int m = e_start[lo] + (tt - e_pref[lo]);
+            cx[t] = (T)((xs[m] + e_sh[lo][0]) - rx);
+            cy[t] = (T)((xs[n + m] + e_sh[lo][1]) - ry);
+            cz[t] = (T)((xs[2 * n + m] + e_sh[lo][2]) - rz);
+            cj[t] = m;
+            cm[t] = (lo << 8) | sps[m];
+          }
+          __syncthreads();
+          if (on) {
+            for (int c0 = split; c0 < tn; c0 += S * 32) {
+              uint32_t hits = 0;
+#pragma unroll 8
+              for (int q = 0; q < 32; ++q) {
+                const int t = c0 + q * S;
+                if (t < tn) {
+                  const T dx = cx[t] - xi, dy = cy[t] - yi, dz = cz[t] - zi;
+                  const T r2 = dx * dx + dy * dy + dz * dz;
+                  if (r2 < c2hi) hits |= 1u << q;
+                }
+              }
+              while (hits) {
+                const int q = __builtin_ctz(hits);
+                hits &= hits - 1;
+                const int t = c0 + q * S;
+                const T dx = cx[t] - xi, dy = cy[t] - yi, dz = cz[t] - zi;
+                const T r2 = dx * dx + dy * dy + dz * dz;
+                const int meta = cm[t], m = cj[t], el = meta >> 8, zj = meta & 255;
+                if (el == ez && m == is) continue;  // the (i, i, 0) self pair
+                if (!(r2 < c2lo)) {
+                  double d[3];
+                  int img[3];
+                  if (!exact_member(g, xid, yid, zid, xs[m], xs[n + m], xs[2 * n + m], e_lat[el], e_sh[el], d,
+                                    img))
+                    continue;
+                }
+                T e, coef;
+                if (KIND == PFW_PAIR_SMOOTH) {
+                  const T a = sA[zsp * nz + zj], a6 = sA6[zsp * nz + zj];
+                  const T qq = (T)1 - r2 * invc2;
+                  const T q2 = qq * qq;
+                  e = a * q2 * qq;
+                  coef = a6 * q2;
+                } else {
+                  T inv;
+                  if constexpr (sizeof(T) == 4)
+                    inv = __builtin_amdgcn_rcpf(r2);
+                  else
+                    inv = (T)1 / r2;
+                  const T s2 = sig2 * inv;
+                  const T s6 = s2 * s2 * s2;
+                  e = c4e * (s6 * s6 - s6) - eshift;
+                  coef = c24e * (s6 - (T)2 * s6 * s6) * inv;
+                }
+                const T gx = coef * dx, gy = coef * dy, gz = coef * dz;
+                eacc += (double)e;
+                fx += (double)gx;
+                fy += (double)gy;
+                fz += (double)gz;
+                w0 += dx * gx;
+                w1 += dy * gy;
+                w2 += dz * gz;
+                w3 += dx * gy;
+                w4 += dx * gz;
+                w5 += dy * gz;
+              }
+            }
+          }
+          __syncthreads();
+        }
+      }
+      if (mask & PFW_FORCES) {
+        red[lane][0] = fx;
+        red[lane][1] = fy;
+        red[lane][2] = fz;
+        __syncthreads();
+        if (on && split == 0) {
+          double sx = 0, sy = 0, sz = 0;
+          for (int s = 0; s < S; ++s) {
+            sx += red[il + s * nb][0];
+            sy += red[il + s * nb][1];
+            sz += red[il + s * nb][2];
+          }
+          const int64_t o = perm[is];
+          F[3 * o] = sx;
+          F[3 * o + 1] = sy;
+          F[3 * o + 2] = sz;
+        }
+        __syncthreads();
+      }
+    }
+  }
+  double v[7] = {eacc, (double)w0, (double)w1, (double)w2, (double)w3, (double)w4, (double)w5};
+  for (int k = 0; k < 7; ++k) v[k] = wave_sum(v[k]);
+  if (lane == 0)
+    for (int k = 0; k < 7; ++k) partials[7 * (int64_t)cell + k] = v[k];
+}
+
+// FP64 reduction of the per-cell partials into E, W (3x3 column-major) and the
+// nonfinite status word (gpu.jl:390-395).
+__global__ void k_reduce(const double* __restrict__ partials, int ncells, double* __restrict__ out) {
+  __shared__ double lds[7][4];
+  double s[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (int c = threadIdx.x; c < ncells; c += blockDim.x)
+    for (int k = 0; k < 7; ++k) s[k] += partials[7 * (int64_t)c + k];
+  for (int k = 0; k < 7; ++k) s[k] = wave_sum(s[k]);
+  int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+  if (lane == 0)
+    for (int k = 0; k < 7; ++k) lds[k][wid] = s[k];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t[7];
+    for (int k = 0; k < 7; ++k) t[k] = lds[k][0] + lds[k][1] + lds[k][2] + lds[k][3];
+    double E = 0.5 * t[0];
+    // P = sum d (x) g over directed pairs: xx yy zz xy xz yz
+    double W[9];
+    W[0] = -0.5 * t[1];
+    W[4] = -0.5 * t[2];
+    W[8] = -0.5 * t[3];
+    W[3] = W[1] = -0.5 * t[4];
+    W[6] = W[2] = -0.5 * t[5];
+    W[7] = W[5] = -0.5 * t[6];
+    bool bad = !isfinite(E);
+    for (int k = 0; k < 9; ++k) bad |= !isfinite(W[k]);
+    out[0] = E;
+    for (int k = 0; k < 9; ++k) out[1 + k] = W[k];
+    out[10] = bad ? 1.0 : 0.0;
+  }
+}
+
+void record_stage(pfw_ctx* ctx, int k) {
+  if (ctx->timing) PFW_HIP(hipEventRecord(ctx->ev[k], ctx->stream));
+}
+
+template <typename T, int KIND>
+static void launch_pair(pfw_ctx* ctx, uint32_t mask) {
+  SystemBuffers& s = ctx->sys;
+  const Grid& g = ctx->grid;
+  LJConsts lj{0, 0, 0, 0};
+  if (KIND == PFW_PAIR_LJ) {
+    double eps = ctx->pair.params[0], sig = ctx->pair.params[1], c = ctx->pair.cutoff;
+    double rc = sig / c, qc2 = rc * rc, qc = qc2 * qc2 * qc2;
+    lj.c4e = 4.0 * eps;
+    lj.c24e = 24.0 * eps;
+    lj.sig2 = sig * sig;
+    lj.eshift = 4.0 * eps * (qc * qc - qc);
+  }
+  hipLaunchKernelGGL((k_pair<T, KIND>), dim3(g.ncells), dim3(kWave), 0, ctx->stream, s.grid, s.n, s.start, s.xs,
+                     s.sps, s.perm, ctx->pair.d_params, ctx->pair.nz, lj, mask, ctx->d_F, ctx->d_partials, g.ncells);
+}
+
+void pair_run(pfw_ctx* ctx, uint32_t mask, int precision) {
+  if (ctx->pair.kind < 0) throw Error(PFW_STALE, "no pair model set");
+  if (ctx->pair.cutoff != ctx->grid.c)
+    throw Error(PFW_STALE, "pair model cutoff differs from the system's cell-list cutoff; set the system again");
+  if (ctx->pair.kind == PFW_PAIR_SMOOTH && ctx->pair.nz > kMaxZ)
+    throw Error(PFW_INVALID_ARGUMENT, "too many species for the pair table");
+  nl_build(ctx);
+  record_stage(ctx, 3);
+  const Grid& g = ctx->grid;
+  hipStream_t st = ctx->stream;
+  if (mask != 0 && ctx->sys.n > 0) {
+    bool fp32 = precision == PFW_MIXED;
+    if (ctx->pair.kind == PFW_PAIR_SMOOTH) {
+      if (fp32)
+        launch_pair<float, PFW_PAIR_SMOOTH>(ctx, mask);
+      else
+        launch_pair<double, PFW_PAIR_SMOOTH>(ctx, mask);
+    } else {
+      if (fp32)
+        launch_pair<float, PFW_PAIR_LJ>(ctx, mask);
+      else
+        launch_pair<double, PFW_PAIR_LJ>(ctx, mask);
+    }
+    PFW_HIP(hipGetLastError());
+    record_stage(ctx, 4);
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, st, ctx->d_partials, g.ncells, ctx->d_out);
+    PFW_HIP(hipGetLastError());
+  } else {
+    record_stage(ctx, 4);
+    PFW_HIP(hipMemsetAsync(ctx->d_out, 0, 11 * sizeof(double), st));
+    if (ctx->sys.n > 0) PFW_HIP(hipMemsetAsync(ctx->d_F, 0, 3 * ctx->sys.n * sizeof(double), st));
+  }
+  PFW_HIP(hipMemcpyAsync(ctx->h_out, ctx->d_out, 11 * sizeof(double), hipMemcpyDeviceToHost, st));
+  record_stage(ctx, 5);
+  PFW_HIP(hipStreamSynchronize(st));
+  if (ctx->timing)
+    for (int k = 0; k < kStages; ++k) {
+      float ms = 0;
+      PFW_HIP(hipEventElapsedTime(&ms, ctx->ev[k], ctx->ev[k + 1]));
+      ctx->stage_ms[k] = ms;
+    }
+  ctx->last_mask = mask;
+  ctx->have_results = true;
+  if (ctx->h_out[10] != 0.0)
+    throw Error(PFW_DOMAIN, "singular origin or nonfinite pair callback output");
+}
+
+}  // namespace pfw

@@ -1,0 +1,119 @@
+// Device-side helpers shared by the cell-list, pair and ACE kernels (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "pfw_internal.h"
+
+namespace pfw {
+
+constexpr int kWave = 64;
+
+// Julia `mod(x::Float64, y::Float64)`: rem-based, result carries the sign of y.
+__device__ __host__ inline double jl_mod(double x, double y) {
+  double r = fmod(x, y);
+  if (r == 0) return copysign(0.0, y);
+  if ((r > 0) != (y > 0)) return r + y;
+  return r;
+}
+__device__ __host__ inline int jl_fld(int a, int b) {
+  int q = a / b;
+  if ((a % b != 0) && ((a < 0) != (b < 0))) q -= 1;
+  return q;
+}
+__device__ __host__ inline int jl_imod(int a, int b) { return a - jl_fld(a, b) * b; }
+
+// Exact FP64 geometry: every product and sum is separately rounded (no FMA
+// contraction), in the association order of the reference's StaticArrays
+// matrix-vector products, so membership decisions are reproducible bit for bit.
+__device__ inline void exact_matvec(const double* M, const double* v, double* o) {
+#pragma clang fp contract(off)
+  for (int a = 0; a < 3; ++a) o[a] = M[a] * v[0] + M[a + 3] * v[1] + M[a + 6] * v[2];
+}
+__device__ inline void exact_shift(const Grid& g, const int* s, double* o) {
+#pragma clang fp contract(off)
+  double v[3] = {(double)s[0], (double)s[1], (double)s[2]};
+  for (int a = 0; a < 3; ++a) o[a] = g.basis[a] * v[0] + g.basis[a + 3] * v[1] + g.basis[a + 6] * v[2];
+}
+// Exact membership of the directed pair (sorted i -> sorted j) through the
+// stencil entry with integer lattice `lat` and Cartesian shift `sh`
+// (geometry.jl:5-19): d = x_j - x_i + H s on the image path; on the
+// minimum-image path d = raw - round(raw/L) L, and the entry must carry that
+// same lattice so that two stencil entries aliasing one cell (dims == 2)
+// cannot both claim the pair. Writes d and the lattice of the image in `img`.
+__device__ inline bool exact_member(const Grid& g, double xi, double yi, double zi, double xj, double yj,
+                                    double zj, const int* lat, const double* sh, double* d, int* img) {
+#pragma clang fp contract(off)
+  double raw[3] = {xj - xi, yj - yi, zj - zi};
+  if (g.minimum_image) {
+    for (int a = 0; a < 3; ++a) {
+      if (g.pb[a]) {
+        double m = rint(raw[a] / g.L[a]);
+        d[a] = raw[a] - m * g.L[a];
+        img[a] = -(int)m;
+        if (img[a] != lat[a]) return false;
+      } else {
+        d[a] = raw[a];
+        img[a] = 0;
+      }
+    }
+  } else {
+    for (int a = 0; a < 3; ++a) {
+      d[a] = raw[a] + sh[a];
+      img[a] = lat[a];
+    }
+  }
+  return d[0] * d[0] + d[1] * d[1] + d[2] * d[2] < g.c2;
+}
+
+__device__ inline void cell_key(const Grid& g, int cell, int* k) {
+  k[0] = cell % g.dims[0];
+  int t = cell / g.dims[0];
+  k[1] = t % g.dims[1];
+  k[2] = t / g.dims[1];
+}
+__device__ inline int cell_index(const Grid& g, const int* k) {
+  return k[0] + g.dims[0] * (k[1] + g.dims[1] * k[2]);
+}
+
+// Stencil entry e of the cell with key k: neighbour cell (or -1 when it lies
+// outside an open axis), integer lattice shift and whether it is the zero
+// offset. Periodic axes wrap with shift fld(k+o, dims) (storage.jl:265-283).
+__device__ inline int stencil_entry(const Grid& g, const int* k, int e, int* lat, bool* zero) {
+  int w0 = 2 * g.radius[0] + 1, w1 = 2 * g.radius[1] + 1;
+  int o[3] = {e % w0 - g.radius[0], (e / w0) % w1 - g.radius[1], e / (w0 * w1) - g.radius[2]};
+  int nk[3];
+  *zero = (o[0] == 0 && o[1] == 0 && o[2] == 0);
+  for (int a = 0; a < 3; ++a) {
+    int v = k[a] + o[a];
+    if (g.pb[a]) {
+      nk[a] = jl_imod(v, g.dims[a]);
+      lat[a] = jl_fld(v, g.dims[a]);
+    } else {
+      if (v < 0 || v >= g.used[a]) return -1;
+      nk[a] = v;
+      lat[a] = 0;
+    }
+  }
+  return cell_index(g, nk);
+}
+
+__device__ inline bool is_finite(double x) { return isfinite(x); }
+
+// Order-preserving map of doubles onto unsigned 64-bit integers (for atomic min/max).
+__device__ inline unsigned long long ordered_bits(double x) {
+  unsigned long long b = __double_as_longlong(x);
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+__device__ inline double from_ordered_bits(unsigned long long b) {
+  return __longlong_as_double((b >> 63) ? (b & 0x7fffffffffffffffull) : ~b);
+}
+
+template <typename T>
+__device__ inline T wave_sum(T v) {
+  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+}  // namespace pfw

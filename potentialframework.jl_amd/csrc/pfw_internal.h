@@ -1,0 +1,113 @@
+// Internal definitions shared by the HIP translation units of libpfw_mi355x.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/pfw_mi355x.h"
+
+namespace pfw {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define PFW_HIP(call)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      throw ::pfw::Error(PFW_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// Cell grid of one system. Static fields are set on the host by pfw_set_system;
+// for open (non-periodic) axes origin/sizecell/used dims are recomputed on the
+// device every step from the atoms' extent.
+struct Grid {
+  int minimum_image;  // storage.jl:140
+  int pb[3];
+  int open_any;
+  int dims[3];     // layout dims (open axes: capacity)
+  int radius[3];   // stencil half-width per axis
+  int ncells;      // product of layout dims
+  int nstencil;    // (2r+1) product
+  double basis[9];  // column-major periodic basis (storage.jl:83-108), identity if minimum image
+  double recip[9];  // its inverse
+  double L[3];      // diag(H) for the minimum-image path
+  double c, c2;
+  double reach[3];
+  double sizecell[3];  // periodic axes: extent/dims; open axes: device-computed
+  double origin[3];    // open axes: device-computed minimum bin coordinate
+  int used[3];         // open axes: device-computed used dims (<= dims)
+};
+
+// Device buffers for one system.
+struct SystemBuffers {
+  int64_t n = 0;
+  int64_t cap = 0;
+  double* X = nullptr;        // 3 x n original order (AoS)
+  int32_t* species = nullptr;  // n original order
+  double* xw = nullptr;       // wrapped coordinates, SoA [3][n], original order
+  double* u = nullptr;        // bin coordinates, SoA [3][n], original order
+  int32_t* cell_of = nullptr;  // n
+  int32_t* count = nullptr;    // ncells
+  int32_t* start = nullptr;    // ncells + 1
+  int32_t* cursor = nullptr;   // ncells
+  int32_t* blocksum = nullptr;  // scan scratch
+  int32_t* perm = nullptr;     // sorted -> original
+  double* xs = nullptr;        // sorted wrapped coordinates SoA [3][n]
+  int32_t* sps = nullptr;      // sorted species
+  uint64_t* bounds = nullptr;  // 6 ordered-int min/max for open axes
+  Grid* grid = nullptr;        // device copy
+};
+
+struct PairModelDev {
+  int kind = -1;
+  int nz = 0;
+  double cutoff = 0;
+  std::vector<double> params;  // host copy
+  double* d_params = nullptr;  // SMOOTH: nz*nz amplitudes; LJ: eps, sigma
+};
+
+constexpr int kStages = 5;
+
+}  // namespace pfw
+
+struct pfw_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  pfw::Grid grid{};
+  pfw::SystemBuffers sys;
+  bool have_system = false;
+  double host_H[9];
+  int host_pb[3];
+  pfw::PairModelDev pair;
+  // outputs
+  double* d_F = nullptr;        // 3 x n original order
+  double* d_partials = nullptr;  // per-wave E + 6 W partials
+  int64_t partial_cap = 0;
+  double* d_out = nullptr;       // E, W[9], status
+  double* h_out = nullptr;       // pinned copy of d_out
+  uint32_t last_mask = 0;
+  bool have_results = false;
+  // timing
+  bool timing = false;
+  hipEvent_t ev[pfw::kStages + 1] = {};
+  double stage_ms[pfw::kStages] = {};
+};
+
+namespace pfw {
+// nl.hip
+void nl_configure(pfw_ctx* ctx, double cutoff);
+void nl_build(pfw_ctx* ctx);
+void nl_export_pairs(pfw_ctx* ctx, int32_t* i, int32_t* j, int32_t* S, double* R, int64_t cap,
+                     int64_t* count);
+void nl_free(pfw_ctx* ctx);
+// pair.hip
+void pair_run(pfw_ctx* ctx, uint32_t mask, int precision);
+void record_stage(pfw_ctx* ctx, int k);
+}  // namespace pfw

@@ -1,0 +1,142 @@
+"""Device workspace and directed neighbour snapshots.
+
+* ``Workspace`` stands where the reference's ``NeighbourList.Workspace`` /
+  ``GPUWorkspace`` stand (src/submodules/NeighbourList/storage.jl:24-72,
+  gpu.jl:61-116): reusable storage passed as ``neighbourlist=`` to the
+  calculators. It owns one device context; do not share it between concurrent
+  calls (README.md:217-219).
+* ``neighbour_list`` / ``neighbours`` mirror src/submodules/NeighbourList/interface.jl:29-179:
+  a materialised directed snapshot in original atom order with integer shifts
+  relative to the original (unwrapped) coordinates.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _capi
+from ._capi import ArgumentError, DimensionMismatch, Context, as_cell, as_positions, ptr
+from .system import System, validate_geometry
+
+PRECISIONS = {"float64": _capi.PFW_FP64, "fp64": _capi.PFW_FP64, "mixed": _capi.PFW_MIXED,
+              "float32": _capi.PFW_MIXED, np.float64: _capi.PFW_FP64, np.float32: _capi.PFW_MIXED,
+              float: _capi.PFW_FP64}
+
+
+def precision_mode(precision):
+    try:
+        return PRECISIONS[precision]
+    except (KeyError, TypeError):
+        raise ArgumentError("precision must be 'float64' (plain FP64) or 'mixed' (FP32 arithmetic, FP64 "
+                            "accumulation)") from None
+
+
+class Workspace:
+    """Reusable device storage for one system at a time.
+
+    ``precision``: ``"float64"`` or ``"mixed"``. ``device``: HIP device ordinal.
+    ``timings`` holds per-stage device milliseconds of the last call when
+    ``timing=True`` (the reference's ``w.timings``)."""
+
+    def __init__(self, precision="float64", device=0, timing=False):
+        self.precision = precision
+        self.mode = precision_mode(precision)
+        self.ctx = Context(device)
+        self._key = None
+        self._model_key = None
+        self.timings = {}
+        self.timing = bool(timing)
+        self.ctx.call("pfw_set_timing", int(self.timing))
+
+    # -- geometry ---------------------------------------------------------------------
+    def prepare(self, sys, cutoff, channels):
+        """Upload geometry; re-uses the cell grid when only positions changed."""
+        validate_geometry(sys)
+        X = as_positions(sys.positions)
+        H = np.asarray(sys.cell, dtype=np.float64)
+        pb = np.array([1 if p else 0 for p in sys.pbc], dtype=np.uint8)
+        key = (len(sys), H.tobytes(), pb.tobytes(), np.asarray(channels, np.int32).tobytes(), float(cutoff))
+        if key == self._key:
+            self.ctx.call("pfw_update_positions", ptr(X))
+            return
+        sp = np.ascontiguousarray(np.asarray(channels, dtype=np.int32))
+        self.ctx.call("pfw_set_system", ctypes.c_int64(len(sys)), ptr(X), ptr(sp, ctypes.c_int32), ptr(as_cell(H)),
+                      ptr(pb, ctypes.c_uint8), ctypes.c_double(float(cutoff)))
+        self._key = key
+        self._model_key = None
+
+    def grid_info(self):
+        nc = np.zeros(1, np.int64)
+        dims = np.zeros(3, np.int32)
+        mi = np.zeros(1, np.int32)
+        self.ctx.call("pfw_grid_info", ptr(nc, ctypes.c_int64), ptr(dims, ctypes.c_int32), ptr(mi, ctypes.c_int32))
+        return dict(ncells=int(nc[0]), dims=tuple(int(d) for d in dims), minimum_image=bool(mi[0]))
+
+    def export_pairs(self):
+        """Directed pairs (i, j, S, R) of the current system (0-based original indices)."""
+        count = np.zeros(1, np.int64)
+        rc = _capi.load().pfw_export_pairs(self.ctx.h, None, None, None, None, ctypes.c_int64(0),
+                                           ptr(count, ctypes.c_int64))
+        if rc not in (_capi.PFW_OK, _capi.PFW_CAPACITY):
+            _capi.check(self.ctx.h, rc)
+        m = int(count[0])
+        i = np.zeros(m, np.int32)
+        j = np.zeros(m, np.int32)
+        S = np.zeros((m, 3), np.int32)
+        R = np.zeros((m, 3), np.float64)
+        self.ctx.call("pfw_export_pairs", ptr(i, ctypes.c_int32), ptr(j, ctypes.c_int32), ptr(S, ctypes.c_int32),
+                      ptr(R), ctypes.c_int64(m), ptr(count, ctypes.c_int64))
+        return i, j, S, R
+
+    def read_timings(self):
+        ms = np.zeros(5)
+        _capi.load().pfw_get_timings(self.ctx.h, ptr(ms), 5)
+        self.timings = dict(zip(("binning", "scan", "scatter_sort", "interaction", "reduction"), ms.tolist()))
+        return self.timings
+
+
+class NeighbourSnapshot:
+    """Materialised directed neighbour snapshot (interface.jl:29-36)."""
+
+    def __init__(self, n, i, j, S, R):
+        order = np.lexsort((S[:, 2], S[:, 1], S[:, 0], j, i)) if len(i) else np.zeros(0, int)
+        i, j, S, R = i[order], j[order], S[order], R[order]
+        bounds = np.searchsorted(i, np.arange(n + 1))
+        self.indices = [j[bounds[k]:bounds[k + 1]].copy() for k in range(n)]
+        self.displacements = [R[bounds[k]:bounds[k + 1]].copy() for k in range(n)]
+        self.shifts = [S[bounds[k]:bounds[k + 1]].copy() for k in range(n)]
+
+
+def neighbours(snapshot, i):
+    """``(js, Rs, Ss)`` for atom ``i`` (0-based), as independent arrays (interface.jl:173-179)."""
+    if not 0 <= i < len(snapshot.indices):
+        raise IndexError(i)
+    return snapshot.indices[i].copy(), snapshot.displacements[i].copy(), snapshot.shifts[i].copy()
+
+
+def neighbour_list(X, cutoff, cell=None, pbc=None, *, workspace=None, device=0):
+    """Directed neighbour snapshot on the GPU.
+
+    ``neighbour_list(X, cutoff, cell_rows, pbc)`` takes a 3x3 cell with lattice
+    vectors in ROWS, like the reference's raw-array API (interface.jl:56-93);
+    ``neighbour_list(system, cutoff)`` takes a ``System``. For raw input
+    ``Rs[k] = X[js[k]] - X[i] + cell.T @ Ss[k]``. All images strictly inside the
+    cutoff are included, including both +/- self images; only the zero-shift
+    self pair is excluded."""
+    if isinstance(X, System):
+        sys = X
+    else:
+        cell = np.asarray(cell, dtype=np.float64)
+        if cell.shape != (3, 3):
+            raise DimensionMismatch("cell must be 3x3 with vectors in rows")
+        if pbc is None or len(pbc) != 3:
+            raise DimensionMismatch("pbc must contain three flags")
+        X = np.asarray(X, dtype=np.float64)
+        if X.ndim != 2 or X.shape[1] != 3:
+            raise DimensionMismatch("positions must be an (n, 3) array")
+        sys = System(X, cell.T, pbc, ["H"] * len(X))
+    if not (np.isfinite(cutoff) and cutoff > 0):
+        raise ArgumentError("cutoff must be finite and positive (Å)")
+    w = workspace or Workspace(device=device)
+    w.prepare(sys, cutoff, np.zeros(len(sys), np.int32))
+    i, j, S, R = w.export_pairs()
+    return NeighbourSnapshot(len(sys), i.astype(np.int64), j.astype(np.int64), S.astype(np.int64), R)

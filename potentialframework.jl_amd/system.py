@@ -1,0 +1,73 @@
+"""Minimal AtomsBase-like system used at the host boundary.
+
+Mirrors what the reference reads from an ``AtomsBase.AbstractSystem{3}``
+(src/submodules/NeighbourList/storage.jl:135-170): positions in Å, cell vectors
+(lattice vectors in COLUMNS of ``cell``), periodicity and species labels.
+Units are plain Å and eV; there is no Unitful layer.
+"""
+import numpy as np
+
+from ._capi import ArgumentError, DimensionMismatch
+
+
+class System:
+    def __init__(self, positions, cell, pbc, species):
+        X = np.array(positions, dtype=np.float64, copy=True)
+        if X.ndim != 2 or X.shape[1] != 3:
+            raise DimensionMismatch("positions must be an (n, 3) array")
+        H = np.array(cell, dtype=np.float64, copy=True)
+        if H.shape != (3, 3):
+            raise DimensionMismatch("cell must be 3x3 with lattice vectors in columns")
+        if len(pbc) != 3:
+            raise DimensionMismatch("pbc must contain three flags")
+        species = list(species)
+        if len(species) != len(X):
+            raise DimensionMismatch("one species label per atom")
+        self.positions = X
+        self.cell = H
+        self.pbc = tuple(bool(p) for p in pbc)
+        self.labels = species
+
+    def __len__(self):
+        return len(self.positions)
+
+    def position(self, i):
+        return self.positions[i]
+
+    def cell_vectors(self):
+        return tuple(self.cell[:, a] for a in range(3))
+
+    def periodicity(self):
+        return self.pbc
+
+    def species(self, i):
+        return self.labels[i]
+
+
+def channels(labels, known):
+    """Map species labels onto model channels (bind_species,
+    src/submodules/PairPotentials/models.jl:50-58). Unknown labels raise ArgumentError."""
+    index = {s: k for k, s in enumerate(known)}
+    out = np.empty(len(labels), dtype=np.int32)
+    for i, s in enumerate(labels):
+        k = index.get(s)
+        if k is None:
+            raise ArgumentError(f"potential has no parameters for species {s}")
+        out[i] = k
+    return out
+
+
+def unique_in_order(labels):
+    seen = []
+    for s in labels:
+        if s not in seen:
+            seen.append(s)
+    return seen
+
+
+def validate_geometry(sys):
+    if not np.all(np.isfinite(sys.positions)):
+        raise ArgumentError("positions must be finite")
+    for a in range(3):
+        if sys.pbc[a] and not np.all(np.isfinite(sys.cell[:, a])):
+            raise ArgumentError("periodic cell vectors must be finite")

@@ -1,0 +1,81 @@
+"""Seeded synthetic workloads (geometry only; no files are read).
+
+* ``argon_fcc``: FCC lattice a = 5.26 Å, m^3 conventional cells, 4 m^3 atoms in
+  a cubic periodic box, Gaussian displacements (the 4,000-atom Lennard-Jones case
+  at m = 10: eps 0.0104 eV, sigma 3.40 Å, cutoff 8.5 Å, energy shifted).
+* ``b2_triclinic``: two-species B2 lattice a = 3.0 Å (species 0 at the origin,
+  1 at the body centre) on nx x ny x nz conventional cells, lattice vectors
+  sheared by seeded uniform +/-5 % off-diagonal strains, Gaussian displacements
+  0.08 Å (the 256,000-atom SmoothPair case at 40 x 40 x 80, cutoff 6.0 Å,
+  amplitudes [1.0 0.7; 0.7 1.3] eV).
+* ``bcc_binary``: BCC a = 3.2 Å on nx x ny x nz cells with iid 50/50 species and
+  Gaussian displacements 0.05 Å (the 32,768-atom ACE case at 32 x 32 x 16).
+* ``fixture``: the reference tests' deterministic scatter (test/fixtures.jl:60-67).
+
+Positions are (n, 3) arrays in Å; cells have lattice vectors in columns.
+"""
+import numpy as np
+
+
+def argon_fcc(m=10, a=5.26, sigma=0.05, seed=1):
+    rng = np.random.default_rng(seed)
+    basis = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]])
+    g = np.stack(np.meshgrid(np.arange(m), np.arange(m), np.arange(m), indexing="ij"), -1).reshape(-1, 3)
+    frac = (g[:, None, :] + basis[None, :, :]).reshape(-1, 3)
+    X = frac * a + rng.normal(0.0, sigma, size=frac.shape)
+    H = np.eye(3) * (m * a)
+    return X, H, (True, True, True), np.zeros(len(X), np.int32)
+
+
+def b2_triclinic(nx=40, ny=40, nz=80, a=3.0, shear=0.05, sigma=0.08, seed=2):
+    rng = np.random.default_rng(seed)
+    L = np.array([nx, ny, nz], float) * a
+    S = rng.uniform(-shear, shear, size=(3, 3))
+    np.fill_diagonal(S, 0.0)
+    H = (np.eye(3) + S) * L[None, :]  # column b = L_b (e_b + sum_a S_ab e_a)
+    basis = np.array([[0, 0, 0], [0.5, 0.5, 0.5]])
+    g = np.stack(np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij"), -1).reshape(-1, 3)
+    cells = (g[:, None, :] + basis[None, :, :]).reshape(-1, 3)
+    frac = cells / np.array([nx, ny, nz], float)
+    X = frac @ H.T + rng.normal(0.0, sigma, size=frac.shape)
+    species = np.tile(np.array([0, 1], np.int32), len(g))
+    return X, H, (True, True, True), species
+
+
+def bcc_binary(nx=32, ny=32, nz=16, a=3.2, sigma=0.05, seed=3):
+    rng = np.random.default_rng(seed)
+    basis = np.array([[0, 0, 0], [0.5, 0.5, 0.5]])
+    g = np.stack(np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij"), -1).reshape(-1, 3)
+    frac = (g[:, None, :] + basis[None, :, :]).reshape(-1, 3)
+    X = frac * a + rng.normal(0.0, sigma, size=frac.shape)
+    H = np.diag(np.array([nx, ny, nz], float) * a)
+    species = rng.integers(0, 2, size=len(X)).astype(np.int32)
+    return X, H, (True, True, True), species
+
+
+def fixture(n, L=9.0, pbc=(True, True, True), cluster=False, mixed=True):
+    i = np.arange(1, n + 1)[:, None]
+    a = np.arange(1, 4)[None, :]
+    X = L * np.mod(np.sin(i * (a + 0.71)) * 437.13, 1.0)
+    if cluster:
+        X[: (n + 1) // 2] *= 0.18
+    H = np.eye(3) * L
+    species = np.array([1 if (mixed and k % 2 == 1) else 0 for k in range(1, n + 1)], np.int32)
+    return X.astype(float), H, tuple(pbc), species
+
+
+# The reference's general-cell test matrices (test/periodic_images.jl:63-69), columns = lattice vectors.
+IMAGE_TEST_CELLS = [
+    np.array([[7., 2.5, -1.], [0., 6., 1.5], [0., 0., 5.]]),
+    np.array([[4., 3.8, 0.], [0., 0.6, 0.], [0., 0., 4.]]),
+    np.array([[1.2, 0.4, 0.], [0., 1.3, 0.3], [0., 0., 1.1]]),
+    np.array([[2., 0, 0], [0, 2., 0], [0, 0, 2.]]),
+    np.array([[0., -5., 0.], [6., 0., 1.], [0., 0., 7.]]),
+]
+
+
+def image_test_system(H, n=7):
+    frac = np.array([[np.mod(np.sin((i + 1) * (a + 1 + 0.37)) * 31.3, 1.0) for a in range(3)] for i in range(n)])
+    X = frac @ np.asarray(H).T
+    species = np.array([k % 2 for k in range(n)], np.int32)
+    return X, species
