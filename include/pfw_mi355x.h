@@ -111,6 +111,60 @@ int pfw_get_timings(pfw_ctx* ctx, double* ms, int n);
 /* Synchronise the context's stream. */
 int pfw_synchronize(pfw_ctx* ctx);
 
+/* ---------------------------------------------------------------- ACE
+ * Resolved ACE model, schema v1 (src/submodules/ACE/resolved.jl:18-142):
+ * piecewise-cubic Agnesi radial splines per (centre, neighbour) species
+ * channel (index centre + nz*neighbour), real L2 spherical harmonics in
+ * l*l+l+m order, density (radial index, harmonic index) pairs, product
+ * multisets of density indices (order <= 4), sparse coupling C (nbasis x
+ * nprod) and readout WB (nbasis x nz), Wpair (nq x nz). Default basis policy:
+ * every readout row enabled, no species projection, no order restriction. */
+enum { PFW_ENV_POLY2SX = 0, PFW_ENV_POLY1SR = 1, PFW_ENV_ACE1_POLY1SR = 2 };
+
+typedef struct {
+  int32_t nr;                    /* radial functions per channel */
+  int32_t nnodes;                /* spline nodes per channel (>= 2, first -1, last 1) */
+  const double* transform;       /* nz*nz x 8: p q a rin r0 yin ycut rcut */
+  const int32_t* envelope_kind;  /* nz*nz: PFW_ENV_* */
+  const double* envelope;        /* nz*nz x 5: (x1 x2 p1 p2 s) | (rcut p) | (rcut r0 p) */
+  const double* nodes;           /* nz*nz x nnodes */
+  const double* coefficients;    /* nz*nz x (4 x nr x (nnodes-1)), power fastest */
+} pfw_ace_radial;
+
+typedef struct {
+  int32_t nz, lmax, ndens, nprod, nbasis;
+  const double* reference;       /* nz reference energies (eV) */
+  pfw_ace_radial radial, pair;   /* pair.nr may be 0 */
+  const int32_t* density;        /* ndens x 2: (radial index, harmonic index), 0-based */
+  const int32_t* product_ptr;    /* nprod + 1 */
+  const int32_t* product_idx;    /* 0-based density indices */
+  int64_t coupling_nnz;
+  const int32_t* coupling_rows;  /* 0-based basis row */
+  const int32_t* coupling_cols;  /* 0-based product column */
+  const double* coupling_vals;
+  const double* WB;              /* nbasis x nz, column-major */
+  const double* Wpair;           /* pair.nr x nz, column-major */
+} pfw_ace_model;
+
+/* Device tables of a resolved model: adapt_model / pack_model
+ * (src/submodules/ACE/adaptation.jl:37-98). The search cutoff is the largest
+ * radial cutoff (src/submodules/ACE/cpu.jl:316); the system must have been set
+ * with that cutoff. */
+int pfw_set_ace_model(pfw_ctx* ctx, const pfw_ace_model* model);
+/* New readout parameters; refreshes the precontraction w = C^T WB
+ * (NI.update_prepared!, src/submodules/ACE/cpu.jl:330-335; ace_weights!, gpu.jl:172-181). */
+int pfw_set_ace_parameters(pfw_ctx* ctx, const double* WB, const double* Wpair);
+/* One ACE step on the device: fresh cell list and directed environments,
+ * site densities, products, readout and force pullback, force gather over
+ * reverse edges (no atomics), FP64 reductions and the status check. Replaces
+ * execute_gpu! (src/submodules/ACE/gpu.jl:329-391) and the CPU
+ * execute_prepared! (src/submodules/ACE/cpu.jl:253-275). */
+int pfw_ace_step(pfw_ctx* ctx, uint32_t request_mask, int precision);
+/* pfw_ace_step + outputs (prepared_result, src/submodules/ACE/cpu.jl:279-281). */
+int pfw_ace_evaluate(pfw_ctx* ctx, uint32_t request_mask, int precision, double* E, double* F, double* W);
+/* Per-site energies of the last ACE step, original order (n doubles). */
+int pfw_ace_site_energies(pfw_ctx* ctx, double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -34,6 +34,8 @@ from .pair import (  # noqa: F401
     evaluate_,
     evaluate,
 )
+from .ace import ACECalculator, ResolvedModel, load_resolved, ace_evaluate  # noqa: F401
+from . import ace, ace_basis  # noqa: F401
 from .calculators import (  # noqa: F401
     potential_energy,
     forces,

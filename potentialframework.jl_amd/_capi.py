@@ -68,6 +68,11 @@ SIGNATURES = [
     ("pfw_set_timing", ctypes.c_int, [_P, ctypes.c_int]),
     ("pfw_get_timings", ctypes.c_int, [_P, _D, ctypes.c_int]),
     ("pfw_synchronize", ctypes.c_int, [_P]),
+    ("pfw_set_ace_model", ctypes.c_int, [_P, ctypes.c_void_p]),
+    ("pfw_set_ace_parameters", ctypes.c_int, [_P, _D, _D]),
+    ("pfw_ace_step", ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_int]),
+    ("pfw_ace_evaluate", ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_int, _D, _D, _D]),
+    ("pfw_ace_site_energies", ctypes.c_int, [_P, _D]),
 ]
 
 

@@ -1,0 +1,253 @@
+"""ACE on the GPU: resolved-model loading and the calculator boundary.
+
+Mirrors src/submodules/ACE/resolved.jl:57-142 (``load_resolved`` / ``read_resolved``
+for schema v1, default basis policy), src/submodules/ACE/environments.jl:1-10
+(strict element channels) and src/submodules/ACE/calculators.jl (``ACECalculator``
+with ``energy``/``forces``/``virial`` results in original atom order).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _capi
+from ._capi import ArgumentError, DimensionMismatch, ptr
+from .neighbourlist import Workspace
+from .pair import ENERGY, FORCES, VIRIAL, request_mask
+
+try:
+    import tomllib as _toml
+except ImportError:  # Python 3.10
+    import tomli as _toml
+
+ENVELOPES = {"PolyEnvelope2sX": 0, "PolyEnvelope1sR": 1, "ACE1_PolyEnvelope1sR": 2}
+ELEMENTS = ["H", "He", "Li", "Be", "B", "C", "N", "O", "F", "Ne", "Na", "Mg", "Al", "Si", "P", "S", "Cl", "Ar",
+            "K", "Ca", "Sc", "Ti", "V", "Cr", "Mn", "Fe", "Co", "Ni", "Cu", "Zn", "Ga", "Ge", "As", "Se", "Br", "Kr",
+            "Rb", "Sr", "Y", "Zr", "Nb", "Mo", "Tc", "Ru", "Rh", "Pd", "Ag", "Cd", "In", "Sn", "Sb", "Te", "I", "Xe",
+            "Cs", "Ba", "La", "Ce", "Pr", "Nd", "Pm", "Sm", "Eu", "Gd", "Tb", "Dy", "Ho", "Er", "Tm", "Yb", "Lu",
+            "Hf", "Ta", "W", "Re", "Os", "Ir", "Pt", "Au", "Hg"]
+
+
+def atomic_number(label):
+    """Element labels only: isotopes or custom names are rejected (environments.jl:1-10)."""
+    if isinstance(label, (int, np.integer)):
+        return int(label)
+    try:
+        return ELEMENTS.index(label) + 1
+    except ValueError:
+        raise ArgumentError(f"ACE element channels do not accept isotopes or named species: {label}") from None
+
+
+class RadialFamily:
+    def __init__(self, D, nz):
+        if D.get("kind") != "piecewise_cubic_agnesi":
+            raise ArgumentError("unsupported resolved radial kind")
+        self.spec = [(int(b["n"]), int(b["l"])) for b in D["spec"]]
+        self.nr = len(self.spec)
+        seen = {}
+        for ch in D["channels"]:
+            key = (int(ch["centre"]) - 1, int(ch["neighbour"]) - 1)
+            if not (0 <= key[0] < nz and 0 <= key[1] < nz) or key in seen:
+                raise ArgumentError("invalid/duplicate radial species channel")
+            seen[key] = ch
+        if len(seen) != nz * nz:
+            raise ArgumentError("missing radial species channels")
+        nn = len(next(iter(seen.values()))["nodes"]) if seen else 2
+        self.nnodes = nn
+        self.transform = np.zeros((nz * nz, 8))
+        self.envelope_kind = np.zeros(nz * nz, np.int32)
+        self.envelope = np.zeros((nz * nz, 5))
+        self.nodes = np.zeros((nz * nz, nn))
+        self.coefficients = np.zeros((nz * nz, 4 * self.nr * (nn - 1)))
+        for (i, j), ch in seen.items():
+            c = i + nz * j
+            t = ch["transform"]
+            if t.get("normalization_rin", t["rin"]) != t["rin"]:
+                raise ArgumentError("unsupported differing transform origins")
+            self.transform[c] = [t["p"], t["q"], t["a"], t["rin"], t["r0"], t["yin"], t["ycut"], t["rcut"]]
+            kind = ch["envelope_kind"]
+            if kind not in ENVELOPES:
+                raise ArgumentError(f"unsupported resolved envelope {kind}")
+            e = ch["envelope"]
+            self.envelope_kind[c] = ENVELOPES[kind]
+            if kind == "PolyEnvelope2sX":
+                self.envelope[c] = [e["x1"], e["x2"], e["p1"], e["p2"], e["s"]]
+            elif kind == "PolyEnvelope1sR":
+                self.envelope[c, :2] = [e["rcut"], e["p"]]
+            else:
+                self.envelope[c, :3] = [e["rcut"], e["r0"], e["p"]]
+            nodes = np.asarray(ch["nodes"], float)
+            if len(nodes) != nn or nodes[0] != -1 or nodes[-1] != 1 or np.any(np.diff(nodes) <= 0):
+                raise ArgumentError("invalid spline nodes")
+            if list(ch["coefficient_shape"]) != [4, self.nr, nn - 1]:
+                raise DimensionMismatch("invalid spline coefficients")
+            coef = np.asarray(ch["coefficients"], float)
+            if not np.all(np.isfinite(coef)):
+                raise ArgumentError("nonfinite spline coefficients")
+            self.nodes[c] = nodes
+            self.coefficients[c] = coef
+        self.cutoff = float(self.transform[:, 7].max()) if nz else 0.0
+
+
+class ResolvedModel:
+    """Device-ready resolved ACE model (schema v1, default policy)."""
+
+    def __init__(self, D):
+        if D.get("schema") != "PotentialFrameWork.resolved_ACE" or D.get("schema_version") != 1:
+            raise ArgumentError("unsupported resolved ACE schema")
+        self.atomic_numbers = [int(v) for v in D["species_atomic_numbers"]]
+        nz = len(self.atomic_numbers)
+        if len(set(self.atomic_numbers)) != nz or min(self.atomic_numbers) <= 0:
+            raise ArgumentError("invalid species channels")
+        ang = D["angular"]
+        if ang["kind"] != "real_spherical" or ang["normalization"] != "L2" or ang["ordering"] != "l*l+l+m+1":
+            raise ArgumentError("unsupported angular convention")
+        self.nz = nz
+        self.lmax = int(ang["lmax"])
+        self.radial = RadialFamily(D["radial"], nz)
+        self.pair = RadialFamily(D["pair_radial"], nz)
+        self.density = np.array([[int(a) - 1, int(b) - 1] for a, b in D["density_indices"]], np.int32).reshape(-1, 2)
+        self.products = [[int(v) - 1 for v in p] for p in D["products"]]
+        c = D["coupling"]
+        self.nbasis, nprod = [int(v) for v in c["shape"]]
+        if nprod != len(self.products):
+            raise DimensionMismatch("coupling/product dimensions")
+        self.c_rows = np.asarray(c["rows"], np.int32) - 1
+        self.c_cols = np.asarray(c["columns"], np.int32) - 1
+        self.c_vals = np.asarray(c["values"], float)
+        p = D["parameters"]
+        if p["ordering"] != "vec(WB), vec(Wpair); Julia column-major":
+            raise ArgumentError("unsupported parameter ordering")
+        flat = np.asarray(p["flat"], float)
+        nq = self.pair.nr
+        if len(flat) != (self.nbasis + nq) * nz:
+            raise DimensionMismatch("parameter length")
+        self.WB = flat[: self.nbasis * nz].reshape(nz, self.nbasis).T.copy()
+        self.Wpair = flat[self.nbasis * nz:].reshape(nz, nq).T.copy()
+        self.reference = np.asarray(D["reference_eV"], float)
+        if len(self.reference) != nz:
+            raise DimensionMismatch("reference energy channels")
+        self.cutoff = max(self.radial.cutoff, self.pair.cutoff)
+        self.source_sha256 = D.get("source_sha256")
+
+    def parameters(self):
+        return {"WB": self.WB.copy(), "Wpair": self.Wpair.copy()}
+
+    def channels(self, labels):
+        z = [atomic_number(s) for s in labels]
+        try:
+            return np.array([self.atomic_numbers.index(v) for v in z], np.int32)
+        except ValueError:
+            raise ArgumentError("ACE model has no element channel for a species in the system") from None
+
+    def _struct(self):
+        keep = []
+
+        def a(x, dt):
+            x = np.ascontiguousarray(np.asarray(x, dtype=dt))
+            keep.append(x)
+            return x.ctypes.data
+
+        def fam(f):
+            return _Radial(f.nr, f.nnodes, a(f.transform, np.float64), a(f.envelope_kind, np.int32),
+                           a(f.envelope, np.float64), a(f.nodes, np.float64), a(f.coefficients, np.float64))
+
+        pp = np.zeros(len(self.products) + 1, np.int32)
+        pp[1:] = np.cumsum([len(p) for p in self.products])
+        pidx = np.concatenate([np.asarray(p, np.int32) for p in self.products]) if self.products else np.zeros(0)
+        m = _Model(self.nz, self.lmax, len(self.density), len(self.products), self.nbasis,
+                   a(self.reference, np.float64), fam(self.radial), fam(self.pair), a(self.density, np.int32),
+                   a(pp, np.int32), a(pidx, np.int32), len(self.c_vals), a(self.c_rows, np.int32),
+                   a(self.c_cols, np.int32), a(self.c_vals, np.float64), a(self.WB.T, np.float64),
+                   a(self.Wpair.T, np.float64))
+        return m, keep
+
+
+class _Radial(ctypes.Structure):
+    _fields_ = [("nr", ctypes.c_int32), ("nnodes", ctypes.c_int32), ("transform", ctypes.c_void_p),
+                ("envelope_kind", ctypes.c_void_p), ("envelope", ctypes.c_void_p), ("nodes", ctypes.c_void_p),
+                ("coefficients", ctypes.c_void_p)]
+
+
+class _Model(ctypes.Structure):
+    _fields_ = [("nz", ctypes.c_int32), ("lmax", ctypes.c_int32), ("ndens", ctypes.c_int32),
+                ("nprod", ctypes.c_int32), ("nbasis", ctypes.c_int32), ("reference", ctypes.c_void_p),
+                ("radial", _Radial), ("pair", _Radial), ("density", ctypes.c_void_p),
+                ("product_ptr", ctypes.c_void_p), ("product_idx", ctypes.c_void_p), ("coupling_nnz", ctypes.c_int64),
+                ("coupling_rows", ctypes.c_void_p), ("coupling_cols", ctypes.c_void_p),
+                ("coupling_vals", ctypes.c_void_p), ("WB", ctypes.c_void_p), ("Wpair", ctypes.c_void_p)]
+
+
+def load_resolved(source):
+    """``load_resolved(path)`` (resolved.jl:94-102); also accepts the parsed dictionary.
+    Returns ``(model, parameters, state)``."""
+    if isinstance(source, dict):
+        D = source
+    else:
+        with open(source, "rb") as f:
+            D = _toml.load(f)
+    m = ResolvedModel(D)
+    return m, m.parameters(), {}
+
+
+class ACECalculator:
+    """``ACECalculator(model, ps; precision)`` (calculators.jl:6-29) on one device context."""
+
+    def __init__(self, model, parameters=None, precision="float64", device=0, workspace=None):
+        self.model = model
+        self.workspace = workspace or Workspace(precision=precision, device=device)
+        self.precision = self.workspace.precision
+        st, keep = model._struct()
+        self.workspace.ctx.call("pfw_set_ace_model", ctypes.byref(st))
+        del keep
+        self.parameters = model.parameters()
+        if parameters is not None:
+            self.set_parameters(parameters)
+
+    def cutoff(self):
+        return self.model.cutoff
+
+    def set_parameters(self, ps):
+        """``update_prepared!``: new WB/Wpair, precontraction refreshed on the host."""
+        WB = np.asarray(ps["WB"], float)
+        Wp = np.asarray(ps.get("Wpair", self.model.Wpair), float)
+        if WB.shape != self.model.WB.shape or Wp.shape != self.model.Wpair.shape:
+            raise DimensionMismatch("readout shape")
+        self.workspace.ctx.call("pfw_set_ace_parameters", ptr(np.ascontiguousarray(WB.T)),
+                                ptr(np.ascontiguousarray(Wp.T)))
+        self.parameters = {"WB": WB.copy(), "Wpair": Wp.copy()}
+
+    def evaluate(self, sys, request=("energy", "forces", "virial"), site_energies=False):
+        mask, names = request_mask(request)
+        ch = self.model.channels(sys.labels)
+        w = self.workspace
+        w.prepare(sys, self.model.cutoff, ch)
+        n = len(sys)
+        E = np.zeros(1)
+        F = np.zeros((n, 3))
+        W = np.zeros(9)
+        w.ctx.call("pfw_ace_evaluate", mask, w.mode, ptr(E), ptr(F), ptr(W))
+        if w.timing:
+            w.read_timings()
+        out = {}
+        if ENERGY in names:
+            out["energy"] = float(E[0])
+        if FORCES in names:
+            out["forces"] = F.T.copy()
+        if VIRIAL in names:
+            out["virial"] = W.reshape(3, 3).T.copy()
+        if site_energies:
+            s = np.zeros(n)
+            w.ctx.call("pfw_ace_site_energies", ptr(s))
+            out["site_energies"] = s
+        return out
+
+
+def ace_evaluate(X, model, ps, species, H, pbc, precision="float64", calculator=None):
+    """Numeric ``ace_evaluate(X, model, ps, env, H)`` (serial.jl:337-370) on the GPU:
+    X (n, 3) Å, species labels, H columns = lattice vectors. Returns E, F (3 x n), W."""
+    from .system import System
+    calc = calculator or ACECalculator(model, ps, precision=precision)
+    if calculator is not None and ps is not None:
+        calc.set_parameters(ps)
+    r = calc.evaluate(System(X, H, pbc, species))
+    return r

@@ -1,0 +1,915 @@
+// ACE site pipeline on gfx950: directed environments from the cell list,
+// Agnesi-spline radials, real L2 spherical harmonics with Cartesian gradients,
+// atomic-base densities, product basis, precontracted linear readout and the
+// force pullback. Neighbour forces are gathered over reverse edges, so no
+// global atomics are used anywhere.
+//
+// Reference: site_data / site_reverse (src/submodules/ACE/serial.jl:260-310),
+// NI.evaluate_site! (src/submodules/ACE/cpu.jl:162-237), the radial spline
+// and envelopes (src/submodules/ACE/resolved.jl:156-250), the Y_lm recurrence
+// (src/submodules/ACE/angular_blocks.jl:25-81), unrestricted_products!
+// (src/submodules/ACE/projection.jl:39-56), reduce_site!
+// (src/submodules/NeighbourList/execution.jl:105-122); the KA kernels being
+// replaced are src/submodules/ACE/gpu.jl:99-270.
+//
+// Layout: one 64-lane wavefront per site (sorted order). Phase A: lanes are
+// edges; each lane evaluates its radial row and Y_lm into LDS (row stride 65
+// to spread banks), then lanes are densities and sum R Y over the chunk.
+// Phase B: lanes are products for the energy and densities for dE/dA (a
+// gather over a per-density CSR, so no LDS atomics). Phase C: lanes are edges
+// again; Y and grad Y live in registers (compile-time lmax), the radial rows
+// are recomputed, and g = r^ sum_r R'_r T_r + sum_m U_m grad Y_m.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "pfw_device.h"
+
+namespace pfw {
+
+constexpr int kStride = kWave + 1;
+
+template <typename T>
+struct FamArgs {
+  int nr, nnodes, nint, uniform;
+  const int* pq;
+  const int* env_kind;
+  const int* env_pow;
+  const T* par;    // nch x 12
+  const T* nodes;  // nch x nnodes
+  const T* coef;   // nch x nint x nr x 4
+};
+
+template <typename T>
+struct AceArgs {
+  int64_t n;
+  int nz, ndens, nprod, ny;
+  const int32_t* perm;
+  const int32_t* sps;
+  const int32_t* estart;
+  const int32_t* enb;
+  const double* eR;
+  const double* reference;
+  FamArgs<T> rad, pair;
+  const int32_t* dens;
+  const int32_t* dens_of;
+  const uint64_t* rmask;
+  const int32_t* prod;
+  const int32_t* darow;
+  const int32_t* dent;
+  const T* w;   // nz x nprod
+  const T* wp;  // nz x nq
+  int grads;
+  double* siteE;
+  double* siteW;
+  double* eg;
+};
+
+template <typename T>
+__device__ inline T ipow(T x, int n) {
+  T v = (T)1;
+  for (int k = 0; k < n; ++k) v *= x;
+  return v;
+}
+
+// Agnesi coordinate x(r), dx/dr, envelope and its r-derivative, spline
+// interval and local coordinate (resolved.jl:156-250).
+template <typename T>
+struct RadialPoint {
+  int k;
+  T u, e, de, dx, invstep;
+};
+
+template <typename T>
+__device__ inline RadialPoint<T> radial_point(const FamArgs<T>& f, int ch, T r) {
+  const T* par = f.par + 12 * ch;
+  const int p = f.pq[2 * ch], q = f.pq[2 * ch + 1];
+  const T a = par[0], rin = par[1], r0 = par[2], yin = par[3], ycut = par[4];
+  const T s = (r - rin) / (r0 - rin);
+  const int h = q - p;
+  const T sq = ipow(s, q), sh = ipow(s, h);
+  const T den = (T)1 + sh;
+  const T y = r <= rin ? (T)1 : (T)1 / ((T)1 + a * sq / den);
+  T x = (T)-1 + (T)2 * (y - yin) / (ycut - yin);
+  x = x < (T)-1 ? (T)-1 : (x > (T)1 ? (T)1 : x);
+  T dx = 0;
+  if (!(r <= rin || x <= (T)-1 || x >= (T)1)) {
+    const T u = sq / den;
+    const T du = ((T)q * ipow(s, q - 1) / den - (h == 0 ? (T)0 : (T)h * ipow(s, q + h - 1) / (den * den))) / (r0 - rin);
+    const T t = (T)1 + a * u;
+    dx = (T)-2 * a * du / (t * t * (ycut - yin));
+  }
+  RadialPoint<T> o;
+  o.dx = dx;
+  const int kind = f.env_kind[ch];
+  const int e0 = f.env_pow[2 * ch], e1 = f.env_pow[2 * ch + 1];
+  if (kind == PFW_ENV_POLY2SX) {
+    const T x1 = par[6], x2 = par[7], sc = par[8];
+    if (x1 < x && x < x2) {
+      o.e = sc * ipow(x - x1, e0) * ipow(x2 - x, e1);
+      o.de = sc * ((T)e0 * ipow(x - x1, e0 - 1) * ipow(x2 - x, e1) - (T)e1 * ipow(x - x1, e0) * ipow(x2 - x, e1 - 1)) * dx;
+    } else {
+      o.e = 0;
+      o.de = 0;
+    }
+  } else if (kind == PFW_ENV_POLY1SR) {
+    const T rc = par[6];
+    if (r >= rc) {
+      o.e = o.de = 0;
+    } else {
+      const T ss = r / rc;
+      const T sp = ipow(ss, e0);
+      o.e = ((T)1 / sp - (T)1) * ((T)1 - ss);
+      o.de = (-(T)e0 / (sp * ss) * ((T)1 - ss) - ((T)1 / sp - (T)1)) / rc;
+    }
+  } else {
+    const T rc = par[6], rr0 = par[7];
+    if (r > rc) {
+      o.e = o.de = 0;
+    } else {
+      const T ss = r / rr0, sc = rc / rr0;
+      o.e = (T)1 / ipow(ss, e0) - (T)1 / ipow(sc, e0) + (T)e0 / ipow(sc, e0 + 1) * (ss - sc);
+      o.de = (T)e0 / rr0 * ((T)1 / ipow(sc, e0 + 1) - (T)1 / ipow(ss, e0 + 1));
+    }
+  }
+  const T* nodes = f.nodes + (int64_t)f.nnodes * ch;
+  int k;
+  if (f.uniform) {
+    k = (int)((x + (T)1) * (T)0.5 * (T)(f.nnodes - 1));
+    k = k < 0 ? 0 : (k > f.nint - 1 ? f.nint - 1 : k);
+    if (k > 0 && x < nodes[k]) --k;
+    if (k < f.nint - 1 && x >= nodes[k + 1]) ++k;
+  } else {
+    int lo = 0, hi = f.nnodes;  // largest index with nodes[idx] <= x
+    while (hi - lo > 1) {
+      int mid = (lo + hi) >> 1;
+      if (nodes[mid] <= x)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    k = lo > f.nint - 1 ? f.nint - 1 : lo;
+  }
+  const T step = nodes[k + 1] - nodes[k];
+  o.k = k;
+  o.invstep = (T)1 / step;
+  o.u = (x - nodes[k]) * o.invstep;
+  return o;
+}
+
+template <typename T>
+__device__ inline void radial_value(const FamArgs<T>& f, int ch, const RadialPoint<T>& P, int n, T& v, T& d) {
+  const T* c = f.coef + (((int64_t)ch * f.nint + P.k) * f.nr + n) * 4;
+  const T c0 = c[0], c1 = c[1], c2 = c[2], c3 = c[3];
+  const T poly = ((c3 * P.u + c2) * P.u + c1) * P.u + c0;
+  const T dpoly = (((T)3 * c3 * P.u + (T)2 * c2) * P.u + c1) * P.invstep;
+  v = P.e * poly;
+  d = P.de * poly + P.e * dpoly * P.dx;
+}
+
+template <typename T>
+struct Jet {
+  T v, g0, g1, g2;
+};
+template <typename T>
+__device__ inline Jet<T> jm(const Jet<T>& a, const Jet<T>& b) {
+  return {a.v * b.v, a.g0 * b.v + a.v * b.g0, a.g1 * b.v + a.v * b.g1, a.g2 * b.v + a.v * b.g2};
+}
+template <typename T>
+__device__ inline Jet<T> js(T s, const Jet<T>& a) {
+  return {s * a.v, s * a.g0, s * a.g1, s * a.g2};
+}
+template <typename T>
+__device__ inline Jet<T> jadd(const Jet<T>& a, const Jet<T>& b) {
+  return {a.v + b.v, a.g0 + b.g0, a.g1 + b.g1, a.g2 + b.g2};
+}
+template <typename T>
+__device__ inline Jet<T> jsub(const Jet<T>& a, const Jet<T>& b) {
+  return {a.v - b.v, a.g0 - b.g0, a.g1 - b.g1, a.g2 - b.g2};
+}
+
+// Real L2 Y_lm (no Condon-Shortley phase, index l*l+l+m) of the direction of
+// (dx, dy, dz) and their Cartesian gradients (angular_blocks.jl:36-81).
+// Fl holds F[l][m] = sqrt((2l+1)/2pi) * prod(-1/sqrt((l+m')(l+1-m'))).
+template <typename T, int L>
+__device__ inline void ylm(T dx, T dy, T dz, const T* Fl, T* Y, T* G) {
+  const T r = sqrt(dx * dx + dy * dy + dz * dz);
+  const T ri = (T)1 / r;
+  const T ux = dx * ri, uy = dy * ri, uz = dz * ri;
+  const Jet<T> x{ux, ((T)1 - ux * ux) * ri, -ux * uy * ri, -ux * uz * ri};
+  const Jet<T> y{uy, -uy * ux * ri, ((T)1 - uy * uy) * ri, -uy * uz * ri};
+  const Jet<T> z{uz, -uz * ux * ri, -uz * uy * ri, ((T)1 - uz * uz) * ri};
+  const Jet<T> r2 = jadd(jadd(jm(x, x), jm(y, y)), jm(z, z));
+  Jet<T> cs[L + 1], sn[L + 1], Q[(L + 1) * (L + 2) / 2];
+  cs[0] = {(T)1, 0, 0, 0};
+  sn[0] = {(T)0, 0, 0, 0};
+#pragma unroll
+  for (int m = 1; m <= L; ++m) {
+    cs[m] = jsub(jm(cs[m - 1], x), jm(sn[m - 1], y));
+    sn[m] = jadd(jm(sn[m - 1], x), jm(cs[m - 1], y));
+  }
+  cs[0] = {(T)0.70710678118654752440, 0, 0, 0};
+#define QI(l, m) ((l) * ((l) + 1) / 2 + (m))
+  Q[0] = {(T)1, 0, 0, 0};
+#pragma unroll
+  for (int l = 0; l <= L; ++l) {
+    if (l > 0) {
+      Q[QI(l, l)] = js((T) - (2 * l - 1), Q[QI(l - 1, l - 1)]);
+      Q[QI(l, l - 1)] = js((T)(2 * l - 1), jm(z, Q[QI(l - 1, l - 1)]));
+#pragma unroll
+      for (int m = 0; m <= l - 2; ++m)
+        Q[QI(l, m)] = js((T)1 / (T)(l - m), jsub(js((T)(2 * l - 1), jm(z, Q[QI(l - 1, m)])),
+                                                 js((T)(l + m - 1), jm(r2, Q[QI(l - 2, m)]))));
+    }
+#pragma unroll
+    for (int m = 0; m <= l; ++m) {
+      const T F = Fl[l * (L + 1) + m];
+      const Jet<T> pos = js(F, jm(Q[QI(l, m)], cs[m]));
+      const int ip = l * l + l + m;
+      Y[ip] = pos.v;
+      G[3 * ip] = pos.g0;
+      G[3 * ip + 1] = pos.g1;
+      G[3 * ip + 2] = pos.g2;
+      if (m > 0) {
+        const Jet<T> neg = js(F, jm(Q[QI(l, m)], sn[m]));
+        const int im = l * l + l - m;
+        Y[im] = neg.v;
+        G[3 * im] = neg.g0;
+        G[3 * im + 1] = neg.g1;
+        G[3 * im + 2] = neg.g2;
+      }
+    }
+  }
+#undef QI
+}
+
+template <typename T, int L>
+__global__ __launch_bounds__(64) void k_ace_site(AceArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int NY = (L + 1) * (L + 1);
+  T* Fl = (T*)smem;                 // (L+1)^2
+  T* A = Fl + NY;                   // ndens
+  T* dA = A + a.ndens;              // ndens
+  T* Rb = dA + a.ndens;             // nr x kStride
+  T* Yb = Rb + a.rad.nr * kStride;  // NY x kStride
+  const int lane = threadIdx.x;
+  const int site = blockIdx.x;
+  if (lane < NY) {
+    int l = 0;
+    while ((l + 1) * (l + 1) <= lane) ++l;
+    int m = lane - l * l;
+    if (m <= l) {
+      double F = sqrt((2.0 * l + 1) / (2 * 3.14159265358979323846));
+      for (int mm = 1; mm <= m; ++mm) F = -F / sqrt((double)(l + mm) * (l + 1 - mm));
+      Fl[l * (L + 1) + m] = (T)F;
+    }
+  }
+  for (int k = lane; k < a.ndens; k += kWave) A[k] = 0;
+  const int iz = a.sps[site];
+  const int e0 = a.estart[site], ne = a.estart[site + 1] - e0;
+  const int nq = a.pair.nr;
+  double epair = 0;
+  __syncthreads();
+  // ---- phase A: densities
+  for (int c0 = 0; c0 < ne; c0 += kWave) {
+    const int e = c0 + lane;
+    if (e < ne) {
+      const double* R = a.eR + 3 * (int64_t)(e0 + e);
+      const T dx = (T)R[0], dy = (T)R[1], dz = (T)R[2];
+      const T r = (T)sqrt(R[0] * R[0] + R[1] * R[1] + R[2] * R[2]);
+      const int ch = iz + a.nz * a.sps[a.enb[e0 + e]];
+      const RadialPoint<T> P = radial_point(a.rad, ch, r);
+      for (int n = 0; n < a.rad.nr; ++n) {
+        T v, d;
+        radial_value(a.rad, ch, P, n, v, d);
+        Rb[n * kStride + lane] = v;
+      }
+      if (nq) {
+        const RadialPoint<T> PQ = radial_point(a.pair, ch, r);
+        T s = 0;
+        for (int n = 0; n < nq; ++n) {
+          T v, d;
+          radial_value(a.pair, ch, PQ, n, v, d);
+          s += v * a.wp[iz * nq + n];
+        }
+        epair += (double)s;
+      }
+      T Y[NY], G[3 * NY];
+      ylm<T, L>(dx, dy, dz, Fl, Y, G);
+#pragma unroll
+      for (int m = 0; m < NY; ++m) Yb[m * kStride + lane] = Y[m];
+    }
+    __syncthreads();
+    const int cn = min(kWave, ne - c0);
+    for (int k = lane; k < a.ndens; k += kWave) {
+      const int rn = a.dens[2 * k], mm = a.dens[2 * k + 1];
+      const T* rr = Rb + rn * kStride;
+      const T* yy = Yb + mm * kStride;
+      T s = 0;
+      for (int t = 0; t < cn; ++t) s += rr[t] * yy[t];
+      A[k] += s;
+    }
+    __syncthreads();
+  }
+  // ---- phase B: products, energy and dE/dA
+  const T* w = a.w + (int64_t)iz * a.nprod;
+  T eprod = 0;
+  for (int p = lane; p < a.nprod; p += kWave) {
+    const int* f = a.prod + 4 * p;
+    T v = A[f[0]];
+    if (f[1] >= 0) v *= A[f[1]];
+    if (f[2] >= 0) v *= A[f[2]];
+    if (f[3] >= 0) v *= A[f[3]];
+    eprod += w[p] * v;
+  }
+  if (a.grads) {
+    for (int k = lane; k < a.ndens; k += kWave) {
+      T s = 0;
+      for (int q = a.darow[k]; q < a.darow[k + 1]; ++q) {
+        const int* d = a.dent + 4 * q;
+        T v = w[d[0]];
+        if (d[1] >= 0) v *= A[d[1]];
+        if (d[2] >= 0) v *= A[d[2]];
+        if (d[3] >= 0) v *= A[d[3]];
+        s += v;
+      }
+      dA[k] = s;
+    }
+  }
+  const double es = wave_sum((double)eprod) + wave_sum(epair);
+  if (lane == 0) a.siteE[site] = a.reference[iz] + es;
+  if (!a.grads) return;
+  __syncthreads();
+  // ---- phase C: force pullback
+  double wv[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int c0 = 0; c0 < ne; c0 += kWave) {
+    const int e = c0 + lane;
+    if (e >= ne) continue;
+    const double* R = a.eR + 3 * (int64_t)(e0 + e);
+    const T dx = (T)R[0], dy = (T)R[1], dz = (T)R[2];
+    const T r = (T)sqrt(R[0] * R[0] + R[1] * R[1] + R[2] * R[2]);
+    const T ri = (T)1 / r;
+    const int ch = iz + a.nz * a.sps[a.enb[e0 + e]];
+    T Y[NY], G[3 * NY], U[NY];
+    ylm<T, L>(dx, dy, dz, Fl, Y, G);
+#pragma unroll
+    for (int m = 0; m < NY; ++m) U[m] = 0;
+    T gs = 0;
+    const RadialPoint<T> P = radial_point(a.rad, ch, r);
+    for (int n = 0; n < a.rad.nr; ++n) {
+      T v, d;
+      radial_value(a.rad, ch, P, n, v, d);
+      const uint64_t mask = a.rmask[n];
+      const int* dof = a.dens_of + n * a.ny;
+      T tr = 0;
+#pragma unroll
+      for (int m = 0; m < NY; ++m) {
+        if ((mask >> m) & 1ull) {
+          const T c = dA[dof[m]];
+          tr += c * Y[m];
+          U[m] += c * v;
+        }
+      }
+      gs += d * tr;
+    }
+    if (nq) {
+      const RadialPoint<T> PQ = radial_point(a.pair, ch, r);
+      for (int n = 0; n < nq; ++n) {
+        T v, d;
+        radial_value(a.pair, ch, PQ, n, v, d);
+        gs += a.wp[iz * nq + n] * d;
+      }
+    }
+    T g0 = gs * dx * ri, g1 = gs * dy * ri, g2 = gs * dz * ri;
+#pragma unroll
+    for (int m = 0; m < NY; ++m) {
+      g0 += U[m] * G[3 * m];
+      g1 += U[m] * G[3 * m + 1];
+      g2 += U[m] * G[3 * m + 2];
+    }
+    double* go = a.eg + 3 * (int64_t)(e0 + e);
+    go[0] = (double)g0;
+    go[1] = (double)g1;
+    go[2] = (double)g2;
+    const double gg[3] = {(double)g0, (double)g1, (double)g2};
+    for (int bb = 0; bb < 3; ++bb)
+      for (int aa = 0; aa < 3; ++aa) wv[aa + 3 * bb] -= R[aa] * gg[bb];
+  }
+  for (int k = 0; k < 9; ++k) {
+    const double v = wave_sum(wv[k]);
+    if (lane == 0) a.siteW[9 * (int64_t)site + k] = v;
+  }
+}
+
+// Directed environments (interface.jl:144-171 + environments.jl:357-391):
+// pass 0 counts per sorted centre, pass 1 writes neighbour and exact R.
+__global__ void k_ace_edges(const Grid* __restrict__ gp, int64_t n, const double* __restrict__ xs,
+                            const int32_t* __restrict__ start, const int32_t* __restrict__ cell_of,
+                            const int32_t* __restrict__ perm, int pass, int32_t* __restrict__ ecount,
+                            const int32_t* __restrict__ estart, int32_t* __restrict__ enb, double* __restrict__ eR) {
+  const Grid& g = *gp;
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int cell = cell_of[perm[k]];
+  int key[3];
+  cell_key(g, cell, key);
+  const double xi = xs[k], yi = xs[n + k], zi = xs[2 * n + k];
+  int64_t pos = pass ? estart[k] : 0;
+  int cnt = 0;
+  for (int e = 0; e < g.nstencil; ++e) {
+    int lat[3];
+    bool zero;
+    const int nc = stencil_entry(g, key, e, lat, &zero);
+    if (nc < 0) continue;
+    double sh[3];
+    exact_shift(g, lat, sh);
+    for (int m = start[nc]; m < start[nc + 1]; ++m) {
+      if (zero && m == k) continue;
+      double d[3];
+      int img[3];
+      if (!exact_member(g, xi, yi, zi, xs[m], xs[n + m], xs[2 * n + m], lat, sh, d, img)) continue;
+      if (pass) {
+        enb[pos] = m;
+        eR[3 * pos] = d[0];
+        eR[3 * pos + 1] = d[1];
+        eR[3 * pos + 2] = d[2];
+        ++pos;
+      }
+      ++cnt;
+    }
+  }
+  if (!pass) ecount[k] = cnt;
+}
+
+// rev[e] = the edge (j -> i, -R) of edge e = (i -> j, R); R negates exactly.
+__global__ void k_ace_rev(int64_t n, const int32_t* __restrict__ estart, const int32_t* __restrict__ enb,
+                          const double* __restrict__ eR, int32_t* __restrict__ erev, int32_t* __restrict__ status) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  for (int e = estart[k]; e < estart[k + 1]; ++e) {
+    const int j = enb[e];
+    const double r0 = -eR[3 * e], r1 = -eR[3 * e + 1], r2 = -eR[3 * e + 2];
+    int found = -1;
+    for (int f = estart[j]; f < estart[j + 1]; ++f)
+      if (enb[f] == (int)k && eR[3 * f] == r0 && eR[3 * f + 1] == r1 && eR[3 * f + 2] == r2) {
+        found = f;
+        break;
+      }
+    erev[e] = found;
+    if (found < 0) atomicOr(status, 1);
+  }
+}
+
+// F_k = sum over k's out-edges of (g_e - g_rev(e)), written in original order.
+__global__ void k_ace_force(int64_t n, const int32_t* __restrict__ estart, const int32_t* __restrict__ erev,
+                            const double* __restrict__ eg, const int32_t* __restrict__ perm, double* __restrict__ F) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  double f0 = 0, f1 = 0, f2 = 0;
+  for (int e = estart[k]; e < estart[k + 1]; ++e) {
+    const int r = erev[e];  // -1 only when the status word already reports the failure
+    f0 += eg[3 * e];
+    f1 += eg[3 * e + 1];
+    f2 += eg[3 * e + 2];
+    if (r >= 0) {
+      f0 -= eg[3 * r];
+      f1 -= eg[3 * r + 1];
+      f2 -= eg[3 * r + 2];
+    }
+  }
+  const int64_t o = perm[k];
+  F[3 * o] = f0;
+  F[3 * o + 1] = f1;
+  F[3 * o + 2] = f2;
+}
+
+__global__ void k_ace_reduce(int64_t n, const double* __restrict__ siteE, const double* __restrict__ siteW, int grads,
+                             const int32_t* __restrict__ status, double* __restrict__ out) {
+  __shared__ double lds[10][4];
+  double s[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int64_t k = threadIdx.x; k < n; k += blockDim.x) {
+    s[0] += siteE[k];
+    if (grads)
+      for (int c = 0; c < 9; ++c) s[1 + c] += siteW[9 * k + c];
+  }
+  for (int c = 0; c < 10; ++c) s[c] = wave_sum(s[c]);
+  const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+  if (lane == 0)
+    for (int c = 0; c < 10; ++c) lds[c][wid] = s[c];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    bool bad = *status != 0;
+    for (int c = 0; c < 10; ++c) {
+      out[c] = lds[c][0] + lds[c][1] + lds[c][2] + lds[c][3];
+      bad |= !isfinite(out[c]);
+    }
+    out[10] = bad ? 1.0 : 0.0;
+  }
+}
+
+__global__ void k_ace_scatter_sites(int64_t n, const int32_t* __restrict__ perm, const double* __restrict__ siteE,
+                                    double* __restrict__ out) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) out[perm[k]] = siteE[k];
+}
+
+// ---------------------------------------------------------------- host
+
+template <typename T>
+static void dfree(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+template <typename T>
+static T* upload(const std::vector<T>& v, hipStream_t st) {
+  T* d = nullptr;
+  PFW_HIP(hipMalloc((void**)&d, std::max<size_t>(v.size(), 1) * sizeof(T)));
+  if (!v.empty()) PFW_HIP(hipMemcpyAsync(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
+  return d;
+}
+
+static void free_family(AceFamilyDev& f) {
+  dfree(f.pq);
+  dfree(f.env_kind);
+  dfree(f.env_pow);
+  dfree(f.par64);
+  dfree(f.par32);
+  dfree(f.nodes64);
+  dfree(f.nodes32);
+  dfree(f.coef64);
+  dfree(f.coef32);
+  f = AceFamilyDev();
+}
+
+void ace_free_edges(pfw_ctx* ctx) {
+  AceDev& a = ctx->ace;
+  dfree(a.ecount);
+  dfree(a.estart);
+  dfree(a.ebs);
+  dfree(a.enb);
+  dfree(a.eR);
+  dfree(a.erev);
+  dfree(a.eg);
+  dfree(a.siteE);
+  dfree(a.siteW);
+  dfree(a.status);
+  a.edge_cap = 0;
+}
+
+void ace_free(pfw_ctx* ctx) {
+  AceDev& a = ctx->ace;
+  ace_free_edges(ctx);
+  free_family(a.radial);
+  free_family(a.pair);
+  dfree(a.reference);
+  dfree(a.prod);
+  dfree(a.darow);
+  dfree(a.dent);
+  dfree(a.dens);
+  dfree(a.dens_of);
+  dfree(a.rmask);
+  dfree(a.w64);
+  dfree(a.w32);
+  dfree(a.wp64);
+  dfree(a.wp32);
+  a = AceDev();
+}
+
+static void load_family(const pfw_ace_radial& src, int nz, AceFamilyDev& f, double& cutoff, hipStream_t st) {
+  f.nr = src.nr;
+  f.nnodes = src.nnodes;
+  f.nch = nz * nz;
+  if (src.nr < 0) throw Error(PFW_INVALID_ARGUMENT, "negative radial count");
+  if (src.nr == 0) return;
+  if (src.nnodes < 2 || !src.transform || !src.envelope_kind || !src.envelope || !src.nodes || !src.coefficients)
+    throw Error(PFW_INVALID_ARGUMENT, "incomplete radial family");
+  const int nch = f.nch, nn = src.nnodes, nint = nn - 1;
+  std::vector<int> pq(2 * nch), kind(nch), epow(2 * nch);
+  std::vector<double> par(12 * nch, 0.0);
+  std::vector<double> nodes(src.nodes, src.nodes + (size_t)nch * nn);
+  std::vector<double> coef(src.coefficients, src.coefficients + (size_t)nch * 4 * src.nr * nint);
+  for (int c = 0; c < nch; ++c) {
+    const double* t = src.transform + 8 * c;
+    int p = (int)t[0], q = (int)t[1];
+    if (!(p > 0 && q >= p && t[2] > 0 && t[7] > t[4] && t[4] > t[3]))
+      throw Error(PFW_INVALID_ARGUMENT, "invalid Agnesi transform");
+    pq[2 * c] = p;
+    pq[2 * c + 1] = q;
+    for (int k = 0; k < 5; ++k) par[12 * c + k] = t[2 + k];  // a rin r0 yin ycut
+    par[12 * c + 5] = t[7];                                // rcut
+    cutoff = std::max(cutoff, t[7]);
+    const double* e = src.envelope + 5 * c;
+    kind[c] = src.envelope_kind[c];
+    if (kind[c] == PFW_ENV_POLY2SX) {
+      par[12 * c + 6] = e[0];
+      par[12 * c + 7] = e[1];
+      par[12 * c + 8] = e[4];
+      epow[2 * c] = (int)e[2];
+      epow[2 * c + 1] = (int)e[3];
+    } else if (kind[c] == PFW_ENV_POLY1SR) {
+      par[12 * c + 6] = e[0];
+      epow[2 * c] = (int)e[1];
+    } else if (kind[c] == PFW_ENV_ACE1_POLY1SR) {
+      par[12 * c + 6] = e[0];
+      par[12 * c + 7] = e[1];
+      epow[2 * c] = (int)e[2];
+    } else {
+      throw Error(PFW_INVALID_ARGUMENT, "unsupported radial envelope");
+    }
+    const double* nd = src.nodes + (size_t)c * nn;
+    if (!(nd[0] == -1.0 && nd[nn - 1] == 1.0)) throw Error(PFW_INVALID_ARGUMENT, "invalid spline nodes");
+    for (int k = 0; k + 1 < nn; ++k)
+      if (!(nd[k + 1] > nd[k])) throw Error(PFW_INVALID_ARGUMENT, "invalid spline nodes");
+    for (int k = 0; k < nn; ++k)
+      if (std::fabs(nd[k] - (-1.0 + 2.0 * k / (nn - 1))) > 1e-12) f.uniform = false;
+  }
+  for (double v : coef)
+    if (!std::isfinite(v)) throw Error(PFW_INVALID_ARGUMENT, "nonfinite spline coefficients");
+  std::vector<float> par32(par.begin(), par.end()), nodes32(nodes.begin(), nodes.end()), coef32(coef.begin(), coef.end());
+  f.pq = upload(pq, st);
+  f.env_kind = upload(kind, st);
+  f.env_pow = upload(epow, st);
+  f.par64 = upload(par, st);
+  f.par32 = upload(par32, st);
+  f.nodes64 = upload(nodes, st);
+  f.nodes32 = upload(nodes32, st);
+  f.coef64 = upload(coef, st);
+  f.coef32 = upload(coef32, st);
+}
+
+void ace_set_parameters(pfw_ctx* ctx, const double* WB, const double* Wpair) {
+  AceDev& a = ctx->ace;
+  if (!a.ready && a.nprod == 0) throw Error(PFW_STALE, "no ACE model set");
+  const int nz = a.nz, nq = a.pair.nr;
+  std::vector<double> w((size_t)nz * a.nprod, 0.0);  // z-major: w[z * nprod + k]
+  for (size_t t = 0; t < a.c_vals.size(); ++t)
+    for (int z = 0; z < nz; ++z) {
+      double b = WB[(size_t)a.c_rows[t] + (size_t)a.nbasis * z];
+      if (!std::isfinite(b)) throw Error(PFW_INVALID_ARGUMENT, "nonfinite readout parameters");
+      w[(size_t)z * a.nprod + a.c_cols[t]] += a.c_vals[t] * b;
+    }
+  std::vector<double> wp((size_t)nz * std::max(nq, 1), 0.0);
+  for (int z = 0; z < nz; ++z)
+    for (int q = 0; q < nq; ++q) {
+      double v = Wpair ? Wpair[q + (size_t)nq * z] : 0.0;
+      if (!std::isfinite(v)) throw Error(PFW_INVALID_ARGUMENT, "nonfinite pair parameters");
+      wp[(size_t)z * nq + q] = v;
+    }
+  std::vector<float> w32(w.begin(), w.end()), wp32(wp.begin(), wp.end());
+  dfree(a.w64);
+  dfree(a.w32);
+  dfree(a.wp64);
+  dfree(a.wp32);
+  hipStream_t st = ctx->stream;
+  a.w64 = upload(w, st);
+  a.w32 = upload(w32, st);
+  a.wp64 = upload(wp, st);
+  a.wp32 = upload(wp32, st);
+  PFW_HIP(hipStreamSynchronize(st));
+}
+
+void ace_set_model(pfw_ctx* ctx, const pfw_ace_model* m) {
+  ace_free(ctx);
+  AceDev& a = ctx->ace;
+  hipStream_t st = ctx->stream;
+  if (m->nz <= 0 || m->lmax < 0 || m->ndens < 0 || m->nprod < 0 || m->nbasis < 0)
+    throw Error(PFW_INVALID_ARGUMENT, "invalid ACE model dimensions");
+  if (m->lmax > 6) throw Error(PFW_INVALID_ARGUMENT, "lmax above 6 is not supported by the device kernels");
+  a.nz = m->nz;
+  a.lmax = m->lmax;
+  a.ndens = m->ndens;
+  a.nprod = m->nprod;
+  a.nbasis = m->nbasis;
+  a.ny = (m->lmax + 1) * (m->lmax + 1);
+  double cutoff = 0;
+  load_family(m->radial, m->nz, a.radial, cutoff, st);
+  load_family(m->pair, m->nz, a.pair, cutoff, st);
+  if (m->radial.nr <= 0) throw Error(PFW_INVALID_ARGUMENT, "ACE model needs radial functions");
+  a.cutoff = cutoff;
+  std::vector<double> ref(m->reference, m->reference + m->nz);
+  for (double v : ref)
+    if (!std::isfinite(v)) throw Error(PFW_INVALID_ARGUMENT, "nonfinite reference energies");
+  a.reference = upload(ref, st);
+  // densities
+  std::vector<int32_t> dens(m->density, m->density + 2 * (size_t)m->ndens);
+  std::vector<int32_t> dof((size_t)a.radial.nr * a.ny, -1);
+  std::vector<uint64_t> rmask(a.radial.nr, 0);
+  for (int k = 0; k < m->ndens; ++k) {
+    int r = dens[2 * k], h = dens[2 * k + 1];
+    if (r < 0 || r >= a.radial.nr || h < 0 || h >= a.ny) throw Error(PFW_INVALID_ARGUMENT, "invalid density indices");
+    if (dof[(size_t)r * a.ny + h] >= 0) throw Error(PFW_INVALID_ARGUMENT, "duplicate density entry");
+    dof[(size_t)r * a.ny + h] = k;
+    rmask[r] |= 1ull << h;
+  }
+  a.dens = upload(dens, st);
+  a.dens_of = upload(dof, st);
+  a.rmask = upload(rmask, st);
+  // products and the per-density derivative CSR
+  std::vector<int32_t> prod(4 * (size_t)m->nprod, -1);
+  std::vector<std::vector<std::array<int32_t, 4>>> rows(m->ndens);
+  for (int k = 0; k < m->nprod; ++k) {
+    int b = m->product_ptr[k], nu = m->product_ptr[k + 1] - b;
+    if (nu <= 0 || nu > 4) throw Error(PFW_INVALID_ARGUMENT, "product orders must be 1..4 on the device");
+    for (int t = 0; t < nu; ++t) {
+      int d = m->product_idx[b + t];
+      if (d < 0 || d >= m->ndens) throw Error(PFW_INVALID_ARGUMENT, "invalid product indices");
+      prod[4 * (size_t)k + t] = d;
+    }
+    for (int t = 0; t < nu; ++t) {  // d P / d A_{idx[t]}: product of the other factors
+      std::array<int32_t, 4> ent{k, -1, -1, -1};
+      int o = 1;
+      for (int s = 0; s < nu; ++s)
+        if (s != t) ent[o++] = m->product_idx[b + s];
+      rows[m->product_idx[b + t]].push_back(ent);
+    }
+  }
+  std::vector<int32_t> darow(m->ndens + 1, 0), dent;
+  for (int d = 0; d < m->ndens; ++d) {
+    darow[d + 1] = darow[d] + (int)rows[d].size();
+    for (auto& e : rows[d]) dent.insert(dent.end(), e.begin(), e.end());
+  }
+  a.prod = upload(prod, st);
+  a.darow = upload(darow, st);
+  a.dent = upload(dent, st);
+  a.c_rows.assign(m->coupling_rows, m->coupling_rows + m->coupling_nnz);
+  a.c_cols.assign(m->coupling_cols, m->coupling_cols + m->coupling_nnz);
+  a.c_vals.assign(m->coupling_vals, m->coupling_vals + m->coupling_nnz);
+  for (int64_t t = 0; t < m->coupling_nnz; ++t)
+    if (a.c_rows[t] < 0 || a.c_rows[t] >= m->nbasis || a.c_cols[t] < 0 || a.c_cols[t] >= m->nprod ||
+        !std::isfinite(a.c_vals[t]))
+      throw Error(PFW_INVALID_ARGUMENT, "invalid coupling entries");
+  ace_set_parameters(ctx, m->WB, m->Wpair);
+  a.ready = true;
+}
+
+template <typename T>
+static FamArgs<T> fam_args(const AceFamilyDev& f) {
+  FamArgs<T> o;
+  o.nr = f.nr;
+  o.nnodes = f.nnodes;
+  o.nint = f.nnodes - 1;
+  o.uniform = f.uniform;
+  o.pq = f.pq;
+  o.env_kind = f.env_kind;
+  o.env_pow = f.env_pow;
+  if constexpr (sizeof(T) == 8) {
+    o.par = f.par64;
+    o.nodes = f.nodes64;
+    o.coef = f.coef64;
+  } else {
+    o.par = f.par32;
+    o.nodes = f.nodes32;
+    o.coef = f.coef32;
+  }
+  return o;
+}
+
+static inline unsigned nblk(int64_t n, int t) { return (unsigned)std::max<int64_t>(1, (n + t - 1) / t); }
+
+template <typename T, int L>
+static void launch_sites(pfw_ctx* ctx, int grads) {
+  AceDev& ad = ctx->ace;
+  SystemBuffers& s = ctx->sys;
+  AceArgs<T> a;
+  a.n = s.n;
+  a.nz = ad.nz;
+  a.ndens = ad.ndens;
+  a.nprod = ad.nprod;
+  a.ny = ad.ny;
+  a.perm = s.perm;
+  a.sps = s.sps;
+  a.estart = ad.estart;
+  a.enb = ad.enb;
+  a.eR = ad.eR;
+  a.reference = ad.reference;
+  a.rad = fam_args<T>(ad.radial);
+  a.pair = fam_args<T>(ad.pair);
+  a.dens = ad.dens;
+  a.dens_of = ad.dens_of;
+  a.rmask = ad.rmask;
+  a.prod = ad.prod;
+  a.darow = ad.darow;
+  a.dent = ad.dent;
+  if constexpr (sizeof(T) == 8) {
+    a.w = ad.w64;
+    a.wp = ad.wp64;
+  } else {
+    a.w = ad.w32;
+    a.wp = ad.wp32;
+  }
+  a.grads = grads;
+  a.siteE = ad.siteE;
+  a.siteW = ad.siteW;
+  a.eg = ad.eg;
+  constexpr int NY = (L + 1) * (L + 1);
+  size_t lds = sizeof(T) * ((size_t)NY + 2 * (size_t)ad.ndens + (size_t)(ad.radial.nr + NY) * kStride);
+  if (lds > 160 * 1024) throw Error(PFW_INVALID_ARGUMENT, "ACE model too large for the site kernel's LDS tiles");
+  if (lds > 64 * 1024)
+    PFW_HIP(hipFuncSetAttribute((const void*)k_ace_site<T, L>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL((k_ace_site<T, L>), dim3((unsigned)s.n), dim3(kWave), lds, ctx->stream, a);
+  PFW_HIP(hipGetLastError());
+}
+
+template <typename T>
+static void dispatch_sites(pfw_ctx* ctx, int grads) {
+  switch (ctx->ace.lmax <= 2 ? 2 : ctx->ace.lmax <= 4 ? 4 : 6) {
+    case 2:
+      launch_sites<T, 2>(ctx, grads);
+      break;
+    case 4:
+      launch_sites<T, 4>(ctx, grads);
+      break;
+    default:
+      launch_sites<T, 6>(ctx, grads);
+  }
+}
+
+void ace_run(pfw_ctx* ctx, uint32_t mask, int precision) {
+  AceDev& ad = ctx->ace;
+  if (!ad.ready) throw Error(PFW_STALE, "no ACE model set");
+  if (ad.cutoff != ctx->grid.c)
+    throw Error(PFW_STALE, "the system's cell list was built for another cutoff than the ACE search cutoff");
+  SystemBuffers& s = ctx->sys;
+  hipStream_t st = ctx->stream;
+  const int64_t n = s.n;
+  nl_build(ctx);
+  record_stage(ctx, 3);
+  if (n == 0 || mask == 0) {
+    record_stage(ctx, 4);
+    PFW_HIP(hipMemsetAsync(ctx->d_out, 0, 11 * sizeof(double), st));
+    if (n > 0) PFW_HIP(hipMemsetAsync(ctx->d_F, 0, 3 * n * sizeof(double), st));
+  } else {
+    if (!ad.ecount) {
+      PFW_HIP(hipMalloc(&ad.ecount, n * sizeof(int32_t)));
+      PFW_HIP(hipMalloc(&ad.estart, (n + 1) * sizeof(int32_t)));
+      PFW_HIP(hipMalloc(&ad.ebs, ((n + 1023) / 1024 + 2) * sizeof(int32_t)));
+      PFW_HIP(hipMalloc(&ad.siteE, n * sizeof(double)));
+      PFW_HIP(hipMalloc(&ad.siteW, 9 * n * sizeof(double)));
+      PFW_HIP(hipMalloc(&ad.status, sizeof(int32_t)));
+    }
+    PFW_HIP(hipMemsetAsync(ad.status, 0, sizeof(int32_t), st));
+    hipLaunchKernelGGL(k_ace_edges, dim3(nblk(n, 128)), dim3(128), 0, st, s.grid, n, s.xs, s.start, s.cell_of, s.perm,
+                       0, ad.ecount, (const int32_t*)nullptr, (int32_t*)nullptr, (double*)nullptr);
+    PFW_HIP(hipGetLastError());
+    scan_counts(ctx, ad.ecount, (int)n, ad.estart, ad.ebs, ad.ecount /* cursor: reused, already consumed */);
+    int32_t total = 0;
+    PFW_HIP(hipMemcpyAsync(&total, ad.estart + n, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    PFW_HIP(hipStreamSynchronize(st));
+    if (total < 0) throw Error(PFW_CAPACITY, "edge count overflow");
+    if (total > ad.edge_cap || !ad.enb) {
+      dfree(ad.enb);
+      dfree(ad.eR);
+      dfree(ad.erev);
+      dfree(ad.eg);
+      int64_t cap = std::max<int64_t>((int64_t)(total * 1.25) + 64, 64);
+      PFW_HIP(hipMalloc(&ad.enb, cap * sizeof(int32_t)));
+      PFW_HIP(hipMalloc(&ad.eR, 3 * cap * sizeof(double)));
+      PFW_HIP(hipMalloc(&ad.erev, cap * sizeof(int32_t)));
+      PFW_HIP(hipMalloc(&ad.eg, 3 * cap * sizeof(double)));
+      ad.edge_cap = cap;
+    }
+    hipLaunchKernelGGL(k_ace_edges, dim3(nblk(n, 128)), dim3(128), 0, st, s.grid, n, s.xs, s.start, s.cell_of, s.perm,
+                       1, ad.ecount, (const int32_t*)ad.estart, ad.enb, ad.eR);
+    PFW_HIP(hipGetLastError());
+    const int grads = (mask & (PFW_FORCES | PFW_VIRIAL)) ? 1 : 0;
+    if (grads)
+      hipLaunchKernelGGL(k_ace_rev, dim3(nblk(n, 128)), dim3(128), 0, st, n, ad.estart, ad.enb, ad.eR, ad.erev,
+                         ad.status);
+    record_stage(ctx, 4);
+    if (precision == PFW_MIXED)
+      dispatch_sites<float>(ctx, grads);
+    else
+      dispatch_sites<double>(ctx, grads);
+    if (grads)
+      hipLaunchKernelGGL(k_ace_force, dim3(nblk(n, 256)), dim3(256), 0, st, n, ad.estart, ad.erev, ad.eg, s.perm,
+                         ctx->d_F);
+    hipLaunchKernelGGL(k_ace_reduce, dim3(1), dim3(256), 0, st, n, ad.siteE, ad.siteW, grads, ad.status, ctx->d_out);
+    PFW_HIP(hipGetLastError());
+  }
+  PFW_HIP(hipMemcpyAsync(ctx->h_out, ctx->d_out, 11 * sizeof(double), hipMemcpyDeviceToHost, st));
+  record_stage(ctx, 5);
+  PFW_HIP(hipStreamSynchronize(st));
+  if (ctx->timing)
+    for (int k = 0; k < kStages; ++k) {
+      float ms = 0;
+      PFW_HIP(hipEventElapsedTime(&ms, ctx->ev[k], ctx->ev[k + 1]));
+      ctx->stage_ms[k] = ms;
+    }
+  ctx->last_mask = mask;
+  ctx->have_results = true;
+  if (ctx->h_out[10] != 0.0) throw Error(PFW_DOMAIN, "nonfinite ACE output (coincident atoms or singular envelope)");
+}
+
+void ace_site_energies(pfw_ctx* ctx, double* out) {
+  SystemBuffers& s = ctx->sys;
+  if (!ctx->ace.siteE || s.n == 0) return;
+  double* d = nullptr;
+  PFW_HIP(hipMalloc(&d, s.n * sizeof(double)));
+  hipLaunchKernelGGL(k_ace_scatter_sites, dim3(nblk(s.n, 256)), dim3(256), 0, ctx->stream, s.n, s.perm,
+                     ctx->ace.siteE, d);
+  PFW_HIP(hipMemcpyAsync(out, d, s.n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  PFW_HIP(hipStreamSynchronize(ctx->stream));
+  (void)hipFree(d);
+}
+
+}  // namespace pfw

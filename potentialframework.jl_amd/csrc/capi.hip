@@ -66,6 +66,7 @@ void pfw_ctx_destroy(pfw_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   pfw::nl_free(ctx);
+  pfw::ace_free(ctx);
   if (ctx->d_F) (void)hipFree(ctx->d_F);
   if (ctx->d_partials) (void)hipFree(ctx->d_partials);
   if (ctx->d_out) (void)hipFree(ctx->d_out);
@@ -91,6 +92,7 @@ int pfw_set_system(pfw_ctx* ctx, int64_t n, const double* X, const int32_t* spec
     ctx->have_system = false;
     ctx->have_results = false;
     pfw::nl_free(ctx);
+    pfw::ace_free_edges(ctx);
     for (int k = 0; k < 9; ++k) ctx->host_H[k] = H[k];
     for (int a = 0; a < 3; ++a) ctx->host_pb[a] = pbc[a] ? 1 : 0;
     // open cell vectors may be infinite or zero (storage.jl:80-82); periodic ones are checked by the basis
@@ -219,6 +221,45 @@ int pfw_get_timings(pfw_ctx* ctx, double* ms, int n) {
   int m = std::min(n, pfw::kStages);
   for (int k = 0; k < m; ++k) ms[k] = ctx->stage_ms[k];
   return m;
+}
+
+int pfw_set_ace_model(pfw_ctx* ctx, const pfw_ace_model* model) {
+  return run(ctx, [&] {
+    if (!model) throw Error(PFW_INVALID_ARGUMENT, "null model");
+    pfw::ace_set_model(ctx, model);
+  });
+}
+
+int pfw_set_ace_parameters(pfw_ctx* ctx, const double* WB, const double* Wpair) {
+  return run(ctx, [&] {
+    if (!ctx->ace.ready) throw Error(PFW_STALE, "no ACE model set");
+    if (!WB) throw Error(PFW_INVALID_ARGUMENT, "null readout parameters");
+    pfw::ace_set_parameters(ctx, WB, Wpair);
+  });
+}
+
+int pfw_ace_step(pfw_ctx* ctx, uint32_t mask, int precision) {
+  return run(ctx, [&] {
+    if (!ctx->have_system) throw Error(PFW_STALE, "no system set");
+    if (precision != PFW_FP64 && precision != PFW_MIXED) throw Error(PFW_INVALID_ARGUMENT, "unknown precision");
+    if (mask & ~7u) throw Error(PFW_INVALID_ARGUMENT, "unknown request bits");
+    ctx->have_results = false;
+    pfw::ace_run(ctx, mask, precision);
+  });
+}
+
+int pfw_ace_evaluate(pfw_ctx* ctx, uint32_t mask, int precision, double* E, double* F, double* W) {
+  int rc = pfw_ace_step(ctx, mask, precision);
+  if (rc != PFW_OK) return rc;
+  return pfw_read_results(ctx, E, F, W);
+}
+
+int pfw_ace_site_energies(pfw_ctx* ctx, double* out) {
+  return run(ctx, [&] {
+    if (!ctx->have_results) throw Error(PFW_STALE, "no results: run an ACE step first");
+    if (!out) throw Error(PFW_INVALID_ARGUMENT, "null output");
+    pfw::ace_site_energies(ctx, out);
+  });
 }
 
 int pfw_synchronize(pfw_ctx* ctx) {

@@ -383,7 +383,12 @@ __global__ void k_scan_final(const int32_t* __restrict__ count, int ncells, cons
     }
     ex += v[k];
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) start[ncells] = (int32_t)n;
+  // n < 0: the total is the scan's own sum (written by the block that holds the last item)
+  if (n >= 0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) start[ncells] = (int32_t)n;
+  } else if (base <= ncells - 1 && ncells - 1 < base + kScanItems) {
+    start[ncells] = ex;
+  }
 }
 
 __global__ void k_scatter(int64_t n, const int32_t* __restrict__ cell_of, const int32_t* __restrict__ start,
@@ -424,6 +429,17 @@ __global__ void k_gather(int64_t n, const int32_t* __restrict__ perm, const doub
 }
 
 static inline unsigned nblocks(int64_t n, int t) { return (unsigned)std::max<int64_t>(1, (n + t - 1) / t); }
+
+// Exclusive scan of m counts into start[0..m] (start[m] = total); also zeroes cursor.
+// blocksum needs (m + 1023) / 1024 + 1 ints.
+void scan_counts(pfw_ctx* ctx, const int32_t* count, int m, int32_t* start, int32_t* blocksum, int32_t* cursor) {
+  hipStream_t st = ctx->stream;
+  int nb = (int)nblocks(m, kScanBlock);
+  hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(kScanThreads), 0, st, count, m, blocksum);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, st, blocksum, nb);
+  hipLaunchKernelGGL(k_scan_final, dim3(nb), dim3(kScanThreads), 0, st, count, m, blocksum, start, cursor, (int64_t)-1);
+  PFW_HIP(hipGetLastError());
+}
 
 void nl_build(pfw_ctx* ctx) {
   SystemBuffers& s = ctx->sys;

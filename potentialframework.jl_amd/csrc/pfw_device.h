@@ -33,6 +33,10 @@ __device__ inline void exact_matvec(const double* M, const double* v, double* o)
 }
 __device__ inline void exact_shift(const Grid& g, const int* s, double* o) {
 #pragma clang fp contract(off)
+  if (g.minimum_image) {  // orthorhombic: image shift s_a L_a (the basis is the identity here)
+    for (int a = 0; a < 3; ++a) o[a] = g.pb[a] ? (double)s[a] * g.L[a] : 0.0;
+    return;
+  }
   double v[3] = {(double)s[0], (double)s[1], (double)s[2]};
   for (int a = 0; a < 3; ++a) o[a] = g.basis[a] * v[0] + g.basis[a + 3] * v[1] + g.basis[a + 6] * v[2];
 }

@@ -74,6 +74,53 @@ struct PairModelDev {
 
 constexpr int kStages = 5;
 
+// One radial family of a resolved ACE model on the device, in both precisions.
+struct AceFamilyDev {
+  int nr = 0, nnodes = 0, nch = 0;
+  bool uniform = true;  // nodes are linspace(-1, 1, nnodes)
+  int* pq = nullptr;    // nch x 2 (p, q)
+  int* env_kind = nullptr;
+  int* env_pow = nullptr;  // nch x 2 integer envelope powers
+  double* par64 = nullptr;  // nch x 12: a rin r0 yin ycut rcut | envelope x1 x2 s (or rc r0 -)
+  float* par32 = nullptr;
+  double* nodes64 = nullptr;
+  float* nodes32 = nullptr;
+  double* coef64 = nullptr;  // nch x nint x nr x 4
+  float* coef32 = nullptr;
+};
+
+struct AceDev {
+  bool ready = false;
+  int nz = 0, lmax = 0, ndens = 0, nprod = 0, nbasis = 0, ny = 0;
+  double cutoff = 0;
+  std::vector<int32_t> c_rows, c_cols;  // host copy of the coupling
+  std::vector<double> c_vals;
+  double* reference = nullptr;
+  AceFamilyDev radial, pair;
+  int32_t* prod = nullptr;      // nprod x 4, -1 padded
+  int32_t* darow = nullptr;     // ndens + 1
+  int32_t* dent = nullptr;      // entries x 4: (product, other factors -1 padded)
+  int32_t* dens = nullptr;      // ndens x 2
+  int32_t* dens_of = nullptr;   // nr x ny density index or -1
+  uint64_t* rmask = nullptr;    // nr: harmonics used with each radial index
+  double* w64 = nullptr;        // nz x nprod precontracted readout
+  float* w32 = nullptr;
+  double* wp64 = nullptr;       // nz x nq
+  float* wp32 = nullptr;
+  // environments and per-step buffers
+  int64_t edge_cap = 0;
+  int32_t* ecount = nullptr;  // n
+  int32_t* estart = nullptr;  // n + 1
+  int32_t* ebs = nullptr;     // scan scratch
+  int32_t* enb = nullptr;     // neighbour (sorted index)
+  double* eR = nullptr;       // 3 x E
+  int32_t* erev = nullptr;
+  double* eg = nullptr;       // 3 x E edge gradients
+  double* siteE = nullptr;    // n (sorted order)
+  double* siteW = nullptr;    // 9 x n
+  int32_t* status = nullptr;
+};
+
 }  // namespace pfw
 
 struct pfw_ctx {
@@ -86,6 +133,7 @@ struct pfw_ctx {
   double host_H[9];
   int host_pb[3];
   pfw::PairModelDev pair;
+  pfw::AceDev ace;
   // outputs
   double* d_F = nullptr;        // 3 x n original order
   double* d_partials = nullptr;  // per-wave E + 6 W partials
@@ -107,7 +155,16 @@ void nl_build(pfw_ctx* ctx);
 void nl_export_pairs(pfw_ctx* ctx, int32_t* i, int32_t* j, int32_t* S, double* R, int64_t cap,
                      int64_t* count);
 void nl_free(pfw_ctx* ctx);
+void scan_counts(pfw_ctx* ctx, const int32_t* count, int m, int32_t* start, int32_t* blocksum,
+                 int32_t* cursor);
 // pair.hip
 void pair_run(pfw_ctx* ctx, uint32_t mask, int precision);
 void record_stage(pfw_ctx* ctx, int k);
+// ace.hip
+void ace_set_model(pfw_ctx* ctx, const pfw_ace_model* m);
+void ace_set_parameters(pfw_ctx* ctx, const double* WB, const double* Wpair);
+void ace_run(pfw_ctx* ctx, uint32_t mask, int precision);
+void ace_free(pfw_ctx* ctx);
+void ace_free_edges(pfw_ctx* ctx);
+void ace_site_energies(pfw_ctx* ctx, double* out);
 }  // namespace pfw

@@ -1,0 +1,133 @@
+"""GPU parity of the ACE site pipeline.
+
+* TiAl resolved model, 33 skew periodic frames of 2 or 54 atoms: FP64 against
+  the ACEpotentials 0.10.2 predictions with the reference's own tolerances
+  (test/neighbour_saved_reference.jl:38-40: E 1e-8 eV, F 1e-9 eV/Å + rtol 1e-10,
+  W 1e-8 eV + rtol 1e-10); mixed precision against the same predictions at
+  E 1e-3 eV, F 1e-4 eV/Å, W 2e-3 eV (FP32 site arithmetic, FP64 sums).
+* Synthetic order-3 / lmax-4 model on BCC binaries: FP64 against the oracle
+  to 1e-9 (scaled), mixed with force RMSE <= 1e-5 eV/Å at the full 32,768 atoms.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle.ace import ace_evaluate as oracle_ace
+from oracle.ace import load_resolved as oracle_load
+from oracle.ace import read_frames
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+try:
+    import tomllib as toml
+except ImportError:
+    import tomli as toml
+
+
+@pytest.fixture(scope="module")
+def tial(pfw):
+    model, ps, _ = pfw.load_resolved(os.path.join(GOLD, "resolved_model_v1.toml"))
+    frames = read_frames(os.path.join(GOLD, "reference_results.xyz"))
+    with open(os.path.join(GOLD, "reloaded_predictions.toml"), "rb") as f:
+        preds = toml.load(f)["frames"]
+    return model, ps, frames, preds
+
+
+def check_frame(r, ref, etol, ftol, frtol, wtol, wrtol):
+    assert abs(r["energy"] - ref["energy_eV"]) <= etol
+    Fref = np.array(ref["forces_eV_per_A"]).T
+    np.testing.assert_allclose(r["forces"], Fref, atol=ftol, rtol=frtol)
+    Wref = np.array(ref["virial_eV_columns"]).T
+    np.testing.assert_allclose(r["virial"], Wref, atol=wtol, rtol=wrtol)
+
+
+@pytest.mark.parametrize("prec", ["float64", "mixed"])
+def test_tial_33_frames(pfw, tial, prec):
+    model, ps, frames, preds = tial
+    calc = pfw.ACECalculator(model, ps, precision=prec)
+    tol = dict(float64=(1e-8, 1e-9, 1e-10, 1e-8, 1e-10), mixed=(1e-3, 1e-4, 0, 2e-3, 0))[prec]
+    worst = np.zeros(3)
+    for f, ref in zip(frames, preds):
+        r = calc.evaluate(pfw.System(f["X"], f["H"], f["pbc"], f["labels"]))
+        worst = np.maximum(worst, [abs(r["energy"] - ref["energy_eV"]),
+                                   np.abs(r["forces"] - np.array(ref["forces_eV_per_A"]).T).max(),
+                                   np.abs(r["virial"] - np.array(ref["virial_eV_columns"]).T).max()])
+        check_frame(r, ref, *tol)
+    print(f"TiAl {prec}: worst |dE|={worst[0]:.3e} |dF|={worst[1]:.3e} |dW|={worst[2]:.3e}")
+
+
+def smoke_case(pfw):
+    """One TiAl frame per precision against the golden predictions (used by smoke())."""
+    model, ps, _ = pfw.load_resolved(os.path.join(GOLD, "resolved_model_v1.toml"))
+    frames = read_frames(os.path.join(GOLD, "reference_results.xyz"))
+    with open(os.path.join(GOLD, "reloaded_predictions.toml"), "rb") as f:
+        ref = toml.load(f)["frames"][-1]
+    f = frames[-1]
+    for prec, tol in (("float64", (1e-8, 1e-9, 1e-10, 1e-8, 1e-10)), ("mixed", (1e-3, 1e-4, 0, 2e-3, 0))):
+        r = pfw.ACECalculator(model, ps, precision=prec).evaluate(pfw.System(f["X"], f["H"], f["pbc"], f["labels"]))
+        check_frame(r, ref, *tol)
+
+
+@pytest.fixture(scope="module")
+def synthetic(pfw):
+    D = pfw.ace_basis.synthetic_model()
+    model, ps, _ = pfw.load_resolved(D)
+    return D, model, ps, oracle_load(D)
+
+
+def labels_of(model, sp):
+    from pfw_amd.ace import ELEMENTS
+    return [ELEMENTS[model.atomic_numbers[s] - 1] for s in sp]
+
+
+@pytest.mark.parametrize("prec", ["float64", "mixed"])
+def test_synthetic_small_vs_oracle(pfw, synthetic, prec):
+    D, model, ps, om = synthetic
+    calc = pfw.ACECalculator(model, ps, precision=prec)
+    for cells, seed in (((3, 3, 3), 11), ((4, 3, 2), 12)):
+        X, H, pbc, sp = pfw.workloads.bcc_binary(*cells, seed=seed)
+        r = calc.evaluate(pfw.System(X, H, pbc, labels_of(model, sp)), site_energies=True)
+        E, F, W, site = oracle_ace(X, sp, H, pbc, om)
+        tol = 1e-9 if prec == "float64" else 1e-4
+        assert abs(r["energy"] - E) <= tol * max(1, abs(E))
+        np.testing.assert_allclose(r["site_energies"], site, atol=tol * 10, rtol=0)
+        np.testing.assert_allclose(r["forces"], F.T, atol=tol, rtol=tol)
+        np.testing.assert_allclose(r["virial"], W, atol=tol * max(1, abs(E)), rtol=tol)
+
+
+def test_request_masks_and_parameter_update(pfw, synthetic):
+    D, model, ps, om = synthetic
+    X, H, pbc, sp = pfw.workloads.bcc_binary(3, 3, 3, seed=4)
+    sys_ = pfw.System(X, H, pbc, labels_of(model, sp))
+    calc = pfw.ACECalculator(model, ps, precision="float64")
+    full = calc.evaluate(sys_)
+    e_only = calc.evaluate(sys_, request=("energy",))
+    assert set(e_only) == {"energy"}
+    assert e_only["energy"] == pytest.approx(full["energy"], abs=1e-10)
+    ps2 = {"WB": ps["WB"] * 2, "Wpair": ps["Wpair"] * 2}
+    calc.set_parameters(ps2)
+    r2 = calc.evaluate(sys_)
+    om2 = dict(om, WB=ps2["WB"], Wpair=ps2["Wpair"])
+    E2, F2, W2, _ = oracle_ace(X, sp, H, pbc, om2)
+    assert r2["energy"] == pytest.approx(E2, abs=1e-9 * abs(E2))
+    np.testing.assert_allclose(r2["forces"], F2.T, atol=1e-9)
+
+
+def test_synthetic_bcc_32k_full_size(pfw, synthetic):
+    D, model, ps, om = synthetic
+    X, H, pbc, sp = pfw.workloads.bcc_binary()
+    sys_ = pfw.System(X, H, pbc, labels_of(model, sp))
+    E, F, W, _ = oracle_ace(X, sp, H, pbc, om, threads=16)
+    r64 = pfw.ACECalculator(model, ps, precision="float64").evaluate(sys_)
+    assert abs(r64["energy"] - E) <= 1e-9 * abs(E)
+    np.testing.assert_allclose(r64["forces"], F.T, atol=1e-9, rtol=1e-9)
+    np.testing.assert_allclose(r64["virial"], W, atol=1e-8 * abs(E), rtol=1e-9)
+    rmx = pfw.ACECalculator(model, ps, precision="mixed").evaluate(sys_)
+    d = rmx["forces"] - F.T
+    frms, fmax = float(np.sqrt(np.mean(d * d))), float(np.abs(d).max())
+    print(f"ACE 32k mixed: max|dF|={fmax:.3e} rms={frms:.3e} dE/atom={(rmx['energy'] - E) / len(X):.3e}")
+    assert frms <= 1e-5
+    assert abs(rmx["energy"] - E) / len(X) <= 1e-5
