@@ -1,15 +1,19 @@
 """Throughput benchmark of the MI355X force-evaluation hot path.
 
-Default workload (one GPU): the 256,000-atom two-species SmoothPair B2 lattice
-in a sheared triclinic cell (cutoff 6.0 Å), one "step" = fresh cell build +
-E+F+V + FP64 reductions + nonfinite status check, positions already resident
-in HBM. ``value`` is the mixed-precision throughput in Matom-evals/s; the
-plain-FP64 throughput of the same step, per-stage device times, the roofline
-of the dominant kernel and a CPU baseline are reported beside it.
+Headline workload (one GPU): the 256,000-atom two-species SmoothPair B2 lattice
+in a sheared triclinic cell (cutoff 6.0 Å, amplitudes [1.0 0.7; 0.7 1.3] eV).
+One "step" = fresh cell build + E+F+V + FP64 reductions + nonfinite status
+read, positions already resident in HBM. ``value`` is the mixed-precision
+throughput in Matom-evals/s. Reported beside it: the plain-FP64 throughput of
+the same step, the 32,768-atom ACE workload (order-3, lmax-4, degree-12
+synthetic basis on a BCC binary, cutoff 5.0 Å) in both precisions, per-stage
+device times, the roofline of each dominant kernel, accuracy against the FP64
+oracle, and the CPU baseline (the oracle's threaded FP64 restatement).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-ace]
 """
 import argparse
+import ctypes
 import importlib.util
 import json
 import os
@@ -22,10 +26,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG_DIR = os.path.join(ROOT, "potentialframework.jl_amd")
 
 METRIC = "Matom-evals/s (E+F+V), 256k-atom pair & 32k-atom ACE; mixed vs FP64 at tol"
-# Published MI355X peaks used for roofline fractions.
-PEAK_FP32_TFLOPS = 157.3
+PEAK_FP32_TFLOPS = 157.3  # MI355X published FP32 (vector/matrix) peak
 PEAK_FP64_TFLOPS = 78.6
-PEAK_HBM_GBS = 8000.0
+STAGES = ("binning", "scan", "scatter_sort", "environments", "interaction", "reduction")
 # Algorithmic flops per directed in-cutoff pair (FMA = 2): d + shift 6, r^2 5,
 # compare 1, q 2, a q^3 3, g 2, force 6, 1/2 E 2, 1/2 W 18.
 FLOPS_PER_PAIR = {"smooth": 45.0, "lj": 50.0}
@@ -40,24 +43,33 @@ def load_package():
     return mod
 
 
-def time_pair_steps(pfw, w, sys_, p, steps, warmup, precision_mode):
-    """Time `steps` device steps of the pair path; returns (seconds, mean stage ms)."""
-    import ctypes
+def ace_flops(model, edges, sites):
+    """Algorithmic ACE flops of one E+F+V evaluation (FMA = 2).
+    Per edge: geometry 9, radial values+derivatives 12 per function (many-body
+    and pair), Y_lm with Cartesian gradients 32 per harmonic, density 2 per
+    density, pullback 4 per density + 8 per harmonic. Per site: each product
+    of order nu costs 2 nu (value and readout) + 2 nu^2 (reverse)."""
+    ny = (model.lmax + 1) ** 2
+    nd = len(model.density)
+    per_edge = 9 + 12 * model.radial.nr + 12 * model.pair.nr + 32 * ny + 2 * nd + 4 * nd + 8 * ny
+    per_site = sum(2 * len(p) + 2 * len(p) ** 2 for p in model.products)
+    return per_edge * edges + per_site * sites, per_edge, per_site
+
+
+def time_steps(lib, h, step, steps, warmup, mode):
+    """Time `steps` device steps (each ends with the status read); mean stage ms."""
     from pfw_amd import _capi
 
-    pfw.evaluate_(w, sys_, p)  # upload geometry + model (outside the timed region)
-    lib = _capi.load()
-    h = w.ctx.h
     for _ in range(warmup):
-        _capi.check(h, lib.pfw_pair_step(h, 7, precision_mode))
+        _capi.check(h, step(h, 7, mode))
     _capi.check(h, lib.pfw_set_timing(h, 1))
-    stage = np.zeros(5)
-    buf = np.zeros(5)
+    stage = np.zeros(len(STAGES))
+    buf = np.zeros(len(STAGES))
     _capi.check(h, lib.pfw_synchronize(h))
     t0 = time.perf_counter()
     for _ in range(steps):
-        _capi.check(h, lib.pfw_pair_step(h, 7, precision_mode))
-        lib.pfw_get_timings(h, buf.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 5)
+        _capi.check(h, step(h, 7, mode))
+        lib.pfw_get_timings(h, buf.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(STAGES))
         stage += buf
     _capi.check(h, lib.pfw_synchronize(h))
     t1 = time.perf_counter()
@@ -66,8 +78,8 @@ def time_pair_steps(pfw, w, sys_, p, steps, warmup, precision_mode):
 
 
 def read_traffic(name):
-    """HBM bytes per launch of the dominant kernel from the committed rocprofv3
-    PMC summary (FETCH_SIZE + WRITE_SIZE, calibrated), if present."""
+    """HBM bytes per launch of a kernel from its committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE, calibrated against a copy kernel), if present."""
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
@@ -78,24 +90,22 @@ def read_traffic(name):
         return None
 
 
-def cpu_baseline_pair(X, sp, H, pbc, A, rc, threads):
-    """The oracle's threaded FP64 restatement of the reference CPU pair path,
-    timed on the host on the full workload (bounded number of repetitions).
-    Also returns its FP64 outputs, used as the accuracy checker."""
-    sys.path.insert(0, ROOT)
-    from oracle import KIND_SMOOTH, pair_evaluate
-
-    pair_evaluate(X, sp, H, pbc, KIND_SMOOTH, A, rc, threads=threads)  # warm-up
-    times = []
-    out = None
-    budget = 20.0
+def cpu_baseline(fn, budget=20.0, reps=7):
+    """Median wall time of the oracle's threaded FP64 restatement (bounded budget)."""
+    fn()  # warm-up
+    times, out = [], None
     t_start = time.perf_counter()
-    while len(times) < 7 and (time.perf_counter() - t_start) < budget:
+    while len(times) < reps and (time.perf_counter() - t_start) < budget:
         t0 = time.perf_counter()
-        out = pair_evaluate(X, sp, H, pbc, KIND_SMOOTH, A, rc, threads=threads)
+        out = fn()
         times.append(time.perf_counter() - t0)
-    med = float(np.median(times))
-    return med, len(times), out
+    return float(np.median(times)), len(times), out
+
+
+def accuracy(res, E, F, W, n):
+    d = res["forces"].T - F
+    return {"force_max": float(np.abs(d).max()), "force_rms": float(np.sqrt(np.mean(d * d))),
+            "energy_per_atom": float(abs(res["energy"] - E) / n), "virial_max": float(np.abs(res["virial"] - W).max())}
 
 
 def main():
@@ -104,6 +114,7 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-ace", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -114,30 +125,30 @@ def main():
     pfw = load_package()
     from pfw_amd import _capi
 
+    lib = _capi.load()
+    threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
+
+    # ---------------------------------------------------------------- pair (headline)
     X, H, pbc, sp = pfw.workloads.b2_triclinic()
     n = len(X)
     A = np.array([[1.0, 0.7], [0.7, 1.3]])
     rc = 6.0
     p = pfw.SmoothPair(["A", "B"], A, rc)
     sys_ = pfw.System(X, H, pbc, ["A" if s == 0 else "B" for s in sp])
-
-    results = {}
+    pair = {}
     for prec, mode in (("mixed", _capi.PFW_MIXED), ("float64", _capi.PFW_FP64)):
         w = pfw.Workspace(precision=prec)
-        secs, stage = time_pair_steps(pfw, w, sys_, p, args.steps, args.warmup, mode)
-        res = pfw.evaluate_(w, sys_, p)
-        results[prec] = dict(seconds=secs, stage_ms=stage, result=res, ws=w)
-
-    npairs = len(results["mixed"]["ws"].export_pairs()[0])
-    mixed = results["mixed"]
-    ms_per_step = 1e3 * mixed["seconds"] / args.steps
+        res = pfw.evaluate_(w, sys_, p)  # uploads geometry + model outside the timed region
+        secs, stage = time_steps(lib, w.ctx.h, lib.pfw_pair_step, args.steps, args.warmup, mode)
+        pair[prec] = dict(seconds=secs, stage_ms=stage, result=res, ws=w)
+    npairs = len(pair["mixed"]["ws"].export_pairs()[0])
+    pflops = npairs * FLOPS_PER_PAIR["smooth"]
+    mixed = pair["mixed"]
     value = n * args.steps / mixed["seconds"] / 1e6
-    fp64_value = n * args.steps / results["float64"]["seconds"] / 1e6
-    k_ms = float(mixed["stage_ms"][3])
-    flops = npairs * FLOPS_PER_PAIR["smooth"]
-    achieved = flops / (k_ms * 1e-3) / 1e12
-    k64_ms = float(results["float64"]["stage_ms"][3])
-    traffic = read_traffic("pair_b2_256k_traffic.json")
+    fp64_value = n * args.steps / pair["float64"]["seconds"] / 1e6
+    k_ms = float(mixed["stage_ms"][4])
+    k64_ms = float(pair["float64"]["stage_ms"][4])
+    achieved = pflops / (k_ms * 1e-3) / 1e12
 
     line = {
         "metric": METRIC,
@@ -146,43 +157,90 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
+        "ms_per_step": round(1e3 * mixed["seconds"] / args.steps, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "b2_smoothpair_256k_triclinic", "atoms": n, "cutoff_A": rc,
-                   "species": 2, "cell": "B2 a=3.0 40x40x80, +/-5% shear", "precision": "mixed",
-                   "step": "cell build + E+F+V + FP64 reductions + status"},
+        "config": {"workload": "b2_smoothpair_256k_triclinic", "atoms": n, "cutoff_A": rc, "species": 2,
+                   "cell": "B2 a=3.0 A, 40x40x80 cells, +/-5% seeded shear", "precision": "mixed",
+                   "step": "cell build + E+F+V + FP64 reductions + status read"},
         "roofline": {"kernel": "k_pair<float,SMOOTH>", "bound": "mfma", "compute_unit": "VALU fp32",
                      "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
+                     "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": read_traffic("pair_b2_256k_traffic.json"),
                      "directed_pairs": npairs, "flops_per_pair": FLOPS_PER_PAIR["smooth"],
                      "kernel_ms": round(k_ms, 5)},
-        "fp64": {"value": round(fp64_value, 3), "ms_per_step": round(1e3 * results["float64"]["seconds"] / args.steps, 4),
+        "fp64": {"value": round(fp64_value, 3), "ms_per_step": round(1e3 * pair["float64"]["seconds"] / args.steps, 4),
                  "kernel_ms": round(k64_ms, 5),
-                 "roofline_frac_fp64": round(flops / (k64_ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS, 4)},
+                 "roofline_frac": round(pflops / (k64_ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS, 4)},
         "mixed_over_fp64": round(value / fp64_value, 3),
-        "stages_ms": {k: round(float(v), 5) for k, v in zip(("binning", "scan", "scatter_sort", "interaction",
-                                                              "reduction"), mixed["stage_ms"])},
+        "stages_ms": {k: round(float(v), 5) for k, v in zip(STAGES, mixed["stage_ms"])},
         "cpu_baseline": None,
     }
+
+    # ---------------------------------------------------------------- ACE
+    if not args.no_ace:
+        D = pfw.ace_basis.synthetic_model()
+        model, ps, _ = pfw.load_resolved(D)
+        Xa, Ha, pba, spa = pfw.workloads.bcc_binary()
+        from pfw_amd.ace import ELEMENTS
+        labels = [ELEMENTS[model.atomic_numbers[s] - 1] for s in spa]
+        sys_a = pfw.System(Xa, Ha, pba, labels)
+        ace = {}
+        for prec, mode in (("mixed", _capi.PFW_MIXED), ("float64", _capi.PFW_FP64)):
+            calc = pfw.ACECalculator(model, ps, precision=prec)
+            res = calc.evaluate(sys_a)
+            h = calc.workspace.ctx.h
+            secs, stage = time_steps(lib, h, lib.pfw_ace_step, args.steps, args.warmup, mode)
+            ace[prec] = dict(seconds=secs, stage_ms=stage, result=res, calc=calc)
+        edges = len(ace["mixed"]["calc"].workspace.export_pairs()[0])
+        na = len(Xa)
+        aflops, per_edge, per_site = ace_flops(model, edges, na)
+        ka = float(ace["mixed"]["stage_ms"][4])
+        ka64 = float(ace["float64"]["stage_ms"][4])
+        line["ace"] = {
+            "workload": "bcc_binary_ace_32k", "atoms": na, "cutoff_A": model.cutoff, "edges": edges,
+            "basis": {"radial": model.radial.nr, "lmax": model.lmax, "densities": len(model.density),
+                      "products": len(model.products), "invariants": model.nbasis, "correlation": 3, "degree": 12},
+            "value": round(na * args.steps / ace["mixed"]["seconds"] / 1e6, 4),
+            "fp64_value": round(na * args.steps / ace["float64"]["seconds"] / 1e6, 4),
+            "unit": "Matom-evals/s",
+            "ms_per_step": round(1e3 * ace["mixed"]["seconds"] / args.steps, 4),
+            "fp64_ms_per_step": round(1e3 * ace["float64"]["seconds"] / args.steps, 4),
+            "roofline": {"kernel": "k_ace_site<float,4>", "bound": "mfma", "compute_unit": "VALU fp32",
+                         "achieved": round(aflops / (ka * 1e-3) / 1e12, 3), "peak": PEAK_FP32_TFLOPS,
+                         "unit": "TFLOP/s", "frac": round(aflops / (ka * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4),
+                         "flops_per_edge": per_edge, "flops_per_site": per_site, "kernel_ms": round(ka, 5),
+                         "traffic": read_traffic("ace_bcc_32k_traffic.json")},
+            "fp64_roofline_frac": round(aflops / (ka64 * 1e-3) / 1e12 / PEAK_FP64_TFLOPS, 4),
+            "fp64_kernel_ms": round(ka64, 5),
+            "stages_ms": {k: round(float(v), 5) for k, v in zip(STAGES, ace["mixed"]["stage_ms"])},
+        }
+
+    # ---------------------------------------------------------------- CPU baseline + accuracy
     if not args.no_cpu_baseline:
-        threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
-        med, reps, (E, F, W) = cpu_baseline_pair(X, sp, H, pbc, A, rc, threads)
+        sys.path.insert(0, ROOT)
+        from oracle import KIND_SMOOTH, pair_evaluate
+
+        med, reps, (E, F, W) = cpu_baseline(lambda: pair_evaluate(X, sp, H, pbc, KIND_SMOOTH, A, rc, threads=threads))
         line["cpu_baseline"] = {"value": round(n / med / 1e6, 4), "unit": "Matom-evals/s", "cores": threads,
                                 "kind": "port",
                                 "sample": f"median of {reps} full 256,000-atom E+F+V evaluations (cell build + "
                                           f"phase-coloured half-list FP64, {threads} threads)"}
-        acc = {}
-        for prec in ("mixed", "float64"):
-            r = results[prec]["result"]
-            d = r["forces"].T - F
-            acc[prec] = {"force_max": float(np.abs(d).max()), "force_rms": float(np.sqrt(np.mean(d * d))),
-                         "energy_per_atom": float(abs(r["energy"] - E) / n),
-                         "virial_max": float(np.abs(r["virial"] - W).max())}
-        line["accuracy_vs_fp64_oracle"] = acc
+        line["accuracy_vs_fp64_oracle"] = {prec: accuracy(pair[prec]["result"], E, F, W, n) for prec in pair}
+        if "ace" in line:
+            from oracle.ace import ace_evaluate as oracle_ace
+            from oracle.ace import load_resolved as oracle_load
+
+            om = oracle_load(D)
+            med, reps, (Ea, Fa, Wa, _) = cpu_baseline(lambda: oracle_ace(Xa, spa, Ha, pba, om, threads=threads))
+            line["ace"]["cpu_baseline"] = {"value": round(na / med / 1e6, 5), "unit": "Matom-evals/s",
+                                           "cores": threads, "kind": "port",
+                                           "sample": f"median of {reps} full 32,768-atom E+F+V evaluations "
+                                                     f"(FP64 site pipeline, {threads} threads)"}
+            line["ace"]["accuracy_vs_fp64_oracle"] = {prec: accuracy(ace[prec]["result"], Ea, Fa, Wa, na)
+                                                      for prec in ace}
     print(json.dumps(line))
 
 

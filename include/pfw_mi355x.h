@@ -103,8 +103,10 @@ int pfw_grid_info(pfw_ctx* ctx, int64_t* ncells, int32_t* dims, int32_t* minimum
 
 /* Per-stage device times of the last step in ms, measured with HIP events on
  * the context's stream when timing is enabled (the reference's w.timings
- * dictionary, storage.jl:185-236). stages: 0 bin, 1 scan, 2 scatter+sort,
- * 3 interaction, 4 reduction. Returns the number written. */
+ * dictionary, storage.jl:185-236). stages: 0 binning, 1 scan, 2 scatter+sort,
+ * 3 environments (ACE edge lists), 4 interaction (the pair or ACE site
+ * kernel), 5 reduction (force gather, FP64 sums, status read). Returns the
+ * number written. */
 int pfw_set_timing(pfw_ctx* ctx, int enabled);
 int pfw_get_timings(pfw_ctx* ctx, double* ms, int n);
 

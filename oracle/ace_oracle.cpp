@@ -365,3 +365,12 @@ int oracle_ace_evaluate(int64_t n, const double* X, const int32_t* species, cons
 }
 
 }  // extern "C"
+
+extern "C" int oracle_ylm(int L, const double* R, double* Y, double* dY) {
+  try {
+    oracle::angular(L, R, Y, dY);
+    return 0;
+  } catch (const std::exception&) {
+    return 3;
+  }
+}

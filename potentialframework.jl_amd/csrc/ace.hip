@@ -52,10 +52,12 @@ struct AceArgs {
   const double* reference;
   FamArgs<T> rad, pair;
   const int32_t* dens;
-  const int32_t* dens_of;
-  const uint64_t* rmask;
+  const int32_t* rlb;     // L + 2: radial indices grouped by l
+  const int32_t* rl;      // (radial index, first density of its m-block)
   const int32_t* prod;
-  const int32_t* darow;
+  int nseg;
+  const int32_t* segb;    // nseg + 1: entry ranges of the dE/dA row segments
+  const int32_t* rowseg;  // ndens + 1: segment ranges of each density row
   const int32_t* dent;
   const T* w;   // nz x nprod
   const T* wp;  // nz x nq
@@ -243,36 +245,50 @@ __device__ inline void ylm(T dx, T dy, T dz, const T* Fl, T* Y, T* G) {
 #undef QI
 }
 
-template <typename T, int L>
+// One wavefront holds SPW sites (SPW = 1 or 2); each site owns LPS = 64 / SPW
+// consecutive lanes. With SPW = 2 every site has at most 32 edges (checked on
+// the host), so each site's edges fit one chunk.
+template <typename T, int L, int SPW>
 __global__ __launch_bounds__(64) void k_ace_site(AceArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int NY = (L + 1) * (L + 1);
-  T* Fl = (T*)smem;                 // (L+1)^2
-  T* A = Fl + NY;                   // ndens
-  T* dA = A + a.ndens;              // ndens
-  T* Rb = dA + a.ndens;             // nr x kStride
-  T* Yb = Rb + a.rad.nr * kStride;  // NY x kStride
+  constexpr int LPS = kWave / SPW;
+  T* Fl = (T*)smem;                      // (L+1)^2
+  T* A = Fl + NY;                        // SPW x ndens
+  T* dA = A + SPW * a.ndens;             // SPW x ndens
+  T* part = dA + SPW * a.ndens;          // SPW x nseg
+  T* Rb = part + SPW * a.nseg;           // nr x kStride
+  T* Yb = Rb + a.rad.nr * kStride;       // NY x kStride
   const int lane = threadIdx.x;
-  const int site = blockIdx.x;
+  const int sl = lane / LPS, ll = lane % LPS;
+  const int64_t site = (int64_t)blockIdx.x * SPW + sl;
+  const bool valid = site < a.n;
   if (lane < NY) {
     int l = 0;
     while ((l + 1) * (l + 1) <= lane) ++l;
-    int m = lane - l * l;
+    const int m = lane - l * l;
     if (m <= l) {
       double F = sqrt((2.0 * l + 1) / (2 * 3.14159265358979323846));
       for (int mm = 1; mm <= m; ++mm) F = -F / sqrt((double)(l + mm) * (l + 1 - mm));
       Fl[l * (L + 1) + m] = (T)F;
     }
   }
-  for (int k = lane; k < a.ndens; k += kWave) A[k] = 0;
-  const int iz = a.sps[site];
-  const int e0 = a.estart[site], ne = a.estart[site + 1] - e0;
+  for (int k = lane; k < SPW * a.ndens; k += kWave) A[k] = 0;
+  T* As = A + sl * a.ndens;
+  T* dAs = dA + sl * a.ndens;
+  T* ps = part + sl * a.nseg;
+  const int iz = valid ? a.sps[site] : 0;
+  const int e0 = valid ? a.estart[site] : 0;
+  const int ne = valid ? a.estart[site + 1] - e0 : 0;
+  int nemax = ne;
+  for (int off = LPS; off < kWave; off <<= 1) nemax = max(nemax, __shfl_xor(nemax, off, kWave));
   const int nq = a.pair.nr;
+  const int col0 = sl * LPS;
   double epair = 0;
   __syncthreads();
-  // ---- phase A: densities
-  for (int c0 = 0; c0 < ne; c0 += kWave) {
-    const int e = c0 + lane;
+  // ---- phase A: densities A[k] = sum_e R[e, r_k] Y[e, m_k]
+  for (int c0 = 0; c0 < nemax; c0 += LPS) {
+    const int e = c0 + ll;
     if (e < ne) {
       const double* R = a.eR + 3 * (int64_t)(e0 + e);
       const T dx = (T)R[0], dy = (T)R[1], dz = (T)R[2];
@@ -300,50 +316,56 @@ __global__ __launch_bounds__(64) void k_ace_site(AceArgs<T> a) {
       for (int m = 0; m < NY; ++m) Yb[m * kStride + lane] = Y[m];
     }
     __syncthreads();
-    const int cn = min(kWave, ne - c0);
-    for (int k = lane; k < a.ndens; k += kWave) {
-      const int rn = a.dens[2 * k], mm = a.dens[2 * k + 1];
-      const T* rr = Rb + rn * kStride;
-      const T* yy = Yb + mm * kStride;
+    const int cn = min(LPS, ne - c0);
+    for (int k = ll; k < a.ndens; k += LPS) {
+      const T* rr = Rb + a.dens[2 * k] * kStride + col0;
+      const T* yy = Yb + a.dens[2 * k + 1] * kStride + col0;
       T s = 0;
       for (int t = 0; t < cn; ++t) s += rr[t] * yy[t];
-      A[k] += s;
+      As[k] += s;
     }
     __syncthreads();
   }
-  // ---- phase B: products, energy and dE/dA
+  // ---- phase B: products, energy and dE/dA over balanced row segments
   const T* w = a.w + (int64_t)iz * a.nprod;
   T eprod = 0;
-  for (int p = lane; p < a.nprod; p += kWave) {
+  for (int p = ll; p < a.nprod; p += LPS) {
     const int* f = a.prod + 4 * p;
-    T v = A[f[0]];
-    if (f[1] >= 0) v *= A[f[1]];
-    if (f[2] >= 0) v *= A[f[2]];
-    if (f[3] >= 0) v *= A[f[3]];
+    T v = As[f[0]];
+    if (f[1] >= 0) v *= As[f[1]];
+    if (f[2] >= 0) v *= As[f[2]];
+    if (f[3] >= 0) v *= As[f[3]];
     eprod += w[p] * v;
   }
   if (a.grads) {
-    for (int k = lane; k < a.ndens; k += kWave) {
-      T s = 0;
-      for (int q = a.darow[k]; q < a.darow[k + 1]; ++q) {
+    for (int s = ll; s < a.nseg; s += LPS) {
+      T acc = 0;
+      for (int q = a.segb[s]; q < a.segb[s + 1]; ++q) {
         const int* d = a.dent + 4 * q;
         T v = w[d[0]];
-        if (d[1] >= 0) v *= A[d[1]];
-        if (d[2] >= 0) v *= A[d[2]];
-        if (d[3] >= 0) v *= A[d[3]];
-        s += v;
+        if (d[1] >= 0) v *= As[d[1]];
+        if (d[2] >= 0) v *= As[d[2]];
+        if (d[3] >= 0) v *= As[d[3]];
+        acc += v;
       }
-      dA[k] = s;
+      ps[s] = acc;
+    }
+    __syncthreads();
+    for (int k = ll; k < a.ndens; k += LPS) {
+      T acc = 0;
+      for (int s = a.rowseg[k]; s < a.rowseg[k + 1]; ++s) acc += ps[s];
+      dAs[k] = acc;
     }
   }
-  const double es = wave_sum((double)eprod) + wave_sum(epair);
-  if (lane == 0) a.siteE[site] = a.reference[iz] + es;
+  double es = (double)eprod + epair;
+  for (int off = LPS / 2; off > 0; off >>= 1) es += __shfl_xor(es, off, kWave);
+  if (valid && ll == 0) a.siteE[site] = a.reference[iz] + es;
   if (!a.grads) return;
   __syncthreads();
-  // ---- phase C: force pullback
+  // ---- phase C: force pullback, g = r^ sum_r R'_r T_r + sum_m U_m grad Y_m
   double wv[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  for (int c0 = 0; c0 < ne; c0 += kWave) {
-    const int e = c0 + lane;
+  for (int c0 = 0; c0 < nemax; c0 += LPS) {
+    const int e = c0 + ll;
     if (e >= ne) continue;
     const double* R = a.eR + 3 * (int64_t)(e0 + e);
     const T dx = (T)R[0], dy = (T)R[1], dz = (T)R[2];
@@ -356,21 +378,21 @@ __global__ __launch_bounds__(64) void k_ace_site(AceArgs<T> a) {
     for (int m = 0; m < NY; ++m) U[m] = 0;
     T gs = 0;
     const RadialPoint<T> P = radial_point(a.rad, ch, r);
-    for (int n = 0; n < a.rad.nr; ++n) {
-      T v, d;
-      radial_value(a.rad, ch, P, n, v, d);
-      const uint64_t mask = a.rmask[n];
-      const int* dof = a.dens_of + n * a.ny;
-      T tr = 0;
 #pragma unroll
-      for (int m = 0; m < NY; ++m) {
-        if ((mask >> m) & 1ull) {
-          const T c = dA[dof[m]];
-          tr += c * Y[m];
-          U[m] += c * v;
+    for (int l = 0; l <= L; ++l) {
+      for (int q = a.rlb[l]; q < a.rlb[l + 1]; ++q) {
+        const int n = a.rl[2 * q], a0 = a.rl[2 * q + 1];
+        T v, d;
+        radial_value(a.rad, ch, P, n, v, d);
+        T tr = 0;
+#pragma unroll
+        for (int mm = 0; mm <= 2 * l; ++mm) {
+          const T c = dAs[a0 + mm];
+          tr += c * Y[l * l + mm];
+          U[l * l + mm] += c * v;
         }
+        gs += d * tr;
       }
-      gs += d * tr;
     }
     if (nq) {
       const RadialPoint<T> PQ = radial_point(a.pair, ch, r);
@@ -396,8 +418,9 @@ __global__ __launch_bounds__(64) void k_ace_site(AceArgs<T> a) {
       for (int aa = 0; aa < 3; ++aa) wv[aa + 3 * bb] -= R[aa] * gg[bb];
   }
   for (int k = 0; k < 9; ++k) {
-    const double v = wave_sum(wv[k]);
-    if (lane == 0) a.siteW[9 * (int64_t)site + k] = v;
+    double v = wv[k];
+    for (int off = LPS / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    if (valid && ll == 0) a.siteW[9 * site + k] = v;
   }
 }
 
@@ -406,7 +429,8 @@ __global__ __launch_bounds__(64) void k_ace_site(AceArgs<T> a) {
 __global__ void k_ace_edges(const Grid* __restrict__ gp, int64_t n, const double* __restrict__ xs,
                             const int32_t* __restrict__ start, const int32_t* __restrict__ cell_of,
                             const int32_t* __restrict__ perm, int pass, int32_t* __restrict__ ecount,
-                            const int32_t* __restrict__ estart, int32_t* __restrict__ enb, double* __restrict__ eR) {
+                            const int32_t* __restrict__ estart, int32_t* __restrict__ enb, double* __restrict__ eR,
+                            int32_t* __restrict__ maxdeg) {
   const Grid& g = *gp;
   int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
@@ -438,7 +462,10 @@ __global__ void k_ace_edges(const Grid* __restrict__ gp, int64_t n, const double
       ++cnt;
     }
   }
-  if (!pass) ecount[k] = cnt;
+  if (!pass) {
+    ecount[k] = cnt;
+    atomicMax(maxdeg, cnt);
+  }
 }
 
 // rev[e] = the edge (j -> i, -R) of edge e = (i -> j, R); R negates exactly.
@@ -483,28 +510,27 @@ __global__ void k_ace_force(int64_t n, const int32_t* __restrict__ estart, const
   F[3 * o + 2] = f2;
 }
 
-__global__ void k_ace_reduce(int64_t n, const double* __restrict__ siteE, const double* __restrict__ siteW, int grads,
-                             const int32_t* __restrict__ status, double* __restrict__ out) {
-  __shared__ double lds[10][4];
-  double s[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-  for (int64_t k = threadIdx.x; k < n; k += blockDim.x) {
-    s[0] += siteE[k];
-    if (grads)
-      for (int c = 0; c < 9; ++c) s[1 + c] += siteW[9 * k + c];
+// siteE and siteW packed as 10-column records for the two-stage reduction.
+__global__ void k_ace_pack(int64_t n, const double* __restrict__ siteE, const double* __restrict__ siteW, int grads,
+                           double* __restrict__ rec) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  rec[10 * k] = siteE[k];
+  for (int c = 0; c < 9; ++c) rec[10 * k + 1 + c] = grads ? siteW[9 * k + c] : 0.0;
+}
+
+__global__ void k_ace_finish(const double* __restrict__ part, int nb, const int32_t* __restrict__ status,
+                             double* __restrict__ out) {
+  if (threadIdx.x != 0) return;
+  double t[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int b = 0; b < nb; ++b)
+    for (int c = 0; c < 10; ++c) t[c] += part[10 * b + c];
+  bool bad = status[0] != 0;
+  for (int c = 0; c < 10; ++c) {
+    out[c] = t[c];
+    bad |= !isfinite(t[c]);
   }
-  for (int c = 0; c < 10; ++c) s[c] = wave_sum(s[c]);
-  const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
-  if (lane == 0)
-    for (int c = 0; c < 10; ++c) lds[c][wid] = s[c];
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    bool bad = *status != 0;
-    for (int c = 0; c < 10; ++c) {
-      out[c] = lds[c][0] + lds[c][1] + lds[c][2] + lds[c][3];
-      bad |= !isfinite(out[c]);
-    }
-    out[10] = bad ? 1.0 : 0.0;
-  }
+  out[10] = bad ? 1.0 : 0.0;
 }
 
 __global__ void k_ace_scatter_sites(int64_t n, const int32_t* __restrict__ perm, const double* __restrict__ siteE,
@@ -552,6 +578,7 @@ void ace_free_edges(pfw_ctx* ctx) {
   dfree(a.eg);
   dfree(a.siteE);
   dfree(a.siteW);
+  dfree(a.rec);
   dfree(a.status);
   a.edge_cap = 0;
 }
@@ -563,11 +590,12 @@ void ace_free(pfw_ctx* ctx) {
   free_family(a.pair);
   dfree(a.reference);
   dfree(a.prod);
-  dfree(a.darow);
+  dfree(a.segb);
+  dfree(a.rowseg);
   dfree(a.dent);
   dfree(a.dens);
-  dfree(a.dens_of);
-  dfree(a.rmask);
+  dfree(a.rlb);
+  dfree(a.rl);
   dfree(a.w64);
   dfree(a.w32);
   dfree(a.wp64);
@@ -690,21 +718,40 @@ void ace_set_model(pfw_ctx* ctx, const pfw_ace_model* m) {
   for (double v : ref)
     if (!std::isfinite(v)) throw Error(PFW_INVALID_ARGUMENT, "nonfinite reference energies");
   a.reference = upload(ref, st);
-  // densities
+  // densities: each radial index must own one contiguous, complete m-block
+  // (r, l*l .. l*l+2l), as the resolved schema's (n, l) channels do.
   std::vector<int32_t> dens(m->density, m->density + 2 * (size_t)m->ndens);
-  std::vector<int32_t> dof((size_t)a.radial.nr * a.ny, -1);
-  std::vector<uint64_t> rmask(a.radial.nr, 0);
+  std::vector<int> first(a.radial.nr, -1), lof(a.radial.nr, -1), cnt(a.radial.nr, 0);
   for (int k = 0; k < m->ndens; ++k) {
     int r = dens[2 * k], h = dens[2 * k + 1];
     if (r < 0 || r >= a.radial.nr || h < 0 || h >= a.ny) throw Error(PFW_INVALID_ARGUMENT, "invalid density indices");
-    if (dof[(size_t)r * a.ny + h] >= 0) throw Error(PFW_INVALID_ARGUMENT, "duplicate density entry");
-    dof[(size_t)r * a.ny + h] = k;
-    rmask[r] |= 1ull << h;
+    int l = 0;
+    while ((l + 1) * (l + 1) <= h) ++l;
+    if (first[r] < 0) {
+      first[r] = k;
+      lof[r] = l;
+    }
+    if (lof[r] != l || k != first[r] + cnt[r] || h != l * l + cnt[r])
+      throw Error(PFW_INVALID_ARGUMENT, "densities of a radial channel must form one contiguous complete m-block");
+    ++cnt[r];
+  }
+  std::vector<int32_t> rlb(a.lmax + 2, 0), rl;
+  for (int l = 0; l <= a.lmax; ++l) {
+    for (int r = 0; r < a.radial.nr; ++r)
+      if (first[r] >= 0 && lof[r] == l) {
+        if (cnt[r] != 2 * l + 1)
+          throw Error(PFW_INVALID_ARGUMENT, "densities of a radial channel must form one contiguous complete m-block");
+        rl.push_back(r);
+        rl.push_back(first[r]);
+      }
+    rlb[l + 1] = (int32_t)(rl.size() / 2);
   }
   a.dens = upload(dens, st);
-  a.dens_of = upload(dof, st);
-  a.rmask = upload(rmask, st);
-  // products and the per-density derivative CSR
+  a.rlb = upload(rlb, st);
+  a.rl = upload(rl, st);
+  // products and the per-density derivative rows, cut into segments of at most
+  // kSegLen entries so that the lanes of the dE/dA phase stay balanced
+  constexpr int kSegLen = 8;
   std::vector<int32_t> prod(4 * (size_t)m->nprod, -1);
   std::vector<std::vector<std::array<int32_t, 4>>> rows(m->ndens);
   for (int k = 0; k < m->nprod; ++k) {
@@ -718,18 +765,22 @@ void ace_set_model(pfw_ctx* ctx, const pfw_ace_model* m) {
     for (int t = 0; t < nu; ++t) {  // d P / d A_{idx[t]}: product of the other factors
       std::array<int32_t, 4> ent{k, -1, -1, -1};
       int o = 1;
-      for (int s = 0; s < nu; ++s)
-        if (s != t) ent[o++] = m->product_idx[b + s];
+      for (int s2 = 0; s2 < nu; ++s2)
+        if (s2 != t) ent[o++] = m->product_idx[b + s2];
       rows[m->product_idx[b + t]].push_back(ent);
     }
   }
-  std::vector<int32_t> darow(m->ndens + 1, 0), dent;
+  std::vector<int32_t> segb(1, 0), rowseg(m->ndens + 1, 0), dent;
   for (int d = 0; d < m->ndens; ++d) {
-    darow[d + 1] = darow[d] + (int)rows[d].size();
+    int len = (int)rows[d].size();
+    for (int q = 0; q < len; q += kSegLen) segb.push_back(segb.back() + std::min(kSegLen, len - q));
+    rowseg[d + 1] = (int32_t)segb.size() - 1;
     for (auto& e : rows[d]) dent.insert(dent.end(), e.begin(), e.end());
   }
+  a.nseg = (int)segb.size() - 1;
   a.prod = upload(prod, st);
-  a.darow = upload(darow, st);
+  a.segb = upload(segb, st);
+  a.rowseg = upload(rowseg, st);
   a.dent = upload(dent, st);
   a.c_rows.assign(m->coupling_rows, m->coupling_rows + m->coupling_nnz);
   a.c_cols.assign(m->coupling_cols, m->coupling_cols + m->coupling_nnz);
@@ -766,7 +817,7 @@ static FamArgs<T> fam_args(const AceFamilyDev& f) {
 
 static inline unsigned nblk(int64_t n, int t) { return (unsigned)std::max<int64_t>(1, (n + t - 1) / t); }
 
-template <typename T, int L>
+template <typename T, int L, int SPW>
 static void launch_sites(pfw_ctx* ctx, int grads) {
   AceDev& ad = ctx->ace;
   SystemBuffers& s = ctx->sys;
@@ -785,10 +836,12 @@ static void launch_sites(pfw_ctx* ctx, int grads) {
   a.rad = fam_args<T>(ad.radial);
   a.pair = fam_args<T>(ad.pair);
   a.dens = ad.dens;
-  a.dens_of = ad.dens_of;
-  a.rmask = ad.rmask;
+  a.rlb = ad.rlb;
+  a.rl = ad.rl;
   a.prod = ad.prod;
-  a.darow = ad.darow;
+  a.nseg = ad.nseg;
+  a.segb = ad.segb;
+  a.rowseg = ad.rowseg;
   a.dent = ad.dent;
   if constexpr (sizeof(T) == 8) {
     a.w = ad.w64;
@@ -802,25 +855,36 @@ static void launch_sites(pfw_ctx* ctx, int grads) {
   a.siteW = ad.siteW;
   a.eg = ad.eg;
   constexpr int NY = (L + 1) * (L + 1);
-  size_t lds = sizeof(T) * ((size_t)NY + 2 * (size_t)ad.ndens + (size_t)(ad.radial.nr + NY) * kStride);
+  size_t lds = sizeof(T) * ((size_t)NY + SPW * (2 * (size_t)ad.ndens + (size_t)ad.nseg) +
+                            (size_t)(ad.radial.nr + NY) * kStride);
   if (lds > 160 * 1024) throw Error(PFW_INVALID_ARGUMENT, "ACE model too large for the site kernel's LDS tiles");
   if (lds > 64 * 1024)
-    PFW_HIP(hipFuncSetAttribute((const void*)k_ace_site<T, L>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  hipLaunchKernelGGL((k_ace_site<T, L>), dim3((unsigned)s.n), dim3(kWave), lds, ctx->stream, a);
+    PFW_HIP(hipFuncSetAttribute((const void*)k_ace_site<T, L, SPW>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds));
+  const unsigned blocks = (unsigned)((s.n + SPW - 1) / SPW);
+  hipLaunchKernelGGL((k_ace_site<T, L, SPW>), dim3(blocks), dim3(kWave), lds, ctx->stream, a);
   PFW_HIP(hipGetLastError());
+}
+
+template <typename T, int L>
+static void launch_sites_spw(pfw_ctx* ctx, int grads) {
+  if (ctx->ace.max_degree <= kWave / 2)
+    launch_sites<T, L, 2>(ctx, grads);
+  else
+    launch_sites<T, L, 1>(ctx, grads);
 }
 
 template <typename T>
 static void dispatch_sites(pfw_ctx* ctx, int grads) {
   switch (ctx->ace.lmax <= 2 ? 2 : ctx->ace.lmax <= 4 ? 4 : 6) {
     case 2:
-      launch_sites<T, 2>(ctx, grads);
+      launch_sites_spw<T, 2>(ctx, grads);
       break;
     case 4:
-      launch_sites<T, 4>(ctx, grads);
+      launch_sites_spw<T, 4>(ctx, grads);
       break;
     default:
-      launch_sites<T, 6>(ctx, grads);
+      launch_sites_spw<T, 6>(ctx, grads);
   }
 }
 
@@ -836,6 +900,7 @@ void ace_run(pfw_ctx* ctx, uint32_t mask, int precision) {
   record_stage(ctx, 3);
   if (n == 0 || mask == 0) {
     record_stage(ctx, 4);
+    record_stage(ctx, 5);
     PFW_HIP(hipMemsetAsync(ctx->d_out, 0, 11 * sizeof(double), st));
     if (n > 0) PFW_HIP(hipMemsetAsync(ctx->d_F, 0, 3 * n * sizeof(double), st));
   } else {
@@ -845,16 +910,19 @@ void ace_run(pfw_ctx* ctx, uint32_t mask, int precision) {
       PFW_HIP(hipMalloc(&ad.ebs, ((n + 1023) / 1024 + 2) * sizeof(int32_t)));
       PFW_HIP(hipMalloc(&ad.siteE, n * sizeof(double)));
       PFW_HIP(hipMalloc(&ad.siteW, 9 * n * sizeof(double)));
-      PFW_HIP(hipMalloc(&ad.status, sizeof(int32_t)));
+      PFW_HIP(hipMalloc(&ad.rec, 10 * n * sizeof(double)));
+      PFW_HIP(hipMalloc(&ad.status, 4 * sizeof(int32_t)));
     }
-    PFW_HIP(hipMemsetAsync(ad.status, 0, sizeof(int32_t), st));
+    PFW_HIP(hipMemsetAsync(ad.status, 0, 4 * sizeof(int32_t), st));
     hipLaunchKernelGGL(k_ace_edges, dim3(nblk(n, 128)), dim3(128), 0, st, s.grid, n, s.xs, s.start, s.cell_of, s.perm,
-                       0, ad.ecount, (const int32_t*)nullptr, (int32_t*)nullptr, (double*)nullptr);
+                       0, ad.ecount, (const int32_t*)nullptr, (int32_t*)nullptr, (double*)nullptr, ad.status + 1);
     PFW_HIP(hipGetLastError());
     scan_counts(ctx, ad.ecount, (int)n, ad.estart, ad.ebs, ad.ecount /* cursor: reused, already consumed */);
-    int32_t total = 0;
+    int32_t total = 0, maxdeg = 0;
     PFW_HIP(hipMemcpyAsync(&total, ad.estart + n, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    PFW_HIP(hipMemcpyAsync(&maxdeg, ad.status + 1, sizeof(int32_t), hipMemcpyDeviceToHost, st));
     PFW_HIP(hipStreamSynchronize(st));
+    ad.max_degree = maxdeg;
     if (total < 0) throw Error(PFW_CAPACITY, "edge count overflow");
     if (total > ad.edge_cap || !ad.enb) {
       dfree(ad.enb);
@@ -869,7 +937,7 @@ void ace_run(pfw_ctx* ctx, uint32_t mask, int precision) {
       ad.edge_cap = cap;
     }
     hipLaunchKernelGGL(k_ace_edges, dim3(nblk(n, 128)), dim3(128), 0, st, s.grid, n, s.xs, s.start, s.cell_of, s.perm,
-                       1, ad.ecount, (const int32_t*)ad.estart, ad.enb, ad.eR);
+                       1, ad.ecount, (const int32_t*)ad.estart, ad.enb, ad.eR, ad.status + 1);
     PFW_HIP(hipGetLastError());
     const int grads = (mask & (PFW_FORCES | PFW_VIRIAL)) ? 1 : 0;
     if (grads)
@@ -880,14 +948,17 @@ void ace_run(pfw_ctx* ctx, uint32_t mask, int precision) {
       dispatch_sites<float>(ctx, grads);
     else
       dispatch_sites<double>(ctx, grads);
+    record_stage(ctx, 5);
     if (grads)
       hipLaunchKernelGGL(k_ace_force, dim3(nblk(n, 256)), dim3(256), 0, st, n, ad.estart, ad.erev, ad.eg, s.perm,
                          ctx->d_F);
-    hipLaunchKernelGGL(k_ace_reduce, dim3(1), dim3(256), 0, st, n, ad.siteE, ad.siteW, grads, ad.status, ctx->d_out);
+    hipLaunchKernelGGL(k_ace_pack, dim3(nblk(n, 256)), dim3(256), 0, st, n, ad.siteE, ad.siteW, grads, ad.rec);
+    hipLaunchKernelGGL(k_colsum, dim3(kRedBlocks), dim3(256), 0, st, ad.rec, n, 10, ctx->d_red);
+    hipLaunchKernelGGL(k_ace_finish, dim3(1), dim3(64), 0, st, ctx->d_red, kRedBlocks, ad.status, ctx->d_out);
     PFW_HIP(hipGetLastError());
   }
   PFW_HIP(hipMemcpyAsync(ctx->h_out, ctx->d_out, 11 * sizeof(double), hipMemcpyDeviceToHost, st));
-  record_stage(ctx, 5);
+  record_stage(ctx, 6);
   PFW_HIP(hipStreamSynchronize(st));
   if (ctx->timing)
     for (int k = 0; k < kStages; ++k) {

@@ -50,6 +50,7 @@ int pfw_ctx_create(int device, pfw_ctx** out) {
   int rc = run(ctx, [&] {
     PFW_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     PFW_HIP(hipMalloc(&ctx->d_out, 16 * sizeof(double)));
+    PFW_HIP(hipMalloc(&ctx->d_red, 64 * 16 * sizeof(double)));
     PFW_HIP(hipHostMalloc(&ctx->h_out, 16 * sizeof(double), hipHostMallocDefault));
     for (auto& e : ctx->ev) PFW_HIP(hipEventCreate(&e));
   });
@@ -70,6 +71,7 @@ void pfw_ctx_destroy(pfw_ctx* ctx) {
   if (ctx->d_F) (void)hipFree(ctx->d_F);
   if (ctx->d_partials) (void)hipFree(ctx->d_partials);
   if (ctx->d_out) (void)hipFree(ctx->d_out);
+  if (ctx->d_red) (void)hipFree(ctx->d_red);
   if (ctx->h_out) (void)hipHostFree(ctx->h_out);
   if (ctx->pair.d_params) (void)hipFree(ctx->pair.d_params);
   for (auto& e : ctx->ev)

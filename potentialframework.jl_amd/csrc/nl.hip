@@ -401,31 +401,68 @@ __global__ void k_scatter(int64_t n, const int32_t* __restrict__ cell_of, const 
 }
 
 // Deterministic order inside each cell: ascending original index (the stable
-// sortperm of storage.jl:199).
-__global__ void k_cell_sort(int ncells, const int32_t* __restrict__ start, int32_t* __restrict__ perm) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncells) return;
-  int b = start[c], e = start[c + 1];
-  for (int k = b + 1; k < e; ++k) {
-    int v = perm[k];
-    int m = k - 1;
-    while (m >= b && perm[m] > v) {
-      perm[m + 1] = perm[m];
-      --m;
-    }
-    perm[m + 1] = v;
+// sortperm of storage.jl:199), then the gather into sorted SoA coordinates.
+// One wavefront per cell: cells of up to 64 atoms are sorted with a register
+// bitonic network over lane shuffles; larger cells fall back to an insertion
+// sort by lane 0 (dense clusters only).
+__global__ __launch_bounds__(64) void k_cell_sort_gather(int ncells, int64_t n, const int32_t* __restrict__ start,
+                                                         int32_t* __restrict__ perm, const double* __restrict__ xw,
+                                                         const int32_t* __restrict__ species, double* __restrict__ xs,
+                                                         int32_t* __restrict__ sps) {
+  const int c = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int b = start[c], e = start[c + 1], cnt = e - b;
+  if (cnt == 0) return;
+  if (cnt <= kWave) {
+    int v = lane < cnt ? perm[b + lane] : 0x7fffffff;
+    for (int k = 2; k <= kWave; k <<= 1)
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        const int o = __shfl_xor(v, j, kWave);
+        const bool up = (lane & k) == 0;
+        const bool lower = (lane & j) == 0;
+        v = (lower == up) ? min(v, o) : max(v, o);
+      }
+    if (lane < cnt) perm[b + lane] = v;
+  } else {
+    if (lane == 0)
+      for (int k = b + 1; k < e; ++k) {
+        int v = perm[k];
+        int m = k - 1;
+        while (m >= b && perm[m] > v) {
+          perm[m + 1] = perm[m];
+          --m;
+        }
+        perm[m + 1] = v;
+      }
+    __syncthreads();
+  }
+  for (int k = b + lane; k < e; k += kWave) {
+    const int i = perm[k];
+    xs[k] = xw[i];
+    xs[n + k] = xw[n + i];
+    xs[2 * n + k] = xw[2 * n + i];
+    sps[k] = species[i];
   }
 }
 
-__global__ void k_gather(int64_t n, const int32_t* __restrict__ perm, const double* __restrict__ xw,
-                         const int32_t* __restrict__ species, double* __restrict__ xs, int32_t* __restrict__ sps) {
-  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  int i = perm[k];
-  xs[k] = xw[i];
-  xs[n + k] = xw[n + i];
-  xs[2 * n + k] = xw[2 * n + i];
-  sps[k] = species[i];
+__global__ void k_colsum(const double* __restrict__ in, int64_t m, int width, double* __restrict__ out) {
+  __shared__ double lds[16][4];
+  const int64_t chunk = (m + gridDim.x - 1) / gridDim.x;
+  const int64_t lo = (int64_t)blockIdx.x * chunk, hi = min(m, lo + chunk);
+  double s[16];
+  for (int c = 0; c < width; ++c) s[c] = 0;
+  for (int64_t r = lo + threadIdx.x; r < hi; r += blockDim.x)
+    for (int c = 0; c < width; ++c) s[c] += in[r * width + c];
+  const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+  for (int c = 0; c < width; ++c) {
+    const double v = wave_sum(s[c]);
+    if (lane == 0) lds[c][wid] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < width) {
+    const int c = threadIdx.x;
+    out[(int64_t)blockIdx.x * width + c] = lds[c][0] + lds[c][1] + lds[c][2] + lds[c][3];
+  }
 }
 
 static inline unsigned nblocks(int64_t n, int t) { return (unsigned)std::max<int64_t>(1, (n + t - 1) / t); }
@@ -475,8 +512,8 @@ void nl_build(pfw_ctx* ctx) {
   record_stage(ctx, 2);
   if (n > 0) {
     hipLaunchKernelGGL(k_scatter, dim3(nblocks(n, 256)), dim3(256), 0, st, n, s.cell_of, s.start, s.cursor, s.perm);
-    hipLaunchKernelGGL(k_cell_sort, dim3(nblocks(g.ncells, 256)), dim3(256), 0, st, g.ncells, s.start, s.perm);
-    hipLaunchKernelGGL(k_gather, dim3(nblocks(n, 256)), dim3(256), 0, st, n, s.perm, s.xw, s.species, s.xs, s.sps);
+    hipLaunchKernelGGL(k_cell_sort_gather, dim3(g.ncells), dim3(kWave), 0, st, g.ncells, n, s.start, s.perm, s.xw,
+                       s.species, s.xs, s.sps);
   }
   PFW_HIP(hipGetLastError());
 }

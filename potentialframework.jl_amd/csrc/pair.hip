@@ -235,36 +235,27 @@ __global__ __launch_bounds__(64) void k_pair(const Grid* __restrict__ gp, int64_
     for (int k = 0; k < 7; ++k) partials[7 * (int64_t)cell + k] = v[k];
 }
 
-// FP64 reduction of the per-cell partials into E, W (3x3 column-major) and the
-// nonfinite status word (gpu.jl:390-395).
-__global__ void k_reduce(const double* __restrict__ partials, int ncells, double* __restrict__ out) {
-  __shared__ double lds[7][4];
-  double s[7] = {0, 0, 0, 0, 0, 0, 0};
-  for (int c = threadIdx.x; c < ncells; c += blockDim.x)
-    for (int k = 0; k < 7; ++k) s[k] += partials[7 * (int64_t)c + k];
-  for (int k = 0; k < 7; ++k) s[k] = wave_sum(s[k]);
-  int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
-  if (lane == 0)
-    for (int k = 0; k < 7; ++k) lds[k][wid] = s[k];
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double t[7];
-    for (int k = 0; k < 7; ++k) t[k] = lds[k][0] + lds[k][1] + lds[k][2] + lds[k][3];
-    double E = 0.5 * t[0];
-    // P = sum d (x) g over directed pairs: xx yy zz xy xz yz
-    double W[9];
-    W[0] = -0.5 * t[1];
-    W[4] = -0.5 * t[2];
-    W[8] = -0.5 * t[3];
-    W[3] = W[1] = -0.5 * t[4];
-    W[6] = W[2] = -0.5 * t[5];
-    W[7] = W[5] = -0.5 * t[6];
-    bool bad = !isfinite(E);
-    for (int k = 0; k < 9; ++k) bad |= !isfinite(W[k]);
-    out[0] = E;
-    for (int k = 0; k < 9; ++k) out[1 + k] = W[k];
-    out[10] = bad ? 1.0 : 0.0;
-  }
+// Final FP64 reduction of the stage-1 column sums into E, W (3x3 column-major)
+// and the nonfinite status word (gpu.jl:390-395).
+__global__ void k_reduce_finish(const double* __restrict__ part, int nb, double* __restrict__ out) {
+  if (threadIdx.x != 0) return;
+  double t[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (int b = 0; b < nb; ++b)
+    for (int k = 0; k < 7; ++k) t[k] += part[7 * b + k];
+  const double E = 0.5 * t[0];
+  // P = sum d (x) g over directed pairs: xx yy zz xy xz yz
+  double W[9];
+  W[0] = -0.5 * t[1];
+  W[4] = -0.5 * t[2];
+  W[8] = -0.5 * t[3];
+  W[3] = W[1] = -0.5 * t[4];
+  W[6] = W[2] = -0.5 * t[5];
+  W[7] = W[5] = -0.5 * t[6];
+  bool bad = !isfinite(E);
+  for (int k = 0; k < 9; ++k) bad |= !isfinite(W[k]);
+  out[0] = E;
+  for (int k = 0; k < 9; ++k) out[1 + k] = W[k];
+  out[10] = bad ? 1.0 : 0.0;
 }
 
 void record_stage(pfw_ctx* ctx, int k) {
@@ -296,6 +287,7 @@ void pair_run(pfw_ctx* ctx, uint32_t mask, int precision) {
     throw Error(PFW_INVALID_ARGUMENT, "too many species for the pair table");
   nl_build(ctx);
   record_stage(ctx, 3);
+  record_stage(ctx, 4);
   const Grid& g = ctx->grid;
   hipStream_t st = ctx->stream;
   if (mask != 0 && ctx->sys.n > 0) {
@@ -312,16 +304,18 @@ void pair_run(pfw_ctx* ctx, uint32_t mask, int precision) {
         launch_pair<double, PFW_PAIR_LJ>(ctx, mask);
     }
     PFW_HIP(hipGetLastError());
-    record_stage(ctx, 4);
-    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, st, ctx->d_partials, g.ncells, ctx->d_out);
+    record_stage(ctx, 5);
+    hipLaunchKernelGGL(k_colsum, dim3(kRedBlocks), dim3(256), 0, st, ctx->d_partials, (int64_t)g.ncells, 7,
+                       ctx->d_red);
+    hipLaunchKernelGGL(k_reduce_finish, dim3(1), dim3(64), 0, st, ctx->d_red, kRedBlocks, ctx->d_out);
     PFW_HIP(hipGetLastError());
   } else {
-    record_stage(ctx, 4);
+    record_stage(ctx, 5);
     PFW_HIP(hipMemsetAsync(ctx->d_out, 0, 11 * sizeof(double), st));
     if (ctx->sys.n > 0) PFW_HIP(hipMemsetAsync(ctx->d_F, 0, 3 * ctx->sys.n * sizeof(double), st));
   }
   PFW_HIP(hipMemcpyAsync(ctx->h_out, ctx->d_out, 11 * sizeof(double), hipMemcpyDeviceToHost, st));
-  record_stage(ctx, 5);
+  record_stage(ctx, 6);
   PFW_HIP(hipStreamSynchronize(st));
   if (ctx->timing)
     for (int k = 0; k < kStages; ++k) {

@@ -120,4 +120,10 @@ __device__ inline T wave_sum(T v) {
   return v;
 }
 
+// Stage 1 of a deterministic two-stage FP64 column reduction: block b sums
+// columns of records [b*chunk, (b+1)*chunk) of an (m x width) row-major table
+// into out[b*width + c] (width <= 16, blockDim 256).
+constexpr int kRedBlocks = 64;
+__global__ void k_colsum(const double* __restrict__ in, int64_t m, int width, double* __restrict__ out);
+
 }  // namespace pfw

@@ -72,7 +72,7 @@ struct PairModelDev {
   double* d_params = nullptr;  // SMOOTH: nz*nz amplitudes; LJ: eps, sigma
 };
 
-constexpr int kStages = 5;
+constexpr int kStages = 6;  // binning, scan, scatter+sort, environments, interaction, reduction
 
 // One radial family of a resolved ACE model on the device, in both precisions.
 struct AceFamilyDev {
@@ -98,11 +98,14 @@ struct AceDev {
   double* reference = nullptr;
   AceFamilyDev radial, pair;
   int32_t* prod = nullptr;      // nprod x 4, -1 padded
-  int32_t* darow = nullptr;     // ndens + 1
+  int nseg = 0;
+  int32_t* segb = nullptr;      // nseg + 1 entry ranges of dE/dA row segments
+  int32_t* rowseg = nullptr;    // ndens + 1 segment ranges per density
   int32_t* dent = nullptr;      // entries x 4: (product, other factors -1 padded)
   int32_t* dens = nullptr;      // ndens x 2
-  int32_t* dens_of = nullptr;   // nr x ny density index or -1
-  uint64_t* rmask = nullptr;    // nr: harmonics used with each radial index
+  int32_t* rlb = nullptr;       // lmax + 2: radial indices grouped by l
+  int32_t* rl = nullptr;        // (radial index, first density of its m-block)
+  int max_degree = 0;           // largest edge count of the last environment build
   double* w64 = nullptr;        // nz x nprod precontracted readout
   float* w32 = nullptr;
   double* wp64 = nullptr;       // nz x nq
@@ -118,6 +121,7 @@ struct AceDev {
   double* eg = nullptr;       // 3 x E edge gradients
   double* siteE = nullptr;    // n (sorted order)
   double* siteW = nullptr;    // 9 x n
+  double* rec = nullptr;      // 10 x n reduction records
   int32_t* status = nullptr;
 };
 
@@ -138,6 +142,7 @@ struct pfw_ctx {
   double* d_F = nullptr;        // 3 x n original order
   double* d_partials = nullptr;  // per-wave E + 6 W partials
   int64_t partial_cap = 0;
+  double* d_red = nullptr;       // stage-1 reduction partials (64 x 16)
   double* d_out = nullptr;       // E, W[9], status
   double* h_out = nullptr;       // pinned copy of d_out
   uint32_t last_mask = 0;

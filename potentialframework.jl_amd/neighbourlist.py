@@ -17,6 +17,7 @@ from . import _capi
 from ._capi import ArgumentError, DimensionMismatch, Context, as_cell, as_positions, ptr
 from .system import System, validate_geometry
 
+STAGES = ("binning", "scan", "scatter_sort", "environments", "interaction", "reduction")
 PRECISIONS = {"float64": _capi.PFW_FP64, "fp64": _capi.PFW_FP64, "mixed": _capi.PFW_MIXED,
               "float32": _capi.PFW_MIXED, np.float64: _capi.PFW_FP64, np.float32: _capi.PFW_MIXED,
               float: _capi.PFW_FP64}
@@ -88,9 +89,9 @@ class Workspace:
         return i, j, S, R
 
     def read_timings(self):
-        ms = np.zeros(5)
-        _capi.load().pfw_get_timings(self.ctx.h, ptr(ms), 5)
-        self.timings = dict(zip(("binning", "scan", "scatter_sort", "interaction", "reduction"), ms.tolist()))
+        ms = np.zeros(len(STAGES))
+        _capi.load().pfw_get_timings(self.ctx.h, ptr(ms), len(STAGES))
+        self.timings = dict(zip(STAGES, ms.tolist()))
         return self.timings
 
 
