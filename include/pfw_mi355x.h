@@ -63,6 +63,21 @@ const char* pfw_last_error(const pfw_ctx* ctx);
  * with pfw_update_positions without changing the cell. */
 int pfw_set_system(pfw_ctx* ctx, int64_t n, const double* X, const int32_t* species,
                    const double* H, const uint8_t* pbc, double cutoff);
+/* A batch of ncfg independent configurations packed into one cell list
+ * (the fitting-set case; the reference evaluates configurations one by one).
+ * atom_offsets (ncfg + 1, first 0) delimit each configuration's atoms in X /
+ * species; H holds ncfg column-major 3 x 3 cells, pbc 3 bytes per
+ * configuration. Every later step covers all configurations with one launch
+ * per stage; pfw_read_batch_results returns per-configuration outputs.
+ * pfw_set_system is the ncfg = 1 case. No counterpart entry point exists in
+ * the reference; it replaces looping evaluate! over a fitting set
+ * (src/submodules/ACE/fitting.jl:18-63). */
+int pfw_set_batch(pfw_ctx* ctx, int32_t ncfg, const int64_t* atom_offsets, const double* X,
+                  const int32_t* species, const double* H, const uint8_t* pbc, double cutoff);
+/* Per-configuration outputs of the last step: E (ncfg), F (3 x N, original
+ * order), W (9 x ncfg, column-major per configuration); NULL skips. */
+int pfw_read_batch_results(pfw_ctx* ctx, double* E, double* F, double* W);
+
 /* New positions for the same atoms and cell (host buffer). */
 int pfw_update_positions(pfw_ctx* ctx, const double* X);
 

@@ -33,6 +33,7 @@ from .pair import (  # noqa: F401
     EnergyForcesVirial,
     evaluate_,
     evaluate,
+    evaluate_batch,
 )
 from .ace import ACECalculator, ResolvedModel, load_resolved, ace_evaluate  # noqa: F401
 from . import ace, ace_basis  # noqa: F401

@@ -58,6 +58,8 @@ SIGNATURES = [
     ("pfw_ctx_destroy", None, [_P]),
     ("pfw_last_error", ctypes.c_char_p, [_P]),
     ("pfw_set_system", ctypes.c_int, [_P, ctypes.c_int64, _D, _I32, _D, _U8, ctypes.c_double]),
+    ("pfw_set_batch", ctypes.c_int, [_P, ctypes.c_int32, _I64, _D, _I32, _D, _U8, ctypes.c_double]),
+    ("pfw_read_batch_results", ctypes.c_int, [_P, _D, _D, _D]),
     ("pfw_update_positions", ctypes.c_int, [_P, _D]),
     ("pfw_set_pair_model", ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, _D, ctypes.c_double]),
     ("pfw_pair_step", ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_int]),

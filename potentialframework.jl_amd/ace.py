@@ -242,6 +242,21 @@ class ACECalculator:
         return out
 
 
+def _evaluate_batch(self, systems, request=("energy", "forces", "virial")):
+    """All configurations of a fitting set in one packed cell list, one launch
+    per stage; one result dictionary per configuration."""
+    mask, names = request_mask(request)
+    w = self.workspace
+    offsets = w.prepare_batch(systems, self.model.cutoff, [self.model.channels(s.labels) for s in systems])
+    w.ctx.call("pfw_ace_step", mask, w.mode)
+    if w.timing:
+        w.read_timings()
+    return w.read_batch(offsets, names)
+
+
+ACECalculator.evaluate_batch = _evaluate_batch
+
+
 def ace_evaluate(X, model, ps, species, H, pbc, precision="float64", calculator=None):
     """Numeric ``ace_evaluate(X, model, ps, env, H)`` (serial.jl:337-370) on the GPU:
     X (n, 3) Å, species labels, H columns = lattice vectors. Returns E, F (3 x n), W."""

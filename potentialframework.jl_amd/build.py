@@ -11,6 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpfw_mi355x.so")
+WL_LIB = os.path.join(HERE, "libpfw_workloads.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 # Packed FP32 instructions are disabled for all device code: the compiler
@@ -52,6 +53,13 @@ def build_library(verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
+    # host-only workload generator (synthetic fitting sets for tests and the benchmark)
+    tmp = WL_LIB + ".tmp"
+    r = subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-o", tmp,
+                        os.path.join(CSRC, "workloads.cpp")], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"workload generator build failed:\n{r.stderr}")
+    os.replace(tmp, WL_LIB)
     return LIB
 
 

@@ -426,14 +426,15 @@ __global__ __launch_bounds__(64) void k_ace_site(AceArgs<T> a) {
 
 // Directed environments (interface.jl:144-171 + environments.jl:357-391):
 // pass 0 counts per sorted centre, pass 1 writes neighbour and exact R.
-__global__ void k_ace_edges(const Grid* __restrict__ gp, int64_t n, const double* __restrict__ xs,
+__global__ void k_ace_edges(const Grid* __restrict__ grids, const int32_t* __restrict__ atom_cfg, int64_t n,
+                            const double* __restrict__ xs,
                             const int32_t* __restrict__ start, const int32_t* __restrict__ cell_of,
                             const int32_t* __restrict__ perm, int pass, int32_t* __restrict__ ecount,
                             const int32_t* __restrict__ estart, int32_t* __restrict__ enb, double* __restrict__ eR,
                             int32_t* __restrict__ maxdeg) {
-  const Grid& g = *gp;
   int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
+  const Grid& g = grids[atom_cfg[perm[k]]];
   const int cell = cell_of[perm[k]];
   int key[3];
   cell_key(g, cell, key);
@@ -891,7 +892,7 @@ static void dispatch_sites(pfw_ctx* ctx, int grads) {
 void ace_run(pfw_ctx* ctx, uint32_t mask, int precision) {
   AceDev& ad = ctx->ace;
   if (!ad.ready) throw Error(PFW_STALE, "no ACE model set");
-  if (ad.cutoff != ctx->grid.c)
+  if (ad.cutoff != ctx->cutoff)
     throw Error(PFW_STALE, "the system's cell list was built for another cutoff than the ACE search cutoff");
   SystemBuffers& s = ctx->sys;
   hipStream_t st = ctx->stream;
@@ -902,6 +903,7 @@ void ace_run(pfw_ctx* ctx, uint32_t mask, int precision) {
     record_stage(ctx, 4);
     record_stage(ctx, 5);
     PFW_HIP(hipMemsetAsync(ctx->d_out, 0, 11 * sizeof(double), st));
+    PFW_HIP(hipMemsetAsync(ctx->d_cfg_out, 0, 11 * (size_t)ctx->ncfg * sizeof(double), st));
     if (n > 0) PFW_HIP(hipMemsetAsync(ctx->d_F, 0, 3 * n * sizeof(double), st));
   } else {
     if (!ad.ecount) {
@@ -914,8 +916,8 @@ void ace_run(pfw_ctx* ctx, uint32_t mask, int precision) {
       PFW_HIP(hipMalloc(&ad.status, 4 * sizeof(int32_t)));
     }
     PFW_HIP(hipMemsetAsync(ad.status, 0, 4 * sizeof(int32_t), st));
-    hipLaunchKernelGGL(k_ace_edges, dim3(nblk(n, 128)), dim3(128), 0, st, s.grid, n, s.xs, s.start, s.cell_of, s.perm,
-                       0, ad.ecount, (const int32_t*)nullptr, (int32_t*)nullptr, (double*)nullptr, ad.status + 1);
+    hipLaunchKernelGGL(k_ace_edges, dim3(nblk(n, 128)), dim3(128), 0, st, s.grids, s.atom_cfg, n, s.xs, s.start,
+                       s.cell_of, s.perm, 0, ad.ecount, (const int32_t*)nullptr, (int32_t*)nullptr, (double*)nullptr, ad.status + 1);
     PFW_HIP(hipGetLastError());
     scan_counts(ctx, ad.ecount, (int)n, ad.estart, ad.ebs, ad.ecount /* cursor: reused, already consumed */);
     int32_t total = 0, maxdeg = 0;
@@ -936,8 +938,8 @@ void ace_run(pfw_ctx* ctx, uint32_t mask, int precision) {
       PFW_HIP(hipMalloc(&ad.eg, 3 * cap * sizeof(double)));
       ad.edge_cap = cap;
     }
-    hipLaunchKernelGGL(k_ace_edges, dim3(nblk(n, 128)), dim3(128), 0, st, s.grid, n, s.xs, s.start, s.cell_of, s.perm,
-                       1, ad.ecount, (const int32_t*)ad.estart, ad.enb, ad.eR, ad.status + 1);
+    hipLaunchKernelGGL(k_ace_edges, dim3(nblk(n, 128)), dim3(128), 0, st, s.grids, s.atom_cfg, n, s.xs, s.start,
+                       s.cell_of, s.perm, 1, ad.ecount, (const int32_t*)ad.estart, ad.enb, ad.eR, ad.status + 1);
     PFW_HIP(hipGetLastError());
     const int grads = (mask & (PFW_FORCES | PFW_VIRIAL)) ? 1 : 0;
     if (grads)
@@ -953,13 +955,25 @@ void ace_run(pfw_ctx* ctx, uint32_t mask, int precision) {
       hipLaunchKernelGGL(k_ace_force, dim3(nblk(n, 256)), dim3(256), 0, st, n, ad.estart, ad.erev, ad.eg, s.perm,
                          ctx->d_F);
     hipLaunchKernelGGL(k_ace_pack, dim3(nblk(n, 256)), dim3(256), 0, st, n, ad.siteE, ad.siteW, grads, ad.rec);
-    hipLaunchKernelGGL(k_colsum, dim3(kRedBlocks), dim3(256), 0, st, ad.rec, n, 10, ctx->d_red);
-    hipLaunchKernelGGL(k_ace_finish, dim3(1), dim3(64), 0, st, ctx->d_red, kRedBlocks, ad.status, ctx->d_out);
+    if (ctx->ncfg == 1) {
+      hipLaunchKernelGGL(k_colsum, dim3(kRedBlocks), dim3(256), 0, st, ad.rec, n, 10, ctx->d_red);
+      hipLaunchKernelGGL(k_ace_finish, dim3(1), dim3(64), 0, st, ctx->d_red, kRedBlocks, ad.status, ctx->d_out);
+      PFW_HIP(hipMemcpyAsync(ctx->d_cfg_out, ctx->d_out, 11 * sizeof(double), hipMemcpyDeviceToDevice, st));
+    } else {
+      batch_reduce(ctx, ad.rec, 10, false, 1);
+    }
     PFW_HIP(hipGetLastError());
   }
   PFW_HIP(hipMemcpyAsync(ctx->h_out, ctx->d_out, 11 * sizeof(double), hipMemcpyDeviceToHost, st));
   record_stage(ctx, 6);
   PFW_HIP(hipStreamSynchronize(st));
+  if (ctx->ncfg > 1) {
+    ctx->h_cfg_out.resize(11 * (size_t)ctx->ncfg);
+    PFW_HIP(hipMemcpy(ctx->h_cfg_out.data(), ctx->d_cfg_out, ctx->h_cfg_out.size() * sizeof(double),
+                      hipMemcpyDeviceToHost));
+  } else {
+    ctx->h_cfg_out.assign(ctx->h_out, ctx->h_out + 11);
+  }
   if (ctx->timing)
     for (int k = 0; k < kStages; ++k) {
       float ms = 0;

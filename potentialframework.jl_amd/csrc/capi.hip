@@ -72,6 +72,8 @@ void pfw_ctx_destroy(pfw_ctx* ctx) {
   if (ctx->d_partials) (void)hipFree(ctx->d_partials);
   if (ctx->d_out) (void)hipFree(ctx->d_out);
   if (ctx->d_red) (void)hipFree(ctx->d_red);
+  if (ctx->d_cfg) (void)hipFree(ctx->d_cfg);
+  if (ctx->d_cfg_out) (void)hipFree(ctx->d_cfg_out);
   if (ctx->h_out) (void)hipHostFree(ctx->h_out);
   if (ctx->pair.d_params) (void)hipFree(ctx->pair.d_params);
   for (auto& e : ctx->ev)
@@ -82,10 +84,15 @@ void pfw_ctx_destroy(pfw_ctx* ctx) {
 
 const char* pfw_last_error(const pfw_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
-int pfw_set_system(pfw_ctx* ctx, int64_t n, const double* X, const int32_t* species, const double* H,
-                   const uint8_t* pbc, double cutoff) {
+int pfw_set_batch(pfw_ctx* ctx, int32_t ncfg, const int64_t* offsets, const double* X, const int32_t* species,
+                  const double* H, const uint8_t* pbc, double cutoff) {
   return run(ctx, [&] {
-    if (n < 0 || n > ((int64_t)1 << 31) - 1) throw Error(PFW_DIMENSION, "atom count out of range");
+    if (ncfg <= 0 || !offsets) throw Error(PFW_DIMENSION, "a batch needs at least one configuration");
+    if (offsets[0] != 0) throw Error(PFW_DIMENSION, "atom offsets must start at 0");
+    for (int k = 0; k < ncfg; ++k)
+      if (offsets[k + 1] < offsets[k]) throw Error(PFW_DIMENSION, "atom offsets must be nondecreasing");
+    const int64_t n = offsets[ncfg];
+    if (n > ((int64_t)1 << 31) - 1) throw Error(PFW_DIMENSION, "atom count out of range");
     if (!H || !pbc || (n > 0 && !species)) throw Error(PFW_INVALID_ARGUMENT, "null geometry pointer");
     if (!(std::isfinite(cutoff) && cutoff > 0)) throw Error(PFW_INVALID_ARGUMENT, "cutoff must be finite and positive (Å)");
     check_positions(n, X);
@@ -95,9 +102,12 @@ int pfw_set_system(pfw_ctx* ctx, int64_t n, const double* X, const int32_t* spec
     ctx->have_results = false;
     pfw::nl_free(ctx);
     pfw::ace_free_edges(ctx);
-    for (int k = 0; k < 9; ++k) ctx->host_H[k] = H[k];
-    for (int a = 0; a < 3; ++a) ctx->host_pb[a] = pbc[a] ? 1 : 0;
+    ctx->ncfg = ncfg;
+    ctx->atom0.assign(offsets, offsets + ncfg + 1);
     // open cell vectors may be infinite or zero (storage.jl:80-82); periodic ones are checked by the basis
+    ctx->host_H.assign(H, H + 9 * (size_t)ncfg);
+    ctx->host_pb.resize(3 * (size_t)ncfg);
+    for (size_t k = 0; k < ctx->host_pb.size(); ++k) ctx->host_pb[k] = pbc[k] ? 1 : 0;
     ctx->sys.n = n;
     PFW_HIP(hipMalloc(&ctx->sys.X, std::max<int64_t>(3 * n, 1) * sizeof(double)));
     PFW_HIP(hipMalloc(&ctx->sys.species, std::max<int64_t>(n, 1) * sizeof(int32_t)));
@@ -106,15 +116,28 @@ int pfw_set_system(pfw_ctx* ctx, int64_t n, const double* X, const int32_t* spec
       PFW_HIP(hipMemcpyAsync(ctx->sys.species, species, n * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
     }
     pfw::nl_configure(ctx, cutoff);
-    if (ctx->d_F) (void)hipFree(ctx->d_F);
-    ctx->d_F = nullptr;
-    PFW_HIP(hipMalloc(&ctx->d_F, std::max<int64_t>(3 * n, 1) * sizeof(double)));
-    if (ctx->d_partials) (void)hipFree(ctx->d_partials);
-    ctx->d_partials = nullptr;
-    PFW_HIP(hipMalloc(&ctx->d_partials, (size_t)ctx->grid.ncells * 7 * sizeof(double)));
+    auto realloc = [](double*& p, size_t count) {
+      if (p) (void)hipFree(p);
+      p = nullptr;
+      PFW_HIP(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(double)));
+    };
+    realloc(ctx->d_F, 3 * (size_t)n);
+    realloc(ctx->d_partials, (size_t)ctx->ncells * 7);
+    realloc(ctx->d_cfg, (size_t)ncfg * 16);
+    realloc(ctx->d_cfg_out, (size_t)ncfg * 11);
     PFW_HIP(hipStreamSynchronize(ctx->stream));
     ctx->have_system = true;
   });
+}
+
+int pfw_set_system(pfw_ctx* ctx, int64_t n, const double* X, const int32_t* species, const double* H,
+                   const uint8_t* pbc, double cutoff) {
+  if (n < 0 || n > ((int64_t)1 << 31) - 1) {
+    if (ctx) ctx->err = "atom count out of range";
+    return ctx ? PFW_DIMENSION : PFW_INVALID_ARGUMENT;
+  }
+  const int64_t offsets[2] = {0, n};
+  return pfw_set_batch(ctx, 1, offsets, X, species, H, pbc, cutoff);
 }
 
 int pfw_update_positions(pfw_ctx* ctx, const double* X) {
@@ -190,6 +213,27 @@ int pfw_read_results(pfw_ctx* ctx, double* E, double* F, double* W) {
   });
 }
 
+int pfw_read_batch_results(pfw_ctx* ctx, double* E, double* F, double* W) {
+  return run(ctx, [&] {
+    if (!ctx->have_results) throw Error(PFW_STALE, "no results: run a step first");
+    const int ncfg = ctx->ncfg;
+    for (int k = 0; k < ncfg; ++k) {
+      const double* o = ctx->h_cfg_out.data() + 11 * (size_t)k;
+      if (E) E[k] = (ctx->last_mask & PFW_ENERGY) ? o[0] : 0.0;
+      if (W)
+        for (int c = 0; c < 9; ++c) W[9 * (size_t)k + c] = (ctx->last_mask & PFW_VIRIAL) ? o[1 + c] : 0.0;
+    }
+    if (F && ctx->sys.n > 0) {
+      if (ctx->last_mask & PFW_FORCES) {
+        PFW_HIP(hipMemcpyAsync(F, ctx->d_F, 3 * ctx->sys.n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        PFW_HIP(hipStreamSynchronize(ctx->stream));
+      } else {
+        std::memset(F, 0, 3 * ctx->sys.n * sizeof(double));
+      }
+    }
+  });
+}
+
 int pfw_pair_evaluate(pfw_ctx* ctx, uint32_t mask, int precision, double* E, double* F, double* W) {
   int rc = pfw_pair_step(ctx, mask, precision);
   if (rc != PFW_OK) return rc;
@@ -207,10 +251,10 @@ int pfw_export_pairs(pfw_ctx* ctx, int32_t* i, int32_t* j, int32_t* S, double* R
 int pfw_grid_info(pfw_ctx* ctx, int64_t* ncells, int32_t* dims, int32_t* minimum_image) {
   return run(ctx, [&] {
     if (!ctx->have_system) throw Error(PFW_STALE, "no system set");
-    if (ncells) *ncells = ctx->grid.ncells;
+    if (ncells) *ncells = ctx->ncells;
     if (dims)
-      for (int a = 0; a < 3; ++a) dims[a] = ctx->grid.dims[a];
-    if (minimum_image) *minimum_image = ctx->grid.minimum_image;
+      for (int a = 0; a < 3; ++a) dims[a] = ctx->grids[0].dims[a];
+    if (minimum_image) *minimum_image = ctx->grids[0].minimum_image;
   });
 }
 

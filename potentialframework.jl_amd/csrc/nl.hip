@@ -100,19 +100,17 @@ static void dmalloc(T** p, size_t count) {
 
 void nl_free(pfw_ctx* ctx) {
   SystemBuffers& s = ctx->sys;
-  void* ptrs[] = {s.X, s.species, s.xw, s.u, s.cell_of, s.count, s.start, s.cursor, s.blocksum,
-                  s.perm, s.xs, s.sps, s.bounds, s.grid};
+  void* ptrs[] = {s.X,     s.species, s.xw,     s.u,      s.cell_of,  s.count,    s.start,    s.cursor,
+                  s.blocksum, s.perm, s.xs,    s.sps,    s.bounds, s.grids, s.atom_cfg, s.cell_cfg, s.cfg_cell0};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   s = SystemBuffers();
 }
 
-// Grid for the uploaded geometry: mirrors prepare!'s choice of wrap path,
-// basis, reach and dims (storage.jl:130-193), with the cell count capped.
-void nl_configure(pfw_ctx* ctx, double c) {
+// Grid of one configuration: mirrors prepare!'s choice of wrap path, basis,
+// reach and dims (storage.jl:130-193), with the cell count capped by `limit`.
+static Grid configure_one(const double* H, const int* pb, int64_t n, double c, double limit) {
   Grid g{};
-  const double* H = ctx->host_H;
-  const int* pb = ctx->host_pb;
   bool axisaligned = true;
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j)
@@ -156,8 +154,6 @@ void nl_configure(pfw_ctx* ctx, double c) {
     else
       ++nopen;
   }
-  int64_t n = ctx->sys.n;
-  double limit = std::max<double>(2.0 * (double)n, 4096.0);
   // periodic axes: reach-sized cells (storage.jl:192), capped in count
   double dimsd[3] = {1, 1, 1};
   for (int a = 0; a < 3; ++a)
@@ -190,10 +186,40 @@ void nl_configure(pfw_ctx* ctx, double c) {
   int64_t nc = (int64_t)g.dims[0] * g.dims[1] * g.dims[2];
   if (nc > (int64_t)1 << 30) throw Error(PFW_INVALID_ARGUMENT, "cell grid too large");
   g.ncells = (int)nc;
+  g.natoms = (int)n;
   g.nstencil = (2 * g.radius[0] + 1) * (2 * g.radius[1] + 1) * (2 * g.radius[2] + 1);
   if (g.nstencil > 4096)
     throw Error(PFW_INVALID_ARGUMENT, "cutoff spans too many periodic images for the cell stencil");
-  ctx->grid = g;
+  return g;
+}
+
+// Grids of every configuration of the uploaded batch, packed one after the
+// other in a single global cell index space.
+void nl_configure(pfw_ctx* ctx, double c) {
+  const int ncfg = ctx->ncfg;
+  const int64_t n = ctx->sys.n;
+  ctx->grids.resize(ncfg);
+  int64_t total = 0;
+  bool open_any = false;
+  std::vector<int32_t> cell0(ncfg + 1, 0), atom_cfg(n);
+  for (int k = 0; k < ncfg; ++k) {
+    const int64_t nk = ctx->atom0[k + 1] - ctx->atom0[k];
+    const double limit = ncfg == 1 ? std::max<double>(2.0 * (double)nk, 4096.0) : std::max<double>(2.0 * (double)nk, 64.0);
+    Grid g = configure_one(&ctx->host_H[9 * k], &ctx->host_pb[3 * k], nk, c, limit);
+    g.cell0 = (int)total;
+    total += g.ncells;
+    if (total > ((int64_t)1 << 31) - 2) throw Error(PFW_INVALID_ARGUMENT, "cell grid too large");
+    cell0[k + 1] = (int32_t)total;
+    open_any |= g.open_any != 0;
+    ctx->grids[k] = g;
+    for (int64_t i = ctx->atom0[k]; i < ctx->atom0[k + 1]; ++i) atom_cfg[i] = k;
+  }
+  ctx->ncells = (int)total;
+  ctx->open_any = open_any;
+  ctx->cutoff = c;
+  std::vector<int32_t> cell_cfg(total);
+  for (int k = 0; k < ncfg; ++k)
+    for (int32_t q = cell0[k]; q < cell0[k + 1]; ++q) cell_cfg[q] = k;
 
   SystemBuffers& s = ctx->sys;
   dmalloc(&s.xw, 3 * n);
@@ -202,13 +228,22 @@ void nl_configure(pfw_ctx* ctx, double c) {
   dmalloc(&s.perm, n);
   dmalloc(&s.xs, 3 * n);
   dmalloc(&s.sps, n);
-  dmalloc(&s.count, g.ncells);
-  dmalloc(&s.start, g.ncells + 1);
-  dmalloc(&s.cursor, g.ncells);
-  dmalloc(&s.blocksum, (g.ncells + 1023) / 1024 + 1);
-  dmalloc(&s.bounds, 6);
-  dmalloc(&s.grid, 1);
-  PFW_HIP(hipMemcpyAsync(s.grid, &ctx->grid, sizeof(Grid), hipMemcpyHostToDevice, ctx->stream));
+  dmalloc(&s.count, total);
+  dmalloc(&s.start, total + 1);
+  dmalloc(&s.cursor, total);
+  dmalloc(&s.blocksum, (total + 1023) / 1024 + 1);
+  dmalloc(&s.bounds, 6 * (size_t)ncfg);
+  dmalloc(&s.grids, ncfg);
+  dmalloc(&s.atom_cfg, n);
+  dmalloc(&s.cell_cfg, total);
+  dmalloc(&s.cfg_cell0, ncfg + 1);
+  hipStream_t st = ctx->stream;
+  PFW_HIP(hipMemcpyAsync(s.grids, ctx->grids.data(), ncfg * sizeof(Grid), hipMemcpyHostToDevice, st));
+  if (n > 0) PFW_HIP(hipMemcpyAsync(s.atom_cfg, atom_cfg.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  if (total > 0)
+    PFW_HIP(hipMemcpyAsync(s.cell_cfg, cell_cfg.data(), total * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  PFW_HIP(hipMemcpyAsync(s.cfg_cell0, cell0.data(), (ncfg + 1) * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  PFW_HIP(hipStreamSynchronize(st));  // host staging vectors go out of scope
 }
 
 // ---------------------------------------------------------------- kernels
@@ -241,75 +276,68 @@ __device__ inline int bin_atom(const Grid& g, const double* u) {
   return cell_index(g, k);
 }
 
-// Wrap every atom; on fully periodic grids also bin and count in the same pass.
-__global__ void k_wrap(const Grid* __restrict__ gp, int64_t n, const double* __restrict__ X,
-                       double* __restrict__ xw, double* __restrict__ u, int32_t* __restrict__ cell_of,
-                       int32_t* __restrict__ count, unsigned long long* __restrict__ bounds) {
-  const Grid& g = *gp;
+// Wrap every atom with its configuration's basis; when no configuration has
+// an open axis also bin and count in the same pass.
+__global__ void k_wrap(const Grid* __restrict__ grids, const int32_t* __restrict__ atom_cfg, int64_t n,
+                       const double* __restrict__ X, double* __restrict__ xw, double* __restrict__ u,
+                       int32_t* __restrict__ cell_of, int32_t* __restrict__ count,
+                       unsigned long long* __restrict__ bounds, int open_any) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-  if (i < n) {
-    double x[3] = {X[3 * i], X[3 * i + 1], X[3 * i + 2]}, w[3], uu[3];
-    wrap_atom(g, x, w, uu);
-    for (int a = 0; a < 3; ++a) {
-      xw[a * n + i] = w[a];
-      u[a * n + i] = uu[a];
-    }
-    if (!g.open_any) {
-      int c = bin_atom(g, uu);
-      cell_of[i] = c;
-      atomicAdd(&count[c], 1);
-    } else {
-      for (int a = 0; a < 3; ++a) lo[a] = hi[a] = uu[a];
-    }
+  if (i >= n) return;
+  const int cfg = atom_cfg[i];
+  const Grid& g = grids[cfg];
+  double x[3] = {X[3 * i], X[3 * i + 1], X[3 * i + 2]}, w[3], uu[3];
+  wrap_atom(g, x, w, uu);
+  for (int a = 0; a < 3; ++a) {
+    xw[a * n + i] = w[a];
+    u[a * n + i] = uu[a];
   }
-  if (g.open_any) {
+  if (!open_any) {
+    const int c = bin_atom(g, uu);
+    cell_of[i] = c;
+    atomicAdd(&count[c], 1);
+  } else if (g.open_any) {
     for (int a = 0; a < 3; ++a) {
       if (g.pb[a]) continue;
-      double l = lo[a], h = hi[a];
-      for (int off = kWave / 2; off > 0; off >>= 1) {
-        l = fmin(l, __shfl_xor(l, off, kWave));
-        h = fmax(h, __shfl_xor(h, off, kWave));
-      }
-      if ((threadIdx.x % kWave) == 0 && isfinite(l)) {
-        atomicMin(&bounds[2 * a], ordered_bits(l));
-        atomicMax(&bounds[2 * a + 1], ordered_bits(h));
-      }
+      atomicMin(&bounds[6 * cfg + 2 * a], ordered_bits(uu[a]));
+      atomicMax(&bounds[6 * cfg + 2 * a + 1], ordered_bits(uu[a]));
     }
   }
 }
 
-// Open axes: grid origin and cell size from the atoms' extent (sparse-vacuum
-// handling of storage.jl:187-197 with a bounded dense grid).
-__global__ void k_open_grid(Grid* __restrict__ gp, const unsigned long long* __restrict__ bounds, int64_t n) {
-  Grid& g = *gp;
+// Open axes: grid origin and cell size of every configuration from its atoms'
+// extent (sparse-vacuum handling of storage.jl:187-197 with a bounded dense grid).
+__global__ void k_open_grid(Grid* __restrict__ grids, const unsigned long long* __restrict__ bounds, int ncfg) {
+  const int cfg = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cfg >= ncfg) return;
+  Grid& g = grids[cfg];
   for (int a = 0; a < 3; ++a) {
     if (g.pb[a]) continue;
-    if (n == 0) {
+    if (g.natoms == 0) {
       g.origin[a] = 0;
       g.sizecell[a] = g.reach[a];
       g.used[a] = 1;
       continue;
     }
-    double lo = from_ordered_bits(bounds[2 * a]), hi = from_ordered_bits(bounds[2 * a + 1]);
-    double span = hi - lo;
+    const double lo = from_ordered_bits(bounds[6 * cfg + 2 * a]), hi = from_ordered_bits(bounds[6 * cfg + 2 * a + 1]);
+    const double span = hi - lo;
     double sc = g.reach[a];
-    double nominal = floor(span / sc) + 1.0;
+    const double nominal = floor(span / sc) + 1.0;
     if (nominal > (double)g.dims[a]) sc = fmax(sc, span / (double)g.dims[a] * (1.0 + 1e-12));
-    int used = (int)fmin((double)g.dims[a], floor(span / sc) + 1.0);
+    const int used = (int)fmin((double)g.dims[a], floor(span / sc) + 1.0);
     g.origin[a] = lo;
     g.sizecell[a] = sc;
     g.used[a] = used < 1 ? 1 : used;
   }
 }
 
-__global__ void k_bin_open(const Grid* __restrict__ gp, int64_t n, const double* __restrict__ u,
-                           int32_t* __restrict__ cell_of, int32_t* __restrict__ count) {
-  const Grid& g = *gp;
+__global__ void k_bin_open(const Grid* __restrict__ grids, const int32_t* __restrict__ atom_cfg, int64_t n,
+                           const double* __restrict__ u, int32_t* __restrict__ cell_of, int32_t* __restrict__ count) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const Grid& g = grids[atom_cfg[i]];
   double uu[3] = {u[i], u[n + i], u[2 * n + i]};
-  int c = bin_atom(g, uu);
+  const int c = bin_atom(g, uu);
   cell_of[i] = c;
   atomicAdd(&count[c], 1);
 }
@@ -480,41 +508,117 @@ void scan_counts(pfw_ctx* ctx, const int32_t* count, int m, int32_t* start, int3
 
 void nl_build(pfw_ctx* ctx) {
   SystemBuffers& s = ctx->sys;
-  const Grid& g = ctx->grid;
   hipStream_t st = ctx->stream;
-  int64_t n = s.n;
+  const int64_t n = s.n;
+  const int ncells = ctx->ncells;
   record_stage(ctx, 0);
-  PFW_HIP(hipMemsetAsync(s.count, 0, sizeof(int32_t) * g.ncells, st));
-  if (g.open_any) {
-    unsigned long long init[6];
-    for (int a = 0; a < 3; ++a) {
-      init[2 * a] = ~0ull;
-      init[2 * a + 1] = 0ull;
-    }
-    PFW_HIP(hipMemcpyAsync(s.bounds, init, sizeof(init), hipMemcpyHostToDevice, st));
+  PFW_HIP(hipMemsetAsync(s.count, 0, sizeof(int32_t) * std::max(ncells, 1), st));
+  if (ctx->open_any) {
+    std::vector<unsigned long long> init(6 * (size_t)ctx->ncfg);
+    for (size_t k = 0; k < init.size(); ++k) init[k] = (k % 2 == 0) ? ~0ull : 0ull;
+    PFW_HIP(hipMemcpyAsync(s.bounds, init.data(), init.size() * sizeof(unsigned long long), hipMemcpyHostToDevice, st));
   }
   if (n > 0)
-    hipLaunchKernelGGL(k_wrap, dim3(nblocks(n, 256)), dim3(256), 0, st, s.grid, n, s.X, s.xw, s.u, s.cell_of,
-                       s.count, (unsigned long long*)s.bounds);
-  if (g.open_any) {
-    hipLaunchKernelGGL(k_open_grid, dim3(1), dim3(1), 0, st, s.grid, (const unsigned long long*)s.bounds, n);
+    hipLaunchKernelGGL(k_wrap, dim3(nblocks(n, 256)), dim3(256), 0, st, s.grids, s.atom_cfg, n, s.X, s.xw, s.u,
+                       s.cell_of, s.count, (unsigned long long*)s.bounds, (int)ctx->open_any);
+  if (ctx->open_any) {
+    hipLaunchKernelGGL(k_open_grid, dim3(nblocks(ctx->ncfg, 64)), dim3(64), 0, st, s.grids,
+                       (const unsigned long long*)s.bounds, ctx->ncfg);
     if (n > 0)
-      hipLaunchKernelGGL(k_bin_open, dim3(nblocks(n, 256)), dim3(256), 0, st, s.grid, n, s.u, s.cell_of, s.count);
+      hipLaunchKernelGGL(k_bin_open, dim3(nblocks(n, 256)), dim3(256), 0, st, s.grids, s.atom_cfg, n, s.u,
+                         s.cell_of, s.count);
+    PFW_HIP(hipStreamSynchronize(st));  // the bounds staging vector goes out of scope
   }
   PFW_HIP(hipGetLastError());
   record_stage(ctx, 1);
-  int nb = (int)nblocks(g.ncells, kScanBlock);
-  hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(kScanThreads), 0, st, s.count, g.ncells, s.blocksum);
+  int nb = (int)nblocks(ncells, kScanBlock);
+  hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(kScanThreads), 0, st, s.count, ncells, s.blocksum);
   hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, st, s.blocksum, nb);
-  hipLaunchKernelGGL(k_scan_final, dim3(nb), dim3(kScanThreads), 0, st, s.count, g.ncells, s.blocksum, s.start,
+  hipLaunchKernelGGL(k_scan_final, dim3(nb), dim3(kScanThreads), 0, st, s.count, ncells, s.blocksum, s.start,
                      s.cursor, n);
   PFW_HIP(hipGetLastError());
   record_stage(ctx, 2);
   if (n > 0) {
     hipLaunchKernelGGL(k_scatter, dim3(nblocks(n, 256)), dim3(256), 0, st, n, s.cell_of, s.start, s.cursor, s.perm);
-    hipLaunchKernelGGL(k_cell_sort_gather, dim3(g.ncells), dim3(kWave), 0, st, g.ncells, n, s.start, s.perm, s.xw,
+    hipLaunchKernelGGL(k_cell_sort_gather, dim3(ncells), dim3(kWave), 0, st, ncells, n, s.start, s.perm, s.xw,
                        s.species, s.xs, s.sps);
   }
+  PFW_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------- per-configuration sums
+
+// Column sums of each configuration's records: rows [lo, hi) with lo/hi the
+// configuration's global cell offsets, or (atoms != nullptr) the sorted-atom
+// ranges start[cell offsets].
+__global__ void k_segsum(const double* __restrict__ in, int width, const int32_t* __restrict__ cfg_cell0,
+                         const int32_t* __restrict__ start, double* __restrict__ out) {
+  __shared__ double lds[16][4];
+  const int cfg = blockIdx.x;
+  int lo = cfg_cell0[cfg], hi = cfg_cell0[cfg + 1];
+  if (start) {
+    lo = start[lo];
+    hi = start[hi];
+  }
+  double s[16];
+  for (int c = 0; c < width; ++c) s[c] = 0;
+  for (int r = lo + threadIdx.x; r < hi; r += blockDim.x)
+    for (int c = 0; c < width; ++c) s[c] += in[(int64_t)r * width + c];
+  const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+  for (int c = 0; c < width; ++c) {
+    const double v = wave_sum(s[c]);
+    if (lane == 0) lds[c][wid] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < width) {
+    const int c = threadIdx.x;
+    out[16 * (int64_t)cfg + c] = lds[c][0] + lds[c][1] + lds[c][2] + lds[c][3];
+  }
+}
+
+// Per-configuration E, W (3x3 column-major) and status from the column sums.
+// kind 0 (pair): columns E, xx yy zz xy xz yz of d (x) g over directed pairs
+// (E and W carry the 1/2 of the full list); kind 1 (ACE): E, W[9].
+__global__ void k_batch_finish(const double* __restrict__ sums, int ncfg, int kind, const int32_t* __restrict__ status,
+                               double* __restrict__ out) {
+  const int cfg = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cfg >= ncfg) return;
+  const double* t = sums + 16 * (int64_t)cfg;
+  double* o = out + 11 * (int64_t)cfg;
+  if (kind == 0) {
+    o[0] = 0.5 * t[0];
+    o[1] = -0.5 * t[1];
+    o[5] = -0.5 * t[2];
+    o[9] = -0.5 * t[3];
+    o[4] = o[2] = -0.5 * t[4];
+    o[7] = o[3] = -0.5 * t[5];
+    o[8] = o[6] = -0.5 * t[6];
+  } else {
+    for (int c = 0; c < 10; ++c) o[c] = t[c];
+  }
+  bool bad = status && status[0] != 0;
+  for (int c = 0; c < 10; ++c) bad |= !isfinite(o[c]);
+  o[10] = bad ? 1.0 : 0.0;
+}
+
+__global__ void k_total(const double* __restrict__ part, int nb, double* __restrict__ out) {
+  if (threadIdx.x >= 11) return;
+  double t = 0;
+  for (int b = 0; b < nb; ++b) t += part[11 * b + threadIdx.x];
+  out[threadIdx.x] = t;
+}
+
+// records: (cells ? ncells : n) x width; kind 0 pair, 1 ACE. Writes per-configuration
+// results to d_cfg_out and the batch totals (status = number of failed configurations) to d_out.
+void batch_reduce(pfw_ctx* ctx, const double* records, int width, bool cells, int kind) {
+  hipStream_t st = ctx->stream;
+  const int ncfg = ctx->ncfg;
+  hipLaunchKernelGGL(k_segsum, dim3(ncfg), dim3(256), 0, st, records, width, ctx->sys.cfg_cell0,
+                     cells ? (const int32_t*)nullptr : (const int32_t*)ctx->sys.start, ctx->d_cfg);
+  hipLaunchKernelGGL(k_batch_finish, dim3(nblocks(ncfg, 64)), dim3(64), 0, st, ctx->d_cfg, ncfg, kind,
+                     kind == 1 ? (const int32_t*)ctx->ace.status : (const int32_t*)nullptr, ctx->d_cfg_out);
+  hipLaunchKernelGGL(k_colsum, dim3(kRedBlocks), dim3(256), 0, st, ctx->d_cfg_out, (int64_t)ncfg, 11, ctx->d_red);
+  hipLaunchKernelGGL(k_total, dim3(1), dim3(64), 0, st, ctx->d_red, kRedBlocks, ctx->d_out);
   PFW_HIP(hipGetLastError());
 }
 
@@ -533,16 +637,17 @@ __device__ inline void winding(const Grid& g, const double* X, int i, int* w) {
 
 // One thread per sorted atom: walk the image stencil and emit every directed
 // pair strictly inside the cutoff (interface.jl:144-171). Pass 0 counts, pass 1 writes.
-__global__ void k_export(const Grid* __restrict__ gp, int64_t n, const double* __restrict__ X,
+__global__ void k_export(const Grid* __restrict__ grids, const int32_t* __restrict__ atom_cfg, int64_t n,
+                         const double* __restrict__ X,
                          const double* __restrict__ xs, const int32_t* __restrict__ start,
                          const int32_t* __restrict__ cell_of, const int32_t* __restrict__ perm,
                          int32_t* __restrict__ counts, const int64_t* __restrict__ offsets, int pass,
                          int32_t* __restrict__ oi, int32_t* __restrict__ oj, int32_t* __restrict__ oS,
                          double* __restrict__ oR) {
-  const Grid& g = *gp;
   int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   int i = perm[k];
+  const Grid& g = grids[atom_cfg[i]];
   int cell = cell_of[i];
   int key[3];
   cell_key(g, cell, key);
@@ -595,8 +700,8 @@ void nl_export_pairs(pfw_ctx* ctx, int32_t* hi, int32_t* hj, int32_t* hS, double
   int64_t* d_off = nullptr;
   PFW_HIP(hipMalloc(&d_counts, n * sizeof(int32_t)));
   PFW_HIP(hipMalloc(&d_off, n * sizeof(int64_t)));
-  hipLaunchKernelGGL(k_export, dim3(nblocks(n, 128)), dim3(128), 0, st, s.grid, n, s.X, s.xs, s.start, s.cell_of,
-                     s.perm, d_counts, (const int64_t*)nullptr, 0, nullptr, nullptr, nullptr, nullptr);
+  hipLaunchKernelGGL(k_export, dim3(nblocks(n, 128)), dim3(128), 0, st, s.grids, s.atom_cfg, n, s.X, s.xs, s.start,
+                     s.cell_of, s.perm, d_counts, (const int64_t*)nullptr, 0, nullptr, nullptr, nullptr, nullptr);
   PFW_HIP(hipGetLastError());
   std::vector<int32_t> counts(n);
   PFW_HIP(hipMemcpyAsync(counts.data(), d_counts, n * sizeof(int32_t), hipMemcpyDeviceToHost, st));
@@ -620,8 +725,8 @@ void nl_export_pairs(pfw_ctx* ctx, int32_t* hi, int32_t* hj, int32_t* hS, double
   PFW_HIP(hipMalloc(&dj, std::max<int64_t>(total, 1) * sizeof(int32_t)));
   PFW_HIP(hipMalloc(&dS, std::max<int64_t>(total, 1) * 3 * sizeof(int32_t)));
   PFW_HIP(hipMalloc(&dR, std::max<int64_t>(total, 1) * 3 * sizeof(double)));
-  hipLaunchKernelGGL(k_export, dim3(nblocks(n, 128)), dim3(128), 0, st, s.grid, n, s.X, s.xs, s.start, s.cell_of,
-                     s.perm, d_counts, (const int64_t*)d_off, 1, di, dj, dS, dR);
+  hipLaunchKernelGGL(k_export, dim3(nblocks(n, 128)), dim3(128), 0, st, s.grids, s.atom_cfg, n, s.X, s.xs, s.start,
+                     s.cell_of, s.perm, d_counts, (const int64_t*)d_off, 1, di, dj, dS, dR);
   PFW_HIP(hipGetLastError());
   if (total > 0) {
     if (hi) PFW_HIP(hipMemcpyAsync(hi, di, total * sizeof(int32_t), hipMemcpyDeviceToHost, st));

@@ -48,7 +48,8 @@ __device__ inline int xcd_remap(int b, int nb) {
 }
 
 template <typename T, int KIND>
-__global__ __launch_bounds__(64) void k_pair(const Grid* __restrict__ gp, int64_t n, const int32_t* __restrict__ start,
+__global__ __launch_bounds__(64) void k_pair(const Grid* __restrict__ grids, const int32_t* __restrict__ cell_cfg,
+                                             int64_t n, const int32_t* __restrict__ start,
                                              const double* __restrict__ xs, const int32_t* __restrict__ sps,
                                              const int32_t* __restrict__ perm, const double* __restrict__ params,
                                              int nz, LJConsts lj, uint32_t mask, double* __restrict__ F,
@@ -61,9 +62,9 @@ __global__ __launch_bounds__(64) void k_pair(const Grid* __restrict__ gp, int64_
   __shared__ double red[kWave][3];
   __shared__ int e_zero;
 
-  const Grid& g = *gp;
   const int lane = threadIdx.x;
   const int cell = xcd_remap(blockIdx.x, gridDim.x);
+  const Grid& g = grids[cell_cfg[cell]];
   const T c2 = (T)g.c2;
   const T c2lo = (T)(g.c2 * (1.0 - Band<T>::rel)), c2hi = (T)(g.c2 * (1.0 + Band<T>::rel));
   const T invc2 = (T)(1.0 / g.c2);
@@ -265,7 +266,6 @@ void record_stage(pfw_ctx* ctx, int k) {
 template <typename T, int KIND>
 static void launch_pair(pfw_ctx* ctx, uint32_t mask) {
   SystemBuffers& s = ctx->sys;
-  const Grid& g = ctx->grid;
   LJConsts lj{0, 0, 0, 0};
   if (KIND == PFW_PAIR_LJ) {
     double eps = ctx->pair.params[0], sig = ctx->pair.params[1], c = ctx->pair.cutoff;
@@ -275,20 +275,20 @@ static void launch_pair(pfw_ctx* ctx, uint32_t mask) {
     lj.sig2 = sig * sig;
     lj.eshift = 4.0 * eps * (qc * qc - qc);
   }
-  hipLaunchKernelGGL((k_pair<T, KIND>), dim3(g.ncells), dim3(kWave), 0, ctx->stream, s.grid, s.n, s.start, s.xs,
-                     s.sps, s.perm, ctx->pair.d_params, ctx->pair.nz, lj, mask, ctx->d_F, ctx->d_partials, g.ncells);
+  hipLaunchKernelGGL((k_pair<T, KIND>), dim3(ctx->ncells), dim3(kWave), 0, ctx->stream, s.grids, s.cell_cfg, s.n,
+                     s.start, s.xs, s.sps, s.perm, ctx->pair.d_params, ctx->pair.nz, lj, mask, ctx->d_F,
+                     ctx->d_partials, ctx->ncells);
 }
 
 void pair_run(pfw_ctx* ctx, uint32_t mask, int precision) {
   if (ctx->pair.kind < 0) throw Error(PFW_STALE, "no pair model set");
-  if (ctx->pair.cutoff != ctx->grid.c)
+  if (ctx->pair.cutoff != ctx->cutoff)
     throw Error(PFW_STALE, "pair model cutoff differs from the system's cell-list cutoff; set the system again");
   if (ctx->pair.kind == PFW_PAIR_SMOOTH && ctx->pair.nz > kMaxZ)
     throw Error(PFW_INVALID_ARGUMENT, "too many species for the pair table");
   nl_build(ctx);
   record_stage(ctx, 3);
   record_stage(ctx, 4);
-  const Grid& g = ctx->grid;
   hipStream_t st = ctx->stream;
   if (mask != 0 && ctx->sys.n > 0) {
     bool fp32 = precision == PFW_MIXED;
@@ -305,18 +305,31 @@ void pair_run(pfw_ctx* ctx, uint32_t mask, int precision) {
     }
     PFW_HIP(hipGetLastError());
     record_stage(ctx, 5);
-    hipLaunchKernelGGL(k_colsum, dim3(kRedBlocks), dim3(256), 0, st, ctx->d_partials, (int64_t)g.ncells, 7,
-                       ctx->d_red);
-    hipLaunchKernelGGL(k_reduce_finish, dim3(1), dim3(64), 0, st, ctx->d_red, kRedBlocks, ctx->d_out);
+    if (ctx->ncfg == 1) {
+      hipLaunchKernelGGL(k_colsum, dim3(kRedBlocks), dim3(256), 0, st, ctx->d_partials, (int64_t)ctx->ncells, 7,
+                         ctx->d_red);
+      hipLaunchKernelGGL(k_reduce_finish, dim3(1), dim3(64), 0, st, ctx->d_red, kRedBlocks, ctx->d_out);
+      PFW_HIP(hipMemcpyAsync(ctx->d_cfg_out, ctx->d_out, 11 * sizeof(double), hipMemcpyDeviceToDevice, st));
+    } else {
+      batch_reduce(ctx, ctx->d_partials, 7, true, 0);
+    }
     PFW_HIP(hipGetLastError());
   } else {
     record_stage(ctx, 5);
     PFW_HIP(hipMemsetAsync(ctx->d_out, 0, 11 * sizeof(double), st));
+    PFW_HIP(hipMemsetAsync(ctx->d_cfg_out, 0, 11 * (size_t)ctx->ncfg * sizeof(double), st));
     if (ctx->sys.n > 0) PFW_HIP(hipMemsetAsync(ctx->d_F, 0, 3 * ctx->sys.n * sizeof(double), st));
   }
   PFW_HIP(hipMemcpyAsync(ctx->h_out, ctx->d_out, 11 * sizeof(double), hipMemcpyDeviceToHost, st));
   record_stage(ctx, 6);
   PFW_HIP(hipStreamSynchronize(st));
+  if (ctx->ncfg > 1) {
+    ctx->h_cfg_out.resize(11 * (size_t)ctx->ncfg);
+    PFW_HIP(hipMemcpy(ctx->h_cfg_out.data(), ctx->d_cfg_out, ctx->h_cfg_out.size() * sizeof(double),
+                      hipMemcpyDeviceToHost));
+  } else {
+    ctx->h_cfg_out.assign(ctx->h_out, ctx->h_out + 11);
+  }
   if (ctx->timing)
     for (int k = 0; k < kStages; ++k) {
       float ms = 0;

@@ -71,17 +71,19 @@ __device__ inline bool exact_member(const Grid& g, double xi, double yi, double 
   return d[0] * d[0] + d[1] * d[1] + d[2] * d[2] < g.c2;
 }
 
+// Global cell index <-> key of the configuration's grid.
 __device__ inline void cell_key(const Grid& g, int cell, int* k) {
-  k[0] = cell % g.dims[0];
-  int t = cell / g.dims[0];
+  const int local = cell - g.cell0;
+  k[0] = local % g.dims[0];
+  const int t = local / g.dims[0];
   k[1] = t % g.dims[1];
   k[2] = t / g.dims[1];
 }
 __device__ inline int cell_index(const Grid& g, const int* k) {
-  return k[0] + g.dims[0] * (k[1] + g.dims[1] * k[2]);
+  return g.cell0 + k[0] + g.dims[0] * (k[1] + g.dims[1] * k[2]);
 }
 
-// Stencil entry e of the cell with key k: neighbour cell (or -1 when it lies
+// Stencil entry e of the cell with key k: global neighbour cell (or -1 when it lies
 // outside an open axis), integer lattice shift and whether it is the zero
 // offset. Periodic axes wrap with shift fld(k+o, dims) (storage.jl:265-283).
 __device__ inline int stencil_entry(const Grid& g, const int* k, int e, int* lat, bool* zero) {

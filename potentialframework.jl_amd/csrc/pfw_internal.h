@@ -27,12 +27,14 @@ struct Error : std::runtime_error {
 // for open (non-periodic) axes origin/sizecell/used dims are recomputed on the
 // device every step from the atoms' extent.
 struct Grid {
+  int cell0;          // first global cell of this configuration
+  int ncells;         // cells of this configuration (product of layout dims)
+  int natoms;         // atoms of this configuration
   int minimum_image;  // storage.jl:140
   int pb[3];
   int open_any;
   int dims[3];     // layout dims (open axes: capacity)
   int radius[3];   // stencil half-width per axis
-  int ncells;      // product of layout dims
   int nstencil;    // (2r+1) product
   double basis[9];  // column-major periodic basis (storage.jl:83-108), identity if minimum image
   double recip[9];  // its inverse
@@ -60,8 +62,11 @@ struct SystemBuffers {
   int32_t* perm = nullptr;     // sorted -> original
   double* xs = nullptr;        // sorted wrapped coordinates SoA [3][n]
   int32_t* sps = nullptr;      // sorted species
-  uint64_t* bounds = nullptr;  // 6 ordered-int min/max for open axes
-  Grid* grid = nullptr;        // device copy
+  uint64_t* bounds = nullptr;  // 6 ordered-int min/max of open axes per configuration
+  Grid* grids = nullptr;       // device copy, one per configuration
+  int32_t* atom_cfg = nullptr;  // n: configuration of each atom (original order)
+  int32_t* cell_cfg = nullptr;  // ncells: configuration of each global cell
+  int32_t* cfg_cell0 = nullptr;  // ncfg + 1: global cell offsets per configuration
 };
 
 struct PairModelDev {
@@ -131,11 +136,17 @@ struct pfw_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   std::string err;
-  pfw::Grid grid{};
+  // geometry: a batch of ncfg independent configurations (1 for a single system)
+  int ncfg = 0;
+  int ncells = 0;        // all configurations
+  bool open_any = false;
+  double cutoff = 0;
+  std::vector<pfw::Grid> grids;       // host copies
+  std::vector<int64_t> atom0;         // ncfg + 1 atom offsets (original order)
+  std::vector<double> host_H;         // 9 per configuration
+  std::vector<int> host_pb;           // 3 per configuration
   pfw::SystemBuffers sys;
   bool have_system = false;
-  double host_H[9];
-  int host_pb[3];
   pfw::PairModelDev pair;
   pfw::AceDev ace;
   // outputs
@@ -143,6 +154,9 @@ struct pfw_ctx {
   double* d_partials = nullptr;  // per-wave E + 6 W partials
   int64_t partial_cap = 0;
   double* d_red = nullptr;       // stage-1 reduction partials (64 x 16)
+  double* d_cfg = nullptr;       // per-configuration sums (ncfg x 16)
+  double* d_cfg_out = nullptr;   // per-configuration E, W[9], status (ncfg x 11)
+  std::vector<double> h_cfg_out;
   double* d_out = nullptr;       // E, W[9], status
   double* h_out = nullptr;       // pinned copy of d_out
   uint32_t last_mask = 0;
@@ -156,6 +170,7 @@ struct pfw_ctx {
 namespace pfw {
 // nl.hip
 void nl_configure(pfw_ctx* ctx, double cutoff);
+void batch_reduce(pfw_ctx* ctx, const double* records, int width, bool cells, int kind);
 void nl_build(pfw_ctx* ctx);
 void nl_export_pairs(pfw_ctx* ctx, int32_t* i, int32_t* j, int32_t* S, double* R, int64_t cap,
                      int64_t* count);

@@ -65,6 +65,42 @@ class Workspace:
         self._key = key
         self._model_key = None
 
+    def prepare_batch(self, systems, cutoff, channels_list):
+        """Upload a batch of independent configurations packed into one cell list."""
+        if not systems:
+            raise DimensionMismatch("a batch needs at least one configuration")
+        for sys in systems:
+            validate_geometry(sys)
+        offsets = np.zeros(len(systems) + 1, np.int64)
+        offsets[1:] = np.cumsum([len(sys) for sys in systems])
+        X = np.ascontiguousarray(np.concatenate([as_positions(sys.positions) for sys in systems]))
+        sp = np.ascontiguousarray(np.concatenate([np.asarray(c, np.int32) for c in channels_list]))
+        H = np.ascontiguousarray(np.concatenate([as_cell(sys.cell).reshape(9) for sys in systems]))
+        pb = np.ascontiguousarray(np.array([[1 if p else 0 for p in sys.pbc] for sys in systems], np.uint8).reshape(-1))
+        self.ctx.call("pfw_set_batch", ctypes.c_int32(len(systems)), ptr(offsets, ctypes.c_int64), ptr(X),
+                      ptr(sp, ctypes.c_int32), ptr(H), ptr(pb, ctypes.c_uint8), ctypes.c_double(float(cutoff)))
+        self._key = None
+        self._model_key = None
+        return offsets
+
+    def read_batch(self, offsets, names):
+        ncfg = len(offsets) - 1
+        E = np.zeros(ncfg)
+        F = np.zeros((int(offsets[-1]), 3))
+        W = np.zeros((ncfg, 9))
+        self.ctx.call("pfw_read_batch_results", ptr(E), ptr(F), ptr(W))
+        out = []
+        for c in range(ncfg):
+            r = {}
+            if "energy" in names:
+                r["energy"] = float(E[c])
+            if "forces" in names:
+                r["forces"] = F[offsets[c]:offsets[c + 1]].T.copy()
+            if "virial" in names:
+                r["virial"] = W[c].reshape(3, 3).T.copy()
+            out.append(r)
+        return out
+
     def grid_info(self):
         nc = np.zeros(1, np.int64)
         dims = np.zeros(3, np.int32)

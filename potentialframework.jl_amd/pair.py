@@ -169,6 +169,25 @@ def evaluate_(w, sys, p, request=EnergyForcesVirial, parameters=None):
     return out
 
 
+def evaluate_batch(w, systems, p, request=EnergyForcesVirial, parameters=None):
+    """Evaluate many independent configurations with one launch per stage
+    (fitting sets). Returns one result dictionary per configuration."""
+    if parameters is not None:
+        p = p.with_parameters(parameters)
+    mask, names = request_mask(request)
+    chans, kind, nz, params = None, None, None, None
+    chans = []
+    for sys in systems:
+        ch, kind, nz, params = p._device_model(sys.labels)
+        chans.append(ch)
+    offsets = w.prepare_batch(systems, p.cutoff(), chans)
+    w.ctx.call("pfw_set_pair_model", kind, nz, ptr(np.ascontiguousarray(params)), float(p.cutoff()))
+    w.ctx.call("pfw_pair_step", mask, w.mode)
+    if w.timing:
+        w.read_timings()
+    return w.read_batch(offsets, names)
+
+
 def evaluate(sys, p, precision="float64", device=0, **kwargs):
     """``evaluate(sys, p; ...)`` with a fresh workspace (evaluation.jl:53-55)."""
     return evaluate_(Workspace(precision=precision, device=device), sys, p, **kwargs)

@@ -10,6 +10,11 @@
   amplitudes [1.0 0.7; 0.7 1.3] eV).
 * ``bcc_binary``: BCC a = 3.2 Å on nx x ny x nz cells with iid 50/50 species and
   Gaussian displacements 0.05 Å (the 32,768-atom ACE case at 32 x 32 x 16).
+* ``fitting_set``: ncfg independent periodic configurations with atom counts
+  log-uniform in [nmin, nmax], each a randomly sheared cubic cell at
+  0.085 atoms/Å^3 with uniformly placed atoms rejected below 1.8 Å from any
+  periodic image of an accepted atom, 2 species 50/50 (the 1,024-configuration
+  fitting-set case).
 * ``fixture``: the reference tests' deterministic scatter (test/fixtures.jl:60-67).
 
 Positions are (n, 3) arrays in Å; cells have lattice vectors in columns.
@@ -51,6 +56,40 @@ def bcc_binary(nx=32, ny=32, nz=16, a=3.2, sigma=0.05, seed=3):
     H = np.diag(np.array([nx, ny, nz], float) * a)
     species = rng.integers(0, 2, size=len(X)).astype(np.int32)
     return X, H, (True, True, True), species
+
+
+def fitting_set(ncfg=1024, nmin=8, nmax=2048, density=0.085, rmin=1.8, shear=0.05, seed=4, max_attempts=500):
+    """List of (X, H, pbc, species) configurations and the total placement attempts.
+
+    Generated by libpfw_workloads.so (csrc/workloads.cpp): random sequential
+    addition with a bin grid; raises when a configuration needs more than
+    max_attempts placement attempts per atom (saturation)."""
+    import ctypes
+    import os
+
+    so = ctypes.CDLL(os.path.join(os.path.dirname(os.path.abspath(__file__)), "libpfw_workloads.so"))
+    f = so.pfw_wl_fitting_set
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                  ctypes.c_uint64, ctypes.c_int64] + [ctypes.c_void_p] * 5
+    off = np.zeros(ncfg + 1, np.int64)
+    rc = f(ncfg, nmin, nmax, density, rmin, shear, seed, max_attempts, off.ctypes.data, None, None, None, None)
+    if rc:
+        raise ValueError("invalid fitting-set parameters")
+    n = int(off[-1])
+    X = np.zeros((n, 3))
+    sp = np.zeros(n, np.int32)
+    H = np.zeros((ncfg, 3, 3))
+    att = np.zeros(1, np.int64)
+    rc = f(ncfg, nmin, nmax, density, rmin, shear, seed, max_attempts, off.ctypes.data, X.ctypes.data, sp.ctypes.data,
+           H.ctypes.data, att.ctypes.data)
+    if rc == 2:
+        raise RuntimeError("random placement saturated")
+    if rc:
+        raise ValueError("invalid fitting-set parameters")
+    out = [(X[off[c]:off[c + 1]].copy(), H[c].T.copy(), (True, True, True), sp[off[c]:off[c + 1]].copy())
+           for c in range(ncfg)]
+    return out, int(att[0])
 
 
 def fixture(n, L=9.0, pbc=(True, True, True), cluster=False, mixed=True):
