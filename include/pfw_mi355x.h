@@ -182,6 +182,29 @@ int pfw_ace_evaluate(pfw_ctx* ctx, uint32_t request_mask, int precision, double*
 /* Per-site energies of the last ACE step, original order (n doubles). */
 int pfw_ace_site_energies(pfw_ctx* ctx, double* out);
 
+/* ---- parameter adjoints (reverse mode over the current system or batch) ----
+ * For cotangents dE (one per configuration), dF (3 x N column-major, original
+ * order) and dW (3 x 3 column-major per configuration, or NULL for zero), the
+ * gradient of L = sum_c dE_c E_c + <dF, F> + sum_c <dW_c, W_c> with respect to
+ * the model parameters. Environments are rebuilt and recomputed on the device;
+ * nothing from a previous step is reused.
+ *
+ * ACE: gWB (nbasis x nz, column-major) and gWpair (pair.nr x nz; may be NULL)
+ * of the readout WB and pair weights. Replaces the ace_evaluate rrule pullback
+ * (ext/PotentialFrameWorkChainRulesCoreExt.jl:54-76) with ace_vjp_impl /
+ * scalar_site_product (ext/PotentialFrameWorkForwardDiffExt.jl:63-113) and
+ * site_parameter_gradient (src/submodules/ACE/serial.jl:103-121). precision:
+ * PFW_FP64, or PFW_MIXED (FP32 site arithmetic, FP64 accumulation). */
+int pfw_ace_adjoint(pfw_ctx* ctx, int precision, const double* dE, const double* dF, const double* dW,
+                    double* gWB, double* gWpair);
+/* SmoothPair: gA (nz x nz) of the amplitude table entry each directed pair
+ * reads, FP64. The gradient with respect to the independent upper-triangle
+ * amplitudes is gA + gA^T off the diagonal and gA on it. Replaces the
+ * pair_evaluate rrule pullback (ext/PotentialFrameWorkChainRulesCoreExt.jl:26-52,
+ * pair_vjp in ext/PotentialFrameWorkForwardDiffExt.jl:27-54) and
+ * energy_parameter_gradient(::SmoothPair) (src/submodules/PairPotentials/parameters.jl:126-138). */
+int pfw_pair_adjoint(pfw_ctx* ctx, const double* dE, const double* dF, const double* dW, double* gA);
+
 #ifdef __cplusplus
 }
 #endif

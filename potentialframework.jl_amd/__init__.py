@@ -37,6 +37,13 @@ from .pair import (  # noqa: F401
 )
 from .ace import ACECalculator, ResolvedModel, load_resolved, ace_evaluate  # noqa: F401
 from . import ace, ace_basis  # noqa: F401
+from .adjoints import (  # noqa: F401
+    pair_parameter_vjp,
+    ace_parameter_vjp,
+    rrule_pair_evaluate,
+    rrule_ace_evaluate,
+    fitting_loss,
+)
 from .calculators import (  # noqa: F401
     potential_energy,
     forces,

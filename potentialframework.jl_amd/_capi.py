@@ -75,6 +75,8 @@ SIGNATURES = [
     ("pfw_ace_step", ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_int]),
     ("pfw_ace_evaluate", ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_int, _D, _D, _D]),
     ("pfw_ace_site_energies", ctypes.c_int, [_P, _D]),
+    ("pfw_ace_adjoint", ctypes.c_int, [_P, ctypes.c_int, _D, _D, _D, _D, _D]),
+    ("pfw_pair_adjoint", ctypes.c_int, [_P, _D, _D, _D, _D]),
 ]
 
 

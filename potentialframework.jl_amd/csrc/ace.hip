@@ -540,6 +540,143 @@ __global__ void k_ace_scatter_sites(int64_t n, const int32_t* __restrict__ perm,
   if (k < n) out[perm[k]] = siteE[k];
 }
 
+// ---------------------------------------------------------------- adjoints
+//
+// Gradient of a loss L(E_c, F) with respect to the precontracted readout
+// w_k(z) and the pair weights Wpair(n, z), given Ebar_c = dL/dE_c and
+// Fbar = dL/dF and Wbar_c = dL/dW_c. With g_e = dE_i/dR_e for edge
+// e = (i -> j), F_i += g_e, F_j -= g_e and W_ab -= R_a g_b, so the force and
+// virial part is sum_e g_e . delta_e with delta_e = Fbar_i - Fbar_j - Wbar^T R_e:
+//   dL/dw_k(z)     = sum_{i: z_i = z} [Ebar P_k(A_i) + sum_a dP_k/dA_a Adot_a]
+//   dL/dWpair(n,z) = sum_{i: z_i = z} sum_e [Ebar Q_n(r_e) + Q'_n(r_e) r^_e . delta_e]
+// with Adot_a = sum_e [R'_ra (r^ . delta_e) Y_ma + R_ra (grad Y_ma . delta_e)],
+// the JVP of the site basis along the edge directions (the reference's
+// ace_vjp_impl / scalar_site_product, ext/PotentialFrameWorkForwardDiffExt.jl:63-109,
+// and site_parameter_gradient, src/submodules/ACE/serial.jl:312-330).
+// Environments (radials, Y_lm) are recomputed, nothing per edge is stored.
+// Each workgroup walks sites with a grid stride and accumulates its partial
+// gradient in LDS; a second kernel sums the workgroup partials in FP64.
+template <typename T, int L>
+__global__ __launch_bounds__(64) void k_ace_adjoint(AceArgs<T> a, const int32_t* __restrict__ atom_cfg,
+                                                    const double* __restrict__ Ebar, const double* __restrict__ Fbar,
+                                                    const double* __restrict__ Wbar, double* __restrict__ partial) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int NY = (L + 1) * (L + 1);
+  const int nq = a.pair.nr;
+  const int gsize = a.nz * (a.nprod + nq);
+  double* G = (double*)smem;             // nz x (nprod + nq) partial gradient, FP64
+  T* Fl = (T*)(G + gsize);               // (L+1)^2
+  T* A = Fl + NY;                        // ndens
+  T* Ad = A + a.ndens;                   // ndens
+  T* Rb = Ad + a.ndens;                  // nr x kStride
+  T* Rdb = Rb + a.rad.nr * kStride;      // nr x kStride
+  T* Yb = Rdb + a.rad.nr * kStride;      // NY x kStride
+  T* Ydb = Yb + NY * kStride;            // NY x kStride
+  const int lane = threadIdx.x;
+  if (lane < NY) {
+    int l = 0;
+    while ((l + 1) * (l + 1) <= lane) ++l;
+    const int m = lane - l * l;
+    if (m <= l) {
+      double F = sqrt((2.0 * l + 1) / (2 * 3.14159265358979323846));
+      for (int mm = 1; mm <= m; ++mm) F = -F / sqrt((double)(l + mm) * (l + 1 - mm));
+      Fl[l * (L + 1) + m] = (T)F;
+    }
+  }
+  for (int k = lane; k < gsize; k += kWave) G[k] = 0;
+  __syncthreads();
+  for (int64_t site = blockIdx.x; site < a.n; site += gridDim.x) {
+    const int iz = a.sps[site];
+    const int oi = a.perm[site];
+    const int cfg = atom_cfg[oi];
+    const T eb = (T)Ebar[cfg];
+    double wb[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) wb[k] = Wbar ? Wbar[9 * cfg + k] : 0.0;
+    const double fi0 = Fbar[3 * oi], fi1 = Fbar[3 * oi + 1], fi2 = Fbar[3 * oi + 2];
+    const int e0 = a.estart[site], ne = a.estart[site + 1] - e0;
+    for (int k = lane; k < a.ndens; k += kWave) A[k] = Ad[k] = 0;
+    __syncthreads();
+    for (int c0 = 0; c0 < ne; c0 += kWave) {
+      const int e = c0 + lane;
+      const bool act = e < ne;
+      T dx = 1, dy = 0, dz = 0, r = 1, de0 = 0, de1 = 0, de2 = 0;
+      int ch = 0;
+      if (act) {
+        const double* R = a.eR + 3 * (int64_t)(e0 + e);
+        const int j = a.enb[e0 + e];
+        const int oj = a.perm[j];
+        dx = (T)R[0];
+        dy = (T)R[1];
+        dz = (T)R[2];
+        r = (T)sqrt(R[0] * R[0] + R[1] * R[1] + R[2] * R[2]);
+        de0 = (T)(fi0 - Fbar[3 * oj] - (wb[0] * R[0] + wb[1] * R[1] + wb[2] * R[2]));
+        de1 = (T)(fi1 - Fbar[3 * oj + 1] - (wb[3] * R[0] + wb[4] * R[1] + wb[5] * R[2]));
+        de2 = (T)(fi2 - Fbar[3 * oj + 2] - (wb[6] * R[0] + wb[7] * R[1] + wb[8] * R[2]));
+        ch = iz + a.nz * a.sps[j];
+      }
+      const T sdot = (dx * de0 + dy * de1 + dz * de2) / r;  // r^ . delta
+      if (act) {
+        const RadialPoint<T> P = radial_point(a.rad, ch, r);
+        for (int n = 0; n < a.rad.nr; ++n) {
+          T v, d;
+          radial_value(a.rad, ch, P, n, v, d);
+          Rb[n * kStride + lane] = v;
+          Rdb[n * kStride + lane] = d * sdot;
+        }
+        T Y[NY], Gy[3 * NY];
+        ylm<T, L>(dx, dy, dz, Fl, Y, Gy);
+#pragma unroll
+        for (int m = 0; m < NY; ++m) {
+          Yb[m * kStride + lane] = Y[m];
+          Ydb[m * kStride + lane] = Gy[3 * m] * de0 + Gy[3 * m + 1] * de1 + Gy[3 * m + 2] * de2;
+        }
+      }
+      if (nq) {  // pair weights: wave sums over the chunk's edges
+        RadialPoint<T> PQ;
+        if (act) PQ = radial_point(a.pair, ch, r);
+        for (int n = 0; n < nq; ++n) {
+          T v = 0, d = 0;
+          if (act) radial_value(a.pair, ch, PQ, n, v, d);
+          const T t = wave_sum(act ? eb * v + d * sdot : (T)0);
+          if (lane == 0) G[iz * (a.nprod + nq) + a.nprod + n] += (double)t;
+        }
+      }
+      __syncthreads();
+      const int cn = min(kWave, ne - c0);
+      for (int k = lane; k < a.ndens; k += kWave) {
+        const int rn = a.dens[2 * k], mm = a.dens[2 * k + 1];
+        const T* rr = Rb + rn * kStride;
+        const T* rd = Rdb + rn * kStride;
+        const T* yy = Yb + mm * kStride;
+        const T* yd = Ydb + mm * kStride;
+        T s = 0, sd = 0;
+        for (int t = 0; t < cn; ++t) {
+          s += rr[t] * yy[t];
+          sd += rd[t] * yy[t] + rr[t] * yd[t];
+        }
+        A[k] += s;
+        Ad[k] += sd;
+      }
+      __syncthreads();
+    }
+    // products: value and tangent; each lane owns its products' gradient entries
+    double* Gz = G + iz * (a.nprod + nq);
+    for (int p = lane; p < a.nprod; p += kWave) {
+      const int* f = a.prod + 4 * p;
+      T v = 1, dv = 0;
+      for (int t = 0; t < 4 && f[t] >= 0; ++t) {
+        const T x = A[f[t]], dx = Ad[f[t]];
+        dv = dv * x + v * dx;
+        v *= x;
+      }
+      Gz[p] += (double)(eb * v + dv);
+    }
+    __syncthreads();
+  }
+  for (int k = lane; k < gsize; k += kWave) partial[(int64_t)blockIdx.x * gsize + k] = G[k];
+}
+
 // ---------------------------------------------------------------- host
 
 template <typename T>
@@ -818,8 +955,8 @@ static FamArgs<T> fam_args(const AceFamilyDev& f) {
 
 static inline unsigned nblk(int64_t n, int t) { return (unsigned)std::max<int64_t>(1, (n + t - 1) / t); }
 
-template <typename T, int L, int SPW>
-static void launch_sites(pfw_ctx* ctx, int grads) {
+template <typename T>
+static AceArgs<T> make_args(pfw_ctx* ctx, int grads) {
   AceDev& ad = ctx->ace;
   SystemBuffers& s = ctx->sys;
   AceArgs<T> a;
@@ -855,6 +992,14 @@ static void launch_sites(pfw_ctx* ctx, int grads) {
   a.siteE = ad.siteE;
   a.siteW = ad.siteW;
   a.eg = ad.eg;
+  return a;
+}
+
+template <typename T, int L, int SPW>
+static void launch_sites(pfw_ctx* ctx, int grads) {
+  AceDev& ad = ctx->ace;
+  SystemBuffers& s = ctx->sys;
+  AceArgs<T> a = make_args<T>(ctx, grads);
   constexpr int NY = (L + 1) * (L + 1);
   size_t lds = sizeof(T) * ((size_t)NY + SPW * (2 * (size_t)ad.ndens + (size_t)ad.nseg) +
                             (size_t)(ad.radial.nr + NY) * kStride);
@@ -889,6 +1034,53 @@ static void dispatch_sites(pfw_ctx* ctx, int grads) {
   }
 }
 
+// Directed environments of every site (CSR by sorted centre) from the cell
+// list built by nl_build; with `rev` also the reverse-edge index.
+static void ace_environment(pfw_ctx* ctx, int rev) {
+  AceDev& ad = ctx->ace;
+  SystemBuffers& s = ctx->sys;
+  hipStream_t st = ctx->stream;
+  const int64_t n = s.n;
+  if (!ad.ecount) {
+    PFW_HIP(hipMalloc(&ad.ecount, n * sizeof(int32_t)));
+    PFW_HIP(hipMalloc(&ad.estart, (n + 1) * sizeof(int32_t)));
+    PFW_HIP(hipMalloc(&ad.ebs, ((n + 1023) / 1024 + 2) * sizeof(int32_t)));
+    PFW_HIP(hipMalloc(&ad.siteE, n * sizeof(double)));
+    PFW_HIP(hipMalloc(&ad.siteW, 9 * n * sizeof(double)));
+    PFW_HIP(hipMalloc(&ad.rec, 10 * n * sizeof(double)));
+    PFW_HIP(hipMalloc(&ad.status, 4 * sizeof(int32_t)));
+  }
+  PFW_HIP(hipMemsetAsync(ad.status, 0, 4 * sizeof(int32_t), st));
+  hipLaunchKernelGGL(k_ace_edges, dim3(nblk(n, 128)), dim3(128), 0, st, s.grids, s.atom_cfg, n, s.xs, s.start,
+                     s.cell_of, s.perm, 0, ad.ecount, (const int32_t*)nullptr, (int32_t*)nullptr, (double*)nullptr, ad.status + 1);
+  PFW_HIP(hipGetLastError());
+  scan_counts(ctx, ad.ecount, (int)n, ad.estart, ad.ebs, ad.ecount /* cursor: reused, already consumed */);
+  int32_t total = 0, maxdeg = 0;
+  PFW_HIP(hipMemcpyAsync(&total, ad.estart + n, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  PFW_HIP(hipMemcpyAsync(&maxdeg, ad.status + 1, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  PFW_HIP(hipStreamSynchronize(st));
+  ad.max_degree = maxdeg;
+  if (total < 0) throw Error(PFW_CAPACITY, "edge count overflow");
+  if (total > ad.edge_cap || !ad.enb) {
+    dfree(ad.enb);
+    dfree(ad.eR);
+    dfree(ad.erev);
+    dfree(ad.eg);
+    int64_t cap = std::max<int64_t>((int64_t)(total * 1.25) + 64, 64);
+    PFW_HIP(hipMalloc(&ad.enb, cap * sizeof(int32_t)));
+    PFW_HIP(hipMalloc(&ad.eR, 3 * cap * sizeof(double)));
+    PFW_HIP(hipMalloc(&ad.erev, cap * sizeof(int32_t)));
+    PFW_HIP(hipMalloc(&ad.eg, 3 * cap * sizeof(double)));
+    ad.edge_cap = cap;
+  }
+  hipLaunchKernelGGL(k_ace_edges, dim3(nblk(n, 128)), dim3(128), 0, st, s.grids, s.atom_cfg, n, s.xs, s.start,
+                     s.cell_of, s.perm, 1, ad.ecount, (const int32_t*)ad.estart, ad.enb, ad.eR, ad.status + 1);
+  PFW_HIP(hipGetLastError());
+  if (rev)
+    hipLaunchKernelGGL(k_ace_rev, dim3(nblk(n, 128)), dim3(128), 0, st, n, ad.estart, ad.enb, ad.eR, ad.erev,
+                       ad.status);
+}
+
 void ace_run(pfw_ctx* ctx, uint32_t mask, int precision) {
   AceDev& ad = ctx->ace;
   if (!ad.ready) throw Error(PFW_STALE, "no ACE model set");
@@ -906,45 +1098,8 @@ void ace_run(pfw_ctx* ctx, uint32_t mask, int precision) {
     PFW_HIP(hipMemsetAsync(ctx->d_cfg_out, 0, 11 * (size_t)ctx->ncfg * sizeof(double), st));
     if (n > 0) PFW_HIP(hipMemsetAsync(ctx->d_F, 0, 3 * n * sizeof(double), st));
   } else {
-    if (!ad.ecount) {
-      PFW_HIP(hipMalloc(&ad.ecount, n * sizeof(int32_t)));
-      PFW_HIP(hipMalloc(&ad.estart, (n + 1) * sizeof(int32_t)));
-      PFW_HIP(hipMalloc(&ad.ebs, ((n + 1023) / 1024 + 2) * sizeof(int32_t)));
-      PFW_HIP(hipMalloc(&ad.siteE, n * sizeof(double)));
-      PFW_HIP(hipMalloc(&ad.siteW, 9 * n * sizeof(double)));
-      PFW_HIP(hipMalloc(&ad.rec, 10 * n * sizeof(double)));
-      PFW_HIP(hipMalloc(&ad.status, 4 * sizeof(int32_t)));
-    }
-    PFW_HIP(hipMemsetAsync(ad.status, 0, 4 * sizeof(int32_t), st));
-    hipLaunchKernelGGL(k_ace_edges, dim3(nblk(n, 128)), dim3(128), 0, st, s.grids, s.atom_cfg, n, s.xs, s.start,
-                       s.cell_of, s.perm, 0, ad.ecount, (const int32_t*)nullptr, (int32_t*)nullptr, (double*)nullptr, ad.status + 1);
-    PFW_HIP(hipGetLastError());
-    scan_counts(ctx, ad.ecount, (int)n, ad.estart, ad.ebs, ad.ecount /* cursor: reused, already consumed */);
-    int32_t total = 0, maxdeg = 0;
-    PFW_HIP(hipMemcpyAsync(&total, ad.estart + n, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-    PFW_HIP(hipMemcpyAsync(&maxdeg, ad.status + 1, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-    PFW_HIP(hipStreamSynchronize(st));
-    ad.max_degree = maxdeg;
-    if (total < 0) throw Error(PFW_CAPACITY, "edge count overflow");
-    if (total > ad.edge_cap || !ad.enb) {
-      dfree(ad.enb);
-      dfree(ad.eR);
-      dfree(ad.erev);
-      dfree(ad.eg);
-      int64_t cap = std::max<int64_t>((int64_t)(total * 1.25) + 64, 64);
-      PFW_HIP(hipMalloc(&ad.enb, cap * sizeof(int32_t)));
-      PFW_HIP(hipMalloc(&ad.eR, 3 * cap * sizeof(double)));
-      PFW_HIP(hipMalloc(&ad.erev, cap * sizeof(int32_t)));
-      PFW_HIP(hipMalloc(&ad.eg, 3 * cap * sizeof(double)));
-      ad.edge_cap = cap;
-    }
-    hipLaunchKernelGGL(k_ace_edges, dim3(nblk(n, 128)), dim3(128), 0, st, s.grids, s.atom_cfg, n, s.xs, s.start,
-                       s.cell_of, s.perm, 1, ad.ecount, (const int32_t*)ad.estart, ad.enb, ad.eR, ad.status + 1);
-    PFW_HIP(hipGetLastError());
     const int grads = (mask & (PFW_FORCES | PFW_VIRIAL)) ? 1 : 0;
-    if (grads)
-      hipLaunchKernelGGL(k_ace_rev, dim3(nblk(n, 128)), dim3(128), 0, st, n, ad.estart, ad.enb, ad.eR, ad.erev,
-                         ad.status);
+    ace_environment(ctx, grads);
     record_stage(ctx, 4);
     if (precision == PFW_MIXED)
       dispatch_sites<float>(ctx, grads);
@@ -983,6 +1138,82 @@ void ace_run(pfw_ctx* ctx, uint32_t mask, int precision) {
   ctx->last_mask = mask;
   ctx->have_results = true;
   if (ctx->h_out[10] != 0.0) throw Error(PFW_DOMAIN, "nonfinite ACE output (coincident atoms or singular envelope)");
+}
+
+template <typename T, int L>
+static void launch_adjoint(pfw_ctx* ctx, const double* Ebar, const double* Fbar, const double* Wbar, double* part,
+                           int nb) {
+  AceDev& ad = ctx->ace;
+  AceArgs<T> a = make_args<T>(ctx, 0);
+  constexpr int NY = (L + 1) * (L + 1);
+  const size_t gsize = (size_t)ad.nz * (ad.nprod + ad.pair.nr);
+  const size_t lds = sizeof(double) * gsize +
+                     sizeof(T) * ((size_t)NY + 2 * (size_t)ad.ndens + 2 * (size_t)(ad.radial.nr + NY) * kStride);
+  if (lds > 160 * 1024) throw Error(PFW_INVALID_ARGUMENT, "ACE model too large for the adjoint kernel's LDS tiles");
+  if (lds > 64 * 1024)
+    PFW_HIP(hipFuncSetAttribute((const void*)k_ace_adjoint<T, L>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL((k_ace_adjoint<T, L>), dim3(nb), dim3(kWave), lds, ctx->stream, a,
+                     (const int32_t*)ctx->sys.atom_cfg, Ebar, Fbar, Wbar, part);
+  PFW_HIP(hipGetLastError());
+}
+
+// dL/dWB (nbasis x nz) and dL/dWpair (nq x nz) of a loss with the given
+// dL/dE per configuration, dL/dF (3 x N, original order) and, optionally,
+// dL/dW (3 x 3 column-major per configuration).
+void ace_adjoint(pfw_ctx* ctx, int precision, const double* Ebar, const double* Fbar, const double* Wbar, double* gWB,
+                 double* gWpair) {
+  AceDev& ad = ctx->ace;
+  if (!ad.ready) throw Error(PFW_STALE, "no ACE model set");
+  if (ad.cutoff != ctx->cutoff)
+    throw Error(PFW_STALE, "the system's cell list was built for another cutoff than the ACE search cutoff");
+  SystemBuffers& s = ctx->sys;
+  hipStream_t st = ctx->stream;
+  const int64_t n = s.n;
+  const int nq = ad.pair.nr;
+  const size_t gsize = (size_t)ad.nz * (ad.nprod + nq);
+  std::vector<double> g(gsize, 0.0);
+  if (n > 0) {
+    for (int c = 0; c < ctx->ncfg; ++c)
+      if (!std::isfinite(Ebar[c])) throw Error(PFW_INVALID_ARGUMENT, "nonfinite energy adjoint");
+    for (int64_t k = 0; k < 3 * n; ++k)
+      if (!std::isfinite(Fbar[k])) throw Error(PFW_INVALID_ARGUMENT, "nonfinite force adjoint");
+    if (Wbar)
+      for (int64_t k = 0; k < 9 * (int64_t)ctx->ncfg; ++k)
+        if (!std::isfinite(Wbar[k])) throw Error(PFW_INVALID_ARGUMENT, "nonfinite virial adjoint");
+    nl_build(ctx);
+    ace_environment(ctx, 0);
+    const int nb = (int)std::min<int64_t>(n, 1024);
+    DeviceArray<double> dE(ctx->ncfg), dF(3 * n), dW(Wbar ? 9 * (size_t)ctx->ncfg : 0), part((size_t)nb * gsize),
+        sum(gsize);
+    PFW_HIP(hipMemcpyAsync(dE.p, Ebar, ctx->ncfg * sizeof(double), hipMemcpyHostToDevice, st));
+    PFW_HIP(hipMemcpyAsync(dF.p, Fbar, 3 * n * sizeof(double), hipMemcpyHostToDevice, st));
+    if (Wbar) PFW_HIP(hipMemcpyAsync(dW.p, Wbar, 9 * (size_t)ctx->ncfg * sizeof(double), hipMemcpyHostToDevice, st));
+    const int L = ad.lmax <= 2 ? 2 : ad.lmax <= 4 ? 4 : 6;
+    auto launch = precision == PFW_MIXED
+                      ? (L == 2 ? launch_adjoint<float, 2> : L == 4 ? launch_adjoint<float, 4> : launch_adjoint<float, 6>)
+                      : (L == 2 ? launch_adjoint<double, 2>
+                                : L == 4 ? launch_adjoint<double, 4> : launch_adjoint<double, 6>);
+    launch(ctx, dE.p, dF.p, dW.p, part.p, nb);
+    hipLaunchKernelGGL(k_sum_partials, dim3(nblk((int64_t)gsize, 256)), dim3(256), 0, st, part.p, nb, (int)gsize,
+                       sum.p);
+    PFW_HIP(hipGetLastError());
+    PFW_HIP(hipMemcpyAsync(g.data(), sum.p, gsize * sizeof(double), hipMemcpyDeviceToHost, st));
+    PFW_HIP(hipStreamSynchronize(st));
+  }
+  // dL/dWB[b, z] = sum_k C[b, k] dL/dw_k(z): the transpose of the precontraction
+  if (gWB) {
+    for (size_t q = 0; q < (size_t)ad.nbasis * ad.nz; ++q) gWB[q] = 0;
+    for (size_t t = 0; t < ad.c_vals.size(); ++t)
+      for (int z = 0; z < ad.nz; ++z)
+        gWB[(size_t)ad.c_rows[t] + (size_t)ad.nbasis * z] +=
+            ad.c_vals[t] * g[(size_t)z * (ad.nprod + nq) + ad.c_cols[t]];
+  }
+  if (gWpair)
+    for (int z = 0; z < ad.nz; ++z)
+      for (int q = 0; q < nq; ++q) gWpair[q + (size_t)nq * z] = g[(size_t)z * (ad.nprod + nq) + ad.nprod + q];
+  bool bad = false;
+  for (double v : g) bad |= !std::isfinite(v);
+  if (bad) throw Error(PFW_DOMAIN, "nonfinite ACE parameter gradient (coincident atoms or singular envelope)");
 }
 
 void ace_site_energies(pfw_ctx* ctx, double* out) {

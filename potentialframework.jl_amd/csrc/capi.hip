@@ -308,6 +308,25 @@ int pfw_ace_site_energies(pfw_ctx* ctx, double* out) {
   });
 }
 
+int pfw_ace_adjoint(pfw_ctx* ctx, int precision, const double* dE, const double* dF, const double* dW, double* gWB,
+                    double* gWpair) {
+  return run(ctx, [&] {
+    if (!ctx->have_system) throw Error(PFW_STALE, "no system set");
+    if (!ctx->ace.ready) throw Error(PFW_STALE, "no ACE model set");
+    if (precision != PFW_FP64 && precision != PFW_MIXED) throw Error(PFW_INVALID_ARGUMENT, "unknown precision");
+    if (!dE || !dF || !gWB) throw Error(PFW_INVALID_ARGUMENT, "null adjoint argument");
+    pfw::ace_adjoint(ctx, precision, dE, dF, dW, gWB, gWpair);
+  });
+}
+
+int pfw_pair_adjoint(pfw_ctx* ctx, const double* dE, const double* dF, const double* dW, double* gA) {
+  return run(ctx, [&] {
+    if (!ctx->have_system) throw Error(PFW_STALE, "no system set");
+    if (!dE || !dF || !gA) throw Error(PFW_INVALID_ARGUMENT, "null adjoint argument");
+    pfw::pair_adjoint(ctx, dE, dF, dW, gA);
+  });
+}
+
 int pfw_synchronize(pfw_ctx* ctx) {
   return run(ctx, [&] { PFW_HIP(hipStreamSynchronize(ctx->stream)); });
 }

@@ -493,6 +493,16 @@ __global__ void k_colsum(const double* __restrict__ in, int64_t m, int width, do
   }
 }
 
+// Column sums of a (nblocks x width) table of per-workgroup partials, in
+// fixed row order (deterministic).
+__global__ void k_sum_partials(const double* __restrict__ partial, int nblocks, int width, double* __restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= width) return;
+  double s = 0;
+  for (int b = 0; b < nblocks; ++b) s += partial[(int64_t)b * width + k];
+  out[k] = s;
+}
+
 static inline unsigned nblocks(int64_t n, int t) { return (unsigned)std::max<int64_t>(1, (n + t - 1) / t); }
 
 // Exclusive scan of m counts into start[0..m] (start[m] = total); also zeroes cursor.

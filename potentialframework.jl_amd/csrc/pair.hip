@@ -15,6 +15,7 @@
 // inside a thin band around the cutoff is re-decided with the exact FP64
 // formula of the oracle, so the pair set is bit-exact in both precisions.
 #include <algorithm>
+#include <cmath>
 
 #include "pfw_device.h"
 
@@ -340,6 +341,123 @@ void pair_run(pfw_ctx* ctx, uint32_t mask, int precision) {
   ctx->have_results = true;
   if (ctx->h_out[10] != 0.0)
     throw Error(PFW_DOMAIN, "singular origin or nonfinite pair callback output");
+}
+
+// Parameter adjoint of SmoothPair: dL/dA (nz x nz, the entry each directed
+// pair reads) for dL/dE per configuration, dL/dF (3 x N original order) and
+// optional dL/dW. E, F and W are linear in A (parameters.jl:126-138), so each
+// directed pair i -> j at d = x_j + S - x_i with q = 1 - r^2/c^2 adds
+//   1/2 Ebar q^3 - (6 q^2 / c^2) (Fbar_i . d - 1/2 d^T Wbar d)
+// to dL/dA[z_i, z_j] (the pair_vjp of ext/PotentialFrameWorkForwardDiffExt.jl:27-54).
+// FP64, one thread per sorted atom enumerating the same exact pair set as the
+// force kernel; per-thread sums per neighbour species, then a fixed-order
+// block reduction (deterministic).
+__global__ __launch_bounds__(256) void k_pair_adjoint(const Grid* __restrict__ grids,
+                                                      const int32_t* __restrict__ atom_cfg, int64_t n,
+                                                      const double* __restrict__ xs,
+                                                      const int32_t* __restrict__ start,
+                                                      const int32_t* __restrict__ cell_of,
+                                                      const int32_t* __restrict__ perm,
+                                                      const int32_t* __restrict__ sps, int nz,
+                                                      const double* __restrict__ Ebar,
+                                                      const double* __restrict__ Fbar,
+                                                      const double* __restrict__ Wbar,
+                                                      double* __restrict__ partial) {
+  __shared__ double tab[256][kMaxZ + 1];
+  __shared__ int zs[256];
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[kMaxZ];
+#pragma unroll
+  for (int z = 0; z < kMaxZ; ++z) acc[z] = 0;
+  int zi = -1;
+  if (k < n) {
+    const int oi = perm[k];
+    const int cfg = atom_cfg[oi];
+    const Grid& g = grids[cfg];
+    zi = sps[k];
+    int key[3];
+    cell_key(g, cell_of[oi], key);
+    const double xi = xs[k], yi = xs[n + k], zz = xs[2 * n + k];
+    const double eb = 0.5 * Ebar[cfg], f0 = Fbar[3 * oi], f1 = Fbar[3 * oi + 1], f2 = Fbar[3 * oi + 2];
+    double wb[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) wb[q] = Wbar ? 0.5 * Wbar[9 * cfg + q] : 0.0;
+    const double invc2 = 1.0 / g.c2, s6 = -6.0 * invc2;
+    for (int e = 0; e < g.nstencil; ++e) {
+      int lat[3];
+      bool zero;
+      const int nc = stencil_entry(g, key, e, lat, &zero);
+      if (nc < 0) continue;
+      double sh[3];
+      exact_shift(g, lat, sh);
+      for (int m = start[nc]; m < start[nc + 1]; ++m) {
+        if (zero && m == k) continue;
+        double d[3];
+        int img[3];
+        if (!exact_member(g, xi, yi, zz, xs[m], xs[n + m], xs[2 * n + m], lat, sh, d, img)) continue;
+        const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        const double qq = 1.0 - r2 * invc2, q2 = qq * qq;
+        const double fd = f0 * d[0] + f1 * d[1] + f2 * d[2];
+        const double dwd = d[0] * (wb[0] * d[0] + wb[3] * d[1] + wb[6] * d[2]) +
+                           d[1] * (wb[1] * d[0] + wb[4] * d[1] + wb[7] * d[2]) +
+                           d[2] * (wb[2] * d[0] + wb[5] * d[1] + wb[8] * d[2]);
+        const double t = eb * q2 * qq + s6 * q2 * (fd - dwd);
+        const int zj = sps[m];
+#pragma unroll
+        for (int z = 0; z < kMaxZ; ++z) acc[z] += z == zj ? t : 0.0;
+      }
+    }
+  }
+  for (int z = 0; z < kMaxZ; ++z) tab[threadIdx.x][z] = acc[z];
+  zs[threadIdx.x] = zi;
+  __syncthreads();
+  const int nn = nz * nz;
+  for (int idx = threadIdx.x; idx < nn; idx += blockDim.x) {
+    const int a = idx / nz, b = idx % nz;
+    double v = 0;
+    for (int t = 0; t < (int)blockDim.x; ++t)
+      if (zs[t] == a) v += tab[t][b];
+    partial[(int64_t)blockIdx.x * nn + idx] = v;
+  }
+}
+
+void pair_adjoint(pfw_ctx* ctx, const double* Ebar, const double* Fbar, const double* Wbar, double* gA) {
+  if (ctx->pair.kind < 0) throw Error(PFW_STALE, "no pair model set");
+  if (ctx->pair.kind != PFW_PAIR_SMOOTH)
+    throw Error(PFW_INVALID_ARGUMENT, "parameter adjoints are implemented for SmoothPair amplitudes");
+  if (ctx->pair.cutoff != ctx->cutoff)
+    throw Error(PFW_STALE, "pair model cutoff differs from the system's cell-list cutoff; set the system again");
+  const int nz = ctx->pair.nz;
+  if (nz > kMaxZ) throw Error(PFW_INVALID_ARGUMENT, "too many species for the pair table");
+  SystemBuffers& s = ctx->sys;
+  const int64_t n = s.n;
+  const int nn = nz * nz;
+  for (int k = 0; k < nn; ++k) gA[k] = 0;
+  if (n == 0) return;
+  for (int c = 0; c < ctx->ncfg; ++c)
+    if (!std::isfinite(Ebar[c])) throw Error(PFW_INVALID_ARGUMENT, "nonfinite energy adjoint");
+  for (int64_t k = 0; k < 3 * n; ++k)
+    if (!std::isfinite(Fbar[k])) throw Error(PFW_INVALID_ARGUMENT, "nonfinite force adjoint");
+  if (Wbar)
+    for (int64_t k = 0; k < 9 * (int64_t)ctx->ncfg; ++k)
+      if (!std::isfinite(Wbar[k])) throw Error(PFW_INVALID_ARGUMENT, "nonfinite virial adjoint");
+  nl_build(ctx);
+  hipStream_t st = ctx->stream;
+  const int nb = (int)((n + 255) / 256);
+  DeviceArray<double> dE(ctx->ncfg), dF(3 * n), dW(Wbar ? 9 * (size_t)ctx->ncfg : 0), part((size_t)nb * nn),
+      sum(nn);
+  PFW_HIP(hipMemcpyAsync(dE.p, Ebar, ctx->ncfg * sizeof(double), hipMemcpyHostToDevice, st));
+  PFW_HIP(hipMemcpyAsync(dF.p, Fbar, 3 * n * sizeof(double), hipMemcpyHostToDevice, st));
+  if (Wbar) PFW_HIP(hipMemcpyAsync(dW.p, Wbar, 9 * (size_t)ctx->ncfg * sizeof(double), hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_pair_adjoint, dim3(nb), dim3(256), 0, st, s.grids, s.atom_cfg, n, s.xs, s.start, s.cell_of,
+                     s.perm, s.sps, nz, dE.p, dF.p, dW.p, part.p);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, part.p, nb, nn, sum.p);
+  PFW_HIP(hipGetLastError());
+  PFW_HIP(hipMemcpyAsync(gA, sum.p, nn * sizeof(double), hipMemcpyDeviceToHost, st));
+  PFW_HIP(hipStreamSynchronize(st));
+  bool bad = false;
+  for (int k = 0; k < nn; ++k) bad |= !std::isfinite(gA[k]);
+  if (bad) throw Error(PFW_DOMAIN, "nonfinite pair parameter gradient");
 }
 
 }  // namespace pfw

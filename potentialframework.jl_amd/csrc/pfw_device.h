@@ -127,5 +127,6 @@ __device__ inline T wave_sum(T v) {
 // into out[b*width + c] (width <= 16, blockDim 256).
 constexpr int kRedBlocks = 64;
 __global__ void k_colsum(const double* __restrict__ in, int64_t m, int width, double* __restrict__ out);
+__global__ void k_sum_partials(const double* __restrict__ partial, int nblocks, int width, double* __restrict__ out);
 
 }  // namespace pfw

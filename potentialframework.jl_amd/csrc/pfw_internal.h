@@ -23,6 +23,20 @@ struct Error : std::runtime_error {
       throw ::pfw::Error(PFW_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
+// Device allocation owned by a scope (temporary buffers of one call).
+template <typename T>
+struct DeviceArray {
+  T* p = nullptr;
+  explicit DeviceArray(size_t n) {
+    if (n) PFW_HIP(hipMalloc(&p, n * sizeof(T)));
+  }
+  ~DeviceArray() {
+    if (p) (void)hipFree(p);
+  }
+  DeviceArray(const DeviceArray&) = delete;
+  DeviceArray& operator=(const DeviceArray&) = delete;
+};
+
 // Cell grid of one system. Static fields are set on the host by pfw_set_system;
 // for open (non-periodic) axes origin/sizecell/used dims are recomputed on the
 // device every step from the atoms' extent.
@@ -180,6 +194,7 @@ void scan_counts(pfw_ctx* ctx, const int32_t* count, int m, int32_t* start, int3
 // pair.hip
 void pair_run(pfw_ctx* ctx, uint32_t mask, int precision);
 void record_stage(pfw_ctx* ctx, int k);
+void pair_adjoint(pfw_ctx* ctx, const double* Ebar, const double* Fbar, const double* Wbar, double* gA);
 // ace.hip
 void ace_set_model(pfw_ctx* ctx, const pfw_ace_model* m);
 void ace_set_parameters(pfw_ctx* ctx, const double* WB, const double* Wpair);
@@ -187,4 +202,6 @@ void ace_run(pfw_ctx* ctx, uint32_t mask, int precision);
 void ace_free(pfw_ctx* ctx);
 void ace_free_edges(pfw_ctx* ctx);
 void ace_site_energies(pfw_ctx* ctx, double* out);
+void ace_adjoint(pfw_ctx* ctx, int precision, const double* Ebar, const double* Fbar, const double* Wbar, double* gWB,
+                 double* gWpair);
 }  // namespace pfw

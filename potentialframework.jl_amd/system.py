@@ -31,6 +31,9 @@ class System:
     def __len__(self):
         return len(self.positions)
 
+    def copy(self):
+        return System(self.positions, self.cell, self.pbc, self.labels)
+
     def position(self, i):
         return self.positions[i]
 

@@ -1,0 +1,229 @@
+"""GPU parameter adjoints (SmoothPair amplitudes, ACE readout WB and pair
+weights) against the oracle.
+
+E, F and W are linear in these parameters, so for fixed cotangents the scalar
+L(theta) = sum_c dE_c E_c + <dF, F> + <dW, W> is affine in theta: its gradient
+is fixed exactly by oracle evaluations at shifted parameters (unit-basis
+evaluations for SmoothPair, as energy_parameter_gradient does in
+src/submodules/PairPotentials/parameters.jl:126-138; random directions and
+single components for ACE). A fitting loss that is quadratic in the parameters
+is checked by central differences, which are exact for quadratics.
+"""
+import numpy as np
+import pytest
+
+from oracle import KIND_SMOOTH, pair_evaluate
+from oracle.ace import ace_evaluate as oracle_ace
+from oracle.ace import load_resolved as oracle_load
+
+pytestmark = pytest.mark.gpu
+
+A = np.array([[1.0, 0.7], [0.7, 1.3]])
+RC = 6.0
+GOLDEN = __import__("os").path.join(__import__("os").path.dirname(__file__), "golden")
+
+
+@pytest.fixture(scope="module")
+def small_set(pfw):
+    cfgs, _ = pfw.workloads.fitting_set(ncfg=6, nmin=8, nmax=120, seed=21)
+    return cfgs
+
+
+@pytest.fixture(scope="module")
+def synthetic(pfw):
+    D = pfw.ace_basis.synthetic_model()
+    model, ps, _ = pfw.load_resolved(D)
+    return model, ps, oracle_load(D)
+
+
+def cotangents(cfgs, seed, virial=True):
+    rng = np.random.default_rng(seed)
+    dE = rng.normal(size=len(cfgs))
+    dF = [rng.normal(size=(3, len(c[0]))) for c in cfgs]
+    dW = [rng.normal(size=(3, 3)) for _ in cfgs] if virial else None
+    return dE, dF, dW
+
+
+def linear_functional(results, dE, dF, dW):
+    """sum_c dE_c E_c + <dF_c, F_c> + <dW_c, W_c>; results as (E, F (3, n), W)."""
+    L = 0.0
+    for c, (E, F, W) in enumerate(results):
+        L += dE[c] * E + float(np.sum(dF[c] * F))
+        if dW is not None:
+            L += float(np.sum(dW[c] * W))
+    return L
+
+
+def oracle_pair_L(cfgs, table, dE, dF, dW):
+    res = []
+    for X, H, pbc, sp in cfgs:
+        E, F, W = pair_evaluate(X, sp, H, pbc, KIND_SMOOTH, table, RC)
+        res.append((E, F.T, W))
+    return linear_functional(res, dE, dF, dW)
+
+
+def systems_of(pfw, cfgs, labels=("A", "B")):
+    return [pfw.System(X, H, pbc, [labels[s] for s in sp]) for X, H, pbc, sp in cfgs]
+
+
+@pytest.mark.parametrize("virial", [False, True])
+def test_pair_adjoint_equals_unit_basis_oracle(pfw, small_set, virial):
+    p = pfw.SmoothPair(["A", "B"], A, RC)
+    dE, dF, dW = cotangents(small_set, 1, virial)
+    g = pfw.pair_parameter_vjp(pfw.Workspace(), systems_of(pfw, small_set), p, dE, dF, dW)["amplitudes"]
+    # the independent upper triangle A11, A12, A22; L is linear in it
+    units = [np.array([[1.0, 0], [0, 0]]), np.array([[0, 1.0], [1.0, 0]]), np.array([[0, 0], [0, 1.0]])]
+    ref = np.array([oracle_pair_L(small_set, U, dE, dF, dW) for U in units])
+    np.testing.assert_allclose(g, ref, rtol=1e-10, atol=1e-10 * np.abs(ref).max())
+
+
+def test_pair_adjoint_single_system_and_rrule(pfw):
+    X, H, pbc, sp = pfw.workloads.b2_triclinic(3, 3, 6)
+    sys = pfw.System(X, H, pbc, ["A" if s == 0 else "B" for s in sp])
+    p = pfw.SmoothPair(["A", "B"], A, RC)
+    w = pfw.Workspace()
+    y, back = pfw.rrule_pair_evaluate(w, sys, p)
+    E, F, W = pair_evaluate(X, sp, H, pbc, KIND_SMOOTH, A, RC)
+    assert y["energy"] == pytest.approx(E, rel=1e-12)
+    rng = np.random.default_rng(3)
+    dF = rng.normal(size=(3, len(X)))
+    dW = rng.normal(size=(3, 3))
+    g = back({"energy": 0.5, "forces": dF, "virial": dW})["amplitudes"]
+    units = [np.array([[1.0, 0], [0, 0]]), np.array([[0, 1.0], [1.0, 0]]), np.array([[0, 0], [0, 1.0]])]
+    cfg = [(X, H, pbc, sp)]
+    ref = np.array([oracle_pair_L(cfg, U, [0.5], [dF], [dW]) for U in units])
+    np.testing.assert_allclose(g, ref, rtol=1e-10, atol=1e-10 * np.abs(ref).max())
+    # energy-only cotangent: the reference's energy_parameter_gradient (unit-basis energies)
+    gE = back({"energy": 1.0})["amplitudes"]
+    refE = [pair_evaluate(X, sp, H, pbc, KIND_SMOOTH, U, RC)[0] for U in units]
+    np.testing.assert_allclose(gE, refE, rtol=1e-11)
+
+
+def test_pair_adjoint_rejects(pfw):
+    X, H, pbc, sp = pfw.workloads.argon_fcc(m=3)
+    sys = pfw.System(X, H, pbc, ["Ar"] * len(X))
+    with pytest.raises(pfw.ArgumentError):
+        pfw.pair_parameter_vjp(pfw.Workspace(), sys, pfw.lennard_jones(), 1.0)
+    p = pfw.SmoothPair(["Ar"], [[1.0]], 5.0)
+    with pytest.raises(pfw.ArgumentError):
+        pfw.pair_parameter_vjp(pfw.Workspace(), sys, p, np.nan)
+
+
+def oracle_ace_L(cfgs, om, WB, Wp, dE, dF, dW):
+    m = dict(om, WB=WB, Wpair=Wp)
+    res = []
+    for X, H, pbc, sp in cfgs:
+        E, F, W, _ = oracle_ace(X, sp, H, pbc, m)
+        res.append((E, F.T, W))
+    return linear_functional(res, dE, dF, dW)
+
+
+def ace_labels(model):
+    from pfw_amd.ace import ELEMENTS
+    return [ELEMENTS[z - 1] for z in model.atomic_numbers]
+
+
+def check_ace_gradient(g, cfgs, om, dE, dF, dW, seed, ncomp=6, rtol=1e-8):
+    WB, Wp = om["WB"], om["Wpair"]
+    L0 = oracle_ace_L(cfgs, om, WB, Wp, dE, dF, dW)
+    rng = np.random.default_rng(seed)
+    VB = rng.normal(size=WB.shape) * np.abs(WB).mean()
+    Vp = rng.normal(size=Wp.shape) * (np.abs(Wp).mean() if Wp.size else 0)
+    dL = oracle_ace_L(cfgs, om, WB + VB, Wp + Vp, dE, dF, dW) - L0
+    pred = float(np.sum(g["WB"] * VB) + np.sum(g["Wpair"] * Vp))
+    scale = float(np.sum(np.abs(g["WB"] * VB)) + np.sum(np.abs(g["Wpair"] * Vp)))
+    assert abs(pred - dL) <= rtol * scale, (pred, dL, scale)
+    # single components, including the pair weights
+    picks = [(b, z) for b, z in zip(rng.integers(0, WB.shape[0], ncomp), rng.integers(0, WB.shape[1], ncomp))]
+    for b, z in picks:
+        E1 = WB.copy()
+        E1[b, z] += 1.0
+        d1 = oracle_ace_L(cfgs, om, E1, Wp, dE, dF, dW) - L0
+        assert g["WB"][b, z] == pytest.approx(d1, rel=1e-7, abs=1e-9 * max(1.0, abs(L0)))
+    if Wp.size:
+        q, z = Wp.shape[0] - 1, Wp.shape[1] - 1
+        P1 = Wp.copy()
+        P1[q, z] += 1.0
+        d1 = oracle_ace_L(cfgs, om, WB, P1, dE, dF, dW) - L0
+        assert g["Wpair"][q, z] == pytest.approx(d1, rel=1e-7, abs=1e-9 * max(1.0, abs(L0)))
+
+
+def test_ace_adjoint_synthetic_batch_vs_oracle(pfw, small_set, synthetic):
+    model, ps, om = synthetic
+    calc = pfw.ACECalculator(model, ps, precision="float64")
+    dE, dF, dW = cotangents(small_set, 5)
+    g = pfw.ace_parameter_vjp(calc, systems_of(pfw, small_set, ace_labels(model)), dE, dF, dW)
+    assert g["WB"].shape == model.WB.shape and g["Wpair"].shape == model.Wpair.shape
+    check_ace_gradient(g, small_set, om, dE, dF, dW, seed=6)
+
+
+def test_ace_adjoint_tial_frames_vs_oracle(pfw):
+    from oracle.ace import read_frames
+    path = f"{GOLDEN}/resolved_model_v1.toml"
+    model, ps, _ = pfw.load_resolved(path)
+    om = oracle_load(path)
+    frames = read_frames(f"{GOLDEN}/reference_results.xyz")
+    pick = [f for f in frames if len(f["X"]) == 2][:2] + [f for f in frames if len(f["X"]) > 2][:1]
+    from oracle.ace import channels
+    cfgs = [(f["X"], f["H"], f["pbc"], channels(om, f["labels"])) for f in pick]
+    systems = [pfw.System(f["X"], f["H"], f["pbc"], f["labels"]) for f in pick]
+    calc = pfw.ACECalculator(model, ps, precision="float64")
+    dE, dF, dW = cotangents(cfgs, 7)
+    g = pfw.ace_parameter_vjp(calc, systems, dE, dF, dW)
+    check_ace_gradient(g, cfgs, om, dE, dF, dW, seed=8)
+    # one configuration through the single-system path and the rrule
+    y, back = pfw.rrule_ace_evaluate(calc, systems[2])
+    X, H, pbc, sp = cfgs[2]
+    E, _, _, _ = oracle_ace(X, sp, H, pbc, om)
+    assert y["energy"] == pytest.approx(E, abs=1e-8)
+    g1 = back({"energy": dE[2], "forces": dF[2], "virial": dW[2]})
+    check_ace_gradient(g1, cfgs[2:], om, dE[2:], dF[2:], dW[2:], seed=9, ncomp=3)
+
+
+def test_ace_adjoint_mixed_vs_fp64(pfw, small_set, synthetic):
+    model, ps, _ = synthetic
+    systems = systems_of(pfw, small_set, ace_labels(model))
+    dE, dF, dW = cotangents(small_set, 10)
+    g64 = pfw.ace_parameter_vjp(pfw.ACECalculator(model, ps, precision="float64"), systems, dE, dF, dW)
+    g32 = pfw.ace_parameter_vjp(pfw.ACECalculator(model, ps, precision="mixed"), systems, dE, dF, dW)
+    for k in ("WB", "Wpair"):
+        err = np.linalg.norm(g32[k] - g64[k]) / np.linalg.norm(g64[k])
+        print(f"mixed vs FP64 adjoint {k}: relative error {err:.2e}")
+        assert err <= 1e-4
+
+
+def test_fitting_loss_gradient_central_difference(pfw, synthetic):
+    """Targets from WB' = WB (1 + 0.01 N(0, 1)); the loss is quadratic in WB,
+    so central differences are exact up to rounding."""
+    model, ps, _ = synthetic
+    cfgs, _ = pfw.workloads.fitting_set(ncfg=24, nmin=8, nmax=200, seed=31)
+    systems = systems_of(pfw, cfgs, ace_labels(model))
+    rng = np.random.default_rng(12)
+    WBt = model.WB * (1 + 0.01 * rng.normal(size=model.WB.shape))
+    calc = pfw.ACECalculator(model, {"WB": WBt, "Wpair": model.Wpair}, precision="float64")
+    targets = calc.evaluate_batch(systems)
+    calc.set_parameters(ps)
+    weights = {"energy": 1.0, "forces": 0.5, "virial": 0.25}
+
+    def loss_at(WB):
+        calc.set_parameters({"WB": WB, "Wpair": model.Wpair})
+        return pfw.fitting_loss(calc.evaluate_batch, lambda dE, dF, dW: pfw.ace_parameter_vjp(
+            calc, systems, dE, dF, dW), systems, targets, weights)
+
+    L, g = loss_at(model.WB)
+    assert L > 0
+    V = rng.normal(size=model.WB.shape) * np.abs(model.WB).mean()
+    h = 0.5
+    Lp, _ = loss_at(model.WB + h * V)
+    Lm, _ = loss_at(model.WB - h * V)
+    fd = (Lp - Lm) / (2 * h)
+    pred = float(np.sum(g["WB"] * V))
+    assert pred == pytest.approx(fd, rel=1e-7)
+    # the quadratic's minimum along V: L(theta + t V) with t = -<g, V> / (2 curvature) decreases
+    curv = (Lp + Lm - 2 * L) / (2 * h * h)
+    Lt, _ = loss_at(model.WB - 0.5 * pred / curv * V)
+    assert Lt < L
+    # at the target parameters the loss and its gradient vanish
+    L0, g0 = loss_at(WBt)
+    assert L0 <= 1e-18 * max(1.0, L)
+    assert np.abs(g0["WB"]).max() <= 1e-9 * np.abs(g["WB"]).max()
