@@ -8,9 +8,11 @@ throughput in Matom-evals/s. Reported beside it: the plain-FP64 throughput of
 the same step, the 32,768-atom ACE workload (order-3, lmax-4, degree-12
 synthetic basis on a BCC binary, cutoff 5.0 Å) in both precisions, per-stage
 device times, the roofline of each dominant kernel, accuracy against the FP64
-oracle, and the CPU baseline (the oracle's threaded FP64 restatement).
+oracle, and the CPU baseline (the oracle's threaded FP64 restatement). The
+1,024-configuration fitting set follows: batched pair and ACE steps and the
+parameter adjoints of a least-squares loss over it.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-ace]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-ace] [--no-batch]
 """
 import argparse
 import ctypes
@@ -108,6 +110,90 @@ def accuracy(res, E, F, W, n):
             "energy_per_atom": float(abs(res["energy"] - E) / n), "virial_max": float(np.abs(res["virial"] - W).max())}
 
 
+def bench_fitting_set(pfw, lib, _capi, model, ps, pair_model, args):
+    """1,024 configurations (8..2,048 atoms, log-uniform) packed in one cell list:
+    batched SmoothPair and ACE steps in both precisions, and the parameter
+    adjoints of the loss sum (E - E*)^2 / N^2 + 0.1 sum |F - F*|^2 with targets
+    from WB* = WB (1 + 0.01 N(0, 1))."""
+    from pfw_amd.ace import ELEMENTS
+
+    t0 = time.perf_counter()
+    cfgs, attempts = pfw.workloads.fitting_set()
+    gen_s = time.perf_counter() - t0
+    natoms = sum(len(c[0]) for c in cfgs)
+    lab = [ELEMENTS[z - 1] for z in model.atomic_numbers]
+    sys_pair = [pfw.System(X, H, pbc, ["A" if s == 0 else "B" for s in sp]) for X, H, pbc, sp in cfgs]
+    sys_ace = [pfw.System(X, H, pbc, [lab[s] for s in sp]) for X, H, pbc, sp in cfgs]
+    K, Wm = args.batch_steps, max(1, args.warmup // 2)
+    out = {"workload": "fitting_set_1024", "configurations": len(cfgs), "atoms": natoms,
+           "placement_attempts": attempts, "generation_s": round(gen_s, 3), "unit": "Matom-evals/s"}
+    for prec, mode in (("mixed", _capi.PFW_MIXED), ("float64", _capi.PFW_FP64)):
+        w = pfw.Workspace(precision=prec)
+        pfw.evaluate_batch(w, sys_pair, pair_model)
+        secs, stage = time_steps(lib, w.ctx.h, lib.pfw_pair_step, K, Wm, mode)
+        out[f"pair_{prec}"] = {"value": round(natoms * K / secs / 1e6, 3), "ms_per_step": round(1e3 * secs / K, 4),
+                               "stages_ms": {k: round(float(v), 5) for k, v in zip(STAGES, stage)}}
+        calc = pfw.ACECalculator(model, ps, precision=prec)
+        calc.evaluate_batch(sys_ace)
+        secs, stage = time_steps(lib, calc.workspace.ctx.h, lib.pfw_ace_step, K, Wm, mode)
+        out[f"ace_{prec}"] = {"value": round(natoms * K / secs / 1e6, 4), "ms_per_step": round(1e3 * secs / K, 4),
+                              "stages_ms": {k: round(float(v), 5) for k, v in zip(STAGES, stage)}}
+    # adjoints: loss and gradient with respect to all ACE readout coefficients
+    rng = np.random.default_rng(5)
+    WBt = model.WB * (1 + 0.01 * rng.normal(size=model.WB.shape))
+    calc = pfw.ACECalculator(model, {"WB": WBt, "Wpair": model.Wpair}, precision="mixed")
+    targets = calc.evaluate_batch(sys_ace)
+    calc.set_parameters(ps)
+    weights = {"energy": 1.0, "forces": 0.1, "virial": 0.0}
+
+    def vjp(dE, dF, dW):
+        return pfw.ace_parameter_vjp(calc, sys_ace, dE, dF, dW)
+
+    L, g = pfw.fitting_loss(calc.evaluate_batch, vjp, sys_ace, targets, weights)
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        pfw.fitting_loss(calc.evaluate_batch, vjp, sys_ace, targets, weights)
+        times.append(time.perf_counter() - t0)
+    # the adjoint entry point alone (cotangent upload, environments, adjoint kernel, gradient download)
+    dE = rng.normal(size=len(cfgs))
+    dF = rng.normal(size=(natoms, 3))
+    gWB = np.zeros((model.nz, model.nbasis))
+    gWp = np.zeros((model.nz, max(model.pair.nr, 1)))
+    h = calc.workspace.ctx.h
+    P = ctypes.POINTER(ctypes.c_double)
+    adj = {}
+    for prec, mode in (("mixed", _capi.PFW_MIXED), ("float64", _capi.PFW_FP64)):
+        args_ = (h, mode, dE.ctypes.data_as(P), dF.ctypes.data_as(P), None, gWB.ctypes.data_as(P),
+                 gWp.ctypes.data_as(P))
+        _capi.check(h, lib.pfw_ace_adjoint(*args_))
+        t0 = time.perf_counter()
+        for _ in range(K):
+            _capi.check(h, lib.pfw_ace_adjoint(*args_))
+        adj[prec] = (time.perf_counter() - t0) / K
+    wp = pfw.Workspace(precision="float64")
+    pfw.pair_parameter_vjp(wp, sys_pair, pair_model, dE, [f.T for f in np.split(dF, np.cumsum(
+        [len(c[0]) for c in cfgs])[:-1])])
+    gA = np.zeros(4)
+    hp = wp.ctx.h
+    t0 = time.perf_counter()
+    for _ in range(K):
+        _capi.check(hp, lib.pfw_pair_adjoint(hp, dE.ctypes.data_as(P), dF.ctypes.data_as(P), None,
+                                             gA.ctypes.data_as(P)))
+    pair_adj = (time.perf_counter() - t0) / K
+    out["ace_adjoint"] = {
+        "loss": "sum (E - E*)^2 / N^2 + 0.1 sum |F - F*|^2, WB* = WB (1 + 0.01 N(0,1))",
+        "parameters": int(model.WB.size + model.Wpair.size), "loss_value": L,
+        "loss_and_gradient_ms": round(1e3 * float(np.median(times)), 3),
+        "adjoint_ms": {k: round(1e3 * v, 3) for k, v in adj.items()},
+        "adjoint_value": {k: round(natoms / v / 1e6, 4) for k, v in adj.items()},
+        "gradient_norm": float(np.linalg.norm(g["WB"])),
+    }
+    out["pair_adjoint"] = {"precision": "float64", "adjoint_ms": round(1e3 * pair_adj, 3),
+                           "adjoint_value": round(natoms / pair_adj / 1e6, 3)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,6 +201,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ace", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-steps", type=int, default=10)
     ap.add_argument("--cpu-threads", type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -217,6 +305,10 @@ def main():
             "fp64_kernel_ms": round(ka64, 5),
             "stages_ms": {k: round(float(v), 5) for k, v in zip(STAGES, ace["mixed"]["stage_ms"])},
         }
+
+    # ---------------------------------------------------------------- fitting set (batch + adjoints)
+    if not args.no_batch and not args.no_ace:
+        line["fitting_set"] = bench_fitting_set(pfw, lib, _capi, model, ps, p, args)
 
     # ---------------------------------------------------------------- CPU baseline + accuracy
     if not args.no_cpu_baseline:
