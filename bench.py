@@ -194,6 +194,19 @@ def bench_fitting_set(pfw, lib, _capi, model, ps, pair_model, args):
     return out
 
 
+def calibrate_copy(nbytes=256 << 20, reps=5):
+    """A plain device copy of known size: the reference point that scales the
+    FETCH_SIZE / WRITE_SIZE counters into HBM bytes."""
+    import torch
+
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "reps": reps}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -204,7 +217,12 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=10)
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a device copy of 256 MiB five times (HBM counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (each GPU box has one GPU); "
               "run with --gpus 1", file=sys.stderr)
@@ -229,6 +247,20 @@ def main():
         res = pfw.evaluate_(w, sys_, p)  # uploads geometry + model outside the timed region
         secs, stage = time_steps(lib, w.ctx.h, lib.pfw_pair_step, args.steps, args.warmup, mode)
         pair[prec] = dict(seconds=secs, stage_ms=stage, result=res, ws=w)
+    # host boundary: positions uploaded from and forces returned to pageable host memory every step
+    wh = pair["mixed"]["ws"]
+    Xh = np.ascontiguousarray(X)
+    Fh = np.zeros((n, 3))
+    Eh, Wh = np.zeros(1), np.zeros(9)
+    P = ctypes.POINTER(ctypes.c_double)
+    hh = wh.ctx.h
+    _capi.check(hh, lib.pfw_synchronize(hh))
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        _capi.check(hh, lib.pfw_update_positions(hh, Xh.ctypes.data_as(P)))
+        _capi.check(hh, lib.pfw_pair_evaluate(hh, 7, _capi.PFW_MIXED, Eh.ctypes.data_as(P), Fh.ctypes.data_as(P),
+                                              Wh.ctypes.data_as(P)))
+    host_secs = time.perf_counter() - t0
     npairs = len(pair["mixed"]["ws"].export_pairs()[0])
     pflops = npairs * FLOPS_PER_PAIR["smooth"]
     mixed = pair["mixed"]
@@ -263,6 +295,9 @@ def main():
                  "kernel_ms": round(k64_ms, 5),
                  "roofline_frac": round(pflops / (k64_ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS, 4)},
         "mixed_over_fp64": round(value / fp64_value, 3),
+        "host_boundary": {"value": round(n * args.steps / host_secs / 1e6, 3),
+                          "ms_per_step": round(1e3 * host_secs / args.steps, 4),
+                          "note": "positions host->device and forces device->host (pageable) inside every step"},
         "stages_ms": {k: round(float(v), 5) for k, v in zip(STAGES, mixed["stage_ms"])},
         "cpu_baseline": None,
     }
