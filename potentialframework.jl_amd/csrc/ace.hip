@@ -64,7 +64,7 @@ struct AceArgs {
   int grads;
   double* siteE;
   double* siteW;
-  double* eg;
+  T* eg;  // 3 x E edge gradients in the arithmetic precision
 };
 
 template <typename T>
@@ -409,10 +409,10 @@ __global__ __launch_bounds__(64) void k_ace_site(AceArgs<T> a) {
       g1 += U[m] * G[3 * m + 1];
       g2 += U[m] * G[3 * m + 2];
     }
-    double* go = a.eg + 3 * (int64_t)(e0 + e);
-    go[0] = (double)g0;
-    go[1] = (double)g1;
-    go[2] = (double)g2;
+    T* go = a.eg + 3 * (int64_t)(e0 + e);
+    go[0] = g0;
+    go[1] = g1;
+    go[2] = g2;
     const double gg[3] = {(double)g0, (double)g1, (double)g2};
     for (int bb = 0; bb < 3; ++bb)
       for (int aa = 0; aa < 3; ++aa) wv[aa + 3 * bb] -= R[aa] * gg[bb];
@@ -425,90 +425,182 @@ __global__ __launch_bounds__(64) void k_ace_site(AceArgs<T> a) {
 }
 
 // Directed environments (interface.jl:144-171 + environments.jl:357-391):
-// pass 0 counts per sorted centre, pass 1 writes neighbour and exact R.
-__global__ void k_ace_edges(const Grid* __restrict__ grids, const int32_t* __restrict__ atom_cfg, int64_t n,
-                            const double* __restrict__ xs,
-                            const int32_t* __restrict__ start, const int32_t* __restrict__ cell_of,
-                            const int32_t* __restrict__ perm, int pass, int32_t* __restrict__ ecount,
-                            const int32_t* __restrict__ estart, int32_t* __restrict__ enb, double* __restrict__ eR,
-                            int32_t* __restrict__ maxdeg) {
-  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  const Grid& g = grids[atom_cfg[perm[k]]];
-  const int cell = cell_of[perm[k]];
+// pass 0 counts per sorted centre, pass 1 writes neighbour, stencil entry and
+// exact R. One wavefront per cell: the stencil entries are staged in LDS,
+// lanes hold 64 candidates at a time (in stencil-entry, cell order) and test
+// them against each centre of the cell in turn; a ballot compacts the hits, so
+// every centre's edges come out sorted by (entry, neighbour) without atomics.
+// A centre's write cursor lives in the lane of the same index within its group
+// of 64 centres.
+__global__ __launch_bounds__(64) void k_ace_edges(const Grid* __restrict__ grids, const int32_t* __restrict__ cell_cfg,
+                                                  int64_t n, const double* __restrict__ xs,
+                                                  const int32_t* __restrict__ start, int pass,
+                                                  int32_t* __restrict__ ecount, const int32_t* __restrict__ estart,
+                                                  int32_t* __restrict__ esrc, int32_t* __restrict__ enb,
+                                                  int32_t* __restrict__ eent, double* __restrict__ eR,
+                                                  int32_t* __restrict__ maxdeg) {
+  __shared__ int s_pref[kWave + 1], s_start[kWave], s_lat[kWave][3];
+  __shared__ double s_sh[kWave][3];
+  __shared__ int s_zero;
+  const int cell = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int b = start[cell], cnt = start[cell + 1] - b;
+  if (cnt == 0) return;
+  const Grid& g = grids[cell_cfg[cell]];
   int key[3];
   cell_key(g, cell, key);
-  const double xi = xs[k], yi = xs[n + k], zi = xs[2 * n + k];
-  int64_t pos = pass ? estart[k] : 0;
-  int cnt = 0;
-  for (int e = 0; e < g.nstencil; ++e) {
-    int lat[3];
-    bool zero;
-    const int nc = stencil_entry(g, key, e, lat, &zero);
-    if (nc < 0) continue;
-    double sh[3];
-    exact_shift(g, lat, sh);
-    for (int m = start[nc]; m < start[nc + 1]; ++m) {
-      if (zero && m == k) continue;
-      double d[3];
-      int img[3];
-      if (!exact_member(g, xi, yi, zi, xs[m], xs[n + m], xs[2 * n + m], lat, sh, d, img)) continue;
-      if (pass) {
-        enb[pos] = m;
-        eR[3 * pos] = d[0];
-        eR[3 * pos + 1] = d[1];
-        eR[3 * pos + 2] = d[2];
-        ++pos;
+  for (int ag = 0; ag < cnt; ag += kWave) {
+    const int na = min(kWave, cnt - ag);
+    const int64_t mine = b + ag + lane;
+    int64_t pos = (pass && lane < na) ? estart[mine] : 0;
+    int deg = 0;
+    for (int e0 = 0; e0 < g.nstencil; e0 += kWave) {
+      const int e = e0 + lane;
+      int c = 0;
+      if (lane == 0) s_zero = -1;
+      __syncthreads();
+      if (e < g.nstencil) {
+        int lat[3];
+        bool zero;
+        const int nc = stencil_entry(g, key, e, lat, &zero);
+        if (nc >= 0) {
+          s_start[lane] = start[nc];
+          c = start[nc + 1] - start[nc];
+          double sh[3];
+          exact_shift(g, lat, sh);
+          for (int q = 0; q < 3; ++q) {
+            s_lat[lane][q] = lat[q];
+            s_sh[lane][q] = sh[q];
+          }
+          if (zero) s_zero = lane;
+        }
       }
-      ++cnt;
+      int incl = c;
+      for (int off = 1; off < kWave; off <<= 1) {
+        const int t = __shfl_up(incl, off, kWave);
+        if (lane >= off) incl += t;
+      }
+      s_pref[lane + 1] = incl;
+      if (lane == 0) s_pref[0] = 0;
+      const int total = __shfl(incl, kWave - 1, kWave);
+      __syncthreads();
+      const int ez = s_zero;
+      for (int t0 = 0; t0 < total; t0 += kWave) {
+        const int t = t0 + lane;
+        const bool cand = t < total;
+        int m = 0, lo = 0;
+        double xj = 0, yj = 0, zj = 0;
+        if (cand) {
+          int hi = kWave;
+          while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (s_pref[mid] <= t)
+              lo = mid;
+            else
+              hi = mid;
+          }
+          m = s_start[lo] + (t - s_pref[lo]);
+          xj = xs[m];
+          yj = xs[n + m];
+          zj = xs[2 * n + m];
+        }
+        for (int a = 0; a < na; ++a) {
+          const int64_t k = b + ag + a;
+          bool hit = false;
+          double d[3] = {0, 0, 0};
+          if (cand && !(lo == ez && m == (int)k)) {
+            int img[3];
+            hit = exact_member(g, xs[k], xs[n + k], xs[2 * n + k], xj, yj, zj, s_lat[lo], s_sh[lo], d, img);
+          }
+          const uint64_t mask = __ballot(hit);
+          const int h = __popcll(mask);
+          const int64_t base = __shfl(pos, a, kWave);  // outside the branch: the source lane must be active
+          if (pass && hit) {
+            const int64_t q = base + __popcll(mask & ((1ull << lane) - 1));
+            esrc[q] = (int32_t)k;
+            enb[q] = m;
+            eent[q] = e0 + lo;
+            eR[3 * q] = d[0];
+            eR[3 * q + 1] = d[1];
+            eR[3 * q + 2] = d[2];
+          }
+          if (lane == a) {
+            pos += h;
+            deg += h;
+          }
+        }
+      }
+      __syncthreads();
     }
-  }
-  if (!pass) {
-    ecount[k] = cnt;
-    atomicMax(maxdeg, cnt);
+    if (!pass && lane < na) {
+      ecount[mine] = deg;
+      atomicMax(maxdeg, deg);
+    }
   }
 }
 
 // rev[e] = the edge (j -> i, -R) of edge e = (i -> j, R); R negates exactly.
-__global__ void k_ace_rev(int64_t n, const int32_t* __restrict__ estart, const int32_t* __restrict__ enb,
-                          const double* __restrict__ eR, int32_t* __restrict__ erev, int32_t* __restrict__ status) {
-  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  for (int e = estart[k]; e < estart[k + 1]; ++e) {
-    const int j = enb[e];
-    const double r0 = -eR[3 * e], r1 = -eR[3 * e + 1], r2 = -eR[3 * e + 2];
-    int found = -1;
-    for (int f = estart[j]; f < estart[j + 1]; ++f)
-      if (enb[f] == (int)k && eR[3 * f] == r0 && eR[3 * f + 1] == r1 && eR[3 * f + 2] == r2) {
-        found = f;
-        break;
-      }
-    erev[e] = found;
-    if (found < 0) atomicOr(status, 1);
+// The reverse of a pair found through stencil offset o is found through -o,
+// the mirrored entry nstencil - 1 - entry, and every centre's edges are sorted
+// by (entry, neighbour), so it is located by binary search in j's list (and
+// its displacement checked to be exactly -R). One thread per edge; every
+// erev entry is written (-1 when not found, which also sets the status word).
+__global__ void k_ace_rev(int64_t nedges, const Grid* __restrict__ grids, const int32_t* __restrict__ atom_cfg,
+                          const int32_t* __restrict__ perm, const int32_t* __restrict__ estart,
+                          const int32_t* __restrict__ esrc, const int32_t* __restrict__ enb,
+                          const int32_t* __restrict__ eent, const double* __restrict__ eR,
+                          int32_t* __restrict__ erev, int32_t* __restrict__ status) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nedges) return;
+  const int i = esrc[e];
+  const int j = enb[e];
+  const int want = grids[atom_cfg[perm[i]]].nstencil - 1 - eent[e];
+  int a = estart[j], b = estart[j + 1];  // first f in [a, b) with (eent, enb)[f] >= (want, i)
+  while (a < b) {
+    const int mid = (a + b) >> 1;
+    const int em = eent[mid];
+    if (em < want || (em == want && enb[mid] < i))
+      a = mid + 1;
+    else
+      b = mid;
   }
+  int found = -1;
+  if (a < estart[j + 1] && eent[a] == want && enb[a] == i && eR[3 * a] == -eR[3 * e] &&
+      eR[3 * a + 1] == -eR[3 * e + 1] && eR[3 * a + 2] == -eR[3 * e + 2])
+    found = a;
+  erev[e] = found;
+  if (found < 0) atomicOr(status, 1);
 }
 
 // F_k = sum over k's out-edges of (g_e - g_rev(e)), written in original order.
-__global__ void k_ace_force(int64_t n, const int32_t* __restrict__ estart, const int32_t* __restrict__ erev,
-                            const double* __restrict__ eg, const int32_t* __restrict__ perm, double* __restrict__ F) {
-  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
+// One wavefront per atom, lanes over its edges, fixed-order wave reduction.
+template <typename G>
+__global__ __launch_bounds__(64) void k_ace_force(int64_t n, const int32_t* __restrict__ estart,
+                                                  const int32_t* __restrict__ erev, const G* __restrict__ eg,
+                                                  const int32_t* __restrict__ perm, double* __restrict__ F) {
+  const int64_t k = blockIdx.x;
+  const int lane = threadIdx.x;
   double f0 = 0, f1 = 0, f2 = 0;
-  for (int e = estart[k]; e < estart[k + 1]; ++e) {
+  for (int e = estart[k] + lane; e < estart[k + 1]; e += kWave) {
     const int r = erev[e];  // -1 only when the status word already reports the failure
-    f0 += eg[3 * e];
-    f1 += eg[3 * e + 1];
-    f2 += eg[3 * e + 2];
+    f0 += (double)eg[3 * e];
+    f1 += (double)eg[3 * e + 1];
+    f2 += (double)eg[3 * e + 2];
     if (r >= 0) {
-      f0 -= eg[3 * r];
-      f1 -= eg[3 * r + 1];
-      f2 -= eg[3 * r + 2];
+      f0 -= (double)eg[3 * r];
+      f1 -= (double)eg[3 * r + 1];
+      f2 -= (double)eg[3 * r + 2];
     }
   }
-  const int64_t o = perm[k];
-  F[3 * o] = f0;
-  F[3 * o + 1] = f1;
-  F[3 * o + 2] = f2;
+  f0 = wave_sum(f0);
+  f1 = wave_sum(f1);
+  f2 = wave_sum(f2);
+  if (lane == 0) {
+    const int64_t o = perm[k];
+    F[3 * o] = f0;
+    F[3 * o + 1] = f1;
+    F[3 * o + 2] = f2;
+  }
 }
 
 // siteE and siteW packed as 10-column records for the two-stage reduction.
@@ -522,10 +614,14 @@ __global__ void k_ace_pack(int64_t n, const double* __restrict__ siteE, const do
 
 __global__ void k_ace_finish(const double* __restrict__ part, int nb, const int32_t* __restrict__ status,
                              double* __restrict__ out) {
+  __shared__ double t[10];
+  if (threadIdx.x < 10) {  // one lane per column, rows in fixed order
+    double v = 0;
+    for (int b = 0; b < nb; ++b) v += part[10 * b + threadIdx.x];
+    t[threadIdx.x] = v;
+  }
+  __syncthreads();
   if (threadIdx.x != 0) return;
-  double t[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-  for (int b = 0; b < nb; ++b)
-    for (int c = 0; c < 10; ++c) t[c] += part[10 * b + c];
   bool bad = status[0] != 0;
   for (int c = 0; c < 10; ++c) {
     out[c] = t[c];
@@ -713,6 +809,8 @@ void ace_free_edges(pfw_ctx* ctx) {
   dfree(a.enb);
   dfree(a.eR);
   dfree(a.erev);
+  dfree(a.eent);
+  dfree(a.esrc);
   dfree(a.eg);
   dfree(a.siteE);
   dfree(a.siteW);
@@ -991,7 +1089,7 @@ static AceArgs<T> make_args(pfw_ctx* ctx, int grads) {
   a.grads = grads;
   a.siteE = ad.siteE;
   a.siteW = ad.siteW;
-  a.eg = ad.eg;
+  a.eg = (T*)ad.eg;
   return a;
 }
 
@@ -1051,8 +1149,9 @@ static void ace_environment(pfw_ctx* ctx, int rev) {
     PFW_HIP(hipMalloc(&ad.status, 4 * sizeof(int32_t)));
   }
   PFW_HIP(hipMemsetAsync(ad.status, 0, 4 * sizeof(int32_t), st));
-  hipLaunchKernelGGL(k_ace_edges, dim3(nblk(n, 128)), dim3(128), 0, st, s.grids, s.atom_cfg, n, s.xs, s.start,
-                     s.cell_of, s.perm, 0, ad.ecount, (const int32_t*)nullptr, (int32_t*)nullptr, (double*)nullptr, ad.status + 1);
+  hipLaunchKernelGGL(k_ace_edges, dim3((unsigned)ctx->ncells), dim3(kWave), 0, st, s.grids, s.cell_cfg, n, s.xs,
+                     s.start, 0, ad.ecount, (const int32_t*)nullptr, (int32_t*)nullptr, (int32_t*)nullptr,
+                     (int32_t*)nullptr, (double*)nullptr, ad.status + 1);
   PFW_HIP(hipGetLastError());
   scan_counts(ctx, ad.ecount, (int)n, ad.estart, ad.ebs, ad.ecount /* cursor: reused, already consumed */);
   int32_t total = 0, maxdeg = 0;
@@ -1065,20 +1164,25 @@ static void ace_environment(pfw_ctx* ctx, int rev) {
     dfree(ad.enb);
     dfree(ad.eR);
     dfree(ad.erev);
+    dfree(ad.eent);
+    dfree(ad.esrc);
     dfree(ad.eg);
     int64_t cap = std::max<int64_t>((int64_t)(total * 1.25) + 64, 64);
     PFW_HIP(hipMalloc(&ad.enb, cap * sizeof(int32_t)));
     PFW_HIP(hipMalloc(&ad.eR, 3 * cap * sizeof(double)));
     PFW_HIP(hipMalloc(&ad.erev, cap * sizeof(int32_t)));
+    PFW_HIP(hipMalloc(&ad.eent, cap * sizeof(int32_t)));
+    PFW_HIP(hipMalloc(&ad.esrc, cap * sizeof(int32_t)));
     PFW_HIP(hipMalloc(&ad.eg, 3 * cap * sizeof(double)));
     ad.edge_cap = cap;
   }
-  hipLaunchKernelGGL(k_ace_edges, dim3(nblk(n, 128)), dim3(128), 0, st, s.grids, s.atom_cfg, n, s.xs, s.start,
-                     s.cell_of, s.perm, 1, ad.ecount, (const int32_t*)ad.estart, ad.enb, ad.eR, ad.status + 1);
+  hipLaunchKernelGGL(k_ace_edges, dim3((unsigned)ctx->ncells), dim3(kWave), 0, st, s.grids, s.cell_cfg, n, s.xs,
+                     s.start, 1, ad.ecount, (const int32_t*)ad.estart, ad.esrc, ad.enb, ad.eent, ad.eR,
+                     ad.status + 1);
   PFW_HIP(hipGetLastError());
   if (rev)
-    hipLaunchKernelGGL(k_ace_rev, dim3(nblk(n, 128)), dim3(128), 0, st, n, ad.estart, ad.enb, ad.eR, ad.erev,
-                       ad.status);
+    hipLaunchKernelGGL(k_ace_rev, dim3(nblk(total, 256)), dim3(256), 0, st, (int64_t)total, s.grids, s.atom_cfg,
+                       s.perm, ad.estart, ad.esrc, ad.enb, ad.eent, ad.eR, ad.erev, ad.status);
 }
 
 void ace_run(pfw_ctx* ctx, uint32_t mask, int precision) {
@@ -1106,9 +1210,12 @@ void ace_run(pfw_ctx* ctx, uint32_t mask, int precision) {
     else
       dispatch_sites<double>(ctx, grads);
     record_stage(ctx, 5);
-    if (grads)
-      hipLaunchKernelGGL(k_ace_force, dim3(nblk(n, 256)), dim3(256), 0, st, n, ad.estart, ad.erev, ad.eg, s.perm,
-                         ctx->d_F);
+    if (grads && precision == PFW_MIXED)
+      hipLaunchKernelGGL(k_ace_force<float>, dim3((unsigned)n), dim3(kWave), 0, st, n, ad.estart, ad.erev,
+                         (const float*)ad.eg, s.perm, ctx->d_F);
+    else if (grads)
+      hipLaunchKernelGGL(k_ace_force<double>, dim3((unsigned)n), dim3(kWave), 0, st, n, ad.estart, ad.erev,
+                         (const double*)ad.eg, s.perm, ctx->d_F);
     hipLaunchKernelGGL(k_ace_pack, dim3(nblk(n, 256)), dim3(256), 0, st, n, ad.siteE, ad.siteW, grads, ad.rec);
     if (ctx->ncfg == 1) {
       hipLaunchKernelGGL(k_colsum, dim3(kRedBlocks), dim3(256), 0, st, ad.rec, n, 10, ctx->d_red);

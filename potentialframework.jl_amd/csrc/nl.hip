@@ -431,8 +431,10 @@ __global__ void k_scatter(int64_t n, const int32_t* __restrict__ cell_of, const 
 // Deterministic order inside each cell: ascending original index (the stable
 // sortperm of storage.jl:199), then the gather into sorted SoA coordinates.
 // One wavefront per cell: cells of up to 64 atoms are sorted with a register
-// bitonic network over lane shuffles; larger cells fall back to an insertion
-// sort by lane 0 (dense clusters only).
+// bitonic network over lane shuffles, cells of up to kRankSort atoms by rank
+// counting over an LDS copy (indices are distinct), larger cells by an
+// insertion sort in lane 0 (dense clusters only).
+constexpr int kRankSort = 2048;
 __global__ __launch_bounds__(64) void k_cell_sort_gather(int ncells, int64_t n, const int32_t* __restrict__ start,
                                                          int32_t* __restrict__ perm, const double* __restrict__ xw,
                                                          const int32_t* __restrict__ species, double* __restrict__ xs,
@@ -451,6 +453,17 @@ __global__ __launch_bounds__(64) void k_cell_sort_gather(int ncells, int64_t n, 
         v = (lower == up) ? min(v, o) : max(v, o);
       }
     if (lane < cnt) perm[b + lane] = v;
+  } else if (cnt <= kRankSort) {
+    __shared__ int buf[kRankSort];
+    for (int k = lane; k < cnt; k += kWave) buf[k] = perm[b + k];
+    __syncthreads();
+    for (int k = lane; k < cnt; k += kWave) {
+      const int v = buf[k];
+      int rank = 0;
+      for (int t = 0; t < cnt; ++t) rank += buf[t] < v;
+      perm[b + rank] = v;
+    }
+    __syncthreads();
   } else {
     if (lane == 0)
       for (int k = b + 1; k < e; ++k) {

@@ -240,10 +240,14 @@ __global__ __launch_bounds__(64) void k_pair(const Grid* __restrict__ grids, con
 // Final FP64 reduction of the stage-1 column sums into E, W (3x3 column-major)
 // and the nonfinite status word (gpu.jl:390-395).
 __global__ void k_reduce_finish(const double* __restrict__ part, int nb, double* __restrict__ out) {
+  __shared__ double t[7];
+  if (threadIdx.x < 7) {  // one lane per column, rows in fixed order
+    double v = 0;
+    for (int b = 0; b < nb; ++b) v += part[7 * b + threadIdx.x];
+    t[threadIdx.x] = v;
+  }
+  __syncthreads();
   if (threadIdx.x != 0) return;
-  double t[7] = {0, 0, 0, 0, 0, 0, 0};
-  for (int b = 0; b < nb; ++b)
-    for (int k = 0; k < 7; ++k) t[k] += part[7 * b + k];
   const double E = 0.5 * t[0];
   // P = sum d (x) g over directed pairs: xx yy zz xy xz yz
   double W[9];

@@ -137,6 +137,8 @@ struct AceDev {
   int32_t* enb = nullptr;     // neighbour (sorted index)
   double* eR = nullptr;       // 3 x E
   int32_t* erev = nullptr;
+  int32_t* eent = nullptr;
+  int32_t* esrc = nullptr;    // centre (sorted index) of each edge    // stencil entry of each edge (edges sorted by entry, neighbour)
   double* eg = nullptr;       // 3 x E edge gradients
   double* siteE = nullptr;    // n (sorted order)
   double* siteW = nullptr;    // 9 x n
