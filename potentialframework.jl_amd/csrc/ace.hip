@@ -55,12 +55,14 @@ struct AceArgs {
   const int32_t* rlb;     // L + 2: radial indices grouped by l
   const int32_t* rl;      // (radial index, first density of its m-block)
   const int32_t* prod;
-  int nseg;
-  const int32_t* segb;    // nseg + 1: entry ranges of the dE/dA row segments
-  const int32_t* rowseg;  // ndens + 1: segment ranges of each density row
-  const int32_t* dent;
   const T* w;   // nz x nprod
   const T* wp;  // nz x nq
+  int nsteps;          // product schedule: nsteps x 64 slots
+  const uint4* sched;  // per slot: 4 x u16 factor indices (0xffff = none), 4 x u16 targets
+  const T* ws;         // nz x slots: readout weight of each slot's product
+  const int32_t* copyb;  // ndens + 1: accumulator copies of each density
+  int ntarget;
+  T* Abuf;              // n x ndens densities, then dE/dA in place
   int grads;
   double* siteE;
   double* siteW;
@@ -73,6 +75,8 @@ __device__ inline T ipow(T x, int n) {
   for (int k = 0; k < n; ++k) v *= x;
   return v;
 }
+
+__host__ __device__ inline size_t align16(size_t b) { return (b + 15) & ~(size_t)15; }
 
 // Agnesi coordinate x(r), dx/dr, envelope and its r-derivative, spline
 // interval and local coordinate (resolved.jl:156-250).
@@ -245,24 +249,21 @@ __device__ inline void ylm(T dx, T dy, T dz, const T* Fl, T* Y, T* G) {
 #undef QI
 }
 
-// One wavefront holds SPW sites (SPW = 1 or 2); each site owns LPS = 64 / SPW
-// consecutive lanes. With SPW = 2 every site has at most 32 edges (checked on
-// the host), so each site's edges fit one chunk.
-template <typename T, int L, int SPW>
-__global__ __launch_bounds__(64) void k_ace_site(AceArgs<T> a) {
-  extern __shared__ __align__(16) unsigned char smem[];
+// The site pipeline runs as three kernels:
+//   k_ace_dens   per-edge radials and Y_lm, densities A (site x ndens) and the
+//                pair-radial energy of each site;
+//   k_ace_prod   products, readout and dE/dA with the model tables in LDS,
+//                dE/dA written over A in place;
+//   k_ace_pull   per-edge force pullback and site virials from dE/dA.
+// k_ace_dens and k_ace_pull give each wavefront SPW sites (SPW = 1 or 2); each
+// site owns LPS = 64 / SPW consecutive lanes, one edge per lane, looping over
+// chunks of LPS edges.
+
+// Fl[l][m] = sqrt((2l+1)/2pi) * prod(-1/sqrt((l+m')(l+1-m'))) (angular_blocks.jl:25-34).
+template <typename T, int L>
+__device__ inline void fill_fl(T* Fl) {
   constexpr int NY = (L + 1) * (L + 1);
-  constexpr int LPS = kWave / SPW;
-  T* Fl = (T*)smem;                      // (L+1)^2
-  T* A = Fl + NY;                        // SPW x ndens
-  T* dA = A + SPW * a.ndens;             // SPW x ndens
-  T* part = dA + SPW * a.ndens;          // SPW x nseg
-  T* Rb = part + SPW * a.nseg;           // nr x kStride
-  T* Yb = Rb + a.rad.nr * kStride;       // NY x kStride
   const int lane = threadIdx.x;
-  const int sl = lane / LPS, ll = lane % LPS;
-  const int64_t site = (int64_t)blockIdx.x * SPW + sl;
-  const bool valid = site < a.n;
   if (lane < NY) {
     int l = 0;
     while ((l + 1) * (l + 1) <= lane) ++l;
@@ -273,10 +274,24 @@ __global__ __launch_bounds__(64) void k_ace_site(AceArgs<T> a) {
       Fl[l * (L + 1) + m] = (T)F;
     }
   }
+}
+
+template <typename T, int L, int SPW>
+__global__ __launch_bounds__(64) void k_ace_dens(AceArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int NY = (L + 1) * (L + 1);
+  constexpr int LPS = kWave / SPW;
+  T* Fl = (T*)smem;                      // (L+1)^2
+  T* A = Fl + NY;                        // SPW x ndens
+  T* Rb = A + SPW * a.ndens;             // nr x kStride
+  T* Yb = Rb + a.rad.nr * kStride;       // NY x kStride
+  const int lane = threadIdx.x;
+  const int sl = lane / LPS, ll = lane % LPS;
+  const int64_t site = (int64_t)blockIdx.x * SPW + sl;
+  const bool valid = site < a.n;
+  fill_fl<T, L>(Fl);
   for (int k = lane; k < SPW * a.ndens; k += kWave) A[k] = 0;
   T* As = A + sl * a.ndens;
-  T* dAs = dA + sl * a.ndens;
-  T* ps = part + sl * a.nseg;
   const int iz = valid ? a.sps[site] : 0;
   const int e0 = valid ? a.estart[site] : 0;
   const int ne = valid ? a.estart[site + 1] - e0 : 0;
@@ -286,7 +301,7 @@ __global__ __launch_bounds__(64) void k_ace_site(AceArgs<T> a) {
   const int col0 = sl * LPS;
   double epair = 0;
   __syncthreads();
-  // ---- phase A: densities A[k] = sum_e R[e, r_k] Y[e, m_k]
+  // A[k] = sum_e R[e, r_k] Y[e, m_k]
   for (int c0 = 0; c0 < nemax; c0 += LPS) {
     const int e = c0 + ll;
     if (e < ne) {
@@ -326,43 +341,116 @@ __global__ __launch_bounds__(64) void k_ace_site(AceArgs<T> a) {
     }
     __syncthreads();
   }
-  // ---- phase B: products, energy and dE/dA over balanced row segments
-  const T* w = a.w + (int64_t)iz * a.nprod;
-  T eprod = 0;
-  for (int p = ll; p < a.nprod; p += LPS) {
-    const int* f = a.prod + 4 * p;
-    T v = As[f[0]];
-    if (f[1] >= 0) v *= As[f[1]];
-    if (f[2] >= 0) v *= As[f[2]];
-    if (f[3] >= 0) v *= As[f[3]];
-    eprod += w[p] * v;
+  if (valid) {
+    for (int k = ll; k < a.ndens; k += LPS) a.Abuf[site * a.ndens + k] = As[k];
+    for (int off = LPS / 2; off > 0; off >>= 1) epair += __shfl_xor(epair, off, kWave);
+    if (ll == 0) a.siteE[site] = epair;
   }
-  if (a.grads) {
-    for (int s = ll; s < a.nseg; s += LPS) {
-      T acc = 0;
-      for (int q = a.segb[s]; q < a.segb[s + 1]; ++q) {
-        const int* d = a.dent + 4 * q;
-        T v = w[d[0]];
-        if (d[1] >= 0) v *= As[d[1]];
-        if (d[2] >= 0) v *= As[d[2]];
-        if (d[3] >= 0) v *= As[d[3]];
-        acc += v;
-      }
-      ps[s] = acc;
-    }
-    __syncthreads();
-    for (int k = ll; k < a.ndens; k += LPS) {
-      T acc = 0;
-      for (int s = a.rowseg[k]; s < a.rowseg[k + 1]; ++s) acc += ps[s];
-      dAs[k] = acc;
-    }
-  }
-  double es = (double)eprod + epair;
-  for (int off = LPS / 2; off > 0; off >>= 1) es += __shfl_xor(es, off, kWave);
-  if (valid && ll == 0) a.siteE[site] = a.reference[iz] + es;
-  if (!a.grads) return;
+}
+
+// Products P_k = prod A[f], E = ref + pair + sum_k w_k P_k, and dE/dA
+// (projection.jl:39-56; cpu.jl:214-236), forward and reverse in one pass.
+// The products run in a host-built schedule: steps of 64 lanes, each lane one
+// product with its factors in slots 0..3 and, per slot, a target accumulator
+// for that factor's partial derivative. A density owns several accumulators
+// (copies) so that within one step and slot no two lanes share a target: the
+// partials are plain read-modify-writes in LDS, race-free without atomics, and
+// each accumulator sums in schedule order (deterministic). The schedule also
+// keeps gathers of A and the accumulator updates in distinct LDS banks as far
+// as it can. dE/dA[a] is the sum of a's copies. The schedule, copy offsets and
+// the readout w (in schedule order, per species) are staged in LDS once per
+// workgroup; each wavefront then walks sites with a grid stride, one site at a
+// time, with no workgroup barrier inside the walk.
+template <typename T>
+__global__ __launch_bounds__(1024) void k_ace_prod(AceArgs<T> a, int grads) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr unsigned kNone = 0xffff;
+  const int nwaves = blockDim.x / kWave;
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const int nslot = a.nsteps * kWave;
+  T* ws = (T*)smem;                                                   // nz x nslot
+  uint4* sched = (uint4*)(smem + align16(sizeof(T) * a.nz * nslot));  // nslot: factors and targets (u16)
+  int* cb = (int*)(sched + nslot);                                    // ndens + 1 copy offsets
+  unsigned char* pw = smem + align16((size_t)((unsigned char*)(cb + a.ndens + 1) - smem));
+  const size_t per_wave = align16(sizeof(T) * a.ndens) + align16(sizeof(T) * a.ntarget);
+  T* A = (T*)(pw + wave * per_wave);
+  T* acc = (T*)((unsigned char*)A + align16(sizeof(T) * a.ndens));
+  for (int i = threadIdx.x; i < a.nz * nslot; i += blockDim.x) ws[i] = a.ws[i];
+  for (int i = threadIdx.x; i < nslot; i += blockDim.x) sched[i] = a.sched[i];
+  for (int i = threadIdx.x; i <= a.ndens; i += blockDim.x) cb[i] = a.copyb[i];
   __syncthreads();
-  // ---- phase C: force pullback, g = r^ sum_r R'_r T_r + sum_m U_m grad Y_m
+  for (int64_t site = (int64_t)blockIdx.x * nwaves + wave; site < a.n; site += (int64_t)gridDim.x * nwaves) {
+    T* Ag = a.Abuf + site * a.ndens;
+    for (int k = lane; k < a.ndens; k += kWave) A[k] = Ag[k];
+    if (grads)
+      for (int k = lane; k < a.ntarget; k += kWave) acc[k] = 0;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int iz = a.sps[site];
+    const T* wz = ws + iz * nslot;
+    double eprod = 0;
+    for (int st = 0; st < a.nsteps; ++st) {
+      const int slot = st * kWave + lane;
+      const uint4 f = sched[slot];
+      const unsigned f0 = f.x & 0xffff, f1 = f.x >> 16, f2 = f.y & 0xffff, f3 = f.y >> 16;
+      if (f0 == kNone) continue;  // padding
+      const T w = wz[slot];
+      const T x0 = A[f0];
+      const T x1 = f1 != kNone ? A[f1] : (T)1;
+      const T x2 = f2 != kNone ? A[f2] : (T)1;
+      const T x3 = f3 != kNone ? A[f3] : (T)1;
+      const T p01 = x0 * x1, p23 = x2 * x3;
+      eprod += (double)(w * p01 * p23);
+      if (grads) {
+        acc[f.z & 0xffff] += w * x1 * p23;
+        if (f1 != kNone) acc[f.z >> 16] += w * x0 * p23;
+        if (f2 != kNone) acc[f.w & 0xffff] += w * p01 * x3;
+        if (f3 != kNone) acc[f.w >> 16] += w * p01 * x2;
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (grads)
+      for (int k = lane; k < a.ndens; k += kWave) {
+        T d = 0;
+        for (int c = cb[k]; c < cb[k + 1]; ++c) d += acc[c];
+        Ag[k] = d;  // dE/dA over A
+      }
+    const double es = wave_sum(eprod);
+    if (lane == 0) a.siteE[site] = a.reference[iz] + a.siteE[site] + es;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+}
+
+// Force pullback g_e = r^ sum_r R'_r T_r + sum_m U_m grad Y_m with
+// T_r = sum_m dE/dA[r, m] Y_m and U_m = sum_r dE/dA[r, m] R_r (serial.jl:85-101;
+// cpu.jl:226-236), plus the pair radials; edge gradients and site virials.
+template <typename T, int L, int SPW>
+__global__ __launch_bounds__(64) void k_ace_pull(AceArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int NY = (L + 1) * (L + 1);
+  constexpr int LPS = kWave / SPW;
+  T* Fl = (T*)smem;                      // (L+1)^2
+  T* dA = Fl + NY;                       // SPW x ndens
+  const int lane = threadIdx.x;
+  const int sl = lane / LPS, ll = lane % LPS;
+  const int64_t site = (int64_t)blockIdx.x * SPW + sl;
+  const bool valid = site < a.n;
+  fill_fl<T, L>(Fl);
+  T* dAs = dA + sl * a.ndens;
+  if (valid)
+    for (int k = ll; k < a.ndens; k += LPS) dAs[k] = a.Abuf[site * a.ndens + k];
+  const int iz = valid ? a.sps[site] : 0;
+  const int e0 = valid ? a.estart[site] : 0;
+  const int ne = valid ? a.estart[site + 1] - e0 : 0;
+  int nemax = ne;
+  for (int off = LPS; off < kWave; off <<= 1) nemax = max(nemax, __shfl_xor(nemax, off, kWave));
+  const int nq = a.pair.nr;
+  __syncthreads();
   double wv[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (int c0 = 0; c0 < nemax; c0 += LPS) {
     const int e = c0 + ll;
@@ -813,6 +901,7 @@ void ace_free_edges(pfw_ctx* ctx) {
   dfree(a.esrc);
   dfree(a.eg);
   dfree(a.siteE);
+  dfree(a.Abuf);
   dfree(a.siteW);
   dfree(a.rec);
   dfree(a.status);
@@ -826,9 +915,10 @@ void ace_free(pfw_ctx* ctx) {
   free_family(a.pair);
   dfree(a.reference);
   dfree(a.prod);
-  dfree(a.segb);
-  dfree(a.rowseg);
-  dfree(a.dent);
+  dfree(a.sched);
+  dfree(a.copyb);
+  dfree(a.ws64);
+  dfree(a.ws32);
   dfree(a.dens);
   dfree(a.rlb);
   dfree(a.rl);
@@ -901,6 +991,100 @@ static void load_family(const pfw_ace_radial& src, int nz, AceFamilyDev& f, doub
   f.coef32 = upload(coef32, st);
 }
 
+// Schedule of the product basis for k_ace_prod. Steps of 64 lanes; each lane
+// takes one product (padding only where nothing fits) with its factors
+// permuted into slots 0..3 and, per slot, an accumulator copy of that factor's
+// density that no other lane of the step uses in the same slot (a density a
+// has copies[a] = ceil(occurrences / ideal steps) accumulators). Among the
+// next candidates in product order a lane takes the first whose gathers of A
+// (same index or a free bank) and accumulator updates (a free bank) are
+// conflict-free within its 32-lane half, else the one with fewest conflicts.
+struct ProductSlot {
+  int32_t k = -1;
+  int32_t f[4] = {-1, -1, -1, -1};
+  int32_t t[4] = {-1, -1, -1, -1};
+};
+
+static std::vector<ProductSlot> schedule_products(const std::vector<std::vector<int>>& factors, int ndens,
+                                                  std::vector<int32_t>& copyb) {
+  const int nprod = (int)factors.size();
+  const int ideal = std::max(1, (nprod + kWave - 1) / kWave);
+  std::vector<int> occ(ndens, 0);
+  for (const auto& f : factors)
+    for (int d : f) ++occ[d];
+  copyb.assign(ndens + 1, 0);
+  for (int d = 0; d < ndens; ++d) copyb[d + 1] = copyb[d] + std::max(1, (occ[d] + ideal - 1) / ideal);
+  std::vector<int> remaining(nprod);
+  for (int k = 0; k < nprod; ++k) remaining[k] = k;
+  std::vector<ProductSlot> out;
+  constexpr size_t kWindow = 512;
+  std::vector<std::vector<int>> cnt(4, std::vector<int>(ndens));
+  while (!remaining.empty()) {
+    for (auto& c : cnt) std::fill(c.begin(), c.end(), 0);
+    int abank[2][4][32];
+    for (auto& h : abank)
+      for (auto& t : h)
+        for (int& v : t) v = -1;
+    uint32_t tbank[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+    std::vector<char> taken(remaining.size(), 0);
+    for (int lane = 0; lane < kWave; ++lane) {
+      const int half = lane / 32;
+      ProductSlot best;
+      int best_cost = 1 << 30;
+      size_t best_r = remaining.size(), seen = 0;
+      for (size_t r = 0; r < remaining.size() && seen < kWindow && best_cost > 0; ++r) {
+        if (taken[r]) continue;
+        ++seen;
+        const std::vector<int>& f = factors[remaining[r]];
+        const int nu = (int)f.size();
+        int perm[4] = {0, 1, 2, 3};
+        do {
+          int cost = 0;
+          bool fits = true;
+          ProductSlot cand;
+          cand.k = remaining[r];
+          for (int t = 0; t < nu && fits; ++t) {
+            const int d = f[perm[t]];
+            const int rank = cnt[t][d];
+            if (copyb[d] + rank >= copyb[d + 1]) {
+              fits = false;
+              break;
+            }
+            const int tgt = copyb[d] + rank;
+            const int ab = abank[half][t][d & 31];
+            cost += (ab >= 0 && ab != d) ? 1 : 0;
+            cost += ((tbank[half][t] >> (tgt & 31)) & 1u) ? 1 : 0;
+            cand.f[t] = d;
+            cand.t[t] = tgt;
+          }
+          if (fits && cost < best_cost) {
+            best_cost = cost;
+            best = cand;
+            best_r = r;
+            if (cost == 0) break;
+          }
+        } while (std::next_permutation(perm, perm + nu));
+      }
+      if (best_r < remaining.size()) {
+        for (int t = 0; t < 4 && best.f[t] >= 0; ++t) {
+          const int d = best.f[t];
+          ++cnt[t][d];
+          if (abank[half][t][d & 31] < 0) abank[half][t][d & 31] = d;
+          tbank[half][t] |= 1u << (best.t[t] & 31);
+        }
+        taken[best_r] = 1;
+      }
+      out.push_back(best);
+    }
+    std::vector<int> rest;
+    for (size_t r = 0; r < remaining.size(); ++r)
+      if (!taken[r]) rest.push_back(remaining[r]);
+    if (rest.size() == remaining.size()) throw Error(PFW_INVALID_ARGUMENT, "product schedule failed");
+    remaining.swap(rest);
+  }
+  return out;
+}
+
 void ace_set_parameters(pfw_ctx* ctx, const double* WB, const double* Wpair) {
   AceDev& a = ctx->ace;
   if (!a.ready && a.nprod == 0) throw Error(PFW_STALE, "no ACE model set");
@@ -920,6 +1104,16 @@ void ace_set_parameters(pfw_ctx* ctx, const double* WB, const double* Wpair) {
       wp[(size_t)z * nq + q] = v;
     }
   std::vector<float> w32(w.begin(), w.end()), wp32(wp.begin(), wp.end());
+  const size_t nslot = a.slot_prod.size();  // readout weights in schedule order
+  std::vector<double> ws((size_t)nz * std::max<size_t>(nslot, 1), 0.0);
+  for (int z = 0; z < nz; ++z)
+    for (size_t q = 0; q < nslot; ++q)
+      if (a.slot_prod[q] >= 0) ws[z * nslot + q] = w[(size_t)z * a.nprod + a.slot_prod[q]];
+  std::vector<float> ws32(ws.begin(), ws.end());
+  dfree(a.ws64);
+  dfree(a.ws32);
+  a.ws64 = upload(ws, ctx->stream);
+  a.ws32 = upload(ws32, ctx->stream);
   dfree(a.w64);
   dfree(a.w32);
   dfree(a.wp64);
@@ -985,11 +1179,10 @@ void ace_set_model(pfw_ctx* ctx, const pfw_ace_model* m) {
   a.dens = upload(dens, st);
   a.rlb = upload(rlb, st);
   a.rl = upload(rl, st);
-  // products and the per-density derivative rows, cut into segments of at most
-  // kSegLen entries so that the lanes of the dE/dA phase stay balanced
-  constexpr int kSegLen = 8;
+  // products (factor lists, -1 padded) and their conflict-free schedule
+  if (m->ndens >= 0xffff) throw Error(PFW_INVALID_ARGUMENT, "more than 65534 densities are not supported on the device");
   std::vector<int32_t> prod(4 * (size_t)m->nprod, -1);
-  std::vector<std::vector<std::array<int32_t, 4>>> rows(m->ndens);
+  std::vector<std::vector<int>> factors(m->nprod);
   for (int k = 0; k < m->nprod; ++k) {
     int b = m->product_ptr[k], nu = m->product_ptr[k + 1] - b;
     if (nu <= 0 || nu > 4) throw Error(PFW_INVALID_ARGUMENT, "product orders must be 1..4 on the device");
@@ -997,27 +1190,26 @@ void ace_set_model(pfw_ctx* ctx, const pfw_ace_model* m) {
       int d = m->product_idx[b + t];
       if (d < 0 || d >= m->ndens) throw Error(PFW_INVALID_ARGUMENT, "invalid product indices");
       prod[4 * (size_t)k + t] = d;
-    }
-    for (int t = 0; t < nu; ++t) {  // d P / d A_{idx[t]}: product of the other factors
-      std::array<int32_t, 4> ent{k, -1, -1, -1};
-      int o = 1;
-      for (int s2 = 0; s2 < nu; ++s2)
-        if (s2 != t) ent[o++] = m->product_idx[b + s2];
-      rows[m->product_idx[b + t]].push_back(ent);
+      factors[k].push_back(d);
     }
   }
-  std::vector<int32_t> segb(1, 0), rowseg(m->ndens + 1, 0), dent;
-  for (int d = 0; d < m->ndens; ++d) {
-    int len = (int)rows[d].size();
-    for (int q = 0; q < len; q += kSegLen) segb.push_back(segb.back() + std::min(kSegLen, len - q));
-    rowseg[d + 1] = (int32_t)segb.size() - 1;
-    for (auto& e : rows[d]) dent.insert(dent.end(), e.begin(), e.end());
-  }
-  a.nseg = (int)segb.size() - 1;
   a.prod = upload(prod, st);
-  a.segb = upload(segb, st);
-  a.rowseg = upload(rowseg, st);
-  a.dent = upload(dent, st);
+  std::vector<int32_t> copyb;
+  std::vector<ProductSlot> slots = schedule_products(factors, m->ndens, copyb);
+  a.nsteps = (int)(slots.size() / kWave);
+  a.ntarget = copyb[m->ndens];
+  if (a.ntarget >= 0xffff) throw Error(PFW_INVALID_ARGUMENT, "product schedule needs too many accumulators");
+  a.slot_prod.resize(slots.size());
+  std::vector<uint4> sched(slots.size());
+  auto u16 = [](int32_t v) { return (uint32_t)(v < 0 ? 0xffff : v); };
+  for (size_t q = 0; q < slots.size(); ++q) {
+    const ProductSlot& e = slots[q];
+    a.slot_prod[q] = e.k;
+    sched[q] = make_uint4(u16(e.f[0]) | (u16(e.f[1]) << 16), u16(e.f[2]) | (u16(e.f[3]) << 16),
+                          u16(e.t[0]) | (u16(e.t[1]) << 16), u16(e.t[2]) | (u16(e.t[3]) << 16));
+  }
+  a.sched = upload(sched, st);
+  a.copyb = upload(copyb, st);
   a.c_rows.assign(m->coupling_rows, m->coupling_rows + m->coupling_nnz);
   a.c_cols.assign(m->coupling_cols, m->coupling_cols + m->coupling_nnz);
   a.c_vals.assign(m->coupling_vals, m->coupling_vals + m->coupling_nnz);
@@ -1075,39 +1267,70 @@ static AceArgs<T> make_args(pfw_ctx* ctx, int grads) {
   a.rlb = ad.rlb;
   a.rl = ad.rl;
   a.prod = ad.prod;
-  a.nseg = ad.nseg;
-  a.segb = ad.segb;
-  a.rowseg = ad.rowseg;
-  a.dent = ad.dent;
   if constexpr (sizeof(T) == 8) {
     a.w = ad.w64;
     a.wp = ad.wp64;
+    a.ws = ad.ws64;
   } else {
     a.w = ad.w32;
     a.wp = ad.wp32;
+    a.ws = ad.ws32;
   }
   a.grads = grads;
   a.siteE = ad.siteE;
   a.siteW = ad.siteW;
   a.eg = (T*)ad.eg;
+  a.nsteps = ad.nsteps;
+  a.sched = ad.sched;
+  a.copyb = ad.copyb;
+  a.ntarget = ad.ntarget;
+  a.Abuf = (T*)ad.Abuf;
   return a;
+}
+
+template <typename K>
+static void allow_lds(K kernel, size_t lds, const char* what) {
+  if (lds > kMaxLds) throw Error(PFW_INVALID_ARGUMENT, std::string("ACE model too large for the LDS tiles of ") + what);
+  if (lds > 64 * 1024)
+    PFW_HIP(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+}
+
+// LDS bytes of k_ace_prod's tables (the kernel's layout) and per wavefront.
+template <typename T>
+static void prod_lds(const AceDev& ad, size_t& tables, size_t& per_wave) {
+  const size_t nslot = (size_t)ad.nsteps * kWave;
+  tables = align16(align16(sizeof(T) * ad.nz * nslot) + 16 * nslot + 4 * (size_t)(ad.ndens + 1));
+  per_wave = align16(sizeof(T) * ad.ndens) + align16(sizeof(T) * ad.ntarget);
 }
 
 template <typename T, int L, int SPW>
 static void launch_sites(pfw_ctx* ctx, int grads) {
   AceDev& ad = ctx->ace;
   SystemBuffers& s = ctx->sys;
+  hipStream_t st = ctx->stream;
   AceArgs<T> a = make_args<T>(ctx, grads);
   constexpr int NY = (L + 1) * (L + 1);
-  size_t lds = sizeof(T) * ((size_t)NY + SPW * (2 * (size_t)ad.ndens + (size_t)ad.nseg) +
-                            (size_t)(ad.radial.nr + NY) * kStride);
-  if (lds > 160 * 1024) throw Error(PFW_INVALID_ARGUMENT, "ACE model too large for the site kernel's LDS tiles");
-  if (lds > 64 * 1024)
-    PFW_HIP(hipFuncSetAttribute((const void*)k_ace_site<T, L, SPW>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)lds));
   const unsigned blocks = (unsigned)((s.n + SPW - 1) / SPW);
-  hipLaunchKernelGGL((k_ace_site<T, L, SPW>), dim3(blocks), dim3(kWave), lds, ctx->stream, a);
+  const size_t lds_dens = sizeof(T) * ((size_t)NY + SPW * (size_t)ad.ndens + (size_t)(ad.radial.nr + NY) * kStride);
+  allow_lds(k_ace_dens<T, L, SPW>, lds_dens, "the density kernel");
+  hipLaunchKernelGGL((k_ace_dens<T, L, SPW>), dim3(blocks), dim3(kWave), lds_dens, st, a);
   PFW_HIP(hipGetLastError());
+  size_t tables, per_wave;
+  prod_lds<T>(ad, tables, per_wave);
+  if (tables + per_wave > kMaxLds) throw Error(PFW_INVALID_ARGUMENT, "ACE product tables too large for LDS");
+  const int waves = (int)std::min<size_t>(16, (kMaxLds - tables) / per_wave);
+  const size_t lds_prod = tables + waves * per_wave;
+  allow_lds(k_ace_prod<T>, lds_prod, "the product kernel");
+  const int64_t want = (s.n + waves - 1) / waves;
+  const unsigned pblocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>(want, ctx->num_cu));
+  hipLaunchKernelGGL(k_ace_prod<T>, dim3(pblocks), dim3(waves * kWave), lds_prod, st, a, grads);
+  PFW_HIP(hipGetLastError());
+  if (grads) {
+    const size_t lds_pull = sizeof(T) * ((size_t)NY + SPW * (size_t)ad.ndens);
+    allow_lds(k_ace_pull<T, L, SPW>, lds_pull, "the pullback kernel");
+    hipLaunchKernelGGL((k_ace_pull<T, L, SPW>), dim3(blocks), dim3(kWave), lds_pull, st, a);
+    PFW_HIP(hipGetLastError());
+  }
 }
 
 template <typename T, int L>
@@ -1144,6 +1367,7 @@ static void ace_environment(pfw_ctx* ctx, int rev) {
     PFW_HIP(hipMalloc(&ad.estart, (n + 1) * sizeof(int32_t)));
     PFW_HIP(hipMalloc(&ad.ebs, ((n + 1023) / 1024 + 2) * sizeof(int32_t)));
     PFW_HIP(hipMalloc(&ad.siteE, n * sizeof(double)));
+    PFW_HIP(hipMalloc(&ad.Abuf, std::max<size_t>(1, (size_t)n * ad.ndens) * sizeof(double)));
     PFW_HIP(hipMalloc(&ad.siteW, 9 * n * sizeof(double)));
     PFW_HIP(hipMalloc(&ad.rec, 10 * n * sizeof(double)));
     PFW_HIP(hipMalloc(&ad.status, 4 * sizeof(int32_t)));

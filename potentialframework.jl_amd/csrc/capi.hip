@@ -49,6 +49,7 @@ int pfw_ctx_create(int device, pfw_ctx** out) {
   ctx->device = device;
   int rc = run(ctx, [&] {
     PFW_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    PFW_HIP(hipDeviceGetAttribute(&ctx->num_cu, hipDeviceAttributeMultiprocessorCount, device));
     PFW_HIP(hipMalloc(&ctx->d_out, 16 * sizeof(double)));
     PFW_HIP(hipMalloc(&ctx->d_red, 64 * 16 * sizeof(double)));
     PFW_HIP(hipHostMalloc(&ctx->h_out, 16 * sizeof(double), hipHostMallocDefault));

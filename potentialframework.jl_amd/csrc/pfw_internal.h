@@ -92,6 +92,7 @@ struct PairModelDev {
 };
 
 constexpr int kStages = 6;  // binning, scan, scatter+sort, environments, interaction, reduction
+constexpr size_t kMaxLds = 160 * 1024;  // LDS per compute unit (gfx950)
 
 // One radial family of a resolved ACE model on the device, in both precisions.
 struct AceFamilyDev {
@@ -117,10 +118,13 @@ struct AceDev {
   double* reference = nullptr;
   AceFamilyDev radial, pair;
   int32_t* prod = nullptr;      // nprod x 4, -1 padded
-  int nseg = 0;
-  int32_t* segb = nullptr;      // nseg + 1 entry ranges of dE/dA row segments
-  int32_t* rowseg = nullptr;    // ndens + 1 segment ranges per density
-  int32_t* dent = nullptr;      // entries x 4: (product, other factors -1 padded)
+  int nsteps = 0;               // product schedule: nsteps x 64 slots
+  uint4* sched = nullptr;       // per slot: 4 x u16 factors, 4 x u16 accumulator targets
+  int32_t* copyb = nullptr;     // ndens + 1: accumulator copies of each density
+  int ntarget = 0;
+  std::vector<int32_t> slot_prod;  // host: product of each slot, -1 = padding
+  double* ws64 = nullptr;       // nz x slots readout weights in schedule order
+  float* ws32 = nullptr;
   int32_t* dens = nullptr;      // ndens x 2
   int32_t* rlb = nullptr;       // lmax + 2: radial indices grouped by l
   int32_t* rl = nullptr;        // (radial index, first density of its m-block)
@@ -141,6 +145,7 @@ struct AceDev {
   int32_t* esrc = nullptr;    // centre (sorted index) of each edge    // stencil entry of each edge (edges sorted by entry, neighbour)
   double* eg = nullptr;       // 3 x E edge gradients
   double* siteE = nullptr;    // n (sorted order)
+  void* Abuf = nullptr;       // n x ndens densities, then dE/dA (T of the step)
   double* siteW = nullptr;    // 9 x n
   double* rec = nullptr;      // 10 x n reduction records
   int32_t* status = nullptr;
@@ -150,6 +155,7 @@ struct AceDev {
 
 struct pfw_ctx {
   int device = 0;
+  int num_cu = 256;  // compute units of the device
   hipStream_t stream = nullptr;
   std::string err;
   // geometry: a batch of ncfg independent configurations (1 for a single system)
