@@ -55,7 +55,7 @@ __global__ __launch_bounds__(64) void k_pair(const Grid* __restrict__ grids, con
                                              const int32_t* __restrict__ perm, const double* __restrict__ params,
                                              int nz, LJConsts lj, uint32_t mask, double* __restrict__ F,
                                              double* __restrict__ partials, int ncells) {
-  __shared__ T cx[kCand], cy[kCand], cz[kCand];
+  __shared__ T cx[kCand + 1], cy[kCand + 1], cz[kCand + 1];  // [kCand]: a far sentinel
   __shared__ int cj[kCand], cm[kCand];
   __shared__ int e_pref[kEnt + 1], e_start[kEnt], e_lat[kEnt][3];
   __shared__ double e_sh[kEnt][3];
@@ -76,6 +76,7 @@ __global__ __launch_bounds__(64) void k_pair(const Grid* __restrict__ grids, con
       sA6[k] = (T)(-6.0 * a / g.c2);
     }
   }
+  if (lane == 0) cx[kCand] = cy[kCand] = cz[kCand] = (T)1e30;
   const T c4e = (T)lj.c4e, c24e = (T)lj.c24e, sig2 = (T)lj.sig2, eshift = (T)lj.eshift;
   double eacc = 0;
   T w0 = 0, w1 = 0, w2 = 0, w3 = 0, w4 = 0, w5 = 0;
@@ -150,18 +151,31 @@ __global__ __launch_bounds__(64) void k_pair(const Grid* __restrict__ grids, con
           __syncthreads();
           if (on) {
             for (int c0 = split; c0 < tn; c0 += S * 32) {
+              // bit 31 - q of `hits` flags candidate c0 + q S; the bits are shifted in
+              // (hits = 2 hits + flag) so that no per-test mask or shift is needed
               uint32_t hits = 0;
+              if (c0 + 31 * S < tn) {  // a full round: every candidate is in the chunk
 #pragma unroll 8
-              for (int q = 0; q < 32; ++q) {
-                const int t = c0 + q * S;
-                if (t < tn) {
+                for (int q = 0; q < 32; ++q) {
+                  const int t = c0 + q * S;
                   const T dx = cx[t] - xi, dy = cy[t] - yi, dz = cz[t] - zi;
                   const T r2 = dx * dx + dy * dy + dz * dz;
-                  if (r2 < c2hi) hits |= 1u << q;
+                  hits = hits + hits + (r2 < c2hi ? 1u : 0u);
+                }
+              } else {
+#pragma unroll 8
+                for (int q = 0; q < 32; ++q) {
+                  const int t = c0 + q * S;
+                  const int tt = t < tn ? t : kCand;  // past the chunk: the sentinel, never a hit
+                  const T dx = cx[tt] - xi, dy = cy[tt] - yi, dz = cz[tt] - zi;
+                  const T r2 = dx * dx + dy * dy + dz * dz;
+                  hits = hits + hits + (r2 < c2hi ? 1u : 0u);
                 }
               }
+              // this round's sums (at most 32 pairs) in T, then added to the FP64 totals
+              T er = 0, fxr = 0, fyr = 0, fzr = 0;
               while (hits) {
-                const int q = __builtin_ctz(hits);
+                const int q = 31 - __builtin_ctz(hits);
                 hits &= hits - 1;
                 const int t = c0 + q * S;
                 const T dx = cx[t] - xi, dy = cy[t] - yi, dz = cz[t] - zi;
@@ -194,10 +208,10 @@ __global__ __launch_bounds__(64) void k_pair(const Grid* __restrict__ grids, con
                   coef = c24e * (s6 - (T)2 * s6 * s6) * inv;
                 }
                 const T gx = coef * dx, gy = coef * dy, gz = coef * dz;
-                eacc += (double)e;
-                fx += (double)gx;
-                fy += (double)gy;
-                fz += (double)gz;
+                er += e;
+                fxr += gx;
+                fyr += gy;
+                fzr += gz;
                 w0 += dx * gx;
                 w1 += dy * gy;
                 w2 += dz * gz;
@@ -205,6 +219,10 @@ __global__ __launch_bounds__(64) void k_pair(const Grid* __restrict__ grids, con
                 w4 += dx * gz;
                 w5 += dy * gz;
               }
+              eacc += (double)er;
+              fx += (double)fxr;
+              fy += (double)fyr;
+              fz += (double)fzr;
             }
           }
           __syncthreads();
