@@ -700,24 +700,6 @@ __global__ void k_ace_pack(int64_t n, const double* __restrict__ siteE, const do
   for (int c = 0; c < 9; ++c) rec[10 * k + 1 + c] = grads ? siteW[9 * k + c] : 0.0;
 }
 
-__global__ void k_ace_finish(const double* __restrict__ part, int nb, const int32_t* __restrict__ status,
-                             double* __restrict__ out) {
-  __shared__ double t[10];
-  if (threadIdx.x < 10) {  // one lane per column, rows in fixed order
-    double v = 0;
-    for (int b = 0; b < nb; ++b) v += part[10 * b + threadIdx.x];
-    t[threadIdx.x] = v;
-  }
-  __syncthreads();
-  if (threadIdx.x != 0) return;
-  bool bad = status[0] != 0;
-  for (int c = 0; c < 10; ++c) {
-    out[c] = t[c];
-    bad |= !isfinite(t[c]);
-  }
-  out[10] = bad ? 1.0 : 0.0;
-}
-
 __global__ void k_ace_scatter_sites(int64_t n, const int32_t* __restrict__ perm, const double* __restrict__ siteE,
                                     double* __restrict__ out) {
   int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1443,8 +1425,8 @@ void ace_run(pfw_ctx* ctx, uint32_t mask, int precision) {
     hipLaunchKernelGGL(k_ace_pack, dim3(nblk(n, 256)), dim3(256), 0, st, n, ad.siteE, ad.siteW, grads, ad.rec);
     if (ctx->ncfg == 1) {
       hipLaunchKernelGGL(k_colsum, dim3(kRedBlocks), dim3(256), 0, st, ad.rec, n, 10, ctx->d_red);
-      hipLaunchKernelGGL(k_ace_finish, dim3(1), dim3(64), 0, st, ctx->d_red, kRedBlocks, ad.status, ctx->d_out);
-      PFW_HIP(hipMemcpyAsync(ctx->d_cfg_out, ctx->d_out, 11 * sizeof(double), hipMemcpyDeviceToDevice, st));
+      hipLaunchKernelGGL(k_finish, dim3(1), dim3(1024), 0, st, ctx->d_red, kRedBlocks, 10, 1, (const int32_t*)ad.status,
+                         ctx->d_out, ctx->d_cfg_out);
     } else {
       batch_reduce(ctx, ad.rec, 10, false, 1);
     }

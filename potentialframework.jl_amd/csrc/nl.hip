@@ -239,6 +239,7 @@ void nl_configure(pfw_ctx* ctx, double c) {
   dmalloc(&s.cfg_cell0, ncfg + 1);
   hipStream_t st = ctx->stream;
   PFW_HIP(hipMemcpyAsync(s.grids, ctx->grids.data(), ncfg * sizeof(Grid), hipMemcpyHostToDevice, st));
+  PFW_HIP(hipMemsetAsync(s.count, 0, sizeof(int32_t) * std::max<int64_t>(total, 1), st));  // later builds leave it zero
   if (n > 0) PFW_HIP(hipMemcpyAsync(s.atom_cfg, atom_cfg.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, st));
   if (total > 0)
     PFW_HIP(hipMemcpyAsync(s.cell_cfg, cell_cfg.data(), total * sizeof(int32_t), hipMemcpyHostToDevice, st));
@@ -349,8 +350,8 @@ constexpr int kScanThreads = 256;
 constexpr int kScanBlock = kScanItems * kScanThreads;
 
 __device__ inline int block_exclusive_scan(int v, int* lds, int* total) {
-  // lds: kScanThreads/64 ints
-  int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+  // lds: blockDim.x / 64 + 1 ints. Wave scans, then one wave scans the wave totals.
+  const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave, nw = blockDim.x / kWave;
   int incl = v;
   for (int off = 1; off < kWave; off <<= 1) {
     int t = __shfl_up(incl, off, kWave);
@@ -358,18 +359,90 @@ __device__ inline int block_exclusive_scan(int v, int* lds, int* total) {
   }
   if (lane == kWave - 1) lds[wid] = incl;
   __syncthreads();
-  int woff = 0, tot = 0;
-  for (int w = 0; w < (int)(blockDim.x / kWave); ++w) {
-    if (w < wid) woff += lds[w];
-    tot += lds[w];
+  if (wid == 0) {
+    const int t = lane < nw ? lds[lane] : 0;
+    int ti = t;
+    for (int off = 1; off < kWave; off <<= 1) {
+      int u = __shfl_up(ti, off, kWave);
+      if (lane >= off) ti += u;
+    }
+    if (lane < nw) lds[lane] = ti - t;
+    if (lane == nw - 1) lds[nw] = ti;
   }
   __syncthreads();
-  *total = tot;
+  const int woff = lds[wid];
+  *total = lds[nw];
+  __syncthreads();
   return woff + incl - v;
 }
 
+// One-workgroup exclusive scan for up to kScanOneMax counts, read as rows of
+// 1024 (coalesced, all in flight at once). All rows are scanned together:
+// wave scans in registers, one pass over the wave totals of every row, one
+// pass over the row totals. Also zeroes `cursor` and, with zero_count, the
+// counts themselves (ready for the next step's binning).
+constexpr int kScanOneThreads = 1024;
+constexpr int kScanOneRows = 32;
+constexpr int kScanOneMax = kScanOneThreads * kScanOneRows;
+__global__ __launch_bounds__(1024) void k_scan_one(int32_t* __restrict__ count, int m, int32_t* __restrict__ start,
+                                                   int32_t* __restrict__ cursor, int64_t n, int zero_count) {
+  constexpr int NW = kScanOneThreads / kWave;
+  __shared__ int wt[kScanOneRows][NW];    // wave totals of each row
+  __shared__ int woff[kScanOneRows][NW];  // exclusive wave offsets within each row
+  __shared__ int rt[kScanOneRows];        // row totals
+  __shared__ int roff[kScanOneRows + 1];  // exclusive row offsets; roff[rows] = total
+  const int tid = threadIdx.x, lane = tid % kWave, wid = tid / kWave;
+  const int rows = (m + kScanOneThreads - 1) / kScanOneThreads;
+  int v[kScanOneRows], incl[kScanOneRows];
+#pragma unroll
+  for (int r = 0; r < kScanOneRows; ++r) {
+    const int k = r * kScanOneThreads + tid;
+    v[r] = (r < rows && k < m) ? count[k] : 0;
+  }
+#pragma unroll
+  for (int r = 0; r < kScanOneRows; ++r) {
+    int x = v[r];
+    if (r < rows) {  // uniform over the workgroup
+      for (int off = 1; off < kWave; off <<= 1) {
+        const int t = __shfl_up(x, off, kWave);
+        if (lane >= off) x += t;
+      }
+      if (lane == kWave - 1) wt[r][wid] = x;
+    }
+    incl[r] = x;
+  }
+  __syncthreads();
+  if (tid < rows * NW) {
+    const int r = tid / NW, w = tid % NW;
+    int sacc = 0;
+    for (int u = 0; u < w; ++u) sacc += wt[r][u];
+    woff[r][w] = sacc;
+    if (w == NW - 1) rt[r] = sacc + wt[r][w];
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int acc = 0;
+    for (int r = 0; r < rows; ++r) {
+      roff[r] = acc;
+      acc += rt[r];
+    }
+    roff[rows] = acc;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kScanOneRows; ++r) {
+    const int k = r * kScanOneThreads + tid;
+    if (r < rows && k < m) {
+      start[k] = roff[r] + woff[r][wid] + incl[r] - v[r];
+      cursor[k] = 0;
+      if (zero_count) count[k] = 0;
+    }
+  }
+  if (tid == 0) start[m] = n >= 0 ? (int32_t)n : roff[rows];
+}
+
 __global__ void k_scan_blocks(const int32_t* __restrict__ count, int ncells, int32_t* __restrict__ blocksum) {
-  __shared__ int lds[kScanThreads / kWave];
+  __shared__ int lds[kScanThreads / kWave + 1];
   int base = blockIdx.x * kScanBlock + threadIdx.x * kScanItems;
   int s = 0;
   for (int k = 0; k < kScanItems; ++k)
@@ -380,7 +453,7 @@ __global__ void k_scan_blocks(const int32_t* __restrict__ count, int ncells, int
 }
 
 __global__ void k_scan_top(int32_t* __restrict__ blocksum, int nb) {
-  __shared__ int lds[kScanThreads / kWave];
+  __shared__ int lds[kScanThreads / kWave + 1];
   int carry = 0;
   for (int base = 0; base < nb; base += kScanThreads) {
     int i = base + threadIdx.x;
@@ -395,7 +468,7 @@ __global__ void k_scan_top(int32_t* __restrict__ blocksum, int nb) {
 
 __global__ void k_scan_final(const int32_t* __restrict__ count, int ncells, const int32_t* __restrict__ blocksum,
                              int32_t* __restrict__ start, int32_t* __restrict__ cursor, int64_t n) {
-  __shared__ int lds[kScanThreads / kWave];
+  __shared__ int lds[kScanThreads / kWave + 1];
   int base = blockIdx.x * kScanBlock + threadIdx.x * kScanItems;
   int v[kScanItems], s = 0;
   for (int k = 0; k < kScanItems; ++k) {
@@ -506,6 +579,45 @@ __global__ void k_colsum(const double* __restrict__ in, int64_t m, int width, do
   }
 }
 
+// Final stage of the energy / virial reduction of one configuration: the
+// stage-1 rows of k_colsum (nb x width) summed in fixed order, one lane per
+// column, then E, W (3x3 column-major) and the status word written to
+// out[0..10] and out2[0..10]. kind 0: pair records (E, xx yy zz xy xz yz of
+// sum d (x) g over directed pairs), halved; kind 1: ACE records (E, W[9]) with
+// the status word of the environment build.
+__global__ void k_finish(const double* __restrict__ part, int nb, int width, int kind,
+                         const int32_t* __restrict__ status, double* __restrict__ out, double* __restrict__ out2) {
+  __shared__ double rows[kRedBlocks * 16];
+  __shared__ double t[16];
+  for (int i = threadIdx.x; i < nb * width; i += blockDim.x) rows[i] = part[i];  // all loads in flight at once
+  __syncthreads();
+  if ((int)threadIdx.x < width) {
+    double v = 0;
+    for (int b = 0; b < nb; ++b) v += rows[b * width + threadIdx.x];
+    t[threadIdx.x] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double E, W[9];
+  if (kind == 0) {
+    E = 0.5 * t[0];
+    W[0] = -0.5 * t[1];
+    W[4] = -0.5 * t[2];
+    W[8] = -0.5 * t[3];
+    W[3] = W[1] = -0.5 * t[4];
+    W[6] = W[2] = -0.5 * t[5];
+    W[7] = W[5] = -0.5 * t[6];
+  } else {
+    E = t[0];
+    for (int k = 0; k < 9; ++k) W[k] = t[1 + k];
+  }
+  bool bad = !isfinite(E) || (status && status[0] != 0);
+  for (int k = 0; k < 9; ++k) bad |= !isfinite(W[k]);
+  out[0] = out2[0] = E;
+  for (int k = 0; k < 9; ++k) out[1 + k] = out2[1 + k] = W[k];
+  out[10] = out2[10] = bad ? 1.0 : 0.0;
+}
+
 // Column sums of a (nblocks x width) table of per-workgroup partials, in
 // fixed row order (deterministic).
 __global__ void k_sum_partials(const double* __restrict__ partial, int nblocks, int width, double* __restrict__ out) {
@@ -522,6 +634,12 @@ static inline unsigned nblocks(int64_t n, int t) { return (unsigned)std::max<int
 // blocksum needs (m + 1023) / 1024 + 1 ints.
 void scan_counts(pfw_ctx* ctx, const int32_t* count, int m, int32_t* start, int32_t* blocksum, int32_t* cursor) {
   hipStream_t st = ctx->stream;
+  if (m <= kScanOneMax) {
+    hipLaunchKernelGGL(k_scan_one, dim3(1), dim3(kScanOneThreads), 0, st, const_cast<int32_t*>(count), m, start, cursor,
+                       (int64_t)-1, 0);
+    PFW_HIP(hipGetLastError());
+    return;
+  }
   int nb = (int)nblocks(m, kScanBlock);
   hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(kScanThreads), 0, st, count, m, blocksum);
   hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, st, blocksum, nb);
@@ -535,7 +653,8 @@ void nl_build(pfw_ctx* ctx) {
   const int64_t n = s.n;
   const int ncells = ctx->ncells;
   record_stage(ctx, 0);
-  PFW_HIP(hipMemsetAsync(s.count, 0, sizeof(int32_t) * std::max(ncells, 1), st));
+  const bool one = ncells <= kScanOneMax;  // counts are then left zero by the previous build's scan
+  if (!one) PFW_HIP(hipMemsetAsync(s.count, 0, sizeof(int32_t) * std::max(ncells, 1), st));
   if (ctx->open_any) {
     std::vector<unsigned long long> init(6 * (size_t)ctx->ncfg);
     for (size_t k = 0; k < init.size(); ++k) init[k] = (k % 2 == 0) ? ~0ull : 0ull;
@@ -554,11 +673,15 @@ void nl_build(pfw_ctx* ctx) {
   }
   PFW_HIP(hipGetLastError());
   record_stage(ctx, 1);
-  int nb = (int)nblocks(ncells, kScanBlock);
-  hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(kScanThreads), 0, st, s.count, ncells, s.blocksum);
-  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, st, s.blocksum, nb);
-  hipLaunchKernelGGL(k_scan_final, dim3(nb), dim3(kScanThreads), 0, st, s.count, ncells, s.blocksum, s.start,
-                     s.cursor, n);
+  if (one) {
+    hipLaunchKernelGGL(k_scan_one, dim3(1), dim3(kScanOneThreads), 0, st, s.count, ncells, s.start, s.cursor, n, 1);
+  } else {
+    int nb = (int)nblocks(ncells, kScanBlock);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(kScanThreads), 0, st, s.count, ncells, s.blocksum);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, st, s.blocksum, nb);
+    hipLaunchKernelGGL(k_scan_final, dim3(nb), dim3(kScanThreads), 0, st, s.count, ncells, s.blocksum, s.start,
+                       s.cursor, n);
+  }
   PFW_HIP(hipGetLastError());
   record_stage(ctx, 2);
   if (n > 0) {

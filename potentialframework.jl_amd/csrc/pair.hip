@@ -255,33 +255,6 @@ __global__ __launch_bounds__(64) void k_pair(const Grid* __restrict__ grids, con
     for (int k = 0; k < 7; ++k) partials[7 * (int64_t)cell + k] = v[k];
 }
 
-// Final FP64 reduction of the stage-1 column sums into E, W (3x3 column-major)
-// and the nonfinite status word (gpu.jl:390-395).
-__global__ void k_reduce_finish(const double* __restrict__ part, int nb, double* __restrict__ out) {
-  __shared__ double t[7];
-  if (threadIdx.x < 7) {  // one lane per column, rows in fixed order
-    double v = 0;
-    for (int b = 0; b < nb; ++b) v += part[7 * b + threadIdx.x];
-    t[threadIdx.x] = v;
-  }
-  __syncthreads();
-  if (threadIdx.x != 0) return;
-  const double E = 0.5 * t[0];
-  // P = sum d (x) g over directed pairs: xx yy zz xy xz yz
-  double W[9];
-  W[0] = -0.5 * t[1];
-  W[4] = -0.5 * t[2];
-  W[8] = -0.5 * t[3];
-  W[3] = W[1] = -0.5 * t[4];
-  W[6] = W[2] = -0.5 * t[5];
-  W[7] = W[5] = -0.5 * t[6];
-  bool bad = !isfinite(E);
-  for (int k = 0; k < 9; ++k) bad |= !isfinite(W[k]);
-  out[0] = E;
-  for (int k = 0; k < 9; ++k) out[1 + k] = W[k];
-  out[10] = bad ? 1.0 : 0.0;
-}
-
 void record_stage(pfw_ctx* ctx, int k) {
   if (ctx->timing) PFW_HIP(hipEventRecord(ctx->ev[k], ctx->stream));
 }
@@ -331,8 +304,8 @@ void pair_run(pfw_ctx* ctx, uint32_t mask, int precision) {
     if (ctx->ncfg == 1) {
       hipLaunchKernelGGL(k_colsum, dim3(kRedBlocks), dim3(256), 0, st, ctx->d_partials, (int64_t)ctx->ncells, 7,
                          ctx->d_red);
-      hipLaunchKernelGGL(k_reduce_finish, dim3(1), dim3(64), 0, st, ctx->d_red, kRedBlocks, ctx->d_out);
-      PFW_HIP(hipMemcpyAsync(ctx->d_cfg_out, ctx->d_out, 11 * sizeof(double), hipMemcpyDeviceToDevice, st));
+      hipLaunchKernelGGL(k_finish, dim3(1), dim3(1024), 0, st, ctx->d_red, kRedBlocks, 7, 0, (const int32_t*)nullptr,
+                         ctx->d_out, ctx->d_cfg_out);
     } else {
       batch_reduce(ctx, ctx->d_partials, 7, true, 0);
     }
