@@ -249,17 +249,24 @@ void nl_configure(pfw_ctx* ctx, double c) {
 
 // ---------------------------------------------------------------- kernels
 
+// Julia mod(x, y) for y > 0 with a fast path for x already in [0, y): mod
+// returns x itself there, except -0.0, which becomes +0.0.
+__device__ inline double mod_pos(double x, double y) {
+  if (x >= 0 && x < y) return x == 0 ? 0.0 : x;
+  return jl_mod(x, y);
+}
+
 __device__ inline void wrap_atom(const Grid& g, const double* x, double* w, double* u) {
   if (g.minimum_image) {
     for (int a = 0; a < 3; ++a) {
-      w[a] = g.pb[a] ? jl_mod(x[a], g.L[a]) : x[a];
+      w[a] = g.pb[a] ? mod_pos(x[a], g.L[a]) : x[a];
       u[a] = w[a];
     }
   } else {
     double f[3];
     exact_matvec(g.recip, x, f);
     for (int a = 0; a < 3; ++a)
-      if (g.pb[a]) f[a] = jl_mod(f[a], 1.0);
+      if (g.pb[a]) f[a] = mod_pos(f[a], 1.0);
     exact_matvec(g.basis, f, w);
     for (int a = 0; a < 3; ++a) u[a] = f[a];
   }
@@ -277,6 +284,19 @@ __device__ inline int bin_atom(const Grid& g, const double* u) {
   return cell_index(g, k);
 }
 
+// count[c] += 1 for every active lane, one atomic per distinct cell of the
+// wavefront (lanes of a wavefront often share a cell).
+__device__ inline void wave_count(int32_t* count, int c) {
+  uint64_t todo = __ballot(true);
+  while (todo) {
+    const int leader = __builtin_ctzll(todo);
+    const int cl = __shfl(c, leader, kWave);
+    const uint64_t same = __ballot(c == cl) & todo;
+    if ((int)(threadIdx.x % kWave) == leader) atomicAdd(&count[cl], __popcll(same));
+    todo &= ~same;
+  }
+}
+
 // Wrap every atom with its configuration's basis; when no configuration has
 // an open axis also bin and count in the same pass.
 __global__ void k_wrap(const Grid* __restrict__ grids, const int32_t* __restrict__ atom_cfg, int64_t n,
@@ -289,15 +309,15 @@ __global__ void k_wrap(const Grid* __restrict__ grids, const int32_t* __restrict
   const Grid& g = grids[cfg];
   double x[3] = {X[3 * i], X[3 * i + 1], X[3 * i + 2]}, w[3], uu[3];
   wrap_atom(g, x, w, uu);
-  for (int a = 0; a < 3; ++a) {
-    xw[a * n + i] = w[a];
-    u[a * n + i] = uu[a];
-  }
-  if (!open_any) {
+  for (int a = 0; a < 3; ++a) xw[a * n + i] = w[a];
+  if (!open_any) {  // bin coordinates are needed later only for open axes
     const int c = bin_atom(g, uu);
     cell_of[i] = c;
-    atomicAdd(&count[c], 1);
-  } else if (g.open_any) {
+    wave_count(count, c);
+    return;
+  }
+  for (int a = 0; a < 3; ++a) u[a * n + i] = uu[a];
+  if (g.open_any) {
     for (int a = 0; a < 3; ++a) {
       if (g.pb[a]) continue;
       atomicMin(&bounds[6 * cfg + 2 * a], ordered_bits(uu[a]));
@@ -496,8 +516,8 @@ __global__ void k_scatter(int64_t n, const int32_t* __restrict__ cell_of, const 
                           int32_t* __restrict__ cursor, int32_t* __restrict__ perm) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  int c = cell_of[i];
-  int pos = start[c] + atomicAdd(&cursor[c], 1);
+  const int c = cell_of[i];
+  const int pos = start[c] + atomicAdd(&cursor[c], 1);  // order inside a cell is fixed later
   perm[pos] = (int32_t)i;
 }
 
