@@ -150,39 +150,47 @@ __global__ __launch_bounds__(64) void k_pair(const Grid* __restrict__ grids, con
           }
           __syncthreads();
           if (on) {
-            for (int c0 = split; c0 < tn; c0 += S * 32) {
-              // bit 31 - q of `hits` flags candidate c0 + q S; the bits are shifted in
-              // (hits = 2 hits + flag) so that no per-test mask or shift is needed
-              uint32_t hits = 0;
-              if (c0 + 31 * S < tn) {  // a full round: every candidate is in the chunk
+            // 32 tests of candidates c0r + q S: bit 31 - q flags a hit; the bits are
+            // shifted in (h = 2 h + flag) so that no per-test mask or shift is needed
+            auto round = [&](int c0r) -> uint32_t {
+              uint32_t h = 0;
+              if (c0r + 31 * S < tn) {  // every candidate is in the chunk
 #pragma unroll 8
                 for (int q = 0; q < 32; ++q) {
-                  const int t = c0 + q * S;
+                  const int t = c0r + q * S;
                   const T dx = cx[t] - xi, dy = cy[t] - yi, dz = cz[t] - zi;
                   const T r2 = dx * dx + dy * dy + dz * dz;
-                  hits = hits + hits + (r2 < c2hi ? 1u : 0u);
+                  h = h + h + (r2 < c2hi ? 1u : 0u);
                 }
               } else {
 #pragma unroll 8
                 for (int q = 0; q < 32; ++q) {
-                  const int t = c0 + q * S;
+                  const int t = c0r + q * S;
                   const int tt = t < tn ? t : kCand;  // past the chunk: the sentinel, never a hit
                   const T dx = cx[tt] - xi, dy = cy[tt] - yi, dz = cz[tt] - zi;
                   const T r2 = dx * dx + dy * dy + dz * dz;
-                  hits = hits + hits + (r2 < c2hi ? 1u : 0u);
+                  h = h + h + (r2 < c2hi ? 1u : 0u);
                 }
               }
-              // this round's sums (at most 32 pairs) in T, then added to the FP64 totals
+              return h;
+            };
+            // the lane's own atom among this chunk's candidates (the (i, i, 0) self pair)
+            const int tself = ez >= 0 ? e_pref[ez] + (is - e_start[ez]) - t0 : -1;
+            for (int c0 = split; c0 < tn; c0 += S * 64) {
+              // two rounds, then their hits: bit b of `hits` flags candidate c0 + (63 - b) S
+              uint64_t hits = (uint64_t)round(c0) << 32;
+              if (c0 + 32 * S < tn) hits |= round(c0 + 32 * S);
+              // this pair of rounds' sums (at most 64 pairs) in T, then added to the FP64 totals
               T er = 0, fxr = 0, fyr = 0, fzr = 0;
               while (hits) {
-                const int q = 31 - __builtin_ctz(hits);
+                const int t = c0 + (63 - __builtin_ctzll(hits)) * S;
                 hits &= hits - 1;
-                const int t = c0 + q * S;
+                if (t == tself) continue;
                 const T dx = cx[t] - xi, dy = cy[t] - yi, dz = cz[t] - zi;
                 const T r2 = dx * dx + dy * dy + dz * dz;
-                const int meta = cm[t], m = cj[t], el = meta >> 8, zj = meta & 255;
-                if (el == ez && m == is) continue;  // the (i, i, 0) self pair
+                const int meta = cm[t], el = meta >> 8, zj = meta & 255;
                 if (!(r2 < c2lo)) {
+                  const int m = cj[t];
                   double d[3];
                   int img[3];
                   if (!exact_member(g, xid, yid, zid, xs[m], xs[n + m], xs[2 * n + m], e_lat[el], e_sh[el], d,
