@@ -55,12 +55,20 @@ __global__ __launch_bounds__(64) void k_pair(const Grid* __restrict__ grids, con
                                              const int32_t* __restrict__ perm, const double* __restrict__ params,
                                              int nz, LJConsts lj, uint32_t mask, double* __restrict__ F,
                                              double* __restrict__ partials, int ncells) {
-  __shared__ T cx[kCand + 1], cy[kCand + 1], cz[kCand + 1];  // [kCand]: a far sentinel
-  __shared__ int cj[kCand], cm[kCand];
+  // candidate coordinates ([kCand]: a far sentinel); after the candidate loops
+  // of an atom group the same bytes hold the per-slice force sums
+  constexpr size_t kCoordBytes = 3 * (kCand + 1) * sizeof(T), kRedBytes = kWave * 3 * sizeof(double);
+  __shared__ __align__(16) unsigned char cbuf[kCoordBytes > kRedBytes ? kCoordBytes : kRedBytes];
+  T* cx = (T*)cbuf;
+  T* cy = cx + kCand + 1;
+  T* cz = cy + kCand + 1;
+  double(*red)[3] = (double(*)[3])cbuf;
+  __shared__ int cm[kCand];  // (stencil entry within the entry chunk) << 8 | species
   __shared__ int e_pref[kEnt + 1], e_start[kEnt], e_lat[kEnt][3];
   __shared__ double e_sh[kEnt][3];
-  __shared__ T sA[kMaxZ * kMaxZ], sA6[kMaxZ * kMaxZ];
-  __shared__ double red[kWave][3];
+  extern __shared__ __align__(16) unsigned char dyn[];  // SmoothPair: nz x nz amplitudes, then -6 A / c^2
+  T* sA = (T*)dyn;
+  T* sA6 = sA + nz * nz;
   __shared__ int e_zero;
 
   const int lane = threadIdx.x;
@@ -145,7 +153,6 @@ __global__ __launch_bounds__(64) void k_pair(const Grid* __restrict__ grids, con
             cx[t] = (T)((xs[m] + e_sh[lo][0]) - rx);
             cy[t] = (T)((xs[n + m] + e_sh[lo][1]) - ry);
             cz[t] = (T)((xs[2 * n + m] + e_sh[lo][2]) - rz);
-            cj[t] = m;
             cm[t] = (lo << 8) | sps[m];
           }
           __syncthreads();
@@ -190,7 +197,7 @@ __global__ __launch_bounds__(64) void k_pair(const Grid* __restrict__ grids, con
                 const T r2 = dx * dx + dy * dy + dz * dz;
                 const int meta = cm[t], el = meta >> 8, zj = meta & 255;
                 if (!(r2 < c2lo)) {
-                  const int m = cj[t];
+                  const int m = e_start[el] + (t0 + t - e_pref[el]);
                   double d[3];
                   int img[3];
                   if (!exact_member(g, xid, yid, zid, xs[m], xs[n + m], xs[2 * n + m], e_lat[el], e_sh[el], d,
@@ -279,7 +286,8 @@ static void launch_pair(pfw_ctx* ctx, uint32_t mask) {
     lj.sig2 = sig * sig;
     lj.eshift = 4.0 * eps * (qc * qc - qc);
   }
-  hipLaunchKernelGGL((k_pair<T, KIND>), dim3(ctx->ncells), dim3(kWave), 0, ctx->stream, s.grids, s.cell_cfg, s.n,
+  const size_t dyn = KIND == PFW_PAIR_SMOOTH ? 2 * sizeof(T) * ctx->pair.nz * ctx->pair.nz : 0;
+  hipLaunchKernelGGL((k_pair<T, KIND>), dim3(ctx->ncells), dim3(kWave), dyn, ctx->stream, s.grids, s.cell_cfg, s.n,
                      s.start, s.xs, s.sps, s.perm, ctx->pair.d_params, ctx->pair.nz, lj, mask, ctx->d_F,
                      ctx->d_partials, ctx->ncells);
 }
