@@ -46,7 +46,8 @@ struct AceArgs {
   int nz, ndens, nprod, ny;
   const int32_t* perm;
   const int32_t* sps;
-  const int32_t* estart;
+  const int32_t* ecount;  // edges of each site (sorted centre); site i's edges start at i * ecap
+  int ecap;
   const int32_t* enb;
   const double* eR;
   const double* reference;
@@ -293,8 +294,8 @@ __global__ __launch_bounds__(64) void k_ace_dens(AceArgs<T> a) {
   for (int k = lane; k < SPW * a.ndens; k += kWave) A[k] = 0;
   T* As = A + sl * a.ndens;
   const int iz = valid ? a.sps[site] : 0;
-  const int e0 = valid ? a.estart[site] : 0;
-  const int ne = valid ? a.estart[site + 1] - e0 : 0;
+  const int64_t e0 = valid ? site * a.ecap : 0;
+  const int ne = valid ? a.ecount[site] : 0;
   int nemax = ne;
   for (int off = LPS; off < kWave; off <<= 1) nemax = max(nemax, __shfl_xor(nemax, off, kWave));
   const int nq = a.pair.nr;
@@ -445,8 +446,8 @@ __global__ __launch_bounds__(64) void k_ace_pull(AceArgs<T> a) {
   if (valid)
     for (int k = ll; k < a.ndens; k += LPS) dAs[k] = a.Abuf[site * a.ndens + k];
   const int iz = valid ? a.sps[site] : 0;
-  const int e0 = valid ? a.estart[site] : 0;
-  const int ne = valid ? a.estart[site + 1] - e0 : 0;
+  const int64_t e0 = valid ? site * a.ecap : 0;
+  const int ne = valid ? a.ecount[site] : 0;
   int nemax = ne;
   for (int off = LPS; off < kWave; off <<= 1) nemax = max(nemax, __shfl_xor(nemax, off, kWave));
   const int nq = a.pair.nr;
@@ -512,9 +513,11 @@ __global__ __launch_bounds__(64) void k_ace_pull(AceArgs<T> a) {
   }
 }
 
-// Directed environments (interface.jl:144-171 + environments.jl:357-391):
-// pass 0 counts per sorted centre, pass 1 writes neighbour, stencil entry and
-// exact R. One wavefront per cell: the stencil entries are staged in LDS,
+// Directed environments (interface.jl:144-171 + environments.jl:357-391) in
+// fixed-capacity rows: site i's edges occupy slots [i ecap, i ecap + ecount[i]).
+// pass 0 only counts (to size ecap); pass 1 writes neighbour, stencil entry and
+// exact R, flags status[2] when a site has more than ecap edges, and records
+// the largest degree in status[1]. One wavefront per cell: the stencil entries are staged in LDS,
 // lanes hold 64 candidates at a time (in stencil-entry, cell order) and test
 // them against each centre of the cell in turn; a ballot compacts the hits, so
 // every centre's edges come out sorted by (entry, neighbour) without atomics.
@@ -522,11 +525,10 @@ __global__ __launch_bounds__(64) void k_ace_pull(AceArgs<T> a) {
 // of 64 centres.
 __global__ __launch_bounds__(64) void k_ace_edges(const Grid* __restrict__ grids, const int32_t* __restrict__ cell_cfg,
                                                   int64_t n, const double* __restrict__ xs,
-                                                  const int32_t* __restrict__ start, int pass,
-                                                  int32_t* __restrict__ ecount, const int32_t* __restrict__ estart,
-                                                  int32_t* __restrict__ esrc, int32_t* __restrict__ enb,
+                                                  const int32_t* __restrict__ start, int pass, int ecap,
+                                                  int32_t* __restrict__ ecount, int32_t* __restrict__ enb,
                                                   int32_t* __restrict__ eent, double* __restrict__ eR,
-                                                  int32_t* __restrict__ maxdeg) {
+                                                  int32_t* __restrict__ status) {
   __shared__ int s_pref[kWave + 1], s_start[kWave], s_lat[kWave][3];
   __shared__ double s_sh[kWave][3];
   __shared__ int s_zero;
@@ -540,7 +542,7 @@ __global__ __launch_bounds__(64) void k_ace_edges(const Grid* __restrict__ grids
   for (int ag = 0; ag < cnt; ag += kWave) {
     const int na = min(kWave, cnt - ag);
     const int64_t mine = b + ag + lane;
-    int64_t pos = (pass && lane < na) ? estart[mine] : 0;
+    int pos = 0;  // the centre's next slot (lane of the same index)
     int deg = 0;
     for (int e0 = 0; e0 < g.nstencil; e0 += kWave) {
       const int e = e0 + lane;
@@ -602,10 +604,10 @@ __global__ __launch_bounds__(64) void k_ace_edges(const Grid* __restrict__ grids
           }
           const uint64_t mask = __ballot(hit);
           const int h = __popcll(mask);
-          const int64_t base = __shfl(pos, a, kWave);  // outside the branch: the source lane must be active
-          if (pass && hit) {
-            const int64_t q = base + __popcll(mask & ((1ull << lane) - 1));
-            esrc[q] = (int32_t)k;
+          const int base = __shfl(pos, a, kWave);  // outside the branch: the source lane must be active
+          const int slot = base + __popcll(mask & ((1ull << lane) - 1));
+          if (pass && hit && slot < ecap) {
+            const int64_t q = k * ecap + slot;
             enb[q] = m;
             eent[q] = e0 + lo;
             eR[3 * q] = d[0];
@@ -620,9 +622,10 @@ __global__ __launch_bounds__(64) void k_ace_edges(const Grid* __restrict__ grids
       }
       __syncthreads();
     }
-    if (!pass && lane < na) {
-      ecount[mine] = deg;
-      atomicMax(maxdeg, deg);
+    if (lane < na) {
+      ecount[mine] = min(deg, ecap);
+      atomicMax(status + 1, deg);
+      if (deg > ecap) atomicOr(status + 2, 1);  // capacity too small: the host repeats the step
     }
   }
 }
@@ -631,21 +634,24 @@ __global__ __launch_bounds__(64) void k_ace_edges(const Grid* __restrict__ grids
 // The reverse of a pair found through stencil offset o is found through -o,
 // the mirrored entry nstencil - 1 - entry, and every centre's edges are sorted
 // by (entry, neighbour), so it is located by binary search in j's list (and
-// its displacement checked to be exactly -R). One thread per edge; every
-// erev entry is written (-1 when not found, which also sets the status word).
-__global__ void k_ace_rev(int64_t nedges, const Grid* __restrict__ grids, const int32_t* __restrict__ atom_cfg,
-                          const int32_t* __restrict__ perm, const int32_t* __restrict__ estart,
-                          const int32_t* __restrict__ esrc, const int32_t* __restrict__ enb,
+// its displacement checked to be exactly -R). One thread per edge slot; every
+// used slot's erev is written (-1 when not found, which also sets the status
+// word).
+__global__ void k_ace_rev(int64_t nslots, int ecap, const Grid* __restrict__ grids,
+                          const int32_t* __restrict__ atom_cfg, const int32_t* __restrict__ perm,
+                          const int32_t* __restrict__ ecount, const int32_t* __restrict__ enb,
                           const int32_t* __restrict__ eent, const double* __restrict__ eR,
                           int32_t* __restrict__ erev, int32_t* __restrict__ status) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nedges) return;
-  const int i = esrc[e];
+  if (e >= nslots) return;
+  const int i = (int)(e / ecap);
+  if ((int)(e - (int64_t)i * ecap) >= ecount[i]) return;  // an unused slot
   const int j = enb[e];
   const int want = grids[atom_cfg[perm[i]]].nstencil - 1 - eent[e];
-  int a = estart[j], b = estart[j + 1];  // first f in [a, b) with (eent, enb)[f] >= (want, i)
+  int64_t a = (int64_t)j * ecap, b = a + ecount[j];  // first f in [a, b) with (eent, enb)[f] >= (want, i)
+  const int64_t end = b;
   while (a < b) {
-    const int mid = (a + b) >> 1;
+    const int64_t mid = (a + b) >> 1;
     const int em = eent[mid];
     if (em < want || (em == want && enb[mid] < i))
       a = mid + 1;
@@ -653,9 +659,9 @@ __global__ void k_ace_rev(int64_t nedges, const Grid* __restrict__ grids, const 
       b = mid;
   }
   int found = -1;
-  if (a < estart[j + 1] && eent[a] == want && enb[a] == i && eR[3 * a] == -eR[3 * e] &&
-      eR[3 * a + 1] == -eR[3 * e + 1] && eR[3 * a + 2] == -eR[3 * e + 2])
-    found = a;
+  if (a < end && eent[a] == want && enb[a] == i && eR[3 * a] == -eR[3 * e] && eR[3 * a + 1] == -eR[3 * e + 1] &&
+      eR[3 * a + 2] == -eR[3 * e + 2])
+    found = (int)a;
   erev[e] = found;
   if (found < 0) atomicOr(status, 1);
 }
@@ -663,13 +669,13 @@ __global__ void k_ace_rev(int64_t nedges, const Grid* __restrict__ grids, const 
 // F_k = sum over k's out-edges of (g_e - g_rev(e)), written in original order.
 // One wavefront per atom, lanes over its edges, fixed-order wave reduction.
 template <typename G>
-__global__ __launch_bounds__(64) void k_ace_force(int64_t n, const int32_t* __restrict__ estart,
+__global__ __launch_bounds__(64) void k_ace_force(int64_t n, int ecap, const int32_t* __restrict__ ecount,
                                                   const int32_t* __restrict__ erev, const G* __restrict__ eg,
                                                   const int32_t* __restrict__ perm, double* __restrict__ F) {
   const int64_t k = blockIdx.x;
   const int lane = threadIdx.x;
   double f0 = 0, f1 = 0, f2 = 0;
-  for (int e = estart[k] + lane; e < estart[k + 1]; e += kWave) {
+  for (int64_t e = k * ecap + lane; e < k * ecap + ecount[k]; e += kWave) {
     const int r = erev[e];  // -1 only when the status word already reports the failure
     f0 += (double)eg[3 * e];
     f1 += (double)eg[3 * e + 1];
@@ -760,7 +766,8 @@ __global__ __launch_bounds__(64) void k_ace_adjoint(AceArgs<T> a, const int32_t*
 #pragma unroll
     for (int k = 0; k < 9; ++k) wb[k] = Wbar ? Wbar[9 * cfg + k] : 0.0;
     const double fi0 = Fbar[3 * oi], fi1 = Fbar[3 * oi + 1], fi2 = Fbar[3 * oi + 2];
-    const int e0 = a.estart[site], ne = a.estart[site + 1] - e0;
+    const int64_t e0 = site * a.ecap;
+    const int ne = a.ecount[site];
     for (int k = lane; k < a.ndens; k += kWave) A[k] = Ad[k] = 0;
     __syncthreads();
     for (int c0 = 0; c0 < ne; c0 += kWave) {
@@ -873,14 +880,13 @@ static void free_family(AceFamilyDev& f) {
 
 void ace_free_edges(pfw_ctx* ctx) {
   AceDev& a = ctx->ace;
+  if (a.h_status) (void)hipHostFree(a.h_status);
+  a.h_status = nullptr;
   dfree(a.ecount);
-  dfree(a.estart);
-  dfree(a.ebs);
   dfree(a.enb);
   dfree(a.eR);
   dfree(a.erev);
   dfree(a.eent);
-  dfree(a.esrc);
   dfree(a.eg);
   dfree(a.siteE);
   dfree(a.Abuf);
@@ -888,6 +894,7 @@ void ace_free_edges(pfw_ctx* ctx) {
   dfree(a.rec);
   dfree(a.status);
   a.edge_cap = 0;
+  a.ecap = 0;
 }
 
 void ace_free(pfw_ctx* ctx) {
@@ -1239,7 +1246,8 @@ static AceArgs<T> make_args(pfw_ctx* ctx, int grads) {
   a.ny = ad.ny;
   a.perm = s.perm;
   a.sps = s.sps;
-  a.estart = ad.estart;
+  a.ecount = ad.ecount;
+  a.ecap = ad.ecap;
   a.enb = ad.enb;
   a.eR = ad.eR;
   a.reference = ad.reference;
@@ -1337,8 +1345,11 @@ static void dispatch_sites(pfw_ctx* ctx, int grads) {
   }
 }
 
-// Directed environments of every site (CSR by sorted centre) from the cell
-// list built by nl_build; with `rev` also the reverse-edge index.
+// Directed environments of every site from the cell list built by nl_build,
+// in rows of ecap slots; with `rev` also the reverse-edge index. The first
+// build counts once to size ecap; later builds write directly and report a
+// capacity overflow in status[2] (the caller repeats the step with the degree
+// found in status[1]).
 static void ace_environment(pfw_ctx* ctx, int rev) {
   AceDev& ad = ctx->ace;
   SystemBuffers& s = ctx->sys;
@@ -1346,49 +1357,54 @@ static void ace_environment(pfw_ctx* ctx, int rev) {
   const int64_t n = s.n;
   if (!ad.ecount) {
     PFW_HIP(hipMalloc(&ad.ecount, n * sizeof(int32_t)));
-    PFW_HIP(hipMalloc(&ad.estart, (n + 1) * sizeof(int32_t)));
-    PFW_HIP(hipMalloc(&ad.ebs, ((n + 1023) / 1024 + 2) * sizeof(int32_t)));
     PFW_HIP(hipMalloc(&ad.siteE, n * sizeof(double)));
     PFW_HIP(hipMalloc(&ad.Abuf, std::max<size_t>(1, (size_t)n * ad.ndens) * sizeof(double)));
     PFW_HIP(hipMalloc(&ad.siteW, 9 * n * sizeof(double)));
     PFW_HIP(hipMalloc(&ad.rec, 10 * n * sizeof(double)));
     PFW_HIP(hipMalloc(&ad.status, 4 * sizeof(int32_t)));
+    PFW_HIP(hipHostMalloc(&ad.h_status, 4 * sizeof(int32_t), hipHostMallocDefault));
   }
-  PFW_HIP(hipMemsetAsync(ad.status, 0, 4 * sizeof(int32_t), st));
-  hipLaunchKernelGGL(k_ace_edges, dim3((unsigned)ctx->ncells), dim3(kWave), 0, st, s.grids, s.cell_cfg, n, s.xs,
-                     s.start, 0, ad.ecount, (const int32_t*)nullptr, (int32_t*)nullptr, (int32_t*)nullptr,
-                     (int32_t*)nullptr, (double*)nullptr, ad.status + 1);
-  PFW_HIP(hipGetLastError());
-  scan_counts(ctx, ad.ecount, (int)n, ad.estart, ad.ebs, ad.ecount /* cursor: reused, already consumed */);
-  int32_t total = 0, maxdeg = 0;
-  PFW_HIP(hipMemcpyAsync(&total, ad.estart + n, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-  PFW_HIP(hipMemcpyAsync(&maxdeg, ad.status + 1, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-  PFW_HIP(hipStreamSynchronize(st));
-  ad.max_degree = maxdeg;
-  if (total < 0) throw Error(PFW_CAPACITY, "edge count overflow");
-  if (total > ad.edge_cap || !ad.enb) {
+  if (ad.ecap == 0) {  // size the rows: one counting pass
+    PFW_HIP(hipMemsetAsync(ad.status, 0, 4 * sizeof(int32_t), st));
+    hipLaunchKernelGGL(k_ace_edges, dim3((unsigned)ctx->ncells), dim3(kWave), 0, st, s.grids, s.cell_cfg, n, s.xs,
+                       s.start, 0, 0, ad.ecount, (int32_t*)nullptr, (int32_t*)nullptr, (double*)nullptr, ad.status);
+    PFW_HIP(hipGetLastError());
+    PFW_HIP(hipMemcpyAsync(ad.h_status, ad.status, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    PFW_HIP(hipStreamSynchronize(st));
+    ad.max_degree = ad.h_status[1];
+    ad.ecap = std::max(1, ad.max_degree + std::max(4, ad.max_degree / 8));
+  }
+  const int64_t slots = n * (int64_t)ad.ecap;
+  if (slots > ad.edge_cap || !ad.enb) {
     dfree(ad.enb);
     dfree(ad.eR);
     dfree(ad.erev);
     dfree(ad.eent);
-    dfree(ad.esrc);
     dfree(ad.eg);
-    int64_t cap = std::max<int64_t>((int64_t)(total * 1.25) + 64, 64);
-    PFW_HIP(hipMalloc(&ad.enb, cap * sizeof(int32_t)));
-    PFW_HIP(hipMalloc(&ad.eR, 3 * cap * sizeof(double)));
-    PFW_HIP(hipMalloc(&ad.erev, cap * sizeof(int32_t)));
-    PFW_HIP(hipMalloc(&ad.eent, cap * sizeof(int32_t)));
-    PFW_HIP(hipMalloc(&ad.esrc, cap * sizeof(int32_t)));
-    PFW_HIP(hipMalloc(&ad.eg, 3 * cap * sizeof(double)));
-    ad.edge_cap = cap;
+    PFW_HIP(hipMalloc(&ad.enb, slots * sizeof(int32_t)));
+    PFW_HIP(hipMalloc(&ad.eR, 3 * slots * sizeof(double)));
+    PFW_HIP(hipMalloc(&ad.erev, slots * sizeof(int32_t)));
+    PFW_HIP(hipMalloc(&ad.eent, slots * sizeof(int32_t)));
+    PFW_HIP(hipMalloc(&ad.eg, 3 * slots * sizeof(double)));
+    ad.edge_cap = slots;
   }
+  PFW_HIP(hipMemsetAsync(ad.status, 0, 4 * sizeof(int32_t), st));
   hipLaunchKernelGGL(k_ace_edges, dim3((unsigned)ctx->ncells), dim3(kWave), 0, st, s.grids, s.cell_cfg, n, s.xs,
-                     s.start, 1, ad.ecount, (const int32_t*)ad.estart, ad.esrc, ad.enb, ad.eent, ad.eR,
-                     ad.status + 1);
+                     s.start, 1, ad.ecap, ad.ecount, ad.enb, ad.eent, ad.eR, ad.status);
   PFW_HIP(hipGetLastError());
   if (rev)
-    hipLaunchKernelGGL(k_ace_rev, dim3(nblk(total, 256)), dim3(256), 0, st, (int64_t)total, s.grids, s.atom_cfg,
-                       s.perm, ad.estart, ad.esrc, ad.enb, ad.eent, ad.eR, ad.erev, ad.status);
+    hipLaunchKernelGGL(k_ace_rev, dim3(nblk(slots, 256)), dim3(256), 0, st, slots, ad.ecap, s.grids, s.atom_cfg,
+                       s.perm, ad.ecount, ad.enb, ad.eent, ad.eR, ad.erev, ad.status);
+  PFW_HIP(hipMemcpyAsync(ad.h_status, ad.status, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+}
+
+// After the step's synchronisation: true when some site had more edges than
+// its row holds; ecap then grows to the degree found, and the step is repeated.
+static bool ace_overflowed(AceDev& ad) {
+  ad.max_degree = ad.h_status[1];
+  if (!ad.h_status[2]) return false;
+  ad.ecap = ad.max_degree + std::max(4, ad.max_degree / 8);
+  return true;
 }
 
 void ace_run(pfw_ctx* ctx, uint32_t mask, int precision) {
@@ -1417,10 +1433,10 @@ void ace_run(pfw_ctx* ctx, uint32_t mask, int precision) {
       dispatch_sites<double>(ctx, grads);
     record_stage(ctx, 5);
     if (grads && precision == PFW_MIXED)
-      hipLaunchKernelGGL(k_ace_force<float>, dim3((unsigned)n), dim3(kWave), 0, st, n, ad.estart, ad.erev,
+      hipLaunchKernelGGL(k_ace_force<float>, dim3((unsigned)n), dim3(kWave), 0, st, n, ad.ecap, ad.ecount, ad.erev,
                          (const float*)ad.eg, s.perm, ctx->d_F);
     else if (grads)
-      hipLaunchKernelGGL(k_ace_force<double>, dim3((unsigned)n), dim3(kWave), 0, st, n, ad.estart, ad.erev,
+      hipLaunchKernelGGL(k_ace_force<double>, dim3((unsigned)n), dim3(kWave), 0, st, n, ad.ecap, ad.ecount, ad.erev,
                          (const double*)ad.eg, s.perm, ctx->d_F);
     hipLaunchKernelGGL(k_ace_pack, dim3(nblk(n, 256)), dim3(256), 0, st, n, ad.siteE, ad.siteW, grads, ad.rec);
     if (ctx->ncfg == 1) {
@@ -1435,6 +1451,10 @@ void ace_run(pfw_ctx* ctx, uint32_t mask, int precision) {
   PFW_HIP(hipMemcpyAsync(ctx->h_out, ctx->d_out, 11 * sizeof(double), hipMemcpyDeviceToHost, st));
   record_stage(ctx, 6);
   PFW_HIP(hipStreamSynchronize(st));
+  if (n > 0 && mask != 0 && ace_overflowed(ad)) {  // an environment outgrew its row: once more, wider
+    ace_run(ctx, mask, precision);
+    return;
+  }
   if (ctx->ncfg > 1) {
     ctx->h_cfg_out.resize(11 * (size_t)ctx->ncfg);
     PFW_HIP(hipMemcpy(ctx->h_cfg_out.data(), ctx->d_cfg_out, ctx->h_cfg_out.size() * sizeof(double),
@@ -1512,6 +1532,10 @@ void ace_adjoint(pfw_ctx* ctx, int precision, const double* Ebar, const double* 
     PFW_HIP(hipGetLastError());
     PFW_HIP(hipMemcpyAsync(g.data(), sum.p, gsize * sizeof(double), hipMemcpyDeviceToHost, st));
     PFW_HIP(hipStreamSynchronize(st));
+    if (ace_overflowed(ad)) {  // an environment outgrew its row: once more, wider
+      ace_adjoint(ctx, precision, Ebar, Fbar, Wbar, gWB, gWpair);
+      return;
+    }
   }
   // dL/dWB[b, z] = sum_k C[b, k] dL/dw_k(z): the transpose of the precontraction
   if (gWB) {

@@ -134,16 +134,15 @@ struct AceDev {
   double* wp64 = nullptr;       // nz x nq
   float* wp32 = nullptr;
   // environments and per-step buffers
-  int64_t edge_cap = 0;
-  int32_t* ecount = nullptr;  // n
-  int32_t* estart = nullptr;  // n + 1
-  int32_t* ebs = nullptr;     // scan scratch
-  int32_t* enb = nullptr;     // neighbour (sorted index)
-  double* eR = nullptr;       // 3 x E
-  int32_t* erev = nullptr;
-  int32_t* eent = nullptr;
-  int32_t* esrc = nullptr;    // centre (sorted index) of each edge    // stencil entry of each edge (edges sorted by entry, neighbour)
-  double* eg = nullptr;       // 3 x E edge gradients
+  int64_t edge_cap = 0;       // allocated edge slots
+  int ecap = 0;               // slots per site (0: not sized yet)
+  int32_t* ecount = nullptr;  // n: edges of each site
+  int32_t* enb = nullptr;     // n x ecap: neighbour (sorted index)
+  double* eR = nullptr;       // 3 x slots
+  int32_t* erev = nullptr;    // slot of the reverse edge
+  int32_t* eent = nullptr;    // stencil entry of each edge (a site's edges are sorted by entry, neighbour)
+  double* eg = nullptr;       // 3 x slots edge gradients
+  int32_t* h_status = nullptr;  // pinned copy of status after each environment build
   double* siteE = nullptr;    // n (sorted order)
   void* Abuf = nullptr;       // n x ndens densities, then dE/dA (T of the step)
   double* siteW = nullptr;    // 9 x n
