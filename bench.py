@@ -336,7 +336,8 @@ def main():
             "unit": "Matom-evals/s",
             "ms_per_step": round(1e3 * ace["mixed"]["seconds"] / args.steps, 4),
             "fp64_ms_per_step": round(1e3 * ace["float64"]["seconds"] / args.steps, 4),
-            "roofline": {"kernel": "k_ace_site<float,4>", "bound": "mfma", "compute_unit": "VALU fp32",
+            "roofline": {"kernel": "k_ace_dens+k_ace_prod+k_ace_pull<float,4>", "bound": "mfma",
+                         "compute_unit": "VALU fp32",
                          "achieved": round(aflops / (ka * 1e-3) / 1e12, 3), "peak": PEAK_FP32_TFLOPS,
                          "unit": "TFLOP/s", "frac": round(aflops / (ka * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4),
                          "flops_per_edge": per_edge, "flops_per_site": per_site, "kernel_ms": round(ka, 5),

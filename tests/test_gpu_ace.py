@@ -131,3 +131,21 @@ def test_synthetic_bcc_32k_full_size(pfw, synthetic):
     print(f"ACE 32k mixed: max|dF|={fmax:.3e} rms={frms:.3e} dE/atom={(rmx['energy'] - E) / len(X):.3e}")
     assert frms <= 1e-5
     assert abs(rmx["energy"] - E) / len(X) <= 1e-5
+
+
+@pytest.mark.parametrize("prec", ["float64", "mixed"])
+def test_environment_growth_between_steps(pfw, synthetic, prec):
+    """Environments live in rows sized by the first step; when the same cell is
+    later filled more densely the rows overflow and the step is repeated with
+    wider rows. Both steps must match the oracle."""
+    D, model, ps, om = synthetic
+    X, H, pbc, sp = pfw.workloads.bcc_binary(3, 3, 3, seed=21)
+    calc = pfw.ACECalculator(model, ps, precision=prec)
+    tol = 1e-9 if prec == "float64" else 1e-4
+    for scale in (1.0, 0.62):  # the second step packs the atoms into 0.62^3 of the cell
+        Xs = X * scale
+        r = calc.evaluate(pfw.System(Xs, H, pbc, labels_of(model, sp)))
+        E, F, W, _ = oracle_ace(Xs, sp, H, pbc, om)
+        assert abs(r["energy"] - E) <= tol * max(1, abs(E))
+        np.testing.assert_allclose(r["forces"], F.T, atol=tol * max(1, np.abs(F).max()), rtol=tol)
+        np.testing.assert_allclose(r["virial"], W, atol=tol * max(1, abs(E)), rtol=tol)
