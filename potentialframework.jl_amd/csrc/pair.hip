@@ -40,6 +40,46 @@ struct Band<double> {
   static constexpr double rel = 1e-11;
 };
 
+// A staged candidate: coordinates relative to the cell's first atom and, in
+// FP32, its squared norm (pre-shrunk for the prefilter), read with one LDS
+// instruction.
+template <typename T>
+struct alignas(4 * sizeof(T)) Cand {
+  T x, y, z, w;
+};
+// FP64: no squared norm (the difference form is used; LDS is scarcer)
+template <>
+struct Cand<double> {
+  double x, y, z;
+};
+
+// Relative slack of the |xi|^2 + |xj|^2 - 2 xi.xj prefilter: it exceeds the
+// rounding error of that form (a few ulp of |xi|^2 + |xj|^2), so with both
+// squares scaled by (1 - slack) the prefilter never drops a pair the exact
+// difference form keeps; extra candidates are rejected in the pair loop.
+template <typename T>
+struct DotSlack;
+template <>
+struct DotSlack<float> {
+  static constexpr float value = 16 * 1.1920929e-7f;
+};
+template <>
+struct DotSlack<double> {
+  static constexpr double value = 16 * 2.220446049250313e-16;
+};
+
+// The test of a candidate against the lane's atom: FP32 by the dot form (one
+// FMA chain), FP64 by differences.
+__device__ inline float prefilter_r2(const Cand<float>& c, float, float, float, float m2x, float m2y, float m2z,
+                                     float xi2) {
+  return m2x * c.x + (m2y * c.y + (m2z * c.z + (xi2 + c.w)));
+}
+__device__ inline double prefilter_r2(const Cand<double>& c, double xi, double yi, double zi, double, double, double,
+                                      double) {
+  const double dx = c.x - xi, dy = c.y - yi, dz = c.z - zi;
+  return dx * dx + dy * dy + dz * dz;
+}
+
 // Block index -> cell, giving each of the 8 XCDs (dispatched round-robin) a
 // contiguous range of cells so neighbouring cells share an L2.
 __device__ inline int xcd_remap(int b, int nb) {
@@ -55,13 +95,12 @@ __global__ __launch_bounds__(64) void k_pair(const Grid* __restrict__ grids, con
                                              const int32_t* __restrict__ perm, const double* __restrict__ params,
                                              int nz, LJConsts lj, uint32_t mask, double* __restrict__ F,
                                              double* __restrict__ partials, int ncells) {
-  // candidate coordinates ([kCand]: a far sentinel); after the candidate loops
-  // of an atom group the same bytes hold the per-slice force sums
-  constexpr size_t kCoordBytes = 3 * (kCand + 1) * sizeof(T), kRedBytes = kWave * 3 * sizeof(double);
+  // candidates as (x, y, z, (1 - slack) |x|^2) ([kCand]: a far sentinel); after
+  // the candidate loops of an atom group the same bytes hold the per-slice
+  // force sums
+  constexpr size_t kCoordBytes = (kCand + 1) * sizeof(Cand<T>), kRedBytes = kWave * 3 * sizeof(double);
   __shared__ __align__(16) unsigned char cbuf[kCoordBytes > kRedBytes ? kCoordBytes : kRedBytes];
-  T* cx = (T*)cbuf;
-  T* cy = cx + kCand + 1;
-  T* cz = cy + kCand + 1;
+  Cand<T>* cand = (Cand<T>*)cbuf;
   double(*red)[3] = (double(*)[3])cbuf;
   __shared__ int cm[kCand];  // (stencil entry within the entry chunk) << 8 | species
   __shared__ int e_pref[kEnt + 1], e_start[kEnt], e_lat[kEnt][3];
@@ -84,7 +123,12 @@ __global__ __launch_bounds__(64) void k_pair(const Grid* __restrict__ grids, con
       sA6[k] = (T)(-6.0 * a / g.c2);
     }
   }
-  if (lane == 0) cx[kCand] = cy[kCand] = cz[kCand] = (T)1e30;
+  if (lane == 0) {
+    if constexpr (sizeof(T) == 4)
+      cand[kCand] = Cand<T>{(T)0, (T)0, (T)0, (T)1e30};
+    else
+      cand[kCand] = Cand<T>{(T)1e30, (T)1e30, (T)1e30};
+  }
   const T c4e = (T)lj.c4e, c24e = (T)lj.c24e, sig2 = (T)lj.sig2, eshift = (T)lj.eshift;
   double eacc = 0;
   T w0 = 0, w1 = 0, w2 = 0, w3 = 0, w4 = 0, w5 = 0;
@@ -103,6 +147,10 @@ __global__ __launch_bounds__(64) void k_pair(const Grid* __restrict__ grids, con
       const int is = b + ib + il;
       double xid = xs[is], yid = xs[n + is], zid = xs[2 * n + is];
       const T xi = (T)(xid - rx), yi = (T)(yid - ry), zi = (T)(zid - rz);
+      // prefilter r^2 = |xi|^2 + |xj|^2 - 2 xi.xj, both squares shrunk by the
+      // slack that bounds its rounding error, so it never misses a pair
+      const T m2x = (T)-2 * xi, m2y = (T)-2 * yi, m2z = (T)-2 * zi;
+      const T xi2 = (xi * xi + yi * yi + zi * zi) * ((T)1 - DotSlack<T>::value);
       const int zsp = sps[is];
       double fx = 0, fy = 0, fz = 0;
       for (int e0 = 0; e0 < g.nstencil; e0 += kEnt) {
@@ -150,9 +198,13 @@ __global__ __launch_bounds__(64) void k_pair(const Grid* __restrict__ grids, con
                 hi = mid;
             }
             const int m = e_start[lo] + (tt - e_pref[lo]);
-            cx[t] = (T)((xs[m] + e_sh[lo][0]) - rx);
-            cy[t] = (T)((xs[n + m] + e_sh[lo][1]) - ry);
-            cz[t] = (T)((xs[2 * n + m] + e_sh[lo][2]) - rz);
+            const T x = (T)((xs[m] + e_sh[lo][0]) - rx);
+            const T y = (T)((xs[n + m] + e_sh[lo][1]) - ry);
+            const T z = (T)((xs[2 * n + m] + e_sh[lo][2]) - rz);
+            if constexpr (sizeof(T) == 4)
+              cand[t] = Cand<T>{x, y, z, (x * x + y * y + z * z) * ((T)1 - DotSlack<T>::value)};
+            else
+              cand[t] = Cand<T>{x, y, z};
             cm[t] = (lo << 8) | sps[m];
           }
           __syncthreads();
@@ -165,18 +217,14 @@ __global__ __launch_bounds__(64) void k_pair(const Grid* __restrict__ grids, con
 #pragma unroll 8
                 for (int q = 0; q < 32; ++q) {
                   const int t = c0r + q * S;
-                  const T dx = cx[t] - xi, dy = cy[t] - yi, dz = cz[t] - zi;
-                  const T r2 = dx * dx + dy * dy + dz * dz;
-                  h = h + h + (r2 < c2hi ? 1u : 0u);
+                  h = h + h + (prefilter_r2(cand[t], xi, yi, zi, m2x, m2y, m2z, xi2) < c2hi ? 1u : 0u);
                 }
               } else {
 #pragma unroll 8
                 for (int q = 0; q < 32; ++q) {
                   const int t = c0r + q * S;
                   const int tt = t < tn ? t : kCand;  // past the chunk: the sentinel, never a hit
-                  const T dx = cx[tt] - xi, dy = cy[tt] - yi, dz = cz[tt] - zi;
-                  const T r2 = dx * dx + dy * dy + dz * dz;
-                  h = h + h + (r2 < c2hi ? 1u : 0u);
+                  h = h + h + (prefilter_r2(cand[tt], xi, yi, zi, m2x, m2y, m2z, xi2) < c2hi ? 1u : 0u);
                 }
               }
               return h;
@@ -193,7 +241,8 @@ __global__ __launch_bounds__(64) void k_pair(const Grid* __restrict__ grids, con
                 const int t = c0 + (63 - __builtin_ctzll(hits)) * S;
                 hits &= hits - 1;
                 if (t == tself) continue;
-                const T dx = cx[t] - xi, dy = cy[t] - yi, dz = cz[t] - zi;
+                const Cand<T> c = cand[t];
+                const T dx = c.x - xi, dy = c.y - yi, dz = c.z - zi;
                 const T r2 = dx * dx + dy * dy + dz * dz;
                 const int meta = cm[t], el = meta >> 8, zj = meta & 255;
                 if (!(r2 < c2lo)) {
