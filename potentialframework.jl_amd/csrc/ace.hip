@@ -311,6 +311,7 @@ __global__ __launch_bounds__(64) void k_ace_dens(AceArgs<T> a) {
       const T r = (T)sqrt(R[0] * R[0] + R[1] * R[1] + R[2] * R[2]);
       const int ch = iz + a.nz * a.sps[a.enb[e0 + e]];
       const RadialPoint<T> P = radial_point(a.rad, ch, r);
+#pragma unroll 8
       for (int n = 0; n < a.rad.nr; ++n) {
         T v, d;
         radial_value(a.rad, ch, P, n, v, d);
@@ -337,6 +338,7 @@ __global__ __launch_bounds__(64) void k_ace_dens(AceArgs<T> a) {
       const T* rr = Rb + a.dens[2 * k] * kStride + col0;
       const T* yy = Yb + a.dens[2 * k + 1] * kStride + col0;
       T s = 0;
+#pragma unroll 8
       for (int t = 0; t < cn; ++t) s += rr[t] * yy[t];
       As[k] += s;
     }
@@ -469,6 +471,7 @@ __global__ __launch_bounds__(64) void k_ace_pull(AceArgs<T> a) {
     const RadialPoint<T> P = radial_point(a.rad, ch, r);
 #pragma unroll
     for (int l = 0; l <= L; ++l) {
+#pragma unroll 4
       for (int q = a.rlb[l]; q < a.rlb[l + 1]; ++q) {
         const int n = a.rl[2 * q], a0 = a.rl[2 * q + 1];
         T v, d;
@@ -531,6 +534,7 @@ __global__ __launch_bounds__(64) void k_ace_edges(const Grid* __restrict__ grids
                                                   int32_t* __restrict__ status) {
   __shared__ int s_pref[kWave + 1], s_start[kWave], s_lat[kWave][3];
   __shared__ double s_sh[kWave][3];
+  __shared__ float s_ax[kWave], s_ay[kWave], s_az[kWave];  // the centres, relative to the cell's first atom
   __shared__ int s_zero;
   const int cell = blockIdx.x;
   const int lane = threadIdx.x;
@@ -539,11 +543,20 @@ __global__ __launch_bounds__(64) void k_ace_edges(const Grid* __restrict__ grids
   const Grid& g = grids[cell_cfg[cell]];
   int key[3];
   cell_key(g, cell, key);
+  const double rx = xs[b], ry = xs[n + b], rz = xs[2 * n + b];
+  // FP32 prefilter: pairs outside a thin band above the cutoff never reach the
+  // exact FP64 test (which decides, and gives R, for the rest)
+  const float c2hi = (float)(g.c2 * (1.0 + 1e-4));
   for (int ag = 0; ag < cnt; ag += kWave) {
     const int na = min(kWave, cnt - ag);
     const int64_t mine = b + ag + lane;
     int pos = 0;  // the centre's next slot (lane of the same index)
     int deg = 0;
+    if (lane < na) {
+      s_ax[lane] = (float)(xs[mine] - rx);
+      s_ay[lane] = (float)(xs[n + mine] - ry);
+      s_az[lane] = (float)(xs[2 * n + mine] - rz);
+    }
     for (int e0 = 0; e0 < g.nstencil; e0 += kWave) {
       const int e = e0 + lane;
       int c = 0;
@@ -580,6 +593,7 @@ __global__ __launch_bounds__(64) void k_ace_edges(const Grid* __restrict__ grids
         const bool cand = t < total;
         int m = 0, lo = 0;
         double xj = 0, yj = 0, zj = 0;
+        float fx = 1e30f, fy = 1e30f, fz = 1e30f;
         if (cand) {
           int hi = kWave;
           while (hi - lo > 1) {
@@ -593,12 +607,16 @@ __global__ __launch_bounds__(64) void k_ace_edges(const Grid* __restrict__ grids
           xj = xs[m];
           yj = xs[n + m];
           zj = xs[2 * n + m];
+          fx = (float)((xj + s_sh[lo][0]) - rx);
+          fy = (float)((yj + s_sh[lo][1]) - ry);
+          fz = (float)((zj + s_sh[lo][2]) - rz);
         }
         for (int a = 0; a < na; ++a) {
           const int64_t k = b + ag + a;
           bool hit = false;
           double d[3] = {0, 0, 0};
-          if (cand && !(lo == ez && m == (int)k)) {
+          const float ux = fx - s_ax[a], uy = fy - s_ay[a], uz = fz - s_az[a];
+          if (ux * ux + uy * uy + uz * uz < c2hi && !(lo == ez && m == (int)k)) {
             int img[3];
             hit = exact_member(g, xs[k], xs[n + k], xs[2 * n + k], xj, yj, zj, s_lat[lo], s_sh[lo], d, img);
           }
@@ -791,6 +809,7 @@ __global__ __launch_bounds__(64) void k_ace_adjoint(AceArgs<T> a, const int32_t*
       const T sdot = (dx * de0 + dy * de1 + dz * de2) / r;  // r^ . delta
       if (act) {
         const RadialPoint<T> P = radial_point(a.rad, ch, r);
+#pragma unroll 8
         for (int n = 0; n < a.rad.nr; ++n) {
           T v, d;
           radial_value(a.rad, ch, P, n, v, d);
@@ -824,6 +843,7 @@ __global__ __launch_bounds__(64) void k_ace_adjoint(AceArgs<T> a, const int32_t*
         const T* yy = Yb + mm * kStride;
         const T* yd = Ydb + mm * kStride;
         T s = 0, sd = 0;
+#pragma unroll 8
         for (int t = 0; t < cn; ++t) {
           s += rr[t] * yy[t];
           sd += rd[t] * yy[t] + rr[t] * yd[t];
