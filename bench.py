@@ -199,6 +199,56 @@ def bench_fitting_set(pfw, lib, _capi, model, ps, pair_model, args):
     return out
 
 
+def bench_verlet(pfw, lib, _capi, X, H, pbc, sp, model, labels, skin, prec, mode, steps=40, amp=0.005):
+    """The reference's evolving-trajectory measurement (benchmark/gpu_pairs.jl:118-142):
+    raw coordinates x_s = x_0 + s v with v[a, i] = amp sin(a + 0.73 i) (1-based a, i),
+    40 steps on a list at cutoff + skin built at x_0; each step's positions are
+    uploaded outside the timed region and every step is timed from the call to
+    its synchronised status read. Also the stage times of a reused step."""
+    n = len(X)
+    i1 = np.arange(1, n + 1)[:, None]
+    a1 = np.arange(1, 4)[None, :]
+    vel = amp * np.sin(a1 + 0.73 * i1)
+    w = pfw.Workspace(precision=prec, skin=skin)
+    sys0 = pfw.System(X, H, pbc, labels)
+    pfw.evaluate_(w, sys0, model)
+    pfw.evaluate_(w, sys0, model)
+    h = w.ctx.h
+    P = ctypes.POINTER(ctypes.c_double)
+    times, rebuilt = [], []
+    for s in range(1, steps + 1):
+        Xs = np.ascontiguousarray(X + s * vel)
+        _capi.check(h, lib.pfw_update_positions(h, Xs.ctypes.data_as(P)))
+        before = w.rebuilds
+        t0 = time.perf_counter()
+        _capi.check(h, lib.pfw_pair_step(h, 7, mode))
+        times.append(time.perf_counter() - t0)
+        rebuilt.append(w.rebuilds > before)
+    times = np.array(times)
+    rebuilt = np.array(rebuilt)
+    reuse = times[~rebuilt]
+    info = w.verlet_info()
+    # stage times of reused steps at the last positions
+    _capi.check(h, lib.pfw_set_timing(h, 1))
+    stage = np.zeros(len(STAGES))
+    buf = np.zeros(len(STAGES))
+    for _ in range(10):
+        _capi.check(h, lib.pfw_pair_step(h, 7, mode))
+        lib.pfw_get_timings(h, buf.ctypes.data_as(P), len(STAGES))
+        stage += buf
+    _capi.check(h, lib.pfw_set_timing(h, 0))
+    stage /= 10
+    return {
+        "atoms": n, "skin_A": skin, "precision": prec, "steps": steps, "step_amplitude_A": amp,
+        "rebuild_steps": int(rebuilt.sum()), "amortized_ms": round(1e3 * float(times.mean()), 4),
+        "value": round(n / float(times.mean()) / 1e6, 3), "unit": "Matom-evals/s",
+        "reuse_median_ms": round(1e3 * float(np.median(reuse)), 4) if len(reuse) else None,
+        "rebuild_ms": [round(1e3 * float(t), 4) for t in times[rebuilt]],
+        "ghosts": info["ghosts"], "slots_per_atom": info["slots"],
+        "reuse_stages_ms": {k: round(float(v), 5) for k, v in zip(STAGES, stage)},
+    }
+
+
 def calibrate_copy(nbytes=256 << 20, reps=5):
     """A plain device copy of known size: the reference point that scales the
     FETCH_SIZE / WRITE_SIZE counters into HBM bytes."""
@@ -220,6 +270,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ace", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-verlet", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=10)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -306,6 +357,30 @@ def main():
         "stages_ms": {k: round(float(v), 5) for k, v in zip(STAGES, mixed["stage_ms"])},
         "cpu_baseline": None,
     }
+
+    # ---------------------------------------------------------------- Verlet reuse (skin 0.5 Å)
+    if not args.no_verlet:
+        labels_b2 = ["A" if s == 0 else "B" for s in sp]
+        vb = {prec: bench_verlet(pfw, lib, _capi, X, H, pbc, sp, p, labels_b2, 0.5, prec, mode)
+              for prec, mode in (("mixed", _capi.PFW_MIXED), ("float64", _capi.PFW_FP64))}
+        kl = vb["mixed"]["reuse_stages_ms"]["interaction"]
+        # the list kernel evaluates the same directed in-cutoff pairs (to within the motion)
+        vb["mixed"]["roofline"] = {"kernel": "k_pair_list<float,SMOOTH>", "bound": "mfma",
+                                   "compute_unit": "VALU fp32",
+                                   "achieved": round(pflops / (kl * 1e-3) / 1e12, 3), "peak": PEAK_FP32_TFLOPS,
+                                   "unit": "TFLOP/s", "frac": round(pflops / (kl * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4),
+                                   "kernel_ms": kl}
+        line["verlet"] = {"workload": "b2_smoothpair_256k_triclinic, trajectory x_s = x_0 + s v", **vb}
+        Xr, Hr, pbr, spr = pfw.workloads.argon_fcc(14, sigma=0.03)
+        lj = pfw.lennard_jones()
+        line["verlet_argon_11k"] = {
+            "workload": "argon FCC 14^3 (10,976 atoms), LJ eps 0.0103 sigma 3.4 rc 8.5, the reference's "
+                        "skin trajectory (benchmark/gpu_pairs.jl:118-142)",
+            "mixed": bench_verlet(pfw, lib, _capi, Xr, Hr, pbr, spr, lj, ["Ar"] * len(Xr), 0.5, "mixed",
+                                  _capi.PFW_MIXED),
+            "reference_rx6800xt_float32": {"amortized_ms": 0.256, "reuse_median_ms": 0.169, "rebuild_steps": 1,
+                                           "source": "docs/implementation/GPU-PAIR-RESULTS.md:87-104"},
+        }
 
     # ---------------------------------------------------------------- ACE
     if not args.no_ace:

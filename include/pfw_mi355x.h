@@ -80,6 +80,12 @@ int pfw_read_batch_results(pfw_ctx* ctx, double* E, double* F, double* W);
 
 /* New positions for the same atoms and cell (host buffer). */
 int pfw_update_positions(pfw_ctx* ctx, const double* X);
+/* The same from device memory (3 x N doubles on the context's device, e.g. an
+ * integrator's array): copied on the device and checked for finiteness, the
+ * call synchronises; on PFW_INVALID_ARGUMENT the previous positions are kept.
+ * Replaces copyto!(executor.positions, ...) before execute_gpu!
+ * (src/submodules/ACE/gpu.jl:329-343; NeighbourList/gpu.jl:363-372). */
+int pfw_update_positions_device(pfw_ctx* ctx, const double* X_device);
 
 /* Pair model (NI.prepare_potential for SmoothPair / RadialPotential{LennardJonesRadial},
  * src/submodules/PairPotentials/models.jl:145-176, lennard_jones.jl:5-22;
@@ -111,6 +117,23 @@ int pfw_pair_evaluate(pfw_ctx* ctx, uint32_t request_mask, int precision, double
  * returned. Pairs are ordered by (i, then traversal). */
 int pfw_export_pairs(pfw_ctx* ctx, int32_t* i, int32_t* j, int32_t* S, double* R, int64_t cap,
                      int64_t* count);
+
+/* Verlet skin (Å, default 0): the GPUConfig(skin=...) option of the
+ * reference's GPU pair workspace (src/submodules/NeighbourList/gpu.jl:61-72,
+ * 504-549; docs/gpu_pairs.md:116-131). With skin > 0, pfw_pair_step keeps a
+ * neighbour list at cutoff + skin and rebuilds it only when some atom has moved
+ * at least skin/2 (times 1 - 16 eps) from its position at the last build, in
+ * the caller's raw coordinates (a wrapped jump rebuilds), or when the system,
+ * skin or cell build changed; every interaction uses the current positions
+ * and the model cutoff. The cell grid is sized for cutoff + skin. Unlike the
+ * reference, triclinic and multi-image cells are allowed (periodic images are
+ * explicit ghost entries). skin = 0 returns to fresh cells every step. */
+int pfw_set_skin(pfw_ctx* ctx, double skin);
+/* Verlet state: info[0] rebuilds, info[1] reused steps (the reference's
+ * w.rebuilds counter, gpu.jl:100,508), info[2] ghost entries, info[3] list
+ * slots per atom; max_displacement: largest displacement since the build seen
+ * by the last step (Å). Either pointer may be NULL. */
+int pfw_verlet_info(pfw_ctx* ctx, int64_t* info, double* max_displacement);
 
 /* Diagnostics: number of cells and the cell grid (dims[3]) of the current
  * system, and whether the minimum-image path is active (storage.jl:140). */

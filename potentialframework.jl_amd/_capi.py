@@ -61,12 +61,15 @@ SIGNATURES = [
     ("pfw_set_batch", ctypes.c_int, [_P, ctypes.c_int32, _I64, _D, _I32, _D, _U8, ctypes.c_double]),
     ("pfw_read_batch_results", ctypes.c_int, [_P, _D, _D, _D]),
     ("pfw_update_positions", ctypes.c_int, [_P, _D]),
+    ("pfw_update_positions_device", ctypes.c_int, [_P, ctypes.c_void_p]),
     ("pfw_set_pair_model", ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, _D, ctypes.c_double]),
     ("pfw_pair_step", ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_int]),
     ("pfw_read_results", ctypes.c_int, [_P, _D, _D, _D]),
     ("pfw_pair_evaluate", ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_int, _D, _D, _D]),
     ("pfw_export_pairs", ctypes.c_int, [_P, _I32, _I32, _I32, _D, ctypes.c_int64, _I64]),
     ("pfw_grid_info", ctypes.c_int, [_P, _I64, _I32, _I32]),
+    ("pfw_set_skin", ctypes.c_int, [_P, ctypes.c_double]),
+    ("pfw_verlet_info", ctypes.c_int, [_P, _I64, _D]),
     ("pfw_set_timing", ctypes.c_int, [_P, ctypes.c_int]),
     ("pfw_get_timings", ctypes.c_int, [_P, _D, ctypes.c_int]),
     ("pfw_synchronize", ctypes.c_int, [_P]),

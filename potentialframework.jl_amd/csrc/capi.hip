@@ -35,7 +35,30 @@ void check_positions(int64_t n, const double* X) {
     if (!std::isfinite(X[k])) throw Error(PFW_INVALID_ARGUMENT, "positions must be finite");
 }
 
+// Copy of caller device positions into the staging buffer, flagging any
+// nonfinite coordinate.
+__global__ void k_copy_positions(int64_t m, const double* __restrict__ src, double* __restrict__ dst,
+                                 int32_t* __restrict__ bad) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  const double v = src[k];
+  dst[k] = v;
+  if (!isfinite(v)) atomicOr(bad, 1);
+}
+
 }  // namespace
+
+static void realloc_doubles(double*& p, size_t count) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  PFW_HIP(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(double)));
+}
+
+// Cell grids for the cutoff and the current skin, with the per-cell partials.
+static void configure_grid(pfw_ctx* ctx, double cutoff) {
+  pfw::nl_configure(ctx, cutoff, ctx->verlet.skin);
+  realloc_doubles(ctx->d_partials, (size_t)ctx->ncells * 7);
+}
 
 extern "C" {
 
@@ -53,6 +76,8 @@ int pfw_ctx_create(int device, pfw_ctx** out) {
     PFW_HIP(hipMalloc(&ctx->d_out, 16 * sizeof(double)));
     PFW_HIP(hipMalloc(&ctx->d_red, 64 * 16 * sizeof(double)));
     PFW_HIP(hipHostMalloc(&ctx->h_out, 16 * sizeof(double), hipHostMallocDefault));
+    PFW_HIP(hipMalloc(&ctx->d_flag, sizeof(int32_t)));
+    PFW_HIP(hipHostMalloc(&ctx->h_flag, sizeof(int32_t), hipHostMallocDefault));
     for (auto& e : ctx->ev) PFW_HIP(hipEventCreate(&e));
   });
   if (rc != PFW_OK) {
@@ -69,6 +94,7 @@ void pfw_ctx_destroy(pfw_ctx* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   pfw::nl_free(ctx);
   pfw::ace_free(ctx);
+  pfw::verlet_free(ctx);
   if (ctx->d_F) (void)hipFree(ctx->d_F);
   if (ctx->d_partials) (void)hipFree(ctx->d_partials);
   if (ctx->d_out) (void)hipFree(ctx->d_out);
@@ -76,6 +102,9 @@ void pfw_ctx_destroy(pfw_ctx* ctx) {
   if (ctx->d_cfg) (void)hipFree(ctx->d_cfg);
   if (ctx->d_cfg_out) (void)hipFree(ctx->d_cfg_out);
   if (ctx->h_out) (void)hipHostFree(ctx->h_out);
+  if (ctx->d_flag) (void)hipFree(ctx->d_flag);
+  if (ctx->h_flag) (void)hipHostFree(ctx->h_flag);
+  if (ctx->d_Xstage) (void)hipFree(ctx->d_Xstage);
   if (ctx->pair.d_params) (void)hipFree(ctx->pair.d_params);
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
@@ -103,6 +132,9 @@ int pfw_set_batch(pfw_ctx* ctx, int32_t ncfg, const int64_t* offsets, const doub
     ctx->have_results = false;
     pfw::nl_free(ctx);
     pfw::ace_free_edges(ctx);
+    pfw::verlet_free(ctx);
+    if (ctx->d_Xstage) (void)hipFree(ctx->d_Xstage);
+    ctx->d_Xstage = nullptr;
     ctx->ncfg = ncfg;
     ctx->atom0.assign(offsets, offsets + ncfg + 1);
     // open cell vectors may be infinite or zero (storage.jl:80-82); periodic ones are checked by the basis
@@ -116,16 +148,10 @@ int pfw_set_batch(pfw_ctx* ctx, int32_t ncfg, const int64_t* offsets, const doub
       PFW_HIP(hipMemcpyAsync(ctx->sys.X, X, 3 * n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
       PFW_HIP(hipMemcpyAsync(ctx->sys.species, species, n * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
     }
-    pfw::nl_configure(ctx, cutoff);
-    auto realloc = [](double*& p, size_t count) {
-      if (p) (void)hipFree(p);
-      p = nullptr;
-      PFW_HIP(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(double)));
-    };
-    realloc(ctx->d_F, 3 * (size_t)n);
-    realloc(ctx->d_partials, (size_t)ctx->ncells * 7);
-    realloc(ctx->d_cfg, (size_t)ncfg * 16);
-    realloc(ctx->d_cfg_out, (size_t)ncfg * 11);
+    configure_grid(ctx, cutoff);
+    realloc_doubles(ctx->d_F, 3 * (size_t)n);
+    realloc_doubles(ctx->d_cfg, (size_t)ncfg * 16);
+    realloc_doubles(ctx->d_cfg_out, (size_t)ncfg * 11);
     PFW_HIP(hipStreamSynchronize(ctx->stream));
     ctx->have_system = true;
   });
@@ -139,6 +165,26 @@ int pfw_set_system(pfw_ctx* ctx, int64_t n, const double* X, const int32_t* spec
   }
   const int64_t offsets[2] = {0, n};
   return pfw_set_batch(ctx, 1, offsets, X, species, H, pbc, cutoff);
+}
+
+int pfw_update_positions_device(pfw_ctx* ctx, const double* X) {
+  return run(ctx, [&] {
+    if (!ctx->have_system) throw Error(PFW_STALE, "no system set");
+    const int64_t n = ctx->sys.n;
+    if (n > 0 && !X) throw Error(PFW_INVALID_ARGUMENT, "positions pointer is null");
+    if (n == 0) return;
+    hipStream_t st = ctx->stream;
+    if (!ctx->d_Xstage) PFW_HIP(hipMalloc(&ctx->d_Xstage, 3 * n * sizeof(double)));
+    PFW_HIP(hipMemsetAsync(ctx->d_flag, 0, sizeof(int32_t), st));
+    hipLaunchKernelGGL(k_copy_positions, dim3((unsigned)((3 * n + 255) / 256)), dim3(256), 0, st, 3 * n, X,
+                       ctx->d_Xstage, ctx->d_flag);
+    PFW_HIP(hipGetLastError());
+    PFW_HIP(hipMemcpyAsync(ctx->h_flag, ctx->d_flag, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    PFW_HIP(hipStreamSynchronize(st));
+    if (*ctx->h_flag) throw Error(PFW_INVALID_ARGUMENT, "positions must be finite");
+    std::swap(ctx->sys.X, ctx->d_Xstage);  // the previous positions stay untouched on failure
+    ctx->have_results = false;
+  });
 }
 
 int pfw_update_positions(pfw_ctx* ctx, const double* X) {
@@ -256,6 +302,35 @@ int pfw_grid_info(pfw_ctx* ctx, int64_t* ncells, int32_t* dims, int32_t* minimum
     if (dims)
       for (int a = 0; a < 3; ++a) dims[a] = ctx->grids[0].dims[a];
     if (minimum_image) *minimum_image = ctx->grids[0].minimum_image;
+  });
+}
+
+int pfw_set_skin(pfw_ctx* ctx, double skin) {
+  return run(ctx, [&] {
+    if (!(std::isfinite(skin) && skin >= 0)) throw Error(PFW_INVALID_ARGUMENT, "skin must be finite and nonnegative (Å)");
+    if (skin == ctx->verlet.skin) return;
+    pfw::verlet_free(ctx);
+    ctx->verlet.skin = skin;
+    if (ctx->have_system) {
+      ctx->have_system = false;
+      configure_grid(ctx, ctx->cutoff);  // cells follow cutoff + skin
+      PFW_HIP(hipStreamSynchronize(ctx->stream));
+      ctx->have_system = true;
+    }
+    ctx->have_results = false;
+  });
+}
+
+int pfw_verlet_info(pfw_ctx* ctx, int64_t* info, double* max_displacement) {
+  return run(ctx, [&] {
+    const pfw::VerletDev& V = ctx->verlet;
+    if (info) {
+      info[0] = V.rebuilds;
+      info[1] = V.reuses;
+      info[2] = V.valid ? V.n_ext - ctx->sys.n : 0;
+      info[3] = V.ecap;
+    }
+    if (max_displacement) *max_displacement = V.last_max_disp;
   });
 }
 

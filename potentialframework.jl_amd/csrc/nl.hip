@@ -109,7 +109,9 @@ void nl_free(pfw_ctx* ctx) {
 
 // Grid of one configuration: mirrors prepare!'s choice of wrap path, basis,
 // reach and dims (storage.jl:130-193), with the cell count capped by `limit`.
-static Grid configure_one(const double* H, const int* pb, int64_t n, double c, double limit) {
+// The wrap path follows the cutoff c; cells are sized for the search radius
+// (c plus the Verlet skin), so a stencil reaches every pair of the list.
+static Grid configure_one(const double* H, const int* pb, int64_t n, double c, double search, double limit) {
   Grid g{};
   bool axisaligned = true;
   for (int i = 0; i < 3; ++i)
@@ -138,15 +140,18 @@ static Grid configure_one(const double* H, const int* pb, int64_t n, double c, d
   }
   g.c = c;
   g.c2 = c * c;
+  // list radius: an absolute pad keeps rounding of the build-time geometry
+  // from dropping a pair that the displacement bound promises to keep
+  g.cs2 = search > c ? (search + 1e-6) * (search + 1e-6) : g.c2;
   double extent[3];
   int nper = 0, nopen = 0;
   for (int a = 0; a < 3; ++a) {
     if (mi) {
-      g.reach[a] = c;
+      g.reach[a] = search;
       extent[a] = L[a];
     } else {
       double r0 = g.recip[a], r1 = g.recip[a + 3], r2 = g.recip[a + 6];
-      g.reach[a] = c * std::sqrt(r0 * r0 + r1 * r1 + r2 * r2);
+      g.reach[a] = search * std::sqrt(r0 * r0 + r1 * r1 + r2 * r2);
       extent[a] = 1.0;
     }
     if (pb[a])
@@ -171,7 +176,8 @@ static Grid configure_one(const double* H, const int* pb, int64_t n, double c, d
     if (pb[a]) {
       g.dims[a] = (int)dimsd[a];
       g.sizecell[a] = extent[a] / g.dims[a];
-      g.radius[a] = mi ? 1 : std::max(1, (int)std::ceil(g.reach[a] / g.sizecell[a]));
+      // minimum image without a skin: L > 2c, so cells of at least c need one ring
+      g.radius[a] = (mi && search == c) ? 1 : std::max(1, (int)std::ceil(g.reach[a] / g.sizecell[a]));
       g.used[a] = g.dims[a];
       g.origin[a] = 0;
     } else {
@@ -190,24 +196,28 @@ static Grid configure_one(const double* H, const int* pb, int64_t n, double c, d
   g.nstencil = (2 * g.radius[0] + 1) * (2 * g.radius[1] + 1) * (2 * g.radius[2] + 1);
   if (g.nstencil > 4096)
     throw Error(PFW_INVALID_ARGUMENT, "cutoff spans too many periodic images for the cell stencil");
+  for (int a = 0; a < 3; ++a) g.edims[a] = pb[a] ? g.dims[a] + 2 * g.radius[a] : g.dims[a];
   return g;
 }
 
 // Grids of every configuration of the uploaded batch, packed one after the
 // other in a single global cell index space.
-void nl_configure(pfw_ctx* ctx, double c) {
+void nl_configure(pfw_ctx* ctx, double c, double skin) {
   const int ncfg = ctx->ncfg;
   const int64_t n = ctx->sys.n;
   ctx->grids.resize(ncfg);
-  int64_t total = 0;
+  int64_t total = 0, ext_total = 0;
   bool open_any = false;
   std::vector<int32_t> cell0(ncfg + 1, 0), atom_cfg(n);
   for (int k = 0; k < ncfg; ++k) {
     const int64_t nk = ctx->atom0[k + 1] - ctx->atom0[k];
     const double limit = ncfg == 1 ? std::max<double>(2.0 * (double)nk, 4096.0) : std::max<double>(2.0 * (double)nk, 64.0);
-    Grid g = configure_one(&ctx->host_H[9 * k], &ctx->host_pb[3 * k], nk, c, limit);
+    Grid g = configure_one(&ctx->host_H[9 * k], &ctx->host_pb[3 * k], nk, c, c + skin, limit);
     g.cell0 = (int)total;
+    g.ecell0 = (int)ext_total;
     total += g.ncells;
+    ext_total += (int64_t)g.edims[0] * g.edims[1] * g.edims[2];
+    if (ext_total > ((int64_t)1 << 31) - 2) throw Error(PFW_INVALID_ARGUMENT, "extended cell grid too large");
     if (total > ((int64_t)1 << 31) - 2) throw Error(PFW_INVALID_ARGUMENT, "cell grid too large");
     cell0[k + 1] = (int32_t)total;
     open_any |= g.open_any != 0;
@@ -217,6 +227,8 @@ void nl_configure(pfw_ctx* ctx, double c) {
   ctx->ncells = (int)total;
   ctx->open_any = open_any;
   ctx->cutoff = c;
+  ctx->verlet.ext_cells = ext_total;
+  ctx->verlet.valid = false;
   std::vector<int32_t> cell_cfg(total);
   for (int k = 0; k < ncfg; ++k)
     for (int32_t q = cell0[k]; q < cell0[k + 1]; ++q) cell_cfg[q] = k;
@@ -668,6 +680,7 @@ void scan_counts(pfw_ctx* ctx, const int32_t* count, int m, int32_t* start, int3
 }
 
 void nl_build(pfw_ctx* ctx) {
+  ++ctx->build_gen;  // sorted indices change: a Verlet list made before is stale
   SystemBuffers& s = ctx->sys;
   hipStream_t st = ctx->stream;
   const int64_t n = s.n;

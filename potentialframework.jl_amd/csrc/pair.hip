@@ -23,11 +23,6 @@ namespace pfw {
 
 constexpr int kCand = 256;   // candidates per LDS chunk
 constexpr int kEnt = 64;     // stencil entries per entry chunk
-constexpr int kMaxZ = 16;    // species table size limit
-
-struct LJConsts {
-  double c4e, c24e, sig2, eshift;
-};
 
 template <typename T>
 struct Band;
@@ -80,13 +75,6 @@ __device__ inline double prefilter_r2(const Cand<double>& c, double xi, double y
   return dx * dx + dy * dy + dz * dz;
 }
 
-// Block index -> cell, giving each of the 8 XCDs (dispatched round-robin) a
-// contiguous range of cells so neighbouring cells share an L2.
-__device__ inline int xcd_remap(int b, int nb) {
-  int q = nb / 8, r = nb % 8;
-  int x = b % 8, idx = b / 8;
-  return x < r ? x * (q + 1) + idx : r * (q + 1) + (x - r) * q + idx;
-}
 
 template <typename T, int KIND>
 __global__ __launch_bounds__(64) void k_pair(const Grid* __restrict__ grids, const int32_t* __restrict__ cell_cfg,
@@ -323,18 +311,23 @@ void record_stage(pfw_ctx* ctx, int k) {
   if (ctx->timing) PFW_HIP(hipEventRecord(ctx->ev[k], ctx->stream));
 }
 
-template <typename T, int KIND>
-static void launch_pair(pfw_ctx* ctx, uint32_t mask) {
-  SystemBuffers& s = ctx->sys;
+LJConsts lj_consts(const PairModelDev& m) {
   LJConsts lj{0, 0, 0, 0};
-  if (KIND == PFW_PAIR_LJ) {
-    double eps = ctx->pair.params[0], sig = ctx->pair.params[1], c = ctx->pair.cutoff;
+  if (m.kind == PFW_PAIR_LJ) {
+    double eps = m.params[0], sig = m.params[1], c = m.cutoff;
     double rc = sig / c, qc2 = rc * rc, qc = qc2 * qc2 * qc2;
     lj.c4e = 4.0 * eps;
     lj.c24e = 24.0 * eps;
     lj.sig2 = sig * sig;
     lj.eshift = 4.0 * eps * (qc * qc - qc);
   }
+  return lj;
+}
+
+template <typename T, int KIND>
+static void launch_pair(pfw_ctx* ctx, uint32_t mask) {
+  SystemBuffers& s = ctx->sys;
+  const LJConsts lj = lj_consts(ctx->pair);
   const size_t dyn = KIND == PFW_PAIR_SMOOTH ? 2 * sizeof(T) * ctx->pair.nz * ctx->pair.nz : 0;
   hipLaunchKernelGGL((k_pair<T, KIND>), dim3(ctx->ncells), dim3(kWave), dyn, ctx->stream, s.grids, s.cell_cfg, s.n,
                      s.start, s.xs, s.sps, s.perm, ctx->pair.d_params, ctx->pair.nz, lj, mask, ctx->d_F,
@@ -347,6 +340,11 @@ void pair_run(pfw_ctx* ctx, uint32_t mask, int precision) {
     throw Error(PFW_STALE, "pair model cutoff differs from the system's cell-list cutoff; set the system again");
   if (ctx->pair.kind == PFW_PAIR_SMOOTH && ctx->pair.nz > kMaxZ)
     throw Error(PFW_INVALID_ARGUMENT, "too many species for the pair table");
+  if (ctx->verlet.skin > 0 && mask != 0 && ctx->sys.n > 0) {
+    verlet_pair_run(ctx, mask, precision);  // ends with the outputs copied and the stream synchronised
+    pair_finish(ctx, mask);
+    return;
+  }
   nl_build(ctx);
   record_stage(ctx, 3);
   record_stage(ctx, 4);
@@ -384,6 +382,12 @@ void pair_run(pfw_ctx* ctx, uint32_t mask, int precision) {
   PFW_HIP(hipMemcpyAsync(ctx->h_out, ctx->d_out, 11 * sizeof(double), hipMemcpyDeviceToHost, st));
   record_stage(ctx, 6);
   PFW_HIP(hipStreamSynchronize(st));
+  pair_finish(ctx, mask);
+}
+
+// After the step's outputs reached h_out: per-configuration rows, stage
+// times, and the status check (DomainError on a singular or nonfinite result).
+void pair_finish(pfw_ctx* ctx, uint32_t mask) {
   if (ctx->ncfg > 1) {
     ctx->h_cfg_out.resize(11 * (size_t)ctx->ncfg);
     PFW_HIP(hipMemcpy(ctx->h_cfg_out.data(), ctx->d_cfg_out, ctx->h_cfg_out.size() * sizeof(double),

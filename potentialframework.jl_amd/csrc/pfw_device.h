@@ -9,6 +9,22 @@
 namespace pfw {
 
 constexpr int kWave = 64;
+constexpr int kMaxZ = 16;  // species table size limit of the pair kernels
+
+// Energy-shifted Lennard-Jones constants (lennard_jones.jl:5-17): 4 eps, 24 eps,
+// sigma^2 and the energy at the cutoff.
+struct LJConsts {
+  double c4e, c24e, sig2, eshift;
+};
+LJConsts lj_consts(const PairModelDev& m);
+
+// Block index -> work item, giving each of the 8 XCDs (dispatched round-robin)
+// a contiguous range so that neighbouring items share an L2.
+__device__ inline int xcd_remap(int b, int nb) {
+  int q = nb / 8, r = nb % 8;
+  int x = b % 8, idx = b / 8;
+  return x < r ? x * (q + 1) + idx : r * (q + 1) + (x - r) * q + idx;
+}
 
 // Julia `mod(x::Float64, y::Float64)`: rem-based, result carries the sign of y.
 __device__ __host__ inline double jl_mod(double x, double y) {

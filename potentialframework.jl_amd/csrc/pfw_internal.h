@@ -54,10 +54,15 @@ struct Grid {
   double recip[9];  // its inverse
   double L[3];      // diag(H) for the minimum-image path
   double c, c2;
-  double reach[3];
+  double cs2;         // squared radius of the Verlet list (cutoff + skin, padded); c2 without a skin
+  double reach[3];    // cell reach of the search radius (cutoff + skin)
   double sizecell[3];  // periodic axes: extent/dims; open axes: device-computed
   double origin[3];    // open axes: device-computed minimum bin coordinate
   int used[3];         // open axes: device-computed used dims (<= dims)
+  // Verlet ghosts: the grid extended by the stencil radius on every periodic
+  // axis; extended cells outside the real range hold images of real cells
+  int ecell0;    // first extended cell of this configuration
+  int edims[3];  // dims + 2 radius on periodic axes, dims on open axes
 };
 
 // Device buffers for one system.
@@ -89,6 +94,42 @@ struct PairModelDev {
   double cutoff = 0;
   std::vector<double> params;  // host copy
   double* d_params = nullptr;  // SMOOTH: nz*nz amplitudes; LJ: eps, sigma
+};
+
+// Verlet list at cutoff + skin, reused while no atom has moved half the skin
+// (the reference's GPUConfig(skin) connectivity cache, NeighbourList/gpu.jl:504-549).
+// Periodic images are explicit ghost entries, so a list entry is a bare index
+// into the extended position array xe: [0, n) the sorted atoms, [n, n + G) the
+// ghosts in extended-cell order.
+struct VerletDev {
+  double skin = 0;              // 0: no list, fresh cells every step
+  bool valid = false;
+  uint64_t gen = 0;             // cell-build generation the list was made from
+  int64_t ext_cells = 0;        // extended cells of all configurations
+  int64_t n_ext = 0;            // n + ghosts
+  int64_t ext_capacity = 0;     // allocated entries of xe / off / srcp
+  int ecap = 0;                 // list slots per atom
+  int64_t list_capacity = 0;    // allocated slots
+  int parity = 0;               // which displacement word the next step accumulates into
+  int64_t rebuilds = 0, reuses = 0;
+  double last_max_disp = 0;     // largest displacement since the build, last step (Å)
+  int32_t* ecfg0 = nullptr;     // ncfg + 1 extended-cell offsets
+  int32_t* gcount = nullptr;    // ext_cells: ghosts of each extended cell
+  int32_t* gstart = nullptr;    // ext_cells + 1
+  int32_t* gcursor = nullptr;   // scan scratch
+  int32_t* gblock = nullptr;
+  int32_t* nbr = nullptr;       // ecap x n, column-major (slot-major): entry index into xe
+  int32_t* ncount = nullptr;    // n: list length of each sorted atom
+  double* off = nullptr;        // 4 x n_ext: build position minus caller position, species bits
+  double* xe = nullptr;         // 4 x n_ext: current extended positions, species bits
+  int32_t* srcp = nullptr;      // n_ext: original atom of each entry
+  double* Xref = nullptr;       // 3 x n caller positions at the build (AoS, original order)
+  unsigned long long* maxd2 = nullptr;  // 2 words: largest squared displacement (bits), double-buffered
+  unsigned long long* h_maxd2 = nullptr;  // pinned copy
+  int32_t* status = nullptr;    // 4: unused, largest degree, overflow
+  int32_t* h_status = nullptr;  // pinned
+  double* part = nullptr;       // list kernel energy / virial records
+  int64_t part_capacity = 0;
 };
 
 constexpr int kStages = 6;  // binning, scan, scatter+sort, environments, interaction, reduction
@@ -170,6 +211,8 @@ struct pfw_ctx {
   bool have_system = false;
   pfw::PairModelDev pair;
   pfw::AceDev ace;
+  pfw::VerletDev verlet;
+  uint64_t build_gen = 0;  // incremented by every cell build
   // outputs
   double* d_F = nullptr;        // 3 x n original order
   double* d_partials = nullptr;  // per-wave E + 6 W partials
@@ -178,6 +221,9 @@ struct pfw_ctx {
   double* d_cfg = nullptr;       // per-configuration sums (ncfg x 16)
   double* d_cfg_out = nullptr;   // per-configuration E, W[9], status (ncfg x 11)
   std::vector<double> h_cfg_out;
+  double* d_Xstage = nullptr;    // 3 x n staging copy of caller device positions
+  int32_t* d_flag = nullptr;     // validation flag of a device position update
+  int32_t* h_flag = nullptr;     // pinned copy
   double* d_out = nullptr;       // E, W[9], status
   double* h_out = nullptr;       // pinned copy of d_out
   uint32_t last_mask = 0;
@@ -190,7 +236,7 @@ struct pfw_ctx {
 
 namespace pfw {
 // nl.hip
-void nl_configure(pfw_ctx* ctx, double cutoff);
+void nl_configure(pfw_ctx* ctx, double cutoff, double skin);
 void batch_reduce(pfw_ctx* ctx, const double* records, int width, bool cells, int kind);
 void nl_build(pfw_ctx* ctx);
 void nl_export_pairs(pfw_ctx* ctx, int32_t* i, int32_t* j, int32_t* S, double* R, int64_t cap,
@@ -200,8 +246,12 @@ void scan_counts(pfw_ctx* ctx, const int32_t* count, int m, int32_t* start, int3
                  int32_t* cursor);
 // pair.hip
 void pair_run(pfw_ctx* ctx, uint32_t mask, int precision);
+void pair_finish(pfw_ctx* ctx, uint32_t mask);
 void record_stage(pfw_ctx* ctx, int k);
 void pair_adjoint(pfw_ctx* ctx, const double* Ebar, const double* Fbar, const double* Wbar, double* gA);
+// verlet.hip
+void verlet_free(pfw_ctx* ctx);
+void verlet_pair_run(pfw_ctx* ctx, uint32_t mask, int precision);
 // ace.hip
 void ace_set_model(pfw_ctx* ctx, const pfw_ace_model* m);
 void ace_set_parameters(pfw_ctx* ctx, const double* WB, const double* Wpair);

@@ -36,17 +36,39 @@ class Workspace:
 
     ``precision``: ``"float64"`` or ``"mixed"``. ``device``: HIP device ordinal.
     ``timings`` holds per-stage device milliseconds of the last call when
-    ``timing=True`` (the reference's ``w.timings``)."""
+    ``timing=True`` (the reference's ``w.timings``). ``skin`` (Å, default 0)
+    is the reference's ``GPUConfig(skin=...)`` (gpu.jl:61-72, 504-549): with
+    ``skin > 0`` pair evaluations keep a neighbour list at cutoff + skin while
+    only positions change and no atom has moved skin/2 since its build;
+    ``rebuilds`` counts the builds."""
 
-    def __init__(self, precision="float64", device=0, timing=False):
+    def __init__(self, precision="float64", device=0, timing=False, skin=0.0):
         self.precision = precision
         self.mode = precision_mode(precision)
+        skin = float(skin)
+        if not (np.isfinite(skin) and skin >= 0):
+            raise ArgumentError("skin must be finite and nonnegative (Å)")
         self.ctx = Context(device)
         self._key = None
         self._model_key = None
         self.timings = {}
         self.timing = bool(timing)
         self.ctx.call("pfw_set_timing", int(self.timing))
+        self.skin = skin
+        self.ctx.call("pfw_set_skin", ctypes.c_double(skin))
+
+    def verlet_info(self):
+        """``rebuilds``, ``reuses``, ``ghosts``, ``slots`` (per atom) and the last
+        step's largest displacement since the build (Å)."""
+        info = np.zeros(4, np.int64)
+        md = np.zeros(1)
+        self.ctx.call("pfw_verlet_info", ptr(info, ctypes.c_int64), ptr(md))
+        return dict(rebuilds=int(info[0]), reuses=int(info[1]), ghosts=int(info[2]), slots=int(info[3]),
+                    max_displacement=float(md[0]))
+
+    @property
+    def rebuilds(self):
+        return self.verlet_info()["rebuilds"]
 
     # -- geometry ---------------------------------------------------------------------
     def prepare(self, sys, cutoff, channels):
