@@ -327,17 +327,24 @@ __global__ __launch_bounds__(64) void k_pair_list(int64_t n, int sentinel, const
   int kmax = cnt;
   for (int s = kWave / 2; s > 0; s >>= 1) kmax = max(kmax, __shfl_xor(kmax, s, kWave));
   T er = 0, gxr = 0, gyr = 0, gzr = 0, w0 = 0, w1 = 0, w2 = 0, w3 = 0, w4 = 0, w5 = 0;
+  // the next step's indices are loaded while this step's positions arrive and
+  // its pairs are computed; the list has one spare step of slots past the row
+  // capacity (a multiple of P U), so the prefetch after the last step stays in bounds
+  int jn[U];
+  const int64_t stride = (int64_t)P * U * n;
+  const int32_t* row = nbr + (int64_t)part * U * n + ii;
+#pragma unroll
+  for (int u = 0; u < U; ++u) jn[u] = row[(int64_t)u * n];
   for (int k0 = part * U; k0 < kmax; k0 += P * U) {
     int j[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int k = k0 + u;  // < the row capacity (a multiple of P U)
-      const int v = nbr[(int64_t)k * n + ii];
-      j[u] = k < cnt ? v : sentinel;
-    }
+    for (int u = 0; u < U; ++u) j[u] = k0 + u < cnt ? jn[u] : sentinel;
     double4 xj[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) xj[u] = xe[j[u]];
+    row += stride;
+#pragma unroll
+    for (int u = 0; u < U; ++u) jn[u] = row[(int64_t)u * n];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const double ddx = xj[u].x - xi.x, ddy = xj[u].y - xi.y, ddz = xj[u].z - xi.z;
@@ -487,9 +494,10 @@ static void grow_list(pfw_ctx* ctx, int degree) {
   const int step = kListParts * kListUnroll;  // the interaction kernel reads whole steps of slots
   V.ecap = std::max(V.ecap, (degree + std::max(8, degree / 8) + step - 1) / step * step);  // headroom for the next builds
   if ((int64_t)V.ecap * n > (int64_t)1 << 33) throw Error(PFW_INVALID_ARGUMENT, "Verlet list too large");
-  if ((int64_t)V.ecap * n > V.list_capacity) {
-    dalloc(V.nbr, (size_t)V.ecap * n);
-    V.list_capacity = (int64_t)V.ecap * n;
+  const int64_t slots = (int64_t)(V.ecap + step) * n;  // + the interaction kernel's prefetch step
+  if (slots > V.list_capacity) {
+    dalloc(V.nbr, (size_t)slots);
+    V.list_capacity = slots;
   }
 }
 
