@@ -199,7 +199,7 @@ def bench_fitting_set(pfw, lib, _capi, model, ps, pair_model, args):
     return out
 
 
-def bench_verlet(pfw, lib, _capi, X, H, pbc, sp, model, labels, skin, prec, mode, steps=40, amp=0.005):
+def bench_verlet(pfw, lib, _capi, X, H, pbc, sp, model, labels, skin, prec, mode, steps=40, amp=0.005, ace=False):
     """The reference's evolving-trajectory measurement (benchmark/gpu_pairs.jl:118-142):
     raw coordinates x_s = x_0 + s v with v[a, i] = amp sin(a + 0.73 i) (1-based a, i),
     40 steps on a list at cutoff + skin built at x_0; each step's positions are
@@ -209,10 +209,18 @@ def bench_verlet(pfw, lib, _capi, X, H, pbc, sp, model, labels, skin, prec, mode
     i1 = np.arange(1, n + 1)[:, None]
     a1 = np.arange(1, 4)[None, :]
     vel = amp * np.sin(a1 + 0.73 * i1)
-    w = pfw.Workspace(precision=prec, skin=skin)
     sys0 = pfw.System(X, H, pbc, labels)
-    pfw.evaluate_(w, sys0, model)
-    pfw.evaluate_(w, sys0, model)
+    if ace:  # model is (resolved model, parameters)
+        calc = pfw.ACECalculator(model[0], model[1], precision=prec, skin=skin)
+        w = calc.workspace
+        calc.evaluate(sys0)
+        calc.evaluate(sys0)
+        stepfn = lib.pfw_ace_step
+    else:
+        w = pfw.Workspace(precision=prec, skin=skin)
+        pfw.evaluate_(w, sys0, model)
+        pfw.evaluate_(w, sys0, model)
+        stepfn = lib.pfw_pair_step
     h = w.ctx.h
     P = ctypes.POINTER(ctypes.c_double)
     times, rebuilt = [], []
@@ -221,7 +229,7 @@ def bench_verlet(pfw, lib, _capi, X, H, pbc, sp, model, labels, skin, prec, mode
         _capi.check(h, lib.pfw_update_positions(h, Xs.ctypes.data_as(P)))
         before = w.rebuilds
         t0 = time.perf_counter()
-        _capi.check(h, lib.pfw_pair_step(h, 7, mode))
+        _capi.check(h, stepfn(h, 7, mode))
         times.append(time.perf_counter() - t0)
         rebuilt.append(w.rebuilds > before)
     times = np.array(times)
@@ -233,7 +241,7 @@ def bench_verlet(pfw, lib, _capi, X, H, pbc, sp, model, labels, skin, prec, mode
     stage = np.zeros(len(STAGES))
     buf = np.zeros(len(STAGES))
     for _ in range(10):
-        _capi.check(h, lib.pfw_pair_step(h, 7, mode))
+        _capi.check(h, stepfn(h, 7, mode))
         lib.pfw_get_timings(h, buf.ctypes.data_as(P), len(STAGES))
         stage += buf
     _capi.check(h, lib.pfw_set_timing(h, 0))
@@ -244,7 +252,7 @@ def bench_verlet(pfw, lib, _capi, X, H, pbc, sp, model, labels, skin, prec, mode
         "value": round(n / float(times.mean()) / 1e6, 3), "unit": "Matom-evals/s",
         "reuse_median_ms": round(1e3 * float(np.median(reuse)), 4) if len(reuse) else None,
         "rebuild_ms": [round(1e3 * float(t), 4) for t in times[rebuilt]],
-        "ghosts": info["ghosts"], "slots_per_atom": info["slots"],
+        **({} if ace else {"ghosts": info["ghosts"], "slots_per_atom": info["slots"]}),
         "reuse_stages_ms": {k: round(float(v), 5) for k, v in zip(STAGES, stage)},
     }
 
@@ -421,6 +429,11 @@ def main():
             "fp64_kernel_ms": round(ka64, 5),
             "stages_ms": {k: round(float(v), 5) for k, v in zip(STAGES, ace["mixed"]["stage_ms"])},
         }
+        if not args.no_verlet:
+            line["ace"]["verlet"] = {
+                "workload": "bcc_binary_ace_32k, trajectory x_s = x_0 + s v (environments at cutoff + skin)",
+                **{prec: bench_verlet(pfw, lib, _capi, Xa, Ha, pba, spa, (model, ps), labels, 0.5, prec, mode, ace=True)
+                   for prec, mode in (("mixed", _capi.PFW_MIXED), ("float64", _capi.PFW_FP64))}}
 
     # ---------------------------------------------------------------- fitting set (batch + adjoints)
     if not args.no_batch and not args.no_ace:

@@ -190,11 +190,14 @@ def load_resolved(source):
 
 
 class ACECalculator:
-    """``ACECalculator(model, ps; precision)`` (calculators.jl:6-29) on one device context."""
+    """``ACECalculator(model, ps; precision)`` (calculators.jl:6-29) on one device context.
+    ``skin`` (Å) keeps the environments at cutoff + skin between evaluations of
+    the same system with moved atoms (GPUExecutor with ``GPUConfig(skin)``,
+    src/submodules/ACE/gpu.jl:301-343)."""
 
-    def __init__(self, model, parameters=None, precision="float64", device=0, workspace=None):
+    def __init__(self, model, parameters=None, precision="float64", device=0, workspace=None, skin=0.0):
         self.model = model
-        self.workspace = workspace or Workspace(precision=precision, device=device)
+        self.workspace = workspace or Workspace(precision=precision, device=device, skin=skin)
         self.precision = self.workspace.precision
         st, keep = model._struct()
         self.workspace.ctx.call("pfw_set_ace_model", ctypes.byref(st))

@@ -531,7 +531,7 @@ __global__ __launch_bounds__(64) void k_ace_edges(const Grid* __restrict__ grids
                                                   const int32_t* __restrict__ start, int pass, int ecap,
                                                   int32_t* __restrict__ ecount, int32_t* __restrict__ enb,
                                                   int32_t* __restrict__ eent, double* __restrict__ eR,
-                                                  int32_t* __restrict__ status) {
+                                                  int32_t* __restrict__ status, int skin_rows) {
   __shared__ int s_pref[kWave + 1], s_start[kWave], s_lat[kWave][3];
   __shared__ double s_sh[kWave][3];
   __shared__ float s_ax[kWave], s_ay[kWave], s_az[kWave];  // the centres, relative to the cell's first atom
@@ -546,12 +546,15 @@ __global__ __launch_bounds__(64) void k_ace_edges(const Grid* __restrict__ grids
   const double rx = xs[b], ry = xs[n + b], rz = xs[2 * n + b];
   // FP32 prefilter: pairs outside a thin band above the cutoff never reach the
   // exact FP64 test (which decides, and gives R, for the rest)
-  const float c2hi = (float)(g.c2 * (1.0 + 1e-4));
+  // skin rows: every stencil image inside the list radius, R = (x_j - x_i) + H s
+  // (so that the reverse edge's R is exactly -R on every path)
+  const double r2max = skin_rows ? g.cs2 : g.c2;
+  const float c2hi = (float)(r2max * (1.0 + 1e-4));
   for (int ag = 0; ag < cnt; ag += kWave) {
     const int na = min(kWave, cnt - ag);
     const int64_t mine = b + ag + lane;
     int pos = 0;  // the centre's next slot (lane of the same index)
-    int deg = 0;
+    int deg = 0, cut = 0;
     if (lane < na) {
       s_ax[lane] = (float)(xs[mine] - rx);
       s_ay[lane] = (float)(xs[n + mine] - ry);
@@ -617,11 +620,23 @@ __global__ __launch_bounds__(64) void k_ace_edges(const Grid* __restrict__ grids
           double d[3] = {0, 0, 0};
           const float ux = fx - s_ax[a], uy = fy - s_ay[a], uz = fz - s_az[a];
           if (ux * ux + uy * uy + uz * uz < c2hi && !(lo == ez && m == (int)k)) {
-            int img[3];
-            hit = exact_member(g, xs[k], xs[n + k], xs[2 * n + k], xj, yj, zj, s_lat[lo], s_sh[lo], d, img);
+            if (skin_rows) {
+#pragma clang fp contract(off)
+              d[0] = (xj - xs[k]) + s_sh[lo][0];
+              d[1] = (yj - xs[n + k]) + s_sh[lo][1];
+              d[2] = (zj - xs[2 * n + k]) + s_sh[lo][2];
+              hit = d[0] * d[0] + d[1] * d[1] + d[2] * d[2] < r2max;
+            } else {
+              int img[3];
+              hit = exact_member(g, xs[k], xs[n + k], xs[2 * n + k], xj, yj, zj, s_lat[lo], s_sh[lo], d, img);
+            }
           }
           const uint64_t mask = __ballot(hit);
           const int h = __popcll(mask);
+          if (skin_rows) {  // of which inside the cutoff itself (a launch-shape hint)
+            const int hc = __popcll(__ballot(hit && d[0] * d[0] + d[1] * d[1] + d[2] * d[2] < g.c2));
+            if (lane == a) cut += hc;
+          }
           const int base = __shfl(pos, a, kWave);  // outside the branch: the source lane must be active
           const int slot = base + __popcll(mask & ((1ull << lane) - 1));
           if (pass && hit && slot < ecap) {
@@ -640,10 +655,17 @@ __global__ __launch_bounds__(64) void k_ace_edges(const Grid* __restrict__ grids
       }
       __syncthreads();
     }
-    if (lane < na) {
-      ecount[mine] = min(deg, ecap);
-      atomicMax(status + 1, deg);
-      if (deg > ecap) atomicOr(status + 2, 1);  // capacity too small: the host repeats the step
+    if (lane < na) ecount[mine] = min(deg, ecap);
+    // one atomic per wavefront and word (same-address atomics serialise across the device)
+    int dmax = deg, cmax = cut;
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+      dmax = max(dmax, __shfl_xor(dmax, off, kWave));
+      cmax = max(cmax, __shfl_xor(cmax, off, kWave));
+    }
+    if (lane == 0) {
+      atomicMax(status + 1, dmax);
+      if (dmax > ecap) atomicOr(status + 2, 1);  // capacity too small: the host repeats the step
+      if (skin_rows) atomicMax(status + 3, cmax);
     }
   }
 }
@@ -713,6 +735,100 @@ __global__ __launch_bounds__(64) void k_ace_force(int64_t n, int ecap, const int
     F[3 * o + 1] = f1;
     F[3 * o + 2] = f2;
   }
+}
+
+// ---- Verlet reuse of the environments (skin > 0; the reference's GPUExecutor
+// with GPUConfig(skin), src/submodules/ACE/gpu.jl:301-343 over gpu_skin!,
+// src/submodules/NeighbourList/gpu.jl:510-549)
+
+// Displacement of every sorted atom since the build (SoA) and the largest
+// squared one, as the bits of a nonnegative double in words[parity]; the other
+// word is cleared for the next step.
+__global__ __launch_bounds__(256) void k_ace_delta(int64_t n, const int32_t* __restrict__ perm,
+                                                   const double* __restrict__ X, const double* __restrict__ Xref,
+                                                   double* __restrict__ dX, unsigned long long* __restrict__ words,
+                                                   int parity) {
+  __shared__ double wmax[256 / kWave];
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double m2 = 0;
+  if (k < n) {
+    const int64_t p = perm[k];
+    const double dx = X[3 * p] - Xref[3 * p], dy = X[3 * p + 1] - Xref[3 * p + 1], dz = X[3 * p + 2] - Xref[3 * p + 2];
+    dX[k] = dx;
+    dX[n + k] = dy;
+    dX[2 * n + k] = dz;
+    m2 = dx * dx + dy * dy + dz * dz;
+  }
+  for (int s = kWave / 2; s > 0; s >>= 1) m2 = fmax(m2, __shfl_xor(m2, s, kWave));
+  const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+  if (lane == 0) wmax[wid] = m2;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = wmax[0];
+    for (int w = 1; w < (int)(blockDim.x / kWave); ++w) m = fmax(m, wmax[w]);
+    if (m > 0) atomicMax(&words[parity], (unsigned long long)__double_as_longlong(m));
+    if (blockIdx.x == 0) words[1 - parity] = 0;
+  }
+}
+
+// The edges of site i inside the cutoff now, R = R_build + (dx_j - dx_i), in
+// skin-row order (a ballot compaction, one wavefront per site). The reverse
+// edge's R is exactly -R (R_build negates exactly, and so does the
+// displacement difference), so an edge and its reverse are kept together.
+__global__ __launch_bounds__(64) void k_ace_compact(int64_t n, int ecap, const int32_t* __restrict__ ecount_s,
+                                                    const int32_t* __restrict__ enb_s,
+                                                    const double* __restrict__ eR_s, const double* __restrict__ dX,
+                                                    double c2, int32_t* __restrict__ ecount,
+                                                    int32_t* __restrict__ enb, double* __restrict__ eR,
+                                                    int32_t* __restrict__ cslot, int32_t* __restrict__ corig) {
+  const int64_t i = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int64_t base = i * ecap;
+  const int cnt = ecount_s[i];
+  const double xi = dX[i], yi = dX[n + i], zi = dX[2 * n + i];
+  int pos = 0;
+  for (int q0 = 0; q0 < cnt; q0 += kWave) {
+    const int q = q0 + lane;
+    bool act = false;
+    int j = 0;
+    double r[3] = {0, 0, 0};
+    if (q < cnt) {
+      const int64_t e = base + q;
+      j = enb_s[e];
+      r[0] = eR_s[3 * e] + (dX[j] - xi);
+      r[1] = eR_s[3 * e + 1] + (dX[n + j] - yi);
+      r[2] = eR_s[3 * e + 2] + (dX[2 * n + j] - zi);
+      act = r[0] * r[0] + r[1] * r[1] + r[2] * r[2] < c2;
+    }
+    const uint64_t mask = __ballot(act);
+    const int c = pos + __popcll(mask & ((1ull << lane) - 1));
+    if (q < cnt) cslot[base + q] = act ? (int32_t)(base + c) : -1;
+    if (act) {
+      const int64_t o = base + c;
+      enb[o] = j;
+      eR[3 * o] = r[0];
+      eR[3 * o + 1] = r[1];
+      eR[3 * o + 2] = r[2];
+      corig[o] = (int32_t)(base + q);
+    }
+    pos += __popcll(mask);
+  }
+  if (lane == 0) ecount[i] = pos;
+}
+
+// Reverse edges of the compacted rows from the skin rows' reverse index.
+__global__ void k_ace_revmap(int64_t slots, int ecap, const int32_t* __restrict__ ecount,
+                             const int32_t* __restrict__ corig, const int32_t* __restrict__ erev_s,
+                             const int32_t* __restrict__ cslot, int32_t* __restrict__ erev,
+                             int32_t* __restrict__ status) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= slots) return;
+  const int64_t i = e / ecap;
+  if (e - i * ecap >= ecount[i]) return;
+  const int rs = erev_s[corig[e]];
+  const int r = rs >= 0 ? cslot[rs] : -1;
+  erev[e] = r;
+  if (r < 0) atomicOr(status, 1);
 }
 
 // siteE and siteW packed as 10-column records for the two-stage reduction.
@@ -915,6 +1031,22 @@ void ace_free_edges(pfw_ctx* ctx) {
   dfree(a.status);
   a.edge_cap = 0;
   a.ecap = 0;
+  dfree(a.corig);
+  dfree(a.ecount_s);
+  dfree(a.enb_s);
+  dfree(a.eent_s);
+  dfree(a.erev_s);
+  dfree(a.eR_s);
+  dfree(a.cslot);
+  dfree(a.Xref);
+  dfree(a.dX);
+  dfree(a.maxd2);
+  if (a.h_maxd2) (void)hipHostFree(a.h_maxd2);
+  a.h_maxd2 = nullptr;
+  a.skin_valid = false;
+  a.compact_degree = 0;
+  a.ecap_s = 0;
+  a.skin_cap = 0;
 }
 
 void ace_free(pfw_ctx* ctx) {
@@ -1370,47 +1502,29 @@ static void dispatch_sites(pfw_ctx* ctx, int grads) {
 // build counts once to size ecap; later builds write directly and report a
 // capacity overflow in status[2] (the caller repeats the step with the degree
 // found in status[1]).
+static void ace_rows_alloc(pfw_ctx* ctx, int ecap);
+
 static void ace_environment(pfw_ctx* ctx, int rev) {
   AceDev& ad = ctx->ace;
   SystemBuffers& s = ctx->sys;
   hipStream_t st = ctx->stream;
   const int64_t n = s.n;
-  if (!ad.ecount) {
-    PFW_HIP(hipMalloc(&ad.ecount, n * sizeof(int32_t)));
-    PFW_HIP(hipMalloc(&ad.siteE, n * sizeof(double)));
-    PFW_HIP(hipMalloc(&ad.Abuf, std::max<size_t>(1, (size_t)n * ad.ndens) * sizeof(double)));
-    PFW_HIP(hipMalloc(&ad.siteW, 9 * n * sizeof(double)));
-    PFW_HIP(hipMalloc(&ad.rec, 10 * n * sizeof(double)));
-    PFW_HIP(hipMalloc(&ad.status, 4 * sizeof(int32_t)));
-    PFW_HIP(hipHostMalloc(&ad.h_status, 4 * sizeof(int32_t), hipHostMallocDefault));
-  }
+  ace_rows_alloc(ctx, std::max(ad.ecap, 1));
   if (ad.ecap == 0) {  // size the rows: one counting pass
     PFW_HIP(hipMemsetAsync(ad.status, 0, 4 * sizeof(int32_t), st));
     hipLaunchKernelGGL(k_ace_edges, dim3((unsigned)ctx->ncells), dim3(kWave), 0, st, s.grids, s.cell_cfg, n, s.xs,
-                       s.start, 0, 0, ad.ecount, (int32_t*)nullptr, (int32_t*)nullptr, (double*)nullptr, ad.status);
+                       s.start, 0, 0, ad.ecount, (int32_t*)nullptr, (int32_t*)nullptr, (double*)nullptr, ad.status, 0);
     PFW_HIP(hipGetLastError());
     PFW_HIP(hipMemcpyAsync(ad.h_status, ad.status, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
     PFW_HIP(hipStreamSynchronize(st));
     ad.max_degree = ad.h_status[1];
     ad.ecap = std::max(1, ad.max_degree + std::max(4, ad.max_degree / 8));
   }
+  ace_rows_alloc(ctx, ad.ecap);
   const int64_t slots = n * (int64_t)ad.ecap;
-  if (slots > ad.edge_cap || !ad.enb) {
-    dfree(ad.enb);
-    dfree(ad.eR);
-    dfree(ad.erev);
-    dfree(ad.eent);
-    dfree(ad.eg);
-    PFW_HIP(hipMalloc(&ad.enb, slots * sizeof(int32_t)));
-    PFW_HIP(hipMalloc(&ad.eR, 3 * slots * sizeof(double)));
-    PFW_HIP(hipMalloc(&ad.erev, slots * sizeof(int32_t)));
-    PFW_HIP(hipMalloc(&ad.eent, slots * sizeof(int32_t)));
-    PFW_HIP(hipMalloc(&ad.eg, 3 * slots * sizeof(double)));
-    ad.edge_cap = slots;
-  }
   PFW_HIP(hipMemsetAsync(ad.status, 0, 4 * sizeof(int32_t), st));
   hipLaunchKernelGGL(k_ace_edges, dim3((unsigned)ctx->ncells), dim3(kWave), 0, st, s.grids, s.cell_cfg, n, s.xs,
-                     s.start, 1, ad.ecap, ad.ecount, ad.enb, ad.eent, ad.eR, ad.status);
+                     s.start, 1, ad.ecap, ad.ecount, ad.enb, ad.eent, ad.eR, ad.status, 0);
   PFW_HIP(hipGetLastError());
   if (rev)
     hipLaunchKernelGGL(k_ace_rev, dim3(nblk(slots, 256)), dim3(256), 0, st, slots, ad.ecap, s.grids, s.atom_cfg,
@@ -1427,6 +1541,183 @@ static bool ace_overflowed(AceDev& ad) {
   return true;
 }
 
+// Sites, force gather and reductions over the current rows (ecount, enb, eR,
+// erev), ending with the output copies and the stream synchronisation.
+static void ace_evaluate_rows(pfw_ctx* ctx, int grads, int precision) {
+  AceDev& ad = ctx->ace;
+  SystemBuffers& s = ctx->sys;
+  hipStream_t st = ctx->stream;
+  const int64_t n = s.n;
+  if (precision == PFW_MIXED)
+    dispatch_sites<float>(ctx, grads);
+  else
+    dispatch_sites<double>(ctx, grads);
+  record_stage(ctx, 5);
+  if (grads && precision == PFW_MIXED)
+    hipLaunchKernelGGL(k_ace_force<float>, dim3((unsigned)n), dim3(kWave), 0, st, n, ad.ecap, ad.ecount, ad.erev,
+                       (const float*)ad.eg, s.perm, ctx->d_F);
+  else if (grads)
+    hipLaunchKernelGGL(k_ace_force<double>, dim3((unsigned)n), dim3(kWave), 0, st, n, ad.ecap, ad.ecount, ad.erev,
+                       (const double*)ad.eg, s.perm, ctx->d_F);
+  hipLaunchKernelGGL(k_ace_pack, dim3(nblk(n, 256)), dim3(256), 0, st, n, ad.siteE, ad.siteW, grads, ad.rec);
+  if (ctx->ncfg == 1) {
+    hipLaunchKernelGGL(k_colsum, dim3(kRedBlocks), dim3(256), 0, st, ad.rec, n, 10, ctx->d_red);
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(1024), 0, st, ctx->d_red, kRedBlocks, 10, 1, (const int32_t*)ad.status,
+                       ctx->d_out, ctx->d_cfg_out);
+  } else {
+    batch_reduce(ctx, ad.rec, 10, false, 1);
+  }
+  PFW_HIP(hipGetLastError());
+}
+
+// Buffers of the per-step rows, sized for n sites of `ecap` slots.
+static void ace_rows_alloc(pfw_ctx* ctx, int ecap) {
+  AceDev& ad = ctx->ace;
+  const int64_t n = ctx->sys.n;
+  if (!ad.ecount) {
+    PFW_HIP(hipMalloc(&ad.ecount, n * sizeof(int32_t)));
+    PFW_HIP(hipMalloc(&ad.siteE, n * sizeof(double)));
+    PFW_HIP(hipMalloc(&ad.Abuf, std::max<size_t>(1, (size_t)n * ad.ndens) * sizeof(double)));
+    PFW_HIP(hipMalloc(&ad.siteW, 9 * n * sizeof(double)));
+    PFW_HIP(hipMalloc(&ad.rec, 10 * n * sizeof(double)));
+    PFW_HIP(hipMalloc(&ad.status, 4 * sizeof(int32_t)));
+    PFW_HIP(hipHostMalloc(&ad.h_status, 4 * sizeof(int32_t), hipHostMallocDefault));
+  }
+  const int64_t slots = n * (int64_t)ecap;
+  if (slots > ad.edge_cap || !ad.enb) {
+    dfree(ad.enb);
+    dfree(ad.eR);
+    dfree(ad.erev);
+    dfree(ad.eent);
+    dfree(ad.eg);
+    dfree(ad.corig);
+    PFW_HIP(hipMalloc(&ad.enb, slots * sizeof(int32_t)));
+    PFW_HIP(hipMalloc(&ad.eR, 3 * slots * sizeof(double)));
+    PFW_HIP(hipMalloc(&ad.erev, slots * sizeof(int32_t)));
+    PFW_HIP(hipMalloc(&ad.eent, slots * sizeof(int32_t)));
+    PFW_HIP(hipMalloc(&ad.eg, 3 * slots * sizeof(double)));
+    PFW_HIP(hipMalloc(&ad.corig, slots * sizeof(int32_t)));
+    ad.edge_cap = slots;
+  }
+}
+
+// Skin rows at cutoff + skin from a fresh cell build, with their reverse
+// index (gpu_build_cells! + gpu_connectivity!, src/submodules/NeighbourList/gpu.jl:497-541).
+static void ace_skin_rebuild(pfw_ctx* ctx) {
+  AceDev& ad = ctx->ace;
+  SystemBuffers& s = ctx->sys;
+  hipStream_t st = ctx->stream;
+  const int64_t n = s.n;
+  nl_build(ctx);
+  record_stage(ctx, 3);
+  ace_rows_alloc(ctx, std::max(ad.ecap, 1));
+  if (!ad.ecount_s) {
+    PFW_HIP(hipMalloc(&ad.ecount_s, n * sizeof(int32_t)));
+    PFW_HIP(hipMalloc(&ad.Xref, 3 * n * sizeof(double)));
+    PFW_HIP(hipMalloc(&ad.dX, 3 * n * sizeof(double)));
+    PFW_HIP(hipMalloc(&ad.maxd2, 2 * sizeof(unsigned long long)));
+    PFW_HIP(hipHostMalloc(&ad.h_maxd2, 2 * sizeof(unsigned long long), hipHostMallocDefault));
+    PFW_HIP(hipMemsetAsync(ad.maxd2, 0, 2 * sizeof(unsigned long long), st));
+    ad.parity = 0;
+  }
+  auto build = [&](int pass) {
+    PFW_HIP(hipMemsetAsync(ad.status, 0, 4 * sizeof(int32_t), st));
+    hipLaunchKernelGGL(k_ace_edges, dim3((unsigned)ctx->ncells), dim3(kWave), 0, st, s.grids, s.cell_cfg, n, s.xs,
+                       s.start, pass, pass ? ad.ecap_s : 0, ad.ecount_s, ad.enb_s, ad.eent_s, ad.eR_s, ad.status, 1);
+    PFW_HIP(hipGetLastError());
+    if (pass)
+      hipLaunchKernelGGL(k_ace_rev, dim3(nblk(n * (int64_t)ad.ecap_s, 256)), dim3(256), 0, st, n * (int64_t)ad.ecap_s,
+                         ad.ecap_s, s.grids, s.atom_cfg, s.perm, ad.ecount_s, ad.enb_s, ad.eent_s, ad.eR_s, ad.erev_s,
+                         ad.status);
+    PFW_HIP(hipGetLastError());
+    PFW_HIP(hipMemcpyAsync(ad.h_status, ad.status, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    PFW_HIP(hipStreamSynchronize(st));
+  };
+  auto size_rows = [&](int degree) {
+    ad.ecap_s = std::max(ad.ecap_s, degree + std::max(4, degree / 8));
+    const int64_t slots = n * (int64_t)ad.ecap_s;
+    if (slots > ad.skin_cap) {
+      dfree(ad.enb_s);
+      dfree(ad.eent_s);
+      dfree(ad.erev_s);
+      dfree(ad.eR_s);
+      dfree(ad.cslot);
+      PFW_HIP(hipMalloc(&ad.enb_s, slots * sizeof(int32_t)));
+      PFW_HIP(hipMalloc(&ad.eent_s, slots * sizeof(int32_t)));
+      PFW_HIP(hipMalloc(&ad.erev_s, slots * sizeof(int32_t)));
+      PFW_HIP(hipMalloc(&ad.eR_s, 3 * slots * sizeof(double)));
+      PFW_HIP(hipMalloc(&ad.cslot, slots * sizeof(int32_t)));
+      ad.skin_cap = slots;
+    }
+  };
+  if (ad.ecap_s == 0) {  // first build: size the rows
+    build(0);
+    size_rows(ad.h_status[1]);
+  }
+  build(1);
+  if (ad.h_status[2]) {  // a row outgrew its capacity since the last build
+    size_rows(ad.h_status[1]);
+    build(1);
+  }
+  if (ad.h_status[2] || ad.h_status[0]) throw Error(PFW_DEVICE, "ACE skin rows could not be built consistently");
+  ad.skin_max_degree = ad.h_status[1];
+  ad.compact_degree = ad.h_status[3];
+  PFW_HIP(hipMemcpyAsync(ad.Xref, s.X, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, st));
+  ace_rows_alloc(ctx, ad.ecap_s);
+  ad.skin_valid = true;
+  ad.skin_gen = ctx->build_gen;
+  ++ctx->verlet.rebuilds;
+}
+
+// One ACE step on the skin rows: displacements, compaction to the edges
+// inside the cutoff now, then the usual sites, force gather and reductions.
+// As for the pair list, the step is evaluated on the current rows and
+// repeated after a rebuild when the displacement read back with the outputs
+// shows that the rows may miss an edge.
+static void ace_skin_step(pfw_ctx* ctx, int grads, int precision) {
+  AceDev& ad = ctx->ace;
+  VerletDev& V = ctx->verlet;
+  SystemBuffers& s = ctx->sys;
+  hipStream_t st = ctx->stream;
+  const int64_t n = s.n;
+  bool fresh = false;
+  if (!ad.skin_valid || ad.skin_gen != ctx->build_gen || !ad.ecount_s) {
+    ace_skin_rebuild(ctx);
+    fresh = true;
+  } else {
+    for (int k = 0; k < 4; ++k) record_stage(ctx, k);
+  }
+  for (;;) {
+    ad.ecap = ad.ecap_s;
+    // sites per wavefront follow the largest in-cutoff degree at the build (any
+    // degree is correct; a smaller one only packs two sites per wavefront)
+    ad.max_degree = ad.compact_degree;
+    PFW_HIP(hipMemsetAsync(ad.status, 0, 4 * sizeof(int32_t), st));
+    hipLaunchKernelGGL(k_ace_delta, dim3(nblk(n, 256)), dim3(256), 0, st, n, s.perm, s.X, ad.Xref, ad.dX, ad.maxd2,
+                       ad.parity);
+    hipLaunchKernelGGL(k_ace_compact, dim3((unsigned)n), dim3(kWave), 0, st, n, ad.ecap, ad.ecount_s, ad.enb_s,
+                       ad.eR_s, ad.dX, ad.cutoff * ad.cutoff, ad.ecount, ad.enb, ad.eR, ad.cslot, ad.corig);
+    if (grads)
+      hipLaunchKernelGGL(k_ace_revmap, dim3(nblk(n * (int64_t)ad.ecap, 256)), dim3(256), 0, st, n * (int64_t)ad.ecap,
+                         ad.ecap, ad.ecount, ad.corig, ad.erev_s, ad.cslot, ad.erev, ad.status);
+    PFW_HIP(hipGetLastError());
+    record_stage(ctx, 4);
+    ace_evaluate_rows(ctx, grads, precision);
+    PFW_HIP(hipMemcpyAsync(ctx->h_out, ctx->d_out, 11 * sizeof(double), hipMemcpyDeviceToHost, st));
+    PFW_HIP(hipMemcpyAsync(ad.h_maxd2, ad.maxd2 + ad.parity, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    record_stage(ctx, 6);
+    PFW_HIP(hipStreamSynchronize(st));
+    ad.parity ^= 1;
+    const double m2 = __builtin_bit_cast(double, (unsigned long long)ad.h_maxd2[0]);
+    V.last_max_disp = std::sqrt(m2);
+    const bool stale = !(2.0 * V.last_max_disp < V.skin * (1.0 - 16.0 * 2.220446049250313e-16));
+    if (fresh || !stale) break;
+    ace_skin_rebuild(ctx);
+    fresh = true;
+  }
+  if (!fresh) ++V.reuses;
+}
+
 void ace_run(pfw_ctx* ctx, uint32_t mask, int precision) {
   AceDev& ad = ctx->ace;
   if (!ad.ready) throw Error(PFW_STALE, "no ACE model set");
@@ -1435,45 +1726,30 @@ void ace_run(pfw_ctx* ctx, uint32_t mask, int precision) {
   SystemBuffers& s = ctx->sys;
   hipStream_t st = ctx->stream;
   const int64_t n = s.n;
-  nl_build(ctx);
-  record_stage(ctx, 3);
-  if (n == 0 || mask == 0) {
-    record_stage(ctx, 4);
-    record_stage(ctx, 5);
-    PFW_HIP(hipMemsetAsync(ctx->d_out, 0, 11 * sizeof(double), st));
-    PFW_HIP(hipMemsetAsync(ctx->d_cfg_out, 0, 11 * (size_t)ctx->ncfg * sizeof(double), st));
-    if (n > 0) PFW_HIP(hipMemsetAsync(ctx->d_F, 0, 3 * n * sizeof(double), st));
+  const int grads = (mask & (PFW_FORCES | PFW_VIRIAL)) ? 1 : 0;
+  if (ctx->verlet.skin > 0 && n > 0 && mask != 0) {
+    ace_skin_step(ctx, grads, precision);
   } else {
-    const int grads = (mask & (PFW_FORCES | PFW_VIRIAL)) ? 1 : 0;
-    ace_environment(ctx, grads);
-    record_stage(ctx, 4);
-    if (precision == PFW_MIXED)
-      dispatch_sites<float>(ctx, grads);
-    else
-      dispatch_sites<double>(ctx, grads);
-    record_stage(ctx, 5);
-    if (grads && precision == PFW_MIXED)
-      hipLaunchKernelGGL(k_ace_force<float>, dim3((unsigned)n), dim3(kWave), 0, st, n, ad.ecap, ad.ecount, ad.erev,
-                         (const float*)ad.eg, s.perm, ctx->d_F);
-    else if (grads)
-      hipLaunchKernelGGL(k_ace_force<double>, dim3((unsigned)n), dim3(kWave), 0, st, n, ad.ecap, ad.ecount, ad.erev,
-                         (const double*)ad.eg, s.perm, ctx->d_F);
-    hipLaunchKernelGGL(k_ace_pack, dim3(nblk(n, 256)), dim3(256), 0, st, n, ad.siteE, ad.siteW, grads, ad.rec);
-    if (ctx->ncfg == 1) {
-      hipLaunchKernelGGL(k_colsum, dim3(kRedBlocks), dim3(256), 0, st, ad.rec, n, 10, ctx->d_red);
-      hipLaunchKernelGGL(k_finish, dim3(1), dim3(1024), 0, st, ctx->d_red, kRedBlocks, 10, 1, (const int32_t*)ad.status,
-                         ctx->d_out, ctx->d_cfg_out);
+    nl_build(ctx);
+    record_stage(ctx, 3);
+    if (n == 0 || mask == 0) {
+      record_stage(ctx, 4);
+      record_stage(ctx, 5);
+      PFW_HIP(hipMemsetAsync(ctx->d_out, 0, 11 * sizeof(double), st));
+      PFW_HIP(hipMemsetAsync(ctx->d_cfg_out, 0, 11 * (size_t)ctx->ncfg * sizeof(double), st));
+      if (n > 0) PFW_HIP(hipMemsetAsync(ctx->d_F, 0, 3 * n * sizeof(double), st));
     } else {
-      batch_reduce(ctx, ad.rec, 10, false, 1);
+      ace_environment(ctx, grads);
+      record_stage(ctx, 4);
+      ace_evaluate_rows(ctx, grads, precision);
     }
-    PFW_HIP(hipGetLastError());
-  }
-  PFW_HIP(hipMemcpyAsync(ctx->h_out, ctx->d_out, 11 * sizeof(double), hipMemcpyDeviceToHost, st));
-  record_stage(ctx, 6);
-  PFW_HIP(hipStreamSynchronize(st));
-  if (n > 0 && mask != 0 && ace_overflowed(ad)) {  // an environment outgrew its row: once more, wider
-    ace_run(ctx, mask, precision);
-    return;
+    PFW_HIP(hipMemcpyAsync(ctx->h_out, ctx->d_out, 11 * sizeof(double), hipMemcpyDeviceToHost, st));
+    record_stage(ctx, 6);
+    PFW_HIP(hipStreamSynchronize(st));
+    if (n > 0 && mask != 0 && ace_overflowed(ad)) {  // an environment outgrew its row: once more, wider
+      ace_run(ctx, mask, precision);
+      return;
+    }
   }
   if (ctx->ncfg > 1) {
     ctx->h_cfg_out.resize(11 * (size_t)ctx->ncfg);

@@ -310,6 +310,7 @@ int pfw_set_skin(pfw_ctx* ctx, double skin) {
     if (!(std::isfinite(skin) && skin >= 0)) throw Error(PFW_INVALID_ARGUMENT, "skin must be finite and nonnegative (Å)");
     if (skin == ctx->verlet.skin) return;
     pfw::verlet_free(ctx);
+    ctx->ace.skin_valid = false;
     ctx->verlet.skin = skin;
     if (ctx->have_system) {
       ctx->have_system = false;

@@ -189,6 +189,26 @@ struct AceDev {
   double* siteW = nullptr;    // 9 x n
   double* rec = nullptr;      // 10 x n reduction records
   int32_t* status = nullptr;
+  // Verlet reuse (skin > 0): rows at cutoff + skin with the build's exact R;
+  // every step compacts them into the rows above (edges inside the cutoff now)
+  bool skin_valid = false;
+  uint64_t skin_gen = 0;        // cell-build generation of the skin rows
+  int ecap_s = 0;               // slots per site of the skin rows
+  int64_t skin_cap = 0;         // allocated skin slots
+  int skin_max_degree = 0;
+  int compact_degree = 0;       // largest in-cutoff degree at the build
+  int32_t* ecount_s = nullptr;  // n
+  int32_t* enb_s = nullptr;     // n x ecap_s
+  int32_t* eent_s = nullptr;
+  int32_t* erev_s = nullptr;
+  double* eR_s = nullptr;       // 3 x slots, at the build
+  int32_t* cslot = nullptr;     // skin slot -> compacted slot (-1: outside the cutoff now)
+  int32_t* corig = nullptr;     // compacted slot -> skin slot
+  double* Xref = nullptr;       // 3 x n caller positions at the build (AoS, original order)
+  double* dX = nullptr;         // 3 x n displacement since the build, sorted SoA
+  unsigned long long* maxd2 = nullptr;    // 2 words, as the pair list's
+  unsigned long long* h_maxd2 = nullptr;  // pinned
+  int parity = 0;
 };
 
 }  // namespace pfw

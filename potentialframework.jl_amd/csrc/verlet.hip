@@ -241,10 +241,12 @@ __global__ __launch_bounds__(64) void k_verlet_build(const Grid* __restrict__ gr
       }
       __syncthreads();
     }
-    if (lane < na) {
-      ncount[mine] = min(pos, ecap);
-      atomicMax(status + 1, pos);
-      if (pos > ecap) atomicOr(status + 2, 1);  // capacity too small: the host grows it and builds again
+    if (lane < na) ncount[mine] = min(pos, ecap);
+    int dmax = lane < na ? pos : 0;  // one atomic per wavefront (same-address atomics serialise)
+    for (int off = kWave / 2; off > 0; off >>= 1) dmax = max(dmax, __shfl_xor(dmax, off, kWave));
+    if (lane == 0) {
+      atomicMax(status + 1, dmax);
+      if (dmax > ecap) atomicOr(status + 2, 1);  // capacity too small: the host grows it and builds again
     }
   }
 }
