@@ -1477,7 +1477,10 @@ static void launch_sites(pfw_ctx* ctx, int grads) {
 
 template <typename T, int L>
 static void launch_sites_spw(pfw_ctx* ctx, int grads) {
-  if (ctx->ace.max_degree <= kWave / 2)
+  // two sites per wavefront unless some site has more than 48 edges: a site
+  // above 32 takes a second pass in its half-wavefront, which costs less than
+  // halving the sites per wavefront for every site
+  if (ctx->ace.max_degree <= 48)
     launch_sites<T, L, 2>(ctx, grads);
   else
     launch_sites<T, L, 1>(ctx, grads);
