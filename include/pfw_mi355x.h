@@ -154,12 +154,14 @@ int pfw_synchronize(pfw_ctx* ctx);
 /* ---------------------------------------------------------------- ACE
  * Resolved ACE model, schema v1 (src/submodules/ACE/resolved.jl:18-142):
  * piecewise-cubic Agnesi radial splines per (centre, neighbour) species
- * channel (index centre + nz*neighbour), real L2 spherical harmonics in
- * l*l+l+m order, density (radial index, harmonic index) pairs, product
+ * channel (index centre + nz*neighbour), real L2 spherical (or solid)
+ * harmonics in l*l+l+m order, density (radial index, harmonic index) pairs, product
  * multisets of density indices (order <= 4), sparse coupling C (nbasis x
  * nprod) and readout WB (nbasis x nz), Wpair (nq x nz). Default basis policy:
  * every readout row enabled, no species projection, no order restriction. */
 enum { PFW_ENV_POLY2SX = 0, PFW_ENV_POLY1SR = 1, PFW_ENV_ACE1_POLY1SR = 2 };
+/* Angular basis (resolved.jl:109-110; angular_blocks.jl:47-48). */
+enum { PFW_ANGULAR_SPHERICAL = 0, PFW_ANGULAR_SOLID = 1 };
 
 typedef struct {
   int32_t nr;                    /* radial functions per channel */
@@ -184,6 +186,7 @@ typedef struct {
   const double* coupling_vals;
   const double* WB;              /* nbasis x nz, column-major */
   const double* Wpair;           /* pair.nr x nz, column-major */
+  int32_t angular;               /* PFW_ANGULAR_*: "real_spherical" or "real_solid" (r^l Y_lm) */
 } pfw_ace_model;
 
 /* Device tables of a resolved model: adapt_model / pack_model

@@ -72,7 +72,11 @@ def load_resolved(path):
         raise ValueError("unsupported resolved ACE schema")
     z = [int(v) for v in D["species_atomic_numbers"]]
     nz = len(z)
-    lmax = int(D["angular"]["lmax"])
+    ang = D["angular"]
+    if ang["kind"] not in ("real_spherical", "real_solid") or ang["normalization"] != "L2" \
+            or ang["ordering"] != "l*l+l+m+1":
+        raise ValueError("unsupported angular convention")  # resolved.jl:109-110
+    lmax = int(ang["lmax"])
     radial = _family(D["radial"], nz)
     pair = _family(D["pair_radial"], nz)
     dens = [(int(a) - 1, int(b) - 1) for a, b in D["density_indices"]]
@@ -86,7 +90,8 @@ def load_resolved(path):
     nq = pair["nr"]
     WB = flat[: m * nz].reshape(nz, m).T.copy()
     Wpair = flat[m * nz:].reshape(nz, nq).T.copy()
-    return dict(atomic_numbers=z, nz=nz, lmax=lmax, reference=np.asarray(D["reference_eV"], float), radial=radial,
+    return dict(atomic_numbers=z, nz=nz, lmax=lmax, solid=ang["kind"] == "real_solid",
+                reference=np.asarray(D["reference_eV"], float), radial=radial,
                 pair=pair, density=dens, products=products, coupling=C, WB=WB, Wpair=Wpair,
                 cutoff=max(radial["rcut"], pair["rcut"]), source_sha256=D.get("source_sha256"))
 
@@ -133,7 +138,8 @@ class _Model(ctypes.Structure):
     _fields_ = [("nz", ctypes.c_int), ("lmax", ctypes.c_int), ("ndens", ctypes.c_int), ("nprod", ctypes.c_int),
                 ("reference", ctypes.c_void_p), ("radial", _Family), ("pair", _Family),
                 ("dens_n", ctypes.c_void_p), ("dens_m", ctypes.c_void_p), ("prod_ptr", ctypes.c_void_p),
-                ("prod_idx", ctypes.c_void_p), ("weights", ctypes.c_void_p), ("wpair", ctypes.c_void_p)]
+                ("prod_idx", ctypes.c_void_p), ("weights", ctypes.c_void_p), ("wpair", ctypes.c_void_p),
+                ("solid", ctypes.c_int)]
 
 
 def _pack(model):
@@ -156,7 +162,8 @@ def _pack(model):
     m = _Model(model["nz"], model["lmax"], len(model["density"]), len(prods), arr(model["reference"], np.float64),
                fam(model["radial"]), fam(model["pair"]), arr([d[0] for d in model["density"]], np.int32),
                arr([d[1] for d in model["density"]], np.int32), arr(ptr, np.int32), arr(idx, np.int32),
-               arr(weights.T.copy(), np.float64), arr(model["Wpair"].T.copy(), np.float64))
+               arr(weights.T.copy(), np.float64), arr(model["Wpair"].T.copy(), np.float64),
+               int(model.get("solid", False)))
     return m, keep
 
 

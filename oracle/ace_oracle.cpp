@@ -42,6 +42,7 @@ extern "C" struct OracleAceModel {
   const int* prod_idx;  // 0-based density indices
   const double* weights;  // nprod x nz, column-major: C^T (WB .* mask)
   const double* wpair;    // nq x nz column-major
+  int solid;              // 1: real solid harmonics r^l Y_lm ("real_solid"), 0: spherical
 };
 
 namespace oracle {
@@ -155,15 +156,18 @@ static inline Jet jscale(double s, const Jet& a) {
 }
 
 // angular_blocks.jl:36-81 for one displacement R: real L2 Y_lm (no
-// Condon-Shortley phase) in l*l+l+m order and their Cartesian gradients.
-static void angular(int L, const double* R, double* Y, double* dY) {
+// Condon-Shortley phase) in l*l+l+m order and their Cartesian gradients. With
+// `solid` the same recurrence runs on the raw coordinates (angular_blocks.jl:47-48),
+// giving the real solid harmonics r^l Y_lm.
+static void angular(int L, const double* R, double* Y, double* dY, int solid = 0) {
   double r = std::sqrt(R[0] * R[0] + R[1] * R[1] + R[2] * R[2]);
   if (!(r > 0)) throw DomainError("physical zero directions are unsupported");
   double u[3] = {R[0] / r, R[1] / r, R[2] / r};
   Jet j[3];
   for (int a = 0; a < 3; ++a) {
-    j[a].v = u[a];
-    for (int c = 0; c < 3; ++c) j[a].g[c] = ((a == c ? 1.0 : 0.0) - u[a] * u[c]) / r;
+    j[a].v = solid ? R[a] : u[a];
+    for (int c = 0; c < 3; ++c)
+      j[a].g[c] = solid ? (a == c ? 1.0 : 0.0) : ((a == c ? 1.0 : 0.0) - u[a] * u[c]) / r;
   }
   Jet x = j[0], y = j[1], z = j[2];
   Jet r2 = jadd(jadd(jmul(x, x), jmul(y, y)), jmul(z, z));
@@ -224,7 +228,7 @@ static void site(const OracleAceModel& m, int iz, const std::vector<Edge>& edges
     int ch = iz + m.nz * edges[e].zj;
     radial_ed(m.radial, ch, rs[e], &R[e * nr], &dR[e * nr]);
     if (nq) radial_ed(m.pair, ch, rs[e], &Q[e * nq], &dQ[e * nq]);
-    angular(m.lmax, d, &Y[e * ny], &dY[e * ny * 3]);
+    angular(m.lmax, d, &Y[e * ny], &dY[e * ny * 3], m.solid);
   }
   std::vector<double> A(m.ndens, 0.0);
   for (int a = 0; a < m.ndens; ++a)
@@ -366,9 +370,9 @@ int oracle_ace_evaluate(int64_t n, const double* X, const int32_t* species, cons
 
 }  // extern "C"
 
-extern "C" int oracle_ylm(int L, const double* R, double* Y, double* dY) {
+extern "C" int oracle_ylm(int L, int solid, const double* R, double* Y, double* dY) {
   try {
-    oracle::angular(L, R, Y, dY);
+    oracle::angular(L, R, Y, dY, solid);
     return 0;
   } catch (const std::exception&) {
     return 3;

@@ -68,6 +68,7 @@ struct AceArgs {
   double* siteE;
   double* siteW;
   T* eg;  // 3 x E edge gradients in the arithmetic precision
+  int solid;  // angular basis: 1 real solid harmonics r^l Y_lm, 0 spherical Y_lm
 };
 
 template <typename T>
@@ -196,16 +197,23 @@ __device__ inline Jet<T> jsub(const Jet<T>& a, const Jet<T>& b) {
 }
 
 // Real L2 Y_lm (no Condon-Shortley phase, index l*l+l+m) of the direction of
-// (dx, dy, dz) and their Cartesian gradients (angular_blocks.jl:36-81).
+// (dx, dy, dz) and their Cartesian gradients (angular_blocks.jl:36-81); with
+// `solid` the same recurrence on the raw coordinates, the real solid harmonics
+// r^l Y_lm (angular_blocks.jl:47-48).
 // Fl holds F[l][m] = sqrt((2l+1)/2pi) * prod(-1/sqrt((l+m')(l+1-m'))).
 template <typename T, int L>
-__device__ inline void ylm(T dx, T dy, T dz, const T* Fl, T* Y, T* G) {
+__device__ inline void ylm(T dx, T dy, T dz, const T* Fl, T* Y, T* G, int solid) {
   const T r = sqrt(dx * dx + dy * dy + dz * dz);
   const T ri = (T)1 / r;
   const T ux = dx * ri, uy = dy * ri, uz = dz * ri;
-  const Jet<T> x{ux, ((T)1 - ux * ux) * ri, -ux * uy * ri, -ux * uz * ri};
-  const Jet<T> y{uy, -uy * ux * ri, ((T)1 - uy * uy) * ri, -uy * uz * ri};
-  const Jet<T> z{uz, -uz * ux * ri, -uz * uy * ri, ((T)1 - uz * uz) * ri};
+  Jet<T> x{ux, ((T)1 - ux * ux) * ri, -ux * uy * ri, -ux * uz * ri};
+  Jet<T> y{uy, -uy * ux * ri, ((T)1 - uy * uy) * ri, -uy * uz * ri};
+  Jet<T> z{uz, -uz * ux * ri, -uz * uy * ri, ((T)1 - uz * uz) * ri};
+  if (solid) {
+    x = {dx, (T)1, (T)0, (T)0};
+    y = {dy, (T)0, (T)1, (T)0};
+    z = {dz, (T)0, (T)0, (T)1};
+  }
   const Jet<T> r2 = jadd(jadd(jm(x, x), jm(y, y)), jm(z, z));
   Jet<T> cs[L + 1], sn[L + 1], Q[(L + 1) * (L + 2) / 2];
   cs[0] = {(T)1, 0, 0, 0};
@@ -328,7 +336,7 @@ __global__ __launch_bounds__(64) void k_ace_dens(AceArgs<T> a) {
         epair += (double)s;
       }
       T Y[NY], G[3 * NY];
-      ylm<T, L>(dx, dy, dz, Fl, Y, G);
+      ylm<T, L>(dx, dy, dz, Fl, Y, G, a.solid);
 #pragma unroll
       for (int m = 0; m < NY; ++m) Yb[m * kStride + lane] = Y[m];
     }
@@ -364,24 +372,40 @@ __global__ __launch_bounds__(64) void k_ace_dens(AceArgs<T> a) {
 // the readout w (in schedule order, per species) are staged in LDS once per
 // workgroup; each wavefront then walks sites with a grid stride, one site at a
 // time, with no workgroup barrier inside the walk.
-template <typename T>
+// GT: the schedule, weights and copy offsets are read from global memory (L2)
+// instead of LDS, for models whose tables do not fit beside the per-wavefront
+// accumulators.
+template <typename T, bool GT>
 __global__ __launch_bounds__(1024) void k_ace_prod(AceArgs<T> a, int grads) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr unsigned kNone = 0xffff;
   const int nwaves = blockDim.x / kWave;
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   const int nslot = a.nsteps * kWave;
-  T* ws = (T*)smem;                                                   // nz x nslot
-  uint4* sched = (uint4*)(smem + align16(sizeof(T) * a.nz * nslot));  // nslot: factors and targets (u16)
-  int* cb = (int*)(sched + nslot);                                    // ndens + 1 copy offsets
-  unsigned char* pw = smem + align16((size_t)((unsigned char*)(cb + a.ndens + 1) - smem));
+  const T* ws;
+  const uint4* sched;
+  const int* cb;
+  unsigned char* pw = smem;
+  if constexpr (GT) {
+    ws = a.ws;
+    sched = a.sched;
+    cb = a.copyb;
+  } else {
+    T* lws = (T*)smem;                                                   // nz x nslot
+    uint4* lsched = (uint4*)(smem + align16(sizeof(T) * a.nz * nslot));  // nslot: factors and targets (u16)
+    int* lcb = (int*)(lsched + nslot);                                   // ndens + 1 copy offsets
+    pw = smem + align16((size_t)((unsigned char*)(lcb + a.ndens + 1) - smem));
+    for (int i = threadIdx.x; i < a.nz * nslot; i += blockDim.x) lws[i] = a.ws[i];
+    for (int i = threadIdx.x; i < nslot; i += blockDim.x) lsched[i] = a.sched[i];
+    for (int i = threadIdx.x; i <= a.ndens; i += blockDim.x) lcb[i] = a.copyb[i];
+    __syncthreads();
+    ws = lws;
+    sched = lsched;
+    cb = lcb;
+  }
   const size_t per_wave = align16(sizeof(T) * a.ndens) + align16(sizeof(T) * a.ntarget);
   T* A = (T*)(pw + wave * per_wave);
   T* acc = (T*)((unsigned char*)A + align16(sizeof(T) * a.ndens));
-  for (int i = threadIdx.x; i < a.nz * nslot; i += blockDim.x) ws[i] = a.ws[i];
-  for (int i = threadIdx.x; i < nslot; i += blockDim.x) sched[i] = a.sched[i];
-  for (int i = threadIdx.x; i <= a.ndens; i += blockDim.x) cb[i] = a.copyb[i];
-  __syncthreads();
   for (int64_t site = (int64_t)blockIdx.x * nwaves + wave; site < a.n; site += (int64_t)gridDim.x * nwaves) {
     T* Ag = a.Abuf + site * a.ndens;
     for (int k = lane; k < a.ndens; k += kWave) A[k] = Ag[k];
@@ -464,7 +488,7 @@ __global__ __launch_bounds__(64) void k_ace_pull(AceArgs<T> a) {
     const T ri = (T)1 / r;
     const int ch = iz + a.nz * a.sps[a.enb[e0 + e]];
     T Y[NY], G[3 * NY], U[NY];
-    ylm<T, L>(dx, dy, dz, Fl, Y, G);
+    ylm<T, L>(dx, dy, dz, Fl, Y, G, a.solid);
 #pragma unroll
     for (int m = 0; m < NY; ++m) U[m] = 0;
     T gs = 0;
@@ -933,7 +957,7 @@ __global__ __launch_bounds__(64) void k_ace_adjoint(AceArgs<T> a, const int32_t*
           Rdb[n * kStride + lane] = d * sdot;
         }
         T Y[NY], Gy[3 * NY];
-        ylm<T, L>(dx, dy, dz, Fl, Y, Gy);
+        ylm<T, L>(dx, dy, dz, Fl, Y, Gy, a.solid);
 #pragma unroll
         for (int m = 0; m < NY; ++m) {
           Yb[m * kStride + lane] = Y[m];
@@ -1276,6 +1300,9 @@ void ace_set_model(pfw_ctx* ctx, const pfw_ace_model* m) {
   if (m->lmax > 6) throw Error(PFW_INVALID_ARGUMENT, "lmax above 6 is not supported by the device kernels");
   a.nz = m->nz;
   a.lmax = m->lmax;
+  if (m->angular != PFW_ANGULAR_SPHERICAL && m->angular != PFW_ANGULAR_SOLID)
+    throw Error(PFW_INVALID_ARGUMENT, "unknown angular basis");
+  a.solid = m->angular == PFW_ANGULAR_SOLID ? 1 : 0;
   a.ndens = m->ndens;
   a.nprod = m->nprod;
   a.nbasis = m->nbasis;
@@ -1427,6 +1454,7 @@ static AceArgs<T> make_args(pfw_ctx* ctx, int grads) {
   a.copyb = ad.copyb;
   a.ntarget = ad.ntarget;
   a.Abuf = (T*)ad.Abuf;
+  a.solid = ad.solid;
   return a;
 }
 
@@ -1459,13 +1487,21 @@ static void launch_sites(pfw_ctx* ctx, int grads) {
   PFW_HIP(hipGetLastError());
   size_t tables, per_wave;
   prod_lds<T>(ad, tables, per_wave);
-  if (tables + per_wave > kMaxLds) throw Error(PFW_INVALID_ARGUMENT, "ACE product tables too large for LDS");
+  // tables in LDS when at least 4 wavefronts' accumulators fit beside them
+  const bool global_tables = tables + 4 * per_wave > kMaxLds;
+  if (global_tables) tables = 0;
+  if (tables + per_wave > kMaxLds) throw Error(PFW_INVALID_ARGUMENT, "ACE product accumulators too large for LDS");
   const int waves = (int)std::min<size_t>(16, (kMaxLds - tables) / per_wave);
   const size_t lds_prod = tables + waves * per_wave;
-  allow_lds(k_ace_prod<T>, lds_prod, "the product kernel");
   const int64_t want = (s.n + waves - 1) / waves;
   const unsigned pblocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>(want, ctx->num_cu));
-  hipLaunchKernelGGL(k_ace_prod<T>, dim3(pblocks), dim3(waves * kWave), lds_prod, st, a, grads);
+  if (global_tables) {
+    allow_lds(k_ace_prod<T, true>, lds_prod, "the product kernel");
+    hipLaunchKernelGGL((k_ace_prod<T, true>), dim3(pblocks), dim3(waves * kWave), lds_prod, st, a, grads);
+  } else {
+    allow_lds(k_ace_prod<T, false>, lds_prod, "the product kernel");
+    hipLaunchKernelGGL((k_ace_prod<T, false>), dim3(pblocks), dim3(waves * kWave), lds_prod, st, a, grads);
+  }
   PFW_HIP(hipGetLastError());
   if (grads) {
     const size_t lds_pull = sizeof(T) * ((size_t)NY + SPW * (size_t)ad.ndens);

@@ -153,6 +153,7 @@ struct AceFamilyDev {
 struct AceDev {
   bool ready = false;
   int nz = 0, lmax = 0, ndens = 0, nprod = 0, nbasis = 0, ny = 0;
+  int solid = 0;  // real solid harmonics (r^l Y_lm) instead of spherical
   double cutoff = 0;
   std::vector<int32_t> c_rows, c_cols;  // host copy of the coupling
   std::vector<double> c_vals;

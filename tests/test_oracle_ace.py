@@ -74,12 +74,12 @@ def test_tial_33_frames_against_acepotentials(tial):
             assert site.sum() == pytest.approx(E, abs=1e-9)
 
 
-def oracle_ylm(L, R):
+def oracle_ylm(L, R, solid=0):
     ny = (L + 1) ** 2
     Y = np.zeros(ny)
     dY = np.zeros(3 * ny)
     R = np.ascontiguousarray(R, float)
-    rc = lib.oracle_ylm(L, R.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+    rc = lib.oracle_ylm(L, int(solid), R.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                         Y.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                         dY.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
     assert rc == 0
@@ -222,3 +222,19 @@ def test_generator_invariants():
         np.testing.assert_allclose(M @ M.T, np.eye(M.shape[0]), atol=1e-12)
     c110 = ab.real_cg(1, 1, 0)[0]
     np.testing.assert_allclose(np.abs(c110), np.eye(3) / np.sqrt(3), atol=1e-14)
+
+
+def test_solid_harmonics_are_r_l_times_spherical():
+    """Real solid harmonics ("real_solid", angular_blocks.jl:47-48) are r^l Y_lm,
+    with gradient r^l dY_lm + l r^(l-2) R Y_lm: the reference's own check
+    (test/ace_may30.jl:22-29) on its three radii of one direction, l <= 6."""
+    for r in (0.3, 1.0, 2.7):
+        R = r * np.array([0.3, -0.7, 0.9])
+        Ys, dYs = oracle_ylm(6, R)
+        Y, dY = oracle_ylm(6, R, solid=1)
+        rn = np.linalg.norm(R)
+        for l in range(7):
+            for k in range(l * l, (l + 1) ** 2):
+                g = np.zeros(3) if l == 0 else l * rn ** (l - 2) * R
+                np.testing.assert_allclose(Y[k], rn ** l * Ys[k], atol=2e-11, rtol=2e-12)
+                np.testing.assert_allclose(dY[k], rn ** l * dYs[k] + g * Ys[k], atol=2e-11, rtol=2e-12)
