@@ -257,6 +257,56 @@ def bench_verlet(pfw, lib, _capi, X, H, pbc, sp, model, labels, skin, prec, mode
     }
 
 
+def bench_may30(pfw, lib, _capi, args, reps=5):
+    """The reference's fitted May30 Ti-Al-V model (real solid harmonics to l = 6,
+    5,652 products, 1,000-node splines) on its omega-240 fixture structure
+    replicated 5 x 5 x 5 (30,000 atoms, nonorthogonal periodic cell), one
+    fresh step per evaluation as for the synthetic ACE workload."""
+    try:
+        import tomllib as toml
+    except ImportError:
+        import tomli as toml
+    from pfw_amd.ace import ELEMENTS
+    gold = os.path.join(ROOT, "tests", "golden")
+    model, ps, _ = pfw.load_resolved(os.path.join(gold, "may30_resolved.npz"))
+    with open(os.path.join(gold, "may30_omega-240.toml"), "rb") as f:
+        d = toml.load(f)
+    n0 = len(d["species"])
+    X0 = np.asarray(d["positions"], float).reshape(n0, 3)
+    H0 = np.asarray(d["cell"], float).reshape(3, 3).T
+    k = 5
+    shifts = np.array([[a, b, c] for a in range(k) for b in range(k) for c in range(k)], float) @ H0.T
+    X = (X0[None, :, :] + shifts[:, None, :]).reshape(-1, 3)
+    H = H0 * k
+    labels = [ELEMENTS[z - 1] for z in d["species"]] * len(shifts)
+    sys_ = pfw.System(X, H, (True, True, True), labels)
+    out = {"workload": "may30 omega-240 x 5^3", "atoms": len(X), "cutoff_A": model.cutoff,
+           "basis": {"radial": model.radial.nr, "lmax": model.lmax, "densities": len(model.density),
+                     "products": len(model.products), "invariants": model.nbasis, "angular": "real_solid"},
+           "unit": "Matom-evals/s",
+           "reference_cpu_context": {"atoms": 1024, "prepared_ms": 74.629, "threads": 1,
+                                     "source": "benchmark/cpu_ace/MAY30-REPORT.md:8-10,133-139 (DAG, bcc-1024)"}}
+    for prec, mode in (("mixed", _capi.PFW_MIXED), ("float64", _capi.PFW_FP64)):
+        calc = pfw.ACECalculator(model, ps, precision=prec)
+        calc.evaluate(sys_)
+        secs, stage = time_steps(lib, calc.workspace.ctx.h, lib.pfw_ace_step, max(3, args.steps // 10),
+                                 min(args.warmup, 2), mode)
+        nsteps = max(3, args.steps // 10)
+        out[prec] = {"value": round(len(X) * nsteps / secs / 1e6, 4), "ms_per_step": round(1e3 * secs / nsteps, 4),
+                     "stages_ms": {k2: round(float(v), 5) for k2, v in zip(STAGES, stage)}}
+        if prec == "mixed":
+            edges = len(calc.workspace.export_pairs()[0])
+            flops, per_edge, per_site = ace_flops(model, edges, len(X))
+            kms = float(stage[4])
+            out["edges"] = edges
+            out["roofline"] = {"kernel": "k_ace_dens+k_ace_prod+k_ace_pull<float,6>", "bound": "mfma",
+                               "compute_unit": "VALU fp32", "achieved": round(flops / (kms * 1e-3) / 1e12, 3),
+                               "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                               "frac": round(flops / (kms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4),
+                               "flops_per_edge": per_edge, "flops_per_site": per_site, "kernel_ms": round(kms, 5)}
+    return out
+
+
 def calibrate_copy(nbytes=256 << 20, reps=5):
     """A plain device copy of known size: the reference point that scales the
     FETCH_SIZE / WRITE_SIZE counters into HBM bytes."""
@@ -434,6 +484,10 @@ def main():
                 "workload": "bcc_binary_ace_32k, trajectory x_s = x_0 + s v (environments at cutoff + skin)",
                 **{prec: bench_verlet(pfw, lib, _capi, Xa, Ha, pba, spa, (model, ps), labels, 0.5, prec, mode, ace=True)
                    for prec, mode in (("mixed", _capi.PFW_MIXED), ("float64", _capi.PFW_FP64))}}
+
+    # ---------------------------------------------------------------- May30 real model
+    if not args.no_ace:
+        line["ace_may30"] = bench_may30(pfw, lib, _capi, args)
 
     # ---------------------------------------------------------------- fitting set (batch + adjoints)
     if not args.no_batch and not args.no_ace:
