@@ -130,6 +130,34 @@ class ResolvedModel:
             raise DimensionMismatch("reference energy channels")
         self.cutoff = max(self.radial.cutoff, self.pair.cutoff)
         self.source_sha256 = D.get("source_sha256")
+        self._read_policy(D.get("interface_policy"))
+        self.source = {k: v for k, v in D.items() if k != "parameters"}  # kept for save_calculator
+
+    def _read_policy(self, P):
+        """The basis policy (interface.jl:2-18, interface_persistence.jl:8-25): the
+        invariant mask (effective_WB = WB .* mask), the pair and reference flags
+        (pair_allowed / reference_value, interface.jl:122-124). Species projection
+        and order restriction are not supported on the device."""
+        self.mask = np.ones(self.nbasis, bool)
+        self.pair_enabled = True
+        self.reference_enabled = True
+        if P is None:
+            return
+        if P.get("version") != 1:
+            raise ArgumentError("unsupported basis policy version")
+        mask = P.get("mask", [True] * self.nbasis)
+        if len(mask) != self.nbasis or not all(isinstance(v, bool) for v in mask):
+            raise ArgumentError("invalid invariant mask")
+        if list(P.get("species", [])) or list(P.get("orders", [])):
+            raise ArgumentError("species-projected or order-restricted ACE bases are not supported on the device")
+        self.mask = np.asarray(mask, bool)
+        self.pair_enabled = bool(P.get("pair", True))
+        self.reference_enabled = bool(P.get("reference", True))
+
+    def effective(self, WB, Wpair):
+        """The readout the device evaluates: WB .* mask, and Wpair only when the
+        pair term is enabled."""
+        return WB * self.mask[:, None], (Wpair if self.pair_enabled else np.zeros_like(Wpair))
 
     def parameters(self):
         return {"WB": self.WB.copy(), "Wpair": self.Wpair.copy()}
@@ -156,11 +184,13 @@ class ResolvedModel:
         pp = np.zeros(len(self.products) + 1, np.int32)
         pp[1:] = np.cumsum([len(p) for p in self.products])
         pidx = np.concatenate([np.asarray(p, np.int32) for p in self.products]) if self.products else np.zeros(0)
+        WB, Wpair = self.effective(self.WB, self.Wpair)
+        ref = self.reference if self.reference_enabled else np.zeros_like(self.reference)
         m = _Model(self.nz, self.lmax, len(self.density), len(self.products), self.nbasis,
-                   a(self.reference, np.float64), fam(self.radial), fam(self.pair), a(self.density, np.int32),
+                   a(ref, np.float64), fam(self.radial), fam(self.pair), a(self.density, np.int32),
                    a(pp, np.int32), a(pidx, np.int32), len(self.c_vals), a(self.c_rows, np.int32),
-                   a(self.c_cols, np.int32), a(self.c_vals, np.float64), a(self.WB.T, np.float64),
-                   a(self.Wpair.T, np.float64), 1 if self.solid else 0)
+                   a(self.c_cols, np.int32), a(self.c_vals, np.float64), a(WB.T, np.float64),
+                   a(Wpair.T, np.float64), 1 if self.solid else 0)
         return m, keep
 
 
@@ -190,8 +220,187 @@ def load_resolved(source):
     else:
         with open(source, "rb") as f:
             D = _toml.load(f)
+    if D.get("schema") == CHECKPOINT_SCHEMA:
+        return _read_checkpoint(D)
     m = ResolvedModel(D)
     return m, m.parameters(), {}
+
+
+# ---- calculator checkpoints (src/submodules/ACE/persistence.jl): a versioned
+# TOML tree {schema, schema_version, model (the resolved dictionary without its
+# parameters), parameters and state in the checkpoint encoding}.
+CHECKPOINT_SCHEMA = "PotentialFrameWork.ACECalculator"
+_CK_TYPES = {"Float64": np.float64, "Float32": np.float32, "Int64": np.int64, "Int32": np.int32,
+             "Int16": np.int16, "Int8": np.int8, "UInt8": np.uint8, "UInt16": np.uint16, "UInt32": np.uint32,
+             "Bool": np.bool_}
+
+
+def checkpoint_decode(d):
+    """persistence.jl:24-52: nested named tuples, tuples, strings, nothing, and
+    column-major numeric arrays / scalars of the listed element types."""
+    kind = d["kind"]
+    if kind == "namedtuple":
+        names, items = d["names"], d["items"]
+        if len(names) != len(items) or len(set(names)) != len(names):
+            raise ArgumentError("invalid checkpoint field names")
+        return {n: checkpoint_decode(v) for n, v in zip(names, items)}
+    if kind == "tuple":
+        return tuple(checkpoint_decode(v) for v in d["items"])
+    if kind == "nothing":
+        return None
+    if kind == "string":
+        return str(d["value"])
+    if kind in ("array", "scalar"):
+        T = _CK_TYPES.get(d["eltype"])
+        if T is None:
+            raise ArgumentError("unsupported checkpoint numeric type")
+        if kind == "scalar":
+            v = T(d["value"])
+            if not np.all(np.isfinite(v)):
+                raise ArgumentError("nonfinite checkpoint value")
+            return v
+        shape = tuple(int(v) for v in d["shape"])
+        vals = np.asarray(d["values"], T)
+        if any(v < 0 for v in shape) or int(np.prod(shape)) != vals.size:
+            raise DimensionMismatch("checkpoint array shape")
+        if not np.all(np.isfinite(vals)):
+            raise ArgumentError("nonfinite checkpoint array")
+        return vals.reshape(shape[::-1]).T if len(shape) > 1 else vals.reshape(shape)
+    raise ArgumentError(f"unsupported checkpoint value kind {kind}")
+
+
+def checkpoint_encode(x):
+    """persistence.jl:6-22 (Julia type names, column-major values)."""
+    if isinstance(x, dict):
+        return {"kind": "namedtuple", "names": list(x), "items": [checkpoint_encode(v) for v in x.values()]}
+    if isinstance(x, tuple):
+        return {"kind": "tuple", "items": [checkpoint_encode(v) for v in x]}
+    if x is None:
+        return {"kind": "nothing"}
+    if isinstance(x, str):
+        return {"kind": "string", "value": x}
+    a = np.asarray(x)
+    name = {np.dtype(v): k for k, v in _CK_TYPES.items()}.get(a.dtype)
+    if name is None:
+        raise ArgumentError(f"unsupported checkpoint numeric type {a.dtype}")
+    if not np.all(np.isfinite(a)):
+        raise ArgumentError("nonfinite checkpoint values")
+    if a.ndim == 0:
+        return {"kind": "scalar", "eltype": name, "value": a.item()}
+    return {"kind": "array", "eltype": name, "shape": list(a.shape), "values": a.T.reshape(-1).tolist()}
+
+
+def _read_checkpoint(d):
+    """ACECalculator(path) for a checkpoint (persistence.jl:83-102, 116-129)."""
+    if d.get("schema_version") not in (1, 2):
+        raise ArgumentError("unsupported ACECalculator checkpoint version")
+    ps = checkpoint_decode(d["parameters"])
+    st = checkpoint_decode(d["state"])
+    if not (isinstance(ps, dict) and "WB" in ps and "Wpair" in ps):
+        raise ArgumentError("checkpoint parameters require WB and Wpair")
+    if set(ps) - {"WB", "Wpair"}:
+        raise ArgumentError("checkpoints with radial scale parameters are not supported on the device")
+    WB, Wpair = np.asarray(ps["WB"], float), np.asarray(ps["Wpair"], float)
+    D = dict(d["model"])
+    nz = len(D["species_atomic_numbers"])
+    if WB.shape != (int(D["coupling"]["shape"][0]), nz) or Wpair.shape != (len(D["pair_radial"]["spec"]), nz):
+        raise DimensionMismatch("checkpoint readout dimensions")
+    D["parameters"] = {"WB_shape": list(WB.shape), "Wpair_shape": list(Wpair.shape),
+                       "ordering": "vec(WB), vec(Wpair); Julia column-major",
+                       "flat": np.concatenate([WB.T.reshape(-1), Wpair.T.reshape(-1)])}
+    m = ResolvedModel(D)
+    return m, m.parameters(), st if st is not None else {}
+
+
+def _toml_value(v):
+    if isinstance(v, bool):
+        return "true" if v else "false"
+    if isinstance(v, (int, np.integer)):
+        return str(int(v))
+    if isinstance(v, (float, np.floating)):
+        if not np.isfinite(v):
+            raise ArgumentError("nonfinite value in a checkpoint")
+        return repr(float(v))
+    if isinstance(v, str):
+        return '"' + v.replace("\\", "\\\\").replace('"', '\\"') + '"'
+    if isinstance(v, np.ndarray):
+        v = v.tolist()
+    if isinstance(v, (list, tuple)):
+        return "[" + ", ".join(_toml_value(x) for x in v) + "]"
+    if isinstance(v, dict):
+        return "{" + ", ".join(f"{_toml_key(k)} = {_toml_value(x)}" for k, x in v.items()) + "}"
+    raise ArgumentError(f"unsupported checkpoint value {type(v)}")
+
+
+def _toml_key(k):
+    k = str(k)
+    return k if k.replace("_", "").replace("-", "").isalnum() else '"' + k + '"'
+
+
+def _toml_table(d, path, out):
+    tables = []
+    for k, v in d.items():
+        if isinstance(v, dict):
+            tables.append((k, v, False))
+        elif isinstance(v, list) and v and all(isinstance(x, dict) for x in v):
+            tables.append((k, v, True))
+        else:
+            out.append(f"{_toml_key(k)} = {_toml_value(v)}")
+    for k, v, many in tables:
+        sub = path + [_toml_key(k)]
+        if many:
+            for item in v:
+                out.append("")
+                out.append("[[" + ".".join(sub) + "]]")
+                _toml_table(item, sub, out)
+        else:
+            out.append("")
+            out.append("[" + ".".join(sub) + "]")
+            _toml_table(v, sub, out)
+
+
+def save_calculator(path, calc, parameters=None, state=None):
+    """``save_calculator(path, calc; parameters, state)`` (persistence.jl:147-172):
+    a self-contained versioned TOML checkpoint of the resolved basis, explicit
+    parameters and state; written to a temporary file and moved into place."""
+    import os
+    import tempfile
+    m = calc.model
+    ps = parameters if parameters is not None else calc.parameters
+    WB, Wpair = np.asarray(ps["WB"], np.float64), np.asarray(ps["Wpair"], np.float64)
+    if WB.shape != m.WB.shape or Wpair.shape != m.Wpair.shape:
+        raise DimensionMismatch("checkpoint readout dimensions")
+    model = dict(m.source)
+    model["interface_policy"] = dict(model.get("interface_policy") or {}, version=1,
+                                     mask=[bool(v) for v in m.mask], pair=m.pair_enabled,
+                                     reference=m.reference_enabled)
+    pol = model["interface_policy"]
+    pol.setdefault("species", [])
+    pol.setdefault("orders", [])
+    # provenance maps of a default policy (interface_persistence.jl:8-25 validates them)
+    pol.setdefault("rows", list(range(1, m.nbasis + 1)))
+    pol.setdefault("products", list(range(1, len(m.products) + 1)))
+    pol.setdefault("density", list(range(1, len(m.density) + 1)))
+    pol.setdefault("radial", list(range(1, m.radial.nr + 1)))
+    if len(str(pol.get("origin", ""))) != 64:
+        import hashlib
+        pol["origin"] = hashlib.sha256(repr((m.source_sha256, m.radial.spec, m.density.tolist(), m.products,
+                                             m.c_vals.tolist())).encode()).hexdigest()
+    d = {"schema": CHECKPOINT_SCHEMA, "schema_version": 2, "writer": "potentialframework.jl_amd", "model": model,
+         "parameters": checkpoint_encode({"WB": WB, "Wpair": Wpair}),
+         "state": checkpoint_encode(state if state is not None else {})}
+    out = []
+    _toml_table(d, [], out)
+    dest = os.path.abspath(path)
+    fd, tmp = tempfile.mkstemp(dir=os.path.dirname(dest))
+    try:
+        with os.fdopen(fd, "w") as f:
+            f.write("\n".join(out) + "\n")
+        os.replace(tmp, dest)
+    finally:
+        if os.path.exists(tmp):
+            os.remove(tmp)
+    return path
 
 
 # Binary form of a schema-v1 resolved model: the same dictionary, with every
@@ -275,8 +484,9 @@ class ACECalculator:
         Wp = np.asarray(ps.get("Wpair", self.model.Wpair), float)
         if WB.shape != self.model.WB.shape or Wp.shape != self.model.Wpair.shape:
             raise DimensionMismatch("readout shape")
-        self.workspace.ctx.call("pfw_set_ace_parameters", ptr(np.ascontiguousarray(WB.T)),
-                                ptr(np.ascontiguousarray(Wp.T)))
+        eWB, eWp = self.model.effective(WB, Wp)
+        self.workspace.ctx.call("pfw_set_ace_parameters", ptr(np.ascontiguousarray(eWB.T)),
+                                ptr(np.ascontiguousarray(eWp.T)))
         self.parameters = {"WB": WB.copy(), "Wpair": Wp.copy()}
 
     def evaluate(self, sys, request=("energy", "forces", "virial"), site_energies=False):

@@ -112,7 +112,9 @@ def ace_parameter_vjp(calc, systems, dE=None, dF=None, dW=None):
     gWB = np.zeros((m.nz, m.nbasis))
     gWp = np.zeros((m.nz, max(m.pair.nr, 1)))
     w.ctx.call("pfw_ace_adjoint", w.mode, ptr(E), ptr(F), ptr(W), ptr(gWB), ptr(gWp))
-    return {"WB": gWB.T.copy(), "Wpair": gWp[:, :m.pair.nr].T.copy()}
+    # the device evaluates effective_WB = WB .* mask and Wpair only with the pair term on
+    gp = gWp[:, :m.pair.nr].T.copy()
+    return {"WB": gWB.T * m.mask[:, None], "Wpair": gp if m.pair_enabled else np.zeros_like(gp)}
 
 
 def _pullback_factory(vjp, systems, single):
