@@ -28,6 +28,7 @@
 namespace pfw {
 
 constexpr int kStride = kWave + 1;
+constexpr int kRadialChunk = 40;  // radial indices per density-kernel tile
 
 template <typename T>
 struct FamArgs {
@@ -53,6 +54,9 @@ struct AceArgs {
   const double* reference;
   FamArgs<T> rad, pair;
   const int32_t* dens;
+  const int32_t* dperm;   // densities ordered by radial index
+  const int32_t* dchunk;  // offsets into dperm of each chunk of rchunk radial indices
+  int rchunk;
   const int32_t* rlb;     // L + 2: radial indices grouped by l
   const int32_t* rl;      // (radial index, first density of its m-block)
   const int32_t* prod;
@@ -292,8 +296,8 @@ __global__ __launch_bounds__(64) void k_ace_dens(AceArgs<T> a) {
   constexpr int LPS = kWave / SPW;
   T* Fl = (T*)smem;                      // (L+1)^2
   T* A = Fl + NY;                        // SPW x ndens
-  T* Rb = A + SPW * a.ndens;             // nr x kStride
-  T* Yb = Rb + a.rad.nr * kStride;       // NY x kStride
+  T* Rb = A + SPW * a.ndens;             // rchunk x kStride: one chunk of radial indices
+  T* Yb = Rb + a.rchunk * kStride;       // NY x kStride
   const int lane = threadIdx.x;
   const int sl = lane / LPS, ll = lane % LPS;
   const int64_t site = (int64_t)blockIdx.x * SPW + sl;
@@ -310,21 +314,20 @@ __global__ __launch_bounds__(64) void k_ace_dens(AceArgs<T> a) {
   const int col0 = sl * LPS;
   double epair = 0;
   __syncthreads();
-  // A[k] = sum_e R[e, r_k] Y[e, m_k]
+  // A[k] = sum_e R[e, r_k] Y[e, m_k], the radial indices taken in chunks of
+  // rchunk (densities in dperm order, grouped by chunk) so that the radial
+  // tile stays small for models with many radial functions
   for (int c0 = 0; c0 < nemax; c0 += LPS) {
     const int e = c0 + ll;
-    if (e < ne) {
+    const bool on = e < ne;
+    int ch = 0;
+    RadialPoint<T> P{};
+    if (on) {
       const double* R = a.eR + 3 * (int64_t)(e0 + e);
       const T dx = (T)R[0], dy = (T)R[1], dz = (T)R[2];
       const T r = (T)sqrt(R[0] * R[0] + R[1] * R[1] + R[2] * R[2]);
-      const int ch = iz + a.nz * a.sps[a.enb[e0 + e]];
-      const RadialPoint<T> P = radial_point(a.rad, ch, r);
-#pragma unroll 8
-      for (int n = 0; n < a.rad.nr; ++n) {
-        T v, d;
-        radial_value(a.rad, ch, P, n, v, d);
-        Rb[n * kStride + lane] = v;
-      }
+      ch = iz + a.nz * a.sps[a.enb[e0 + e]];
+      P = radial_point(a.rad, ch, r);
       if (nq) {
         const RadialPoint<T> PQ = radial_point(a.pair, ch, r);
         T s = 0;
@@ -340,17 +343,30 @@ __global__ __launch_bounds__(64) void k_ace_dens(AceArgs<T> a) {
 #pragma unroll
       for (int m = 0; m < NY; ++m) Yb[m * kStride + lane] = Y[m];
     }
-    __syncthreads();
-    const int cn = min(LPS, ne - c0);
-    for (int k = ll; k < a.ndens; k += LPS) {
-      const T* rr = Rb + a.dens[2 * k] * kStride + col0;
-      const T* yy = Yb + a.dens[2 * k + 1] * kStride + col0;
-      T s = 0;
+      const int cn = min(LPS, ne - c0);
+    const bool single = a.rchunk >= a.rad.nr;
+    for (int r0 = 0, q = 0; r0 < a.rad.nr; r0 += a.rchunk, ++q) {
+      if (on) {
+        const int r1 = min(a.rad.nr, r0 + a.rchunk);
 #pragma unroll 8
-      for (int t = 0; t < cn; ++t) s += rr[t] * yy[t];
-      As[k] += s;
+        for (int n = r0; n < r1; ++n) {
+          T v, d;
+          radial_value(a.rad, ch, P, n, v, d);
+          Rb[(n - r0) * kStride + lane] = v;
+        }
+      }
+      __syncthreads();
+      for (int i = a.dchunk[q] + ll; i < a.dchunk[q + 1]; i += LPS) {
+        const int k = single ? i : a.dperm[i];  // one chunk: dperm is the identity
+        const T* rr = Rb + (a.dens[2 * k] - r0) * kStride + col0;
+        const T* yy = Yb + a.dens[2 * k + 1] * kStride + col0;
+        T s = 0;
+#pragma unroll 8
+        for (int t = 0; t < cn; ++t) s += rr[t] * yy[t];
+        As[k] += s;
+      }
+      __syncthreads();
     }
-    __syncthreads();
   }
   if (valid) {
     for (int k = ll; k < a.ndens; k += LPS) a.Abuf[site * a.ndens + k] = As[k];
@@ -1086,6 +1102,8 @@ void ace_free(pfw_ctx* ctx) {
   dfree(a.ws32);
   dfree(a.dens);
   dfree(a.rlb);
+  dfree(a.dperm);
+  dfree(a.dchunk);
   dfree(a.rl);
   dfree(a.w64);
   dfree(a.w32);
@@ -1345,6 +1363,18 @@ void ace_set_model(pfw_ctx* ctx, const pfw_ace_model* m) {
     rlb[l + 1] = (int32_t)(rl.size() / 2);
   }
   a.dens = upload(dens, st);
+  // densities grouped by chunks of radial indices for the density kernel's tile
+  a.rchunk = std::max(1, std::min(a.radial.nr, kRadialChunk));
+  {
+    std::vector<int32_t> dperm, dchunk(1, 0);
+    for (int r0 = 0; r0 < std::max(a.radial.nr, 1); r0 += a.rchunk) {
+      for (int k = 0; k < m->ndens; ++k)
+        if (dens[2 * k] >= r0 && dens[2 * k] < r0 + a.rchunk) dperm.push_back(k);
+      dchunk.push_back((int32_t)dperm.size());
+    }
+    a.dperm = upload(dperm, st);
+    a.dchunk = upload(dchunk, st);
+  }
   a.rlb = upload(rlb, st);
   a.rl = upload(rl, st);
   // products (factor lists, -1 padded) and their conflict-free schedule
@@ -1433,6 +1463,9 @@ static AceArgs<T> make_args(pfw_ctx* ctx, int grads) {
   a.rad = fam_args<T>(ad.radial);
   a.pair = fam_args<T>(ad.pair);
   a.dens = ad.dens;
+  a.dperm = ad.dperm;
+  a.dchunk = ad.dchunk;
+  a.rchunk = ad.rchunk;
   a.rlb = ad.rlb;
   a.rl = ad.rl;
   a.prod = ad.prod;
@@ -1481,7 +1514,7 @@ static void launch_sites(pfw_ctx* ctx, int grads) {
   AceArgs<T> a = make_args<T>(ctx, grads);
   constexpr int NY = (L + 1) * (L + 1);
   const unsigned blocks = (unsigned)((s.n + SPW - 1) / SPW);
-  const size_t lds_dens = sizeof(T) * ((size_t)NY + SPW * (size_t)ad.ndens + (size_t)(ad.radial.nr + NY) * kStride);
+  const size_t lds_dens = sizeof(T) * ((size_t)NY + SPW * (size_t)ad.ndens + (size_t)(ad.rchunk + NY) * kStride);
   allow_lds(k_ace_dens<T, L, SPW>, lds_dens, "the density kernel");
   hipLaunchKernelGGL((k_ace_dens<T, L, SPW>), dim3(blocks), dim3(kWave), lds_dens, st, a);
   PFW_HIP(hipGetLastError());

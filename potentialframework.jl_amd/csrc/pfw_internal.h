@@ -168,6 +168,9 @@ struct AceDev {
   double* ws64 = nullptr;       // nz x slots readout weights in schedule order
   float* ws32 = nullptr;
   int32_t* dens = nullptr;      // ndens x 2
+  int32_t* dperm = nullptr;     // densities ordered by chunks of radial indices
+  int32_t* dchunk = nullptr;    // chunk offsets into dperm
+  int rchunk = 1;               // radial indices per chunk
   int32_t* rlb = nullptr;       // lmax + 2: radial indices grouped by l
   int32_t* rl = nullptr;        // (radial index, first density of its m-block)
   int max_degree = 0;           // largest edge count of the last environment build
