@@ -203,10 +203,12 @@ __device__ inline Jet<T> jsub(const Jet<T>& a, const Jet<T>& b) {
 // Real L2 Y_lm (no Condon-Shortley phase, index l*l+l+m) of the direction of
 // (dx, dy, dz) and their Cartesian gradients (angular_blocks.jl:36-81); with
 // `solid` the same recurrence on the raw coordinates, the real solid harmonics
-// r^l Y_lm (angular_blocks.jl:47-48).
+// r^l Y_lm (angular_blocks.jl:47-48). Each harmonic is handed to
+// emit(index, jet) as soon as it is formed, so a caller that contracts them
+// need not hold all values and gradients at once.
 // Fl holds F[l][m] = sqrt((2l+1)/2pi) * prod(-1/sqrt((l+m')(l+1-m'))).
-template <typename T, int L>
-__device__ inline void ylm(T dx, T dy, T dz, const T* Fl, T* Y, T* G, int solid) {
+template <typename T, int L, typename Emit>
+__device__ inline void ylm_emit(T dx, T dy, T dz, const T* Fl, int solid, Emit emit) {
   const T r = sqrt(dx * dx + dy * dy + dz * dz);
   const T ri = (T)1 / r;
   const T ux = dx * ri, uy = dy * ri, uz = dz * ri;
@@ -219,47 +221,49 @@ __device__ inline void ylm(T dx, T dy, T dz, const T* Fl, T* Y, T* G, int solid)
     z = {dz, (T)0, (T)0, (T)1};
   }
   const Jet<T> r2 = jadd(jadd(jm(x, x), jm(y, y)), jm(z, z));
-  Jet<T> cs[L + 1], sn[L + 1], Q[(L + 1) * (L + 2) / 2];
-  cs[0] = {(T)1, 0, 0, 0};
-  sn[0] = {(T)0, 0, 0, 0};
+  // m outer, l inner: the recurrences Q[l][l] = -(2l-1) Q[l-1][l-1],
+  // Q[l][l-1] = (2l-1) z Q[l-1][l-1] and, for m <= l-2,
+  // Q[l][m] = ((2l-1) z Q[l-1][m] - (l+m-1) r^2 Q[l-2][m]) / (l-m), with
+  // cos/sin(m phi) advanced with m; every value is formed by the same
+  // operations as in l-outer order, but only a few jets are live at a time
+  Jet<T> c{(T)1, 0, 0, 0}, sgn{(T)0, 0, 0, 0}, diag{(T)1, 0, 0, 0};
 #pragma unroll
-  for (int m = 1; m <= L; ++m) {
-    cs[m] = jsub(jm(cs[m - 1], x), jm(sn[m - 1], y));
-    sn[m] = jadd(jm(sn[m - 1], x), jm(cs[m - 1], y));
-  }
-  cs[0] = {(T)0.70710678118654752440, 0, 0, 0};
-#define QI(l, m) ((l) * ((l) + 1) / 2 + (m))
-  Q[0] = {(T)1, 0, 0, 0};
-#pragma unroll
-  for (int l = 0; l <= L; ++l) {
-    if (l > 0) {
-      Q[QI(l, l)] = js((T) - (2 * l - 1), Q[QI(l - 1, l - 1)]);
-      Q[QI(l, l - 1)] = js((T)(2 * l - 1), jm(z, Q[QI(l - 1, l - 1)]));
-#pragma unroll
-      for (int m = 0; m <= l - 2; ++m)
-        Q[QI(l, m)] = js((T)1 / (T)(l - m), jsub(js((T)(2 * l - 1), jm(z, Q[QI(l - 1, m)])),
-                                                 js((T)(l + m - 1), jm(r2, Q[QI(l - 2, m)]))));
+  for (int m = 0; m <= L; ++m) {
+    if (m > 0) {
+      const Jet<T> c1 = jsub(jm(c, x), jm(sgn, y));
+      sgn = jadd(jm(sgn, x), jm(c, y));
+      c = c1;
+      diag = js((T) - (2 * m - 1), diag);
     }
+    const Jet<T> cm = m == 0 ? Jet<T>{(T)0.70710678118654752440, 0, 0, 0} : c;
+    Jet<T> q2 = diag, q1 = diag;  // Q[l-2][m], Q[l-1][m]
 #pragma unroll
-    for (int m = 0; m <= l; ++m) {
+    for (int l = m; l <= L; ++l) {
+      Jet<T> q;
+      if (l == m)
+        q = diag;
+      else if (l == m + 1)
+        q = js((T)(2 * l - 1), jm(z, q1));
+      else
+        q = js((T)1 / (T)(l - m), jsub(js((T)(2 * l - 1), jm(z, q1)), js((T)(l + m - 1), jm(r2, q2))));
       const T F = Fl[l * (L + 1) + m];
-      const Jet<T> pos = js(F, jm(Q[QI(l, m)], cs[m]));
-      const int ip = l * l + l + m;
-      Y[ip] = pos.v;
-      G[3 * ip] = pos.g0;
-      G[3 * ip + 1] = pos.g1;
-      G[3 * ip + 2] = pos.g2;
-      if (m > 0) {
-        const Jet<T> neg = js(F, jm(Q[QI(l, m)], sn[m]));
-        const int im = l * l + l - m;
-        Y[im] = neg.v;
-        G[3 * im] = neg.g0;
-        G[3 * im + 1] = neg.g1;
-        G[3 * im + 2] = neg.g2;
-      }
+      emit(l * l + l + m, js(F, jm(q, cm)));
+      if (m > 0) emit(l * l + l - m, js(F, jm(q, sgn)));
+      q2 = q1;
+      q1 = q;
     }
   }
-#undef QI
+}
+
+// All (L+1)^2 values Y and gradients G (3 per harmonic).
+template <typename T, int L>
+__device__ inline void ylm(T dx, T dy, T dz, const T* Fl, T* Y, T* G, int solid) {
+  ylm_emit<T, L>(dx, dy, dz, Fl, solid, [&](int i, const Jet<T>& j) {
+    Y[i] = j.v;
+    G[3 * i] = j.g0;
+    G[3 * i + 1] = j.g1;
+    G[3 * i + 2] = j.g2;
+  });
 }
 
 // The site pipeline runs as three kernels:
@@ -503,10 +507,12 @@ __global__ __launch_bounds__(64) void k_ace_pull(AceArgs<T> a) {
     const T r = (T)sqrt(R[0] * R[0] + R[1] * R[1] + R[2] * R[2]);
     const T ri = (T)1 / r;
     const int ch = iz + a.nz * a.sps[a.enb[e0 + e]];
-    T Y[NY], G[3 * NY], U[NY];
-    ylm<T, L>(dx, dy, dz, Fl, Y, G, a.solid);
+    // U_m = sum_r dE/dA[r, m] R_r and V_m = sum_r dE/dA[r, m] R'_r per harmonic;
+    // then g = (sum_m V_m Y_m) r^ + sum_m U_m grad Y_m, the harmonics contracted
+    // as they are formed (no arrays of values and gradients held)
+    T U[NY], V[NY];
 #pragma unroll
-    for (int m = 0; m < NY; ++m) U[m] = 0;
+    for (int m = 0; m < NY; ++m) U[m] = V[m] = 0;
     T gs = 0;
     const RadialPoint<T> P = radial_point(a.rad, ch, r);
 #pragma unroll
@@ -516,14 +522,12 @@ __global__ __launch_bounds__(64) void k_ace_pull(AceArgs<T> a) {
         const int n = a.rl[2 * q], a0 = a.rl[2 * q + 1];
         T v, d;
         radial_value(a.rad, ch, P, n, v, d);
-        T tr = 0;
 #pragma unroll
         for (int mm = 0; mm <= 2 * l; ++mm) {
           const T c = dAs[a0 + mm];
-          tr += c * Y[l * l + mm];
           U[l * l + mm] += c * v;
+          V[l * l + mm] += c * d;
         }
-        gs += d * tr;
       }
     }
     if (nq) {
@@ -534,13 +538,14 @@ __global__ __launch_bounds__(64) void k_ace_pull(AceArgs<T> a) {
         gs += a.wp[iz * nq + n] * d;
       }
     }
-    T g0 = gs * dx * ri, g1 = gs * dy * ri, g2 = gs * dz * ri;
-#pragma unroll
-    for (int m = 0; m < NY; ++m) {
-      g0 += U[m] * G[3 * m];
-      g1 += U[m] * G[3 * m + 1];
-      g2 += U[m] * G[3 * m + 2];
-    }
+    T h0 = 0, h1 = 0, h2 = 0;
+    ylm_emit<T, L>(dx, dy, dz, Fl, a.solid, [&](int i, const Jet<T>& j) {
+      gs += V[i] * j.v;
+      h0 += U[i] * j.g0;
+      h1 += U[i] * j.g1;
+      h2 += U[i] * j.g2;
+    });
+    const T g0 = gs * dx * ri + h0, g1 = gs * dy * ri + h1, g2 = gs * dz * ri + h2;
     T* go = a.eg + 3 * (int64_t)(e0 + e);
     go[0] = g0;
     go[1] = g1;
