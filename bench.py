@@ -30,6 +30,7 @@ PKG_DIR = os.path.join(ROOT, "potentialframework.jl_amd")
 METRIC = "Matom-evals/s (E+F+V), 256k-atom pair & 32k-atom ACE; mixed vs FP64 at tol"
 PEAK_FP32_TFLOPS = 157.3  # MI355X published FP32 (vector/matrix) peak
 PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0  # MI355X HBM3E
 STAGES = ("binning", "scan", "scatter_sort", "environments", "interaction", "reduction")
 # Algorithmic flops per directed in-cutoff pair (FMA = 2): d + shift 6, r^2 5,
 # compare 1, q 2, a q^3 3, g 2, force 6, 1/2 E 2, 1/2 W 18.
@@ -252,7 +253,7 @@ def bench_verlet(pfw, lib, _capi, X, H, pbc, sp, model, labels, skin, prec, mode
         "value": round(n / float(times.mean()) / 1e6, 3), "unit": "Matom-evals/s",
         "reuse_median_ms": round(1e3 * float(np.median(reuse)), 4) if len(reuse) else None,
         "rebuild_ms": [round(1e3 * float(t), 4) for t in times[rebuilt]],
-        **({} if ace else {"ghosts": info["ghosts"], "slots_per_atom": info["slots"]}),
+        **({} if ace else {"ghosts": info["ghosts"], "slots_per_atom": info["slots"], "list_entries": info["entries"]}),
         "reuse_stages_ms": {k: round(float(v), 5) for k, v in zip(STAGES, stage)},
     }
 
@@ -422,11 +423,20 @@ def main():
         vb = {prec: bench_verlet(pfw, lib, _capi, X, H, pbc, sp, p, labels_b2, 0.5, prec, mode)
               for prec, mode in (("mixed", _capi.PFW_MIXED), ("float64", _capi.PFW_FP64))}
         kl = vb["mixed"]["reuse_stages_ms"]["interaction"]
-        # the list kernel evaluates the same directed in-cutoff pairs (to within the motion)
-        vb["mixed"]["roofline"] = {"kernel": "k_pair_list<float,SMOOTH>", "bound": "mfma",
-                                   "compute_unit": "VALU fp32",
-                                   "achieved": round(pflops / (kl * 1e-3) / 1e12, 3), "peak": PEAK_FP32_TFLOPS,
-                                   "unit": "TFLOP/s", "frac": round(pflops / (kl * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4),
+        # HBM is the nearer roof of the list kernel: per list entry a 4 B index;
+        # per extended entry (atoms, ghosts, the sentinel) its 32 B position
+        # record; per atom its row length, permutation slot and FP64 force
+        # (4 + 4 + 24 B); per 16-atom wavefront a 56 B energy / virial record.
+        ne = vb["mixed"]["list_entries"]
+        next_ = n + vb["mixed"]["ghosts"] + 1
+        lbytes = 4 * ne + 32 * next_ + 32 * n + 56 * ((n + 15) // 16)
+        # the FLOP side: the same directed in-cutoff pairs as the fresh step (to within the motion)
+        vb["mixed"]["roofline"] = {"kernel": "k_pair_list<float,SMOOTH>", "bound": "hbm",
+                                   "achieved": round(lbytes / (kl * 1e-3) / 1e9, 1), "peak": PEAK_HBM_GBS,
+                                   "unit": "GB/s", "frac": round(lbytes / (kl * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+                                   "algorithmic_bytes": lbytes,
+                                   "traffic": read_traffic("pair_list_b2_256k_traffic.json"),
+                                   "valu_fp32_frac": round(pflops / (kl * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4),
                                    "kernel_ms": kl}
         line["verlet"] = {"workload": "b2_smoothpair_256k_triclinic, trajectory x_s = x_0 + s v", **vb}
         Xr, Hr, pbr, spr = pfw.workloads.argon_fcc(14, sigma=0.03)

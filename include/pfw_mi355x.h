@@ -129,9 +129,10 @@ int pfw_export_pairs(pfw_ctx* ctx, int32_t* i, int32_t* j, int32_t* S, double* R
  * reference, triclinic and multi-image cells are allowed (periodic images are
  * explicit ghost entries). skin = 0 returns to fresh cells every step. */
 int pfw_set_skin(pfw_ctx* ctx, double skin);
-/* Verlet state: info[0] rebuilds, info[1] reused steps (the reference's
- * w.rebuilds counter, gpu.jl:100,508), info[2] ghost entries, info[3] list
- * slots per atom; max_displacement: largest displacement since the build seen
+/* Verlet state, info[5]: info[0] rebuilds, info[1] reused steps (the
+ * reference's w.rebuilds counter, gpu.jl:100,508), info[2] ghost entries,
+ * info[3] list slots per atom, info[4] list entries (pairs within cutoff +
+ * skin at the last build, each direction counted); max_displacement: largest displacement since the build seen
  * by the last step (Å). Either pointer may be NULL. */
 int pfw_verlet_info(pfw_ctx* ctx, int64_t* info, double* max_displacement);
 

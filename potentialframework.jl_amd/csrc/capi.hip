@@ -330,6 +330,7 @@ int pfw_verlet_info(pfw_ctx* ctx, int64_t* info, double* max_displacement) {
       info[1] = V.reuses;
       info[2] = V.valid ? V.n_ext - ctx->sys.n : 0;
       info[3] = V.ecap;
+      info[4] = V.valid ? V.entries : 0;
     }
     if (max_displacement) *max_displacement = V.last_max_disp;
   });

@@ -126,8 +126,9 @@ struct VerletDev {
   double* Xref = nullptr;       // 3 x n caller positions at the build (AoS, original order)
   unsigned long long* maxd2 = nullptr;  // 2 words: largest squared displacement (bits), double-buffered
   unsigned long long* h_maxd2 = nullptr;  // pinned copy
-  int32_t* status = nullptr;    // 4: unused, largest degree, overflow
+  int32_t* status = nullptr;    // 6: unused, largest degree, overflow, unused, list entries (uint64)
   int32_t* h_status = nullptr;  // pinned
+  int64_t entries = 0;          // list entries of the last build
   double* part = nullptr;       // list kernel energy / virial records
   int64_t part_capacity = 0;
 };

@@ -243,10 +243,15 @@ __global__ __launch_bounds__(64) void k_verlet_build(const Grid* __restrict__ gr
     }
     if (lane < na) ncount[mine] = min(pos, ecap);
     int dmax = lane < na ? pos : 0;  // one atomic per wavefront (same-address atomics serialise)
-    for (int off = kWave / 2; off > 0; off >>= 1) dmax = max(dmax, __shfl_xor(dmax, off, kWave));
+    int dsum = lane < na ? min(pos, ecap) : 0;
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+      dmax = max(dmax, __shfl_xor(dmax, off, kWave));
+      dsum += __shfl_xor(dsum, off, kWave);
+    }
     if (lane == 0) {
       atomicMax(status + 1, dmax);
       if (dmax > ecap) atomicOr(status + 2, 1);  // capacity too small: the host grows it and builds again
+      if (pass) atomicAdd(reinterpret_cast<unsigned long long*>(status + 4), (unsigned long long)dsum);
     }
   }
 }
@@ -463,11 +468,11 @@ static void verlet_setup(pfw_ctx* ctx) {
   dalloc(V.ncount, n);
   dalloc(V.Xref, 3 * n);
   dalloc(V.maxd2, 2);
-  dalloc(V.status, 4);
+  dalloc(V.status, 6);
   dalloc(V.part, 7 * n);
   V.part_capacity = 7 * n;
   if (!V.h_maxd2) PFW_HIP(hipHostMalloc((void**)&V.h_maxd2, 2 * sizeof(unsigned long long), hipHostMallocDefault));
-  if (!V.h_status) PFW_HIP(hipHostMalloc((void**)&V.h_status, 4 * sizeof(int32_t), hipHostMallocDefault));
+  if (!V.h_status) PFW_HIP(hipHostMalloc((void**)&V.h_status, 6 * sizeof(int32_t), hipHostMallocDefault));
   std::vector<int32_t> e0(ctx->ncfg + 1);
   for (int k = 0; k < ctx->ncfg; ++k) e0[k] = ctx->grids[k].ecell0;
   e0[ctx->ncfg] = (int32_t)ec;
@@ -482,12 +487,13 @@ static void build_pass(pfw_ctx* ctx, int pass) {
   VerletDev& V = ctx->verlet;
   SystemBuffers& s = ctx->sys;
   hipStream_t st = ctx->stream;
-  PFW_HIP(hipMemsetAsync(V.status, 0, 4 * sizeof(int32_t), st));
+  PFW_HIP(hipMemsetAsync(V.status, 0, 6 * sizeof(int32_t), st));
   hipLaunchKernelGGL(k_verlet_build, dim3(ctx->ncells), dim3(kWave), 0, st, s.grids, s.cell_cfg, s.n, s.xs, s.start,
                      V.gstart, pass, pass ? V.ecap : 0, V.ncount, V.nbr, V.status);
   PFW_HIP(hipGetLastError());
-  PFW_HIP(hipMemcpyAsync(V.h_status, V.status, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  PFW_HIP(hipMemcpyAsync(V.h_status, V.status, 6 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
   PFW_HIP(hipStreamSynchronize(st));
+  if (pass) V.entries = (int64_t)(*reinterpret_cast<const unsigned long long*>(V.h_status + 4));
 }
 
 static void grow_list(pfw_ctx* ctx, int degree) {

@@ -58,13 +58,14 @@ class Workspace:
         self.ctx.call("pfw_set_skin", ctypes.c_double(skin))
 
     def verlet_info(self):
-        """``rebuilds``, ``reuses``, ``ghosts``, ``slots`` (per atom) and the last
-        step's largest displacement since the build (Å)."""
-        info = np.zeros(4, np.int64)
+        """``rebuilds``, ``reuses``, ``ghosts``, ``slots`` (per atom), ``entries``
+        (list entries at the last build) and the last step's largest
+        displacement since the build (Å)."""
+        info = np.zeros(5, np.int64)
         md = np.zeros(1)
         self.ctx.call("pfw_verlet_info", ptr(info, ctypes.c_int64), ptr(md))
         return dict(rebuilds=int(info[0]), reuses=int(info[1]), ghosts=int(info[2]), slots=int(info[3]),
-                    max_displacement=float(md[0]))
+                    entries=int(info[4]), max_displacement=float(md[0]))
 
     @property
     def rebuilds(self):

@@ -13,7 +13,7 @@ cell build by another entry point between two steps.
 import numpy as np
 import pytest
 
-from oracle import KIND_LJ, KIND_SMOOTH, pair_evaluate
+from oracle import KIND_LJ, KIND_SMOOTH, nl_pairs, pair_evaluate
 
 pytestmark = pytest.mark.gpu
 
@@ -93,6 +93,8 @@ def test_skin_random_walk_triclinic(pfw, prec):
     rng = np.random.default_rng(3)
     for step in range(16):
         check(pfw, w, prec, X, H, pbc, sp, 6.0)
+        if step == 0:  # the list holds every directed pair within cutoff + skin
+            assert w.verlet_info()["entries"] == len(nl_pairs(X, H, pbc, 6.5)[0])
         X = X + rng.normal(0, 0.03, X.shape)
     info = w.verlet_info()
     assert info["rebuilds"] >= 2 and info["reuses"] >= 8, info
@@ -111,6 +113,8 @@ def test_skin_multi_image_cells(pfw, prec):
             Y = X.copy()
             for step in range(4):
                 check(pfw, w, prec, Y, H, pbc, sp, 2.0)
+                if step == 0:  # every image pair within cutoff + skin, ghosts included
+                    assert w.verlet_info()["entries"] == len(nl_pairs(Y, H, pbc, 2.3)[0])
                 Y = Y + rng.normal(0, 0.02, Y.shape)
             assert w.rebuilds >= 1
 
