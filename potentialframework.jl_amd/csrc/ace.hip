@@ -626,11 +626,7 @@ __global__ __launch_bounds__(64) void k_ace_edges(const Grid* __restrict__ grids
           if (zero) s_zero = lane;
         }
       }
-      int incl = c;
-      for (int off = 1; off < kWave; off <<= 1) {
-        const int t = __shfl_up(incl, off, kWave);
-        if (lane >= off) incl += t;
-      }
+      const int incl = wave_incl_scan(c);
       s_pref[lane + 1] = incl;
       if (lane == 0) s_pref[0] = 0;
       const int total = __shfl(incl, kWave - 1, kWave);

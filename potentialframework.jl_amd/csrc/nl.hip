@@ -100,7 +100,7 @@ static void dmalloc(T** p, size_t count) {
 
 void nl_free(pfw_ctx* ctx) {
   SystemBuffers& s = ctx->sys;
-  void* ptrs[] = {s.X,     s.species, s.xw,     s.u,      s.cell_of,  s.count,    s.start,    s.cursor,
+  void* ptrs[] = {s.X,     s.species, s.xw,     s.u,      s.cell_of,  s.count,    s.start,    s.rank,
                   s.blocksum, s.perm, s.xs,    s.sps,    s.bounds, s.grids, s.atom_cfg, s.cell_cfg, s.cfg_cell0};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -242,7 +242,7 @@ void nl_configure(pfw_ctx* ctx, double c, double skin) {
   dmalloc(&s.sps, n);
   dmalloc(&s.count, total);
   dmalloc(&s.start, total + 1);
-  dmalloc(&s.cursor, total);
+  dmalloc(&s.rank, n);
   dmalloc(&s.blocksum, (total + 1023) / 1024 + 1);
   dmalloc(&s.bounds, 6 * (size_t)ncfg);
   dmalloc(&s.grids, ncfg);
@@ -296,24 +296,38 @@ __device__ inline int bin_atom(const Grid& g, const double* u) {
   return cell_index(g, k);
 }
 
-// count[c] += 1 for every active lane, one atomic per distinct cell of the
-// wavefront (lanes of a wavefront often share a cell).
-__device__ inline void wave_count(int32_t* count, int c) {
+// count[c] += 1 for every active lane, with one atomic per distinct cell of the
+// wavefront (lanes of a wavefront often share a cell), all issued together.
+// Returns the lane's rank among the atoms counted into its cell: the count
+// before this wavefront's atomic plus the lane's place among the wavefront's
+// lanes of that cell. The ranks of a cell are 0 .. count - 1, in an order that
+// depends on scheduling; the cell sort fixes the order afterwards.
+__device__ inline int wave_rank_count(int32_t* count, int c) {
+  const int lane = (int)(threadIdx.x % kWave);
+  const uint64_t below = (1ull << lane) - 1;
   uint64_t todo = __ballot(true);
-  while (todo) {
-    const int leader = __builtin_ctzll(todo);
-    const int cl = __shfl(c, leader, kWave);
+  int leader = 0, before = 0, group = 0;
+  while (todo) {  // group the active lanes by cell; the loop control is uniform
+    const int first = __builtin_ctzll(todo);
+    const int cl = __builtin_amdgcn_readlane(c, first);
     const uint64_t same = __ballot(c == cl) & todo;
-    if ((int)(threadIdx.x % kWave) == leader) atomicAdd(&count[cl], __popcll(same));
+    if (c == cl) {
+      leader = first;
+      before = __popcll(same & below);
+      group = __popcll(same);
+    }
     todo &= ~same;
   }
+  int old = 0;
+  if (lane == leader) old = atomicAdd(&count[c], group);
+  return __shfl(old, leader, kWave) + before;
 }
 
 // Wrap every atom with its configuration's basis; when no configuration has
 // an open axis also bin and count in the same pass.
 __global__ void k_wrap(const Grid* __restrict__ grids, const int32_t* __restrict__ atom_cfg, int64_t n,
                        const double* __restrict__ X, double* __restrict__ xw, double* __restrict__ u,
-                       int32_t* __restrict__ cell_of, int32_t* __restrict__ count,
+                       int32_t* __restrict__ cell_of, int32_t* __restrict__ count, int32_t* __restrict__ rank,
                        unsigned long long* __restrict__ bounds, int open_any) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -325,7 +339,7 @@ __global__ void k_wrap(const Grid* __restrict__ grids, const int32_t* __restrict
   if (!open_any) {  // bin coordinates are needed later only for open axes
     const int c = bin_atom(g, uu);
     cell_of[i] = c;
-    wave_count(count, c);
+    rank[i] = wave_rank_count(count, c);
     return;
   }
   for (int a = 0; a < 3; ++a) u[a * n + i] = uu[a];
@@ -365,14 +379,15 @@ __global__ void k_open_grid(Grid* __restrict__ grids, const unsigned long long* 
 }
 
 __global__ void k_bin_open(const Grid* __restrict__ grids, const int32_t* __restrict__ atom_cfg, int64_t n,
-                           const double* __restrict__ u, int32_t* __restrict__ cell_of, int32_t* __restrict__ count) {
+                           const double* __restrict__ u, int32_t* __restrict__ cell_of, int32_t* __restrict__ count,
+                           int32_t* __restrict__ rank) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const Grid& g = grids[atom_cfg[i]];
   double uu[3] = {u[i], u[n + i], u[2 * n + i]};
   const int c = bin_atom(g, uu);
   cell_of[i] = c;
-  atomicAdd(&count[c], 1);
+  rank[i] = wave_rank_count(count, c);
 }
 
 // Exclusive scan of cell counts: per-block totals, a one-block scan of the
@@ -384,20 +399,12 @@ constexpr int kScanBlock = kScanItems * kScanThreads;
 __device__ inline int block_exclusive_scan(int v, int* lds, int* total) {
   // lds: blockDim.x / 64 + 1 ints. Wave scans, then one wave scans the wave totals.
   const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave, nw = blockDim.x / kWave;
-  int incl = v;
-  for (int off = 1; off < kWave; off <<= 1) {
-    int t = __shfl_up(incl, off, kWave);
-    if (lane >= off) incl += t;
-  }
+  const int incl = wave_incl_scan(v);
   if (lane == kWave - 1) lds[wid] = incl;
   __syncthreads();
   if (wid == 0) {
     const int t = lane < nw ? lds[lane] : 0;
-    int ti = t;
-    for (int off = 1; off < kWave; off <<= 1) {
-      int u = __shfl_up(ti, off, kWave);
-      if (lane >= off) ti += u;
-    }
+    const int ti = wave_incl_scan(t);
     if (lane < nw) lds[lane] = ti - t;
     if (lane == nw - 1) lds[nw] = ti;
   }
@@ -411,13 +418,13 @@ __device__ inline int block_exclusive_scan(int v, int* lds, int* total) {
 // One-workgroup exclusive scan for up to kScanOneMax counts, read as rows of
 // 1024 (coalesced, all in flight at once). All rows are scanned together:
 // wave scans in registers, one pass over the wave totals of every row, one
-// pass over the row totals. Also zeroes `cursor` and, with zero_count, the
+// pass over the row totals. With zero_count it also zeroes the
 // counts themselves (ready for the next step's binning).
 constexpr int kScanOneThreads = 1024;
 constexpr int kScanOneRows = 32;
 constexpr int kScanOneMax = kScanOneThreads * kScanOneRows;
 __global__ __launch_bounds__(1024) void k_scan_one(int32_t* __restrict__ count, int m, int32_t* __restrict__ start,
-                                                   int32_t* __restrict__ cursor, int64_t n, int zero_count) {
+                                                   int64_t n, int zero_count) {
   constexpr int NW = kScanOneThreads / kWave;
   __shared__ int wt[kScanOneRows][NW];    // wave totals of each row
   __shared__ int woff[kScanOneRows][NW];  // exclusive wave offsets within each row
@@ -435,10 +442,7 @@ __global__ __launch_bounds__(1024) void k_scan_one(int32_t* __restrict__ count, 
   for (int r = 0; r < kScanOneRows; ++r) {
     int x = v[r];
     if (r < rows) {  // uniform over the workgroup
-      for (int off = 1; off < kWave; off <<= 1) {
-        const int t = __shfl_up(x, off, kWave);
-        if (lane >= off) x += t;
-      }
+      x = wave_incl_scan(x);
       if (lane == kWave - 1) wt[r][wid] = x;
     }
     incl[r] = x;
@@ -466,7 +470,6 @@ __global__ __launch_bounds__(1024) void k_scan_one(int32_t* __restrict__ count, 
     const int k = r * kScanOneThreads + tid;
     if (r < rows && k < m) {
       start[k] = roff[r] + woff[r][wid] + incl[r] - v[r];
-      cursor[k] = 0;
       if (zero_count) count[k] = 0;
     }
   }
@@ -499,7 +502,7 @@ __global__ void k_scan_top(int32_t* __restrict__ blocksum, int nb) {
 }
 
 __global__ void k_scan_final(const int32_t* __restrict__ count, int ncells, const int32_t* __restrict__ blocksum,
-                             int32_t* __restrict__ start, int32_t* __restrict__ cursor, int64_t n) {
+                             int32_t* __restrict__ start, int64_t n) {
   __shared__ int lds[kScanThreads / kWave + 1];
   int base = blockIdx.x * kScanBlock + threadIdx.x * kScanItems;
   int v[kScanItems], s = 0;
@@ -512,7 +515,6 @@ __global__ void k_scan_final(const int32_t* __restrict__ count, int ncells, cons
   for (int k = 0; k < kScanItems; ++k) {
     if (base + k < ncells) {
       start[base + k] = ex;
-      cursor[base + k] = 0;
     }
     ex += v[k];
   }
@@ -524,13 +526,15 @@ __global__ void k_scan_final(const int32_t* __restrict__ count, int ncells, cons
   }
 }
 
+// Count-sort placement from the ranks of the counting pass (no atomics); the
+// order inside a cell is fixed by the cell sort that follows.
 __global__ void k_scatter(int64_t n, const int32_t* __restrict__ cell_of, const int32_t* __restrict__ start,
-                          int32_t* __restrict__ cursor, int32_t* __restrict__ perm) {
+                          const int32_t* __restrict__ rank, int32_t* __restrict__ perm) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int c = cell_of[i];
-  const int pos = start[c] + atomicAdd(&cursor[c], 1);  // order inside a cell is fixed later
-  perm[pos] = (int32_t)i;
+  const int pos = start[c] + rank[i];
+  if (pos < start[c + 1]) perm[pos] = (int32_t)i;  // ranks are 0 .. count - 1 by construction
 }
 
 // Deterministic order inside each cell: ascending original index (the stable
@@ -662,12 +666,12 @@ __global__ void k_sum_partials(const double* __restrict__ partial, int nblocks, 
 
 static inline unsigned nblocks(int64_t n, int t) { return (unsigned)std::max<int64_t>(1, (n + t - 1) / t); }
 
-// Exclusive scan of m counts into start[0..m] (start[m] = total); also zeroes cursor.
+// Exclusive scan of m counts into start[0..m] (start[m] = total).
 // blocksum needs (m + 1023) / 1024 + 1 ints.
-void scan_counts(pfw_ctx* ctx, const int32_t* count, int m, int32_t* start, int32_t* blocksum, int32_t* cursor) {
+void scan_counts(pfw_ctx* ctx, const int32_t* count, int m, int32_t* start, int32_t* blocksum) {
   hipStream_t st = ctx->stream;
   if (m <= kScanOneMax) {
-    hipLaunchKernelGGL(k_scan_one, dim3(1), dim3(kScanOneThreads), 0, st, const_cast<int32_t*>(count), m, start, cursor,
+    hipLaunchKernelGGL(k_scan_one, dim3(1), dim3(kScanOneThreads), 0, st, const_cast<int32_t*>(count), m, start,
                        (int64_t)-1, 0);
     PFW_HIP(hipGetLastError());
     return;
@@ -675,7 +679,7 @@ void scan_counts(pfw_ctx* ctx, const int32_t* count, int m, int32_t* start, int3
   int nb = (int)nblocks(m, kScanBlock);
   hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(kScanThreads), 0, st, count, m, blocksum);
   hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, st, blocksum, nb);
-  hipLaunchKernelGGL(k_scan_final, dim3(nb), dim3(kScanThreads), 0, st, count, m, blocksum, start, cursor, (int64_t)-1);
+  hipLaunchKernelGGL(k_scan_final, dim3(nb), dim3(kScanThreads), 0, st, count, m, blocksum, start, (int64_t)-1);
   PFW_HIP(hipGetLastError());
 }
 
@@ -695,30 +699,29 @@ void nl_build(pfw_ctx* ctx) {
   }
   if (n > 0)
     hipLaunchKernelGGL(k_wrap, dim3(nblocks(n, 256)), dim3(256), 0, st, s.grids, s.atom_cfg, n, s.X, s.xw, s.u,
-                       s.cell_of, s.count, (unsigned long long*)s.bounds, (int)ctx->open_any);
+                       s.cell_of, s.count, s.rank, (unsigned long long*)s.bounds, (int)ctx->open_any);
   if (ctx->open_any) {
     hipLaunchKernelGGL(k_open_grid, dim3(nblocks(ctx->ncfg, 64)), dim3(64), 0, st, s.grids,
                        (const unsigned long long*)s.bounds, ctx->ncfg);
     if (n > 0)
       hipLaunchKernelGGL(k_bin_open, dim3(nblocks(n, 256)), dim3(256), 0, st, s.grids, s.atom_cfg, n, s.u,
-                         s.cell_of, s.count);
+                         s.cell_of, s.count, s.rank);
     PFW_HIP(hipStreamSynchronize(st));  // the bounds staging vector goes out of scope
   }
   PFW_HIP(hipGetLastError());
   record_stage(ctx, 1);
   if (one) {
-    hipLaunchKernelGGL(k_scan_one, dim3(1), dim3(kScanOneThreads), 0, st, s.count, ncells, s.start, s.cursor, n, 1);
+    hipLaunchKernelGGL(k_scan_one, dim3(1), dim3(kScanOneThreads), 0, st, s.count, ncells, s.start, n, 1);
   } else {
     int nb = (int)nblocks(ncells, kScanBlock);
     hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(kScanThreads), 0, st, s.count, ncells, s.blocksum);
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, st, s.blocksum, nb);
-    hipLaunchKernelGGL(k_scan_final, dim3(nb), dim3(kScanThreads), 0, st, s.count, ncells, s.blocksum, s.start,
-                       s.cursor, n);
+    hipLaunchKernelGGL(k_scan_final, dim3(nb), dim3(kScanThreads), 0, st, s.count, ncells, s.blocksum, s.start, n);
   }
   PFW_HIP(hipGetLastError());
   record_stage(ctx, 2);
   if (n > 0) {
-    hipLaunchKernelGGL(k_scatter, dim3(nblocks(n, 256)), dim3(256), 0, st, n, s.cell_of, s.start, s.cursor, s.perm);
+    hipLaunchKernelGGL(k_scatter, dim3(nblocks(n, 256)), dim3(256), 0, st, n, s.cell_of, s.start, s.rank, s.perm);
     hipLaunchKernelGGL(k_cell_sort_gather, dim3(ncells), dim3(kWave), 0, st, ncells, n, s.start, s.perm, s.xw,
                        s.species, s.xs, s.sps);
   }

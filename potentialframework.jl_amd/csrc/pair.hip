@@ -163,11 +163,7 @@ __global__ __launch_bounds__(64) void k_pair(const Grid* __restrict__ grids, con
             if (zero) e_zero = lane;
           }
         }
-        int incl = cntE;
-        for (int off = 1; off < kWave; off <<= 1) {
-          int t = __shfl_up(incl, off, kWave);
-          if (lane >= off) incl += t;
-        }
+        const int incl = wave_incl_scan(cntE);
         e_pref[lane + 1] = incl;
         if (lane == 0) e_pref[0] = 0;
         const int total = __shfl(incl, kWave - 1, kWave);

@@ -138,6 +138,21 @@ __device__ inline T wave_sum(T v) {
   return v;
 }
 
+// Inclusive prefix sum over the 64 lanes of a wavefront with DPP moves (no LDS
+// round trips): Hillis-Steele within each row of 16 lanes (row_shr 1, 2, 4, 8;
+// lanes whose source lies before the row start add 0), then row 0's total into
+// rows 1 and 3 (row_bcast:15) and rows 0 + 1's total into rows 2 and 3
+// (row_bcast:31). Every lane of the wavefront must be active.
+__device__ inline int wave_incl_scan(int x) {
+  x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);  // row_shr:1
+  x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);  // row_shr:2
+  x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);  // row_shr:4
+  x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);  // row_shr:8
+  x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);  // row_bcast:15 into rows 1, 3
+  x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);  // row_bcast:31 into rows 2, 3
+  return x;
+}
+
 // Stage 1 of a deterministic two-stage FP64 column reduction: block b sums
 // columns of records [b*chunk, (b+1)*chunk) of an (m x width) row-major table
 // into out[b*width + c] (width <= 16, blockDim 256).

@@ -76,7 +76,7 @@ struct SystemBuffers {
   int32_t* cell_of = nullptr;  // n
   int32_t* count = nullptr;    // ncells
   int32_t* start = nullptr;    // ncells + 1
-  int32_t* cursor = nullptr;   // ncells
+  int32_t* rank = nullptr;     // n: place of each atom among its cell's atoms (count-sort pass)
   int32_t* blocksum = nullptr;  // scan scratch
   int32_t* perm = nullptr;     // sorted -> original
   double* xs = nullptr;        // sorted wrapped coordinates SoA [3][n]
@@ -116,7 +116,6 @@ struct VerletDev {
   int32_t* ecfg0 = nullptr;     // ncfg + 1 extended-cell offsets
   int32_t* gcount = nullptr;    // ext_cells: ghosts of each extended cell
   int32_t* gstart = nullptr;    // ext_cells + 1
-  int32_t* gcursor = nullptr;   // scan scratch
   int32_t* gblock = nullptr;
   int32_t* nbr = nullptr;       // ecap x n, column-major (slot-major): entry index into xe
   int32_t* ncount = nullptr;    // n: list length of each sorted atom
@@ -267,8 +266,7 @@ void nl_build(pfw_ctx* ctx);
 void nl_export_pairs(pfw_ctx* ctx, int32_t* i, int32_t* j, int32_t* S, double* R, int64_t cap,
                      int64_t* count);
 void nl_free(pfw_ctx* ctx);
-void scan_counts(pfw_ctx* ctx, const int32_t* count, int m, int32_t* start, int32_t* blocksum,
-                 int32_t* cursor);
+void scan_counts(pfw_ctx* ctx, const int32_t* count, int m, int32_t* start, int32_t* blocksum);
 // pair.hip
 void pair_run(pfw_ctx* ctx, uint32_t mask, int precision);
 void pair_finish(pfw_ctx* ctx, uint32_t mask);

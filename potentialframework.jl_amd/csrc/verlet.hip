@@ -190,11 +190,7 @@ __global__ __launch_bounds__(64) void k_verlet_build(const Grid* __restrict__ gr
           if (zero) s_zero = lane;
         }
       }
-      int incl = c;
-      for (int off = 1; off < kWave; off <<= 1) {
-        const int t = __shfl_up(incl, off, kWave);
-        if (lane >= off) incl += t;
-      }
+      const int incl = wave_incl_scan(c);
       s_pref[lane + 1] = incl;
       if (lane == 0) s_pref[0] = 0;
       const int total = __shfl(incl, kWave - 1, kWave);
@@ -427,7 +423,6 @@ void verlet_free(pfw_ctx* ctx) {
   dfree(V.ecfg0);
   dfree(V.gcount);
   dfree(V.gstart);
-  dfree(V.gcursor);
   dfree(V.gblock);
   dfree(V.nbr);
   dfree(V.ncount);
@@ -463,7 +458,6 @@ static void verlet_setup(pfw_ctx* ctx) {
   dalloc(V.ecfg0, ctx->ncfg + 1);
   dalloc(V.gcount, ec);
   dalloc(V.gstart, ec + 1);
-  dalloc(V.gcursor, ec);
   dalloc(V.gblock, (ec + 1023) / 1024 + 1);
   dalloc(V.ncount, n);
   dalloc(V.Xref, 3 * n);
@@ -523,7 +517,7 @@ static void verlet_rebuild(pfw_ctx* ctx) {
   hipLaunchKernelGGL(k_ext_count, dim3(nblk(ec, 256)), dim3(256), 0, st, ec, V.ecfg0, ctx->ncfg, s.grids, s.start,
                      V.gcount);
   PFW_HIP(hipGetLastError());
-  scan_counts(ctx, V.gcount, (int)ec, V.gstart, V.gblock, V.gcursor);
+  scan_counts(ctx, V.gcount, (int)ec, V.gstart, V.gblock);
   int32_t ghosts = 0;
   PFW_HIP(hipMemcpyAsync(&ghosts, V.gstart + ec, sizeof(int32_t), hipMemcpyDeviceToHost, st));
   PFW_HIP(hipStreamSynchronize(st));
