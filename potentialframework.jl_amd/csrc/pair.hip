@@ -10,8 +10,9 @@
 // Layout: one 64-lane wavefront per cell. The neighbour cells' atoms are staged
 // in LDS as coordinates relative to the cell's first atom (computed in FP64,
 // stored in T). Lanes are (atom, slice) pairs: a cell of n_i atoms uses
-// S = 64 / n_i slices, each lane testing every S-th candidate. Hits are
-// compacted per lane with a 32-bit mask before the model body runs. Membership
+// S = min(4, 64 / n_i) slices, slice s testing one contiguous range of each
+// LDS chunk (so its candidate reads take immediate offsets). Hits are
+// compacted per lane into a 64-bit mask before the model body runs. Membership
 // inside a thin band around the cutoff is re-decided with the exact FP64
 // formula of the oracle, so the pair set is bit-exact in both precisions.
 #include <algorithm>
@@ -21,8 +22,18 @@
 
 namespace pfw {
 
-constexpr int kCand = 256;   // candidates per LDS chunk
-constexpr int kEnt = 64;     // stencil entries per entry chunk
+constexpr int kEnt = 32;     // stencil entries per entry chunk (the 27 of radius 1 in one)
+constexpr int kSlices = 4;   // most lanes per atom
+// Candidates per LDS chunk. Slice s takes the chunk's candidates s, s + S,
+// s + 2 S, ... (interleaved, so that every slice sees every stencil cell and
+// the slices' hit counts stay alike), stored contiguously: L entries, L a
+// multiple of 8 (the test unroll), starting L + 1 entries after the previous
+// slice so that the slices' concurrent reads fall in different LDS banks. The
+// slices' tails past the chunk hold far sentinels.
+template <typename T>
+constexpr int kCandOf = 256;
+template <typename T>
+constexpr int kCandPadOf = kCandOf<T> + 9 * kSlices;
 
 template <typename T>
 struct Band;
@@ -76,21 +87,25 @@ __device__ inline double prefilter_r2(const Cand<double>& c, double xi, double y
 }
 
 
+// Five wavefronts per SIMD for the FP32 kernel (at most 96 registers): the
+// candidate fills and LDS round trips of one wavefront hide behind the tests
+// of the others.
 template <typename T, int KIND>
-__global__ __launch_bounds__(64) void k_pair(const Grid* __restrict__ grids, const int32_t* __restrict__ cell_cfg,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 5 : 4))) void k_pair(const Grid* __restrict__ grids, const int32_t* __restrict__ cell_cfg,
                                              int64_t n, const int32_t* __restrict__ start,
                                              const double* __restrict__ xs, const int32_t* __restrict__ sps,
                                              const int32_t* __restrict__ perm, const double* __restrict__ params,
                                              int nz, LJConsts lj, uint32_t mask, double* __restrict__ F,
                                              double* __restrict__ partials, int ncells) {
-  // candidates as (x, y, z, (1 - slack) |x|^2) ([kCand]: a far sentinel); after
-  // the candidate loops of an atom group the same bytes hold the per-slice
-  // force sums
-  constexpr size_t kCoordBytes = (kCand + 1) * sizeof(Cand<T>), kRedBytes = kWave * 3 * sizeof(double);
+  // candidates as (x, y, z, (1 - slack) |x|^2), then far sentinels up to the end
+  // of the last slice's range; after the candidate loops of an atom group the
+  // same bytes hold the per-slice force sums
+  constexpr int kCand = kCandOf<T>;
+  constexpr size_t kCoordBytes = kCandPadOf<T> * sizeof(Cand<T>), kRedBytes = kWave * 3 * sizeof(double);
   __shared__ __align__(16) unsigned char cbuf[kCoordBytes > kRedBytes ? kCoordBytes : kRedBytes];
   Cand<T>* cand = (Cand<T>*)cbuf;
   double(*red)[3] = (double(*)[3])cbuf;
-  __shared__ int cm[kCand];  // (stencil entry within the entry chunk) << 8 | species
+  __shared__ int cm[kCandPadOf<T>];  // per candidate position: (stencil entry of the chunk) << 8 | species
   __shared__ int e_pref[kEnt + 1], e_start[kEnt], e_lat[kEnt][3];
   __shared__ double e_sh[kEnt][3];
   extern __shared__ __align__(16) unsigned char dyn[];  // SmoothPair: nz x nz amplitudes, then -6 A / c^2
@@ -111,12 +126,11 @@ __global__ __launch_bounds__(64) void k_pair(const Grid* __restrict__ grids, con
       sA6[k] = (T)(-6.0 * a / g.c2);
     }
   }
-  if (lane == 0) {
-    if constexpr (sizeof(T) == 4)
-      cand[kCand] = Cand<T>{(T)0, (T)0, (T)0, (T)1e30};
-    else
-      cand[kCand] = Cand<T>{(T)1e30, (T)1e30, (T)1e30};
-  }
+  Cand<T> far;
+  if constexpr (sizeof(T) == 4)
+    far = Cand<T>{(T)0, (T)0, (T)0, (T)1e30};
+  else
+    far = Cand<T>{(T)1e30, (T)1e30, (T)1e30};
   const T c4e = (T)lj.c4e, c24e = (T)lj.c24e, sig2 = (T)lj.sig2, eshift = (T)lj.eshift;
   double eacc = 0;
   T w0 = 0, w1 = 0, w2 = 0, w3 = 0, w4 = 0, w5 = 0;
@@ -129,12 +143,13 @@ __global__ __launch_bounds__(64) void k_pair(const Grid* __restrict__ grids, con
     const double rx = xs[b], ry = xs[n + b], rz = xs[2 * n + b];
     for (int ib = 0; ib < ncnt; ib += kWave) {
       const int nb = min(kWave, ncnt - ib);
-      const int S = kWave / nb;
+      const int S = min(kSlices, kWave / nb);
       const int il = lane % nb, split = lane / nb;
       const bool on = split < S;
       const int is = b + ib + il;
-      double xid = xs[is], yid = xs[n + is], zid = xs[2 * n + is];
-      const T xi = (T)(xid - rx), yi = (T)(yid - ry), zi = (T)(zid - rz);
+      // the FP64 coordinates are read again in the rare exact-membership path
+      // (fewer live registers, more wavefronts per SIMD)
+      const T xi = (T)(xs[is] - rx), yi = (T)(xs[n + is] - ry), zi = (T)(xs[2 * n + is] - rz);
       // prefilter r^2 = |xi|^2 + |xj|^2 - 2 xi.xj, both squares shrunk by the
       // slack that bounds its rounding error, so it never misses a pair
       const T m2x = (T)-2 * xi, m2y = (T)-2 * yi, m2z = (T)-2 * zi;
@@ -146,7 +161,7 @@ __global__ __launch_bounds__(64) void k_pair(const Grid* __restrict__ grids, con
         const int e = e0 + lane;
         if (lane == 0) e_zero = -1;
         __syncthreads();
-        if (e < g.nstencil) {
+        if (lane < kEnt && e < g.nstencil) {
           int lat[3];
           bool zero;
           int nc = stencil_entry(g, key, e, lat, &zero);
@@ -163,17 +178,22 @@ __global__ __launch_bounds__(64) void k_pair(const Grid* __restrict__ grids, con
             if (zero) e_zero = lane;
           }
         }
-        const int incl = wave_incl_scan(cntE);
-        e_pref[lane + 1] = incl;
+        const int incl = wave_incl_scan(cntE);  // lanes past kEnt add nothing
+        if (lane < kEnt) e_pref[lane + 1] = incl;
         if (lane == 0) e_pref[0] = 0;
         const int total = __shfl(incl, kWave - 1, kWave);
         __syncthreads();
         const int ez = e_zero;
+        const int ne = min(kEnt, g.nstencil - e0);  // entries of this chunk
         for (int t0 = 0; t0 < total; t0 += kCand) {
           const int tn = min(kCand, total - t0);
+          // slice s at positions [s (L + 1), s (L + 1) + L)
+          const int L = ((tn + S - 1) / S + 7) / 8 * 8;
+          const int stride = L + 1;
+          const uint32_t invS = (65536u + S - 1) / S;  // t / S = (t invS) >> 16 for t < 4096
           for (int t = lane; t < tn; t += kWave) {
             const int tt = t0 + t;
-            int lo = 0, hi = kEnt;
+            int lo = 0, hi = ne;
             while (hi - lo > 1) {
               int mid = (lo + hi) >> 1;
               if (e_pref[mid] <= tt)
@@ -185,56 +205,64 @@ __global__ __launch_bounds__(64) void k_pair(const Grid* __restrict__ grids, con
             const T x = (T)((xs[m] + e_sh[lo][0]) - rx);
             const T y = (T)((xs[n + m] + e_sh[lo][1]) - ry);
             const T z = (T)((xs[2 * n + m] + e_sh[lo][2]) - rz);
+            const int q = (int)(((uint32_t)t * invS) >> 16);
+            const int pos = (t - q * S) * stride + q;
             if constexpr (sizeof(T) == 4)
-              cand[t] = Cand<T>{x, y, z, (x * x + y * y + z * z) * ((T)1 - DotSlack<T>::value)};
+              cand[pos] = Cand<T>{x, y, z, (x * x + y * y + z * z) * ((T)1 - DotSlack<T>::value)};
             else
-              cand[t] = Cand<T>{x, y, z};
-            cm[t] = (lo << 8) | sps[m];
+              cand[pos] = Cand<T>{x, y, z};
+            cm[pos] = (lo << 8) | sps[m];
           }
+          // the gaps and the slices' tails past the chunk hold far sentinels
+          for (int k = max(0, tn / S - 1) + lane; k <= L; k += kWave)
+            for (int sl = 0; sl < S; ++sl)
+              if (k == L || k * S + sl >= tn) cand[sl * stride + k] = far;
           __syncthreads();
           if (on) {
-            // 32 tests of candidates c0r + q S: bit 31 - q flags a hit; the bits are
-            // shifted in (h = 2 h + flag) so that no per-test mask or shift is needed
-            auto round = [&](int c0r) -> uint32_t {
+            // cnt (8 .. 32, a multiple of 8) tests of candidates p[0 .. cnt): bit
+            // 31 - q flags candidate q; the bits are shifted in (h = 2 h + flag) so
+            // that no per-test mask or shift is needed
+            auto round = [&](const Cand<T>* p, int cnt) -> uint32_t {
               uint32_t h = 0;
-              if (c0r + 31 * S < tn) {  // every candidate is in the chunk
-#pragma unroll 8
-                for (int q = 0; q < 32; ++q) {
-                  const int t = c0r + q * S;
-                  h = h + h + (prefilter_r2(cand[t], xi, yi, zi, m2x, m2y, m2z, xi2) < c2hi ? 1u : 0u);
-                }
-              } else {
-#pragma unroll 8
-                for (int q = 0; q < 32; ++q) {
-                  const int t = c0r + q * S;
-                  const int tt = t < tn ? t : kCand;  // past the chunk: the sentinel, never a hit
-                  h = h + h + (prefilter_r2(cand[tt], xi, yi, zi, m2x, m2y, m2z, xi2) < c2hi ? 1u : 0u);
-                }
+              for (int q0 = 0; q0 < cnt; q0 += 8) {
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                  h = h + h + (prefilter_r2(p[q0 + q], xi, yi, zi, m2x, m2y, m2z, xi2) < c2hi ? 1u : 0u);
               }
-              return h;
+              return h << (32 - cnt);
             };
-            // the lane's own atom among this chunk's candidates (the (i, i, 0) self pair)
+            // the lane's own atom among this chunk's candidates (the (i, i, 0) self
+            // pair): its place in this lane's slice, or -1
             const int tself = ez >= 0 ? e_pref[ez] + (is - e_start[ez]) - t0 : -1;
-            for (int c0 = split; c0 < tn; c0 += S * 64) {
-              // two rounds, then their hits: bit b of `hits` flags candidate c0 + (63 - b) S
-              uint64_t hits = (uint64_t)round(c0) << 32;
-              if (c0 + 32 * S < tn) hits |= round(c0 + 32 * S);
+            int kself = -1;
+            if (tself >= 0 && tself < tn) {
+              const int q = (int)(((uint32_t)tself * invS) >> 16);
+              if (tself - q * S == split) kself = q;
+            }
+            const Cand<T>* mine = cand + split * stride;
+            const int* cmine = cm + split * stride;
+            for (int r0 = 0; r0 < L; r0 += 64) {
+              // two rounds, then their hits: bit b of `hits` flags the slice's candidate r0 + 63 - b
+              const int n1 = min(32, L - r0), n2 = min(32, L - r0 - 32);
+              uint64_t hits = (uint64_t)round(mine + r0, n1) << 32;
+              if (n2 > 0) hits |= round(mine + r0 + 32, n2);
+              if (kself >= r0 && kself < r0 + 64) hits &= ~(1ull << (63 - (kself - r0)));
               // this pair of rounds' sums (at most 64 pairs) in T, then added to the FP64 totals
               T er = 0, fxr = 0, fyr = 0, fzr = 0;
               while (hits) {
-                const int t = c0 + (63 - __builtin_ctzll(hits)) * S;
+                const int k = r0 + 63 - __builtin_ctzll(hits);  // within the slice
                 hits &= hits - 1;
-                if (t == tself) continue;
-                const Cand<T> c = cand[t];
+                const Cand<T> c = mine[k];
                 const T dx = c.x - xi, dy = c.y - yi, dz = c.z - zi;
                 const T r2 = dx * dx + dy * dy + dz * dz;
-                const int meta = cm[t], el = meta >> 8, zj = meta & 255;
+                const int meta = cmine[k], el = meta >> 8, zj = meta & 255;
                 if (!(r2 < c2lo)) {
+                  const int t = k * S + split;  // within the chunk
                   const int m = e_start[el] + (t0 + t - e_pref[el]);
                   double d[3];
                   int img[3];
-                  if (!exact_member(g, xid, yid, zid, xs[m], xs[n + m], xs[2 * n + m], e_lat[el], e_sh[el], d,
-                                    img))
+                  if (!exact_member(g, xs[is], xs[n + is], xs[2 * n + is], xs[m], xs[n + m], xs[2 * n + m], e_lat[el],
+                                    e_sh[el], d, img))
                     continue;
                 }
                 T e, coef;

@@ -103,15 +103,36 @@ __device__ inline int cell_index(const Grid& g, const int* k) {
 // outside an open axis), integer lattice shift and whether it is the zero
 // offset. Periodic axes wrap with shift fld(k+o, dims) (storage.jl:265-283).
 __device__ inline int stencil_entry(const Grid& g, const int* k, int e, int* lat, bool* zero) {
-  int w0 = 2 * g.radius[0] + 1, w1 = 2 * g.radius[1] + 1;
-  int o[3] = {e % w0 - g.radius[0], (e / w0) % w1 - g.radius[1], e / (w0 * w1) - g.radius[2]};
+  int o[3];
+  if (g.radius[0] == 1 && g.radius[1] == 1 && g.radius[2] == 1) {  // 3 x 3 x 3: constant divisors
+    o[0] = e % 3 - 1;
+    o[1] = (e / 3) % 3 - 1;
+    o[2] = e / 9 - 1;
+  } else {
+    const int w0 = 2 * g.radius[0] + 1, w1 = 2 * g.radius[1] + 1;
+    o[0] = e % w0 - g.radius[0];
+    o[1] = (e / w0) % w1 - g.radius[1];
+    o[2] = e / (w0 * w1) - g.radius[2];
+  }
   int nk[3];
   *zero = (o[0] == 0 && o[1] == 0 && o[2] == 0);
   for (int a = 0; a < 3; ++a) {
     int v = k[a] + o[a];
     if (g.pb[a]) {
-      nk[a] = jl_imod(v, g.dims[a]);
-      lat[a] = jl_fld(v, g.dims[a]);
+      const int d = g.dims[a];
+      if (v >= 0 && v < d) {  // fld(v, d) = 0 without a division
+        nk[a] = v;
+        lat[a] = 0;
+      } else if (v < 0 && v >= -d) {
+        nk[a] = v + d;
+        lat[a] = -1;
+      } else if (v >= d && v < 2 * d) {
+        nk[a] = v - d;
+        lat[a] = 1;
+      } else {
+        nk[a] = jl_imod(v, d);
+        lat[a] = jl_fld(v, d);
+      }
     } else {
       if (v < 0 || v >= g.used[a]) return -1;
       nk[a] = v;
