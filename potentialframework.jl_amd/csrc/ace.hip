@@ -1634,9 +1634,7 @@ static void ace_evaluate_rows(pfw_ctx* ctx, int grads, int precision) {
                        (const double*)ad.eg, s.perm, ctx->d_F);
   hipLaunchKernelGGL(k_ace_pack, dim3(nblk(n, 256)), dim3(256), 0, st, n, ad.siteE, ad.siteW, grads, ad.rec);
   if (ctx->ncfg == 1) {
-    hipLaunchKernelGGL(k_colsum, dim3(kRedBlocks), dim3(256), 0, st, ad.rec, n, 10, ctx->d_red);
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(1024), 0, st, ctx->d_red, kRedBlocks, 10, 1, (const int32_t*)ad.status,
-                       ctx->d_out, ctx->d_cfg_out);
+    reduce_outputs(ctx, ad.rec, n, 10, 1, ad.status);
   } else {
     batch_reduce(ctx, ad.rec, 10, false, 1);
   }
@@ -1776,7 +1774,7 @@ static void ace_skin_step(pfw_ctx* ctx, int grads, int precision) {
     PFW_HIP(hipGetLastError());
     record_stage(ctx, 4);
     ace_evaluate_rows(ctx, grads, precision);
-    PFW_HIP(hipMemcpyAsync(ctx->h_out, ctx->d_out, 11 * sizeof(double), hipMemcpyDeviceToHost, st));
+    copy_outputs(ctx);
     PFW_HIP(hipMemcpyAsync(ad.h_maxd2, ad.maxd2 + ad.parity, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
     record_stage(ctx, 6);
     PFW_HIP(hipStreamSynchronize(st));
@@ -1792,6 +1790,7 @@ static void ace_skin_step(pfw_ctx* ctx, int grads, int precision) {
 }
 
 void ace_run(pfw_ctx* ctx, uint32_t mask, int precision) {
+  ctx->host_written = false;
   AceDev& ad = ctx->ace;
   if (!ad.ready) throw Error(PFW_STALE, "no ACE model set");
   if (ad.cutoff != ctx->cutoff)
@@ -1816,7 +1815,7 @@ void ace_run(pfw_ctx* ctx, uint32_t mask, int precision) {
       record_stage(ctx, 4);
       ace_evaluate_rows(ctx, grads, precision);
     }
-    PFW_HIP(hipMemcpyAsync(ctx->h_out, ctx->d_out, 11 * sizeof(double), hipMemcpyDeviceToHost, st));
+    copy_outputs(ctx);
     record_stage(ctx, 6);
     PFW_HIP(hipStreamSynchronize(st));
     if (n > 0 && mask != 0 && ace_overflowed(ad)) {  // an environment outgrew its row: once more, wider

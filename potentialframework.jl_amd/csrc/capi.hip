@@ -75,7 +75,8 @@ int pfw_ctx_create(int device, pfw_ctx** out) {
     PFW_HIP(hipDeviceGetAttribute(&ctx->num_cu, hipDeviceAttributeMultiprocessorCount, device));
     PFW_HIP(hipMalloc(&ctx->d_out, 16 * sizeof(double)));
     PFW_HIP(hipMalloc(&ctx->d_red, 64 * 16 * sizeof(double)));
-    PFW_HIP(hipHostMalloc(&ctx->h_out, 16 * sizeof(double), hipHostMallocDefault));
+    PFW_HIP(hipHostMalloc(&ctx->h_out, 16 * sizeof(double), hipHostMallocMapped | hipHostMallocCoherent));
+    PFW_HIP(hipHostGetDevicePointer((void**)&ctx->h_out_dev, ctx->h_out, 0));
     PFW_HIP(hipMalloc(&ctx->d_flag, sizeof(int32_t)));
     PFW_HIP(hipHostMalloc(&ctx->h_flag, sizeof(int32_t), hipHostMallocDefault));
     for (auto& e : ctx->ev) PFW_HIP(hipEventCreate(&e));

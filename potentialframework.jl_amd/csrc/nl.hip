@@ -323,16 +323,12 @@ __device__ inline int wave_rank_count(int32_t* count, int c) {
   return __shfl(old, leader, kWave) + before;
 }
 
-// Wrap every atom with its configuration's basis; when no configuration has
-// an open axis also bin and count in the same pass.
-__global__ void k_wrap(const Grid* __restrict__ grids, const int32_t* __restrict__ atom_cfg, int64_t n,
-                       const double* __restrict__ X, double* __restrict__ xw, double* __restrict__ u,
-                       int32_t* __restrict__ cell_of, int32_t* __restrict__ count, int32_t* __restrict__ rank,
-                       unsigned long long* __restrict__ bounds, int open_any) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int cfg = atom_cfg[i];
-  const Grid& g = grids[cfg];
+// Wrap atom i with its configuration's basis; when no configuration has an
+// open axis also bin and count it.
+__device__ inline void wrap_one(const Grid& g, int cfg, int64_t i, int64_t n, const double* __restrict__ X,
+                                double* __restrict__ xw, double* __restrict__ u, int32_t* __restrict__ cell_of,
+                                int32_t* __restrict__ count, int32_t* __restrict__ rank,
+                                unsigned long long* __restrict__ bounds, int open_any) {
   double x[3] = {X[3 * i], X[3 * i + 1], X[3 * i + 2]}, w[3], uu[3];
   wrap_atom(g, x, w, uu);
   for (int a = 0; a < 3; ++a) xw[a * n + i] = w[a];
@@ -349,6 +345,22 @@ __global__ void k_wrap(const Grid* __restrict__ grids, const int32_t* __restrict
       atomicMin(&bounds[6 * cfg + 2 * a], ordered_bits(uu[a]));
       atomicMax(&bounds[6 * cfg + 2 * a + 1], ordered_bits(uu[a]));
     }
+  }
+}
+
+// single: one configuration, whose grid is then read with scalar loads (no
+// dependence on a per-atom load).
+__global__ void k_wrap(const Grid* __restrict__ grids, const int32_t* __restrict__ atom_cfg, int64_t n,
+                       const double* __restrict__ X, double* __restrict__ xw, double* __restrict__ u,
+                       int32_t* __restrict__ cell_of, int32_t* __restrict__ count, int32_t* __restrict__ rank,
+                       unsigned long long* __restrict__ bounds, int open_any, int single) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (single) {
+    wrap_one(grids[0], 0, i, n, X, xw, u, cell_of, count, rank, bounds, open_any);
+  } else {
+    const int cfg = atom_cfg[i];
+    wrap_one(grids[cfg], cfg, i, n, X, xw, u, cell_of, count, rank, bounds, open_any);
   }
 }
 
@@ -540,21 +552,23 @@ __global__ void k_scatter(int64_t n, const int32_t* __restrict__ cell_of, const 
 // Deterministic order inside each cell: ascending original index (the stable
 // sortperm of storage.jl:199), then the gather into sorted SoA coordinates.
 // One wavefront per cell: cells of up to 64 atoms are sorted with a register
-// bitonic network over lane shuffles, cells of up to kRankSort atoms by rank
-// counting over an LDS copy (indices are distinct), larger cells by an
-// insertion sort in lane 0 (dense clusters only).
-constexpr int kRankSort = 2048;
+// bitonic network over lane shuffles (of the smallest power of two that holds
+// the cell), cells of up to kRankSort atoms by rank counting over an LDS copy
+// (indices are distinct), larger cells (dense clusters only) by rank counting
+// over a copy in `scratch` (n entries, free at this point).
+constexpr int kRankSort = 256;
 __global__ __launch_bounds__(64) void k_cell_sort_gather(int ncells, int64_t n, const int32_t* __restrict__ start,
                                                          int32_t* __restrict__ perm, const double* __restrict__ xw,
                                                          const int32_t* __restrict__ species, double* __restrict__ xs,
-                                                         int32_t* __restrict__ sps) {
+                                                         int32_t* __restrict__ sps, int32_t* __restrict__ scratch) {
   const int c = blockIdx.x;
   const int lane = threadIdx.x;
   const int b = start[c], e = start[c + 1], cnt = e - b;
   if (cnt == 0) return;
   if (cnt <= kWave) {
     int v = lane < cnt ? perm[b + lane] : 0x7fffffff;
-    for (int k = 2; k <= kWave; k <<= 1)
+    const int size = cnt <= 16 ? 16 : (cnt <= 32 ? 32 : kWave);  // lanes [0, size) end ascending
+    for (int k = 2; k <= size; k <<= 1)
       for (int j = k >> 1; j > 0; j >>= 1) {
         const int o = __shfl_xor(v, j, kWave);
         const bool up = (lane & k) == 0;
@@ -574,16 +588,14 @@ __global__ __launch_bounds__(64) void k_cell_sort_gather(int ncells, int64_t n, 
     }
     __syncthreads();
   } else {
-    if (lane == 0)
-      for (int k = b + 1; k < e; ++k) {
-        int v = perm[k];
-        int m = k - 1;
-        while (m >= b && perm[m] > v) {
-          perm[m + 1] = perm[m];
-          --m;
-        }
-        perm[m + 1] = v;
-      }
+    for (int k = b + lane; k < e; k += kWave) scratch[k] = perm[k];
+    __syncthreads();  // one wavefront: orders its own global writes and reads
+    for (int k = b + lane; k < e; k += kWave) {
+      const int v = scratch[k];
+      int rank = 0;
+      for (int t = b; t < e; ++t) rank += scratch[t] < v;
+      perm[b + rank] = v;
+    }
     __syncthreads();
   }
   for (int k = b + lane; k < e; k += kWave) {
@@ -618,11 +630,12 @@ __global__ void k_colsum(const double* __restrict__ in, int64_t m, int width, do
 // Final stage of the energy / virial reduction of one configuration: the
 // stage-1 rows of k_colsum (nb x width) summed in fixed order, one lane per
 // column, then E, W (3x3 column-major) and the status word written to
-// out[0..10] and out2[0..10]. kind 0: pair records (E, xx yy zz xy xz yz of
+// out[0..10], out2[0..10] and, if given, out3[0..10] (host memory). kind 0: pair records (E, xx yy zz xy xz yz of
 // sum d (x) g over directed pairs), halved; kind 1: ACE records (E, W[9]) with
 // the status word of the environment build.
 __global__ void k_finish(const double* __restrict__ part, int nb, int width, int kind,
-                         const int32_t* __restrict__ status, double* __restrict__ out, double* __restrict__ out2) {
+                         const int32_t* __restrict__ status, double* __restrict__ out, double* __restrict__ out2,
+                         double* __restrict__ out3) {
   __shared__ double rows[kRedBlocks * 16];
   __shared__ double t[16];
   for (int i = threadIdx.x; i < nb * width; i += blockDim.x) rows[i] = part[i];  // all loads in flight at once
@@ -652,6 +665,28 @@ __global__ void k_finish(const double* __restrict__ part, int nb, int width, int
   out[0] = out2[0] = E;
   for (int k = 0; k < 9; ++k) out[1 + k] = out2[1 + k] = W[k];
   out[10] = out2[10] = bad ? 1.0 : 0.0;
+  if (out3) {
+    out3[0] = E;
+    for (int k = 0; k < 9; ++k) out3[1 + k] = W[k];
+    out3[10] = bad ? 1.0 : 0.0;
+  }
+}
+
+// E, W and the status word of one configuration: k_colsum, then k_finish.
+// (A single launch whose last block finishes the sum needs a device-scope
+// fence in every block, which writes back the L2 and was measured slower.)
+void reduce_outputs(pfw_ctx* ctx, const double* rec, int64_t m, int width, int kind, const int32_t* status) {
+  hipLaunchKernelGGL(k_colsum, dim3(kRedBlocks), dim3(256), 0, ctx->stream, rec, m, width, ctx->d_red);
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(1024), 0, ctx->stream, ctx->d_red, kRedBlocks, width, kind, status,
+                     ctx->d_out, ctx->d_cfg_out, ctx->h_out_dev);
+  PFW_HIP(hipGetLastError());
+  ctx->host_written = true;
+}
+
+void copy_outputs(pfw_ctx* ctx) {
+  if (!ctx->host_written)
+    PFW_HIP(hipMemcpyAsync(ctx->h_out, ctx->d_out, 11 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  ctx->host_written = false;
 }
 
 // Column sums of a (nblocks x width) table of per-workgroup partials, in
@@ -699,7 +734,8 @@ void nl_build(pfw_ctx* ctx) {
   }
   if (n > 0)
     hipLaunchKernelGGL(k_wrap, dim3(nblocks(n, 256)), dim3(256), 0, st, s.grids, s.atom_cfg, n, s.X, s.xw, s.u,
-                       s.cell_of, s.count, s.rank, (unsigned long long*)s.bounds, (int)ctx->open_any);
+                       s.cell_of, s.count, s.rank, (unsigned long long*)s.bounds, (int)ctx->open_any,
+                       (int)(ctx->ncfg == 1));
   if (ctx->open_any) {
     hipLaunchKernelGGL(k_open_grid, dim3(nblocks(ctx->ncfg, 64)), dim3(64), 0, st, s.grids,
                        (const unsigned long long*)s.bounds, ctx->ncfg);
@@ -723,7 +759,7 @@ void nl_build(pfw_ctx* ctx) {
   if (n > 0) {
     hipLaunchKernelGGL(k_scatter, dim3(nblocks(n, 256)), dim3(256), 0, st, n, s.cell_of, s.start, s.rank, s.perm);
     hipLaunchKernelGGL(k_cell_sort_gather, dim3(ncells), dim3(kWave), 0, st, ncells, n, s.start, s.perm, s.xw,
-                       s.species, s.xs, s.sps);
+                       s.species, s.xs, s.sps, s.rank);
   }
   PFW_HIP(hipGetLastError());
 }
@@ -793,6 +829,7 @@ __global__ void k_total(const double* __restrict__ part, int nb, double* __restr
 // records: (cells ? ncells : n) x width; kind 0 pair, 1 ACE. Writes per-configuration
 // results to d_cfg_out and the batch totals (status = number of failed configurations) to d_out.
 void batch_reduce(pfw_ctx* ctx, const double* records, int width, bool cells, int kind) {
+  ctx->host_written = false;
   hipStream_t st = ctx->stream;
   const int ncfg = ctx->ncfg;
   hipLaunchKernelGGL(k_segsum, dim3(ncfg), dim3(256), 0, st, records, width, ctx->sys.cfg_cell0,

@@ -359,6 +359,7 @@ static void launch_pair(pfw_ctx* ctx, uint32_t mask) {
 }
 
 void pair_run(pfw_ctx* ctx, uint32_t mask, int precision) {
+  ctx->host_written = false;
   if (ctx->pair.kind < 0) throw Error(PFW_STALE, "no pair model set");
   if (ctx->pair.cutoff != ctx->cutoff)
     throw Error(PFW_STALE, "pair model cutoff differs from the system's cell-list cutoff; set the system again");
@@ -389,10 +390,7 @@ void pair_run(pfw_ctx* ctx, uint32_t mask, int precision) {
     PFW_HIP(hipGetLastError());
     record_stage(ctx, 5);
     if (ctx->ncfg == 1) {
-      hipLaunchKernelGGL(k_colsum, dim3(kRedBlocks), dim3(256), 0, st, ctx->d_partials, (int64_t)ctx->ncells, 7,
-                         ctx->d_red);
-      hipLaunchKernelGGL(k_finish, dim3(1), dim3(1024), 0, st, ctx->d_red, kRedBlocks, 7, 0, (const int32_t*)nullptr,
-                         ctx->d_out, ctx->d_cfg_out);
+      reduce_outputs(ctx, ctx->d_partials, ctx->ncells, 7, 0, nullptr);
     } else {
       batch_reduce(ctx, ctx->d_partials, 7, true, 0);
     }
@@ -403,7 +401,7 @@ void pair_run(pfw_ctx* ctx, uint32_t mask, int precision) {
     PFW_HIP(hipMemsetAsync(ctx->d_cfg_out, 0, 11 * (size_t)ctx->ncfg * sizeof(double), st));
     if (ctx->sys.n > 0) PFW_HIP(hipMemsetAsync(ctx->d_F, 0, 3 * ctx->sys.n * sizeof(double), st));
   }
-  PFW_HIP(hipMemcpyAsync(ctx->h_out, ctx->d_out, 11 * sizeof(double), hipMemcpyDeviceToHost, st));
+  copy_outputs(ctx);
   record_stage(ctx, 6);
   PFW_HIP(hipStreamSynchronize(st));
   pair_finish(ctx, mask);

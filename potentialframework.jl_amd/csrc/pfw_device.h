@@ -181,6 +181,7 @@ constexpr int kRedBlocks = 64;
 __global__ void k_colsum(const double* __restrict__ in, int64_t m, int width, double* __restrict__ out);
 __global__ void k_sum_partials(const double* __restrict__ partial, int nblocks, int width, double* __restrict__ out);
 __global__ void k_finish(const double* __restrict__ part, int nb, int width, int kind,
-                         const int32_t* __restrict__ status, double* __restrict__ out, double* __restrict__ out2);
+                         const int32_t* __restrict__ status, double* __restrict__ out, double* __restrict__ out2,
+                         double* __restrict__ out3);
 
 }  // namespace pfw

@@ -250,6 +250,8 @@ struct pfw_ctx {
   int32_t* h_flag = nullptr;     // pinned copy
   double* d_out = nullptr;       // E, W[9], status
   double* h_out = nullptr;       // pinned copy of d_out
+  double* h_out_dev = nullptr;   // device address of h_out: the finish kernel writes it directly
+  bool host_written = false;     // h_out already holds this step's outputs (no copy needed)
   uint32_t last_mask = 0;
   bool have_results = false;
   // timing
@@ -267,6 +269,11 @@ void nl_export_pairs(pfw_ctx* ctx, int32_t* i, int32_t* j, int32_t* S, double* R
                      int64_t* count);
 void nl_free(pfw_ctx* ctx);
 void scan_counts(pfw_ctx* ctx, const int32_t* count, int m, int32_t* start, int32_t* blocksum);
+// E, W and the status word of one configuration from m records of `width`
+// columns (kind 0: pair records, kind 1: ACE records) into d_out / d_cfg_out.
+void reduce_outputs(pfw_ctx* ctx, const double* rec, int64_t m, int width, int kind, const int32_t* status);
+// d_out's 11 words into h_out on the stream, unless the finish kernel wrote them there.
+void copy_outputs(pfw_ctx* ctx);
 // pair.hip
 void pair_run(pfw_ctx* ctx, uint32_t mask, int precision);
 void pair_finish(pfw_ctx* ctx, uint32_t mask);

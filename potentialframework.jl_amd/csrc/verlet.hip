@@ -571,6 +571,7 @@ static void launch_list(pfw_ctx* ctx, uint32_t mask, bool per_atom) {
 // rebuilt and the step repeated (the reference instead reads the displacement
 // before evaluating, gpu.jl:516-520).
 void verlet_pair_run(pfw_ctx* ctx, uint32_t mask, int precision) {
+  ctx->host_written = false;
   VerletDev& V = ctx->verlet;
   SystemBuffers& s = ctx->sys;
   hipStream_t st = ctx->stream;
@@ -605,13 +606,10 @@ void verlet_pair_run(pfw_ctx* ctx, uint32_t mask, int precision) {
     if (per_atom) {
       batch_reduce(ctx, V.part, 7, false, 0);
     } else {
-      hipLaunchKernelGGL(k_colsum, dim3(kRedBlocks), dim3(256), 0, st, V.part, (int64_t)nblk(n, kWave / kListParts), 7,
-                         ctx->d_red);
-      hipLaunchKernelGGL(k_finish, dim3(1), dim3(1024), 0, st, ctx->d_red, kRedBlocks, 7, 0, (const int32_t*)nullptr,
-                         ctx->d_out, ctx->d_cfg_out);
+      reduce_outputs(ctx, V.part, (int64_t)nblk(n, kWave / kListParts), 7, 0, nullptr);
     }
     PFW_HIP(hipGetLastError());
-    PFW_HIP(hipMemcpyAsync(ctx->h_out, ctx->d_out, 11 * sizeof(double), hipMemcpyDeviceToHost, st));
+    copy_outputs(ctx);
     PFW_HIP(hipMemcpyAsync(V.h_maxd2, V.maxd2 + V.parity, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
     record_stage(ctx, 6);
     PFW_HIP(hipStreamSynchronize(st));

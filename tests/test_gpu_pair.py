@@ -47,7 +47,7 @@ def systems(pfw):
         X, sp = wl.image_test_system(H)
         for pbc in ((1, 1, 1), (1, 0, 1), (0, 1, 0)):
             out.append((f"cell{k}-{pbc}", X, H, pbc, sp))
-    for n in (0, 1, 2, 7, 65, 257):
+    for n in (0, 1, 2, 7, 65, 257, 700):  # 700: a cell of ~350 atoms (the large-cell sort)
         for pbc in ((0, 0, 0), (1, 0, 1), (1, 1, 1)):
             X, H, _, sp = wl.fixture(n, pbc=pbc, cluster=True)
             out.append((f"fixture{n}-{pbc}", X, H, pbc, sp))
