@@ -197,6 +197,20 @@ def bench_fitting_set(pfw, lib, _capi, model, ps, pair_model, args):
     }
     out["pair_adjoint"] = {"precision": "float64", "adjoint_ms": round(1e3 * pair_adj, 3),
                            "adjoint_value": round(natoms / pair_adj / 1e6, 3)}
+    # the linear design (energy, force and virial rows per readout coefficient)
+    # of the first 64 configurations, one batch on the device
+    sub = sys_ace[:64]
+    design = {"configurations": len(sub), "atoms": sum(len(x) for x in sub)}
+    for prec in ("mixed", "float64"):
+        dcalc = pfw.ACECalculator(model, ps, precision=prec)
+        pfw.design_matrix(dcalc, sub)
+        t0 = time.perf_counter()
+        d = pfw.design_matrix(dcalc, sub)
+        secs = time.perf_counter() - t0
+        design["rows"], design["columns"] = d["matrix"].shape
+        design[prec] = {"ms": round(1e3 * secs, 2), "ms_per_column": round(1e3 * secs / d["matrix"].shape[1], 4)}
+        del d
+    out["design_matrix"] = design
     return out
 
 

@@ -232,6 +232,21 @@ int pfw_ace_adjoint(pfw_ctx* ctx, int precision, const double* dE, const double*
  * energy_parameter_gradient(::SmoothPair) (src/submodules/PairPotentials/parameters.jl:126-138). */
 int pfw_pair_adjoint(pfw_ctx* ctx, const double* dE, const double* dF, const double* dW, double* gA);
 
+/* ---- linear fitting design (design_matrix_impl, src/submodules/ACE/fitting.jl:18-63) ----
+ * The outputs of nsets ACE parameter sets over the current system or batch,
+ * on one environment build. Set k is WB + k nbasis nz (nbasis x nz,
+ * column-major) and Wpair + k nq nz (pair.nr x nz; may be NULL when the
+ * model has no pair functions). Column k of out (rows x nsets, column-major,
+ * rows = 10 ncfg + 3 N) holds, for every configuration in order, its energy,
+ * its forces vec(F) (3 x n, original order) and its virial vec(W) (3 x 3,
+ * column-major): the row layout of design_matrix_impl. The outputs are linear
+ * in the parameters, so unit sets give the design's columns plus the
+ * baseline (the zero set's column). The parameters set with
+ * pfw_set_ace_parameters stay in force; the last step's results are cleared.
+ * precision: PFW_FP64 or PFW_MIXED. */
+int pfw_ace_parameter_sets(pfw_ctx* ctx, int precision, int64_t nsets, const double* WB, const double* Wpair,
+                           double* out);
+
 #ifdef __cplusplus
 }
 #endif

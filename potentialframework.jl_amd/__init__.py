@@ -44,6 +44,7 @@ from .adjoints import (  # noqa: F401
     rrule_ace_evaluate,
     fitting_loss,
 )
+from .fitting import design_matrix, training_rows, linear_parameters, install_readout  # noqa: F401
 from .calculators import (  # noqa: F401
     potential_energy,
     forces,

@@ -570,8 +570,11 @@ __global__ __launch_bounds__(64) void k_ace_pull(AceArgs<T> a) {
 // them against each centre of the cell in turn; a ballot compacts the hits, so
 // every centre's edges come out sorted by (entry, neighbour) without atomics.
 // A centre's write cursor lives in the lane of the same index within its group
-// of 64 centres.
-__global__ __launch_bounds__(64) void k_ace_edges(const Grid* __restrict__ grids, const int32_t* __restrict__ cell_cfg,
+// of 64 centres. (Several wavefronts per cell, each staging the candidates
+// for a share of the centres, were measured twice as slow: the staging, not
+// the centre loop, is the larger part.)
+constexpr int kEdgeWaves = 1;
+__global__ __launch_bounds__(kWave * kEdgeWaves) void k_ace_edges(const Grid* __restrict__ grids, const int32_t* __restrict__ cell_cfg,
                                                   int64_t n, const double* __restrict__ xs,
                                                   const int32_t* __restrict__ start, int pass, int ecap,
                                                   int32_t* __restrict__ ecount, int32_t* __restrict__ enb,
@@ -580,9 +583,10 @@ __global__ __launch_bounds__(64) void k_ace_edges(const Grid* __restrict__ grids
   __shared__ int s_pref[kWave + 1], s_start[kWave], s_lat[kWave][3];
   __shared__ double s_sh[kWave][3];
   __shared__ float s_ax[kWave], s_ay[kWave], s_az[kWave];  // the centres, relative to the cell's first atom
+  __shared__ double s_cx[kWave], s_cy[kWave], s_cz[kWave];  // and exactly
   __shared__ int s_zero;
   const int cell = blockIdx.x;
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
   const int b = start[cell], cnt = start[cell + 1] - b;
   if (cnt == 0) return;
   const Grid& g = grids[cell_cfg[cell]];
@@ -600,7 +604,10 @@ __global__ __launch_bounds__(64) void k_ace_edges(const Grid* __restrict__ grids
     const int64_t mine = b + ag + lane;
     int pos = 0;  // the centre's next slot (lane of the same index)
     int deg = 0, cut = 0;
-    if (lane < na) {
+    if (wid == 0 && lane < na) {
+      s_cx[lane] = xs[mine];
+      s_cy[lane] = xs[n + mine];
+      s_cz[lane] = xs[2 * n + mine];
       s_ax[lane] = (float)(xs[mine] - rx);
       s_ay[lane] = (float)(xs[n + mine] - ry);
       s_az[lane] = (float)(xs[2 * n + mine] - rz);
@@ -608,9 +615,9 @@ __global__ __launch_bounds__(64) void k_ace_edges(const Grid* __restrict__ grids
     for (int e0 = 0; e0 < g.nstencil; e0 += kWave) {
       const int e = e0 + lane;
       int c = 0;
-      if (lane == 0) s_zero = -1;
+      if (threadIdx.x == 0) s_zero = -1;
       __syncthreads();
-      if (e < g.nstencil) {
+      if (wid == 0 && e < g.nstencil) {  // wavefront 0 stages the stencil entries
         int lat[3];
         bool zero;
         const int nc = stencil_entry(g, key, e, lat, &zero);
@@ -626,11 +633,13 @@ __global__ __launch_bounds__(64) void k_ace_edges(const Grid* __restrict__ grids
           if (zero) s_zero = lane;
         }
       }
-      const int incl = wave_incl_scan(c);
-      s_pref[lane + 1] = incl;
-      if (lane == 0) s_pref[0] = 0;
-      const int total = __shfl(incl, kWave - 1, kWave);
+      if (wid == 0) {
+        const int incl = wave_incl_scan(c);
+        s_pref[lane + 1] = incl;
+        if (lane == 0) s_pref[0] = 0;
+      }
       __syncthreads();
+      const int total = s_pref[kWave];
       const int ez = s_zero;
       for (int t0 = 0; t0 < total; t0 += kWave) {
         const int t = t0 + lane;
@@ -655,7 +664,7 @@ __global__ __launch_bounds__(64) void k_ace_edges(const Grid* __restrict__ grids
           fy = (float)((yj + s_sh[lo][1]) - ry);
           fz = (float)((zj + s_sh[lo][2]) - rz);
         }
-        for (int a = 0; a < na; ++a) {
+        for (int a = wid; a < na; a += kEdgeWaves) {
           const int64_t k = b + ag + a;
           bool hit = false;
           double d[3] = {0, 0, 0};
@@ -663,13 +672,13 @@ __global__ __launch_bounds__(64) void k_ace_edges(const Grid* __restrict__ grids
           if (ux * ux + uy * uy + uz * uz < c2hi && !(lo == ez && m == (int)k)) {
             if (skin_rows) {
 #pragma clang fp contract(off)
-              d[0] = (xj - xs[k]) + s_sh[lo][0];
-              d[1] = (yj - xs[n + k]) + s_sh[lo][1];
-              d[2] = (zj - xs[2 * n + k]) + s_sh[lo][2];
+              d[0] = (xj - s_cx[a]) + s_sh[lo][0];
+              d[1] = (yj - s_cy[a]) + s_sh[lo][1];
+              d[2] = (zj - s_cz[a]) + s_sh[lo][2];
               hit = d[0] * d[0] + d[1] * d[1] + d[2] * d[2] < r2max;
             } else {
               int img[3];
-              hit = exact_member(g, xs[k], xs[n + k], xs[2 * n + k], xj, yj, zj, s_lat[lo], s_sh[lo], d, img);
+              hit = exact_member(g, s_cx[a], s_cy[a], s_cz[a], xj, yj, zj, s_lat[lo], s_sh[lo], d, img);
             }
           }
           const uint64_t mask = __ballot(hit);
@@ -696,7 +705,7 @@ __global__ __launch_bounds__(64) void k_ace_edges(const Grid* __restrict__ grids
       }
       __syncthreads();
     }
-    if (lane < na) ecount[mine] = min(deg, ecap);
+    if (lane < na && lane % kEdgeWaves == wid) ecount[mine] = min(deg, ecap);
     // one atomic per wavefront and word (same-address atomics serialise across the device)
     int dmax = deg, cmax = cut;
     for (int off = kWave / 2; off > 0; off >>= 1) {
@@ -1269,31 +1278,39 @@ static std::vector<ProductSlot> schedule_products(const std::vector<std::vector<
   return out;
 }
 
-void ace_set_parameters(pfw_ctx* ctx, const double* WB, const double* Wpair) {
-  AceDev& a = ctx->ace;
-  if (!a.ready && a.nprod == 0) throw Error(PFW_STALE, "no ACE model set");
+// Device readout weights of (WB, Wpair): the precontracted product weights
+// w (nz x nprod), the same in product-schedule order ws (nz x slots) and the
+// pair weights wp (nz x nq).
+static void readout_weights(const AceDev& a, const double* WB, const double* Wpair, std::vector<double>& w,
+                            std::vector<double>& ws, std::vector<double>& wp) {
   const int nz = a.nz, nq = a.pair.nr;
-  std::vector<double> w((size_t)nz * a.nprod, 0.0);  // z-major: w[z * nprod + k]
+  w.assign((size_t)nz * a.nprod, 0.0);  // z-major: w[z * nprod + k]
   for (size_t t = 0; t < a.c_vals.size(); ++t)
     for (int z = 0; z < nz; ++z) {
       double b = WB[(size_t)a.c_rows[t] + (size_t)a.nbasis * z];
       if (!std::isfinite(b)) throw Error(PFW_INVALID_ARGUMENT, "nonfinite readout parameters");
       w[(size_t)z * a.nprod + a.c_cols[t]] += a.c_vals[t] * b;
     }
-  std::vector<double> wp((size_t)nz * std::max(nq, 1), 0.0);
+  wp.assign((size_t)nz * std::max(nq, 1), 0.0);
   for (int z = 0; z < nz; ++z)
     for (int q = 0; q < nq; ++q) {
       double v = Wpair ? Wpair[q + (size_t)nq * z] : 0.0;
       if (!std::isfinite(v)) throw Error(PFW_INVALID_ARGUMENT, "nonfinite pair parameters");
       wp[(size_t)z * nq + q] = v;
     }
-  std::vector<float> w32(w.begin(), w.end()), wp32(wp.begin(), wp.end());
   const size_t nslot = a.slot_prod.size();  // readout weights in schedule order
-  std::vector<double> ws((size_t)nz * std::max<size_t>(nslot, 1), 0.0);
+  ws.assign((size_t)nz * std::max<size_t>(nslot, 1), 0.0);
   for (int z = 0; z < nz; ++z)
     for (size_t q = 0; q < nslot; ++q)
       if (a.slot_prod[q] >= 0) ws[z * nslot + q] = w[(size_t)z * a.nprod + a.slot_prod[q]];
-  std::vector<float> ws32(ws.begin(), ws.end());
+}
+
+void ace_set_parameters(pfw_ctx* ctx, const double* WB, const double* Wpair) {
+  AceDev& a = ctx->ace;
+  if (!a.ready && a.nprod == 0) throw Error(PFW_STALE, "no ACE model set");
+  std::vector<double> w, ws, wp;
+  readout_weights(a, WB, Wpair, w, ws, wp);
+  std::vector<float> w32(w.begin(), w.end()), wp32(wp.begin(), wp.end()), ws32(ws.begin(), ws.end());
   dfree(a.ws64);
   dfree(a.ws32);
   a.ws64 = upload(ws, ctx->stream);
@@ -1585,7 +1602,7 @@ static void ace_environment(pfw_ctx* ctx, int rev) {
   ace_rows_alloc(ctx, std::max(ad.ecap, 1));
   if (ad.ecap == 0) {  // size the rows: one counting pass
     PFW_HIP(hipMemsetAsync(ad.status, 0, 4 * sizeof(int32_t), st));
-    hipLaunchKernelGGL(k_ace_edges, dim3((unsigned)ctx->ncells), dim3(kWave), 0, st, s.grids, s.cell_cfg, n, s.xs,
+    hipLaunchKernelGGL(k_ace_edges, dim3((unsigned)ctx->ncells), dim3(kWave * kEdgeWaves), 0, st, s.grids, s.cell_cfg, n, s.xs,
                        s.start, 0, 0, ad.ecount, (int32_t*)nullptr, (int32_t*)nullptr, (double*)nullptr, ad.status, 0);
     PFW_HIP(hipGetLastError());
     PFW_HIP(hipMemcpyAsync(ad.h_status, ad.status, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
@@ -1596,7 +1613,7 @@ static void ace_environment(pfw_ctx* ctx, int rev) {
   ace_rows_alloc(ctx, ad.ecap);
   const int64_t slots = n * (int64_t)ad.ecap;
   PFW_HIP(hipMemsetAsync(ad.status, 0, 4 * sizeof(int32_t), st));
-  hipLaunchKernelGGL(k_ace_edges, dim3((unsigned)ctx->ncells), dim3(kWave), 0, st, s.grids, s.cell_cfg, n, s.xs,
+  hipLaunchKernelGGL(k_ace_edges, dim3((unsigned)ctx->ncells), dim3(kWave * kEdgeWaves), 0, st, s.grids, s.cell_cfg, n, s.xs,
                      s.start, 1, ad.ecap, ad.ecount, ad.enb, ad.eent, ad.eR, ad.status, 0);
   PFW_HIP(hipGetLastError());
   if (rev)
@@ -1693,7 +1710,7 @@ static void ace_skin_rebuild(pfw_ctx* ctx) {
   }
   auto build = [&](int pass) {
     PFW_HIP(hipMemsetAsync(ad.status, 0, 4 * sizeof(int32_t), st));
-    hipLaunchKernelGGL(k_ace_edges, dim3((unsigned)ctx->ncells), dim3(kWave), 0, st, s.grids, s.cell_cfg, n, s.xs,
+    hipLaunchKernelGGL(k_ace_edges, dim3((unsigned)ctx->ncells), dim3(kWave * kEdgeWaves), 0, st, s.grids, s.cell_cfg, n, s.xs,
                        s.start, pass, pass ? ad.ecap_s : 0, ad.ecount_s, ad.enb_s, ad.eent_s, ad.eR_s, ad.status, 1);
     PFW_HIP(hipGetLastError());
     if (pass)
@@ -1839,6 +1856,146 @@ void ace_run(pfw_ctx* ctx, uint32_t mask, int precision) {
   ctx->last_mask = mask;
   ctx->have_results = true;
   if (ctx->h_out[10] != 0.0) throw Error(PFW_DOMAIN, "nonfinite ACE output (coincident atoms or singular envelope)");
+}
+
+// Force rows of one output column: atom i (original order) of configuration c
+// has its rows at 10 c + 1 + 3 i + axis (each configuration's block holds its
+// energy, 3 n_c force rows and 9 virial rows).
+__global__ void k_design_forces(int64_t n, const int32_t* __restrict__ atom_cfg, const double* __restrict__ F,
+                                double* __restrict__ col) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= 3 * n) return;
+  col[10 * (int64_t)atom_cfg[k / 3] + 1 + k] = F[k];
+}
+
+// Energy and virial rows of one output column from the per-configuration
+// rows (E, W[9], status) of the reductions; a nonfinite status sets `bad`.
+__global__ void k_design_scalars(int ncfg, const int64_t* __restrict__ atom0, const double* __restrict__ cfg_out,
+                                 double* __restrict__ col, int32_t* __restrict__ bad) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncfg) return;
+  const double* o = cfg_out + 11 * (int64_t)c;
+  col[10 * (int64_t)c + 3 * atom0[c]] = o[0];
+  for (int k = 0; k < 9; ++k) col[10 * (int64_t)c + 3 * atom0[c + 1] + 1 + k] = o[1 + k];
+  if (o[10] != 0.0) atomicOr(bad, 1);
+}
+
+// Device buffers owned by one call (freed on every exit path).
+struct DeviceScratch {
+  std::vector<void*> p;
+  template <typename T>
+  T* get(size_t count) {
+    void* q = nullptr;
+    PFW_HIP(hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T)));
+    p.push_back(q);
+    return (T*)q;
+  }
+  ~DeviceScratch() {
+    for (void* q : p) (void)hipFree(q);
+  }
+};
+
+// Outputs of nsets parameter sets on one environment build: set k's outputs
+// form column k of `out` (rows x nsets, column-major, rows = 10 ncfg + 3 n):
+// for every configuration in order its energy, vec(F) (3 x n_c, original atom
+// order) and vec(W) (3 x 3, column-major). E, F and W are linear in the
+// readout, so with unit parameter sets the columns, less the column of the
+// zero set, are the linear design of design_matrix_impl
+// (src/submodules/ACE/fitting.jl:18-63), and the zero set's column is its
+// baseline. Each set runs the site kernels, the force gather and the
+// reductions of a step; the parameters set before the call stay in force.
+void ace_parameter_sets(pfw_ctx* ctx, int precision, int64_t nsets, const double* WB, const double* Wpair,
+                        double* out) {
+  ctx->host_written = false;
+  AceDev& ad = ctx->ace;
+  if (!ad.ready) throw Error(PFW_STALE, "no ACE model set");
+  if (ad.cutoff != ctx->cutoff)
+    throw Error(PFW_STALE, "the system's cell list was built for another cutoff than the ACE search cutoff");
+  const int nz = ad.nz, nb = ad.nbasis, nq = ad.pair.nr;
+  if (nsets < 0) throw Error(PFW_INVALID_ARGUMENT, "negative number of parameter sets");
+  if (nsets == 0) return;
+  if (!WB || !out || (nq > 0 && !Wpair)) throw Error(PFW_INVALID_ARGUMENT, "null parameter sets or output");
+  SystemBuffers& s = ctx->sys;
+  hipStream_t st = ctx->stream;
+  const int64_t n = s.n, rows = 10 * (int64_t)ctx->ncfg + 3 * n;
+  if (rows > 0 && nsets > ((int64_t)1 << 40) / rows) throw Error(PFW_INVALID_ARGUMENT, "design output too large");
+  for (;;) {  // environments with the reverse index, as in a step
+    nl_build(ctx);
+    if (n == 0) break;
+    ace_environment(ctx, 1);
+    PFW_HIP(hipStreamSynchronize(st));
+    if (!ace_overflowed(ad)) break;
+  }
+  // every set's device weights at once; the kernels are pointed at one set after the other
+  std::vector<double> w, ws, wp, W, WS, WP;
+  for (int64_t k = 0; k < nsets; ++k) {
+    readout_weights(ad, WB + k * (int64_t)nb * nz, nq > 0 ? Wpair + k * (int64_t)nq * nz : nullptr, w, ws, wp);
+    W.insert(W.end(), w.begin(), w.end());
+    WS.insert(WS.end(), ws.begin(), ws.end());
+    WP.insert(WP.end(), wp.begin(), wp.end());
+  }
+  const size_t nw = w.size(), nws = ws.size(), nwp = wp.size();
+  std::vector<float> W32(W.begin(), W.end()), WS32(WS.begin(), WS.end()), WP32(WP.begin(), WP.end());
+  DeviceScratch dev;
+  double* W64d = dev.get<double>(W.size());
+  double* WS64d = dev.get<double>(WS.size());
+  double* WP64d = dev.get<double>(WP.size());
+  float* W32d = dev.get<float>(W32.size());
+  float* WS32d = dev.get<float>(WS32.size());
+  float* WP32d = dev.get<float>(WP32.size());
+  PFW_HIP(hipMemcpy(W64d, W.data(), W.size() * sizeof(double), hipMemcpyHostToDevice));
+  PFW_HIP(hipMemcpy(WS64d, WS.data(), WS.size() * sizeof(double), hipMemcpyHostToDevice));
+  PFW_HIP(hipMemcpy(WP64d, WP.data(), WP.size() * sizeof(double), hipMemcpyHostToDevice));
+  PFW_HIP(hipMemcpy(W32d, W32.data(), W32.size() * sizeof(float), hipMemcpyHostToDevice));
+  PFW_HIP(hipMemcpy(WS32d, WS32.data(), WS32.size() * sizeof(float), hipMemcpyHostToDevice));
+  PFW_HIP(hipMemcpy(WP32d, WP32.data(), WP32.size() * sizeof(float), hipMemcpyHostToDevice));
+  double* dout = dev.get<double>((size_t)rows * nsets);
+  int64_t* atom0 = dev.get<int64_t>(ctx->atom0.size());
+  int32_t* bad = dev.get<int32_t>(1);
+  PFW_HIP(hipMemcpy(atom0, ctx->atom0.data(), ctx->atom0.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+  PFW_HIP(hipMemset(bad, 0, sizeof(int32_t)));
+  double* keep64[3] = {ad.w64, ad.ws64, ad.wp64};
+  float* keep32[3] = {ad.w32, ad.ws32, ad.wp32};
+  auto restore = [&] {
+    ad.w64 = keep64[0];
+    ad.ws64 = keep64[1];
+    ad.wp64 = keep64[2];
+    ad.w32 = keep32[0];
+    ad.ws32 = keep32[1];
+    ad.wp32 = keep32[2];
+  };
+  try {
+    for (int64_t k = 0; k < nsets; ++k) {
+      ad.w64 = W64d + k * nw;
+      ad.ws64 = WS64d + k * nws;
+      ad.wp64 = WP64d + k * nwp;
+      ad.w32 = W32d + k * nw;
+      ad.ws32 = WS32d + k * nws;
+      ad.wp32 = WP32d + k * nwp;
+      double* col = dout + k * rows;
+      if (n > 0) {
+        ace_evaluate_rows(ctx, 1, precision);
+        hipLaunchKernelGGL(k_design_forces, dim3(nblk(3 * n, 256)), dim3(256), 0, st, n, s.atom_cfg, ctx->d_F, col);
+      } else {
+        PFW_HIP(hipMemsetAsync(ctx->d_cfg_out, 0, 11 * (size_t)ctx->ncfg * sizeof(double), st));
+      }
+      hipLaunchKernelGGL(k_design_scalars, dim3(nblk(ctx->ncfg, 64)), dim3(64), 0, st, ctx->ncfg, atom0,
+                         ctx->d_cfg_out, col, bad);
+      PFW_HIP(hipGetLastError());
+    }
+    PFW_HIP(hipStreamSynchronize(st));
+  } catch (...) {
+    restore();
+    ctx->host_written = false;
+    throw;
+  }
+  restore();
+  ctx->host_written = false;
+  ctx->have_results = false;  // d_out / d_F hold the last set's outputs, not a step's
+  int32_t h_bad = 0;
+  PFW_HIP(hipMemcpy(&h_bad, bad, sizeof(int32_t), hipMemcpyDeviceToHost));
+  if (h_bad) throw Error(PFW_DOMAIN, "nonfinite ACE output (coincident atoms or singular envelope)");
+  PFW_HIP(hipMemcpy(out, dout, (size_t)rows * nsets * sizeof(double), hipMemcpyDeviceToHost));
 }
 
 template <typename T, int L>

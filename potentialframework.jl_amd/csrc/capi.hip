@@ -398,6 +398,16 @@ int pfw_ace_adjoint(pfw_ctx* ctx, int precision, const double* dE, const double*
   });
 }
 
+int pfw_ace_parameter_sets(pfw_ctx* ctx, int precision, int64_t nsets, const double* WB, const double* Wpair,
+                           double* out) {
+  return run(ctx, [&] {
+    if (!ctx->have_system) throw Error(PFW_STALE, "no system set");
+    if (precision != PFW_FP64 && precision != PFW_MIXED) throw Error(PFW_INVALID_ARGUMENT, "unknown precision");
+    ctx->have_results = false;
+    pfw::ace_parameter_sets(ctx, precision, nsets, WB, Wpair, out);
+  });
+}
+
 int pfw_pair_adjoint(pfw_ctx* ctx, const double* dE, const double* dF, const double* dW, double* gA) {
   return run(ctx, [&] {
     if (!ctx->have_system) throw Error(PFW_STALE, "no system set");

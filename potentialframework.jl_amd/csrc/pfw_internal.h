@@ -289,6 +289,8 @@ void ace_run(pfw_ctx* ctx, uint32_t mask, int precision);
 void ace_free(pfw_ctx* ctx);
 void ace_free_edges(pfw_ctx* ctx);
 void ace_site_energies(pfw_ctx* ctx, double* out);
+void ace_parameter_sets(pfw_ctx* ctx, int precision, int64_t nsets, const double* WB, const double* Wpair,
+                        double* out);
 void ace_adjoint(pfw_ctx* ctx, int precision, const double* Ebar, const double* Fbar, const double* Wbar, double* gWB,
                  double* gWpair);
 }  // namespace pfw
