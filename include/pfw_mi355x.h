@@ -247,6 +247,39 @@ int pfw_pair_adjoint(pfw_ctx* ctx, const double* dE, const double* dF, const dou
 int pfw_ace_parameter_sets(pfw_ctx* ctx, int precision, int64_t nsets, const double* WB, const double* Wpair,
                            double* out);
 
+/* ---- equivariant features (ace_features, src/submodules/ACE/features.jl:141-178) ---- */
+enum { PFW_RADIAL_SMOOTH_EXPONENTIAL = 1 };
+/* Numerical tables of an ACEFeatureLayer (features.jl:9-19) and its
+ * parameters. Indices are 0-based. density: neighbour species channel
+ * dens_z (0 = every species, 1..nz = that species), radial dens_n and
+ * harmonic dens_h (l*l+l+m); products: factors prod_idx[prod_ptr[k] ..
+ * prod_ptr[k+1]); coupling: the rows of every block in turn ((2L+1) x nbasis
+ * rows per block, m fastest) as CSR (row_ptr, col = product, val); mixing:
+ * per block (nbasis x channels x nz), column-major, blocks in turn. Radials:
+ * the smooth exponential family (radials.jl:193-206), amplitude (r/rc)^p
+ * (1 - r/rc)^4 exp(-rate r) with rc = radial_cutoff and one power per radial,
+ * times scale (nr x nz x nz: radial, centre, neighbour species, column-major). */
+typedef struct {
+  int32_t nz, nr, lmax, ndens, nprod, nblocks;
+  const int32_t *dens_z, *dens_n, *dens_h;
+  const int32_t *prod_ptr, *prod_idx;
+  const int32_t *block_L, *block_channels, *block_nbasis;
+  const int32_t *row_ptr, *col;
+  const double* val;
+  const double* mixing;
+  double cutoff;        /* the layer's neighbour cutoff; the system must be set with it */
+  double radial_cutoff; /* the radial family's cutoff (<= cutoff) */
+  int32_t radial_kind;  /* PFW_RADIAL_SMOOTH_EXPONENTIAL */
+  const int32_t* powers;
+  double rate, amplitude;
+  const double* scale;
+} pfw_feature_layer;
+/* Features of every atom of the current system (one configuration): for each
+ * block in turn a (2L+1) x channels x N array, column-major, original atom
+ * order, FP64. Replaces ace_features (features.jl:164-178) over a fresh
+ * cell list of the layer's cutoff. Coincident atoms give PFW_DOMAIN. */
+int pfw_features(pfw_ctx* ctx, const pfw_feature_layer* layer, double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -194,3 +194,19 @@ def ace_evaluate(X, species, H, pbc, model, threads=1, mask=7):
     if rc != 0:
         raise OracleError(rc, err.value.decode())
     return float(E[0]), F, W.reshape(3, 3).T.copy(), site
+
+
+lib.oracle_ylm.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+lib.oracle_ylm.restype = ctypes.c_int
+
+
+def ylm(L, R, solid=False):
+    """Real L2-normalised Y_lm (index l*l+l+m) of the direction of R (r^l Y_lm
+    with ``solid``), from the oracle's angular basis."""
+    R = np.ascontiguousarray(np.asarray(R, float).reshape(3))
+    Y = np.zeros((L + 1) ** 2)
+    dY = np.zeros(3 * (L + 1) ** 2)
+    rc = lib.oracle_ylm(int(L), int(solid), R.ctypes.data, Y.ctypes.data, dY.ctypes.data)
+    if rc != 0:
+        raise OracleError(rc, "harmonics of a zero vector")
+    return Y

@@ -45,6 +45,8 @@ from .adjoints import (  # noqa: F401
     fitting_loss,
 )
 from .fitting import design_matrix, training_rows, linear_parameters, install_readout  # noqa: F401
+from . import features  # noqa: F401
+from .features import ACEFeatureLayer, ace_features, feature_parameters, feature_basis_summary  # noqa: F401
 from .calculators import (  # noqa: F401
     potential_energy,
     forces,

@@ -1858,6 +1858,250 @@ void ace_run(pfw_ctx* ctx, uint32_t mask, int precision) {
   if (ctx->h_out[10] != 0.0) throw Error(PFW_DOMAIN, "nonfinite ACE output (coincident atoms or singular envelope)");
 }
 
+// ---------------------------------------------------------------- equivariant features
+
+// Per-atom equivariant features (ace_features, src/submodules/ACE/features.jl:141-178)
+// of a layer with smooth exponential radials (radials.jl:193-206). One
+// wavefront per atom: its edges, 64 at a time, put their radials, harmonics
+// and neighbour species in LDS; each lane then sums, for its densities, the
+// chunk's edges in order (species-resolved channel 0 = any species); then the
+// products, the coupling rows and the mixed outputs, lanes over entries.
+struct FeatArgs {
+  int64_t n;
+  const int64_t* off;  // n + 1 edge offsets per sorted atom
+  const int32_t* nbr;  // neighbour (original index) of each edge
+  const double* R;     // 3 per edge: x_j + H s - x_i
+  const int32_t* perm;
+  const int32_t* species;  // original order, 0..nz-1
+  int nz, nr, ndens, nprod, nrows, nblocks;
+  const int32_t *dz, *dn, *dh;
+  const int32_t *pptr, *pidx;
+  const int32_t *rptr, *col;
+  const double* val;
+  const int64_t* boff;  // per block: row offset, mixing offset, output offset, dim, nbasis, channels
+  const double* mix;
+  double rc, rate, amplitude;
+  const int32_t* powers;
+  const double* scale;  // nr x nz x nz (radial, centre, neighbour), column-major
+  double* out;
+  int32_t* status;  // set when two atoms coincide
+};
+
+template <int L>
+__global__ __launch_bounds__(64) void k_features(FeatArgs a) {
+  constexpr int NY = (L + 1) * (L + 1);
+  extern __shared__ __align__(16) unsigned char fsm[];
+  double* Fl = (double*)fsm;   // (L+1)^2
+  double* Ysh = Fl + NY;       // NY x 64
+  double* Rsh = Ysh + NY * 64;  // nr x 64
+  double* A = Rsh + a.nr * 64;  // ndens
+  double* P = A + a.ndens;      // nprod
+  double* B = P + a.nprod;      // nrows
+  int* Zsh = (int*)(B + a.nrows);  // 64
+  const int lane = threadIdx.x;
+  const int64_t k = blockIdx.x;
+  const int i = a.perm[k], zi = a.species[i];
+  fill_fl<double, L>(Fl);
+  for (int t = lane; t < a.ndens; t += kWave) A[t] = 0;
+  const int64_t e0 = a.off[k], e1 = a.off[k + 1];
+  __syncthreads();
+  for (int64_t c0 = e0; c0 < e1; c0 += kWave) {
+    const int64_t e = c0 + lane;
+    int zj = -1;
+    if (e < e1) {
+      const double dx = a.R[3 * e], dy = a.R[3 * e + 1], dz = a.R[3 * e + 2];
+      const double r = sqrt(dx * dx + dy * dy + dz * dz);
+      if (r > 0) {
+        zj = a.species[a.nbr[e]];
+        const double x = r / a.rc, q = fmax(1.0 - x, 0.0), q2 = q * q;
+        const double base = a.amplitude * q2 * q2 * exp(-a.rate * r);
+        for (int nn = 0; nn < a.nr; ++nn) {
+          double xn = 1;
+          for (int p = 0; p < a.powers[nn]; ++p) xn *= x;
+          Rsh[nn * 64 + lane] = base * xn * a.scale[nn + a.nr * (zi + a.nz * zj)];
+        }
+        ylm_emit<double, L>(dx, dy, dz, Fl, 0, [&](int h, const Jet<double>& jv) { Ysh[h * 64 + lane] = jv.v; });
+      } else {
+        atomicOr(a.status, 1);  // coincident atoms: undefined harmonics
+      }
+    }
+    Zsh[lane] = zj;
+    __syncthreads();
+    const int ne = (int)min((int64_t)kWave, e1 - c0);
+    for (int t = lane; t < a.ndens; t += kWave) {
+      const int zc = a.dz[t], rn = a.dn[t], h = a.dh[t];
+      double sum = 0;
+      for (int u = 0; u < ne; ++u) {
+        const int zu = Zsh[u];
+        if (zu >= 0 && (zc == 0 || zc == zu + 1)) sum += Rsh[rn * 64 + u] * Ysh[h * 64 + u];
+      }
+      A[t] += sum;
+    }
+    __syncthreads();
+  }
+  for (int t = lane; t < a.nprod; t += kWave) {
+    double v = 1;
+    for (int q = a.pptr[t]; q < a.pptr[t + 1]; ++q) v *= A[a.pidx[q]];
+    P[t] = v;
+  }
+  __syncthreads();
+  for (int t = lane; t < a.nrows; t += kWave) {
+    double v = 0;
+    for (int q = a.rptr[t]; q < a.rptr[t + 1]; ++q) v += a.val[q] * P[a.col[q]];
+    B[t] = v;
+  }
+  __syncthreads();
+  for (int b = 0; b < a.nblocks; ++b) {
+    const int64_t* o = a.boff + 6 * b;
+    const int dim = (int)o[3], nb = (int)o[4], ch = (int)o[5];
+    for (int t = lane; t < dim * ch; t += kWave) {
+      const int m = t % dim, c = t / dim;
+      const double* W = a.mix + o[1] + (int64_t)nb * (c + (int64_t)ch * zi);
+      double v = 0;
+      for (int beta = 0; beta < nb; ++beta) v += B[o[0] + beta * dim + m] * W[beta];
+      a.out[o[2] + m + (int64_t)dim * (c + (int64_t)ch * i)] = v;
+    }
+  }
+}
+
+template <typename T>
+static T* upload_raw(const T* p, size_t count, std::vector<void*>& owned) {
+  T* d = nullptr;
+  PFW_HIP(hipMalloc((void**)&d, std::max<size_t>(count, 1) * sizeof(T)));
+  owned.push_back(d);
+  if (count) PFW_HIP(hipMemcpy(d, p, count * sizeof(T), hipMemcpyHostToDevice));
+  return d;
+}
+
+void features_run(pfw_ctx* ctx, const pfw_feature_layer* l, double* out) {
+  if (!l || !out) throw Error(PFW_INVALID_ARGUMENT, "null feature layer or output");
+  if (l->radial_kind != PFW_RADIAL_SMOOTH_EXPONENTIAL)
+    throw Error(PFW_INVALID_ARGUMENT, "unsupported feature radial family");
+  if (l->nz <= 0 || l->nr <= 0 || l->lmax < 0 || l->ndens < 0 || l->nprod < 0 || l->nblocks <= 0)
+    throw Error(PFW_INVALID_ARGUMENT, "invalid feature layer dimensions");
+  if (l->lmax > 6) throw Error(PFW_INVALID_ARGUMENT, "lmax above 6 is not supported by the device kernels");
+  if (!(std::isfinite(l->cutoff) && l->cutoff > 0 && std::isfinite(l->rate) && std::isfinite(l->amplitude)))
+    throw Error(PFW_INVALID_ARGUMENT, "nonfinite feature radial parameters");
+  if (l->cutoff != ctx->cutoff)
+    throw Error(PFW_STALE, "the system's cell list was built for another cutoff than the feature layer's");
+  const int ny = (l->lmax + 1) * (l->lmax + 1);
+  for (int t = 0; t < l->ndens; ++t)
+    if (l->dens_z[t] < 0 || l->dens_z[t] > l->nz || l->dens_n[t] < 0 || l->dens_n[t] >= l->nr || l->dens_h[t] < 0 ||
+        l->dens_h[t] >= ny)
+      throw Error(PFW_INVALID_ARGUMENT, "invalid feature density entry");
+  if (l->prod_ptr[0] != 0) throw Error(PFW_INVALID_ARGUMENT, "invalid feature product offsets");
+  for (int t = 0; t < l->nprod; ++t) {
+    if (l->prod_ptr[t + 1] <= l->prod_ptr[t]) throw Error(PFW_INVALID_ARGUMENT, "empty feature product");
+    for (int q = l->prod_ptr[t]; q < l->prod_ptr[t + 1]; ++q)
+      if (l->prod_idx[q] < 0 || l->prod_idx[q] >= l->ndens) throw Error(PFW_INVALID_ARGUMENT, "invalid product factor");
+  }
+  int64_t nrows = 0, nmix = 0;
+  const int64_t n = ctx->sys.n;
+  std::vector<int64_t> boff(6 * (size_t)l->nblocks);
+  int64_t nout = 0;
+  for (int b = 0; b < l->nblocks; ++b) {
+    const int dim = 2 * l->block_L[b] + 1, nb = l->block_nbasis[b], ch = l->block_channels[b];
+    if (l->block_L[b] < 0 || nb <= 0 || ch <= 0) throw Error(PFW_INVALID_ARGUMENT, "invalid feature block");
+    boff[6 * b] = nrows;
+    boff[6 * b + 1] = nmix;
+    boff[6 * b + 2] = nout;
+    boff[6 * b + 3] = dim;
+    boff[6 * b + 4] = nb;
+    boff[6 * b + 5] = ch;
+    nrows += (int64_t)dim * nb;
+    nmix += (int64_t)nb * ch * l->nz;
+    nout += (int64_t)dim * ch * n;
+  }
+  if (l->row_ptr[0] != 0) throw Error(PFW_INVALID_ARGUMENT, "invalid coupling row offsets");
+  for (int64_t r = 0; r < nrows; ++r) {
+    if (l->row_ptr[r + 1] < l->row_ptr[r]) throw Error(PFW_INVALID_ARGUMENT, "invalid coupling row offsets");
+    for (int q = l->row_ptr[r]; q < l->row_ptr[r + 1]; ++q)
+      if (l->col[q] < 0 || l->col[q] >= l->nprod || !std::isfinite(l->val[q]))
+        throw Error(PFW_INVALID_ARGUMENT, "invalid coupling entry");
+  }
+  for (int t = 0; t < l->nr; ++t)
+    if (l->powers[t] < 0) throw Error(PFW_INVALID_ARGUMENT, "negative radial power");
+  const int Lk = l->lmax <= 2 ? 2 : l->lmax <= 4 ? 4 : 6;  // the kernel's harmonic degree
+  const size_t nyk = (size_t)(Lk + 1) * (Lk + 1);
+  const size_t lds = sizeof(double) * (nyk + (nyk + l->nr) * 64 + l->ndens + l->nprod + nrows) + 64 * sizeof(int);
+  if (lds > kMaxLds) throw Error(PFW_INVALID_ARGUMENT, "feature tables too large for LDS");
+  std::vector<int32_t> sp(n);
+  if (n > 0) {
+    PFW_HIP(hipMemcpy(sp.data(), ctx->sys.species, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    for (int64_t t = 0; t < n; ++t)
+      if (sp[t] < 0 || sp[t] >= l->nz) throw Error(PFW_INVALID_ARGUMENT, "species channel outside the feature layer");
+  }
+  std::fill(out, out + nout, 0.0);
+  if (n == 0) return;
+  DeviceExport ex;
+  nl_export_device(ctx, ex, false);
+  std::vector<void*> owned;
+  struct Free {
+    std::vector<void*>& o;
+    ~Free() {
+      for (void* q : o) (void)hipFree(q);
+    }
+  } guard{owned};
+  const int64_t nval = l->row_ptr[nrows];
+  FeatArgs a;
+  a.n = n;
+  a.off = ex.off;
+  a.nbr = ex.j;
+  a.R = ex.R;
+  a.perm = ctx->sys.perm;
+  a.species = ctx->sys.species;
+  a.nz = l->nz;
+  a.nr = l->nr;
+  a.ndens = l->ndens;
+  a.nprod = l->nprod;
+  a.nrows = (int)nrows;
+  a.nblocks = l->nblocks;
+  a.dz = upload_raw(l->dens_z, l->ndens, owned);
+  a.dn = upload_raw(l->dens_n, l->ndens, owned);
+  a.dh = upload_raw(l->dens_h, l->ndens, owned);
+  a.pptr = upload_raw(l->prod_ptr, (size_t)l->nprod + 1, owned);
+  a.pidx = upload_raw(l->prod_idx, (size_t)l->prod_ptr[l->nprod], owned);
+  a.rptr = upload_raw(l->row_ptr, (size_t)nrows + 1, owned);
+  a.col = upload_raw(l->col, (size_t)nval, owned);
+  a.val = upload_raw(l->val, (size_t)nval, owned);
+  a.boff = upload_raw(boff.data(), boff.size(), owned);
+  a.mix = upload_raw(l->mixing, (size_t)nmix, owned);
+  a.rc = l->radial_cutoff;
+  a.rate = l->rate;
+  a.amplitude = l->amplitude;
+  a.powers = upload_raw(l->powers, (size_t)l->nr, owned);
+  a.scale = upload_raw(l->scale, (size_t)l->nr * l->nz * l->nz, owned);
+  double* dout = nullptr;
+  PFW_HIP(hipMalloc(&dout, std::max<int64_t>(nout, 1) * sizeof(double)));
+  owned.push_back(dout);
+  a.out = dout;
+  int32_t* status = nullptr;
+  PFW_HIP(hipMalloc(&status, sizeof(int32_t)));
+  owned.push_back(status);
+  hipStream_t st = ctx->stream;
+  PFW_HIP(hipMemsetAsync(status, 0, sizeof(int32_t), st));
+  a.status = status;
+  switch (Lk) {
+    case 2:
+      allow_lds(k_features<2>, lds, "the feature kernel");
+      hipLaunchKernelGGL(k_features<2>, dim3((unsigned)n), dim3(kWave), lds, st, a);
+      break;
+    case 4:
+      allow_lds(k_features<4>, lds, "the feature kernel");
+      hipLaunchKernelGGL(k_features<4>, dim3((unsigned)n), dim3(kWave), lds, st, a);
+      break;
+    default:
+      allow_lds(k_features<6>, lds, "the feature kernel");
+      hipLaunchKernelGGL(k_features<6>, dim3((unsigned)n), dim3(kWave), lds, st, a);
+  }
+  PFW_HIP(hipGetLastError());
+  int32_t h_status = 0;
+  PFW_HIP(hipMemcpyAsync(&h_status, status, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  PFW_HIP(hipMemcpyAsync(out, dout, nout * sizeof(double), hipMemcpyDeviceToHost, st));
+  PFW_HIP(hipStreamSynchronize(st));
+  if (h_status) throw Error(PFW_DOMAIN, "coincident physical atoms have undefined harmonics");
+}
+
 // Force rows of one output column: atom i (original order) of configuration c
 // has its rows at 10 c + 1 + 3 i + axis (each configuration's block holds its
 // energy, 3 n_c force rows and 9 virial rows).

@@ -408,6 +408,15 @@ int pfw_ace_parameter_sets(pfw_ctx* ctx, int precision, int64_t nsets, const dou
   });
 }
 
+int pfw_features(pfw_ctx* ctx, const pfw_feature_layer* layer, double* out) {
+  return run(ctx, [&] {
+    if (!ctx->have_system) throw Error(PFW_STALE, "no system set");
+    if (ctx->ncfg != 1) throw Error(PFW_INVALID_ARGUMENT, "features are evaluated for one configuration");
+    ctx->have_results = false;
+    pfw::features_run(ctx, layer, out);
+  });
+}
+
 int pfw_pair_adjoint(pfw_ctx* ctx, const double* dE, const double* dF, const double* dW, double* gA) {
   return run(ctx, [&] {
     if (!ctx->have_system) throw Error(PFW_STALE, "no system set");

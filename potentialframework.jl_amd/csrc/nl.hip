@@ -905,61 +905,70 @@ __global__ void k_export(const Grid* __restrict__ grids, const int32_t* __restri
   if (!pass) counts[k] = cnt;
 }
 
-void nl_export_pairs(pfw_ctx* ctx, int32_t* hi, int32_t* hj, int32_t* hS, double* hR, int64_t cap,
-                     int64_t* count) {
+// The directed pairs of the current system on the device, grouped by sorted
+// atom: pairs of sorted atom k at [off[k], off[k + 1]) (fresh cell build).
+// Every array is allocated here (DeviceExport frees them); S may be skipped.
+void nl_export_device(pfw_ctx* ctx, DeviceExport& ex, bool with_S) {
   SystemBuffers& s = ctx->sys;
   hipStream_t st = ctx->stream;
-  int64_t n = s.n;
+  const int64_t n = s.n;
   nl_build(ctx);
-  if (n == 0) {
-    *count = 0;
-    return;
-  }
+  ex.release();
+  ex.total = 0;
+  if (n == 0) return;
   int32_t* d_counts = nullptr;
-  int64_t* d_off = nullptr;
   PFW_HIP(hipMalloc(&d_counts, n * sizeof(int32_t)));
-  PFW_HIP(hipMalloc(&d_off, n * sizeof(int64_t)));
+  ex.counts = d_counts;
+  PFW_HIP(hipMalloc(&ex.off, (n + 1) * sizeof(int64_t)));
   hipLaunchKernelGGL(k_export, dim3(nblocks(n, 128)), dim3(128), 0, st, s.grids, s.atom_cfg, n, s.X, s.xs, s.start,
                      s.cell_of, s.perm, d_counts, (const int64_t*)nullptr, 0, nullptr, nullptr, nullptr, nullptr);
   PFW_HIP(hipGetLastError());
   std::vector<int32_t> counts(n);
   PFW_HIP(hipMemcpyAsync(counts.data(), d_counts, n * sizeof(int32_t), hipMemcpyDeviceToHost, st));
   PFW_HIP(hipStreamSynchronize(st));
-  std::vector<int64_t> off(n);
+  std::vector<int64_t> off(n + 1);
   int64_t total = 0;
   for (int64_t k = 0; k < n; ++k) {
     off[k] = total;
     total += counts[k];
   }
-  *count = total;
-  if (total > cap) {
-    (void)hipFree(d_counts);
-    (void)hipFree(d_off);
-    throw Error(PFW_CAPACITY, "pair export capacity too small");
-  }
-  int32_t *di = nullptr, *dj = nullptr, *dS = nullptr;
-  double* dR = nullptr;
-  PFW_HIP(hipMemcpyAsync(d_off, off.data(), n * sizeof(int64_t), hipMemcpyHostToDevice, st));
-  PFW_HIP(hipMalloc(&di, std::max<int64_t>(total, 1) * sizeof(int32_t)));
-  PFW_HIP(hipMalloc(&dj, std::max<int64_t>(total, 1) * sizeof(int32_t)));
-  PFW_HIP(hipMalloc(&dS, std::max<int64_t>(total, 1) * 3 * sizeof(int32_t)));
-  PFW_HIP(hipMalloc(&dR, std::max<int64_t>(total, 1) * 3 * sizeof(double)));
+  off[n] = total;
+  ex.total = total;
+  PFW_HIP(hipMemcpyAsync(ex.off, off.data(), (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
+  PFW_HIP(hipMalloc(&ex.i, std::max<int64_t>(total, 1) * sizeof(int32_t)));
+  PFW_HIP(hipMalloc(&ex.j, std::max<int64_t>(total, 1) * sizeof(int32_t)));
+  if (with_S) PFW_HIP(hipMalloc(&ex.S, std::max<int64_t>(total, 1) * 3 * sizeof(int32_t)));
+  PFW_HIP(hipMalloc(&ex.R, std::max<int64_t>(total, 1) * 3 * sizeof(double)));
   hipLaunchKernelGGL(k_export, dim3(nblocks(n, 128)), dim3(128), 0, st, s.grids, s.atom_cfg, n, s.X, s.xs, s.start,
-                     s.cell_of, s.perm, d_counts, (const int64_t*)d_off, 1, di, dj, dS, dR);
+                     s.cell_of, s.perm, d_counts, (const int64_t*)ex.off, 1, ex.i, ex.j, ex.S, ex.R);
   PFW_HIP(hipGetLastError());
+  PFW_HIP(hipStreamSynchronize(st));  // the staging vector goes out of scope
+}
+
+void DeviceExport::release() {
+  void* p[] = {counts, off, i, j, S, R};
+  for (void* q : p)
+    if (q) (void)hipFree(q);
+  counts = i = j = S = nullptr;
+  off = nullptr;
+  R = nullptr;
+}
+
+void nl_export_pairs(pfw_ctx* ctx, int32_t* hi, int32_t* hj, int32_t* hS, double* hR, int64_t cap,
+                     int64_t* count) {
+  hipStream_t st = ctx->stream;
+  DeviceExport ex;
+  nl_export_device(ctx, ex, true);
+  const int64_t total = ex.total;
+  *count = total;
+  if (total > cap) throw Error(PFW_CAPACITY, "pair export capacity too small");
   if (total > 0) {
-    if (hi) PFW_HIP(hipMemcpyAsync(hi, di, total * sizeof(int32_t), hipMemcpyDeviceToHost, st));
-    if (hj) PFW_HIP(hipMemcpyAsync(hj, dj, total * sizeof(int32_t), hipMemcpyDeviceToHost, st));
-    if (hS) PFW_HIP(hipMemcpyAsync(hS, dS, total * 3 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
-    if (hR) PFW_HIP(hipMemcpyAsync(hR, dR, total * 3 * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (hi) PFW_HIP(hipMemcpyAsync(hi, ex.i, total * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    if (hj) PFW_HIP(hipMemcpyAsync(hj, ex.j, total * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    if (hS) PFW_HIP(hipMemcpyAsync(hS, ex.S, total * 3 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    if (hR) PFW_HIP(hipMemcpyAsync(hR, ex.R, total * 3 * sizeof(double), hipMemcpyDeviceToHost, st));
   }
   PFW_HIP(hipStreamSynchronize(st));
-  (void)hipFree(d_counts);
-  (void)hipFree(d_off);
-  (void)hipFree(di);
-  (void)hipFree(dj);
-  (void)hipFree(dS);
-  (void)hipFree(dR);
 }
 
 }  // namespace pfw

@@ -265,6 +265,18 @@ namespace pfw {
 void nl_configure(pfw_ctx* ctx, double cutoff, double skin);
 void batch_reduce(pfw_ctx* ctx, const double* records, int width, bool cells, int kind);
 void nl_build(pfw_ctx* ctx);
+// Directed pairs on the device, grouped by sorted atom (offsets off[0..n]);
+// the destructor frees them.
+struct DeviceExport {
+  int64_t total = 0;
+  int32_t* counts = nullptr;
+  int64_t* off = nullptr;
+  int32_t *i = nullptr, *j = nullptr, *S = nullptr;
+  double* R = nullptr;
+  void release();
+  ~DeviceExport() { release(); }
+};
+void nl_export_device(pfw_ctx* ctx, DeviceExport& ex, bool with_S);
 void nl_export_pairs(pfw_ctx* ctx, int32_t* i, int32_t* j, int32_t* S, double* R, int64_t cap,
                      int64_t* count);
 void nl_free(pfw_ctx* ctx);
@@ -291,6 +303,7 @@ void ace_free_edges(pfw_ctx* ctx);
 void ace_site_energies(pfw_ctx* ctx, double* out);
 void ace_parameter_sets(pfw_ctx* ctx, int precision, int64_t nsets, const double* WB, const double* Wpair,
                         double* out);
+void features_run(pfw_ctx* ctx, const pfw_feature_layer* layer, double* out);
 void ace_adjoint(pfw_ctx* ctx, int precision, const double* Ebar, const double* Fbar, const double* Wbar, double* gWB,
                  double* gWpair);
 }  // namespace pfw
