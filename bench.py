@@ -322,6 +322,56 @@ def bench_may30(pfw, lib, _capi, args, reps=5):
     return out
 
 
+def bench_features(pfw, args):
+    """Per-atom equivariant features of the reference's feature fixture layer
+    (test/ace_features.jl:16-27: Si/O, smooth exponential radials
+    [(1,0),(1,1),(2,1),(1,2)] with cutoff 4 A, correlation 3, lmax 2, blocks
+    0e x3, 1o x2, 1e x2, 3o x2, 2e x2) on a 32,768-atom periodic BCC cell
+    (a = 3.2 A), FP64. The CPU baseline is the oracle's Python restatement on
+    a 54-atom cell of the same lattice."""
+    F = pfw.features
+    spec = [(1, 0), (1, 1), (2, 1), (1, 2)]
+    irreps = [dict(L=0, parity="even", channels=3), dict(L=1, parity="odd", channels=2),
+              dict(L=1, parity="even", channels=2), dict(L=3, parity="odd", channels=2),
+              dict(L=2, parity="even", channels=2)]
+    layer = F.ACEFeatureLayer(["Si", "O"], F.SmoothExponential(spec, 4.0, [0, 0, 1, 0]), 3, 2, irreps)
+    ps = F.feature_parameters(layer, custom_radial={"rate": [0.4], "amplitude": [1.3]})
+    for b, key in enumerate(sorted(ps["mixing"], key=lambda k: int(k.split("_")[1]))):
+        W = ps["mixing"][key]
+        W[...] = np.sin(b + 1 + np.arange(1, W.size + 1)).reshape(W.shape, order="F")
+    X, H, pbc, sp = pfw.workloads.bcc_binary()
+    sys_ = pfw.System(X, H, pbc, ["Si" if s == 0 else "O" for s in sp])
+    w = pfw.Workspace(precision="float64")
+    pfw.ace_features(layer, sys_, ps, workspace=w)
+    reps = max(3, args.steps // 10)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        pfw.ace_features(layer, sys_, ps, workspace=w)
+    secs = (time.perf_counter() - t0) / reps
+    nnz = sum(int(np.count_nonzero(c)) for c in layer.coupling)
+    out = {"workload": "feature fixture layer (correlation 3, lmax 2, L <= 3) on bcc_binary 32k",
+           "atoms": len(X), "densities": len(layer.density), "products": len(layer.products),
+           "coupling_nonzeros": nnz, "blocks": [(s.L, s.parity, s.channels) for s in layer.irreps],
+           "value": round(len(X) / secs / 1e6, 4), "unit": "Matom-features/s", "ms_per_call": round(1e3 * secs, 3),
+           "call": "fresh cell list + pair export on the device, feature kernel, outputs to host"}
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, ROOT)
+        from oracle.features import ace_features as oracle_features
+        Xs, Hs, pbs, sps = pfw.workloads.bcc_binary(3, 3, 3, seed=3)
+        T = dict(density=layer.density, products=layer.products, coupling=layer.coupling,
+                 irreps=[(s.L, s.channels) for s in layer.irreps], lmax=layer.lmax, cutoff=layer.cutoff,
+                 radial_cutoff=layer.radial.cutoff, powers=layer.radial.powers)
+        P = dict(mixing=[ps["mixing"][f"block_{b + 1}"] for b in range(len(layer.irreps))],
+                 scale=ps["radial"]["scale"], rate=0.4, amplitude=1.3)
+        t0 = time.perf_counter()
+        oracle_features(Xs, np.asarray(sps), Hs, pbs, T, P)
+        cs = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(len(Xs) / cs / 1e6, 6), "unit": "Matom-features/s", "cores": 1,
+                               "kind": "port", "sample": "the oracle's Python restatement on 54 atoms of the "
+                                                        "same BCC lattice"}
+    return out
+
+
 def calibrate_copy(nbytes=256 << 20, reps=5):
     """A plain device copy of known size: the reference point that scales the
     FETCH_SIZE / WRITE_SIZE counters into HBM bytes."""
@@ -512,6 +562,10 @@ def main():
     # ---------------------------------------------------------------- May30 real model
     if not args.no_ace:
         line["ace_may30"] = bench_may30(pfw, lib, _capi, args)
+
+    # ---------------------------------------------------------------- equivariant features
+    if not args.no_ace:
+        line["features"] = bench_features(pfw, args)
 
     # ---------------------------------------------------------------- fitting set (batch + adjoints)
     if not args.no_batch and not args.no_ace:
