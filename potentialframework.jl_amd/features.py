@@ -327,15 +327,13 @@ def ace_features(layer, system, ps, workspace=None):
 
 
 def _channels(layer, system):
-    z = list(layer.atomic_numbers)
     from .ace import ELEMENTS
-    out = []
-    for lab in system.labels:
-        zz = ELEMENTS.index(lab) + 1 if isinstance(lab, str) and lab in ELEMENTS else None
-        if zz not in z:
-            raise ArgumentError(f"species {lab} is not in the feature layer")
-        out.append(z.index(zz))
-    return np.asarray(out, np.int32)
+    lut = {ELEMENTS[z - 1]: c for c, z in enumerate(layer.atomic_numbers)}
+    try:
+        return np.fromiter((lut[lab] for lab in system.labels), np.int32, count=len(system.labels))
+    except (KeyError, TypeError):
+        bad = next(lab for lab in system.labels if not isinstance(lab, str) or lab not in lut)
+        raise ArgumentError(f"species {bad} is not in the feature layer") from None
 
 
 class _Layer(ctypes.Structure):
@@ -350,36 +348,43 @@ class _Layer(ctypes.Structure):
                 ("amplitude", ctypes.c_double), ("scale", ctypes.c_void_p)]
 
 
-def _struct(layer, ps, rate, amplitude, scale):
-    keep = []
-
-    def arr(a, dt):
-        a = np.ascontiguousarray(np.asarray(a, dtype=dt))
-        keep.append(a)
-        return a.ctypes.data
-
-    d = np.asarray(layer.density, np.int64).reshape(-1, 3)
+def _tables(layer):
+    """The layer's flat device tables (0-based density columns, product and
+    coupling CSR over all blocks), built once per layer."""
+    t = getattr(layer, "_flat", None)
+    if t is not None:
+        return t
+    d = np.asarray(layer.density, np.int32).reshape(-1, 3)
     pptr = np.zeros(len(layer.products) + 1, np.int32)
     pptr[1:] = np.cumsum([len(p) for p in layer.products])
-    pidx = np.concatenate([np.asarray(p, np.int32) for p in layer.products]) if layer.products else np.zeros(0)
-    rows, cols, vals, mix = [0], [], [], []
-    for b, s in enumerate(layer.irreps):
-        C = layer.coupling[b]
-        for r in range(C.shape[0]):
-            nzc = np.nonzero(C[r])[0]
-            cols.extend(nzc)
-            vals.extend(C[r, nzc])
-            rows.append(len(cols))
-        mix.append(np.asarray(ps["mixing"][f"block_{b + 1}"], float).transpose(2, 1, 0).reshape(-1))  # column-major
-    st = _Layer(len(layer.atomic_numbers), len(layer.radial_spec), layer.lmax, len(d), len(layer.products),
-                len(layer.irreps), arr(d[:, 0], np.int32), arr(d[:, 1], np.int32), arr(d[:, 2], np.int32),
-                arr(pptr, np.int32), arr(pidx, np.int32), arr([s.L for s in layer.irreps], np.int32),
-                arr([s.channels for s in layer.irreps], np.int32),
-                arr([layer.basis_channels(b) for b in range(len(layer.irreps))], np.int32),
-                arr(rows, np.int32), arr(cols, np.int32), arr(vals, np.float64),
-                arr(np.concatenate(mix) if mix else np.zeros(0), np.float64), layer.cutoff, layer.radial.cutoff,
-                PFW_RADIAL_SMOOTH_EXPONENTIAL, arr(layer.radial.powers, np.int32), float(rate), float(amplitude),
-                arr(scale.transpose(2, 1, 0).reshape(-1), np.float64))
+    pidx = np.concatenate([np.asarray(p, np.int32) for p in layer.products]) if layer.products else np.zeros(0, np.int32)
+    C = np.vstack(layer.coupling)
+    r, c = np.nonzero(C)  # row-major: rows in turn, columns ascending
+    rows = np.zeros(C.shape[0] + 1, np.int32)
+    rows[1:] = np.cumsum(np.bincount(r, minlength=C.shape[0]))
+    t = dict(dz=np.ascontiguousarray(d[:, 0]), dn=np.ascontiguousarray(d[:, 1]), dh=np.ascontiguousarray(d[:, 2]),
+             pptr=pptr, pidx=pidx.astype(np.int32), rows=rows, cols=c.astype(np.int32),
+             vals=np.ascontiguousarray(C[r, c], np.float64),
+             L=np.array([s.L for s in layer.irreps], np.int32),
+             ch=np.array([s.channels for s in layer.irreps], np.int32),
+             nb=np.array([layer.basis_channels(b) for b in range(len(layer.irreps))], np.int32),
+             powers=np.asarray(layer.radial.powers, np.int32))
+    layer._flat = t
+    return t
+
+
+def _struct(layer, ps, rate, amplitude, scale):
+    t = _tables(layer)
+    mix = np.concatenate([np.asarray(ps["mixing"][f"block_{b + 1}"], float).transpose(2, 1, 0).reshape(-1)
+                          for b in range(len(layer.irreps))])  # column-major blocks in turn
+    sc = np.ascontiguousarray(scale.transpose(2, 1, 0).reshape(-1), np.float64)
+    keep = [t, mix, sc]
+    st = _Layer(len(layer.atomic_numbers), len(layer.radial_spec), layer.lmax, len(t["dz"]), len(layer.products),
+                len(layer.irreps), t["dz"].ctypes.data, t["dn"].ctypes.data, t["dh"].ctypes.data,
+                t["pptr"].ctypes.data, t["pidx"].ctypes.data, t["L"].ctypes.data, t["ch"].ctypes.data,
+                t["nb"].ctypes.data, t["rows"].ctypes.data, t["cols"].ctypes.data, t["vals"].ctypes.data,
+                mix.ctypes.data, layer.cutoff, layer.radial.cutoff, PFW_RADIAL_SMOOTH_EXPONENTIAL,
+                t["powers"].ctypes.data, float(rate), float(amplitude), sc.ctypes.data)
     return st, keep
 
 
