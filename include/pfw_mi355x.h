@@ -279,6 +279,15 @@ typedef struct {
  * order, FP64. Replaces ace_features (features.jl:164-178) over a fresh
  * cell list of the layer's cutoff. Coincident atoms give PFW_DOMAIN. */
 int pfw_features(pfw_ctx* ctx, const pfw_feature_layer* layer, double* out);
+/* Pullback of pfw_features for cotangents laid out as its output: replaces
+ * feature_vjp (features.jl:204-225) with feature_local_reverse (:227-255).
+ * dx: 3 x N (column-major, original order), the gradient of
+ * sum(cotangents . features) with respect to the positions; dmixing: laid
+ * out as layer->mixing; dscale: as layer->scale; dradial[2]: the rate and
+ * amplitude gradients. Coincident atoms give PFW_DOMAIN; nonfinite
+ * cotangents PFW_INVALID_ARGUMENT. */
+int pfw_features_vjp(pfw_ctx* ctx, const pfw_feature_layer* layer, const double* cotangents, double* dx,
+                     double* dmixing, double* dscale, double* dradial);
 
 #ifdef __cplusplus
 }

@@ -200,13 +200,14 @@ lib.oracle_ylm.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c
 lib.oracle_ylm.restype = ctypes.c_int
 
 
-def ylm(L, R, solid=False):
+def ylm(L, R, solid=False, grad=False):
     """Real L2-normalised Y_lm (index l*l+l+m) of the direction of R (r^l Y_lm
-    with ``solid``), from the oracle's angular basis."""
+    with ``solid``), from the oracle's angular basis; with ``grad`` also the
+    Cartesian gradients ((L+1)^2 x 3)."""
     R = np.ascontiguousarray(np.asarray(R, float).reshape(3))
     Y = np.zeros((L + 1) ** 2)
     dY = np.zeros(3 * (L + 1) ** 2)
     rc = lib.oracle_ylm(int(L), int(solid), R.ctypes.data, Y.ctypes.data, dY.ctypes.data)
     if rc != 0:
         raise OracleError(rc, "harmonics of a zero vector")
-    return Y
+    return (Y, dY.reshape(-1, 3)) if grad else Y

@@ -54,3 +54,73 @@ def ace_features(X, channels, H, pbc, tables, params):
             Bb = (tables["coupling"][b] @ P).reshape(-1, dim).T  # (dim, basis), m fastest
             out[b][:, :, a] = Bb @ params["mixing"][b][:, :, za]
     return out
+
+
+def feature_vjp(X, channels, H, pbc, tables, params, cot):
+    """The reverse of ``ace_features`` for cotangents ``cot`` (one
+    (2L+1, channels, n) array per block), following feature_vjp and
+    feature_local_reverse (features.jl:204-255): mixing B^T G, dB = G W^T,
+    product weights coupling^T vec(dB), dA by the product rule, then per edge
+    g = sum_a dA_a (R'_n Y_m r^ + R_n grad Y_m) with dx_i -= g, dx_j += g, the
+    radial scale gradient dA_a R_n Y_m, and the smooth exponential's rate and
+    amplitude gradients. Returns dx (n, 3) and a dict of parameter gradients."""
+    X = np.asarray(X, float).reshape(-1, 3)
+    n = len(X)
+    dx = np.zeros((n, 3))
+    gmix = [np.zeros_like(W) for W in params["mixing"]]
+    gscale = np.zeros_like(params["scale"])
+    grate = gamp = 0.0
+    if n == 0:
+        return dx, dict(mixing=gmix, scale=gscale, rate=grate, amplitude=gamp)
+    i, j, _, R, _ = nl_pairs(X, H, pbc, tables["cutoff"])
+    dens, prods = tables["density"], tables["products"]
+    rc, powers, rate, amp = tables["radial_cutoff"], tables["powers"], params["rate"], params["amplitude"]
+    for a in range(n):
+        za = channels[a]
+        edges = np.nonzero(i == a)[0]
+        data = []
+        A = np.zeros(len(dens))
+        for e in edges:
+            d = R[e]
+            r = float(np.linalg.norm(d))
+            zj = channels[j[e]]
+            x = r / rc
+            q = max(1.0 - x, 0.0)
+            ex = np.exp(-rate * r)
+            raw = np.array([amp * x ** p * q ** 4 * ex for p in powers])
+            draw = np.array([amp * ex * ((p * x ** (p - 1) / rc if p else 0.0) * q ** 4 - 4 * x ** p * q ** 3 / rc
+                                         - rate * x ** p * q ** 4) for p in powers])
+            sc = params["scale"][:, za, zj]
+            Y, dY = ylm(tables["lmax"], d, grad=True)
+            data.append((e, d, r, zj, raw, draw, sc, Y, dY))
+            for k, (zc, rn, h) in enumerate(dens):
+                if zc == 0 or zc == zj + 1:
+                    A[k] += raw[rn] * sc[rn] * Y[h]
+        P = np.array([np.prod([A[t] for t in p]) for p in prods])
+        weights = np.zeros(len(prods))
+        for b, (L, ch) in enumerate(tables["irreps"]):
+            dim = 2 * L + 1
+            Bb = (tables["coupling"][b] @ P).reshape(-1, dim).T  # (dim, basis)
+            G = cot[b][:, :, a]
+            W = params["mixing"][b][:, :, za]
+            gmix[b][:, :, za] += Bb.T @ G
+            dB = G @ W.T  # (dim, basis)
+            weights += tables["coupling"][b].T @ dB.T.reshape(-1)  # vec(dB), m fastest
+        dA = np.zeros(len(dens))
+        for k, p in enumerate(prods):
+            for q in range(len(p)):
+                others = np.prod([A[p[t]] for t in range(len(p)) if t != q])
+                dA[p[q]] += weights[k] * others
+        for (e, d, r, zj, raw, draw, sc, Y, dY) in data:
+            u = d / r
+            g = np.zeros(3)
+            for k, (zc, rn, h) in enumerate(dens):
+                if not (zc == 0 or zc == zj + 1):
+                    continue
+                g += dA[k] * sc[rn] * (draw[rn] * Y[h] * u + raw[rn] * dY[h])
+                gscale[rn, za, zj] += dA[k] * raw[rn] * Y[h]
+                grate += dA[k] * sc[rn] * Y[h] * (-r * raw[rn])
+                gamp += dA[k] * sc[rn] * Y[h] * (raw[rn] / amp)
+            dx[a] -= g
+            dx[j[e]] += g
+    return dx, dict(mixing=gmix, scale=gscale, rate=grate, amplitude=gamp)

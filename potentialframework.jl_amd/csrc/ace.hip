@@ -1887,24 +1887,12 @@ struct FeatArgs {
   int32_t* status;  // set when two atoms coincide
 };
 
+// The densities A of one atom (edges [e0, e1), centre species zi) into LDS;
+// Ysh (NY x 64), Rsh (nr x 64) and Zsh (64) are the chunk staging.
 template <int L>
-__global__ __launch_bounds__(64) void k_features(FeatArgs a) {
-  constexpr int NY = (L + 1) * (L + 1);
-  extern __shared__ __align__(16) unsigned char fsm[];
-  double* Fl = (double*)fsm;   // (L+1)^2
-  double* Ysh = Fl + NY;       // NY x 64
-  double* Rsh = Ysh + NY * 64;  // nr x 64
-  double* A = Rsh + a.nr * 64;  // ndens
-  double* P = A + a.ndens;      // nprod
-  double* B = P + a.nprod;      // nrows
-  int* Zsh = (int*)(B + a.nrows);  // 64
+__device__ inline void feat_densities(const FeatArgs& a, int64_t e0, int64_t e1, int zi, const double* Fl,
+                                      double* Ysh, double* Rsh, int* Zsh, double* A) {
   const int lane = threadIdx.x;
-  const int64_t k = blockIdx.x;
-  const int i = a.perm[k], zi = a.species[i];
-  fill_fl<double, L>(Fl);
-  for (int t = lane; t < a.ndens; t += kWave) A[t] = 0;
-  const int64_t e0 = a.off[k], e1 = a.off[k + 1];
-  __syncthreads();
   for (int64_t c0 = e0; c0 < e1; c0 += kWave) {
     const int64_t e = c0 + lane;
     int zj = -1;
@@ -1939,6 +1927,11 @@ __global__ __launch_bounds__(64) void k_features(FeatArgs a) {
     }
     __syncthreads();
   }
+}
+
+// Products P_k = prod A and coupled rows B = C P of one atom (LDS).
+__device__ inline void feat_rows(const FeatArgs& a, const double* A, double* P, double* B) {
+  const int lane = threadIdx.x;
   for (int t = lane; t < a.nprod; t += kWave) {
     double v = 1;
     for (int q = a.pptr[t]; q < a.pptr[t + 1]; ++q) v *= A[a.pidx[q]];
@@ -1951,6 +1944,27 @@ __global__ __launch_bounds__(64) void k_features(FeatArgs a) {
     B[t] = v;
   }
   __syncthreads();
+}
+
+template <int L>
+__global__ __launch_bounds__(64) void k_features(FeatArgs a) {
+  constexpr int NY = (L + 1) * (L + 1);
+  extern __shared__ __align__(16) unsigned char fsm[];
+  double* Fl = (double*)fsm;   // (L+1)^2
+  double* Ysh = Fl + NY;       // NY x 64
+  double* Rsh = Ysh + NY * 64;  // nr x 64
+  double* A = Rsh + a.nr * 64;  // ndens
+  double* P = A + a.ndens;      // nprod
+  double* B = P + a.nprod;      // nrows
+  int* Zsh = (int*)(B + a.nrows);  // 64
+  const int lane = threadIdx.x;
+  const int64_t k = blockIdx.x;
+  const int i = a.perm[k], zi = a.species[i];
+  fill_fl<double, L>(Fl);
+  for (int t = lane; t < a.ndens; t += kWave) A[t] = 0;
+  __syncthreads();
+  feat_densities<L>(a, a.off[k], a.off[k + 1], zi, Fl, Ysh, Rsh, Zsh, A);
+  feat_rows(a, A, P, B);
   for (int b = 0; b < a.nblocks; ++b) {
     const int64_t* o = a.boff + 6 * b;
     const int dim = (int)o[3], nb = (int)o[4], ch = (int)o[5];
@@ -1964,6 +1978,254 @@ __global__ __launch_bounds__(64) void k_features(FeatArgs a) {
   }
 }
 
+// The pullback of the features (feature_vjp / feature_local_reverse,
+// features.jl:204-255) for one atom per wavefront: the forward densities,
+// products and rows again; the mixing partial B^T G; dB = G W^T over the rows;
+// product weights C^T vec(dB) through the coupling columns; dA by the product
+// rule through the density -> (product, factor) index; then per edge, lane per
+// edge, g = sum_a dA_a (R'_n Y_m r^ + R_n grad Y_m), the densities of each
+// harmonic visited as ylm_emit forms it. Per atom it writes one parameter row
+// (mixing partials, radial scale partials per neighbour species, rate and
+// amplitude partials); g goes to the edge.
+struct FeatVjpArgs {
+  const double* cot;  // the cotangents, laid out as the forward output
+  const int32_t *cptr, *crow;  // coupling columns: product -> rows
+  const double* cval;
+  const int32_t *aptr, *aprod, *apos;  // density -> (product, factor position)
+  const int32_t *hptr, *hidx;          // harmonic -> densities
+  double* gE;    // 3 per edge
+  double* prow;  // n x wp, original atom order
+  int wp, so;    // row width; offset of the scale partials (after the mixing partials)
+};
+
+template <int L>
+__global__ __launch_bounds__(64) void k_features_vjp(FeatArgs a, FeatVjpArgs v) {
+  constexpr int NY = (L + 1) * (L + 1);
+  extern __shared__ __align__(16) unsigned char fsm[];
+  double* Fl = (double*)fsm;          // (L+1)^2
+  double* Ysh = Fl + NY;              // NY x 64
+  double* Rsh = Ysh + NY * 64;        // nr x 64: scaled radials
+  double* Ush = Rsh + a.nr * 64;      // nr x 64: radials at unit amplitude and scale
+  double* Tsh = Ush + a.nr * 64;      // nr x 64: sum of dA Y per radial
+  double* A = Tsh + a.nr * 64;        // ndens
+  double* dA = A + a.ndens;           // ndens
+  double* P = dA + a.ndens;           // nprod, then the product weights
+  double* B = P + a.nprod;            // nrows, then dB
+  double* Gsc = B + a.nrows;          // nr x nz: scale partials (radial, neighbour species)
+  int* Zsh = (int*)(Gsc + a.nr * a.nz);  // 64
+  const int lane = threadIdx.x;
+  const int64_t k = blockIdx.x;
+  const int i = a.perm[k], zi = a.species[i];
+  const int64_t e0 = a.off[k], e1 = a.off[k + 1];
+  fill_fl<double, L>(Fl);
+  for (int t = lane; t < a.ndens; t += kWave) A[t] = 0;
+  for (int t = lane; t < a.nr * a.nz; t += kWave) Gsc[t] = 0;
+  __syncthreads();
+  feat_densities<L>(a, e0, e1, zi, Fl, Ysh, Rsh, Zsh, A);
+  feat_rows(a, A, P, B);
+  double* row = v.prow + (int64_t)i * v.wp;
+  int64_t mo = 0;
+  for (int b = 0; b < a.nblocks; ++b) {  // mixing partials B^T G
+    const int64_t* o = a.boff + 6 * b;
+    const int dim = (int)o[3], nb = (int)o[4], ch = (int)o[5];
+    const double* G = v.cot + o[2] + (int64_t)dim * ch * i;
+    for (int t = lane; t < nb * ch; t += kWave) {
+      const int beta = t % nb, c = t / nb;
+      double s = 0;
+      for (int m = 0; m < dim; ++m) s += B[o[0] + beta * dim + m] * G[m + dim * c];
+      row[mo + t] = s;
+    }
+    mo += (int64_t)nb * ch;
+  }
+  __syncthreads();
+  for (int b = 0; b < a.nblocks; ++b) {  // dB = G W^T, over B
+    const int64_t* o = a.boff + 6 * b;
+    const int dim = (int)o[3], nb = (int)o[4], ch = (int)o[5];
+    const double* G = v.cot + o[2] + (int64_t)dim * ch * i;
+    const double* W = a.mix + o[1] + (int64_t)nb * ch * zi;
+    for (int t = lane; t < dim * nb; t += kWave) {
+      const int m = t % dim, beta = t / dim;
+      double s = 0;
+      for (int c = 0; c < ch; ++c) s += G[m + dim * c] * W[beta + nb * c];
+      B[o[0] + t] = s;
+    }
+  }
+  __syncthreads();
+  for (int t = lane; t < a.nprod; t += kWave) {  // product weights C^T vec(dB), over P
+    double s = 0;
+    for (int q = v.cptr[t]; q < v.cptr[t + 1]; ++q) s += v.cval[q] * B[v.crow[q]];
+    P[t] = s;
+  }
+  __syncthreads();
+  for (int t = lane; t < a.ndens; t += kWave) {  // dA by the product rule
+    double s = 0;
+    for (int q = v.aptr[t]; q < v.aptr[t + 1]; ++q) {
+      const int p = v.aprod[q], pos = v.apos[q];
+      double f = P[p];
+      for (int u = a.pptr[p]; u < a.pptr[p + 1]; ++u)
+        if (u - a.pptr[p] != pos) f *= A[a.pidx[u]];
+      s += f;
+    }
+    dA[t] = s;
+  }
+  __syncthreads();
+  double grate = 0, gamp = 0;
+  for (int64_t c0 = e0; c0 < e1; c0 += kWave) {
+    const int64_t e = c0 + lane;
+    int zj = -1;
+    for (int nn = 0; nn < a.nr; ++nn) Tsh[nn * 64 + lane] = 0;
+    if (e < e1) {
+      const double dx = a.R[3 * e], dy = a.R[3 * e + 1], dz = a.R[3 * e + 2];
+      const double r = sqrt(dx * dx + dy * dy + dz * dz);
+      double gx = 0, gy = 0, gz = 0;
+      if (r > 0) {
+        zj = a.species[a.nbr[e]];
+        const double* sc = a.scale + a.nr * (zi + a.nz * zj);
+        const double x = r / a.rc, q = fmax(1.0 - x, 0.0), q2 = q * q;
+        const double ex = exp(-a.rate * r), base = q2 * q2 * ex;
+        for (int nn = 0; nn < a.nr; ++nn) {
+          double xn = 1;
+          for (int p = 0; p < a.powers[nn]; ++p) xn *= x;
+          Ush[nn * 64 + lane] = base * xn;
+          Rsh[nn * 64 + lane] = a.amplitude * base * xn * sc[nn];
+        }
+        ylm_emit<double, L>(dx, dy, dz, Fl, 0, [&](int h, const Jet<double>& jv) {
+          for (int u = v.hptr[h]; u < v.hptr[h + 1]; ++u) {
+            const int t = v.hidx[u], zc = a.dz[t];
+            if (zc == 0 || zc == zj + 1) {
+              const int rn = a.dn[t];
+              const double w = dA[t];
+              Tsh[rn * 64 + lane] += w * jv.v;
+              const double cf = w * Rsh[rn * 64 + lane];
+              gx += cf * jv.g0;
+              gy += cf * jv.g1;
+              gz += cf * jv.g2;
+            }
+          }
+        });
+        const double ri = 1.0 / r, ux = dx * ri, uy = dy * ri, uz = dz * ri;
+        double gr = 0;
+        for (int nn = 0; nn < a.nr; ++nn) {
+          const int pw = a.powers[nn];
+          double xn = 1, xm = 0;  // x^p and p x^(p-1)
+          for (int p = 0; p < pw; ++p) {
+            xm = xm * x + xn;
+            xn *= x;
+          }
+          const double T = Tsh[nn * 64 + lane], U = Ush[nn * 64 + lane];
+          const double dU = ex * ((xm * q - 4.0 * xn) * q2 * q / a.rc - a.rate * xn * q2 * q2);
+          gr += a.amplitude * sc[nn] * dU * T;
+          grate -= sc[nn] * T * r * a.amplitude * U;
+          gamp += sc[nn] * T * U;
+        }
+        gx += gr * ux;
+        gy += gr * uy;
+        gz += gr * uz;
+      }
+      v.gE[3 * e] = gx;
+      v.gE[3 * e + 1] = gy;
+      v.gE[3 * e + 2] = gz;
+    }
+    Zsh[lane] = zj;
+    __syncthreads();
+    const int ne = (int)min((int64_t)kWave, e1 - c0);
+    for (int t = lane; t < a.nr * a.nz; t += kWave) {  // scale partials by neighbour species
+      const int nn = t % a.nr, z = t / a.nr;
+      double s = 0;
+      for (int u = 0; u < ne; ++u)
+        if (Zsh[u] == z) s += Tsh[nn * 64 + u] * a.amplitude * Ush[nn * 64 + u];
+      Gsc[t] += s;
+    }
+    __syncthreads();
+  }
+  for (int t = lane; t < a.nr * a.nz; t += kWave) row[v.so + t] = Gsc[t];
+  grate = wave_sum(grate);
+  gamp = wave_sum(gamp);
+  if (lane == 0) {
+    row[v.so + a.nr * a.nz] = grate;
+    row[v.so + a.nr * a.nz + 1] = gamp;
+  }
+}
+
+// Position gradient of each atom (original order) from the edge vectors g:
+// dx_i = sum over its edges e of (g of the reverse edge - g_e), the reverse
+// of (i -> j, S) being (j -> i, -S) in j's edges; one wavefront per sorted
+// atom, lane per edge, fixed summation order. An edge whose reverse is absent
+// (a pair on the cutoff sphere to rounding) adds its g to `orphan` atomically
+// and sets status bit 2.
+__global__ __launch_bounds__(64) void k_features_dx(int64_t n, const int64_t* __restrict__ off,
+                                                    const int32_t* __restrict__ nbr, const int32_t* __restrict__ S,
+                                                    const int32_t* __restrict__ perm, const int32_t* __restrict__ inv,
+                                                    const double* __restrict__ gE, double* __restrict__ dx,
+                                                    double* __restrict__ orphan, int32_t* __restrict__ status) {
+  const int64_t k = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int i = perm[k];
+  double sx = 0, sy = 0, sz = 0;
+  for (int64_t e = off[k] + lane; e < off[k + 1]; e += kWave) {
+    sx -= gE[3 * e];
+    sy -= gE[3 * e + 1];
+    sz -= gE[3 * e + 2];
+    const int j = nbr[e];
+    const int s0 = -S[3 * e], s1 = -S[3 * e + 1], s2 = -S[3 * e + 2];
+    const int kj = inv[j];
+    bool found = false;
+    for (int64_t f = off[kj]; f < off[kj + 1]; ++f)
+      if (nbr[f] == i && S[3 * f] == s0 && S[3 * f + 1] == s1 && S[3 * f + 2] == s2) {
+        sx += gE[3 * f];
+        sy += gE[3 * f + 1];
+        sz += gE[3 * f + 2];
+        found = true;
+        break;
+      }
+    if (!found) {
+      atomicAdd(&orphan[3 * (int64_t)j], gE[3 * e]);
+      atomicAdd(&orphan[3 * (int64_t)j + 1], gE[3 * e + 1]);
+      atomicAdd(&orphan[3 * (int64_t)j + 2], gE[3 * e + 2]);
+      atomicOr(status, 2);
+    }
+  }
+  sx = wave_sum(sx);
+  sy = wave_sum(sy);
+  sz = wave_sum(sz);
+  if (lane == 0) {
+    dx[3 * (int64_t)i] = sx;
+    dx[3 * (int64_t)i + 1] = sy;
+    dx[3 * (int64_t)i + 2] = sz;
+  }
+}
+
+__global__ void k_inverse_perm(int64_t n, const int32_t* __restrict__ perm, int32_t* __restrict__ inv) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) inv[perm[k]] = (int32_t)k;
+}
+
+// Per-species column sums of the per-atom parameter rows: chunk c of
+// kRowChunk atoms gives part[(c nz + z) wp + col]; k_species_total sums the
+// chunks in order. Both fixed order.
+constexpr int kRowChunk = 1024;
+__global__ void k_species_rows(int64_t n, int wp, int nz, const double* __restrict__ prow,
+                               const int32_t* __restrict__ species, double* __restrict__ part) {
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= wp) return;
+  const int64_t c = blockIdx.y, a0 = c * kRowChunk, a1 = min(n, a0 + kRowChunk);
+  for (int z = 0; z < nz; ++z) {
+    double s = 0;
+    for (int64_t t = a0; t < a1; ++t)
+      if (species[t] == z) s += prow[t * wp + col];
+    part[(c * nz + z) * wp + col] = s;
+  }
+}
+
+__global__ void k_species_total(int64_t nchunks, int64_t width, const double* __restrict__ part,
+                                double* __restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= width) return;
+  double s = 0;
+  for (int64_t c = 0; c < nchunks; ++c) s += part[c * width + t];
+  out[t] = s;
+}
+
 template <typename T>
 static T* upload_raw(const T* p, size_t count, std::vector<void*>& owned) {
   T* d = nullptr;
@@ -1973,8 +2235,35 @@ static T* upload_raw(const T* p, size_t count, std::vector<void*>& owned) {
   return d;
 }
 
-void features_run(pfw_ctx* ctx, const pfw_feature_layer* l, double* out) {
-  if (!l || !out) throw Error(PFW_INVALID_ARGUMENT, "null feature layer or output");
+namespace {
+// A validated feature layer with its tables on the device and the current
+// system's pairs exported (when it has atoms).
+struct FeatPlan {
+  FeatArgs a{};
+  DeviceExport ex;
+  std::vector<void*> owned;
+  std::vector<int64_t> boff;
+  int64_t n = 0, nrows = 0, nmix = 0, nout = 0;
+  int Lk = 2;
+  size_t nyk = 9;
+  ~FeatPlan() {
+    for (void* q : owned) (void)hipFree(q);
+  }
+  template <typename T>
+  T* up(const T* p, size_t count) {
+    return upload_raw(p, count, owned);
+  }
+  template <typename T>
+  T* alloc(size_t count) {
+    T* d = nullptr;
+    PFW_HIP(hipMalloc((void**)&d, std::max<size_t>(count, 1) * sizeof(T)));
+    owned.push_back(d);
+    return d;
+  }
+};
+
+void feat_prepare(pfw_ctx* ctx, const pfw_feature_layer* l, FeatPlan& p, bool with_S) {
+  if (!l) throw Error(PFW_INVALID_ARGUMENT, "null feature layer");
   if (l->radial_kind != PFW_RADIAL_SMOOTH_EXPONENTIAL)
     throw Error(PFW_INVALID_ARGUMENT, "unsupported feature radial family");
   if (l->nz <= 0 || l->nr <= 0 || l->lmax < 0 || l->ndens < 0 || l->nprod < 0 || l->nblocks <= 0)
@@ -1995,25 +2284,24 @@ void features_run(pfw_ctx* ctx, const pfw_feature_layer* l, double* out) {
     for (int q = l->prod_ptr[t]; q < l->prod_ptr[t + 1]; ++q)
       if (l->prod_idx[q] < 0 || l->prod_idx[q] >= l->ndens) throw Error(PFW_INVALID_ARGUMENT, "invalid product factor");
   }
-  int64_t nrows = 0, nmix = 0;
-  const int64_t n = ctx->sys.n;
-  std::vector<int64_t> boff(6 * (size_t)l->nblocks);
-  int64_t nout = 0;
+  p.n = ctx->sys.n;
+  p.boff.assign(6 * (size_t)l->nblocks, 0);
   for (int b = 0; b < l->nblocks; ++b) {
     const int dim = 2 * l->block_L[b] + 1, nb = l->block_nbasis[b], ch = l->block_channels[b];
     if (l->block_L[b] < 0 || nb <= 0 || ch <= 0) throw Error(PFW_INVALID_ARGUMENT, "invalid feature block");
-    boff[6 * b] = nrows;
-    boff[6 * b + 1] = nmix;
-    boff[6 * b + 2] = nout;
-    boff[6 * b + 3] = dim;
-    boff[6 * b + 4] = nb;
-    boff[6 * b + 5] = ch;
-    nrows += (int64_t)dim * nb;
-    nmix += (int64_t)nb * ch * l->nz;
-    nout += (int64_t)dim * ch * n;
+    int64_t* o = p.boff.data() + 6 * b;
+    o[0] = p.nrows;
+    o[1] = p.nmix;
+    o[2] = p.nout;
+    o[3] = dim;
+    o[4] = nb;
+    o[5] = ch;
+    p.nrows += (int64_t)dim * nb;
+    p.nmix += (int64_t)nb * ch * l->nz;
+    p.nout += (int64_t)dim * ch * p.n;
   }
   if (l->row_ptr[0] != 0) throw Error(PFW_INVALID_ARGUMENT, "invalid coupling row offsets");
-  for (int64_t r = 0; r < nrows; ++r) {
+  for (int64_t r = 0; r < p.nrows; ++r) {
     if (l->row_ptr[r + 1] < l->row_ptr[r]) throw Error(PFW_INVALID_ARGUMENT, "invalid coupling row offsets");
     for (int q = l->row_ptr[r]; q < l->row_ptr[r + 1]; ++q)
       if (l->col[q] < 0 || l->col[q] >= l->nprod || !std::isfinite(l->val[q]))
@@ -2021,85 +2309,209 @@ void features_run(pfw_ctx* ctx, const pfw_feature_layer* l, double* out) {
   }
   for (int t = 0; t < l->nr; ++t)
     if (l->powers[t] < 0) throw Error(PFW_INVALID_ARGUMENT, "negative radial power");
-  const int Lk = l->lmax <= 2 ? 2 : l->lmax <= 4 ? 4 : 6;  // the kernel's harmonic degree
-  const size_t nyk = (size_t)(Lk + 1) * (Lk + 1);
-  const size_t lds = sizeof(double) * (nyk + (nyk + l->nr) * 64 + l->ndens + l->nprod + nrows) + 64 * sizeof(int);
-  if (lds > kMaxLds) throw Error(PFW_INVALID_ARGUMENT, "feature tables too large for LDS");
-  std::vector<int32_t> sp(n);
-  if (n > 0) {
-    PFW_HIP(hipMemcpy(sp.data(), ctx->sys.species, n * sizeof(int32_t), hipMemcpyDeviceToHost));
-    for (int64_t t = 0; t < n; ++t)
+  p.Lk = l->lmax <= 2 ? 2 : l->lmax <= 4 ? 4 : 6;  // the kernels' harmonic degree
+  p.nyk = (size_t)(p.Lk + 1) * (p.Lk + 1);
+  if (p.n > 0) {
+    std::vector<int32_t> sp(p.n);
+    PFW_HIP(hipMemcpy(sp.data(), ctx->sys.species, p.n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    for (int64_t t = 0; t < p.n; ++t)
       if (sp[t] < 0 || sp[t] >= l->nz) throw Error(PFW_INVALID_ARGUMENT, "species channel outside the feature layer");
   }
-  std::fill(out, out + nout, 0.0);
-  if (n == 0) return;
-  DeviceExport ex;
-  nl_export_device(ctx, ex, false);
-  std::vector<void*> owned;
-  struct Free {
-    std::vector<void*>& o;
-    ~Free() {
-      for (void* q : o) (void)hipFree(q);
-    }
-  } guard{owned};
-  const int64_t nval = l->row_ptr[nrows];
-  FeatArgs a;
-  a.n = n;
-  a.off = ex.off;
-  a.nbr = ex.j;
-  a.R = ex.R;
+  if (p.n == 0) return;
+  nl_export_device(ctx, p.ex, with_S);
+  const int64_t nval = l->row_ptr[p.nrows];
+  FeatArgs& a = p.a;
+  a.n = p.n;
+  a.off = p.ex.off;
+  a.nbr = p.ex.j;
+  a.R = p.ex.R;
   a.perm = ctx->sys.perm;
   a.species = ctx->sys.species;
   a.nz = l->nz;
   a.nr = l->nr;
   a.ndens = l->ndens;
   a.nprod = l->nprod;
-  a.nrows = (int)nrows;
+  a.nrows = (int)p.nrows;
   a.nblocks = l->nblocks;
-  a.dz = upload_raw(l->dens_z, l->ndens, owned);
-  a.dn = upload_raw(l->dens_n, l->ndens, owned);
-  a.dh = upload_raw(l->dens_h, l->ndens, owned);
-  a.pptr = upload_raw(l->prod_ptr, (size_t)l->nprod + 1, owned);
-  a.pidx = upload_raw(l->prod_idx, (size_t)l->prod_ptr[l->nprod], owned);
-  a.rptr = upload_raw(l->row_ptr, (size_t)nrows + 1, owned);
-  a.col = upload_raw(l->col, (size_t)nval, owned);
-  a.val = upload_raw(l->val, (size_t)nval, owned);
-  a.boff = upload_raw(boff.data(), boff.size(), owned);
-  a.mix = upload_raw(l->mixing, (size_t)nmix, owned);
+  a.dz = p.up(l->dens_z, l->ndens);
+  a.dn = p.up(l->dens_n, l->ndens);
+  a.dh = p.up(l->dens_h, l->ndens);
+  a.pptr = p.up(l->prod_ptr, (size_t)l->nprod + 1);
+  a.pidx = p.up(l->prod_idx, (size_t)l->prod_ptr[l->nprod]);
+  a.rptr = p.up(l->row_ptr, (size_t)p.nrows + 1);
+  a.col = p.up(l->col, (size_t)nval);
+  a.val = p.up(l->val, (size_t)nval);
+  a.boff = p.up(p.boff.data(), p.boff.size());
+  a.mix = p.up(l->mixing, (size_t)p.nmix);
   a.rc = l->radial_cutoff;
   a.rate = l->rate;
   a.amplitude = l->amplitude;
-  a.powers = upload_raw(l->powers, (size_t)l->nr, owned);
-  a.scale = upload_raw(l->scale, (size_t)l->nr * l->nz * l->nz, owned);
-  double* dout = nullptr;
-  PFW_HIP(hipMalloc(&dout, std::max<int64_t>(nout, 1) * sizeof(double)));
-  owned.push_back(dout);
-  a.out = dout;
-  int32_t* status = nullptr;
-  PFW_HIP(hipMalloc(&status, sizeof(int32_t)));
-  owned.push_back(status);
-  hipStream_t st = ctx->stream;
-  PFW_HIP(hipMemsetAsync(status, 0, sizeof(int32_t), st));
-  a.status = status;
+  a.powers = p.up(l->powers, (size_t)l->nr);
+  a.scale = p.up(l->scale, (size_t)l->nr * l->nz * l->nz);
+  a.status = p.alloc<int32_t>(1);
+  PFW_HIP(hipMemsetAsync(a.status, 0, sizeof(int32_t), ctx->stream));
+}
+
+template <template <int> class K, typename... Args>
+void launch_lk(int Lk, size_t lds, int64_t n, hipStream_t st, Args... args) {
   switch (Lk) {
     case 2:
-      allow_lds(k_features<2>, lds, "the feature kernel");
-      hipLaunchKernelGGL(k_features<2>, dim3((unsigned)n), dim3(kWave), lds, st, a);
+      allow_lds(K<2>::fn, lds, "the feature kernels");
+      hipLaunchKernelGGL(K<2>::fn, dim3((unsigned)n), dim3(kWave), lds, st, args...);
       break;
     case 4:
-      allow_lds(k_features<4>, lds, "the feature kernel");
-      hipLaunchKernelGGL(k_features<4>, dim3((unsigned)n), dim3(kWave), lds, st, a);
+      allow_lds(K<4>::fn, lds, "the feature kernels");
+      hipLaunchKernelGGL(K<4>::fn, dim3((unsigned)n), dim3(kWave), lds, st, args...);
       break;
     default:
-      allow_lds(k_features<6>, lds, "the feature kernel");
-      hipLaunchKernelGGL(k_features<6>, dim3((unsigned)n), dim3(kWave), lds, st, a);
+      allow_lds(K<6>::fn, lds, "the feature kernels");
+      hipLaunchKernelGGL(K<6>::fn, dim3((unsigned)n), dim3(kWave), lds, st, args...);
   }
   PFW_HIP(hipGetLastError());
+}
+template <int L>
+struct FeatFwd {
+  static constexpr auto fn = k_features<L>;
+};
+template <int L>
+struct FeatVjp {
+  static constexpr auto fn = k_features_vjp<L>;
+};
+}  // namespace
+
+void features_run(pfw_ctx* ctx, const pfw_feature_layer* l, double* out) {
+  if (!out) throw Error(PFW_INVALID_ARGUMENT, "null feature output");
+  FeatPlan p;
+  feat_prepare(ctx, l, p, false);
+  const size_t lds =
+      sizeof(double) * (p.nyk + (p.nyk + l->nr) * 64 + l->ndens + l->nprod + p.nrows) + 64 * sizeof(int);
+  if (lds > kMaxLds) throw Error(PFW_INVALID_ARGUMENT, "feature tables too large for LDS");
+  std::fill(out, out + p.nout, 0.0);
+  if (p.n == 0) return;
+  double* dout = p.alloc<double>(p.nout);
+  p.a.out = dout;
+  hipStream_t st = ctx->stream;
+  launch_lk<FeatFwd>(p.Lk, lds, p.n, st, p.a);
   int32_t h_status = 0;
-  PFW_HIP(hipMemcpyAsync(&h_status, status, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-  PFW_HIP(hipMemcpyAsync(out, dout, nout * sizeof(double), hipMemcpyDeviceToHost, st));
+  PFW_HIP(hipMemcpyAsync(&h_status, p.a.status, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  PFW_HIP(hipMemcpyAsync(out, dout, p.nout * sizeof(double), hipMemcpyDeviceToHost, st));
   PFW_HIP(hipStreamSynchronize(st));
   if (h_status) throw Error(PFW_DOMAIN, "coincident physical atoms have undefined harmonics");
+}
+
+void features_vjp_run(pfw_ctx* ctx, const pfw_feature_layer* l, const double* cot, double* dx, double* dmixing,
+                      double* dscale, double* dradial) {
+  if (!cot || !dx || !dmixing || !dscale || !dradial) throw Error(PFW_INVALID_ARGUMENT, "null feature pullback buffer");
+  FeatPlan p;
+  feat_prepare(ctx, l, p, true);
+  const int nz = l->nz, nr = l->nr;
+  const int64_t nsc = (int64_t)nr * nz * nz;
+  const size_t lds = sizeof(double) * (p.nyk + (p.nyk + 3 * (size_t)nr) * 64 + 2 * (size_t)l->ndens + l->nprod +
+                                       p.nrows + (size_t)nr * nz) +
+                     64 * sizeof(int);
+  if (lds > kMaxLds) throw Error(PFW_INVALID_ARGUMENT, "feature tables too large for LDS");
+  std::fill(dx, dx + 3 * p.n, 0.0);
+  std::fill(dmixing, dmixing + p.nmix, 0.0);
+  std::fill(dscale, dscale + nsc, 0.0);
+  dradial[0] = dradial[1] = 0;
+  if (p.n == 0) return;
+  for (int64_t t = 0; t < p.nout; ++t)
+    if (!std::isfinite(cot[t])) throw Error(PFW_INVALID_ARGUMENT, "nonfinite feature cotangent");
+  // coupling columns, density -> (product, position) and harmonic -> densities
+  const int nprod = l->nprod, ndens = l->ndens;
+  std::vector<int32_t> cptr(nprod + 1, 0), crow(l->row_ptr[p.nrows]);
+  std::vector<double> cval(crow.size());
+  for (int64_t r = 0; r < p.nrows; ++r)
+    for (int q = l->row_ptr[r]; q < l->row_ptr[r + 1]; ++q) ++cptr[l->col[q] + 1];
+  for (int t = 0; t < nprod; ++t) cptr[t + 1] += cptr[t];
+  {
+    std::vector<int32_t> fill(cptr.begin(), cptr.end() - 1);
+    for (int64_t r = 0; r < p.nrows; ++r)  // rows ascending within each column
+      for (int q = l->row_ptr[r]; q < l->row_ptr[r + 1]; ++q) {
+        const int at = fill[l->col[q]]++;
+        crow[at] = (int32_t)r;
+        cval[at] = l->val[q];
+      }
+  }
+  std::vector<int32_t> aptr(ndens + 1, 0), aprod(l->prod_ptr[nprod]), apos(aprod.size());
+  for (int q = 0; q < l->prod_ptr[nprod]; ++q) ++aptr[l->prod_idx[q] + 1];
+  for (int t = 0; t < ndens; ++t) aptr[t + 1] += aptr[t];
+  {
+    std::vector<int32_t> fill(aptr.begin(), aptr.end() - 1);
+    for (int k = 0; k < nprod; ++k)
+      for (int q = l->prod_ptr[k]; q < l->prod_ptr[k + 1]; ++q) {
+        const int at = fill[l->prod_idx[q]]++;
+        aprod[at] = k;
+        apos[at] = q - l->prod_ptr[k];
+      }
+  }
+  std::vector<int32_t> hptr(p.nyk + 1, 0), hidx(ndens);
+  for (int t = 0; t < ndens; ++t) ++hptr[l->dens_h[t] + 1];
+  for (size_t h = 0; h < p.nyk; ++h) hptr[h + 1] += hptr[h];
+  {
+    std::vector<int32_t> fill(hptr.begin(), hptr.end() - 1);
+    for (int t = 0; t < ndens; ++t) hidx[fill[l->dens_h[t]]++] = t;
+  }
+  FeatVjpArgs v;
+  v.cot = p.up(cot, (size_t)p.nout);
+  v.cptr = p.up(cptr.data(), cptr.size());
+  v.crow = p.up(crow.data(), crow.size());
+  v.cval = p.up(cval.data(), cval.size());
+  v.aptr = p.up(aptr.data(), aptr.size());
+  v.aprod = p.up(aprod.data(), aprod.size());
+  v.apos = p.up(apos.data(), apos.size());
+  v.hptr = p.up(hptr.data(), hptr.size());
+  v.hidx = p.up(hidx.data(), hidx.size());
+  const int64_t total = p.ex.total;
+  v.gE = p.alloc<double>(3 * (size_t)total);
+  v.so = (int)(p.nmix / nz);
+  v.wp = v.so + nr * nz + 2;
+  v.prow = p.alloc<double>((size_t)p.n * v.wp);
+  hipStream_t st = ctx->stream;
+  launch_lk<FeatVjp>(p.Lk, lds, p.n, st, p.a, v);
+  int32_t* inv = p.alloc<int32_t>(p.n);
+  hipLaunchKernelGGL(k_inverse_perm, dim3((unsigned)((p.n + 255) / 256)), dim3(256), 0, st, p.n, (const int32_t*)ctx->sys.perm,
+                     inv);
+  double* d_dx = p.alloc<double>(3 * (size_t)p.n);
+  double* orphan = p.alloc<double>(3 * (size_t)p.n);
+  PFW_HIP(hipMemsetAsync(orphan, 0, 3 * p.n * sizeof(double), st));
+  hipLaunchKernelGGL(k_features_dx, dim3((unsigned)p.n), dim3(kWave), 0, st, p.n, (const int64_t*)p.ex.off,
+                     (const int32_t*)p.ex.j, (const int32_t*)p.ex.S, (const int32_t*)ctx->sys.perm,
+                     (const int32_t*)inv, (const double*)v.gE, d_dx, orphan, p.a.status);
+  PFW_HIP(hipGetLastError());
+  const int64_t nchunks = (p.n + kRowChunk - 1) / kRowChunk, width = (int64_t)nz * v.wp;
+  double* part = p.alloc<double>((size_t)nchunks * width);
+  double* dsum = p.alloc<double>((size_t)width);
+  hipLaunchKernelGGL(k_species_rows, dim3((unsigned)((v.wp + 63) / 64), (unsigned)nchunks), dim3(64), 0, st, p.n,
+                     v.wp, nz, (const double*)v.prow, (const int32_t*)ctx->sys.species, part);
+  hipLaunchKernelGGL(k_species_total, dim3((unsigned)((width + 255) / 256)), dim3(256), 0, st, nchunks, width,
+                     (const double*)part, dsum);
+  PFW_HIP(hipGetLastError());
+  int32_t h_status = 0;
+  std::vector<double> hsum(width), horph(3 * p.n);
+  PFW_HIP(hipMemcpyAsync(&h_status, p.a.status, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  PFW_HIP(hipMemcpyAsync(dx, d_dx, 3 * p.n * sizeof(double), hipMemcpyDeviceToHost, st));
+  PFW_HIP(hipMemcpyAsync(hsum.data(), dsum, width * sizeof(double), hipMemcpyDeviceToHost, st));
+  PFW_HIP(hipMemcpyAsync(horph.data(), orphan, 3 * p.n * sizeof(double), hipMemcpyDeviceToHost, st));
+  PFW_HIP(hipStreamSynchronize(st));
+  if (h_status & 1) throw Error(PFW_DOMAIN, "coincident physical atoms have undefined harmonics");
+  if (h_status & 2)
+    for (int64_t t = 0; t < 3 * p.n; ++t) dx[t] += horph[t];
+  // species rows -> mixing (basis, channels, species per block), scale (radial, centre, neighbour), rate, amplitude
+  for (int z = 0; z < nz; ++z) {
+    const double* s = hsum.data() + (int64_t)z * v.wp;
+    int64_t mo = 0;
+    for (int b = 0; b < l->nblocks; ++b) {
+      const int64_t* o = p.boff.data() + 6 * b;
+      const int64_t w = o[4] * o[5];
+      for (int64_t t = 0; t < w; ++t) dmixing[o[1] + w * z + t] = s[mo + t];
+      mo += w;
+    }
+    for (int zj = 0; zj < nz; ++zj)
+      for (int nn = 0; nn < nr; ++nn) dscale[nn + (int64_t)nr * (z + (int64_t)nz * zj)] = s[v.so + nn + nr * zj];
+    dradial[0] += s[v.so + nr * nz];
+    dradial[1] += s[v.so + nr * nz + 1];
+  }
 }
 
 // Force rows of one output column: atom i (original order) of configuration c

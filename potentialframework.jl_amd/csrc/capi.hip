@@ -417,6 +417,16 @@ int pfw_features(pfw_ctx* ctx, const pfw_feature_layer* layer, double* out) {
   });
 }
 
+int pfw_features_vjp(pfw_ctx* ctx, const pfw_feature_layer* layer, const double* cotangents, double* dx,
+                     double* dmixing, double* dscale, double* dradial) {
+  return run(ctx, [&] {
+    if (!ctx->have_system) throw Error(PFW_STALE, "no system set");
+    if (ctx->ncfg != 1) throw Error(PFW_INVALID_ARGUMENT, "features are evaluated for one configuration");
+    ctx->have_results = false;
+    pfw::features_vjp_run(ctx, layer, cotangents, dx, dmixing, dscale, dradial);
+  });
+}
+
 int pfw_pair_adjoint(pfw_ctx* ctx, const double* dE, const double* dF, const double* dW, double* gA) {
   return run(ctx, [&] {
     if (!ctx->have_system) throw Error(PFW_STALE, "no system set");

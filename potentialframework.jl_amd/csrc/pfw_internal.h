@@ -304,6 +304,8 @@ void ace_site_energies(pfw_ctx* ctx, double* out);
 void ace_parameter_sets(pfw_ctx* ctx, int precision, int64_t nsets, const double* WB, const double* Wpair,
                         double* out);
 void features_run(pfw_ctx* ctx, const pfw_feature_layer* layer, double* out);
+void features_vjp_run(pfw_ctx* ctx, const pfw_feature_layer* layer, const double* cot, double* dx, double* dmixing,
+                      double* dscale, double* dradial);
 void ace_adjoint(pfw_ctx* ctx, int precision, const double* Ebar, const double* Fbar, const double* Wbar, double* gWB,
                  double* gWpair);
 }  // namespace pfw

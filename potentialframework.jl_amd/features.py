@@ -294,13 +294,10 @@ def _check(layer, ps):
             raise DimensionMismatch(f"mixing block {b + 1}")
 
 
-def ace_features(layer, system, ps, workspace=None):
-    """``ace_features(X, layer, ps, environment, H)`` (features.jl:164-178) on
-    the device: one (2L+1, channels, n) array per irrep, input atom order."""
+def _prepared(layer, system, ps, workspace):
+    """Checks the parameters, sets the system on the workspace at the layer's
+    cutoff and packs the layer's device tables."""
     from .neighbourlist import Workspace
-    _check(layer, ps)
-    if layer.resolved is not None:
-        return _resolved_features(layer, system, ps)
     w = workspace or Workspace(precision="float64")
     nz = len(layer.atomic_numbers)
     ch = _channels(layer, system)
@@ -314,6 +311,16 @@ def ace_features(layer, system, ps, workspace=None):
     if scale.shape != (nr, nz, nz):
         raise DimensionMismatch("radial scale shape")
     st, keep = _struct(layer, ps, rate[0], amp[0], scale)
+    return w, st, keep
+
+
+def ace_features(layer, system, ps, workspace=None):
+    """``ace_features(X, layer, ps, environment, H)`` (features.jl:164-178) on
+    the device: one (2L+1, channels, n) array per irrep, input atom order."""
+    _check(layer, ps)
+    if layer.resolved is not None:
+        return _resolved_features(layer, system, ps)
+    w, st, keep = _prepared(layer, system, ps, workspace)
     n = len(system)
     sizes = [(2 * s.L + 1) * s.channels * n for s in layer.irreps]
     out = np.zeros(sum(sizes))
@@ -324,6 +331,66 @@ def ace_features(layer, system, ps, workspace=None):
         res.append(out[o:o + k].reshape(n, s.channels, 2 * s.L + 1).transpose(2, 1, 0).copy())
         o += k
     return tuple(res)
+
+
+def feature_vjp(layer, system, ps, cotangents, workspace=None):
+    """``feature_vjp(X, layer, ps, environment, H, cotangents)`` (features.jl:204-225)
+    on the device: the gradient of sum_b <cotangents[b], features[b]> with
+    respect to the positions ((n, 3), input order) and the parameters (the
+    structure of ``ps``: mixing blocks, radial scale, custom_radial rate and
+    amplitude). A cotangent may be None (a zero block)."""
+    _check(layer, ps)
+    if layer.resolved is not None:
+        raise ArgumentError("the feature pullback needs a layer with smooth exponential radials")
+    n = len(system)
+    blocks = []
+    for s, c in zip(layer.irreps, cotangents):
+        shape = (2 * s.L + 1, s.channels, n)
+        c = np.zeros(shape) if c is None else np.asarray(c, float)
+        if c.shape != shape:
+            raise DimensionMismatch(f"cotangent of block L={s.L} has shape {c.shape}, expected {shape}")
+        blocks.append(c.transpose(2, 1, 0).reshape(-1))
+    if len(blocks) != len(layer.irreps):
+        raise DimensionMismatch("one cotangent per feature block")
+    cot = np.ascontiguousarray(np.concatenate(blocks))
+    w, st, keep = _prepared(layer, system, ps, workspace)
+    nz, nr = len(layer.atomic_numbers), len(layer.radial_spec)
+    dx = np.zeros(3 * n)
+    dmix = np.zeros(sum(layer.basis_channels(b) * s.channels * nz for b, s in enumerate(layer.irreps)))
+    dscale = np.zeros(nr * nz * nz)
+    drad = np.zeros(2)
+    w.ctx.call("pfw_features_vjp", ctypes.byref(st), ptr(cot), ptr(dx), ptr(dmix), ptr(dscale), ptr(drad))
+    del keep
+    mixing, o = {}, 0
+    for b, s in enumerate(layer.irreps):
+        nb = layer.basis_channels(b)
+        k = nb * s.channels * nz
+        mixing[f"block_{b + 1}"] = dmix[o:o + k].reshape(nz, s.channels, nb).transpose(2, 1, 0).copy()
+        o += k
+    custom = {}
+    for key, val in zip(("rate", "amplitude"), drad):
+        if key in ps["custom_radial"]:
+            custom[key] = np.full(np.shape(ps["custom_radial"][key]), val)
+    return {"positions": dx.reshape(n, 3),
+            "parameters": {"mixing": mixing, "radial": {"scale": dscale.reshape(nz, nz, nr).transpose(2, 1, 0).copy()},
+                           "custom_radial": custom}}
+
+
+def ace_features_rrule(layer, system, ps, workspace=None):
+    """``rrule(ace_features, ...)`` (ext/PotentialFrameWorkChainRulesCoreExt.jl:77-98): the features and a
+    pullback over the saved positions and parameters (copies, so that later
+    changes by the caller do not alter it). The pullback takes one cotangent
+    per block (None for a zero block) and returns the ``feature_vjp`` dict."""
+    import copy
+    y = ace_features(layer, system, ps, workspace)
+    saved_sys, saved_ps = copy.deepcopy(system), copy.deepcopy(ps)
+
+    def pullback(cotangents):
+        if cotangents is None:
+            return None
+        return feature_vjp(layer, saved_sys, saved_ps, cotangents, workspace)
+
+    return y, pullback
 
 
 def _channels(layer, system):
