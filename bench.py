@@ -354,6 +354,17 @@ def bench_features(pfw, args):
            "coupling_nonzeros": nnz, "blocks": [(s.L, s.parity, s.channels) for s in layer.irreps],
            "value": round(len(X) / secs / 1e6, 4), "unit": "Matom-features/s", "ms_per_call": round(1e3 * secs, 3),
            "call": "fresh cell list + pair export on the device, feature kernel, outputs to host"}
+    y = pfw.ace_features(layer, sys_, ps, workspace=w)
+    cot = [np.sin(np.arange(1, b.size + 1)).reshape(b.shape, order="F") for b in y]
+    pfw.feature_vjp(layer, sys_, ps, cot, workspace=w)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        pfw.feature_vjp(layer, sys_, ps, cot, workspace=w)
+    vsecs = (time.perf_counter() - t0) / reps
+    out["pullback"] = {"value": round(len(X) / vsecs / 1e6, 4), "unit": "Matom-pullbacks/s",
+                       "ms_per_call": round(1e3 * vsecs, 3),
+                       "call": "cotangents to the device, fresh cell list + pair export, per-atom reverse kernel, "
+                               "position gather, species sums of the parameter gradients, gradients to host"}
     if not args.no_cpu_baseline:
         sys.path.insert(0, ROOT)
         from oracle.features import ace_features as oracle_features

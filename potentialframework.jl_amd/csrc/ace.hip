@@ -1866,6 +1866,14 @@ void ace_run(pfw_ctx* ctx, uint32_t mask, int precision) {
 // and neighbour species in LDS; each lane then sums, for its densities, the
 // chunk's edges in order (species-resolved channel 0 = any species); then the
 // products, the coupling rows and the mixed outputs, lanes over entries.
+// One coupling nonzero: its value and the density indices of its product's
+// factors, 16 bits each, unused factors pointing at the constant 1 that
+// follows the densities (products of at most four factors).
+struct FeatEntry {
+  double val;
+  uint64_t f;
+};
+
 struct FeatArgs {
   int64_t n;
   const int64_t* off;  // n + 1 edge offsets per sorted atom
@@ -1873,11 +1881,12 @@ struct FeatArgs {
   const double* R;     // 3 per edge: x_j + H s - x_i
   const int32_t* perm;
   const int32_t* species;  // original order, 0..nz-1
-  int nz, nr, ndens, nprod, nrows, nblocks;
+  int nz, nr, ndens, nrows, nblocks;
+  int64_t nnz;
   const int32_t *dz, *dn, *dh;
-  const int32_t *pptr, *pidx;
-  const int32_t *rptr, *col;
-  const double* val;
+  const int32_t* rptr;      // coupling row offsets (nrows + 1)
+  const FeatEntry* ent;     // coupling entries in row order, each with its product's factors
+  const int32_t* erow;      // row of each entry
   const int64_t* boff;  // per block: row offset, mixing offset, output offset, dim, nbasis, channels
   const double* mix;
   double rc, rate, amplitude;
@@ -1929,18 +1938,17 @@ __device__ inline void feat_densities(const FeatArgs& a, int64_t e0, int64_t e1,
   }
 }
 
-// Products P_k = prod A and coupled rows B = C P of one atom (LDS).
-__device__ inline void feat_rows(const FeatArgs& a, const double* A, double* P, double* B) {
+// Coupled rows B = C P of one atom, each product P_k = prod A formed where its
+// coupling entry is read (A[ndens] = 1 pads the factors).
+__device__ inline void feat_rows(const FeatArgs& a, const double* A, double* B) {
   const int lane = threadIdx.x;
-  for (int t = lane; t < a.nprod; t += kWave) {
-    double v = 1;
-    for (int q = a.pptr[t]; q < a.pptr[t + 1]; ++q) v *= A[a.pidx[q]];
-    P[t] = v;
-  }
-  __syncthreads();
   for (int t = lane; t < a.nrows; t += kWave) {
     double v = 0;
-    for (int q = a.rptr[t]; q < a.rptr[t + 1]; ++q) v += a.val[q] * P[a.col[q]];
+    for (int q = a.rptr[t]; q < a.rptr[t + 1]; ++q) {
+      const FeatEntry e = a.ent[q];
+      const double p = A[e.f & 0xffff] * A[(e.f >> 16) & 0xffff] * A[(e.f >> 32) & 0xffff] * A[e.f >> 48];
+      v += e.val * p;
+    }
     B[t] = v;
   }
   __syncthreads();
@@ -1953,18 +1961,17 @@ __global__ __launch_bounds__(64) void k_features(FeatArgs a) {
   double* Fl = (double*)fsm;   // (L+1)^2
   double* Ysh = Fl + NY;       // NY x 64
   double* Rsh = Ysh + NY * 64;  // nr x 64
-  double* A = Rsh + a.nr * 64;  // ndens
-  double* P = A + a.ndens;      // nprod
-  double* B = P + a.nprod;      // nrows
+  double* A = Rsh + a.nr * 64;  // ndens + 1 (the last is 1)
+  double* B = A + a.ndens + 1;  // nrows
   int* Zsh = (int*)(B + a.nrows);  // 64
   const int lane = threadIdx.x;
   const int64_t k = blockIdx.x;
   const int i = a.perm[k], zi = a.species[i];
   fill_fl<double, L>(Fl);
-  for (int t = lane; t < a.ndens; t += kWave) A[t] = 0;
+  for (int t = lane; t <= a.ndens; t += kWave) A[t] = t < a.ndens ? 0.0 : 1.0;
   __syncthreads();
   feat_densities<L>(a, a.off[k], a.off[k + 1], zi, Fl, Ysh, Rsh, Zsh, A);
-  feat_rows(a, A, P, B);
+  feat_rows(a, A, B);
   for (int b = 0; b < a.nblocks; ++b) {
     const int64_t* o = a.boff + 6 * b;
     const int dim = (int)o[3], nb = (int)o[4], ch = (int)o[5];
@@ -1979,24 +1986,25 @@ __global__ __launch_bounds__(64) void k_features(FeatArgs a) {
 }
 
 // The pullback of the features (feature_vjp / feature_local_reverse,
-// features.jl:204-255) for one atom per wavefront: the forward densities,
-// products and rows again; the mixing partial B^T G; dB = G W^T over the rows;
-// product weights C^T vec(dB) through the coupling columns; dA by the product
-// rule through the density -> (product, factor) index; then per edge, lane per
-// edge, g = sum_a dA_a (R'_n Y_m r^ + R_n grad Y_m), the densities of each
-// harmonic visited as ylm_emit forms it. Per atom it writes one parameter row
-// (mixing partials, radial scale partials per neighbour species, rate and
-// amplitude partials); g goes to the edge.
+// features.jl:204-255) for one atom per wavefront: the forward densities and
+// rows again; the mixing partial B^T G; dB = G W^T over the rows; then dA by
+// the product rule, lanes over the coupling entries, each adding
+// val dB[row] dP/dA_f to its own column of a per-lane table that is summed
+// over lanes in a fixed order; then per edge, lane per edge,
+// g = sum_a dA_a (R'_n Y_m r^ + R_n grad Y_m), the densities of each harmonic
+// visited as ylm_emit forms it. Per atom it writes one parameter row (mixing
+// partials, radial scale partials per neighbour species, rate and amplitude
+// partials); g goes to the edge.
 struct FeatVjpArgs {
   const double* cot;  // the cotangents, laid out as the forward output
-  const int32_t *cptr, *crow;  // coupling columns: product -> rows
-  const double* cval;
-  const int32_t *aptr, *aprod, *apos;  // density -> (product, factor position)
-  const int32_t *hptr, *hidx;          // harmonic -> densities
+  const int32_t *hptr, *hidx;  // harmonic -> densities
   double* gE;    // 3 per edge
   double* prow;  // n x wp, original atom order
   int wp, so;    // row width; offset of the scale partials (after the mixing partials)
+  int gper;      // cotangents per atom (sum over blocks of dim x channels)
 };
+
+constexpr int kLaneStride = 65;  // per-lane dA table: column stride (odd: no bank conflicts in the sums)
 
 template <int L>
 __global__ __launch_bounds__(64) void k_features_vjp(FeatArgs a, FeatVjpArgs v) {
@@ -2007,28 +2015,37 @@ __global__ __launch_bounds__(64) void k_features_vjp(FeatArgs a, FeatVjpArgs v) 
   double* Rsh = Ysh + NY * 64;        // nr x 64: scaled radials
   double* Ush = Rsh + a.nr * 64;      // nr x 64: radials at unit amplitude and scale
   double* Tsh = Ush + a.nr * 64;      // nr x 64: sum of dA Y per radial
-  double* A = Tsh + a.nr * 64;        // ndens
-  double* dA = A + a.ndens;           // ndens
-  double* P = dA + a.ndens;           // nprod, then the product weights
-  double* B = P + a.nprod;            // nrows, then dB
-  double* Gsc = B + a.nrows;          // nr x nz: scale partials (radial, neighbour species)
-  int* Zsh = (int*)(Gsc + a.nr * a.nz);  // 64
+  double* A = Tsh + a.nr * 64;        // ndens + 1 (the last is 1)
+  double* dA = A + a.ndens + 1;       // ndens
+  double* B = dA + a.ndens;           // nrows, then dB
+  double* dAl = B + a.nrows;          // (ndens + 1) x kLaneStride
+  double* Gsc = dAl + (a.ndens + 1) * kLaneStride;  // nr x nz: scale partials (radial, neighbour species)
+  double* Gl = Gsc + a.nr * a.nz;     // the atom's cotangents (sum over blocks of dim x channels)
+  int* Zsh = (int*)(Gl + v.gper);     // 64
   const int lane = threadIdx.x;
   const int64_t k = blockIdx.x;
   const int i = a.perm[k], zi = a.species[i];
   const int64_t e0 = a.off[k], e1 = a.off[k + 1];
   fill_fl<double, L>(Fl);
-  for (int t = lane; t < a.ndens; t += kWave) A[t] = 0;
+  for (int t = lane; t <= a.ndens; t += kWave) A[t] = t < a.ndens ? 0.0 : 1.0;
   for (int t = lane; t < a.nr * a.nz; t += kWave) Gsc[t] = 0;
+  for (int t = 0; t <= a.ndens; ++t) dAl[t * kLaneStride + lane] = 0;
+  for (int b = 0, g0 = 0; b < a.nblocks; ++b) {
+    const int64_t* o = a.boff + 6 * b;
+    const int w = (int)(o[3] * o[5]);
+    for (int t = lane; t < w; t += kWave) Gl[g0 + t] = v.cot[o[2] + (int64_t)w * i + t];
+    g0 += w;
+  }
   __syncthreads();
   feat_densities<L>(a, e0, e1, zi, Fl, Ysh, Rsh, Zsh, A);
-  feat_rows(a, A, P, B);
+  feat_rows(a, A, B);
   double* row = v.prow + (int64_t)i * v.wp;
   int64_t mo = 0;
-  for (int b = 0; b < a.nblocks; ++b) {  // mixing partials B^T G
+  for (int b = 0, g0 = 0; b < a.nblocks; ++b) {  // mixing partials B^T G
     const int64_t* o = a.boff + 6 * b;
     const int dim = (int)o[3], nb = (int)o[4], ch = (int)o[5];
-    const double* G = v.cot + o[2] + (int64_t)dim * ch * i;
+    const double* G = Gl + g0;
+    g0 += dim * ch;
     for (int t = lane; t < nb * ch; t += kWave) {
       const int beta = t % nb, c = t / nb;
       double s = 0;
@@ -2038,10 +2055,11 @@ __global__ __launch_bounds__(64) void k_features_vjp(FeatArgs a, FeatVjpArgs v) 
     mo += (int64_t)nb * ch;
   }
   __syncthreads();
-  for (int b = 0; b < a.nblocks; ++b) {  // dB = G W^T, over B
+  for (int b = 0, g0 = 0; b < a.nblocks; ++b) {  // dB = G W^T, over B
     const int64_t* o = a.boff + 6 * b;
     const int dim = (int)o[3], nb = (int)o[4], ch = (int)o[5];
-    const double* G = v.cot + o[2] + (int64_t)dim * ch * i;
+    const double* G = Gl + g0;
+    g0 += dim * ch;
     const double* W = a.mix + o[1] + (int64_t)nb * ch * zi;
     for (int t = lane; t < dim * nb; t += kWave) {
       const int m = t % dim, beta = t / dim;
@@ -2051,21 +2069,22 @@ __global__ __launch_bounds__(64) void k_features_vjp(FeatArgs a, FeatVjpArgs v) 
     }
   }
   __syncthreads();
-  for (int t = lane; t < a.nprod; t += kWave) {  // product weights C^T vec(dB), over P
-    double s = 0;
-    for (int q = v.cptr[t]; q < v.cptr[t + 1]; ++q) s += v.cval[q] * B[v.crow[q]];
-    P[t] = s;
+  for (int64_t q = lane; q < a.nnz; q += kWave) {  // dA by the product rule, per lane
+    const FeatEntry e = a.ent[q];
+    const double c = e.val * B[a.erow[q]];
+    const int f0 = (int)(e.f & 0xffff), f1 = (int)((e.f >> 16) & 0xffff), f2 = (int)((e.f >> 32) & 0xffff),
+              f3 = (int)(e.f >> 48);
+    const double a0 = A[f0], a1 = A[f1], a2 = A[f2], a3 = A[f3];
+    const double p01 = a0 * a1, p23 = a2 * a3;
+    dAl[f0 * kLaneStride + lane] += c * (a1 * p23);
+    dAl[f1 * kLaneStride + lane] += c * (a0 * p23);
+    dAl[f2 * kLaneStride + lane] += c * (p01 * a3);
+    dAl[f3 * kLaneStride + lane] += c * (p01 * a2);
   }
   __syncthreads();
-  for (int t = lane; t < a.ndens; t += kWave) {  // dA by the product rule
+  for (int t = lane; t < a.ndens; t += kWave) {
     double s = 0;
-    for (int q = v.aptr[t]; q < v.aptr[t + 1]; ++q) {
-      const int p = v.aprod[q], pos = v.apos[q];
-      double f = P[p];
-      for (int u = a.pptr[p]; u < a.pptr[p + 1]; ++u)
-        if (u - a.pptr[p] != pos) f *= A[a.pidx[u]];
-      s += f;
-    }
+    for (int u = 0; u < kWave; ++u) s += dAl[t * kLaneStride + u];
     dA[t] = s;
   }
   __syncthreads();
@@ -2203,17 +2222,25 @@ __global__ void k_inverse_perm(int64_t n, const int32_t* __restrict__ perm, int3
 // Per-species column sums of the per-atom parameter rows: chunk c of
 // kRowChunk atoms gives part[(c nz + z) wp + col]; k_species_total sums the
 // chunks in order. Both fixed order.
-constexpr int kRowChunk = 1024;
+constexpr int kRowChunk = 128;
 __global__ void k_species_rows(int64_t n, int wp, int nz, const double* __restrict__ prow,
                                const int32_t* __restrict__ species, double* __restrict__ part) {
   const int col = blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= wp) return;
   const int64_t c = blockIdx.y, a0 = c * kRowChunk, a1 = min(n, a0 + kRowChunk);
-  for (int z = 0; z < nz; ++z) {
-    double s = 0;
-    for (int64_t t = a0; t < a1; ++t)
-      if (species[t] == z) s += prow[t * wp + col];
-    part[(c * nz + z) * wp + col] = s;
+  constexpr int kZ = 8;  // species per pass over the rows
+  for (int z0 = 0; z0 < nz; z0 += kZ) {
+    double s[kZ];
+#pragma unroll
+    for (int u = 0; u < kZ; ++u) s[u] = 0;
+    for (int64_t t = a0; t < a1; ++t) {
+      const int z = species[t] - z0;
+      const double x = prow[t * wp + col];
+#pragma unroll
+      for (int u = 0; u < kZ; ++u)
+        if (z == u) s[u] += x;
+    }
+    for (int u = 0; u < kZ && z0 + u < nz; ++u) part[(c * nz + z0 + u) * wp + col] = s[u];
   }
 }
 
@@ -2236,6 +2263,8 @@ static T* upload_raw(const T* p, size_t count, std::vector<void*>& owned) {
 }
 
 namespace {
+enum { kPoolOut, kPoolCot, kPoolEdge, kPoolRows, kPoolPart };  // context pool slots of the feature calls
+
 // A validated feature layer with its tables on the device and the current
 // system's pairs exported (when it has atoms).
 struct FeatPlan {
@@ -2279,8 +2308,11 @@ void feat_prepare(pfw_ctx* ctx, const pfw_feature_layer* l, FeatPlan& p, bool wi
         l->dens_h[t] >= ny)
       throw Error(PFW_INVALID_ARGUMENT, "invalid feature density entry");
   if (l->prod_ptr[0] != 0) throw Error(PFW_INVALID_ARGUMENT, "invalid feature product offsets");
+  if (l->ndens > 0xfffe) throw Error(PFW_INVALID_ARGUMENT, "more than 65534 feature densities");
   for (int t = 0; t < l->nprod; ++t) {
     if (l->prod_ptr[t + 1] <= l->prod_ptr[t]) throw Error(PFW_INVALID_ARGUMENT, "empty feature product");
+    if (l->prod_ptr[t + 1] - l->prod_ptr[t] > 4)
+      throw Error(PFW_INVALID_ARGUMENT, "feature products of more than four factors are not supported by the device kernels");
     for (int q = l->prod_ptr[t]; q < l->prod_ptr[t + 1]; ++q)
       if (l->prod_idx[q] < 0 || l->prod_idx[q] >= l->ndens) throw Error(PFW_INVALID_ARGUMENT, "invalid product factor");
   }
@@ -2330,17 +2362,31 @@ void feat_prepare(pfw_ctx* ctx, const pfw_feature_layer* l, FeatPlan& p, bool wi
   a.nz = l->nz;
   a.nr = l->nr;
   a.ndens = l->ndens;
-  a.nprod = l->nprod;
   a.nrows = (int)p.nrows;
+  a.nnz = nval;
   a.nblocks = l->nblocks;
   a.dz = p.up(l->dens_z, l->ndens);
   a.dn = p.up(l->dens_n, l->ndens);
   a.dh = p.up(l->dens_h, l->ndens);
-  a.pptr = p.up(l->prod_ptr, (size_t)l->nprod + 1);
-  a.pidx = p.up(l->prod_idx, (size_t)l->prod_ptr[l->nprod]);
   a.rptr = p.up(l->row_ptr, (size_t)p.nrows + 1);
-  a.col = p.up(l->col, (size_t)nval);
-  a.val = p.up(l->val, (size_t)nval);
+  {  // coupling entries with their products' factors, in row order
+    std::vector<FeatEntry> ent(nval);
+    std::vector<int32_t> erow(nval);
+    for (int64_t r = 0; r < p.nrows; ++r)
+      for (int q = l->row_ptr[r]; q < l->row_ptr[r + 1]; ++q) {
+        const int k = l->col[q];
+        uint64_t f = 0;
+        for (int u = 0; u < 4; ++u) {
+          const int at = l->prod_ptr[k] + u;
+          const uint64_t d = at < l->prod_ptr[k + 1] ? (uint64_t)l->prod_idx[at] : (uint64_t)l->ndens;
+          f |= d << (16 * u);
+        }
+        ent[q] = FeatEntry{l->val[q], f};
+        erow[q] = (int32_t)r;
+      }
+    a.ent = p.up(ent.data(), ent.size());
+    a.erow = p.up(erow.data(), erow.size());
+  }
   a.boff = p.up(p.boff.data(), p.boff.size());
   a.mix = p.up(l->mixing, (size_t)p.nmix);
   a.rc = l->radial_cutoff;
@@ -2384,11 +2430,11 @@ void features_run(pfw_ctx* ctx, const pfw_feature_layer* l, double* out) {
   FeatPlan p;
   feat_prepare(ctx, l, p, false);
   const size_t lds =
-      sizeof(double) * (p.nyk + (p.nyk + l->nr) * 64 + l->ndens + l->nprod + p.nrows) + 64 * sizeof(int);
+      sizeof(double) * (p.nyk + (p.nyk + l->nr) * 64 + l->ndens + 1 + p.nrows) + 64 * sizeof(int);
   if (lds > kMaxLds) throw Error(PFW_INVALID_ARGUMENT, "feature tables too large for LDS");
   std::fill(out, out + p.nout, 0.0);
   if (p.n == 0) return;
-  double* dout = p.alloc<double>(p.nout);
+  double* dout = ctx->pool.get<double>(kPoolOut, p.nout);
   p.a.out = dout;
   hipStream_t st = ctx->stream;
   launch_lk<FeatFwd>(p.Lk, lds, p.n, st, p.a);
@@ -2406,8 +2452,8 @@ void features_vjp_run(pfw_ctx* ctx, const pfw_feature_layer* l, const double* co
   feat_prepare(ctx, l, p, true);
   const int nz = l->nz, nr = l->nr;
   const int64_t nsc = (int64_t)nr * nz * nz;
-  const size_t lds = sizeof(double) * (p.nyk + (p.nyk + 3 * (size_t)nr) * 64 + 2 * (size_t)l->ndens + l->nprod +
-                                       p.nrows + (size_t)nr * nz) +
+  const size_t lds = sizeof(double) * (p.nyk + (p.nyk + 3 * (size_t)nr) * 64 + 2 * (size_t)l->ndens + 1 + p.nrows +
+                                       (size_t)(l->ndens + 1) * kLaneStride + (size_t)nr * nz + p.nout / std::max<int64_t>(p.n, 1)) +
                      64 * sizeof(int);
   if (lds > kMaxLds) throw Error(PFW_INVALID_ARGUMENT, "feature tables too large for LDS");
   std::fill(dx, dx + 3 * p.n, 0.0);
@@ -2417,35 +2463,8 @@ void features_vjp_run(pfw_ctx* ctx, const pfw_feature_layer* l, const double* co
   if (p.n == 0) return;
   for (int64_t t = 0; t < p.nout; ++t)
     if (!std::isfinite(cot[t])) throw Error(PFW_INVALID_ARGUMENT, "nonfinite feature cotangent");
-  // coupling columns, density -> (product, position) and harmonic -> densities
-  const int nprod = l->nprod, ndens = l->ndens;
-  std::vector<int32_t> cptr(nprod + 1, 0), crow(l->row_ptr[p.nrows]);
-  std::vector<double> cval(crow.size());
-  for (int64_t r = 0; r < p.nrows; ++r)
-    for (int q = l->row_ptr[r]; q < l->row_ptr[r + 1]; ++q) ++cptr[l->col[q] + 1];
-  for (int t = 0; t < nprod; ++t) cptr[t + 1] += cptr[t];
-  {
-    std::vector<int32_t> fill(cptr.begin(), cptr.end() - 1);
-    for (int64_t r = 0; r < p.nrows; ++r)  // rows ascending within each column
-      for (int q = l->row_ptr[r]; q < l->row_ptr[r + 1]; ++q) {
-        const int at = fill[l->col[q]]++;
-        crow[at] = (int32_t)r;
-        cval[at] = l->val[q];
-      }
-  }
-  std::vector<int32_t> aptr(ndens + 1, 0), aprod(l->prod_ptr[nprod]), apos(aprod.size());
-  for (int q = 0; q < l->prod_ptr[nprod]; ++q) ++aptr[l->prod_idx[q] + 1];
-  for (int t = 0; t < ndens; ++t) aptr[t + 1] += aptr[t];
-  {
-    std::vector<int32_t> fill(aptr.begin(), aptr.end() - 1);
-    for (int k = 0; k < nprod; ++k)
-      for (int q = l->prod_ptr[k]; q < l->prod_ptr[k + 1]; ++q) {
-        const int at = fill[l->prod_idx[q]]++;
-        aprod[at] = k;
-        apos[at] = q - l->prod_ptr[k];
-      }
-  }
-  std::vector<int32_t> hptr(p.nyk + 1, 0), hidx(ndens);
+  const int ndens = l->ndens;
+  std::vector<int32_t> hptr(p.nyk + 1, 0), hidx(ndens);  // harmonic -> densities
   for (int t = 0; t < ndens; ++t) ++hptr[l->dens_h[t] + 1];
   for (size_t h = 0; h < p.nyk; ++h) hptr[h + 1] += hptr[h];
   {
@@ -2453,20 +2472,16 @@ void features_vjp_run(pfw_ctx* ctx, const pfw_feature_layer* l, const double* co
     for (int t = 0; t < ndens; ++t) hidx[fill[l->dens_h[t]]++] = t;
   }
   FeatVjpArgs v;
-  v.cot = p.up(cot, (size_t)p.nout);
-  v.cptr = p.up(cptr.data(), cptr.size());
-  v.crow = p.up(crow.data(), crow.size());
-  v.cval = p.up(cval.data(), cval.size());
-  v.aptr = p.up(aptr.data(), aptr.size());
-  v.aprod = p.up(aprod.data(), aprod.size());
-  v.apos = p.up(apos.data(), apos.size());
+  v.cot = ctx->pool.get<double>(kPoolCot, p.nout);
+  PFW_HIP(hipMemcpyAsync((void*)v.cot, cot, p.nout * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
   v.hptr = p.up(hptr.data(), hptr.size());
   v.hidx = p.up(hidx.data(), hidx.size());
   const int64_t total = p.ex.total;
-  v.gE = p.alloc<double>(3 * (size_t)total);
+  v.gE = ctx->pool.get<double>(kPoolEdge, 3 * (size_t)total);
   v.so = (int)(p.nmix / nz);
+  v.gper = (int)(p.nout / p.n);
   v.wp = v.so + nr * nz + 2;
-  v.prow = p.alloc<double>((size_t)p.n * v.wp);
+  v.prow = ctx->pool.get<double>(kPoolRows, (size_t)p.n * v.wp);
   hipStream_t st = ctx->stream;
   launch_lk<FeatVjp>(p.Lk, lds, p.n, st, p.a, v);
   int32_t* inv = p.alloc<int32_t>(p.n);
@@ -2480,7 +2495,7 @@ void features_vjp_run(pfw_ctx* ctx, const pfw_feature_layer* l, const double* co
                      (const int32_t*)inv, (const double*)v.gE, d_dx, orphan, p.a.status);
   PFW_HIP(hipGetLastError());
   const int64_t nchunks = (p.n + kRowChunk - 1) / kRowChunk, width = (int64_t)nz * v.wp;
-  double* part = p.alloc<double>((size_t)nchunks * width);
+  double* part = ctx->pool.get<double>(kPoolPart, (size_t)nchunks * width);
   double* dsum = p.alloc<double>((size_t)width);
   hipLaunchKernelGGL(k_species_rows, dim3((unsigned)((v.wp + 63) / 64), (unsigned)nchunks), dim3(64), 0, st, p.n,
                      v.wp, nz, (const double*)v.prow, (const int32_t*)ctx->sys.species, part);

@@ -96,6 +96,7 @@ void pfw_ctx_destroy(pfw_ctx* ctx) {
   pfw::nl_free(ctx);
   pfw::ace_free(ctx);
   pfw::verlet_free(ctx);
+  ctx->pool.release();
   if (ctx->d_F) (void)hipFree(ctx->d_F);
   if (ctx->d_partials) (void)hipFree(ctx->d_partials);
   if (ctx->d_out) (void)hipFree(ctx->d_out);

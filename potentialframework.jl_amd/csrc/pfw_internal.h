@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -217,6 +218,37 @@ struct AceDev {
 
 }  // namespace pfw
 
+namespace pfw {
+// Grow-only device buffers kept between calls (one per slot) for the large
+// per-call arrays of the feature kernels; freed with the context.
+struct GrowPool {
+  struct Buf {
+    void* p = nullptr;
+    size_t bytes = 0;
+  };
+  std::vector<Buf> slots;
+  template <typename T>
+  T* get(size_t slot, size_t count) {
+    if (slots.size() <= slot) slots.resize(slot + 1);
+    Buf& b = slots[slot];
+    const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    if (b.bytes < bytes) {
+      if (b.p) PFW_HIP(hipFree(b.p));
+      b.p = nullptr;
+      b.bytes = 0;
+      PFW_HIP(hipMalloc(&b.p, bytes));
+      b.bytes = bytes;
+    }
+    return (T*)b.p;
+  }
+  void release() {
+    for (Buf& b : slots)
+      if (b.p) (void)hipFree(b.p);
+    slots.clear();
+  }
+};
+}  // namespace pfw
+
 struct pfw_ctx {
   int device = 0;
   int num_cu = 256;  // compute units of the device
@@ -236,6 +268,7 @@ struct pfw_ctx {
   pfw::PairModelDev pair;
   pfw::AceDev ace;
   pfw::VerletDev verlet;
+  pfw::GrowPool pool;
   uint64_t build_gen = 0;  // incremented by every cell build
   // outputs
   double* d_F = nullptr;        // 3 x n original order

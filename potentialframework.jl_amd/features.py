@@ -316,7 +316,8 @@ def _prepared(layer, system, ps, workspace):
 
 def ace_features(layer, system, ps, workspace=None):
     """``ace_features(X, layer, ps, environment, H)`` (features.jl:164-178) on
-    the device: one (2L+1, channels, n) array per irrep, input atom order."""
+    the device: one (2L+1, channels, n) array per irrep, input atom order
+    (column-major views of one buffer, as the reference's Julia arrays)."""
     _check(layer, ps)
     if layer.resolved is not None:
         return _resolved_features(layer, system, ps)
@@ -328,7 +329,7 @@ def ace_features(layer, system, ps, workspace=None):
     del keep
     res, o = [], 0
     for s, k in zip(layer.irreps, sizes):
-        res.append(out[o:o + k].reshape(n, s.channels, 2 * s.L + 1).transpose(2, 1, 0).copy())
+        res.append(out[o:o + k].reshape((2 * s.L + 1, s.channels, n), order="F"))  # column-major, no copy
         o += k
     return tuple(res)
 
@@ -349,7 +350,7 @@ def feature_vjp(layer, system, ps, cotangents, workspace=None):
         c = np.zeros(shape) if c is None else np.asarray(c, float)
         if c.shape != shape:
             raise DimensionMismatch(f"cotangent of block L={s.L} has shape {c.shape}, expected {shape}")
-        blocks.append(c.transpose(2, 1, 0).reshape(-1))
+        blocks.append(c.reshape(-1, order="F"))
     if len(blocks) != len(layer.irreps):
         raise DimensionMismatch("one cotangent per feature block")
     cot = np.ascontiguousarray(np.concatenate(blocks))
