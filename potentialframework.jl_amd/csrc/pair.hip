@@ -89,8 +89,10 @@ __device__ inline double prefilter_r2(const Cand<double>& c, double xi, double y
 
 // Five wavefronts per SIMD for the FP32 kernel (at most 96 registers): the
 // candidate fills and LDS round trips of one wavefront hide behind the tests
-// of the others.
-template <typename T, int KIND>
+// of the others. Z2 (SmoothPair with two species): each lane keeps its row of
+// amplitudes in registers, so a pair's amplitude costs a select instead of an
+// LDS round trip.
+template <typename T, int KIND, bool Z2>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 5 : 4))) void k_pair(const Grid* __restrict__ grids, const int32_t* __restrict__ cell_cfg,
                                              int64_t n, const int32_t* __restrict__ start,
                                              const double* __restrict__ xs, const int32_t* __restrict__ sps,
@@ -107,6 +109,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
   double(*red)[3] = (double(*)[3])cbuf;
   __shared__ int cm[kCandPadOf<T>];  // per candidate position: (stencil entry of the chunk) << 8 | species
   __shared__ int e_pref[kEnt + 1], e_start[kEnt], e_lat[kEnt][3];
+  __shared__ uint8_t e_of[kCandOf<T>];  // stencil entry (of the entry chunk) of each candidate of the LDS chunk
   __shared__ double e_sh[kEnt][3];
   extern __shared__ __align__(16) unsigned char dyn[];  // SmoothPair: nz x nz amplitudes, then -6 A / c^2
   T* sA = (T*)dyn;
@@ -119,6 +122,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
   const T c2 = (T)g.c2;
   const T c2lo = (T)(g.c2 * (1.0 - Band<T>::rel)), c2hi = (T)(g.c2 * (1.0 + Band<T>::rel));
   const T invc2 = (T)(1.0 / g.c2);
+  const T m6c2 = (T)(-6.0 / g.c2);
   if (KIND == PFW_PAIR_SMOOTH) {
     for (int k = lane; k < nz * nz; k += kWave) {
       double a = params[k];
@@ -155,6 +159,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
       const T m2x = (T)-2 * xi, m2y = (T)-2 * yi, m2z = (T)-2 * zi;
       const T xi2 = (xi * xi + yi * yi + zi * zi) * ((T)1 - DotSlack<T>::value);
       const int zsp = sps[is];
+      // two species: the lane's row of amplitudes in registers
+      T ar0 = 0, ar1 = 0;
+      if (Z2 && KIND == PFW_PAIR_SMOOTH) {
+        ar0 = (T)params[zsp * nz];
+        ar1 = (T)params[zsp * nz + 1];
+      }
       double fx = 0, fy = 0, fz = 0;
       for (int e0 = 0; e0 < g.nstencil; e0 += kEnt) {
         int cntE = 0;
@@ -191,16 +201,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
           const int L = ((tn + S - 1) / S + 7) / 8 * 8;
           const int stride = L + 1;
           const uint32_t invS = (65536u + S - 1) / S;  // t / S = (t invS) >> 16 for t < 4096
+          if (lane < ne) {  // each entry marks its candidates of this chunk
+            const int a0 = max(e_pref[lane], t0) - t0, a1 = min(e_pref[lane + 1], t0 + tn) - t0;
+            for (int t = a0; t < a1; ++t) e_of[t] = (uint8_t)lane;
+          }
+          __syncthreads();
           for (int t = lane; t < tn; t += kWave) {
             const int tt = t0 + t;
-            int lo = 0, hi = ne;
-            while (hi - lo > 1) {
-              int mid = (lo + hi) >> 1;
-              if (e_pref[mid] <= tt)
-                lo = mid;
-              else
-                hi = mid;
-            }
+            const int lo = e_of[t];
             const int m = e_start[lo] + (tt - e_pref[lo]);
             const T x = (T)((xs[m] + e_sh[lo][0]) - rx);
             const T y = (T)((xs[n + m] + e_sh[lo][1]) - ry);
@@ -267,7 +275,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
                 }
                 T e, coef;
                 if (KIND == PFW_PAIR_SMOOTH) {
-                  const T a = sA[zsp * nz + zj], a6 = sA6[zsp * nz + zj];
+                  T a, a6;
+                  if (Z2) {
+                    a = zj ? ar1 : ar0;
+                    a6 = a * m6c2;
+                  } else {
+                    a = sA[zsp * nz + zj];
+                    a6 = sA6[zsp * nz + zj];
+                  }
                   const T qq = (T)1 - r2 * invc2;
                   const T q2 = qq * qq;
                   e = a * q2 * qq;
@@ -353,9 +368,14 @@ static void launch_pair(pfw_ctx* ctx, uint32_t mask) {
   SystemBuffers& s = ctx->sys;
   const LJConsts lj = lj_consts(ctx->pair);
   const size_t dyn = KIND == PFW_PAIR_SMOOTH ? 2 * sizeof(T) * ctx->pair.nz * ctx->pair.nz : 0;
-  hipLaunchKernelGGL((k_pair<T, KIND>), dim3(ctx->ncells), dim3(kWave), dyn, ctx->stream, s.grids, s.cell_cfg, s.n,
-                     s.start, s.xs, s.sps, s.perm, ctx->pair.d_params, ctx->pair.nz, lj, mask, ctx->d_F,
-                     ctx->d_partials, ctx->ncells);
+  if (KIND == PFW_PAIR_SMOOTH && ctx->pair.nz == 2)
+    hipLaunchKernelGGL((k_pair<T, KIND, true>), dim3(ctx->ncells), dim3(kWave), dyn, ctx->stream, s.grids, s.cell_cfg,
+                       s.n, s.start, s.xs, s.sps, s.perm, ctx->pair.d_params, ctx->pair.nz, lj, mask, ctx->d_F,
+                       ctx->d_partials, ctx->ncells);
+  else
+    hipLaunchKernelGGL((k_pair<T, KIND, false>), dim3(ctx->ncells), dim3(kWave), dyn, ctx->stream, s.grids, s.cell_cfg,
+                       s.n, s.start, s.xs, s.sps, s.perm, ctx->pair.d_params, ctx->pair.nz, lj, mask, ctx->d_F,
+                       ctx->d_partials, ctx->ncells);
 }
 
 void pair_run(pfw_ctx* ctx, uint32_t mask, int precision) {
