@@ -89,10 +89,8 @@ __device__ inline double prefilter_r2(const Cand<double>& c, double xi, double y
 
 // Five wavefronts per SIMD for the FP32 kernel (at most 96 registers): the
 // candidate fills and LDS round trips of one wavefront hide behind the tests
-// of the others. Z2 (SmoothPair with two species): each lane keeps its row of
-// amplitudes in registers, so a pair's amplitude costs a select instead of an
-// LDS round trip.
-template <typename T, int KIND, bool Z2>
+// of the others.
+template <typename T, int KIND>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 5 : 4))) void k_pair(const Grid* __restrict__ grids, const int32_t* __restrict__ cell_cfg,
                                              int64_t n, const int32_t* __restrict__ start,
                                              const double* __restrict__ xs, const int32_t* __restrict__ sps,
@@ -122,7 +120,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
   const T c2 = (T)g.c2;
   const T c2lo = (T)(g.c2 * (1.0 - Band<T>::rel)), c2hi = (T)(g.c2 * (1.0 + Band<T>::rel));
   const T invc2 = (T)(1.0 / g.c2);
-  const T m6c2 = (T)(-6.0 / g.c2);
   if (KIND == PFW_PAIR_SMOOTH) {
     for (int k = lane; k < nz * nz; k += kWave) {
       double a = params[k];
@@ -159,12 +156,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
       const T m2x = (T)-2 * xi, m2y = (T)-2 * yi, m2z = (T)-2 * zi;
       const T xi2 = (xi * xi + yi * yi + zi * zi) * ((T)1 - DotSlack<T>::value);
       const int zsp = sps[is];
-      // two species: the lane's row of amplitudes in registers
-      T ar0 = 0, ar1 = 0;
-      if (Z2 && KIND == PFW_PAIR_SMOOTH) {
-        ar0 = (T)params[zsp * nz];
-        ar1 = (T)params[zsp * nz + 1];
-      }
       double fx = 0, fy = 0, fz = 0;
       for (int e0 = 0; e0 < g.nstencil; e0 += kEnt) {
         int cntE = 0;
@@ -275,14 +266,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
                 }
                 T e, coef;
                 if (KIND == PFW_PAIR_SMOOTH) {
-                  T a, a6;
-                  if (Z2) {
-                    a = zj ? ar1 : ar0;
-                    a6 = a * m6c2;
-                  } else {
-                    a = sA[zsp * nz + zj];
-                    a6 = sA6[zsp * nz + zj];
-                  }
+                  const T a = sA[zsp * nz + zj], a6 = sA6[zsp * nz + zj];
                   const T qq = (T)1 - r2 * invc2;
                   const T q2 = qq * qq;
                   e = a * q2 * qq;
@@ -368,14 +352,9 @@ static void launch_pair(pfw_ctx* ctx, uint32_t mask) {
   SystemBuffers& s = ctx->sys;
   const LJConsts lj = lj_consts(ctx->pair);
   const size_t dyn = KIND == PFW_PAIR_SMOOTH ? 2 * sizeof(T) * ctx->pair.nz * ctx->pair.nz : 0;
-  if (KIND == PFW_PAIR_SMOOTH && ctx->pair.nz == 2)
-    hipLaunchKernelGGL((k_pair<T, KIND, true>), dim3(ctx->ncells), dim3(kWave), dyn, ctx->stream, s.grids, s.cell_cfg,
-                       s.n, s.start, s.xs, s.sps, s.perm, ctx->pair.d_params, ctx->pair.nz, lj, mask, ctx->d_F,
-                       ctx->d_partials, ctx->ncells);
-  else
-    hipLaunchKernelGGL((k_pair<T, KIND, false>), dim3(ctx->ncells), dim3(kWave), dyn, ctx->stream, s.grids, s.cell_cfg,
-                       s.n, s.start, s.xs, s.sps, s.perm, ctx->pair.d_params, ctx->pair.nz, lj, mask, ctx->d_F,
-                       ctx->d_partials, ctx->ncells);
+  hipLaunchKernelGGL((k_pair<T, KIND>), dim3(ctx->ncells), dim3(kWave), dyn, ctx->stream, s.grids, s.cell_cfg, s.n,
+                     s.start, s.xs, s.sps, s.perm, ctx->pair.d_params, ctx->pair.nz, lj, mask, ctx->d_F,
+                     ctx->d_partials, ctx->ncells);
 }
 
 void pair_run(pfw_ctx* ctx, uint32_t mask, int precision) {
