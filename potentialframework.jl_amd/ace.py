@@ -163,11 +163,16 @@ class ResolvedModel:
         return {"WB": self.WB.copy(), "Wpair": self.Wpair.copy()}
 
     def channels(self, labels):
-        z = [atomic_number(s) for s in labels]
-        try:
-            return np.array([self.atomic_numbers.index(v) for v in z], np.int32)
-        except ValueError:
-            raise ArgumentError("ACE model has no element channel for a species in the system") from None
+        from .system import map_labels
+
+        def lookup(s):
+            z = atomic_number(s)
+            return self.atomic_numbers.index(z) if z in self.atomic_numbers else None
+
+        out, bad = map_labels(labels, lookup)
+        if out is None:
+            raise ArgumentError("ACE model has no element channel for a species in the system")
+        return out
 
     def _struct(self):
         keep = []

@@ -396,12 +396,12 @@ def ace_features_rrule(layer, system, ps, workspace=None):
 
 def _channels(layer, system):
     from .ace import ELEMENTS
+    from .system import map_labels
     lut = {ELEMENTS[z - 1]: c for c, z in enumerate(layer.atomic_numbers)}
-    try:
-        return np.fromiter((lut[lab] for lab in system.labels), np.int32, count=len(system.labels))
-    except (KeyError, TypeError):
-        bad = next(lab for lab in system.labels if not isinstance(lab, str) or lab not in lut)
-        raise ArgumentError(f"species {bad} is not in the feature layer") from None
+    out, bad = map_labels(system.labels, lambda s: lut.get(s) if isinstance(s, str) else None)
+    if out is None:
+        raise ArgumentError(f"species {bad} is not in the feature layer")
+    return out
 
 
 class _Layer(ctypes.Structure):

@@ -10,6 +10,39 @@ import numpy as np
 from ._capi import ArgumentError, DimensionMismatch
 
 
+class Labels(tuple):
+    """Immutable species labels of a system. The distinct labels (in order of
+    first appearance) and each atom's index among them are computed once, so
+    mapping the labels onto a model's channels costs one lookup per distinct
+    label and a gather, not a Python loop over the atoms, on every call."""
+
+    def codes(self):
+        c = self.__dict__.get("_codes")
+        if c is None:
+            index = {}
+            codes = np.fromiter((index.setdefault(s, len(index)) for s in self), np.int32, count=len(self))
+            c = self.__dict__["_codes"] = (list(index), codes)
+        return c
+
+
+def map_labels(labels, lookup):
+    """Channel of every label: ``lookup(label)`` (None for an unknown label)
+    evaluated once per distinct label. Returns (int32 channels, first unknown
+    label or None)."""
+    labels = labels if isinstance(labels, Labels) else Labels(labels)
+    try:
+        distinct, codes = labels.codes()
+    except TypeError:
+        raise ArgumentError("species labels must be hashable") from None
+    lut = np.empty(len(distinct), np.int32)
+    for u, s in enumerate(distinct):
+        k = lookup(s)
+        if k is None:
+            return None, s
+        lut[u] = k
+    return lut[codes], None
+
+
 class System:
     def __init__(self, positions, cell, pbc, species):
         X = np.array(positions, dtype=np.float64, copy=True)
@@ -20,7 +53,7 @@ class System:
             raise DimensionMismatch("cell must be 3x3 with lattice vectors in columns")
         if len(pbc) != 3:
             raise DimensionMismatch("pbc must contain three flags")
-        species = list(species)
+        species = species if isinstance(species, Labels) else Labels(species)
         if len(species) != len(X):
             raise DimensionMismatch("one species label per atom")
         self.positions = X
@@ -51,21 +84,15 @@ def channels(labels, known):
     """Map species labels onto model channels (bind_species,
     src/submodules/PairPotentials/models.jl:50-58). Unknown labels raise ArgumentError."""
     index = {s: k for k, s in enumerate(known)}
-    out = np.empty(len(labels), dtype=np.int32)
-    for i, s in enumerate(labels):
-        k = index.get(s)
-        if k is None:
-            raise ArgumentError(f"potential has no parameters for species {s}")
-        out[i] = k
+    out, bad = map_labels(labels, index.get)
+    if out is None:
+        raise ArgumentError(f"potential has no parameters for species {bad}")
     return out
 
 
 def unique_in_order(labels):
-    seen = []
-    for s in labels:
-        if s not in seen:
-            seen.append(s)
-    return seen
+    labels = labels if isinstance(labels, Labels) else Labels(labels)
+    return list(labels.codes()[0])
 
 
 def validate_geometry(sys):
