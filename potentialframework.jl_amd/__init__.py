@@ -45,6 +45,7 @@ from .adjoints import (  # noqa: F401
     fitting_loss,
 )
 from .fitting import design_matrix, training_rows, linear_parameters, install_readout  # noqa: F401
+from .precision import compare_precision, PrecisionReport  # noqa: F401
 from . import features  # noqa: F401
 from .features import feature_vjp, ace_features_rrule  # noqa: F401
 from .features import ACEFeatureLayer, ace_features, feature_parameters, feature_basis_summary  # noqa: F401
