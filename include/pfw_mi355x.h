@@ -152,6 +152,16 @@ int pfw_get_timings(pfw_ctx* ctx, double* ms, int n);
 /* Synchronise the context's stream. */
 int pfw_synchronize(pfw_ctx* ctx);
 
+/* Page-locked host memory for result arrays, from a process-wide cache: the
+ * library's device-to-host copies into such a block run as one DMA, where a
+ * pageable destination is copied through the runtime's staging buffers. No
+ * counterpart in the reference (Julia owns its arrays); for host bindings that
+ * want owning result arrays filled at DMA speed. pfw_host_alloc returns NULL
+ * when no block can be had (bytes == 0 included); pfw_host_free returns a block
+ * to the cache (NULL and foreign pointers are ignored). Thread-safe. */
+void* pfw_host_alloc(size_t bytes);
+void pfw_host_free(void* p);
+
 /* ---------------------------------------------------------------- ACE
  * Resolved ACE model, schema v1 (src/submodules/ACE/resolved.jl:18-142):
  * piecewise-cubic Agnesi radial splines per (centre, neighbour) species

@@ -73,6 +73,8 @@ SIGNATURES = [
     ("pfw_set_timing", ctypes.c_int, [_P, ctypes.c_int]),
     ("pfw_get_timings", ctypes.c_int, [_P, _D, ctypes.c_int]),
     ("pfw_synchronize", ctypes.c_int, [_P]),
+    ("pfw_host_alloc", ctypes.c_void_p, [ctypes.c_size_t]),
+    ("pfw_host_free", None, [ctypes.c_void_p]),
     ("pfw_set_ace_model", ctypes.c_int, [_P, ctypes.c_void_p]),
     ("pfw_set_ace_parameters", ctypes.c_int, [_P, _D, _D]),
     ("pfw_ace_step", ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_int]),
@@ -101,6 +103,25 @@ def load():
         f.argtypes = args
     _lib = lib
     return lib
+
+
+def pinned_empty(count, dtype=np.float64):
+    """A flat array of ``count`` elements in a page-locked block of the
+    library's host cache (pfw_host_alloc), handed back to the cache when the
+    last array viewing it is freed; an ordinary array when no block can be had.
+    The library's device-to-host result copies into it run as one DMA."""
+    import weakref
+    dt = np.dtype(dtype)
+    nbytes = int(count) * dt.itemsize
+    if nbytes == 0:
+        return np.empty(int(count), dt)
+    lib = load()
+    p = lib.pfw_host_alloc(nbytes)
+    if not p:
+        return np.empty(int(count), dt)
+    buf = (ctypes.c_char * nbytes).from_address(p)
+    weakref.finalize(buf, lib.pfw_host_free, p).atexit = False
+    return np.frombuffer(buf, dtype=dt, count=int(count))
 
 
 def ptr(a, t=ctypes.c_double):

@@ -2432,8 +2432,7 @@ void features_run(pfw_ctx* ctx, const pfw_feature_layer* l, double* out) {
   const size_t lds =
       sizeof(double) * (p.nyk + (p.nyk + l->nr) * 64 + l->ndens + 1 + p.nrows) + 64 * sizeof(int);
   if (lds > kMaxLds) throw Error(PFW_INVALID_ARGUMENT, "feature tables too large for LDS");
-  std::fill(out, out + p.nout, 0.0);
-  if (p.n == 0) return;
+  if (p.n == 0) return;  // no outputs (nout = 0); otherwise the copy below writes all of out
   double* dout = ctx->pool.get<double>(kPoolOut, p.nout);
   p.a.out = dout;
   hipStream_t st = ctx->stream;

@@ -5,6 +5,8 @@
 // model output (gpu.jl:390-395).
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 
 #include "pfw_device.h"
 
@@ -434,6 +436,61 @@ int pfw_pair_adjoint(pfw_ctx* ctx, const double* dE, const double* dF, const dou
     if (!dE || !dF || !gA) throw Error(PFW_INVALID_ARGUMENT, "null adjoint argument");
     pfw::pair_adjoint(ctx, dE, dF, dW, gA);
   });
+}
+
+// Pinned host blocks for result arrays: a process-wide cache keyed by size
+// (a request takes the smallest cached block of at least its size and at most
+// twice it, else a new hipHostMalloc block); returned blocks are kept up to
+// kHostCacheBytes in total, beyond that freed. Blocks still cached at exit are
+// left to the process teardown.
+namespace {
+constexpr size_t kHostCacheBytes = size_t(2) << 30;
+struct HostPool {
+  std::mutex mu;
+  std::multimap<size_t, void*> free;  // size -> block
+  std::map<void*, size_t> size_of;    // every live or cached block
+  size_t cached = 0;
+};
+HostPool& host_pool() {
+  static HostPool* p = new HostPool;  // never destroyed: outlives every caller
+  return *p;
+}
+}  // namespace
+
+void* pfw_host_alloc(size_t bytes) {
+  if (bytes == 0) return nullptr;
+  const size_t want = (bytes + 65535) & ~size_t(65535);
+  HostPool& P = host_pool();
+  {
+    std::lock_guard<std::mutex> g(P.mu);
+    auto it = P.free.lower_bound(want);
+    if (it != P.free.end() && it->first <= 2 * want) {
+      void* q = it->second;
+      P.cached -= it->first;
+      P.free.erase(it);
+      return q;
+    }
+  }
+  void* q = nullptr;
+  if (hipHostMalloc(&q, want, hipHostMallocDefault) != hipSuccess) return nullptr;
+  std::lock_guard<std::mutex> g(P.mu);
+  P.size_of[q] = want;
+  return q;
+}
+
+void pfw_host_free(void* p) {
+  if (!p) return;
+  HostPool& P = host_pool();
+  std::lock_guard<std::mutex> g(P.mu);
+  auto it = P.size_of.find(p);
+  if (it == P.size_of.end()) return;  // not a block of this pool
+  if (P.cached + it->second > kHostCacheBytes) {
+    (void)hipHostFree(p);
+    P.size_of.erase(it);
+    return;
+  }
+  P.cached += it->second;
+  P.free.emplace(it->second, p);
 }
 
 int pfw_synchronize(pfw_ctx* ctx) {

@@ -32,7 +32,7 @@ import math
 
 import numpy as np
 
-from ._capi import ArgumentError, DimensionMismatch, ptr
+from ._capi import ArgumentError, DimensionMismatch, pinned_empty, ptr
 from .ace_basis import _couple, _independent, real_cg
 
 PFW_RADIAL_SMOOTH_EXPONENTIAL = 1
@@ -324,7 +324,7 @@ def ace_features(layer, system, ps, workspace=None):
     w, st, keep = _prepared(layer, system, ps, workspace)
     n = len(system)
     sizes = [(2 * s.L + 1) * s.channels * n for s in layer.irreps]
-    out = np.zeros(sum(sizes))
+    out = pinned_empty(sum(sizes))  # filled completely by the call
     w.ctx.call("pfw_features", ctypes.byref(st), ptr(out))
     del keep
     res, o = [], 0
@@ -356,7 +356,7 @@ def feature_vjp(layer, system, ps, cotangents, workspace=None):
     cot = np.ascontiguousarray(np.concatenate(blocks))
     w, st, keep = _prepared(layer, system, ps, workspace)
     nz, nr = len(layer.atomic_numbers), len(layer.radial_spec)
-    dx = np.zeros(3 * n)
+    dx = pinned_empty(3 * n)  # filled completely by the call
     dmix = np.zeros(sum(layer.basis_channels(b) * s.channels * nz for b, s in enumerate(layer.irreps)))
     dscale = np.zeros(nr * nz * nz)
     drad = np.zeros(2)

@@ -87,3 +87,34 @@ def test_empty_and_coincident(pfw):
     X = np.array([[0.1, 0.2, 0.3], [0.1, 0.2, 0.3], [1.0, 1.0, 1.0]])
     with pytest.raises(pfw.DomainError):
         pfw.ace_features(layer, pfw.System(X, 9 * np.eye(3), (0, 0, 0), ["Si", "O", "Si"]), ps)
+
+
+def test_results_own_their_storage(pfw):
+    """Feature outputs sit in page-locked blocks of the library's host cache:
+    each call's arrays keep their values through later calls, and a freed
+    block is handed out again."""
+    import gc
+    from pfw_amd import _capi
+    layer, ps = fixture(pfw)
+    X = np.array([[0.1, 0.2, 0.3], [1.2, 0.4, 1.1], [0.7, 1.8, 0.5]])
+    H = 9 * np.eye(3)
+    first = pfw.ace_features(layer, pfw.System(X, H, (0, 0, 0), ["Si", "O", "Si"]), ps)
+    keep = [b.copy() for b in first]
+    second = pfw.ace_features(layer, pfw.System(X * 1.1, H, (0, 0, 0), ["Si", "O", "Si"]), ps)
+    for a, b, c in zip(first, keep, second):
+        assert np.array_equal(a, b) and not np.array_equal(a, c)
+    lib = _capi.load()
+    p = lib.pfw_host_alloc(1 << 20)
+    assert p
+    lib.pfw_host_free(p)
+    assert lib.pfw_host_alloc(1 << 20) == p  # the cached block
+    lib.pfw_host_free(p)
+    lib.pfw_host_free(None)
+    assert not lib.pfw_host_alloc(0)
+    a = _capi.pinned_empty(1000)
+    a[:] = 3.0
+    addr = a.ctypes.data
+    del a
+    gc.collect()
+    b = _capi.pinned_empty(1000)
+    assert b.ctypes.data == addr
