@@ -451,6 +451,17 @@ def main():
         _capi.check(hh, lib.pfw_pair_evaluate(hh, 7, _capi.PFW_MIXED, Eh.ctypes.data_as(P), Fh.ctypes.data_as(P),
                                               Wh.ctypes.data_as(P)))
     host_secs = time.perf_counter() - t0
+    # the same with page-locked host buffers from the library's cache (one DMA each way)
+    Xp = _capi.pinned_empty(3 * n).reshape(n, 3)
+    Xp[...] = Xh
+    Fp = _capi.pinned_empty(3 * n).reshape(n, 3)
+    _capi.check(hh, lib.pfw_synchronize(hh))
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        _capi.check(hh, lib.pfw_update_positions(hh, Xp.ctypes.data_as(P)))
+        _capi.check(hh, lib.pfw_pair_evaluate(hh, 7, _capi.PFW_MIXED, Eh.ctypes.data_as(P), Fp.ctypes.data_as(P),
+                                              Wh.ctypes.data_as(P)))
+    pinned_secs = time.perf_counter() - t0
     npairs = len(pair["mixed"]["ws"].export_pairs()[0])
     pflops = npairs * FLOPS_PER_PAIR["smooth"]
     mixed = pair["mixed"]
@@ -487,7 +498,10 @@ def main():
         "mixed_over_fp64": round(value / fp64_value, 3),
         "host_boundary": {"value": round(n * args.steps / host_secs / 1e6, 3),
                           "ms_per_step": round(1e3 * host_secs / args.steps, 4),
-                          "note": "positions host->device and forces device->host (pageable) inside every step"},
+                          "note": "positions host->device and forces device->host (pageable) inside every step",
+                          "pinned": {"value": round(n * args.steps / pinned_secs / 1e6, 3),
+                                     "ms_per_step": round(1e3 * pinned_secs / args.steps, 4),
+                                     "note": "the same with page-locked buffers from pfw_host_alloc"}},
         "stages_ms": {k: round(float(v), 5) for k, v in zip(STAGES, mixed["stage_ms"])},
         "cpu_baseline": None,
     }

@@ -10,7 +10,7 @@ import ctypes
 import numpy as np
 
 from . import _capi
-from ._capi import ArgumentError, DimensionMismatch, ptr
+from ._capi import ArgumentError, DimensionMismatch, pinned_empty, ptr
 from .neighbourlist import Workspace
 from .pair import ENERGY, FORCES, VIRIAL, request_mask
 
@@ -501,7 +501,7 @@ class ACECalculator:
         w.prepare(sys, self.model.cutoff, ch)
         n = len(sys)
         E = np.zeros(1)
-        F = np.zeros((n, 3))
+        F = pinned_empty(3 * n).reshape(n, 3)  # written whole when forces are requested
         W = np.zeros(9)
         w.ctx.call("pfw_ace_evaluate", mask, w.mode, ptr(E), ptr(F), ptr(W))
         if w.timing:
@@ -510,7 +510,7 @@ class ACECalculator:
         if ENERGY in names:
             out["energy"] = float(E[0])
         if FORCES in names:
-            out["forces"] = F.T.copy()
+            out["forces"] = F.T  # 3 x n, an owning view (nothing else holds F)
         if VIRIAL in names:
             out["virial"] = W.reshape(3, 3).T.copy()
         if site_energies:

@@ -9,7 +9,7 @@ evaluation.jl:31-73 (``evaluate!`` / ``evaluate``). Results own their storage:
 import numpy as np
 
 from . import _capi
-from ._capi import ArgumentError, ptr
+from ._capi import ArgumentError, pinned_empty, ptr
 from .neighbourlist import Workspace
 from .system import channels, unique_in_order
 
@@ -154,7 +154,7 @@ def evaluate_(w, sys, p, request=EnergyForcesVirial, parameters=None):
         w._model_key = model_key
     n = len(sys)
     E = np.zeros(1)
-    F = np.zeros((n, 3))
+    F = pinned_empty(3 * n).reshape(n, 3)  # written whole when forces are requested
     W = np.zeros(9)
     w.ctx.call("pfw_pair_evaluate", mask, w.mode, ptr(E), ptr(F), ptr(W))
     if w.timing:
@@ -163,7 +163,7 @@ def evaluate_(w, sys, p, request=EnergyForcesVirial, parameters=None):
     if ENERGY in names:
         out["energy"] = float(E[0])
     if FORCES in names:
-        out["forces"] = F.T.copy()
+        out["forces"] = F.T  # 3 x n, an owning view (nothing else holds F)
     if VIRIAL in names:
         out["virial"] = W.reshape(3, 3).T.copy()
     return out
