@@ -1893,7 +1893,7 @@ struct FeatArgs {
   const int32_t* powers;
   const double* scale;  // nr x nz x nz (radial, centre, neighbour), column-major
   double* out;
-  int32_t* status;  // set when two atoms coincide
+  int32_t* status;  // bits: 1 two atoms coincide, 2 an edge without its reverse, 4 a nonfinite cotangent
 };
 
 // The densities A of one atom (edges [e0, e1), centre species zi) into LDS;
@@ -2033,7 +2033,11 @@ __global__ __launch_bounds__(64) void k_features_vjp(FeatArgs a, FeatVjpArgs v) 
   for (int b = 0, g0 = 0; b < a.nblocks; ++b) {
     const int64_t* o = a.boff + 6 * b;
     const int w = (int)(o[3] * o[5]);
-    for (int t = lane; t < w; t += kWave) Gl[g0 + t] = v.cot[o[2] + (int64_t)w * i + t];
+    for (int t = lane; t < w; t += kWave) {
+      const double c = v.cot[o[2] + (int64_t)w * i + t];
+      if (!isfinite(c)) atomicOr(a.status, 4);  // nonfinite cotangent
+      Gl[g0 + t] = c;
+    }
     g0 += w;
   }
   __syncthreads();
@@ -2460,8 +2464,6 @@ void features_vjp_run(pfw_ctx* ctx, const pfw_feature_layer* l, const double* co
   std::fill(dscale, dscale + nsc, 0.0);
   dradial[0] = dradial[1] = 0;
   if (p.n == 0) return;
-  for (int64_t t = 0; t < p.nout; ++t)
-    if (!std::isfinite(cot[t])) throw Error(PFW_INVALID_ARGUMENT, "nonfinite feature cotangent");
   const int ndens = l->ndens;
   std::vector<int32_t> hptr(p.nyk + 1, 0), hidx(ndens);  // harmonic -> densities
   for (int t = 0; t < ndens; ++t) ++hptr[l->dens_h[t] + 1];
@@ -2508,6 +2510,7 @@ void features_vjp_run(pfw_ctx* ctx, const pfw_feature_layer* l, const double* co
   PFW_HIP(hipMemcpyAsync(hsum.data(), dsum, width * sizeof(double), hipMemcpyDeviceToHost, st));
   PFW_HIP(hipMemcpyAsync(horph.data(), orphan, 3 * p.n * sizeof(double), hipMemcpyDeviceToHost, st));
   PFW_HIP(hipStreamSynchronize(st));
+  if (h_status & 4) throw Error(PFW_INVALID_ARGUMENT, "nonfinite feature cotangent");
   if (h_status & 1) throw Error(PFW_DOMAIN, "coincident physical atoms have undefined harmonics");
   if (h_status & 2)
     for (int64_t t = 0; t < 3 * p.n; ++t) dx[t] += horph[t];

@@ -353,7 +353,9 @@ def feature_vjp(layer, system, ps, cotangents, workspace=None):
         blocks.append(c.reshape(-1, order="F"))
     if len(blocks) != len(layer.irreps):
         raise DimensionMismatch("one cotangent per feature block")
-    cot = np.ascontiguousarray(np.concatenate(blocks))
+    cot = pinned_empty(sum(b.size for b in blocks))  # the upload is then one DMA
+    if cot.size:
+        np.concatenate(blocks, out=cot)
     w, st, keep = _prepared(layer, system, ps, workspace)
     nz, nr = len(layer.atomic_numbers), len(layer.radial_spec)
     dx = pinned_empty(3 * n)  # filled completely by the call

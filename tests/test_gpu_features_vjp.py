@@ -6,7 +6,7 @@ the fixture layer open and periodic with all blocks and single-block
 cotangents, a correlation-3 layer with L up to 3, a 54-atom periodic cell,
 translation invariance of the position gradient, the rrule's saved inputs,
 an empty system, the periodic self-image atom (zero position gradient,
-nonzero mixing gradient) and bad cotangent shapes."""
+nonzero mixing gradient), bad cotangent shapes and a NaN cotangent."""
 import numpy as np
 import pytest
 
@@ -106,6 +106,10 @@ def test_empty_self_image_and_shapes(pfw):
     assert np.abs(g["positions"]).max() < 1e-12
     assert np.linalg.norm(g["parameters"]["mixing"]["block_1"]) > 0
     sys_ = pfw.System(X3, H3, (0, 0, 0), ["Si", "O", "Si"])
+    bad = [np.ones((2 * s.L + 1, s.channels, 3)) for s in layer.irreps]
+    bad[1][0, 0, 2] = np.nan
+    with pytest.raises(pfw.ArgumentError):
+        pfw.feature_vjp(layer, sys_, ps, bad)
     with pytest.raises(pfw.DimensionMismatch):
         pfw.feature_vjp(layer, sys_, ps, [np.ones((1, 3, 2))] + [None] * (len(layer.irreps) - 1))
     with pytest.raises(pfw.DimensionMismatch):
