@@ -1944,6 +1944,7 @@ __device__ inline void feat_rows(const FeatArgs& a, const double* A, double* B) 
   const int lane = threadIdx.x;
   for (int t = lane; t < a.nrows; t += kWave) {
     double v = 0;
+#pragma unroll 4
     for (int q = a.rptr[t]; q < a.rptr[t + 1]; ++q) {
       const FeatEntry e = a.ent[q];
       const double p = A[e.f & 0xffff] * A[(e.f >> 16) & 0xffff] * A[(e.f >> 32) & 0xffff] * A[e.f >> 48];
@@ -2073,6 +2074,7 @@ __global__ __launch_bounds__(64) void k_features_vjp(FeatArgs a, FeatVjpArgs v) 
     }
   }
   __syncthreads();
+#pragma unroll 4
   for (int64_t q = lane; q < a.nnz; q += kWave) {  // dA by the product rule, per lane
     const FeatEntry e = a.ent[q];
     const double c = e.val * B[a.erow[q]];
