@@ -2239,6 +2239,7 @@ __global__ void k_species_rows(int64_t n, int wp, int nz, const double* __restri
     double s[kZ];
 #pragma unroll
     for (int u = 0; u < kZ; ++u) s[u] = 0;
+#pragma unroll 4
     for (int64_t t = a0; t < a1; ++t) {
       const int z = species[t] - z0;
       const double x = prow[t * wp + col];
@@ -2255,6 +2256,7 @@ __global__ void k_species_total(int64_t nchunks, int64_t width, const double* __
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= width) return;
   double s = 0;
+#pragma unroll 8
   for (int64_t c = 0; c < nchunks; ++c) s += part[c * width + t];
   out[t] = s;
 }
