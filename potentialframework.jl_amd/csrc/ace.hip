@@ -1976,12 +1976,23 @@ __global__ __launch_bounds__(64) void k_features(FeatArgs a) {
   for (int b = 0; b < a.nblocks; ++b) {
     const int64_t* o = a.boff + 6 * b;
     const int dim = (int)o[3], nb = (int)o[4], ch = (int)o[5];
-    for (int t = lane; t < dim * ch; t += kWave) {
-      const int m = t % dim, c = t / dim;
-      const double* W = a.mix + o[1] + (int64_t)nb * (c + (int64_t)ch * zi);
+    const int nout = dim * ch;
+    int P = 1;  // lanes per output (a power of two), summing every P-th basis function
+    while (P < 8 && nout * P * 2 <= kWave) P *= 2;
+    for (int t0 = 0; t0 < nout * P; t0 += kWave) {
+      const int t = t0 + lane, oi = t / P, part = t % P;
       double v = 0;
-      for (int beta = 0; beta < nb; ++beta) v += B[o[0] + beta * dim + m] * W[beta];
-      a.out[o[2] + m + (int64_t)dim * (c + (int64_t)ch * i)] = v;
+      if (oi < nout) {
+        const int m = oi % dim, c = oi / dim;
+        const double* W = a.mix + o[1] + (int64_t)nb * (c + (int64_t)ch * zi);
+#pragma unroll 4
+        for (int beta = part; beta < nb; beta += P) v += B[o[0] + beta * dim + m] * W[beta];
+      }
+      for (int off = 1; off < P; off <<= 1) v += __shfl_xor(v, off, kWave);  // fixed tree over the P lanes
+      if (oi < nout && part == 0) {
+        const int m = oi % dim, c = oi / dim;
+        a.out[o[2] + m + (int64_t)dim * (c + (int64_t)ch * i)] = v;
+      }
     }
   }
 }
