@@ -149,6 +149,24 @@ int pfw_grid_info(pfw_ctx* ctx, int64_t* ncells, int32_t* dims, int32_t* minimum
 int pfw_set_timing(pfw_ctx* ctx, int enabled);
 int pfw_get_timings(pfw_ctx* ctx, double* ms, int n);
 
+/* Pair-set audit, a diagnostic for parity tests (no counterpart in the
+ * reference, whose CPU executor and GPU kernels read one neighbour set). While
+ * enabled, every pair step (fresh cells or Verlet list) and ACE step records,
+ * per centre atom i in original order, the number of directed pairs (j, S)
+ * the evaluating kernel actually accumulated (count) and the wrapping 64-bit
+ * sum of a mix of each pair's identity (hash): j the neighbour's original
+ * index, S the integer image shift relative to the caller's original
+ * coordinates, R = x_j - x_i + H S, as in the directed snapshot of
+ * visit_directed_images! (src/submodules/NeighbourList/interface.jl:116-143).
+ * The mix is splitmix64's finaliser of j | (S+1024 & 2047) << 32 | ... << 43 |
+ * ... << 54 after adding 0x9e3779b97f4a7c15, so a caller can digest a
+ * reference pair set the same way and compare bit for bit. Audited steps are
+ * slower (one atomic per pair). pfw_read_audit copies the last pair or ACE
+ * step's digests (n each; either pointer may be NULL); PFW_STALE when no step
+ * ran since the audit was enabled or the system changed. */
+int pfw_set_audit(pfw_ctx* ctx, int enabled);
+int pfw_read_audit(pfw_ctx* ctx, int32_t* count, uint64_t* hash);
+
 /* Synchronise the context's stream. */
 int pfw_synchronize(pfw_ctx* ctx);
 

@@ -23,7 +23,7 @@ from ._capi import (  # noqa: F401
 )
 from .system import System  # noqa: F401
 from . import workloads  # noqa: F401
-from .neighbourlist import Workspace, neighbour_list, neighbours, NeighbourSnapshot  # noqa: F401
+from .neighbourlist import Workspace, neighbour_list, neighbours, NeighbourSnapshot, pair_set_digest  # noqa: F401
 from .pair import (  # noqa: F401
     SmoothPair,
     RadialPotential,

@@ -72,6 +72,8 @@ SIGNATURES = [
     ("pfw_verlet_info", ctypes.c_int, [_P, _I64, _D]),
     ("pfw_set_timing", ctypes.c_int, [_P, ctypes.c_int]),
     ("pfw_get_timings", ctypes.c_int, [_P, _D, ctypes.c_int]),
+    ("pfw_set_audit", ctypes.c_int, [_P, ctypes.c_int]),
+    ("pfw_read_audit", ctypes.c_int, [_P, _I32, ctypes.POINTER(ctypes.c_uint64)]),
     ("pfw_synchronize", ctypes.c_int, [_P]),
     ("pfw_host_alloc", ctypes.c_void_p, [ctypes.c_size_t]),
     ("pfw_host_free", None, [ctypes.c_void_p]),

@@ -825,12 +825,18 @@ __global__ __launch_bounds__(256) void k_ace_delta(int64_t n, const int32_t* __r
 // skin-row order (a ballot compaction, one wavefront per site). The reverse
 // edge's R is exactly -R (R_build negates exactly, and so does the
 // displacement difference), so an edge and its reverse are kept together.
+// An edge whose r^2 lies within a relative 1e-10 of c^2 is re-decided by
+// exact_image_member from the caller's positions X, as a fresh build decides
+// it (a symmetric test), so the kept set is a fresh build's bit for bit.
 __global__ __launch_bounds__(64) void k_ace_compact(int64_t n, int ecap, const int32_t* __restrict__ ecount_s,
                                                     const int32_t* __restrict__ enb_s,
                                                     const double* __restrict__ eR_s, const double* __restrict__ dX,
                                                     double c2, int32_t* __restrict__ ecount,
                                                     int32_t* __restrict__ enb, double* __restrict__ eR,
-                                                    int32_t* __restrict__ cslot, int32_t* __restrict__ corig) {
+                                                    int32_t* __restrict__ cslot, int32_t* __restrict__ corig,
+                                                    const int32_t* __restrict__ perm, const Grid* __restrict__ grids,
+                                                    const int32_t* __restrict__ atom_cfg,
+                                                    const double* __restrict__ X) {
   const int64_t i = blockIdx.x;
   const int lane = threadIdx.x;
   const int64_t base = i * ecap;
@@ -848,7 +854,15 @@ __global__ __launch_bounds__(64) void k_ace_compact(int64_t n, int ecap, const i
       r[0] = eR_s[3 * e] + (dX[j] - xi);
       r[1] = eR_s[3 * e + 1] + (dX[n + j] - yi);
       r[2] = eR_s[3 * e + 2] + (dX[2 * n + j] - zi);
-      act = r[0] * r[0] + r[1] * r[1] + r[2] * r[2] < c2;
+      const double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+      act = r2 < c2;
+      if (fabs(r2 - c2) <= c2 * 1e-10) {
+        const int oi = perm[i], oj = perm[j];
+        const Grid& g = grids[atom_cfg[oi]];
+        int S[3];
+        audit_shift(g, r, X, oi, oj, S);
+        act = exact_image_member(g, X, oi, oj, S);
+      }
     }
     const uint64_t mask = __ballot(act);
     const int c = pos + __popcll(mask & ((1ull << lane) - 1));
@@ -879,6 +893,21 @@ __global__ void k_ace_revmap(int64_t slots, int ecap, const int32_t* __restrict_
   const int r = rs >= 0 ? cslot[rs] : -1;
   erev[e] = r;
   if (r < 0) atomicOr(status, 1);
+}
+
+// Pair-set digest of the rows the site kernels evaluated: every used slot's
+// (neighbour, image) with its exact FP64 R.
+__global__ void k_ace_audit(int64_t slots, int ecap, const int32_t* __restrict__ ecount,
+                            const int32_t* __restrict__ enb, const double* __restrict__ eR,
+                            const int32_t* __restrict__ perm, const Grid* __restrict__ grids,
+                            const int32_t* __restrict__ atom_cfg, AuditBufs au) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= slots) return;
+  const int64_t i = e / ecap;
+  if (e - i * ecap >= ecount[i]) return;
+  const int oi = perm[i];
+  const double d[3] = {eR[3 * e], eR[3 * e + 1], eR[3 * e + 2]};
+  audit_add(au, grids[atom_cfg[oi]], d, oi, perm[enb[e]]);
 }
 
 // siteE and siteW packed as 10-column records for the two-stage reduction.
@@ -1784,7 +1813,8 @@ static void ace_skin_step(pfw_ctx* ctx, int grads, int precision) {
     hipLaunchKernelGGL(k_ace_delta, dim3(nblk(n, 256)), dim3(256), 0, st, n, s.perm, s.X, ad.Xref, ad.dX, ad.maxd2,
                        ad.parity);
     hipLaunchKernelGGL(k_ace_compact, dim3((unsigned)n), dim3(kWave), 0, st, n, ad.ecap, ad.ecount_s, ad.enb_s,
-                       ad.eR_s, ad.dX, ad.cutoff * ad.cutoff, ad.ecount, ad.enb, ad.eR, ad.cslot, ad.corig);
+                       ad.eR_s, ad.dX, ad.cutoff * ad.cutoff, ad.ecount, ad.enb, ad.eR, ad.cslot, ad.corig,
+                       s.perm, s.grids, s.atom_cfg, s.X);
     if (grads)
       hipLaunchKernelGGL(k_ace_revmap, dim3(nblk(n * (int64_t)ad.ecap, 256)), dim3(256), 0, st, n * (int64_t)ad.ecap,
                          ad.ecap, ad.ecount, ad.corig, ad.erev_s, ad.cslot, ad.erev, ad.status);
@@ -1808,6 +1838,7 @@ static void ace_skin_step(pfw_ctx* ctx, int grads, int precision) {
 
 void ace_run(pfw_ctx* ctx, uint32_t mask, int precision) {
   ctx->host_written = false;
+  ctx->audit_valid = false;
   AceDev& ad = ctx->ace;
   if (!ad.ready) throw Error(PFW_STALE, "no ACE model set");
   if (ad.cutoff != ctx->cutoff)
@@ -1839,6 +1870,16 @@ void ace_run(pfw_ctx* ctx, uint32_t mask, int precision) {
       ace_run(ctx, mask, precision);
       return;
     }
+  }
+  if (ctx->audit) {
+    const AuditBufs au = audit_begin(ctx);
+    if (n > 0 && mask != 0) {
+      const int64_t slots = n * (int64_t)ad.ecap;
+      hipLaunchKernelGGL(k_ace_audit, dim3(nblk(slots, 256)), dim3(256), 0, st, slots, ad.ecap, ad.ecount, ad.enb,
+                         ad.eR, s.perm, s.grids, s.atom_cfg, au);
+      PFW_HIP(hipGetLastError());
+    }
+    PFW_HIP(hipStreamSynchronize(st));
   }
   if (ctx->ncfg > 1) {
     ctx->h_cfg_out.resize(11 * (size_t)ctx->ncfg);

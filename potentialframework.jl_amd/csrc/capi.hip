@@ -109,6 +109,8 @@ void pfw_ctx_destroy(pfw_ctx* ctx) {
   if (ctx->d_flag) (void)hipFree(ctx->d_flag);
   if (ctx->h_flag) (void)hipHostFree(ctx->h_flag);
   if (ctx->d_Xstage) (void)hipFree(ctx->d_Xstage);
+  if (ctx->audit_count) (void)hipFree(ctx->audit_count);
+  if (ctx->audit_hash) (void)hipFree(ctx->audit_hash);
   if (ctx->pair.d_params) (void)hipFree(ctx->pair.d_params);
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
@@ -134,6 +136,7 @@ int pfw_set_batch(pfw_ctx* ctx, int32_t ncfg, const int64_t* offsets, const doub
       if (species[k] < 0) throw Error(PFW_INVALID_ARGUMENT, "species channels must be nonnegative");
     ctx->have_system = false;
     ctx->have_results = false;
+    ctx->audit_valid = false;
     pfw::nl_free(ctx);
     pfw::ace_free_edges(ctx);
     pfw::verlet_free(ctx);
@@ -349,6 +352,23 @@ int pfw_get_timings(pfw_ctx* ctx, double* ms, int n) {
   int m = std::min(n, pfw::kStages);
   for (int k = 0; k < m; ++k) ms[k] = ctx->stage_ms[k];
   return m;
+}
+
+int pfw_set_audit(pfw_ctx* ctx, int enabled) {
+  return run(ctx, [&] {
+    ctx->audit = enabled != 0;
+    ctx->audit_valid = false;
+  });
+}
+
+int pfw_read_audit(pfw_ctx* ctx, int32_t* count, uint64_t* hash) {
+  return run(ctx, [&] {
+    if (!ctx->audit || !ctx->audit_valid) throw Error(PFW_STALE, "no audited pair or ACE step since the audit was enabled");
+    const int64_t n = ctx->sys.n;
+    if (n == 0) return;
+    if (count) PFW_HIP(hipMemcpy(count, ctx->audit_count, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (hash) PFW_HIP(hipMemcpy(hash, ctx->audit_hash, n * sizeof(uint64_t), hipMemcpyDeviceToHost));
+  });
 }
 
 int pfw_set_ace_model(pfw_ctx* ctx, const pfw_ace_model* model) {

@@ -261,29 +261,6 @@ void nl_configure(pfw_ctx* ctx, double c, double skin) {
 
 // ---------------------------------------------------------------- kernels
 
-// Julia mod(x, y) for y > 0 with a fast path for x already in [0, y): mod
-// returns x itself there, except -0.0, which becomes +0.0.
-__device__ inline double mod_pos(double x, double y) {
-  if (x >= 0 && x < y) return x == 0 ? 0.0 : x;
-  return jl_mod(x, y);
-}
-
-__device__ inline void wrap_atom(const Grid& g, const double* x, double* w, double* u) {
-  if (g.minimum_image) {
-    for (int a = 0; a < 3; ++a) {
-      w[a] = g.pb[a] ? mod_pos(x[a], g.L[a]) : x[a];
-      u[a] = w[a];
-    }
-  } else {
-    double f[3];
-    exact_matvec(g.recip, x, f);
-    for (int a = 0; a < 3; ++a)
-      if (g.pb[a]) f[a] = mod_pos(f[a], 1.0);
-    exact_matvec(g.basis, f, w);
-    for (int a = 0; a < 3; ++a) u[a] = f[a];
-  }
-}
-
 __device__ inline int bin_atom(const Grid& g, const double* u) {
   int k[3];
   for (int a = 0; a < 3; ++a) {
@@ -681,6 +658,24 @@ void reduce_outputs(pfw_ctx* ctx, const double* rec, int64_t m, int width, int k
                      ctx->d_out, ctx->d_cfg_out, ctx->h_out_dev);
   PFW_HIP(hipGetLastError());
   ctx->host_written = true;
+}
+
+AuditBufs audit_begin(pfw_ctx* ctx) {
+  const int64_t n = ctx->sys.n;
+  if (ctx->audit_cap < n || !ctx->audit_count) {
+    if (ctx->audit_count) (void)hipFree(ctx->audit_count);
+    if (ctx->audit_hash) (void)hipFree(ctx->audit_hash);
+    ctx->audit_count = nullptr;
+    ctx->audit_hash = nullptr;
+    ctx->audit_cap = 0;
+    PFW_HIP(hipMalloc(&ctx->audit_count, std::max<int64_t>(n, 1) * sizeof(int32_t)));
+    PFW_HIP(hipMalloc(&ctx->audit_hash, std::max<int64_t>(n, 1) * sizeof(unsigned long long)));
+    ctx->audit_cap = std::max<int64_t>(n, 1);
+  }
+  PFW_HIP(hipMemsetAsync(ctx->audit_count, 0, std::max<int64_t>(n, 1) * sizeof(int32_t), ctx->stream));
+  PFW_HIP(hipMemsetAsync(ctx->audit_hash, 0, std::max<int64_t>(n, 1) * sizeof(unsigned long long), ctx->stream));
+  ctx->audit_valid = true;
+  return AuditBufs{ctx->audit_count, ctx->audit_hash, ctx->sys.X};
 }
 
 void copy_outputs(pfw_ctx* ctx) {

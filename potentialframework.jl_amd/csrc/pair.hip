@@ -90,13 +90,14 @@ __device__ inline double prefilter_r2(const Cand<double>& c, double xi, double y
 // Five wavefronts per SIMD for the FP32 kernel (at most 96 registers): the
 // candidate fills and LDS round trips of one wavefront hide behind the tests
 // of the others.
-template <typename T, int KIND>
+// AUDIT: also add every accumulated pair to the per-atom pair-set digest.
+template <typename T, int KIND, bool AUDIT>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 5 : 4))) void k_pair(const Grid* __restrict__ grids, const int32_t* __restrict__ cell_cfg,
                                              int64_t n, const int32_t* __restrict__ start,
                                              const double* __restrict__ xs, const int32_t* __restrict__ sps,
                                              const int32_t* __restrict__ perm, const double* __restrict__ params,
                                              int nz, LJConsts lj, uint32_t mask, double* __restrict__ F,
-                                             double* __restrict__ partials, int ncells) {
+                                             double* __restrict__ partials, int ncells, AuditBufs au) {
   // candidates as (x, y, z, (1 - slack) |x|^2), then far sentinels up to the end
   // of the last slice's range; after the candidate loops of an atom group the
   // same bytes hold the per-slice force sums
@@ -264,6 +265,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
                                     e_sh[el], d, img))
                     continue;
                 }
+                if constexpr (AUDIT) {
+                  const int t = k * S + split;
+                  const int m = e_start[el] + (t0 + t - e_pref[el]);
+                  double d[3];
+                  for (int a = 0; a < 3; ++a) d[a] = (xs[a * n + m] + e_sh[el][a]) - xs[a * n + is];
+                  audit_add(au, g, d, perm[is], perm[m]);
+                }
                 T e, coef;
                 if (KIND == PFW_PAIR_SMOOTH) {
                   const T a = sA[zsp * nz + zj], a6 = sA6[zsp * nz + zj];
@@ -352,13 +360,21 @@ static void launch_pair(pfw_ctx* ctx, uint32_t mask) {
   SystemBuffers& s = ctx->sys;
   const LJConsts lj = lj_consts(ctx->pair);
   const size_t dyn = KIND == PFW_PAIR_SMOOTH ? 2 * sizeof(T) * ctx->pair.nz * ctx->pair.nz : 0;
-  hipLaunchKernelGGL((k_pair<T, KIND>), dim3(ctx->ncells), dim3(kWave), dyn, ctx->stream, s.grids, s.cell_cfg, s.n,
-                     s.start, s.xs, s.sps, s.perm, ctx->pair.d_params, ctx->pair.nz, lj, mask, ctx->d_F,
-                     ctx->d_partials, ctx->ncells);
+  if (ctx->audit) {
+    const AuditBufs au = audit_begin(ctx);
+    hipLaunchKernelGGL((k_pair<T, KIND, true>), dim3(ctx->ncells), dim3(kWave), dyn, ctx->stream, s.grids,
+                       s.cell_cfg, s.n, s.start, s.xs, s.sps, s.perm, ctx->pair.d_params, ctx->pair.nz, lj, mask,
+                       ctx->d_F, ctx->d_partials, ctx->ncells, au);
+  } else {
+    hipLaunchKernelGGL((k_pair<T, KIND, false>), dim3(ctx->ncells), dim3(kWave), dyn, ctx->stream, s.grids,
+                       s.cell_cfg, s.n, s.start, s.xs, s.sps, s.perm, ctx->pair.d_params, ctx->pair.nz, lj, mask,
+                       ctx->d_F, ctx->d_partials, ctx->ncells, AuditBufs{});
+  }
 }
 
 void pair_run(pfw_ctx* ctx, uint32_t mask, int precision) {
   ctx->host_written = false;
+  ctx->audit_valid = false;
   if (ctx->pair.kind < 0) throw Error(PFW_STALE, "no pair model set");
   if (ctx->pair.cutoff != ctx->cutoff)
     throw Error(PFW_STALE, "pair model cutoff differs from the system's cell-list cutoff; set the system again");
@@ -395,6 +411,7 @@ void pair_run(pfw_ctx* ctx, uint32_t mask, int precision) {
     }
     PFW_HIP(hipGetLastError());
   } else {
+    if (ctx->audit) audit_begin(ctx);  // nothing accumulated: empty digests
     record_stage(ctx, 5);
     PFW_HIP(hipMemsetAsync(ctx->d_out, 0, 11 * sizeof(double), st));
     PFW_HIP(hipMemsetAsync(ctx->d_cfg_out, 0, 11 * (size_t)ctx->ncfg * sizeof(double), st));

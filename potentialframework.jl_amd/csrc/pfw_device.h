@@ -142,6 +142,112 @@ __device__ inline int stencil_entry(const Grid& g, const int* k, int e, int* lat
   return cell_index(g, nk);
 }
 
+// Julia mod(x, y) for y > 0 with a fast path for x already in [0, y): mod
+// returns x itself there, except -0.0, which becomes +0.0.
+__device__ inline double mod_pos(double x, double y) {
+  if (x >= 0 && x < y) return x == 0 ? 0.0 : x;
+  return jl_mod(x, y);
+}
+
+__device__ inline void wrap_atom(const Grid& g, const double* x, double* w, double* u) {
+  if (g.minimum_image) {
+    for (int a = 0; a < 3; ++a) {
+      w[a] = g.pb[a] ? mod_pos(x[a], g.L[a]) : x[a];
+      u[a] = w[a];
+    }
+  } else {
+    double f[3];
+    exact_matvec(g.recip, x, f);
+    for (int a = 0; a < 3; ++a)
+      if (g.pb[a]) f[a] = mod_pos(f[a], 1.0);
+    exact_matvec(g.basis, f, w);
+    for (int a = 0; a < 3; ++a) u[a] = f[a];
+  }
+}
+
+// Winding of caller position x (original coordinates): the lattice vector
+// that wrap_atom takes off, fld of the fractional coordinate on periodic axes.
+__device__ inline void winding_of(const Grid& g, const double* x, int* w) {
+  if (g.minimum_image) {
+    for (int a = 0; a < 3; ++a) w[a] = g.pb[a] ? (int)floor(x[a] / g.L[a]) : 0;
+  } else {
+    double f[3];
+    exact_matvec(g.recip, x, f);
+    for (int a = 0; a < 3; ++a) w[a] = g.pb[a] ? (int)floor(f[a]) : 0;
+  }
+}
+
+// Exact membership of the image (oj, S) of centre oi (original indices, S
+// relative to the caller's coordinates X) decided as a fresh cell build
+// decides it: both positions wrapped now, the image lattice in the wrapped
+// frame l = S - w_i + w_j, then exact_member. Used to re-decide pairs found
+// by other arithmetic (list reuse) inside a thin band around the cutoff.
+__device__ inline bool exact_image_member(const Grid& g, const double* __restrict__ X, int oi, int oj, const int* S) {
+  double xi[3], xj[3], wi_[3], wj_[3], ui[3], uj[3];
+  int wi[3], wj[3], lat[3];
+  for (int a = 0; a < 3; ++a) {
+    xi[a] = X[3 * (int64_t)oi + a];
+    xj[a] = X[3 * (int64_t)oj + a];
+  }
+  wrap_atom(g, xi, wi_, ui);
+  wrap_atom(g, xj, wj_, uj);
+  winding_of(g, xi, wi);
+  winding_of(g, xj, wj);
+  for (int a = 0; a < 3; ++a) lat[a] = S[a] - wi[a] + wj[a];
+  double sh[3], d[3];
+  int img[3];
+  exact_shift(g, lat, sh);
+  return exact_member(g, wi_[0], wi_[1], wi_[2], wj_[0], wj_[1], wj_[2], lat, sh, d, img);
+}
+
+// ---- pair-set audit: an order-independent digest of the directed pairs an
+// evaluating kernel actually accumulated, per centre atom (original order).
+// A pair is identified by the neighbour's original index j and its integer
+// image shift S relative to the caller's original coordinates, R = x_j - x_i +
+// H S (the convention of the directed snapshot, interface.jl:116-143). The
+// digest is the count and the wrapping sum of a 64-bit mix of (j, S).
+__device__ __host__ inline unsigned long long audit_mix(int j, const int* S) {
+  unsigned long long z = (unsigned long long)(uint32_t)j;
+  z |= (unsigned long long)((S[0] + 1024) & 2047) << 32;
+  z |= (unsigned long long)((S[1] + 1024) & 2047) << 43;
+  z |= (unsigned long long)((S[2] + 1024) & 2047) << 54;
+  // splitmix64 finaliser (Steele, Lea and Flood, OOPSLA 2014)
+  z += 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+// The image shift of a pair from the FP64 displacement d the kernel used and
+// the caller's positions X (3 x N, original order): H S = d - (x_j - x_i) up
+// to rounding, so S is that difference in lattice coordinates, rounded.
+__device__ inline void audit_shift(const Grid& g, const double* d, const double* __restrict__ X, int oi, int oj,
+                                   int* S) {
+  double t[3];
+  for (int a = 0; a < 3; ++a) t[a] = d[a] - (X[3 * (int64_t)oj + a] - X[3 * (int64_t)oi + a]);
+  if (g.minimum_image) {
+    for (int a = 0; a < 3; ++a) S[a] = g.pb[a] ? (int)rint(t[a] / g.L[a]) : 0;
+  } else {
+    for (int a = 0; a < 3; ++a) {
+      const double f = g.recip[a] * t[0] + g.recip[a + 3] * t[1] + g.recip[a + 6] * t[2];
+      S[a] = g.pb[a] ? (int)rint(f) : 0;
+    }
+  }
+}
+struct AuditBufs {
+  int32_t* count;             // n, original order
+  unsigned long long* hash;   // n, original order
+  const double* X;            // the caller's positions, 3 x N original order
+};
+__device__ inline void audit_add(const AuditBufs& au, const Grid& g, const double* d, int oi, int oj) {
+  int S[3];
+  audit_shift(g, d, au.X, oi, oj, S);
+  atomicAdd(au.count + oi, 1);
+  atomicAdd(au.hash + oi, audit_mix(oj, S));
+}
+
+// Cleared digest buffers for the next evaluating kernel (allocated on first use).
+AuditBufs audit_begin(pfw_ctx* ctx);
+
 __device__ inline bool is_finite(double x) { return isfinite(x); }
 
 // Order-preserving map of doubles onto unsigned 64-bit integers (for atomic min/max).

@@ -287,6 +287,13 @@ struct pfw_ctx {
   bool host_written = false;     // h_out already holds this step's outputs (no copy needed)
   uint32_t last_mask = 0;
   bool have_results = false;
+  // pair-set audit (a diagnostic: per-atom digests of the pairs the
+  // evaluating kernels accumulated)
+  bool audit = false;
+  int32_t* audit_count = nullptr;           // n, original order
+  unsigned long long* audit_hash = nullptr;  // n, original order
+  int64_t audit_cap = 0;
+  bool audit_valid = false;                 // the digests belong to the last step
   // timing
   bool timing = false;
   hipEvent_t ev[pfw::kStages + 1] = {};

@@ -298,12 +298,21 @@ __global__ __launch_bounds__(256) void k_verlet_refresh(int64_t n, int64_t n_ext
 // Records per wavefront (E, xx yy zz xy xz yz of d (x) g) or, for a batch, per atom.
 constexpr int kListParts = 4;   // lanes per atom
 constexpr int kListUnroll = 4;  // slots per lane per step; the row capacity is a multiple of P U
-template <typename T, int KIND>
+// Membership: r^2 < c^2 of the current FP64 displacement; a pair whose r^2
+// lies within a relative 1e-10 of c^2 is re-decided by exact_image_member from
+// the caller's positions, as a fresh cell build decides it, so the pair set
+// is that of a fresh build bit for bit (X, grids, atom_cfg and srcp are read
+// only there). AUDIT: also add every accumulated pair to the per-atom
+// pair-set digest.
+template <typename T, int KIND, bool AUDIT>
 __global__ __launch_bounds__(64) void k_pair_list(int64_t n, int sentinel, const int32_t* __restrict__ ncount,
                                                   const int32_t* __restrict__ nbr, const double4* __restrict__ xe,
                                                   const int32_t* __restrict__ perm, const double* __restrict__ params,
                                                   int nz, double c2, LJConsts lj, uint32_t mask,
-                                                  double* __restrict__ F, double* __restrict__ rec, int per_atom) {
+                                                  double* __restrict__ F, double* __restrict__ rec, int per_atom,
+                                                  AuditBufs au, const Grid* __restrict__ grids,
+                                                  const int32_t* __restrict__ atom_cfg,
+                                                  const int32_t* __restrict__ srcp, const double* __restrict__ X) {
   constexpr int P = kListParts, U = kListUnroll, NA = kWave / P;
   extern __shared__ __align__(16) unsigned char dyn[];  // SmoothPair: nz x nz amplitudes, then -6 A / c^2
   T* sA = (T*)dyn;
@@ -352,7 +361,22 @@ __global__ __launch_bounds__(64) void k_pair_list(int64_t n, int sentinel, const
     for (int u = 0; u < U; ++u) {
       const double ddx = xj[u].x - xi.x, ddy = xj[u].y - xi.y, ddz = xj[u].z - xi.z;
       const double rr = ddx * ddx + ddy * ddy + ddz * ddz;
-      const bool hit = rr < c2;
+      bool hit = rr < c2;
+      if (fabs(rr - c2) <= c2 * 1e-10) {  // rare: decide as a fresh build would
+        const double d[3] = {ddx, ddy, ddz};
+        const int oi = perm[i], oj = srcp[j[u]];
+        const Grid& g = grids[atom_cfg[oi]];
+        int S[3];
+        audit_shift(g, d, X, oi, oj, S);
+        hit = exact_image_member(g, X, oi, oj, S);
+      }
+      if constexpr (AUDIT) {
+        if (hit) {
+          const double d[3] = {ddx, ddy, ddz};
+          const int oi = perm[i];
+          audit_add(au, grids[atom_cfg[oi]], d, oi, srcp[j[u]]);
+        }
+      }
       const int zj = (int)__double_as_longlong(xj[u].w);
       const T dx = (T)ddx, dy = (T)ddy, dz = (T)ddz, r2 = (T)rr;
       T e, coef;
@@ -557,11 +581,20 @@ static void launch_list(pfw_ctx* ctx, uint32_t mask, bool per_atom) {
   VerletDev& V = ctx->verlet;
   const int64_t n = ctx->sys.n;
   const size_t dyn = KIND == PFW_PAIR_SMOOTH ? 2 * sizeof(T) * ctx->pair.nz * ctx->pair.nz : 0;
-  hipLaunchKernelGGL((k_pair_list<T, KIND>), dim3(nblk(n, kWave / kListParts)), dim3(kWave), dyn, ctx->stream, n,
-                     (int)V.n_ext, V.ncount, V.nbr,
-                     (const double4*)V.xe, ctx->sys.perm, ctx->pair.d_params, ctx->pair.nz,
-                     ctx->pair.cutoff * ctx->pair.cutoff, lj_consts(ctx->pair), mask, ctx->d_F, V.part,
-                     (int)per_atom);
+  if (ctx->audit) {
+    const AuditBufs au = audit_begin(ctx);
+    hipLaunchKernelGGL((k_pair_list<T, KIND, true>), dim3(nblk(n, kWave / kListParts)), dim3(kWave), dyn,
+                       ctx->stream, n, (int)V.n_ext, V.ncount, V.nbr, (const double4*)V.xe, ctx->sys.perm,
+                       ctx->pair.d_params, ctx->pair.nz, ctx->pair.cutoff * ctx->pair.cutoff, lj_consts(ctx->pair),
+                       mask, ctx->d_F, V.part, (int)per_atom, au, ctx->sys.grids, ctx->sys.atom_cfg, V.srcp,
+                       ctx->sys.X);
+  } else {
+    hipLaunchKernelGGL((k_pair_list<T, KIND, false>), dim3(nblk(n, kWave / kListParts)), dim3(kWave), dyn,
+                       ctx->stream, n, (int)V.n_ext, V.ncount, V.nbr, (const double4*)V.xe, ctx->sys.perm,
+                       ctx->pair.d_params, ctx->pair.nz, ctx->pair.cutoff * ctx->pair.cutoff, lj_consts(ctx->pair),
+                       mask, ctx->d_F, V.part, (int)per_atom, AuditBufs{}, ctx->sys.grids, ctx->sys.atom_cfg,
+                       V.srcp, ctx->sys.X);
+  }
 }
 
 // One step on the list: refresh of the extended positions with the

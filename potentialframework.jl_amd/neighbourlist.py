@@ -147,11 +147,55 @@ class Workspace:
                       ptr(R), ctypes.c_int64(m), ptr(count, ctypes.c_int64))
         return i, j, S, R
 
+    # -- pair-set audit ---------------------------------------------------------------
+    def set_audit(self, enabled=True):
+        """Record, for every later pair or ACE step, per-atom digests of the directed
+        pairs the evaluating kernel accumulated (``pfw_set_audit``)."""
+        self.ctx.call("pfw_set_audit", int(bool(enabled)))
+
+    def read_audit(self, n):
+        """``(count, hash)`` of the last audited step: per centre atom (original
+        order) the number of accumulated directed pairs and the wrapping uint64
+        sum of ``pair_digest`` over them."""
+        count = np.zeros(n, np.int32)
+        digest = np.zeros(n, np.uint64)
+        self.ctx.call("pfw_read_audit", ptr(count, ctypes.c_int32), ptr(digest, ctypes.c_uint64))
+        return count, digest
+
     def read_timings(self):
         ms = np.zeros(len(STAGES))
         _capi.load().pfw_get_timings(self.ctx.h, ptr(ms), len(STAGES))
         self.timings = dict(zip(STAGES, ms.tolist()))
         return self.timings
+
+
+_MASK64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def pair_digest(j, S):
+    """The 64-bit mix of pair identities ``(j, S)`` used by the pair-set audit
+    (``pfw_set_audit``): splitmix64's finaliser of j | (S + 1024 & 2047) << 32 |
+    ... << 43 | ... << 54, after adding the golden-ratio increment."""
+    j = np.asarray(j, np.int64).astype(np.uint64) & np.uint64(0xFFFFFFFF)
+    S = (np.asarray(S, np.int64).reshape(-1, 3) + 1024) & 2047
+    z = j | (S[:, 0].astype(np.uint64) << np.uint64(32)) | (S[:, 1].astype(np.uint64) << np.uint64(43)) \
+        | (S[:, 2].astype(np.uint64) << np.uint64(54))
+    with np.errstate(over="ignore"):
+        z = z + np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def pair_set_digest(n, i, j, S):
+    """Per-atom ``(count, hash)`` of a directed pair set given as arrays
+    ``(i, j, S)`` (original indices), in the audit's convention."""
+    i = np.asarray(i, np.int64)
+    count = np.bincount(i, minlength=n).astype(np.int32)
+    digest = np.zeros(n, np.uint64)
+    if len(i):
+        np.add.at(digest, i, pair_digest(j, S))
+    return count, digest
 
 
 class NeighbourSnapshot:

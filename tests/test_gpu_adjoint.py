@@ -227,3 +227,62 @@ def test_fitting_loss_gradient_central_difference(pfw, synthetic):
     L0, g0 = loss_at(WBt)
     assert L0 <= 1e-18 * max(1.0, L)
     assert np.abs(g0["WB"]).max() <= 1e-9 * np.abs(g["WB"]).max()
+
+
+def test_c5_full_fitting_set(pfw, synthetic):
+    """The configuration-5 workload itself: cotangents over all 1,024
+    fitting-set configurations (3.6e5 atoms). The FP64 ACE adjoint meets the
+    oracle's exact linear functional along one random direction and at two
+    single components; the mixed adjoint meets the FP64 one to 1e-4 and the
+    same directional check to 1e-4; the SmoothPair FP64 adjoint equals the
+    oracle's unit-basis evaluations (parameters.jl:126-138)."""
+    model, ps, om = synthetic
+    cfgs, _ = pfw.workloads.fitting_set()
+    assert len(cfgs) == 1024
+    systems = systems_of(pfw, cfgs, ace_labels(model))
+    dE, dF, dW = cotangents(cfgs, 41)
+    g64 = pfw.ace_parameter_vjp(pfw.ACECalculator(model, ps, precision="float64"), systems, dE, dF, dW)
+    g32 = pfw.ace_parameter_vjp(pfw.ACECalculator(model, ps, precision="mixed"), systems, dE, dF, dW)
+    for k in ("WB", "Wpair"):
+        err = np.linalg.norm(g32[k] - g64[k]) / np.linalg.norm(g64[k])
+        print(f"C5 1024 configurations, mixed vs FP64 adjoint {k}: relative error {err:.2e}")
+        assert err <= 1e-4
+
+    def L_at(WB, Wp):
+        m = dict(om, WB=WB, Wpair=Wp)
+        res = []
+        for X, H, pbc, sp in cfgs:
+            E, F, W, _ = oracle_ace(X, sp, H, pbc, m, threads=16)
+            res.append((E, F.T, W))
+        return linear_functional(res, dE, dF, dW)
+
+    # E, F and W are linear in (WB, Wpair) with the reference energies as the
+    # only offset: at zero parameters E_c = sum_i ref[z_i] and F = W = 0, so
+    # L(0) needs no evaluation and each check below takes one oracle pass
+    WB, Wp = om["WB"], om["Wpair"]
+    L0 = float(sum(dE[c] * om["reference"][sp].sum() for c, (_, _, _, sp) in enumerate(cfgs)))
+    rng = np.random.default_rng(43)
+    VB = rng.normal(size=WB.shape) * np.abs(WB).mean()
+    Vp = rng.normal(size=Wp.shape) * np.abs(Wp).mean()
+    dL = L_at(VB, Vp) - L0
+    for g, rtol in ((g64, 1e-8), (g32, 1e-4)):
+        pred = float(np.sum(g["WB"] * VB) + np.sum(g["Wpair"] * Vp))
+        scale = float(np.sum(np.abs(g64["WB"] * VB)) + np.sum(np.abs(g64["Wpair"] * Vp)))
+        assert abs(pred - dL) <= rtol * scale, (pred, dL, scale)
+    b = int(rng.integers(WB.shape[0]))
+    E1 = np.zeros_like(WB)
+    E1[b, 1] = 1.0
+    d1 = L_at(E1, np.zeros_like(Wp)) - L0
+    assert g64["WB"][b, 1] == pytest.approx(d1, rel=1e-7, abs=1e-9 * max(1.0, abs(L0)))
+
+    p = pfw.SmoothPair(["A", "B"], A, RC)
+    gA = pfw.pair_parameter_vjp(pfw.Workspace(), systems_of(pfw, cfgs), p, dE, dF, dW)["amplitudes"]
+    units = [np.array([[1.0, 0], [0, 0]]), np.array([[0, 1.0], [1.0, 0]]), np.array([[0, 0], [0, 1.0]])]
+    ref = []
+    for U in units:
+        res = []
+        for X, H, pbc, sp in cfgs:
+            E, F, W = pair_evaluate(X, sp, H, pbc, KIND_SMOOTH, U, RC, threads=16)
+            res.append((E, F.T, W))
+        ref.append(linear_functional(res, dE, dF, dW))
+    np.testing.assert_allclose(gA, ref, rtol=1e-10, atol=1e-10 * np.abs(ref).max())
