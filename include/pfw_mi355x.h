@@ -149,6 +149,27 @@ int pfw_grid_info(pfw_ctx* ctx, int64_t* ncells, int32_t* dims, int32_t* minimum
 int pfw_set_timing(pfw_ctx* ctx, int enabled);
 int pfw_get_timings(pfw_ctx* ctx, double* ms, int n);
 
+/* Allocation-free pair step, for callers that cannot allocate device memory
+ * during the call (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid allocation). No counterpart in the reference: it is
+ * pfw_set_system + pfw_set_pair_model + pfw_pair_evaluate in one call whose
+ * intermediate buffers (cell grid, sorted positions, partial sums) live in a
+ * static device area of about 42 MB that each device holds from the first
+ * context created on it until the process ends; nothing is allocated or
+ * freed. Inputs X_device (3 x n doubles, column-major, original order) and
+ * species_device (n channels) and outputs E_device (1), F_device (3 x n,
+ * original order) and W_device (3 x 3, column-major) are device pointers on
+ * the context's device (an output may be NULL when its quantity is not
+ * requested); H, pbc and params are host values as for pfw_set_system and
+ * pfw_set_pair_model. Limits: n <= 262,144 atoms and a cell grid of at most
+ * 262,144 cells (else PFW_CAPACITY). Calls that overlap on one device share
+ * the static area and race; the call synchronises the context's stream
+ * before it returns. */
+int pfw_pair_evaluate_noalloc(pfw_ctx* ctx, int kind, int nz, const double* params, double cutoff, int64_t n,
+                              const double* X_device, const int32_t* species_device, const double* H,
+                              const uint8_t* pbc, uint32_t request_mask, int precision, double* E_device,
+                              double* F_device, double* W_device);
+
 /* Pair-set audit, a diagnostic for parity tests (no counterpart in the
  * reference, whose CPU executor and GPU kernels read one neighbour set). While
  * enabled, every pair step (fresh cells or Verlet list) and ACE step records,

@@ -72,6 +72,9 @@ SIGNATURES = [
     ("pfw_verlet_info", ctypes.c_int, [_P, _I64, _D]),
     ("pfw_set_timing", ctypes.c_int, [_P, ctypes.c_int]),
     ("pfw_get_timings", ctypes.c_int, [_P, _D, ctypes.c_int]),
+    ("pfw_pair_evaluate_noalloc", ctypes.c_int,
+     [_P, ctypes.c_int, ctypes.c_int, _D, ctypes.c_double, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, _D, _U8,
+      ctypes.c_uint32, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     ("pfw_set_audit", ctypes.c_int, [_P, ctypes.c_int]),
     ("pfw_read_audit", ctypes.c_int, [_P, _I32, ctypes.POINTER(ctypes.c_uint64)]),
     ("pfw_synchronize", ctypes.c_int, [_P]),

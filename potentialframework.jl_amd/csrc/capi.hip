@@ -82,6 +82,7 @@ int pfw_ctx_create(int device, pfw_ctx** out) {
     PFW_HIP(hipMalloc(&ctx->d_flag, sizeof(int32_t)));
     PFW_HIP(hipHostMalloc(&ctx->h_flag, sizeof(int32_t), hipHostMallocDefault));
     for (auto& e : ctx->ev) PFW_HIP(hipEventCreate(&e));
+    pfw::static_scratch_init(device);
   });
   if (rc != PFW_OK) {
     pfw_ctx_destroy(ctx);
@@ -352,6 +353,16 @@ int pfw_get_timings(pfw_ctx* ctx, double* ms, int n) {
   int m = std::min(n, pfw::kStages);
   for (int k = 0; k < m; ++k) ms[k] = ctx->stage_ms[k];
   return m;
+}
+
+int pfw_pair_evaluate_noalloc(pfw_ctx* ctx, int kind, int nz, const double* params, double cutoff, int64_t n,
+                              const double* X_device, const int32_t* species_device, const double* H,
+                              const uint8_t* pbc, uint32_t request_mask, int precision, double* E_device,
+                              double* F_device, double* W_device) {
+  return run(ctx, [&] {
+    pfw::pair_run_static(ctx, kind, nz, params, cutoff, n, X_device, species_device, H, pbc, request_mask, precision,
+                         E_device, F_device, W_device);
+  });
 }
 
 int pfw_set_audit(pfw_ctx* ctx, int enabled) {

@@ -111,7 +111,7 @@ void nl_free(pfw_ctx* ctx) {
 // reach and dims (storage.jl:130-193), with the cell count capped by `limit`.
 // The wrap path follows the cutoff c; cells are sized for the search radius
 // (c plus the Verlet skin), so a stencil reaches every pair of the list.
-static Grid configure_one(const double* H, const int* pb, int64_t n, double c, double search, double limit) {
+Grid nl_configure_one(const double* H, const int* pb, int64_t n, double c, double search, double limit) {
   Grid g{};
   bool axisaligned = true;
   for (int i = 0; i < 3; ++i)
@@ -212,7 +212,7 @@ void nl_configure(pfw_ctx* ctx, double c, double skin) {
   for (int k = 0; k < ncfg; ++k) {
     const int64_t nk = ctx->atom0[k + 1] - ctx->atom0[k];
     const double limit = ncfg == 1 ? std::max<double>(2.0 * (double)nk, 4096.0) : std::max<double>(2.0 * (double)nk, 64.0);
-    Grid g = configure_one(&ctx->host_H[9 * k], &ctx->host_pb[3 * k], nk, c, c + skin, limit);
+    Grid g = nl_configure_one(&ctx->host_H[9 * k], &ctx->host_pb[3 * k], nk, c, c + skin, limit);
     g.cell0 = (int)total;
     g.ecell0 = (int)ext_total;
     total += g.ncells;
@@ -652,11 +652,15 @@ __global__ void k_finish(const double* __restrict__ part, int nb, int width, int
 // E, W and the status word of one configuration: k_colsum, then k_finish.
 // (A single launch whose last block finishes the sum needs a device-scope
 // fence in every block, which writes back the L2 and was measured slower.)
-void reduce_outputs(pfw_ctx* ctx, const double* rec, int64_t m, int width, int kind, const int32_t* status) {
-  hipLaunchKernelGGL(k_colsum, dim3(kRedBlocks), dim3(256), 0, ctx->stream, rec, m, width, ctx->d_red);
-  hipLaunchKernelGGL(k_finish, dim3(1), dim3(1024), 0, ctx->stream, ctx->d_red, kRedBlocks, width, kind, status,
-                     ctx->d_out, ctx->d_cfg_out, ctx->h_out_dev);
+void reduce_core(const double* rec, int64_t m, int width, int kind, const int32_t* status, double* red, double* out,
+                 double* out2, double* out3, hipStream_t st) {
+  hipLaunchKernelGGL(k_colsum, dim3(kRedBlocks), dim3(256), 0, st, rec, m, width, red);
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(1024), 0, st, red, kRedBlocks, width, kind, status, out, out2, out3);
   PFW_HIP(hipGetLastError());
+}
+
+void reduce_outputs(pfw_ctx* ctx, const double* rec, int64_t m, int width, int kind, const int32_t* status) {
+  reduce_core(rec, m, width, kind, status, ctx->d_red, ctx->d_out, ctx->d_cfg_out, ctx->h_out_dev, ctx->stream);
   ctx->host_written = true;
 }
 
@@ -713,50 +717,75 @@ void scan_counts(pfw_ctx* ctx, const int32_t* count, int m, int32_t* start, int3
   PFW_HIP(hipGetLastError());
 }
 
+// Open-axis extent words: minimum (all ones) and maximum (zero) per axis.
+__global__ void k_bounds_init(unsigned long long* __restrict__ bounds, int words) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < words) bounds[k] = (k % 2 == 0) ? ~0ull : 0ull;
+}
+
+__global__ void k_fill_i32(int32_t* __restrict__ p, int64_t m, int32_t v) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < m) p[k] = v;
+}
+
+void fill_i32(int32_t* p, int64_t m, int32_t v, hipStream_t st) {
+  if (m > 0) hipLaunchKernelGGL(k_fill_i32, dim3(nblocks(m, 256)), dim3(256), 0, st, p, m, v);
+}
+
+// The cell build over explicit buffers: wrap and count, (open axes: extent,
+// grid, binning), scan, scatter and the per-cell gather. `counts_zero`: the
+// count array is already zero (the previous build's one-workgroup scan leaves
+// it so). Stage events are recorded on `timing` when it is given.
+void nl_build_core(const BuildView& v, hipStream_t st, bool counts_zero, pfw_ctx* timing) {
+  const int64_t n = v.n;
+  const int ncells = v.ncells;
+  auto stage = [&](int k) {
+    if (timing) record_stage(timing, k);
+  };
+  stage(0);
+  const bool one = ncells <= kScanOneMax;  // the one-workgroup scan leaves the counts zero
+  if (!counts_zero) fill_i32(v.count, std::max(ncells, 1), 0, st);
+  if (v.open_any)
+    hipLaunchKernelGGL(k_bounds_init, dim3(nblocks(6 * (int64_t)v.ncfg, 64)), dim3(64), 0, st,
+                       (unsigned long long*)v.bounds, 6 * v.ncfg);
+  if (n > 0)
+    hipLaunchKernelGGL(k_wrap, dim3(nblocks(n, 256)), dim3(256), 0, st, v.grids, v.atom_cfg, n, v.X, v.xw, v.u,
+                       v.cell_of, v.count, v.rank, (unsigned long long*)v.bounds, (int)v.open_any,
+                       (int)(v.ncfg == 1));
+  if (v.open_any) {
+    hipLaunchKernelGGL(k_open_grid, dim3(nblocks(v.ncfg, 64)), dim3(64), 0, st, v.grids,
+                       (const unsigned long long*)v.bounds, v.ncfg);
+    if (n > 0)
+      hipLaunchKernelGGL(k_bin_open, dim3(nblocks(n, 256)), dim3(256), 0, st, v.grids, v.atom_cfg, n, v.u,
+                         v.cell_of, v.count, v.rank);
+  }
+  PFW_HIP(hipGetLastError());
+  stage(1);
+  if (one) {
+    hipLaunchKernelGGL(k_scan_one, dim3(1), dim3(kScanOneThreads), 0, st, v.count, ncells, v.start, n, 1);
+  } else {
+    int nb = (int)nblocks(ncells, kScanBlock);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(kScanThreads), 0, st, v.count, ncells, v.blocksum);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, st, v.blocksum, nb);
+    hipLaunchKernelGGL(k_scan_final, dim3(nb), dim3(kScanThreads), 0, st, v.count, ncells, v.blocksum, v.start, n);
+  }
+  PFW_HIP(hipGetLastError());
+  stage(2);
+  if (n > 0) {
+    hipLaunchKernelGGL(k_scatter, dim3(nblocks(n, 256)), dim3(256), 0, st, n, v.cell_of, v.start, v.rank, v.perm);
+    hipLaunchKernelGGL(k_cell_sort_gather, dim3(ncells), dim3(kWave), 0, st, ncells, n, v.start, v.perm, v.xw,
+                       v.species, v.xs, v.sps, v.rank);
+  }
+  PFW_HIP(hipGetLastError());
+}
+
 void nl_build(pfw_ctx* ctx) {
   ++ctx->build_gen;  // sorted indices change: a Verlet list made before is stale
   SystemBuffers& s = ctx->sys;
-  hipStream_t st = ctx->stream;
-  const int64_t n = s.n;
-  const int ncells = ctx->ncells;
-  record_stage(ctx, 0);
-  const bool one = ncells <= kScanOneMax;  // counts are then left zero by the previous build's scan
-  if (!one) PFW_HIP(hipMemsetAsync(s.count, 0, sizeof(int32_t) * std::max(ncells, 1), st));
-  if (ctx->open_any) {
-    std::vector<unsigned long long> init(6 * (size_t)ctx->ncfg);
-    for (size_t k = 0; k < init.size(); ++k) init[k] = (k % 2 == 0) ? ~0ull : 0ull;
-    PFW_HIP(hipMemcpyAsync(s.bounds, init.data(), init.size() * sizeof(unsigned long long), hipMemcpyHostToDevice, st));
-  }
-  if (n > 0)
-    hipLaunchKernelGGL(k_wrap, dim3(nblocks(n, 256)), dim3(256), 0, st, s.grids, s.atom_cfg, n, s.X, s.xw, s.u,
-                       s.cell_of, s.count, s.rank, (unsigned long long*)s.bounds, (int)ctx->open_any,
-                       (int)(ctx->ncfg == 1));
-  if (ctx->open_any) {
-    hipLaunchKernelGGL(k_open_grid, dim3(nblocks(ctx->ncfg, 64)), dim3(64), 0, st, s.grids,
-                       (const unsigned long long*)s.bounds, ctx->ncfg);
-    if (n > 0)
-      hipLaunchKernelGGL(k_bin_open, dim3(nblocks(n, 256)), dim3(256), 0, st, s.grids, s.atom_cfg, n, s.u,
-                         s.cell_of, s.count, s.rank);
-    PFW_HIP(hipStreamSynchronize(st));  // the bounds staging vector goes out of scope
-  }
-  PFW_HIP(hipGetLastError());
-  record_stage(ctx, 1);
-  if (one) {
-    hipLaunchKernelGGL(k_scan_one, dim3(1), dim3(kScanOneThreads), 0, st, s.count, ncells, s.start, n, 1);
-  } else {
-    int nb = (int)nblocks(ncells, kScanBlock);
-    hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(kScanThreads), 0, st, s.count, ncells, s.blocksum);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, st, s.blocksum, nb);
-    hipLaunchKernelGGL(k_scan_final, dim3(nb), dim3(kScanThreads), 0, st, s.count, ncells, s.blocksum, s.start, n);
-  }
-  PFW_HIP(hipGetLastError());
-  record_stage(ctx, 2);
-  if (n > 0) {
-    hipLaunchKernelGGL(k_scatter, dim3(nblocks(n, 256)), dim3(256), 0, st, n, s.cell_of, s.start, s.rank, s.perm);
-    hipLaunchKernelGGL(k_cell_sort_gather, dim3(ncells), dim3(kWave), 0, st, ncells, n, s.start, s.perm, s.xw,
-                       s.species, s.xs, s.sps, s.rank);
-  }
-  PFW_HIP(hipGetLastError());
+  BuildView v{s.n,    ctx->ncells, ctx->ncfg, ctx->open_any, s.X,     s.xw,      s.u,      s.cell_of, s.count,
+              s.rank, s.bounds,    s.grids,   s.atom_cfg,    s.start, s.blocksum, s.perm,  s.xs,      s.species,
+              s.sps};
+  nl_build_core(v, ctx->stream, ctx->ncells <= kScanOneMax, ctx);
 }
 
 // ---------------------------------------------------------------- per-configuration sums

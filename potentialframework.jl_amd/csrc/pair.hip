@@ -17,6 +17,7 @@
 // formula of the oracle, so the pair set is bit-exact in both precisions.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 
 #include "pfw_device.h"
 
@@ -342,17 +343,19 @@ void record_stage(pfw_ctx* ctx, int k) {
   if (ctx->timing) PFW_HIP(hipEventRecord(ctx->ev[k], ctx->stream));
 }
 
-LJConsts lj_consts(const PairModelDev& m) {
-  LJConsts lj{0, 0, 0, 0};
-  if (m.kind == PFW_PAIR_LJ) {
-    double eps = m.params[0], sig = m.params[1], c = m.cutoff;
-    double rc = sig / c, qc2 = rc * rc, qc = qc2 * qc2 * qc2;
-    lj.c4e = 4.0 * eps;
-    lj.c24e = 24.0 * eps;
-    lj.sig2 = sig * sig;
-    lj.eshift = 4.0 * eps * (qc * qc - qc);
-  }
+static LJConsts lj_consts_of(double eps, double sig, double c) {
+  LJConsts lj;
+  const double rc = sig / c, qc2 = rc * rc, qc = qc2 * qc2 * qc2;
+  lj.c4e = 4.0 * eps;
+  lj.c24e = 24.0 * eps;
+  lj.sig2 = sig * sig;
+  lj.eshift = 4.0 * eps * (qc * qc - qc);
   return lj;
+}
+
+LJConsts lj_consts(const PairModelDev& m) {
+  if (m.kind == PFW_PAIR_LJ) return lj_consts_of(m.params[0], m.params[1], m.cutoff);
+  return LJConsts{0, 0, 0, 0};
 }
 
 template <typename T, int KIND>
@@ -443,6 +446,219 @@ void pair_finish(pfw_ctx* ctx, uint32_t mask) {
   ctx->have_results = true;
   if (ctx->h_out[10] != 0.0)
     throw Error(PFW_DOMAIN, "singular origin or nonfinite pair callback output");
+}
+
+// ---------------------------------------------------------------- allocation-free pair step
+//
+// pfw_pair_evaluate_noalloc: one complete pair step (cell build, k_pair, FP64
+// reductions) whose every intermediate buffer lies in `scratch`, a static
+// device array of fixed size, so the call allocates nothing. Each device
+// holds its own copy of the array, at an address taken once per device with
+// hipGetSymbolAddress when a context is created there (static_scratch_init);
+// host code uses that address, never the array's name. All calls on one
+// device share the array: calls that overlap on a device race.
+constexpr int64_t kStaticAtoms = 262144;  // atoms of one system
+constexpr int64_t kStaticCells = 262144;  // cells of its grid
+constexpr size_t kStaticAlign = 256;
+
+constexpr size_t static_round(size_t b) { return (b + kStaticAlign - 1) / kStaticAlign * kStaticAlign; }
+
+// Byte offsets of the sub-buffers, in order.
+struct StaticLayout {
+  size_t grid, params, flag, out, out2, red, cell_cfg, atom_cfg, count, start, blocksum, bounds, cell_of, rank, perm,
+      sps, xw, u, xs, partials, total;
+};
+constexpr StaticLayout static_layout() {
+  StaticLayout L{};
+  size_t o = 0;
+  auto take = [&o](size_t bytes) {
+    const size_t at = o;
+    o += static_round(bytes);
+    return at;
+  };
+  L.grid = take(sizeof(Grid));
+  L.params = take(sizeof(double) * kMaxZ * kMaxZ);
+  L.flag = take(sizeof(int32_t) * 4);
+  L.out = take(sizeof(double) * 16);
+  L.out2 = take(sizeof(double) * 16);
+  L.red = take(sizeof(double) * 64 * 16);
+  L.cell_cfg = take(sizeof(int32_t) * kStaticCells);
+  L.atom_cfg = take(sizeof(int32_t) * kStaticAtoms);
+  L.count = take(sizeof(int32_t) * kStaticCells);
+  L.start = take(sizeof(int32_t) * (kStaticCells + 1));
+  L.blocksum = take(sizeof(int32_t) * ((kStaticCells + 1023) / 1024 + 1));
+  L.bounds = take(sizeof(uint64_t) * 6);
+  L.cell_of = take(sizeof(int32_t) * kStaticAtoms);
+  L.rank = take(sizeof(int32_t) * kStaticAtoms);
+  L.perm = take(sizeof(int32_t) * kStaticAtoms);
+  L.sps = take(sizeof(int32_t) * kStaticAtoms);
+  L.xw = take(sizeof(double) * 3 * kStaticAtoms);
+  L.u = take(sizeof(double) * 3 * kStaticAtoms);
+  L.xs = take(sizeof(double) * 3 * kStaticAtoms);
+  L.partials = take(sizeof(double) * 7 * kStaticCells);
+  L.total = o;
+  return L;
+}
+constexpr StaticLayout kStatic = static_layout();  // about 42 MB
+
+static __device__ __align__(256) unsigned char scratch[kStatic.total];
+
+constexpr int kMaxDevices = 64;
+static unsigned char* g_scratch[kMaxDevices] = {};
+static std::mutex g_scratch_mutex;
+
+// Called by pfw_ctx_create with `device` current: the device address of
+// that device's copy of `scratch`, once per device.
+void static_scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) throw Error(PFW_INVALID_ARGUMENT, "device ordinal out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (g_scratch[device]) return;
+  void* p = nullptr;
+  PFW_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  if (!p) throw Error(PFW_DEVICE, "no device address for the static scratch area");
+  g_scratch[device] = (unsigned char*)p;
+}
+
+// Nonfinite positions or species outside 0..nz-1 set flag[0].
+__global__ void k_static_check(int64_t n, const double* __restrict__ X, const int32_t* __restrict__ sp, int nz,
+                               int32_t* __restrict__ flag) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  bool bad = !(isfinite(X[3 * k]) && isfinite(X[3 * k + 1]) && isfinite(X[3 * k + 2]));
+  bad |= sp[k] < 0 || sp[k] >= nz;
+  if (bad) atomicOr(flag, 1);
+}
+
+__global__ void k_static_outputs(const double* __restrict__ out, double* __restrict__ E, double* __restrict__ W,
+                                 int32_t* __restrict__ flag) {
+  const int k = threadIdx.x;
+  if (k == 0 && E) E[0] = out[0];
+  if (k < 9 && W) W[k] = out[1 + k];
+  if (k == 0) flag[1] = out[10] != 0.0 ? 1 : 0;
+}
+
+__global__ void k_static_zero(double* __restrict__ out) {
+  if (threadIdx.x < 11) out[threadIdx.x] = 0.0;
+}
+
+template <typename T, int KIND>
+static void launch_pair_static(const Grid* grid, const int32_t* cell_cfg, int64_t n, const int32_t* start,
+                               const double* xs, const int32_t* sps, const int32_t* perm, const double* params,
+                               int nz, LJConsts lj, uint32_t mask, double* F, double* partials, int ncells,
+                               hipStream_t st) {
+  const size_t dyn = KIND == PFW_PAIR_SMOOTH ? 2 * sizeof(T) * nz * nz : 0;
+  hipLaunchKernelGGL((k_pair<T, KIND, false>), dim3(ncells), dim3(kWave), dyn, st, grid, cell_cfg, n, start, xs, sps,
+                     perm, params, nz, lj, mask, F, partials, ncells, AuditBufs{});
+}
+
+void pair_run_static(pfw_ctx* ctx, int kind, int nz, const double* params, double cutoff, int64_t n,
+                     const double* X, const int32_t* species, const double* H, const uint8_t* pbc, uint32_t mask,
+                     int precision, double* E, double* F, double* W) {
+  unsigned char* base = ctx->device < kMaxDevices ? g_scratch[ctx->device] : nullptr;
+  if (!base) throw Error(PFW_DEVICE, "static scratch area not initialised on this device");
+  if (kind != PFW_PAIR_SMOOTH && kind != PFW_PAIR_LJ) throw Error(PFW_INVALID_ARGUMENT, "unknown pair model kind");
+  if (precision != PFW_FP64 && precision != PFW_MIXED) throw Error(PFW_INVALID_ARGUMENT, "unknown precision mode");
+  if (mask > 7) throw Error(PFW_INVALID_ARGUMENT, "unknown request bits");
+  if (n < 0) throw Error(PFW_DIMENSION, "negative atom count");
+  if (n > kStaticAtoms)
+    throw Error(PFW_CAPACITY, "the allocation-free pair step holds at most 262,144 atoms");
+  if (!H || !pbc || !params || (n > 0 && (!X || !species)))
+    throw Error(PFW_INVALID_ARGUMENT, "null pointer argument");
+  if ((mask & PFW_FORCES) && n > 0 && !F) throw Error(PFW_INVALID_ARGUMENT, "forces requested without an output");
+  if (!(std::isfinite(cutoff) && cutoff > 0)) throw Error(PFW_INVALID_ARGUMENT, "cutoff must be finite and positive (Å)");
+  for (int k = 0; k < 9; ++k)
+    if (!std::isfinite(H[k])) throw Error(PFW_INVALID_ARGUMENT, "cell must be finite");
+  LJConsts lj{0, 0, 0, 0};
+  if (kind == PFW_PAIR_SMOOTH) {
+    if (nz < 1 || nz > kMaxZ) throw Error(PFW_INVALID_ARGUMENT, "species count outside 1..16");
+    for (int k = 0; k < nz * nz; ++k)
+      if (!std::isfinite(params[k])) throw Error(PFW_INVALID_ARGUMENT, "amplitudes must be finite");
+  } else {
+    if (!(std::isfinite(params[0]) && std::isfinite(params[1]) && params[1] > 0))
+      throw Error(PFW_INVALID_ARGUMENT, "Lennard-Jones epsilon and sigma must be finite, sigma positive");
+    lj = lj_consts_of(params[0], params[1], cutoff);
+    nz = kMaxZ;  // species are not read by the model; any channel below 16 is accepted
+  }
+  int pb[3];
+  for (int a = 0; a < 3; ++a) {
+    if (pbc[a] > 1) throw Error(PFW_INVALID_ARGUMENT, "pbc flags must be 0 or 1");
+    pb[a] = pbc[a];
+  }
+  Grid g = nl_configure_one(H, pb, n, cutoff, cutoff, std::max<double>(2.0 * (double)n, 4096.0));
+  g.cell0 = 0;
+  g.ecell0 = 0;
+  if (g.ncells > kStaticCells)
+    throw Error(PFW_CAPACITY, "the cell grid exceeds the 262,144 cells of the allocation-free pair step");
+  const int ncells = g.ncells;
+  hipStream_t st = ctx->stream;
+  const StaticLayout& L = kStatic;
+  Grid* d_grid = (Grid*)(base + L.grid);
+  double* d_params = (double*)(base + L.params);
+  int32_t* d_flag = (int32_t*)(base + L.flag);
+  double* d_out = (double*)(base + L.out);
+  // host copies of the small inputs; the stream is synchronised before return
+  PFW_HIP(hipMemcpyAsync(d_grid, &g, sizeof(Grid), hipMemcpyHostToDevice, st));
+  if (kind == PFW_PAIR_SMOOTH)
+    PFW_HIP(hipMemcpyAsync(d_params, params, sizeof(double) * nz * nz, hipMemcpyHostToDevice, st));
+  fill_i32(d_flag, 4, 0, st);
+  int32_t* h_flag = ctx->h_flag;  // page-locked, from the context's creation
+  if (n > 0) {
+    hipLaunchKernelGGL(k_static_check, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, X, species, nz, d_flag);
+    PFW_HIP(hipGetLastError());
+    PFW_HIP(hipMemcpyAsync(h_flag, d_flag, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    PFW_HIP(hipStreamSynchronize(st));
+    if (h_flag[0]) throw Error(PFW_INVALID_ARGUMENT, "positions must be finite and species channels in range");
+  }
+  fill_i32((int32_t*)(base + L.cell_cfg), ncells, 0, st);
+  fill_i32((int32_t*)(base + L.atom_cfg), n, 0, st);
+  BuildView v{n,
+              ncells,
+              1,
+              g.open_any != 0,
+              X,
+              (double*)(base + L.xw),
+              (double*)(base + L.u),
+              (int32_t*)(base + L.cell_of),
+              (int32_t*)(base + L.count),
+              (int32_t*)(base + L.rank),
+              (uint64_t*)(base + L.bounds),
+              d_grid,
+              (const int32_t*)(base + L.atom_cfg),
+              (int32_t*)(base + L.start),
+              (int32_t*)(base + L.blocksum),
+              (int32_t*)(base + L.perm),
+              (double*)(base + L.xs),
+              species,
+              (int32_t*)(base + L.sps)};
+  nl_build_core(v, st, false, nullptr);
+  if (mask != 0 && n > 0) {
+    const bool fp32 = precision == PFW_MIXED;
+    double* part = (double*)(base + L.partials);
+#define PFW_STATIC_LAUNCH(T, K)                                                                                  \
+  launch_pair_static<T, K>(d_grid, (const int32_t*)(base + L.cell_cfg), n, v.start, v.xs, v.sps, v.perm, d_params, \
+                           nz, lj, mask, F, part, ncells, st)
+    if (kind == PFW_PAIR_SMOOTH) {
+      if (fp32)
+        PFW_STATIC_LAUNCH(float, PFW_PAIR_SMOOTH);
+      else
+        PFW_STATIC_LAUNCH(double, PFW_PAIR_SMOOTH);
+    } else {
+      if (fp32)
+        PFW_STATIC_LAUNCH(float, PFW_PAIR_LJ);
+      else
+        PFW_STATIC_LAUNCH(double, PFW_PAIR_LJ);
+    }
+#undef PFW_STATIC_LAUNCH
+    PFW_HIP(hipGetLastError());
+    reduce_core(part, ncells, 7, 0, nullptr, (double*)(base + L.red), d_out, (double*)(base + L.out2), nullptr, st);
+  } else {
+    hipLaunchKernelGGL(k_static_zero, dim3(1), dim3(64), 0, st, d_out);
+  }
+  hipLaunchKernelGGL(k_static_outputs, dim3(1), dim3(64), 0, st, d_out, E, W, d_flag);
+  PFW_HIP(hipGetLastError());
+  PFW_HIP(hipMemcpyAsync(h_flag, d_flag + 1, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  PFW_HIP(hipStreamSynchronize(st));
+  if (h_flag[0]) throw Error(PFW_DOMAIN, "singular origin or nonfinite pair callback output");
 }
 
 // Parameter adjoint of SmoothPair: dL/dA (nz x nz, the entry each directed

@@ -305,6 +305,27 @@ namespace pfw {
 void nl_configure(pfw_ctx* ctx, double cutoff, double skin);
 void batch_reduce(pfw_ctx* ctx, const double* records, int width, bool cells, int kind);
 void nl_build(pfw_ctx* ctx);
+// The buffers of one cell build (the context's, or views into a caller's
+// scratch area).
+struct BuildView {
+  int64_t n;
+  int ncells, ncfg;
+  bool open_any;
+  const double* X;
+  double *xw, *u;
+  int32_t *cell_of, *count, *rank;
+  uint64_t* bounds;
+  Grid* grids;
+  const int32_t* atom_cfg;
+  int32_t *start, *blocksum, *perm;
+  double* xs;
+  const int32_t* species;
+  int32_t* sps;
+};
+void nl_build_core(const BuildView& v, hipStream_t st, bool counts_zero, pfw_ctx* timing);
+void fill_i32(int32_t* p, int64_t m, int32_t v, hipStream_t st);
+// Host-side grid of one configuration (prepare!'s choices, storage.jl:130-193).
+Grid nl_configure_one(const double* H, const int* pb, int64_t n, double c, double search, double limit);
 // Directed pairs on the device, grouped by sorted atom (offsets off[0..n]);
 // the destructor frees them.
 struct DeviceExport {
@@ -324,6 +345,10 @@ void scan_counts(pfw_ctx* ctx, const int32_t* count, int m, int32_t* start, int3
 // E, W and the status word of one configuration from m records of `width`
 // columns (kind 0: pair records, kind 1: ACE records) into d_out / d_cfg_out.
 void reduce_outputs(pfw_ctx* ctx, const double* rec, int64_t m, int width, int kind, const int32_t* status);
+// The same over explicit buffers: red holds 64 x 16 doubles, out and out2 11
+// each (out3 may be NULL).
+void reduce_core(const double* rec, int64_t m, int width, int kind, const int32_t* status, double* red, double* out,
+                 double* out2, double* out3, hipStream_t st);
 // d_out's 11 words into h_out on the stream, unless the finish kernel wrote them there.
 void copy_outputs(pfw_ctx* ctx);
 // pair.hip
@@ -331,6 +356,10 @@ void pair_run(pfw_ctx* ctx, uint32_t mask, int precision);
 void pair_finish(pfw_ctx* ctx, uint32_t mask);
 void record_stage(pfw_ctx* ctx, int k);
 void pair_adjoint(pfw_ctx* ctx, const double* Ebar, const double* Fbar, const double* Wbar, double* gA);
+void static_scratch_init(int device);
+void pair_run_static(pfw_ctx* ctx, int kind, int nz, const double* params, double cutoff, int64_t n,
+                     const double* X, const int32_t* species, const double* H, const uint8_t* pbc, uint32_t mask,
+                     int precision, double* E, double* F, double* W);
 // verlet.hip
 void verlet_free(pfw_ctx* ctx);
 void verlet_pair_run(pfw_ctx* ctx, uint32_t mask, int precision);
