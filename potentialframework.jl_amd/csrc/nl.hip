@@ -101,7 +101,7 @@ static void dmalloc(T** p, size_t count) {
 void nl_free(pfw_ctx* ctx) {
   SystemBuffers& s = ctx->sys;
   void* ptrs[] = {s.X,     s.species, s.xw,     s.u,      s.cell_of,  s.count,    s.start,    s.rank,
-                  s.blocksum, s.perm, s.xs,    s.sps,    s.bounds, s.grids, s.atom_cfg, s.cell_cfg, s.cfg_cell0};
+                  s.blocksum, s.perm, s.xs,    s.sps,    s.bounds, s.grids, s.atom_cfg, s.cell_cfg, s.cfg_cell0, s.xr};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   s = SystemBuffers();
@@ -240,6 +240,7 @@ void nl_configure(pfw_ctx* ctx, double c, double skin) {
   dmalloc(&s.perm, n);
   dmalloc(&s.xs, 3 * n);
   dmalloc(&s.sps, n);
+  dmalloc(&s.xr, n);
   dmalloc(&s.count, total);
   dmalloc(&s.start, total + 1);
   dmalloc(&s.rank, n);
@@ -537,7 +538,8 @@ constexpr int kRankSort = 256;
 __global__ __launch_bounds__(64) void k_cell_sort_gather(int ncells, int64_t n, const int32_t* __restrict__ start,
                                                          int32_t* __restrict__ perm, const double* __restrict__ xw,
                                                          const int32_t* __restrict__ species, double* __restrict__ xs,
-                                                         int32_t* __restrict__ sps, int32_t* __restrict__ scratch) {
+                                                         int32_t* __restrict__ sps, float4* __restrict__ xr,
+                                                         int32_t* __restrict__ scratch) {
   const int c = blockIdx.x;
   const int lane = threadIdx.x;
   const int b = start[c], e = start[c + 1], cnt = e - b;
@@ -575,12 +577,17 @@ __global__ __launch_bounds__(64) void k_cell_sort_gather(int ncells, int64_t n, 
     }
     __syncthreads();
   }
+  // xr: FP32 positions relative to the cell's first atom, species bits in w
+  const int i0 = perm[b];
+  const double ox = xw[i0], oy = xw[n + i0], oz = xw[2 * n + i0];
   for (int k = b + lane; k < e; k += kWave) {
     const int i = perm[k];
-    xs[k] = xw[i];
-    xs[n + k] = xw[n + i];
-    xs[2 * n + k] = xw[2 * n + i];
+    const double x = xw[i], y = xw[n + i], z = xw[2 * n + i];
+    xs[k] = x;
+    xs[n + k] = y;
+    xs[2 * n + k] = z;
     sps[k] = species[i];
+    xr[k] = make_float4((float)(x - ox), (float)(y - oy), (float)(z - oz), __int_as_float(species[i]));
   }
 }
 
@@ -774,7 +781,7 @@ void nl_build_core(const BuildView& v, hipStream_t st, bool counts_zero, pfw_ctx
   if (n > 0) {
     hipLaunchKernelGGL(k_scatter, dim3(nblocks(n, 256)), dim3(256), 0, st, n, v.cell_of, v.start, v.rank, v.perm);
     hipLaunchKernelGGL(k_cell_sort_gather, dim3(ncells), dim3(kWave), 0, st, ncells, n, v.start, v.perm, v.xw,
-                       v.species, v.xs, v.sps, v.rank);
+                       v.species, v.xs, v.sps, v.xr, v.rank);
   }
   PFW_HIP(hipGetLastError());
 }
@@ -782,9 +789,9 @@ void nl_build_core(const BuildView& v, hipStream_t st, bool counts_zero, pfw_ctx
 void nl_build(pfw_ctx* ctx) {
   ++ctx->build_gen;  // sorted indices change: a Verlet list made before is stale
   SystemBuffers& s = ctx->sys;
-  BuildView v{s.n,    ctx->ncells, ctx->ncfg, ctx->open_any, s.X,     s.xw,      s.u,      s.cell_of, s.count,
-              s.rank, s.bounds,    s.grids,   s.atom_cfg,    s.start, s.blocksum, s.perm,  s.xs,      s.species,
-              s.sps};
+  BuildView v{s.n,    ctx->ncells, ctx->ncfg, ctx->open_any, s.X,     s.xw,      s.u,    s.cell_of, s.count,
+              s.rank, s.bounds,    s.grids,   s.atom_cfg,    s.start, s.blocksum, s.perm, s.xs,      s.species,
+              s.sps,  s.xr};
   nl_build_core(v, ctx->stream, ctx->ncells <= kScanOneMax, ctx);
 }
 

@@ -75,18 +75,32 @@ struct DotSlack<double> {
   static constexpr double value = 16 * 2.220446049250313e-16;
 };
 
-// The test of a candidate against the lane's atom: FP32 by the dot form (one
-// FMA chain), FP64 by differences.
-__device__ inline float prefilter_r2(const Cand<float>& c, float, float, float, float m2x, float m2y, float m2z,
-                                     float xi2) {
-  return m2x * c.x + (m2y * c.y + (m2z * c.z + (xi2 + c.w)));
+// The test of a candidate against the lane's atom, as r^2 - c_hi^2 (negative
+// for a candidate of the band-widened sphere): FP32 by the dot form (one FMA
+// chain, with -c_hi^2 folded into the lane's constant), FP64 by differences.
+__device__ inline float prefilter_d(const Cand<float>& c, float, float, float, float m2x, float m2y, float m2z,
+                                    float base, float) {
+  return m2x * c.x + (m2y * c.y + (m2z * c.z + (base + c.w)));
 }
-__device__ inline double prefilter_r2(const Cand<double>& c, double xi, double yi, double zi, double, double, double,
-                                      double) {
+__device__ inline double prefilter_d(const Cand<double>& c, double xi, double yi, double zi, double, double, double,
+                                     double, double c2hi) {
   const double dx = c.x - xi, dy = c.y - yi, dz = c.z - zi;
-  return dx * dx + dy * dy + dz * dz;
+  return (dx * dx + dy * dy + dz * dz) - c2hi;
 }
+// The sign bit of a test value (its high word for FP64).
+__device__ inline uint32_t sign_word(float v) { return __float_as_uint(v); }
+__device__ inline uint32_t sign_word(double v) { return (uint32_t)__double2hiint(v); }
 
+template <typename T>
+struct Pair2;
+template <>
+struct Pair2<float> {
+  using type = float2;
+};
+template <>
+struct Pair2<double> {
+  using type = double2;
+};
 
 // Five wavefronts per SIMD for the FP32 kernel (at most 96 registers): the
 // candidate fills and LDS round trips of one wavefront hide behind the tests
@@ -95,7 +109,8 @@ __device__ inline double prefilter_r2(const Cand<double>& c, double xi, double y
 template <typename T, int KIND, bool AUDIT>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 5 : 4))) void k_pair(const Grid* __restrict__ grids, const int32_t* __restrict__ cell_cfg,
                                              int64_t n, const int32_t* __restrict__ start,
-                                             const double* __restrict__ xs, const int32_t* __restrict__ sps,
+                                             const double* __restrict__ xs, const float4* __restrict__ xr,
+                                             const int32_t* __restrict__ sps,
                                              const int32_t* __restrict__ perm, const double* __restrict__ params,
                                              int nz, LJConsts lj, uint32_t mask, double* __restrict__ F,
                                              double* __restrict__ partials, int ncells, AuditBufs au) {
@@ -111,9 +126,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
   __shared__ int e_pref[kEnt + 1], e_start[kEnt], e_lat[kEnt][3];
   __shared__ uint8_t e_of[kCandOf<T>];  // stencil entry (of the entry chunk) of each candidate of the LDS chunk
   __shared__ double e_sh[kEnt][3];
-  extern __shared__ __align__(16) unsigned char dyn[];  // SmoothPair: nz x nz amplitudes, then -6 A / c^2
-  T* sA = (T*)dyn;
-  T* sA6 = sA + nz * nz;
+  // FP32: each entry's cell origin (its first atom) plus image shift, relative
+  // to this cell's first atom; a candidate is its cell-relative position plus this
+  __shared__ __align__(16) float e_off[kEnt][4];
+  extern __shared__ __align__(16) unsigned char dyn[];  // SmoothPair: per (z_i, z_j) the pair (A, -6 A / c^2)
+  using T2 = typename Pair2<T>::type;
+  const T2* sAA = (const T2*)dyn;
   __shared__ int e_zero;
 
   const int lane = threadIdx.x;
@@ -124,9 +142,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
   const T invc2 = (T)(1.0 / g.c2);
   if (KIND == PFW_PAIR_SMOOTH) {
     for (int k = lane; k < nz * nz; k += kWave) {
-      double a = params[k];
-      sA[k] = (T)a;
-      sA6[k] = (T)(-6.0 * a / g.c2);
+      const double a = params[k];
+      ((T2*)dyn)[k] = T2{(T)a, (T)(-6.0 * a / g.c2)};
     }
   }
   Cand<T> far;
@@ -152,11 +169,22 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
       const int is = b + ib + il;
       // the FP64 coordinates are read again in the rare exact-membership path
       // (fewer live registers, more wavefronts per SIMD)
-      const T xi = (T)(xs[is] - rx), yi = (T)(xs[n + is] - ry), zi = (T)(xs[2 * n + is] - rz);
+      T xi, yi, zi;
+      if constexpr (sizeof(T) == 4) {  // the cell-relative copy: the same values
+        const float4 v = xr[is];
+        xi = v.x;
+        yi = v.y;
+        zi = v.z;
+      } else {
+        xi = (T)(xs[is] - rx);
+        yi = (T)(xs[n + is] - ry);
+        zi = (T)(xs[2 * n + is] - rz);
+      }
       // prefilter r^2 = |xi|^2 + |xj|^2 - 2 xi.xj, both squares shrunk by the
-      // slack that bounds its rounding error, so it never misses a pair
+      // slack that bounds its rounding error, so it never misses a pair; the
+      // test value is that minus c_hi^2
       const T m2x = (T)-2 * xi, m2y = (T)-2 * yi, m2z = (T)-2 * zi;
-      const T xi2 = (xi * xi + yi * yi + zi * zi) * ((T)1 - DotSlack<T>::value);
+      const T base = (xi * xi + yi * yi + zi * zi) * ((T)1 - DotSlack<T>::value) - c2hi;
       const int zsp = sps[is];
       double fx = 0, fy = 0, fz = 0;
       for (int e0 = 0; e0 < g.nstencil; e0 += kEnt) {
@@ -177,6 +205,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
             for (int a = 0; a < 3; ++a) {
               e_sh[lane][a] = sh[a];
               e_lat[lane][a] = lat[a];
+            }
+            if constexpr (sizeof(T) == 4) {
+              e_off[lane][0] = (float)((xs[s0] + sh[0]) - rx);
+              e_off[lane][1] = (float)((xs[n + s0] + sh[1]) - ry);
+              e_off[lane][2] = (float)((xs[2 * n + s0] + sh[2]) - rz);
             }
             if (zero) e_zero = lane;
           }
@@ -203,16 +236,21 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
             const int tt = t0 + t;
             const int lo = e_of[t];
             const int m = e_start[lo] + (tt - e_pref[lo]);
-            const T x = (T)((xs[m] + e_sh[lo][0]) - rx);
-            const T y = (T)((xs[n + m] + e_sh[lo][1]) - ry);
-            const T z = (T)((xs[2 * n + m] + e_sh[lo][2]) - rz);
             const int q = (int)(((uint32_t)t * invS) >> 16);
             const int pos = (t - q * S) * stride + q;
-            if constexpr (sizeof(T) == 4)
+            if constexpr (sizeof(T) == 4) {
+              const float4 v = xr[m];
+              const float4 o = *(const float4*)e_off[lo];
+              const float x = v.x + o.x, y = v.y + o.y, z = v.z + o.z;
               cand[pos] = Cand<T>{x, y, z, (x * x + y * y + z * z) * ((T)1 - DotSlack<T>::value)};
-            else
+              cm[pos] = (lo << 8) | __float_as_int(v.w);
+            } else {
+              const T x = (T)((xs[m] + e_sh[lo][0]) - rx);
+              const T y = (T)((xs[n + m] + e_sh[lo][1]) - ry);
+              const T z = (T)((xs[2 * n + m] + e_sh[lo][2]) - rz);
               cand[pos] = Cand<T>{x, y, z};
-            cm[pos] = (lo << 8) | sps[m];
+              cm[pos] = (lo << 8) | sps[m];
+            }
           }
           // the gaps and the slices' tails past the chunk hold far sentinels
           for (int k = max(0, tn / S - 1) + lane; k <= L; k += kWave)
@@ -223,12 +261,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
             // cnt (8 .. 32, a multiple of 8) tests of candidates p[0 .. cnt): bit
             // 31 - q flags candidate q; the bits are shifted in (h = 2 h + flag) so
             // that no per-test mask or shift is needed
+            // the sign bit of each test value is shifted in with one alignbit
             auto round = [&](const Cand<T>* p, int cnt) -> uint32_t {
               uint32_t h = 0;
               for (int q0 = 0; q0 < cnt; q0 += 8) {
 #pragma unroll
                 for (int q = 0; q < 8; ++q)
-                  h = h + h + (prefilter_r2(p[q0 + q], xi, yi, zi, m2x, m2y, m2z, xi2) < c2hi ? 1u : 0u);
+                  h = __builtin_amdgcn_alignbit(
+                      h, sign_word(prefilter_d(p[q0 + q], xi, yi, zi, m2x, m2y, m2z, base, c2hi)), 31);
               }
               return h << (32 - cnt);
             };
@@ -275,7 +315,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
                 }
                 T e, coef;
                 if (KIND == PFW_PAIR_SMOOTH) {
-                  const T a = sA[zsp * nz + zj], a6 = sA6[zsp * nz + zj];
+                  const T2 aa = sAA[zsp * nz + zj];
+                  const T a = aa.x, a6 = aa.y;
                   const T qq = (T)1 - r2 * invc2;
                   const T q2 = qq * qq;
                   e = a * q2 * qq;
@@ -366,12 +407,12 @@ static void launch_pair(pfw_ctx* ctx, uint32_t mask) {
   if (ctx->audit) {
     const AuditBufs au = audit_begin(ctx);
     hipLaunchKernelGGL((k_pair<T, KIND, true>), dim3(ctx->ncells), dim3(kWave), dyn, ctx->stream, s.grids,
-                       s.cell_cfg, s.n, s.start, s.xs, s.sps, s.perm, ctx->pair.d_params, ctx->pair.nz, lj, mask,
-                       ctx->d_F, ctx->d_partials, ctx->ncells, au);
+                       s.cell_cfg, s.n, s.start, s.xs, s.xr, s.sps, s.perm, ctx->pair.d_params, ctx->pair.nz, lj,
+                       mask, ctx->d_F, ctx->d_partials, ctx->ncells, au);
   } else {
     hipLaunchKernelGGL((k_pair<T, KIND, false>), dim3(ctx->ncells), dim3(kWave), dyn, ctx->stream, s.grids,
-                       s.cell_cfg, s.n, s.start, s.xs, s.sps, s.perm, ctx->pair.d_params, ctx->pair.nz, lj, mask,
-                       ctx->d_F, ctx->d_partials, ctx->ncells, AuditBufs{});
+                       s.cell_cfg, s.n, s.start, s.xs, s.xr, s.sps, s.perm, ctx->pair.d_params, ctx->pair.nz, lj,
+                       mask, ctx->d_F, ctx->d_partials, ctx->ncells, AuditBufs{});
   }
 }
 
@@ -466,7 +507,7 @@ constexpr size_t static_round(size_t b) { return (b + kStaticAlign - 1) / kStati
 // Byte offsets of the sub-buffers, in order.
 struct StaticLayout {
   size_t grid, params, flag, out, out2, red, cell_cfg, atom_cfg, count, start, blocksum, bounds, cell_of, rank, perm,
-      sps, xw, u, xs, partials, total;
+      sps, xw, u, xs, xr, partials, total;
 };
 constexpr StaticLayout static_layout() {
   StaticLayout L{};
@@ -495,11 +536,12 @@ constexpr StaticLayout static_layout() {
   L.xw = take(sizeof(double) * 3 * kStaticAtoms);
   L.u = take(sizeof(double) * 3 * kStaticAtoms);
   L.xs = take(sizeof(double) * 3 * kStaticAtoms);
+  L.xr = take(sizeof(float) * 4 * kStaticAtoms);
   L.partials = take(sizeof(double) * 7 * kStaticCells);
   L.total = o;
   return L;
 }
-constexpr StaticLayout kStatic = static_layout();  // about 42 MB
+constexpr StaticLayout kStatic = static_layout();  // about 46 MB
 
 static __device__ __align__(256) unsigned char scratch[kStatic.total];
 
@@ -543,12 +585,12 @@ __global__ void k_static_zero(double* __restrict__ out) {
 
 template <typename T, int KIND>
 static void launch_pair_static(const Grid* grid, const int32_t* cell_cfg, int64_t n, const int32_t* start,
-                               const double* xs, const int32_t* sps, const int32_t* perm, const double* params,
-                               int nz, LJConsts lj, uint32_t mask, double* F, double* partials, int ncells,
-                               hipStream_t st) {
+                               const double* xs, const float4* xr, const int32_t* sps, const int32_t* perm,
+                               const double* params, int nz, LJConsts lj, uint32_t mask, double* F, double* partials,
+                               int ncells, hipStream_t st) {
   const size_t dyn = KIND == PFW_PAIR_SMOOTH ? 2 * sizeof(T) * nz * nz : 0;
-  hipLaunchKernelGGL((k_pair<T, KIND, false>), dim3(ncells), dim3(kWave), dyn, st, grid, cell_cfg, n, start, xs, sps,
-                     perm, params, nz, lj, mask, F, partials, ncells, AuditBufs{});
+  hipLaunchKernelGGL((k_pair<T, KIND, false>), dim3(ncells), dim3(kWave), dyn, st, grid, cell_cfg, n, start, xs, xr,
+                     sps, perm, params, nz, lj, mask, F, partials, ncells, AuditBufs{});
 }
 
 void pair_run_static(pfw_ctx* ctx, int kind, int nz, const double* params, double cutoff, int64_t n,
@@ -629,14 +671,15 @@ void pair_run_static(pfw_ctx* ctx, int kind, int nz, const double* params, doubl
               (int32_t*)(base + L.perm),
               (double*)(base + L.xs),
               species,
-              (int32_t*)(base + L.sps)};
+              (int32_t*)(base + L.sps),
+              (float4*)(base + L.xr)};
   nl_build_core(v, st, false, nullptr);
   if (mask != 0 && n > 0) {
     const bool fp32 = precision == PFW_MIXED;
     double* part = (double*)(base + L.partials);
 #define PFW_STATIC_LAUNCH(T, K)                                                                                  \
-  launch_pair_static<T, K>(d_grid, (const int32_t*)(base + L.cell_cfg), n, v.start, v.xs, v.sps, v.perm, d_params, \
-                           nz, lj, mask, F, part, ncells, st)
+  launch_pair_static<T, K>(d_grid, (const int32_t*)(base + L.cell_cfg), n, v.start, v.xs, v.xr, v.sps, v.perm,     \
+                           d_params, nz, lj, mask, F, part, ncells, st)
     if (kind == PFW_PAIR_SMOOTH) {
       if (fp32)
         PFW_STATIC_LAUNCH(float, PFW_PAIR_SMOOTH);
