@@ -207,8 +207,9 @@ void pfw_host_free(void* p);
  * channel (index centre + nz*neighbour), real L2 spherical (or solid)
  * harmonics in l*l+l+m order, density (radial index, harmonic index) pairs, product
  * multisets of density indices (order <= 4), sparse coupling C (nbasis x
- * nprod) and readout WB (nbasis x nz), Wpair (nq x nz). Default basis policy:
- * every readout row enabled, no species projection, no order restriction. */
+ * nprod), readout WB (nbasis x nz), Wpair (nq x nz), and the basis policy's
+ * species projection (the caller applies the readout mask and the pair and
+ * reference flags to WB, Wpair and the reference energies). */
 enum { PFW_ENV_POLY2SX = 0, PFW_ENV_POLY1SR = 1, PFW_ENV_ACE1_POLY1SR = 2 };
 /* Angular basis (resolved.jl:109-110; angular_blocks.jl:47-48). */
 enum { PFW_ANGULAR_SPHERICAL = 0, PFW_ANGULAR_SOLID = 1 };
@@ -237,6 +238,15 @@ typedef struct {
   const double* WB;              /* nbasis x nz, column-major */
   const double* Wpair;           /* pair.nr x nz, column-major */
   int32_t angular;               /* PFW_ANGULAR_*: "real_spherical" or "real_solid" (r^l Y_lm) */
+  /* Basis policy of project_species (src/submodules/ACE/interface.jl:115-125):
+   * order_lo = order_hi = -1 for no order restriction; otherwise every product
+   * keeps the orders order_lo..order_hi of prod(U + t V), U and V the densities
+   * of neighbours outside and inside the projected species (selected: nz flags,
+   * NULL = none; at most 32 channels), and a pair term counts as order 1 for a
+   * projected neighbour, 0 otherwise (pair_allowed). The reference energies
+   * (order 0) are the caller's to zero (reference_value). */
+  const int32_t* selected;
+  int32_t order_lo, order_hi;
 } pfw_ace_model;
 
 /* Device tables of a resolved model: adapt_model / pack_model

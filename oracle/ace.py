@@ -90,10 +90,33 @@ def load_resolved(path):
     nq = pair["nr"]
     WB = flat[: m * nz].reshape(nz, m).T.copy()
     Wpair = flat[m * nz:].reshape(nz, nq).T.copy()
-    return dict(atomic_numbers=z, nz=nz, lmax=lmax, solid=ang["kind"] == "real_solid",
-                reference=np.asarray(D["reference_eV"], float), radial=radial,
-                pair=pair, density=dens, products=products, coupling=C, WB=WB, Wpair=Wpair,
-                cutoff=max(radial["rcut"], pair["rcut"]), source_sha256=D.get("source_sha256"))
+    out = dict(atomic_numbers=z, nz=nz, lmax=lmax, solid=ang["kind"] == "real_solid",
+               reference=np.asarray(D["reference_eV"], float), radial=radial,
+               pair=pair, density=dens, products=products, coupling=C, WB=WB, Wpair=Wpair,
+               cutoff=max(radial["rcut"], pair["rcut"]), source_sha256=D.get("source_sha256"))
+    return with_policy(out, D.get("interface_policy"))
+
+
+def with_policy(model, P):
+    """The basis policy of a resolved file (interface_persistence.jl:3-25) applied
+    the way the evaluation uses it: WB .* mask (effective_WB), no pair term when
+    disabled (pair_allowed), no reference energies when disabled
+    (reference_value), and the projected species (0-based channels) and orders
+    (interface.jl:115-125)."""
+    if not P:
+        return model
+    m = dict(model)
+    mask = np.asarray(P.get("mask", [True] * m["WB"].shape[0]), bool)
+    m["WB"] = m["WB"] * mask[:, None]
+    if not P.get("pair", True):
+        m["Wpair"] = np.zeros_like(m["Wpair"])
+    if not P.get("reference", True):
+        m["reference"] = np.zeros_like(m["reference"])
+    orders = list(P.get("orders", []))
+    if orders:
+        m["orders"] = (int(orders[0]), int(orders[1]))
+        m["species"] = tuple(int(v) - 1 for v in P.get("species", []))
+    return m
 
 
 def read_frames(path):
@@ -139,7 +162,8 @@ class _Model(ctypes.Structure):
                 ("reference", ctypes.c_void_p), ("radial", _Family), ("pair", _Family),
                 ("dens_n", ctypes.c_void_p), ("dens_m", ctypes.c_void_p), ("prod_ptr", ctypes.c_void_p),
                 ("prod_idx", ctypes.c_void_p), ("weights", ctypes.c_void_p), ("wpair", ctypes.c_void_p),
-                ("solid", ctypes.c_int)]
+                ("solid", ctypes.c_int), ("selected", ctypes.c_void_p), ("order_lo", ctypes.c_int),
+                ("order_hi", ctypes.c_int), ("exclude", ctypes.c_int)]
 
 
 def _pack(model):
@@ -159,12 +183,35 @@ def _pack(model):
     ptr[1:] = np.cumsum([len(p) for p in prods])
     idx = np.concatenate([np.asarray(p, np.int32) for p in prods]) if prods else np.zeros(0, np.int32)
     weights = model["coupling"].T @ model["WB"]  # C^T (WB .* mask), default mask
+    orders = model.get("orders")  # None, or (lo, hi) of project_species
+    sel = np.zeros(model["nz"], np.int32)
+    for z in model.get("species", ()):
+        sel[z] = 1
     m = _Model(model["nz"], model["lmax"], len(model["density"]), len(prods), arr(model["reference"], np.float64),
                fam(model["radial"]), fam(model["pair"]), arr([d[0] for d in model["density"]], np.int32),
                arr([d[1] for d in model["density"]], np.int32), arr(ptr, np.int32), arr(idx, np.int32),
                arr(weights.T.copy(), np.float64), arr(model["Wpair"].T.copy(), np.float64),
-               int(model.get("solid", False)))
+               int(model.get("solid", False)), arr(sel, np.int32), -1 if orders is None else int(orders[0]),
+               -1 if orders is None else int(orders[1]), int(model.get("exclude", -1)))
     return m, keep
+
+
+lib.oracle_projected_product.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_void_p,
+                                         ctypes.c_void_p]
+lib.oracle_projected_product.restype = ctypes.c_double
+
+
+def projected_product(indices, U, V, lo, hi, w=1.0):
+    """(P, dU, dV) of one product under orders lo:hi (projection.jl:7-38)."""
+    idx = np.ascontiguousarray(np.asarray(indices, np.int32))
+    U = np.ascontiguousarray(np.asarray(U, float))
+    V = np.ascontiguousarray(np.asarray(V, float))
+    dU = np.zeros(len(U))
+    dV = np.zeros(len(V))
+    P = lib.oracle_projected_product(idx.ctypes.data, len(idx), U.ctypes.data, V.ctypes.data, int(lo), int(hi),
+                                     float(w), dU.ctypes.data, dV.ctypes.data)
+    return P, dU, dV
 
 
 lib.oracle_ace_evaluate.argtypes = [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,

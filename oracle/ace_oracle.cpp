@@ -1,18 +1,20 @@
 // Test infrastructure only (the oracle). CPU FP64 restatement of the ACE site
-// pipeline of PotentialFrameWork.jl for resolved (schema v1) models with the
-// default basis policy (all readout rows enabled, no species projection, no
-// order restriction):
-//   radial_ed / transformed / envelope(_derivative)  src/submodules/ACE/resolved.jl:156-250
+// pipeline of PotentialFrameWork.jl for resolved (schema v1) models, with the
+// basis policy's species projection and order restriction:
+//   radial_ed / transformed / envelope(_derivative)  src/submodules/ACE/resolved.jl:156-209
 //   BlockAngular recurrence with Cartesian jets      src/submodules/ACE/angular_blocks.jl:25-81
-//   site_data / site_reverse                         src/submodules/ACE/serial.jl:260-310
+//   site_data / site_reverse (U/V species split)     src/submodules/ACE/serial.jl:51-101
 //   unrestricted_products!                           src/submodules/ACE/projection.jl:39-56
+//   product_projection! (truncated (U + tV) products) src/submodules/ACE/projection.jl:7-38
+//   order_allowed / pair_allowed / reference_value   src/submodules/ACE/interface.jl:122-124
 //   reduce_site!                                     src/submodules/NeighbourList/execution.jl:105-122
 // Environments are the directed snapshot of the neighbour-list oracle grouped
-// by neighbour species in traversal order (src/submodules/NeighbourList/environments.jl:357-391).
+// by neighbour species in traversal order (src/submodules/NeighbourList/environments.jl:42-102).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <thread>
+#include <vector>
 
 #include "nl_oracle.h"
 
@@ -43,6 +45,13 @@ extern "C" struct OracleAceModel {
   const double* weights;  // nprod x nz, column-major: C^T (WB .* mask)
   const double* wpair;    // nq x nz column-major
   int solid;              // 1: real solid harmonics r^l Y_lm ("real_solid"), 0: spherical
+  // basis policy: order_lo < 0 means no order restriction (every product is
+  // the full product of its densities); otherwise orders order_lo:order_hi of
+  // the species projection, with selected[z] != 0 for the projected neighbour
+  // species channels (project_species, interface.jl:115-124)
+  const int* selected;    // nz flags, or NULL
+  int order_lo, order_hi;
+  int exclude;            // test aid: neighbour channel whose edges are dropped (-1: none)
 };
 
 namespace oracle {
@@ -217,6 +226,43 @@ struct SiteOut {
   std::vector<double> g;  // 3 per edge
 };
 
+// The truncated product projection of projection.jl:7-38 for one product of
+// nu factors: P = sum over k in [lo, hi] of the coefficient of t^k in
+// prod_s (U[a_s] + t V[a_s]), i.e. of the terms taking k factors from V.
+// fwd[k][t] holds that coefficient for the first t factors; the reverse sweep
+// carries the adjoint weight down the same recurrence, adding w dP/dU and
+// w dP/dV into dU and dV. Returns P.
+static double projected_product(const int* idx, int nu, const double* U, const double* V, int lo, int hi, double w,
+                                double* dU, double* dV) {
+  const int K = std::min(nu, hi);  // coefficients above the highest kept order never matter
+  std::vector<double> fwd((K + 1) * (nu + 1), 0.0), rev((K + 1) * (nu + 1), 0.0);
+  auto F = [&](int k, int t) -> double& { return fwd[k * (nu + 1) + t]; };
+  auto G = [&](int k, int t) -> double& { return rev[k * (nu + 1) + t]; };
+  F(0, 0) = 1;
+  for (int t = 1; t <= nu; ++t) {
+    const int a = idx[t - 1];
+    for (int k = 0; k <= std::min(t, K); ++k) F(k, t) = U[a] * F(k, t - 1) + (k > 0 ? V[a] * F(k - 1, t - 1) : 0.0);
+  }
+  double value = 0;
+  for (int k = std::max(lo, 0); k <= K; ++k) {
+    value += F(k, nu);
+    G(k, nu) = w;
+  }
+  for (int t = nu; t >= 1; --t) {
+    const int a = idx[t - 1];
+    for (int k = 0; k <= std::min(t, K); ++k) {
+      const double g = G(k, t);
+      dU[a] += g * F(k, t - 1);
+      G(k, t - 1) += g * U[a];
+      if (k > 0) {
+        dV[a] += g * F(k - 1, t - 1);
+        G(k - 1, t - 1) += g * V[a];
+      }
+    }
+  }
+  return value;
+}
+
 // serial.jl:260-310 with cpu.jl:214-236 for the readout and gradients.
 static void site(const OracleAceModel& m, int iz, const std::vector<Edge>& edges, bool grads, SiteOut& out) {
   int ne = (int)edges.size(), nr = m.radial.nr, nq = m.pair.nr, ny = (m.lmax + 1) * (m.lmax + 1);
@@ -230,14 +276,31 @@ static void site(const OracleAceModel& m, int iz, const std::vector<Edge>& edges
     if (nq) radial_ed(m.pair, ch, rs[e], &Q[e * nq], &dQ[e * nq]);
     angular(m.lmax, d, &Y[e * ny], &dY[e * ny * 3], m.solid);
   }
-  std::vector<double> A(m.ndens, 0.0);
+  // restricted orders: densities split by neighbour species into U (others)
+  // and V (projected species); the pair term of an edge counts as order 1
+  // when its neighbour is projected, else 0; the reference energy as order 0
+  const bool proj = m.order_lo >= 0;
+  auto order_allowed = [&](int k) { return !proj || (m.order_lo <= k && k <= m.order_hi); };
+  auto selected = [&](int z) { return proj && m.selected && m.selected[z] != 0; };
+  std::vector<double> A(m.ndens, 0.0), Vd(m.ndens, 0.0);
   for (int a = 0; a < m.ndens; ++a)
-    for (int e = 0; e < ne; ++e) A[a] += R[e * nr + m.dens_n[a]] * Y[e * ny + m.dens_m[a]];
+    for (int e = 0; e < ne; ++e) {
+      const double v = R[e * nr + m.dens_n[a]] * Y[e * ny + m.dens_m[a]];
+      if (selected(edges[e].zj))
+        Vd[a] += v;
+      else
+        A[a] += v;
+    }
   const double* w = m.weights + (size_t)m.nprod * iz;
-  double energy = m.reference[iz];
-  std::vector<double> dA(m.ndens, 0.0);
+  double energy = order_allowed(0) ? m.reference[iz] : 0.0;
+  std::vector<double> dA(m.ndens, 0.0), dV(m.ndens, 0.0);
   for (int k = 0; k < m.nprod; ++k) {
     int b = m.prod_ptr[k], nu = m.prod_ptr[k + 1] - b;
+    if (proj) {
+      energy += w[k] * projected_product(m.prod_idx + b, nu, A.data(), Vd.data(), m.order_lo, m.order_hi, w[k],
+                                         dA.data(), dV.data());
+      continue;
+    }
     double pre[8];
     pre[0] = 1;
     for (int t = 0; t < nu; ++t) pre[t + 1] = pre[t] * A[m.prod_idx[b + t]];
@@ -251,20 +314,24 @@ static void site(const OracleAceModel& m, int iz, const std::vector<Edge>& edges
   }
   const double* wp = m.wpair + (size_t)nq * iz;
   for (int e = 0; e < ne; ++e)
-    for (int n = 0; n < nq; ++n) energy += Q[e * nq + n] * wp[n];
+    if (order_allowed(selected(edges[e].zj) ? 1 : 0))
+      for (int n = 0; n < nq; ++n) energy += Q[e * nq + n] * wp[n];
   out.energy = energy;
   if (!grads) return;
   out.g.assign(3 * ne, 0.0);
   for (int e = 0; e < ne; ++e) {
     double dir[3] = {edges[e].R[0] / rs[e], edges[e].R[1] / rs[e], edges[e].R[2] / rs[e]};
     double g[3] = {0, 0, 0};
+    const bool sel = selected(edges[e].zj);
+    const std::vector<double>& adj = sel ? dV : dA;
     for (int a = 0; a < m.ndens; ++a) {
       int n = m.dens_n[a], mm = m.dens_m[a];
       for (int c = 0; c < 3; ++c)
-        g[c] += dA[a] * (dR[e * nr + n] * Y[e * ny + mm] * dir[c] + R[e * nr + n] * dY[(e * ny + mm) * 3 + c]);
+        g[c] += adj[a] * (dR[e * nr + n] * Y[e * ny + mm] * dir[c] + R[e * nr + n] * dY[(e * ny + mm) * 3 + c]);
     }
     double sq = 0;
-    for (int n = 0; n < nq; ++n) sq += wp[n] * dQ[e * nq + n];
+    if (order_allowed(sel ? 1 : 0))
+      for (int n = 0; n < nq; ++n) sq += wp[n] * dQ[e * nq + n];
     for (int c = 0; c < 3; ++c) out.g[3 * e + c] = g[c] + sq * dir[c];
   }
 }
@@ -300,7 +367,8 @@ int oracle_ace_evaluate(int64_t n, const double* X, const int32_t* species, cons
       if (species[i] < 0 || species[i] >= m->nz) throw ArgumentError("invalid species channel");
     // per-centre environments in traversal order, grouped by neighbour species (stable)
     std::vector<std::vector<Edge>> env(n);
-    for (auto& p : pairs) env[p.i].push_back(Edge{p.j, species[p.j], {p.R[0], p.R[1], p.R[2]}});
+    for (auto& p : pairs)
+      if (species[p.j] != m->exclude) env[p.i].push_back(Edge{p.j, species[p.j], {p.R[0], p.R[1], p.R[2]}});
     for (auto& e : env)
       std::stable_sort(e.begin(), e.end(), [](const Edge& a, const Edge& b) { return a.zj < b.zj; });
     bool grads = (mask & 6) != 0;
@@ -369,6 +437,13 @@ int oracle_ace_evaluate(int64_t n, const double* X, const int32_t* species, cons
 }
 
 }  // extern "C"
+
+// The projected product of one product (P, with w dP/dU and w dP/dV added into
+// dU and dV) for tests against a brute-force enumeration.
+extern "C" double oracle_projected_product(const int* idx, int nu, const double* U, const double* V, int lo, int hi,
+                                           double w, double* dU, double* dV) {
+  return oracle::projected_product(idx, nu, U, V, lo, hi, w, dU, dV);
+}
 
 extern "C" int oracle_ylm(int L, int solid, const double* R, double* Y, double* dY) {
   try {

@@ -35,7 +35,7 @@ from .pair import (  # noqa: F401
     evaluate,
     evaluate_batch,
 )
-from .ace import ACECalculator, ResolvedModel, load_resolved, ace_evaluate  # noqa: F401
+from .ace import ACECalculator, ResolvedModel, load_resolved, ace_evaluate, project_species  # noqa: F401
 from . import ace, ace_basis  # noqa: F401
 from .adjoints import (  # noqa: F401
     pair_parameter_vjp,

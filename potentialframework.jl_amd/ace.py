@@ -1,7 +1,8 @@
 """ACE on the GPU: resolved-model loading and the calculator boundary.
 
 Mirrors src/submodules/ACE/resolved.jl:57-142 (``load_resolved`` / ``read_resolved``
-for schema v1, default basis policy), src/submodules/ACE/environments.jl:1-10
+for schema v1, with the basis policy of src/submodules/ACE/interface.jl:1-125),
+src/submodules/ACE/environments.jl:1-10
 (strict element channels) and src/submodules/ACE/calculators.jl (``ACECalculator``
 with ``energy``/``forces``/``virial`` results in original atom order).
 """
@@ -89,7 +90,7 @@ class RadialFamily:
 
 
 class ResolvedModel:
-    """Device-ready resolved ACE model (schema v1, default policy)."""
+    """Device-ready resolved ACE model (schema v1 with its basis policy)."""
 
     def __init__(self, D):
         if D.get("schema") != "PotentialFrameWork.resolved_ACE" or D.get("schema_version") != 1:
@@ -134,13 +135,16 @@ class ResolvedModel:
         self.source = {k: v for k, v in D.items() if k != "parameters"}  # kept for save_calculator
 
     def _read_policy(self, P):
-        """The basis policy (interface.jl:2-18, interface_persistence.jl:8-25): the
+        """The basis policy (interface.jl:2-18, interface_persistence.jl:3-25): the
         invariant mask (effective_WB = WB .* mask), the pair and reference flags
-        (pair_allowed / reference_value, interface.jl:122-124). Species projection
-        and order restriction are not supported on the device."""
+        (pair_allowed / reference_value, interface.jl:122-124), and the species
+        projection with its orders (project_species, interface.jl:115-121):
+        ``species`` holds 0-based channels, ``orders`` is None or ``(lo, hi)``."""
         self.mask = np.ones(self.nbasis, bool)
         self.pair_enabled = True
         self.reference_enabled = True
+        self.species = ()
+        self.orders = None
         if P is None:
             return
         if P.get("version") != 1:
@@ -148,11 +152,21 @@ class ResolvedModel:
         mask = P.get("mask", [True] * self.nbasis)
         if len(mask) != self.nbasis or not all(isinstance(v, bool) for v in mask):
             raise ArgumentError("invalid invariant mask")
-        if list(P.get("species", [])) or list(P.get("orders", [])):
-            raise ArgumentError("species-projected or order-restricted ACE bases are not supported on the device")
+        sp = [int(v) for v in P.get("species", [])]
+        if not all(1 <= v <= self.nz for v in sp) or len(set(sp)) != len(sp):
+            raise ArgumentError("invalid projected species")
+        o = [int(v) for v in P.get("orders", [])]
+        if o and not (len(o) == 2 and 0 <= o[0] <= o[1]):
+            raise ArgumentError("invalid projected orders")
         self.mask = np.asarray(mask, bool)
         self.pair_enabled = bool(P.get("pair", True))
         self.reference_enabled = bool(P.get("reference", True))
+        self.species = tuple(sorted(v - 1 for v in sp))
+        self.orders = (o[0], o[1]) if o else None
+
+    def order_allowed(self, k):
+        """order_allowed (interface.jl:122)."""
+        return self.orders is None or self.orders[0] <= k <= self.orders[1]
 
     def effective(self, WB, Wpair):
         """The readout the device evaluates: WB .* mask, and Wpair only when the
@@ -190,12 +204,16 @@ class ResolvedModel:
         pp[1:] = np.cumsum([len(p) for p in self.products])
         pidx = np.concatenate([np.asarray(p, np.int32) for p in self.products]) if self.products else np.zeros(0)
         WB, Wpair = self.effective(self.WB, self.Wpair)
-        ref = self.reference if self.reference_enabled else np.zeros_like(self.reference)
+        # reference_value (interface.jl:124): the reference energy is order 0
+        ref = self.reference if (self.reference_enabled and self.order_allowed(0)) else np.zeros_like(self.reference)
+        sel = np.zeros(self.nz, np.int32)
+        sel[list(self.species)] = 1
+        lo, hi = self.orders if self.orders is not None else (-1, -1)
         m = _Model(self.nz, self.lmax, len(self.density), len(self.products), self.nbasis,
                    a(ref, np.float64), fam(self.radial), fam(self.pair), a(self.density, np.int32),
                    a(pp, np.int32), a(pidx, np.int32), len(self.c_vals), a(self.c_rows, np.int32),
                    a(self.c_cols, np.int32), a(self.c_vals, np.float64), a(WB.T, np.float64),
-                   a(Wpair.T, np.float64), 1 if self.solid else 0)
+                   a(Wpair.T, np.float64), 1 if self.solid else 0, a(sel, np.int32), lo, hi)
         return m, keep
 
 
@@ -212,7 +230,39 @@ class _Model(ctypes.Structure):
                 ("product_ptr", ctypes.c_void_p), ("product_idx", ctypes.c_void_p), ("coupling_nnz", ctypes.c_int64),
                 ("coupling_rows", ctypes.c_void_p), ("coupling_cols", ctypes.c_void_p),
                 ("coupling_vals", ctypes.c_void_p), ("WB", ctypes.c_void_p), ("Wpair", ctypes.c_void_p),
-                ("angular", ctypes.c_int32)]
+                ("angular", ctypes.c_int32), ("selected", ctypes.c_void_p), ("order_lo", ctypes.c_int32),
+                ("order_hi", ctypes.c_int32)]
+
+
+def project_species(model, species, orders):
+    """``project_species(m, species, orders)`` (interface.jl:115-121): a copy of
+    the model whose products keep only the orders ``orders`` (a ``range`` with
+    step 1, nonnegative, nonempty) of prod(U + t V), V the densities of
+    neighbours of the given species (an element label, atomic number, or a
+    list of them) and U those of the others; the pair term of a projected
+    neighbour counts as order 1, otherwise 0, and the reference energy as
+    order 0 (order_allowed / pair_allowed / reference_value, :122-124)."""
+    import copy
+    if not isinstance(orders, range) or orders.step != 1:
+        raise ArgumentError("orders must be a nonnegative inclusive unit range")
+    if len(orders) == 0 or orders.start < 0:
+        raise ArgumentError("orders must be a nonnegative inclusive range")
+    group = (species,) if isinstance(species, (str, int, np.integer)) else tuple(species)
+    chans = []
+    for s_ in group:
+        z = atomic_number(s_)
+        if z not in model.atomic_numbers:
+            raise ArgumentError(f"ACE model has no element channel for species {s_}")
+        chans.append(model.atomic_numbers.index(z))
+    out = copy.copy(model)
+    out.species = tuple(sorted(set(chans)))
+    out.orders = (orders.start, orders[-1])
+    out.source = dict(model.source)
+    pol = dict(out.source.get("interface_policy") or {}, version=1)
+    pol["species"] = [c + 1 for c in out.species]
+    pol["orders"] = [out.orders[0], out.orders[1]]
+    out.source["interface_policy"] = pol
+    return out
 
 
 def load_resolved(source):
@@ -380,8 +430,8 @@ def save_calculator(path, calc, parameters=None, state=None):
                                      mask=[bool(v) for v in m.mask], pair=m.pair_enabled,
                                      reference=m.reference_enabled)
     pol = model["interface_policy"]
-    pol.setdefault("species", [])
-    pol.setdefault("orders", [])
+    pol["species"] = [c + 1 for c in m.species]
+    pol["orders"] = list(m.orders) if m.orders is not None else []
     # provenance maps of a default policy (interface_persistence.jl:8-25 validates them)
     pol.setdefault("rows", list(range(1, m.nbasis + 1)))
     pol.setdefault("products", list(range(1, len(m.products) + 1)))

@@ -73,6 +73,13 @@ struct AceArgs {
   double* siteW;
   T* eg;  // 3 x E edge gradients in the arithmetic precision
   int solid;  // angular basis: 1 real solid harmonics r^l Y_lm, 0 spherical Y_lm
+  // basis policy (interface.jl:115-125): with proj, each site has densities U
+  // (neighbours outside the projected species) and V (inside), stored as
+  // [U | V] (dstride = 2 ndens), and a product keeps only the orders
+  // olo..ohi of prod (U + t V); the pair term of a neighbour of species z is
+  // kept when bit z of pair_gate is set
+  int proj, dstride, olo, ohi;
+  uint32_t sel, pair_gate;
 };
 
 template <typename T>
@@ -299,16 +306,17 @@ __global__ __launch_bounds__(64) void k_ace_dens(AceArgs<T> a) {
   constexpr int NY = (L + 1) * (L + 1);
   constexpr int LPS = kWave / SPW;
   T* Fl = (T*)smem;                      // (L+1)^2
-  T* A = Fl + NY;                        // SPW x ndens
-  T* Rb = A + SPW * a.ndens;             // rchunk x kStride: one chunk of radial indices
-  T* Yb = Rb + a.rchunk * kStride;       // NY x kStride
+  T* A = Fl + NY;                        // SPW x dstride
+  T* Rb = A + SPW * a.dstride;           // rchunk x kStride: one chunk of radial indices
+  T* Yb = Rb + a.rchunk * kStride;       // NY x kStride (with proj: the U part)
+  T* Yv = Yb + NY * kStride;             // with proj: NY x kStride, the V part
   const int lane = threadIdx.x;
   const int sl = lane / LPS, ll = lane % LPS;
   const int64_t site = (int64_t)blockIdx.x * SPW + sl;
   const bool valid = site < a.n;
   fill_fl<T, L>(Fl);
-  for (int k = lane; k < SPW * a.ndens; k += kWave) A[k] = 0;
-  T* As = A + sl * a.ndens;
+  for (int k = lane; k < SPW * a.dstride; k += kWave) A[k] = 0;
+  T* As = A + sl * a.dstride;
   const int iz = valid ? a.sps[site] : 0;
   const int64_t e0 = valid ? site * a.ecap : 0;
   const int ne = valid ? a.ecount[site] : 0;
@@ -330,9 +338,10 @@ __global__ __launch_bounds__(64) void k_ace_dens(AceArgs<T> a) {
       const double* R = a.eR + 3 * (int64_t)(e0 + e);
       const T dx = (T)R[0], dy = (T)R[1], dz = (T)R[2];
       const T r = (T)sqrt(R[0] * R[0] + R[1] * R[1] + R[2] * R[2]);
-      ch = iz + a.nz * a.sps[a.enb[e0 + e]];
+      const int jz = a.sps[a.enb[e0 + e]];
+      ch = iz + a.nz * jz;
       P = radial_point(a.rad, ch, r);
-      if (nq) {
+      if (nq && ((a.pair_gate >> jz) & 1u)) {
         const RadialPoint<T> PQ = radial_point(a.pair, ch, r);
         T s = 0;
         for (int n = 0; n < nq; ++n) {
@@ -344,8 +353,17 @@ __global__ __launch_bounds__(64) void k_ace_dens(AceArgs<T> a) {
       }
       T Y[NY], G[3 * NY];
       ylm<T, L>(dx, dy, dz, Fl, Y, G, a.solid);
+      if (a.proj) {
+        const bool in_v = (a.sel >> jz) & 1u;
 #pragma unroll
-      for (int m = 0; m < NY; ++m) Yb[m * kStride + lane] = Y[m];
+        for (int m = 0; m < NY; ++m) {
+          Yb[m * kStride + lane] = in_v ? (T)0 : Y[m];
+          Yv[m * kStride + lane] = in_v ? Y[m] : (T)0;
+        }
+      } else {
+#pragma unroll
+        for (int m = 0; m < NY; ++m) Yb[m * kStride + lane] = Y[m];
+      }
     }
       const int cn = min(LPS, ne - c0);
     const bool single = a.rchunk >= a.rad.nr;
@@ -368,12 +386,19 @@ __global__ __launch_bounds__(64) void k_ace_dens(AceArgs<T> a) {
 #pragma unroll 8
         for (int t = 0; t < cn; ++t) s += rr[t] * yy[t];
         As[k] += s;
+        if (a.proj) {
+          const T* yv = Yv + a.dens[2 * k + 1] * kStride + col0;
+          T sv = 0;
+#pragma unroll 8
+          for (int t = 0; t < cn; ++t) sv += rr[t] * yv[t];
+          As[a.ndens + k] += sv;
+        }
       }
       __syncthreads();
     }
   }
   if (valid) {
-    for (int k = ll; k < a.ndens; k += LPS) a.Abuf[site * a.ndens + k] = As[k];
+    for (int k = ll; k < a.dstride; k += LPS) a.Abuf[site * a.dstride + k] = As[k];
     for (int off = LPS / 2; off > 0; off >>= 1) epair += __shfl_xor(epair, off, kWave);
     if (ll == 0) a.siteE[site] = epair;
   }
@@ -423,14 +448,18 @@ __global__ __launch_bounds__(1024) void k_ace_prod(AceArgs<T> a, int grads) {
     sched = lsched;
     cb = lcb;
   }
-  const size_t per_wave = align16(sizeof(T) * a.ndens) + align16(sizeof(T) * a.ntarget);
+  const int nacc = a.proj ? 2 * a.ntarget : a.ntarget;  // with proj: dE/dU targets, then dE/dV targets
+  const size_t per_wave = align16(sizeof(T) * a.dstride) + align16(sizeof(T) * nacc);
   T* A = (T*)(pw + wave * per_wave);
-  T* acc = (T*)((unsigned char*)A + align16(sizeof(T) * a.ndens));
+  T* acc = (T*)((unsigned char*)A + align16(sizeof(T) * a.dstride));
+  // kept orders as weights of the coefficients of t^0 .. t^4
+  T om[6];
+  for (int k = 0; k < 6; ++k) om[k] = (k <= 4 && a.olo <= k && k <= a.ohi) ? (T)1 : (T)0;
   for (int64_t site = (int64_t)blockIdx.x * nwaves + wave; site < a.n; site += (int64_t)gridDim.x * nwaves) {
-    T* Ag = a.Abuf + site * a.ndens;
-    for (int k = lane; k < a.ndens; k += kWave) A[k] = Ag[k];
+    T* Ag = a.Abuf + site * a.dstride;
+    for (int k = lane; k < a.dstride; k += kWave) A[k] = Ag[k];
     if (grads)
-      for (int k = lane; k < a.ntarget; k += kWave) acc[k] = 0;
+      for (int k = lane; k < nacc; k += kWave) acc[k] = 0;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -443,6 +472,55 @@ __global__ __launch_bounds__(1024) void k_ace_prod(AceArgs<T> a, int grads) {
       const unsigned f0 = f.x & 0xffff, f1 = f.x >> 16, f2 = f.y & 0xffff, f3 = f.y >> 16;
       if (f0 == kNone) continue;  // padding
       const T w = wz[slot];
+      if (a.proj) {
+        // the truncated product of projection.jl:7-38: the coefficients c_k of
+        // prod_s (U_s + t V_s) over the (at most four) factors; P = sum of the
+        // kept orders; d/dU_s and d/dV_s from the coefficients o_k of the
+        // product of the other factors: dP/dU_s = sum om_k o_k,
+        // dP/dV_s = sum om_{k+1} o_k (unused factors are 1 + 0 t)
+        const unsigned fs[4] = {f0, f1, f2, f3};
+        T u[4], v[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          u[q] = fs[q] != kNone ? A[fs[q]] : (T)1;
+          v[q] = fs[q] != kNone ? A[a.ndens + fs[q]] : (T)0;
+        }
+        T c[5] = {(T)1, (T)0, (T)0, (T)0, (T)0};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+#pragma unroll
+          for (int k = 4; k > 0; --k) c[k] = u[q] * c[k] + v[q] * c[k - 1];
+          c[0] = u[q] * c[0];
+        }
+        T pv = 0;
+#pragma unroll
+        for (int k = 0; k <= 4; ++k) pv += om[k] * c[k];
+        eprod += (double)(w * pv);
+        if (grads) {
+          const unsigned ts[4] = {f.z & 0xffff, f.z >> 16, f.w & 0xffff, f.w >> 16};
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            if (fs[q] == kNone) continue;
+            T o[4] = {(T)1, (T)0, (T)0, (T)0};
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              if (r == q) continue;
+#pragma unroll
+              for (int k = 3; k > 0; --k) o[k] = u[r] * o[k] + v[r] * o[k - 1];
+              o[0] = u[r] * o[0];
+            }
+            T du = 0, dv = 0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+              du += om[k] * o[k];
+              dv += om[k + 1] * o[k];
+            }
+            acc[ts[q]] += w * du;
+            acc[a.ntarget + ts[q]] += w * dv;
+          }
+        }
+        continue;
+      }
       const T x0 = A[f0];
       const T x1 = f1 != kNone ? A[f1] : (T)1;
       const T x2 = f2 != kNone ? A[f2] : (T)1;
@@ -460,10 +538,12 @@ __global__ __launch_bounds__(1024) void k_ace_prod(AceArgs<T> a, int grads) {
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     if (grads)
-      for (int k = lane; k < a.ndens; k += kWave) {
+      for (int k = lane; k < a.dstride; k += kWave) {
+        const int kd = k < a.ndens ? k : k - a.ndens;
+        const T* ac = k < a.ndens ? acc : acc + a.ntarget;
         T d = 0;
-        for (int c = cb[k]; c < cb[k + 1]; ++c) d += acc[c];
-        Ag[k] = d;  // dE/dA over A
+        for (int c = cb[kd]; c < cb[kd + 1]; ++c) d += ac[c];
+        Ag[k] = d;  // dE/dA (with proj: dE/dU, then dE/dV) over A
       }
     const double es = wave_sum(eprod);
     if (lane == 0) a.siteE[site] = a.reference[iz] + a.siteE[site] + es;
@@ -482,15 +562,15 @@ __global__ __launch_bounds__(64) void k_ace_pull(AceArgs<T> a) {
   constexpr int NY = (L + 1) * (L + 1);
   constexpr int LPS = kWave / SPW;
   T* Fl = (T*)smem;                      // (L+1)^2
-  T* dA = Fl + NY;                       // SPW x ndens
+  T* dA = Fl + NY;                       // SPW x dstride
   const int lane = threadIdx.x;
   const int sl = lane / LPS, ll = lane % LPS;
   const int64_t site = (int64_t)blockIdx.x * SPW + sl;
   const bool valid = site < a.n;
   fill_fl<T, L>(Fl);
-  T* dAs = dA + sl * a.ndens;
+  T* dAs = dA + sl * a.dstride;
   if (valid)
-    for (int k = ll; k < a.ndens; k += LPS) dAs[k] = a.Abuf[site * a.ndens + k];
+    for (int k = ll; k < a.dstride; k += LPS) dAs[k] = a.Abuf[site * a.dstride + k];
   const int iz = valid ? a.sps[site] : 0;
   const int64_t e0 = valid ? site * a.ecap : 0;
   const int ne = valid ? a.ecount[site] : 0;
@@ -506,7 +586,10 @@ __global__ __launch_bounds__(64) void k_ace_pull(AceArgs<T> a) {
     const T dx = (T)R[0], dy = (T)R[1], dz = (T)R[2];
     const T r = (T)sqrt(R[0] * R[0] + R[1] * R[1] + R[2] * R[2]);
     const T ri = (T)1 / r;
-    const int ch = iz + a.nz * a.sps[a.enb[e0 + e]];
+    const int jz = a.sps[a.enb[e0 + e]];
+    const int ch = iz + a.nz * jz;
+    // with proj, the edge reads dE/dV when its neighbour is projected, else dE/dU
+    const T* dsel = dAs + ((a.proj && ((a.sel >> jz) & 1u)) ? a.ndens : 0);
     // U_m = sum_r dE/dA[r, m] R_r and V_m = sum_r dE/dA[r, m] R'_r per harmonic;
     // then g = (sum_m V_m Y_m) r^ + sum_m U_m grad Y_m, the harmonics contracted
     // as they are formed (no arrays of values and gradients held)
@@ -524,13 +607,13 @@ __global__ __launch_bounds__(64) void k_ace_pull(AceArgs<T> a) {
         radial_value(a.rad, ch, P, n, v, d);
 #pragma unroll
         for (int mm = 0; mm <= 2 * l; ++mm) {
-          const T c = dAs[a0 + mm];
+          const T c = dsel[a0 + mm];
           U[l * l + mm] += c * v;
           V[l * l + mm] += c * d;
         }
       }
     }
-    if (nq) {
+    if (nq && ((a.pair_gate >> jz) & 1u)) {
       const RadialPoint<T> PQ = radial_point(a.pair, ch, r);
       for (int n = 0; n < nq; ++n) {
         T v, d;
@@ -1368,6 +1451,26 @@ void ace_set_model(pfw_ctx* ctx, const pfw_ace_model* m) {
   if (m->angular != PFW_ANGULAR_SPHERICAL && m->angular != PFW_ANGULAR_SOLID)
     throw Error(PFW_INVALID_ARGUMENT, "unknown angular basis");
   a.solid = m->angular == PFW_ANGULAR_SOLID ? 1 : 0;
+  // basis policy: species projection with restricted orders (interface.jl:115-125)
+  a.proj = m->order_lo >= 0 ? 1 : 0;
+  a.order_lo = m->order_lo;
+  a.order_hi = m->order_hi;
+  a.sel = 0;
+  a.pair_gate = 0xffffffffu;
+  if (a.proj) {
+    if (m->order_hi < m->order_lo) throw Error(PFW_INVALID_ARGUMENT, "orders must be a nonnegative inclusive range");
+    if (m->nz > 32) throw Error(PFW_INVALID_ARGUMENT, "species projection supports at most 32 species channels");
+    for (int z = 0; z < m->nz; ++z)
+      if (m->selected && m->selected[z]) a.sel |= 1u << z;
+    // order_allowed(z in species ? 1 : 0) of pair_allowed (interface.jl:123)
+    a.pair_gate = 0;
+    for (int z = 0; z < m->nz; ++z) {
+      const int k = ((a.sel >> z) & 1u) ? 1 : 0;
+      if (m->order_lo <= k && k <= m->order_hi) a.pair_gate |= 1u << z;
+    }
+  } else if (m->order_hi >= 0) {
+    throw Error(PFW_INVALID_ARGUMENT, "orders must be a nonnegative inclusive range");
+  }
   a.ndens = m->ndens;
   a.nprod = m->nprod;
   a.nbasis = m->nbasis;
@@ -1535,6 +1638,12 @@ static AceArgs<T> make_args(pfw_ctx* ctx, int grads) {
   a.ntarget = ad.ntarget;
   a.Abuf = (T*)ad.Abuf;
   a.solid = ad.solid;
+  a.proj = ad.proj;
+  a.dstride = ad.proj ? 2 * ad.ndens : ad.ndens;
+  a.olo = ad.order_lo;
+  a.ohi = ad.order_hi;
+  a.sel = ad.sel;
+  a.pair_gate = ad.pair_gate;
   return a;
 }
 
@@ -1550,7 +1659,8 @@ template <typename T>
 static void prod_lds(const AceDev& ad, size_t& tables, size_t& per_wave) {
   const size_t nslot = (size_t)ad.nsteps * kWave;
   tables = align16(align16(sizeof(T) * ad.nz * nslot) + 16 * nslot + 4 * (size_t)(ad.ndens + 1));
-  per_wave = align16(sizeof(T) * ad.ndens) + align16(sizeof(T) * ad.ntarget);
+  const int k = ad.proj ? 2 : 1;
+  per_wave = align16(sizeof(T) * k * ad.ndens) + align16(sizeof(T) * k * ad.ntarget);
 }
 
 template <typename T, int L, int SPW>
@@ -1561,7 +1671,9 @@ static void launch_sites(pfw_ctx* ctx, int grads) {
   AceArgs<T> a = make_args<T>(ctx, grads);
   constexpr int NY = (L + 1) * (L + 1);
   const unsigned blocks = (unsigned)((s.n + SPW - 1) / SPW);
-  const size_t lds_dens = sizeof(T) * ((size_t)NY + SPW * (size_t)ad.ndens + (size_t)(ad.rchunk + NY) * kStride);
+  const size_t dstride = ad.proj ? 2 * (size_t)ad.ndens : (size_t)ad.ndens;
+  const size_t lds_dens =
+      sizeof(T) * ((size_t)NY + SPW * dstride + (size_t)(ad.rchunk + (ad.proj ? 2 : 1) * NY) * kStride);
   allow_lds(k_ace_dens<T, L, SPW>, lds_dens, "the density kernel");
   hipLaunchKernelGGL((k_ace_dens<T, L, SPW>), dim3(blocks), dim3(kWave), lds_dens, st, a);
   PFW_HIP(hipGetLastError());
@@ -1584,7 +1696,7 @@ static void launch_sites(pfw_ctx* ctx, int grads) {
   }
   PFW_HIP(hipGetLastError());
   if (grads) {
-    const size_t lds_pull = sizeof(T) * ((size_t)NY + SPW * (size_t)ad.ndens);
+    const size_t lds_pull = sizeof(T) * ((size_t)NY + SPW * dstride);
     allow_lds(k_ace_pull<T, L, SPW>, lds_pull, "the pullback kernel");
     hipLaunchKernelGGL((k_ace_pull<T, L, SPW>), dim3(blocks), dim3(kWave), lds_pull, st, a);
     PFW_HIP(hipGetLastError());
@@ -1694,7 +1806,7 @@ static void ace_rows_alloc(pfw_ctx* ctx, int ecap) {
   if (!ad.ecount) {
     PFW_HIP(hipMalloc(&ad.ecount, n * sizeof(int32_t)));
     PFW_HIP(hipMalloc(&ad.siteE, n * sizeof(double)));
-    PFW_HIP(hipMalloc(&ad.Abuf, std::max<size_t>(1, (size_t)n * ad.ndens) * sizeof(double)));
+    PFW_HIP(hipMalloc(&ad.Abuf, std::max<size_t>(1, (size_t)n * ad.ndens * (ad.proj ? 2 : 1)) * sizeof(double)));
     PFW_HIP(hipMalloc(&ad.siteW, 9 * n * sizeof(double)));
     PFW_HIP(hipMalloc(&ad.rec, 10 * n * sizeof(double)));
     PFW_HIP(hipMalloc(&ad.status, 4 * sizeof(int32_t)));
@@ -2753,6 +2865,9 @@ void ace_adjoint(pfw_ctx* ctx, int precision, const double* Ebar, const double* 
   if (!ad.ready) throw Error(PFW_STALE, "no ACE model set");
   if (ad.cutoff != ctx->cutoff)
     throw Error(PFW_STALE, "the system's cell list was built for another cutoff than the ACE search cutoff");
+  if (ad.proj)
+    throw Error(PFW_INVALID_ARGUMENT,
+                "parameter adjoints of species-projected, order-restricted ACE bases are not implemented on the device");
   SystemBuffers& s = ctx->sys;
   hipStream_t st = ctx->stream;
   const int64_t n = s.n;

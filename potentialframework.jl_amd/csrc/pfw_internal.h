@@ -156,6 +156,11 @@ struct AceDev {
   bool ready = false;
   int nz = 0, lmax = 0, ndens = 0, nprod = 0, nbasis = 0, ny = 0;
   int solid = 0;  // real solid harmonics (r^l Y_lm) instead of spherical
+  // basis policy: species projection with orders order_lo..order_hi (proj), the
+  // projected neighbour species as bits of sel, the neighbour species whose
+  // pair term is kept as bits of pair_gate
+  int proj = 0, order_lo = -1, order_hi = -1;
+  uint32_t sel = 0, pair_gate = 0xffffffffu;
   double cutoff = 0;
   std::vector<int32_t> c_rows, c_cols;  // host copy of the coupling
   std::vector<double> c_vals;

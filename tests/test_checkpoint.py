@@ -102,6 +102,11 @@ def test_mask_policy_is_the_masked_readout(pfw):
     WBz = ps["WB"].copy()
     WBz[~mask] = 0
     assert e_masked == oracle_ace(f["X"], sp, f["H"], f["pbc"], dict(om, WB=WBz))[0]
-    with pytest.raises(pfw.ArgumentError):
-        pfw.load_resolved(dict(D, interface_policy={"version": 1, "mask": mask.tolist(), "species": [1],
-                                                    "orders": [0, 2], "pair": True, "reference": True}))
+    # a projected policy loads with its species (0-based channels) and orders
+    mp, _, _ = pfw.load_resolved(dict(D, interface_policy={"version": 1, "mask": mask.tolist(), "species": [1],
+                                                           "orders": [0, 2], "pair": True, "reference": True}))
+    assert mp.species == (0,) and mp.orders == (0, 2)
+    for bad in ({"species": [9], "orders": [0, 2]}, {"species": [1], "orders": [2, 1]},
+                {"species": [1, 1], "orders": [0, 1]}, {"species": [1], "orders": [-1, 1]}):
+        with pytest.raises(pfw.ArgumentError):
+            pfw.load_resolved(dict(D, interface_policy=dict({"version": 1, "mask": mask.tolist()}, **bad)))

@@ -238,3 +238,98 @@ def test_solid_harmonics_are_r_l_times_spherical():
                 g = np.zeros(3) if l == 0 else l * rn ** (l - 2) * R
                 np.testing.assert_allclose(Y[k], rn ** l * Ys[k], atol=2e-11, rtol=2e-12)
                 np.testing.assert_allclose(dY[k], rn ** l * dYs[k] + g * Ys[k], atol=2e-11, rtol=2e-12)
+
+
+# ---- species projection and order restriction (interface.jl:115-125,
+# projection.jl:7-38, serial.jl:76-79), pinned by the reference's own checks
+
+def test_projected_product_against_subset_enumeration():
+    """The 2^nu factor-assignment enumeration of test/ace_interface.jl:316-330:
+    the order-k part of prod(U + t V) sums the products taking k factors from
+    V. The reverse is pinned by central differences (exact for polynomials)."""
+    from oracle.ace import projected_product
+    products = [[0, 1], [0, 0, 1], [1], [0, 2, 0], [2, 1, 0, 2]]
+    for U in ([0., 2., -0.3], [1.2, -0.7, 0.2]):
+        for V in ([3., 0., 0.4], [0., -0.2, 1.1]):
+            for lo, hi in ((0, 0), (1, 1), (2, 2), (3, 3), (4, 4), (1, 3), (0, 4), (2, 9)):
+                for idx in products:
+                    expect = 0.0
+                    for bits in range(2 ** len(idx)):
+                        if lo <= bin(bits).count("1") <= hi:
+                            expect += np.prod([V[a] if (bits >> t) & 1 else U[a] for t, a in enumerate(idx)])
+                    P, dU, dV = projected_product(idx, U, V, lo, hi, w=0.7)
+                    assert P == pytest.approx(expect, abs=1e-13)
+                    h = 1e-3
+                    for a in range(3):
+                        for arr, grad in ((U, dU), (V, dV)):
+                            up, um = list(arr), list(arr)
+                            up[a] += h
+                            um[a] -= h
+                            args = (up, V) if arr is U else (U, up)
+                            argm = (um, V) if arr is U else (U, um)
+                            fd = (projected_product(idx, *args, lo, hi)[0] - projected_product(idx, *argm, lo, hi)[0]) / (2 * h)
+                            assert grad[a] == pytest.approx(0.7 * fd, abs=1e-9)
+
+
+def projected(model, species_channels, lo, hi):
+    return dict(model, species=tuple(species_channels), orders=(lo, hi))
+
+
+def tial_cell(tial):
+    """The reference's three-atom skew cell (test/ace_interface.jl:331-333)."""
+    H = np.array([[3.7, 0.6, 0.1], [0, 3.8, 0.2], [0, 0, 3.6]])
+    X = np.array([[0.1, 2.3, 1.1], [0.2, 0.7, 2.4], [0.3, 1.2, 1.5]]).T
+    return X, channels(tial, ["Ti", "Al", "Ti"]), H, (True, True, True)
+
+
+def test_projection_identities(tial):
+    """test/ace_interface.jl:64-79: the per-order projections k = 0..3 on Ti sum
+    to the model; Ti+Al over 0:3 and the empty set over 0:3 equal it; the
+    empty set over 1:3 has no forces; orders 9:10 give zero energy."""
+    frames = read_frames(os.path.join(GOLD, "reference_results.xyz"))
+    ti = tial["atomic_numbers"].index(22)
+    al = tial["atomic_numbers"].index(13)
+    for X, sp, H, pbc in [tial_cell(tial)] + [(f["X"], channels(tial, f["labels"]), f["H"], f["pbc"])
+                                              for f in (frames[0], frames[-1])]:
+        E, F, W, _ = ace_evaluate(X, sp, H, pbc, tial)
+        parts = [ace_evaluate(X, sp, H, pbc, projected(tial, [ti], k, k)) for k in range(4)]
+        scale = max(1.0, abs(E))
+        assert sum(p[0] for p in parts) == pytest.approx(E, abs=1e-9 * scale)
+        np.testing.assert_allclose(sum(p[1] for p in parts), F, atol=1e-9)
+        np.testing.assert_allclose(sum(p[2] for p in parts), W, atol=1e-9 * scale)
+        for m in (projected(tial, [ti, al], 0, 3), projected(tial, [], 0, 3)):
+            E2, F2, W2, _ = ace_evaluate(X, sp, H, pbc, m)
+            assert E2 == pytest.approx(E, abs=1e-9 * scale)
+            np.testing.assert_allclose(F2, F, atol=1e-9)
+        assert not np.any(ace_evaluate(X, sp, H, pbc, projected(tial, [], 1, 3))[1])
+        assert ace_evaluate(X, sp, H, pbc, projected(tial, [ti], 9, 10))[0] == 0.0
+        # at least one higher-order part is nonzero, so the split is exercised
+        assert any(abs(p[0]) > 1e-6 for p in parts[1:])
+
+
+def test_zero_order_projection_equals_excluded_edges(tial):
+    """test/ace_interface.jl:334-343: orders 0:0 on Ti keep only what involves
+    no Ti neighbour, i.e. the model with every edge to a Ti neighbour dropped."""
+    ti = tial["atomic_numbers"].index(22)
+    X, sp, H, pbc = tial_cell(tial)
+    z = ace_evaluate(X, sp, H, pbc, projected(tial, [ti], 0, 0))
+    x = ace_evaluate(X, sp, H, pbc, dict(tial, exclude=ti))
+    assert z[0] == pytest.approx(x[0], abs=1e-10)
+    np.testing.assert_allclose(z[1], x[1], atol=1e-10)
+    np.testing.assert_allclose(z[2], x[2], atol=1e-10)
+
+
+def test_projected_forces_by_finite_differences(tial):
+    ti = tial["atomic_numbers"].index(22)
+    X, sp, H, pbc = tial_cell(tial)
+    for lo, hi in ((1, 2), (2, 3)):
+        m = projected(tial, [ti], lo, hi)
+        E, F, W, _ = ace_evaluate(X, sp, H, pbc, m)
+        h = 1e-5
+        for i in range(3):
+            for a in range(3):
+                xp, xm = X.copy(), X.copy()
+                xp[i, a] += h
+                xm[i, a] -= h
+                fd = -(ace_evaluate(xp, sp, H, pbc, m, mask=1)[0] - ace_evaluate(xm, sp, H, pbc, m, mask=1)[0]) / (2 * h)
+                assert F[i, a] == pytest.approx(fd, abs=3e-6)
