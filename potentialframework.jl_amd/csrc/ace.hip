@@ -80,6 +80,7 @@ struct AceArgs {
   // kept when bit z of pair_gate is set
   int proj, dstride, olo, ohi;
   uint32_t sel, pair_gate;
+  int ostep[5];  // product schedule: steps [ostep[nu - 1], ostep[nu]) have nu factors
 };
 
 template <typename T>
@@ -448,7 +449,10 @@ __global__ __launch_bounds__(1024) void k_ace_prod(AceArgs<T> a, int grads) {
     sched = lsched;
     cb = lcb;
   }
-  const int nacc = a.proj ? 2 * a.ntarget : a.ntarget;  // with proj: dE/dU targets, then dE/dV targets
+  // accumulators: ntarget + 1 (the last one takes the padding slots' updates);
+  // with proj the dE/dV accumulators follow the dE/dU ones
+  const int nacc1 = a.ntarget + 1;
+  const int nacc = a.proj ? 2 * nacc1 : nacc1;
   const size_t per_wave = align16(sizeof(T) * a.dstride) + align16(sizeof(T) * nacc);
   T* A = (T*)(pw + wave * per_wave);
   T* acc = (T*)((unsigned char*)A + align16(sizeof(T) * a.dstride));
@@ -466,19 +470,17 @@ __global__ __launch_bounds__(1024) void k_ace_prod(AceArgs<T> a, int grads) {
     const int iz = a.sps[site];
     const T* wz = ws + iz * nslot;
     double eprod = 0;
-    for (int st = 0; st < a.nsteps; ++st) {
-      const int slot = st * kWave + lane;
-      const uint4 f = sched[slot];
-      const unsigned f0 = f.x & 0xffff, f1 = f.x >> 16, f2 = f.y & 0xffff, f3 = f.y >> 16;
-      if (f0 == kNone) continue;  // padding
-      const T w = wz[slot];
-      if (a.proj) {
+    if (a.proj) {
+      for (int st = 0; st < a.nsteps; ++st) {
+        const int slot = st * kWave + lane;
+        const uint4 f = sched[slot];
+        const T w = wz[slot];
         // the truncated product of projection.jl:7-38: the coefficients c_k of
         // prod_s (U_s + t V_s) over the (at most four) factors; P = sum of the
         // kept orders; d/dU_s and d/dV_s from the coefficients o_k of the
         // product of the other factors: dP/dU_s = sum om_k o_k,
         // dP/dV_s = sum om_{k+1} o_k (unused factors are 1 + 0 t)
-        const unsigned fs[4] = {f0, f1, f2, f3};
+        const unsigned fs[4] = {f.x & 0xffff, f.x >> 16, f.y & 0xffff, f.y >> 16};
         T u[4], v[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -516,23 +518,51 @@ __global__ __launch_bounds__(1024) void k_ace_prod(AceArgs<T> a, int grads) {
               dv += om[k + 1] * o[k];
             }
             acc[ts[q]] += w * du;
-            acc[a.ntarget + ts[q]] += w * dv;
+            acc[nacc1 + ts[q]] += w * dv;
           }
         }
-        continue;
       }
-      const T x0 = A[f0];
-      const T x1 = f1 != kNone ? A[f1] : (T)1;
-      const T x2 = f2 != kNone ? A[f2] : (T)1;
-      const T x3 = f3 != kNone ? A[f3] : (T)1;
-      const T p01 = x0 * x1, p23 = x2 * x3;
-      eprod += (double)(w * p01 * p23);
-      if (grads) {
-        acc[f.z & 0xffff] += w * x1 * p23;
-        if (f1 != kNone) acc[f.z >> 16] += w * x0 * p23;
-        if (f2 != kNone) acc[f.w & 0xffff] += w * p01 * x3;
-        if (f3 != kNone) acc[f.w >> 16] += w * p01 * x2;
-      }
+    } else {
+      // one order class at a time: every slot of a step has nu factors
+#pragma unroll
+      for (int nu = 1; nu <= 4; ++nu)
+        for (int st = a.ostep[nu - 1]; st < a.ostep[nu]; ++st) {
+          const int slot = st * kWave + lane;
+          const uint4 f = sched[slot];
+          const T w = wz[slot];
+          const T x0 = A[f.x & 0xffff];
+          if (nu == 1) {
+            eprod += (double)(w * x0);
+            if (grads) acc[f.z & 0xffff] += w;
+          } else if (nu == 2) {
+            const T x1 = A[f.x >> 16];
+            eprod += (double)(w * x0 * x1);
+            if (grads) {
+              acc[f.z & 0xffff] += w * x1;
+              acc[f.z >> 16] += w * x0;
+            }
+          } else if (nu == 3) {
+            const T x1 = A[f.x >> 16], x2 = A[f.y & 0xffff];
+            const T w2 = w * x2;
+            eprod += (double)(w2 * x0 * x1);
+            if (grads) {
+              acc[f.z & 0xffff] += w2 * x1;
+              acc[f.z >> 16] += w2 * x0;
+              acc[f.w & 0xffff] += w * x0 * x1;
+            }
+          } else {
+            const T x1 = A[f.x >> 16], x2 = A[f.y & 0xffff], x3 = A[f.y >> 16];
+            const T p01 = x0 * x1, p23 = x2 * x3;
+            eprod += (double)(w * p01 * p23);
+            if (grads) {
+              const T w23 = w * p23, w01 = w * p01;
+              acc[f.z & 0xffff] += w23 * x1;
+              acc[f.z >> 16] += w23 * x0;
+              acc[f.w & 0xffff] += w01 * x3;
+              acc[f.w >> 16] += w01 * x2;
+            }
+          }
+        }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -540,7 +570,7 @@ __global__ __launch_bounds__(1024) void k_ace_prod(AceArgs<T> a, int grads) {
     if (grads)
       for (int k = lane; k < a.dstride; k += kWave) {
         const int kd = k < a.ndens ? k : k - a.ndens;
-        const T* ac = k < a.ndens ? acc : acc + a.ntarget;
+        const T* ac = k < a.ndens ? acc : acc + nacc1;
         T d = 0;
         for (int c = cb[kd]; c < cb[kd + 1]; ++c) d += ac[c];
         Ag[k] = d;  // dE/dA (with proj: dE/dU, then dE/dV) over A
@@ -1310,8 +1340,17 @@ struct ProductSlot {
   int32_t t[4] = {-1, -1, -1, -1};
 };
 
+// Products are scheduled one order class at a time (all products of one
+// step have the same number of factors, so the kernel runs a body without
+// per-lane factor checks); steps [order_step[nu - 1], order_step[nu]) hold the
+// products of nu factors. Padding slots read density 0 and update the spare
+// accumulator ntarget (never read), with a zero readout weight.
+static std::vector<ProductSlot> schedule_class(const std::vector<std::vector<int>>& factors,
+                                               std::vector<int> remaining, const std::vector<int32_t>& copyb,
+                                               int ndens, int ntarget, int nu);
+
 static std::vector<ProductSlot> schedule_products(const std::vector<std::vector<int>>& factors, int ndens,
-                                                  std::vector<int32_t>& copyb) {
+                                                  std::vector<int32_t>& copyb, int* order_step) {
   const int nprod = (int)factors.size();
   const int ideal = std::max(1, (nprod + kWave - 1) / kWave);
   std::vector<int> occ(ndens, 0);
@@ -1319,8 +1358,24 @@ static std::vector<ProductSlot> schedule_products(const std::vector<std::vector<
     for (int d : f) ++occ[d];
   copyb.assign(ndens + 1, 0);
   for (int d = 0; d < ndens; ++d) copyb[d + 1] = copyb[d] + std::max(1, (occ[d] + ideal - 1) / ideal);
-  std::vector<int> remaining(nprod);
-  for (int k = 0; k < nprod; ++k) remaining[k] = k;
+  std::vector<ProductSlot> out;
+  order_step[0] = 0;
+  for (int nu = 1; nu <= 4; ++nu) {
+    std::vector<int> cls;
+    for (int k = 0; k < nprod; ++k)
+      if ((int)factors[k].size() == nu) cls.push_back(k);
+    if (!cls.empty()) {
+      std::vector<ProductSlot> part = schedule_class(factors, cls, copyb, ndens, copyb[ndens], nu);
+      out.insert(out.end(), part.begin(), part.end());
+    }
+    order_step[nu] = (int)(out.size() / kWave);
+  }
+  return out;
+}
+
+static std::vector<ProductSlot> schedule_class(const std::vector<std::vector<int>>& factors,
+                                               std::vector<int> remaining, const std::vector<int32_t>& copyb,
+                                               int ndens, int ntarget, int nu_class) {
   std::vector<ProductSlot> out;
   constexpr size_t kWindow = 512;
   std::vector<std::vector<int>> cnt(4, std::vector<int>(ndens));
@@ -1378,6 +1433,11 @@ static std::vector<ProductSlot> schedule_products(const std::vector<std::vector<
           tbank[half][t] |= 1u << (best.t[t] & 31);
         }
         taken[best_r] = 1;
+      } else {  // padding: density 0 into the spare accumulator
+        for (int t = 0; t < nu_class; ++t) {
+          best.f[t] = 0;
+          best.t[t] = ntarget;
+        }
       }
       out.push_back(best);
     }
@@ -1543,7 +1603,7 @@ void ace_set_model(pfw_ctx* ctx, const pfw_ace_model* m) {
   }
   a.prod = upload(prod, st);
   std::vector<int32_t> copyb;
-  std::vector<ProductSlot> slots = schedule_products(factors, m->ndens, copyb);
+  std::vector<ProductSlot> slots = schedule_products(factors, m->ndens, copyb, a.order_step);
   a.nsteps = (int)(slots.size() / kWave);
   a.ntarget = copyb[m->ndens];
   if (a.ntarget >= 0xffff) throw Error(PFW_INVALID_ARGUMENT, "product schedule needs too many accumulators");
@@ -1644,6 +1704,7 @@ static AceArgs<T> make_args(pfw_ctx* ctx, int grads) {
   a.ohi = ad.order_hi;
   a.sel = ad.sel;
   a.pair_gate = ad.pair_gate;
+  for (int k = 0; k < 5; ++k) a.ostep[k] = ad.order_step[k];
   return a;
 }
 
@@ -1660,7 +1721,7 @@ static void prod_lds(const AceDev& ad, size_t& tables, size_t& per_wave) {
   const size_t nslot = (size_t)ad.nsteps * kWave;
   tables = align16(align16(sizeof(T) * ad.nz * nslot) + 16 * nslot + 4 * (size_t)(ad.ndens + 1));
   const int k = ad.proj ? 2 : 1;
-  per_wave = align16(sizeof(T) * k * ad.ndens) + align16(sizeof(T) * k * ad.ntarget);
+  per_wave = align16(sizeof(T) * k * ad.ndens) + align16(sizeof(T) * k * (ad.ntarget + 1));
 }
 
 template <typename T, int L, int SPW>

@@ -7,6 +7,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <vector>
 
 #include "pfw_device.h"
 
@@ -470,16 +471,17 @@ int pfw_pair_adjoint(pfw_ctx* ctx, const double* dE, const double* dF, const dou
 }
 
 // Pinned host blocks for result arrays: a process-wide cache keyed by size
-// (a request takes the smallest cached block of at least its size and at most
-// twice it, else a new hipHostMalloc block); returned blocks are kept up to
-// kHostCacheBytes in total, beyond that freed. Blocks still cached at exit are
-// left to the process teardown.
+// (a request takes the smallest cached size of at least its own and at most
+// twice it, the most recently returned block of that size first, else a new
+// hipHostMalloc block); returned blocks are kept up to kHostCacheBytes in
+// total, beyond that freed. Blocks still cached at exit are left to the
+// process teardown.
 namespace {
 constexpr size_t kHostCacheBytes = size_t(2) << 30;
 struct HostPool {
   std::mutex mu;
-  std::multimap<size_t, void*> free;  // size -> block
-  std::map<void*, size_t> size_of;    // every live or cached block
+  std::map<size_t, std::vector<void*>> free;  // size -> blocks, most recent last
+  std::map<void*, size_t> size_of;            // every live or cached block
   size_t cached = 0;
 };
 HostPool& host_pool() {
@@ -496,9 +498,10 @@ void* pfw_host_alloc(size_t bytes) {
     std::lock_guard<std::mutex> g(P.mu);
     auto it = P.free.lower_bound(want);
     if (it != P.free.end() && it->first <= 2 * want) {
-      void* q = it->second;
+      void* q = it->second.back();
       P.cached -= it->first;
-      P.free.erase(it);
+      it->second.pop_back();
+      if (it->second.empty()) P.free.erase(it);
       return q;
     }
   }
@@ -521,7 +524,7 @@ void pfw_host_free(void* p) {
     return;
   }
   P.cached += it->second;
-  P.free.emplace(it->second, p);
+  P.free[it->second].push_back(p);
 }
 
 int pfw_synchronize(pfw_ctx* ctx) {

@@ -168,6 +168,7 @@ struct AceDev {
   AceFamilyDev radial, pair;
   int32_t* prod = nullptr;      // nprod x 4, -1 padded
   int nsteps = 0;               // product schedule: nsteps x 64 slots
+  int order_step[5] = {0, 0, 0, 0, 0};  // steps [order_step[nu - 1], order_step[nu]) hold products of nu factors
   uint4* sched = nullptr;       // per slot: 4 x u16 factors, 4 x u16 accumulator targets
   int32_t* copyb = nullptr;     // ndens + 1: accumulator copies of each density
   int ntarget = 0;
