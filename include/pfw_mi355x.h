@@ -191,7 +191,8 @@ int pfw_read_audit(pfw_ctx* ctx, int32_t* count, uint64_t* hash);
 /* Synchronise the context's stream. */
 int pfw_synchronize(pfw_ctx* ctx);
 
-/* Page-locked host memory for result arrays, from a process-wide cache: the
+/* Page-locked host memory for result arrays, from a process-wide cache (at
+ * most 256 MiB of returned blocks are kept; beyond that they are freed): the
  * library's device-to-host copies into such a block run as one DMA, where a
  * pageable destination is copied through the runtime's staging buffers. No
  * counterpart in the reference (Julia owns its arrays); for host bindings that

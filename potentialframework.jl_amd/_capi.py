@@ -110,22 +110,37 @@ def load():
     return lib
 
 
+# Page-locked bytes that live result arrays may hold at once; past this,
+# results are ordinary (pageable) arrays, so a caller that keeps every step's
+# forces cannot pin an unbounded amount of host memory.
+PINNED_LIVE_CAP = 256 << 20
+_pinned_live = [0]
+
+
+def _release_pinned(p, nbytes):
+    _pinned_live[0] -= nbytes
+    load().pfw_host_free(p)
+
+
 def pinned_empty(count, dtype=np.float64):
     """A flat array of ``count`` elements in a page-locked block of the
     library's host cache (pfw_host_alloc), handed back to the cache when the
-    last array viewing it is freed; an ordinary array when no block can be had.
-    The library's device-to-host result copies into it run as one DMA."""
+    last array viewing it is freed; an ordinary array when no block can be had
+    or when live page-locked results already hold PINNED_LIVE_CAP bytes. The
+    library's device-to-host result copies into a page-locked block run as one
+    DMA. Either way the array is C-contiguous and owns nothing else."""
     import weakref
     dt = np.dtype(dtype)
     nbytes = int(count) * dt.itemsize
-    if nbytes == 0:
+    if nbytes == 0 or _pinned_live[0] + nbytes > PINNED_LIVE_CAP:
         return np.empty(int(count), dt)
     lib = load()
     p = lib.pfw_host_alloc(nbytes)
     if not p:
         return np.empty(int(count), dt)
+    _pinned_live[0] += nbytes
     buf = (ctypes.c_char * nbytes).from_address(p)
-    weakref.finalize(buf, lib.pfw_host_free, p).atexit = False
+    weakref.finalize(buf, _release_pinned, p, nbytes).atexit = False
     return np.frombuffer(buf, dtype=dt, count=int(count))
 
 

@@ -477,7 +477,7 @@ int pfw_pair_adjoint(pfw_ctx* ctx, const double* dE, const double* dF, const dou
 // total, beyond that freed. Blocks still cached at exit are left to the
 // process teardown.
 namespace {
-constexpr size_t kHostCacheBytes = size_t(2) << 30;
+constexpr size_t kHostCacheBytes = size_t(256) << 20;
 struct HostPool {
   std::mutex mu;
   std::map<size_t, std::vector<void*>> free;  // size -> blocks, most recent last

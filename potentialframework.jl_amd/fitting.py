@@ -111,7 +111,15 @@ def training_rows(design, labels, weights=None):
                 raise DimensionMismatch("forces need three components per atom")
             v = np.concatenate([np.asarray(x, float) for x in v])
         else:
-            v = np.asarray(v, float).T.reshape(-1)  # vec() of a column-major 3 x n / 3 x 3 array
+            v = np.asarray(v, float)
+            # a 2-D force label must be the reference's 3 x n matrix: an (n, 3)
+            # array (one row per atom) would be laid out axis by axis here
+            if key == "forces" and v.ndim == 2 and v.shape[0] != 3:
+                raise DimensionMismatch("2-D force labels must be 3 x n (one column per atom); pass an (n, 3) "
+                                        "array as a list of three-vectors or transpose it")
+            if key == "virial" and v.ndim == 2 and v.shape != (3, 3):
+                raise DimensionMismatch("virial labels must be 3 x 3")
+            v = v.T.reshape(-1)  # vec() of a column-major 3 x n / 3 x 3 array
         if len(v) != len(rng):
             raise DimensionMismatch(f"{key} label dimensions")
         wt = wts[key]

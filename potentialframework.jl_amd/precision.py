@@ -3,8 +3,14 @@ structures: ``compare_precision`` and ``PrecisionReport``
 (src/submodules/ACE/precision.jl:1-121).
 
 Both versions run on the device from the same model and parameters: the FP64
-kernels and the mixed kernels (FP32 per-edge and per-site arithmetic, FP64
-reductions). Per structure and property the report holds the maximum and RMS
+kernels on the structure as given, and the mixed kernels (FP32 per-edge and
+per-site arithmetic, FP64 reductions) on the structure with its positions and
+cell rounded to Float32 first, as the reference's Float32 leg receives them
+(precision_geometry, src/submodules/ACE/adaptation.jl:122-127). The low-precision
+leg therefore carries the geometry rounding of the reference's Float32 path;
+its arithmetic is the device's mixed path, not a pure Float32 evaluation (the
+neighbour search and the sums stay FP64), which the metadata states.
+Per structure and property the report holds the maximum and RMS
 absolute component error, where it occurs, and (with tolerances) a pass/fail
 status; the aggregate pools all components. A failure (preparation, an error
 in either evaluation, nonfinite output) is recorded with its message, makes
@@ -53,6 +59,15 @@ def _device_evaluators(model, ps, props):
     return out
 
 
+def precision_geometry(sys_):
+    """The structure with positions and cell rounded to Float32 (held as FP64
+    values), species and periodicity unchanged (adaptation.jl:122-127)."""
+    from .system import System
+    X = np.asarray(sys_.positions, float).astype(np.float32).astype(np.float64)
+    H = np.asarray(sys_.cell, float).astype(np.float32).astype(np.float64)
+    return System(X, H, sys_.pbc, sys_.labels)
+
+
 def _precision_of(a):
     a = np.asarray(a)
     return "Float32" if a.dtype == np.float32 else "Float64"
@@ -86,7 +101,7 @@ def compare_precision(model, ps, systems, properties=PROPERTIES, tolerances=None
             try:
                 if isinstance(f, str):
                     raise RuntimeError("preparation failed: " + f)
-                r = f(sys_)
+                r = f(sys_ if name == "Float64" else precision_geometry(sys_))
                 if not all(np.all(np.isfinite(np.asarray(r[p], float))) for p in props):
                     raise RuntimeError("nonfinite output")
                 results.append(r)
@@ -167,6 +182,8 @@ def compare_precision(model, ps, systems, properties=PROPERTIES, tolerances=None
     metadata = dict(source_precision=_precision_of(WB) if WB is not None else "Float64",
                     parameter_precision=_precision_of(WB) if WB is not None else "Float64",
                     stored_precision="Float64", storage="device (MI355X HBM)",
+                    low_precision_leg=("mixed device kernels (FP32 per-edge and per-site arithmetic, FP64 neighbour "
+                                       "search and reductions) on positions and cell rounded to Float32"),
                     model_sha256=getattr(model, "source_sha256", None), properties=props, tolerances=tolerances,
                     preparation_errors=tuple(v if isinstance(v, str) else None for v in (ev["Float64"], ev["Float32"])),
                     units=dict(energy="eV", forces="eV/Å", virial="eV"),

@@ -43,6 +43,12 @@ def test_training_rows(pfw):
     assert np.allclose(t2["labels"], F.T.reshape(-1) - base[1:7])
     with pytest.raises(pfw.DimensionMismatch):
         pfw.training_rows(design, {"forces": np.zeros((3, 3))})
+    # an (n, 3) array (one row per atom, as calculators.forces returns) has the
+    # right length but the wrong layout: rejected, not silently reordered
+    with pytest.raises(pfw.DimensionMismatch):
+        pfw.training_rows(design, {"forces": F.T.copy()})
+    t3 = pfw.training_rows(design, {"forces": list(F.T)})
+    assert np.allclose(t3["labels"], F.T.reshape(-1) - base[1:7])
     with pytest.raises(pfw.DimensionMismatch):
         pfw.training_rows(design, {"forces": [np.zeros(2), np.zeros(2)]})
     with pytest.raises(pfw.ArgumentError):

@@ -118,3 +118,25 @@ def test_results_own_their_storage(pfw):
     gc.collect()
     b = _capi.pinned_empty(1000)
     assert b.ctypes.data == addr
+
+
+def test_pinned_results_are_capped(pfw, monkeypatch):
+    """Live page-locked result arrays are capped (PINNED_LIVE_CAP); past the cap
+    results are ordinary arrays, and freeing a page-locked one releases its share."""
+    import gc
+    from pfw_amd import _capi
+    gc.collect()
+    base = _capi._pinned_live[0]
+    monkeypatch.setattr(_capi, "PINNED_LIVE_CAP", base + 2 * 65536)
+    a = _capi.pinned_empty(8192)  # 64 KiB each
+    b = _capi.pinned_empty(8192)
+    assert _capi._pinned_live[0] == base + 2 * 65536
+    c = _capi.pinned_empty(8192)  # over the cap: pageable
+    assert _capi._pinned_live[0] == base + 2 * 65536 and c.flags.owndata
+    a[:] = 1.0
+    b[:] = 2.0
+    c[:] = 3.0
+    del a
+    gc.collect()
+    assert _capi._pinned_live[0] == base + 65536
+    assert np.all(b == 2.0) and np.all(c == 3.0)

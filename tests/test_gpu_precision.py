@@ -1,9 +1,10 @@
 """compare_precision on the device (src/submodules/ACE/precision.jl:25-121) with
 the cases of test/ace_adaptation.jl:62-90 on the TiAl resolved model: a
 two-atom periodic structure (descriptive report, six force components, finite
-pooled RMS, the maximum equal to the FP64-vs-mixed difference of direct
-evaluations), an empty collection, zero tolerances (failed), and a NaN readout
-coefficient (failed, infinite pooled RMS)."""
+pooled RMS, the maximum equal to the difference of direct evaluations: FP64
+on the structure, mixed on its Float32-rounded geometry), an empty collection,
+zero tolerances (failed), and a NaN readout coefficient (failed, infinite
+pooled RMS)."""
 import math
 import os
 
@@ -24,7 +25,7 @@ def test_compare_precision_device(pfw):
     assert len(r.structures) == 1 and r.aggregate["status"] == "descriptive"
     assert r.aggregate["forces"]["components"] == 6 and math.isfinite(r.aggregate["forces"]["pooled_rms"])
     f64 = pfw.ACECalculator(model, ps, precision="float64").evaluate(s)
-    f32 = pfw.ACECalculator(model, ps, precision="mixed").evaluate(s)
+    f32 = pfw.ACECalculator(model, ps, precision="mixed").evaluate(pfw.precision.precision_geometry(s))
     assert r.aggregate["forces"]["maximum"] == pytest.approx(np.abs(f32["forces"] - f64["forces"]).max(), rel=1e-12,
                                                              abs=1e-300)
     assert r.aggregate["forces"]["maximum"] < 1e-4

@@ -88,3 +88,30 @@ def test_tolerances_and_failures(pfw):
         pfw.compare_precision(None, {}, [s], properties=("energy", "stress"), evaluators=fake())
     only = pfw.compare_precision(None, {}, [s], properties=("forces",), evaluators=fake())
     assert set(only.aggregate) == {"status", "forces"}
+
+
+def test_low_precision_leg_receives_float32_geometry(pfw):
+    """The Float32 leg evaluates the structure with positions and cell rounded to
+    Float32 (precision_geometry, adaptation.jl:122-127); the Float64 leg the
+    structure as given. Stand-in evaluators record what they receive."""
+    seen = {}
+
+    def record(name):
+        def f(s):
+            seen[name] = (np.array(s.positions), np.array(s.cell), tuple(s.pbc), list(s.labels))
+            return {"energy": 0.0, "forces": np.zeros((3, len(s))), "virial": np.zeros((3, 3))}
+        return f
+
+    X = np.array([[0.1, 0.2, 0.3], [1.0 / 3.0, 2.0 / 3.0, np.pi]])
+    H = np.array([[5.0 + 1e-9, 0.1, 0.0], [0.0, 5.3, 0.0], [0.0, 0.0, np.e]])
+    s = pfw.System(X, H, (1, 0, 1), ["Ti", "Al"])
+    r = pfw.compare_precision(None, {}, [s], evaluators={"Float64": record("Float64"), "Float32": record("Float32")})
+    X64, H64, pb64, lab64 = seen["Float64"]
+    X32, H32, pb32, lab32 = seen["Float32"]
+    assert np.array_equal(X64, X) and np.array_equal(H64, H)
+    assert np.array_equal(X32, X.astype(np.float32).astype(np.float64))
+    assert np.array_equal(H32, H.astype(np.float32).astype(np.float64))
+    assert not np.array_equal(X32, X)
+    assert pb32 == pb64 == (True, False, True) and lab32 == lab64 == ["Ti", "Al"]
+    assert "rounded to Float32" in r.metadata["low_precision_leg"]
+    assert r.structures[0]["position_rounding_max_A"] == pytest.approx(np.abs(X32 - X).max())
