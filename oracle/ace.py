@@ -221,8 +221,18 @@ lib.oracle_ace_evaluate.argtypes = [ctypes.c_int64, ctypes.c_void_p, ctypes.c_vo
 lib.oracle_ace_evaluate.restype = ctypes.c_int
 
 
-def ace_evaluate(X, species, H, pbc, model, threads=1, mask=7):
-    """FP64 (E, F (n,3), W (3x3), site energies (n)) of the resolved model."""
+lib.oracle_ace_evaluate_scheduled.argtypes = [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                              ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(_Model), ctypes.c_int,
+                                              ctypes.c_int, ctypes.c_uint, ctypes.c_void_p, ctypes.c_void_p,
+                                              ctypes.c_void_p, ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]
+lib.oracle_ace_evaluate_scheduled.restype = ctypes.c_int
+
+
+def ace_evaluate(X, species, H, pbc, model, threads=1, mask=7, coloured=False):
+    """FP64 (E, F (n,3), W (3x3), site energies (n)) of the resolved model. With
+    threads > 1 and ``coloured`` the sites run phase by phase over the greedy
+    footprint colouring with one shared force array (execute_sites!);
+    otherwise each thread keeps private buffers."""
     X = np.ascontiguousarray(np.asarray(X, float).reshape(-1, 3))
     n = len(X)
     sp = np.ascontiguousarray(np.asarray(species, np.int32))
@@ -234,9 +244,10 @@ def ace_evaluate(X, species, H, pbc, model, threads=1, mask=7):
     W = np.zeros(9)
     site = np.zeros(n)
     err = ctypes.create_string_buffer(512)
-    rc = lib.oracle_ace_evaluate(n, X.ctypes.data, sp.ctypes.data, Hc.ctypes.data, pb.ctypes.data,
-                                 float(model["cutoff"]), ctypes.byref(m), int(threads), int(mask), E.ctypes.data,
-                                 F.ctypes.data, W.ctypes.data, site.ctypes.data, err, 512)
+    rc = lib.oracle_ace_evaluate_scheduled(n, X.ctypes.data, sp.ctypes.data, Hc.ctypes.data, pb.ctypes.data,
+                                           float(model["cutoff"]), ctypes.byref(m), int(threads), int(bool(coloured)),
+                                           int(mask), E.ctypes.data, F.ctypes.data, W.ctypes.data,
+                                           site.ctypes.data, err, 512)
     del keep
     if rc != 0:
         raise OracleError(rc, err.value.decode())

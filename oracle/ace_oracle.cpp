@@ -11,8 +11,10 @@
 // Environments are the directed snapshot of the neighbour-list oracle grouped
 // by neighbour species in traversal order (src/submodules/NeighbourList/environments.jl:42-102).
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -350,13 +352,55 @@ static int ace_fail(char* err, int errlen, const char* msg, int code) {
 
 extern "C" {
 
+// Greedy site colouring of color_environment! (NeighbourList/environments.jl:78-95):
+// a site's footprint is itself and every neighbour it writes; each site takes
+// the smallest colour not yet used by a site whose footprint shares an atom,
+// so the sites of one colour (phase) write disjoint force entries.
+static std::vector<std::vector<int64_t>> colour_sites(int64_t n, const std::vector<std::vector<Edge>>& env) {
+  std::vector<std::vector<int>> incidence(n);  // colours touching each atom
+  std::vector<std::vector<int64_t>> phases;
+  std::vector<int64_t> fp;
+  std::vector<char> forbidden;
+  for (int64_t i = 0; i < n; ++i) {
+    fp.assign(1, i);
+    for (const Edge& e : env[i]) fp.push_back(e.j);
+    std::sort(fp.begin(), fp.end());
+    fp.erase(std::unique(fp.begin(), fp.end()), fp.end());
+    forbidden.assign(phases.size() + 1, 0);
+    for (int64_t a : fp)
+      for (int c : incidence[a]) forbidden[c] = 1;
+    int c = 0;
+    while (forbidden[c]) ++c;
+    if (c == (int)phases.size()) phases.emplace_back();
+    phases[c].push_back(i);
+    for (int64_t a : fp) incidence[a].push_back(c);
+  }
+  return phases;
+}
+
 // E, F (3 x n original order), W (3 x 3 column-major), site energies (n) of a
 // resolved ACE model; cutoff is the model's search cutoff. threads > 1 splits
-// sites over threads with private force/virial buffers. Returns 0, 1
-// (ArgumentError) or 3 (DomainError).
+// sites over threads: with coloured = 0 each thread keeps private force and
+// virial buffers; with coloured = 1 the sites run phase by phase over the
+// greedy colouring (execute_sites!, NeighbourList/execution.jl:153-167), the
+// threads of a phase taking every threads-th site and writing one shared
+// force array. Returns 0, 1 (ArgumentError) or 3 (DomainError).
+int oracle_ace_evaluate_scheduled(int64_t n, const double* X, const int32_t* species, const double* H,
+                                  const uint8_t* pbc, double cutoff, const OracleAceModel* m, int threads,
+                                  int coloured, unsigned mask, double* E, double* F, double* W, double* siteE,
+                                  char* err, int errlen);
+
 int oracle_ace_evaluate(int64_t n, const double* X, const int32_t* species, const double* H, const uint8_t* pbc,
                         double cutoff, const OracleAceModel* m, int threads, unsigned mask, double* E, double* F,
                         double* W, double* siteE, char* err, int errlen) {
+  return oracle_ace_evaluate_scheduled(n, X, species, H, pbc, cutoff, m, threads, 0, mask, E, F, W, siteE, err,
+                                       errlen);
+}
+
+int oracle_ace_evaluate_scheduled(int64_t n, const double* X, const int32_t* species, const double* H,
+                                  const uint8_t* pbc, double cutoff, const OracleAceModel* m, int threads,
+                                  int coloured, unsigned mask, double* E, double* F, double* W, double* siteE,
+                                  char* err, int errlen) {
   try {
     bool pb[3] = {pbc[0] != 0, pbc[1] != 0, pbc[2] != 0};
     Prepared w;
@@ -373,6 +417,58 @@ int oracle_ace_evaluate(int64_t n, const double* X, const int32_t* species, cons
       std::stable_sort(e.begin(), e.end(), [](const Edge& a, const Edge& b) { return a.zj < b.zj; });
     bool grads = (mask & 6) != 0;
     int T = std::max(1, threads);
+    if (coloured && T > 1) {
+      const std::vector<std::vector<int64_t>> phases = colour_sites(n, env);
+      std::vector<double> Fs(grads ? 3 * n : 0, 0.0);
+      std::vector<std::array<double, 9>> Wb(T);
+      for (auto& w : Wb) w.fill(0.0);
+      std::vector<double> es(n, 0.0);
+      std::vector<std::string> errs(T);
+      for (const auto& phase : phases) {
+        auto work = [&](int t) {
+          try {
+            SiteOut so;
+            for (size_t k = t; k < phase.size(); k += T) {
+              const int64_t i = phase[k];
+              site(*m, species[i], env[i], grads, so);
+              es[i] = so.energy;
+              if (!grads) continue;
+              for (size_t e = 0; e < env[i].size(); ++e) {
+                const double* g = &so.g[3 * e];
+                const int64_t j = env[i][e].j;
+                for (int a = 0; a < 3; ++a) {
+                  Fs[3 * i + a] += g[a];
+                  Fs[3 * j + a] -= g[a];
+                  for (int b = 0; b < 3; ++b) Wb[t][a + 3 * b] -= env[i][e].R[a] * g[b];
+                }
+              }
+            }
+          } catch (const std::exception& e) {
+            errs[t] = e.what();
+          }
+        };
+        std::vector<std::thread> pool;
+        for (int t = 0; t < T; ++t) pool.emplace_back(work, t);
+        for (auto& th : pool) th.join();
+        for (auto& s_ : errs)
+          if (!s_.empty()) throw DomainError(s_);
+      }
+      double e = 0;
+      for (double v : es) e += v;
+      if (E) *E = (mask & 1) ? e : 0.0;
+      if (siteE)
+        for (int64_t i = 0; i < n; ++i) siteE[i] = es[i];
+      if (F)
+        for (int64_t k = 0; k < 3 * n; ++k) F[k] = (mask & 2) ? Fs[k] : 0.0;
+      if (W)
+        for (int k = 0; k < 9; ++k) {
+          double s_ = 0;
+          if (mask & 4)
+            for (int t = 0; t < T; ++t) s_ += Wb[t][k];
+          W[k] = s_;
+        }
+      return 0;
+    }
     std::vector<std::vector<double>> Fb(T, std::vector<double>(grads ? 3 * n : 0, 0.0));
     std::vector<std::array<double, 9>> Wb(T);
     std::vector<double> es(n, 0.0);

@@ -333,3 +333,17 @@ def test_projected_forces_by_finite_differences(tial):
                 xm[i, a] -= h
                 fd = -(ace_evaluate(xp, sp, H, pbc, m, mask=1)[0] - ace_evaluate(xm, sp, H, pbc, m, mask=1)[0]) / (2 * h)
                 assert F[i, a] == pytest.approx(fd, abs=3e-6)
+
+
+def test_coloured_site_execution_matches_serial(synthetic):
+    """The coloured phase executor (execute_sites!, execution.jl:153-167, with
+    color_environment!'s greedy footprint colouring) used as the threaded CPU
+    baseline gives the serial result to rounding."""
+    _, m = synthetic
+    X, H, pbc, sp = small_bcc(seed=9)
+    E, F, W, site = ace_evaluate(X, sp, H, pbc, m)
+    Ec, Fc, Wc, sitec = ace_evaluate(X, sp, H, pbc, m, threads=4, coloured=True)
+    assert Ec == pytest.approx(E, abs=1e-10 * abs(E))
+    np.testing.assert_allclose(Fc, F, atol=1e-11)
+    np.testing.assert_allclose(Wc, W, atol=1e-10 * abs(E))
+    np.testing.assert_array_equal(sitec, site)
