@@ -8,9 +8,13 @@ throughput in Matom-evals/s. Reported beside it: the plain-FP64 throughput of
 the same step, the 32,768-atom ACE workload (order-3, lmax-4, degree-12
 synthetic basis on a BCC binary, cutoff 5.0 Å) in both precisions, per-stage
 device times, the roofline of each dominant kernel, accuracy against the FP64
-oracle, and the CPU baseline (the oracle's threaded FP64 restatement). The
-1,024-configuration fitting set follows: batched pair and ACE steps and the
-parameter adjoints of a least-squares loss over it.
+oracle, a bit-exact audit of the neighbour sets the evaluating kernels
+accumulated, and the CPU baseline (the oracle's threaded FP64 restatement, on
+all physical cores of the process's affinity up to 16 and on one core, with
+the host described). The 4,000-atom argon Lennard-Jones cell, the
+1,024-configuration fitting set (batched pair and ACE steps and the parameter
+adjoints of a least-squares loss over it), the fitted May30 model, the
+equivariant features and the Verlet-reuse trajectories follow.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-ace] [--no-batch]
 """
@@ -19,6 +23,7 @@ import ctypes
 import importlib.util
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -30,6 +35,10 @@ PKG_DIR = os.path.join(ROOT, "potentialframework.jl_amd")
 METRIC = "Matom-evals/s (E+F+V), 256k-atom pair & 32k-atom ACE; mixed vs FP64 at tol"
 PEAK_FP32_TFLOPS = 157.3  # MI355X published FP32 (vector/matrix) peak
 PEAK_FP64_TFLOPS = 78.6
+# FP32 VALU issue rate without packed (two-wide) FP32, which these kernels are
+# built without: one FMA per lane per cycle, the FP64 vector rate
+PEAK_FP32_UNPACKED_TFLOPS = 78.6
+CPU_THREAD_CAP = 16  # the CPU share of one GPU on the measurement host
 PEAK_HBM_GBS = 8000.0  # MI355X HBM3E
 STAGES = ("binning", "scan", "scatter_sort", "environments", "interaction", "reduction")
 # Algorithmic flops per directed in-cutoff pair (FMA = 2): d + shift 6, r^2 5,
@@ -98,22 +107,129 @@ def read_traffic(name):
         return None
 
 
-def cpu_baseline(fn, budget=20.0, reps=7):
-    """Median wall time of the oracle's threaded FP64 restatement (bounded budget)."""
-    fn()  # warm-up
-    times, out = [], None
+def host_info():
+    """The CPU the baseline ran on: model, sockets, SMT, the logical CPUs and
+    physical cores of this process's affinity mask."""
+    info = {"model": None, "sockets": None, "threads_per_core": None}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        kv = {}
+        for line in out.splitlines():
+            if ":" in line:
+                k, v = line.split(":", 1)
+                kv[k.strip()] = v.strip()
+        info["model"] = kv.get("Model name")
+        info["sockets"] = int(kv["Socket(s)"]) if kv.get("Socket(s)", "").isdigit() else None
+        tpc = kv.get("Thread(s) per core", "")
+        info["threads_per_core"] = int(tpc) if tpc.isdigit() else None
+    except Exception:
+        pass
+    if info["model"] is None:
+        try:
+            with open("/proc/cpuinfo") as f:
+                info["model"] = next(l.split(":", 1)[1].strip() for l in f if l.startswith("model name"))
+        except Exception:
+            pass
+    aff = sorted(os.sched_getaffinity(0))
+    cores = set()
+    for c in aff:
+        try:
+            base = f"/sys/devices/system/cpu/cpu{c}/topology/"
+            with open(base + "physical_package_id") as f1, open(base + "core_id") as f2:
+                cores.add((f1.read().strip(), f2.read().strip()))
+        except OSError:
+            cores.add(("?", str(c)))
+    ranges, start = [], None
+    for k, c in enumerate(aff):
+        if start is None:
+            start = c
+        if k + 1 == len(aff) or aff[k + 1] != c + 1:
+            ranges.append(f"{start}-{c}" if c > start else str(start))
+            start = None
+    info.update(affinity_cpus=len(aff), affinity_physical_cores=len(cores), affinity=",".join(ranges),
+                smt_in_affinity=len(aff) > len(cores), pinning="none (threads float within the affinity mask)")
+    return info
+
+
+def baseline_threads(host):
+    return max(1, min(CPU_THREAD_CAP, host["affinity_physical_cores"]))
+
+
+def cpu_timing(fn, warmups=3, samples=7, budget=20.0):
+    """Wall times of a CPU evaluation: warm-ups, then up to `samples` timed
+    calls while the budget (seconds, warm-ups included) lasts; at least one."""
     t_start = time.perf_counter()
-    while len(times) < reps and (time.perf_counter() - t_start) < budget:
+    out = None
+    for _ in range(warmups):
+        out = fn()
+        if time.perf_counter() - t_start > budget / 3:
+            break
+    times = []
+    while len(times) < samples and (not times or time.perf_counter() - t_start < budget):
         t0 = time.perf_counter()
         out = fn()
         times.append(time.perf_counter() - t0)
-    return float(np.median(times)), len(times), out
+    return np.array(times), out
+
+
+def cpu_baseline(make, atoms, what, threads, host, all_kind="", single_kind=""):
+    """The oracle's FP64 restatement timed on `threads` cores (3 warm-ups,
+    median of 7) and on one core (1 warm-up, median of up to 3 within 20 s).
+    `make(t)` returns a callable evaluating with t threads. Returns the
+    cpu_baseline object and the all-core result."""
+    times, out = cpu_timing(make(threads))
+    one, _ = cpu_timing(make(1), warmups=1, samples=3, budget=20.0)
+    med, med1 = float(np.median(times)), float(np.median(one))
+    return {"value": round(atoms / med / 1e6, 5), "unit": "Matom-evals/s", "cores": threads, "kind": "port",
+            "sample": f"{what}; {threads} threads{all_kind}: median of {len(times)} after 3 warm-ups",
+            "median_ms": round(1e3 * med, 3), "min_ms": round(1e3 * float(times.min()), 3),
+            "max_ms": round(1e3 * float(times.max()), 3),
+            "single_core": {"value": round(atoms / med1 / 1e6, 5), "median_ms": round(1e3 * med1, 3),
+                            "samples": len(one), "schedule": single_kind or "serial"},
+            "host": host}, out
+
+
+def neighbour_audit(pfw, w, run, X, H, pbc, rc, delta=0.05):
+    """One audited step: per-atom count and digest of the directed pairs (j, S)
+    the evaluating kernel accumulated, compared bit-exactly with the oracle's
+    neighbour list; and the smallest | |R| - rc | over all images within
+    rc + delta (how close the workload comes to the membership boundary)."""
+    from oracle.core import nl_pairs
+
+    n = len(X)
+    w.set_audit(True)
+    run()
+    got = w.read_audit(n)
+    w.set_audit(False)
+    i, j, S, _, _ = nl_pairs(X, H, pbc, rc)
+    ref = pfw.pair_set_digest(n, i, j, S)
+    bad = int(np.count_nonzero((got[0] != ref[0]) | (got[1] != ref[1])))
+    _, _, _, R, _ = nl_pairs(X, H, pbc, rc + delta)
+    margin = float(np.abs(np.sqrt((R * R).sum(1)) - rc).min()) if len(R) else None
+    return {"exact": bad == 0, "atoms_differing": bad, "directed_pairs": int(got[0].sum()),
+            "min_abs_r_minus_rc_A": margin}
 
 
 def accuracy(res, E, F, W, n):
     d = res["forces"].T - F
     return {"force_max": float(np.abs(d).max()), "force_rms": float(np.sqrt(np.mean(d * d))),
             "energy_per_atom": float(abs(res["energy"] - E) / n), "virial_max": float(np.abs(res["virial"] - W).max())}
+
+
+def pooled_accuracy(pairs):
+    """Accuracy over several structures: maxima, the force RMS pooled over all
+    components, and the largest per-structure energy-per-atom error."""
+    fmax = emax = wmax = 0.0
+    ss = cnt = 0
+    for res, (E, F, W) in pairs:
+        d = res["forces"].T - F
+        fmax = max(fmax, float(np.abs(d).max()) if d.size else 0.0)
+        ss += float((d * d).sum())
+        cnt += d.size
+        emax = max(emax, abs(res["energy"] - E) / max(1, F.shape[0]))
+        wmax = max(wmax, float(np.abs(res["virial"] - W).max()))
+    return {"force_max": fmax, "force_rms": float(np.sqrt(ss / max(1, cnt))), "energy_per_atom": emax,
+            "virial_max": wmax, "structures": len(pairs)}
 
 
 def bench_fitting_set(pfw, lib, _capi, model, ps, pair_model, args):
@@ -133,17 +249,33 @@ def bench_fitting_set(pfw, lib, _capi, model, ps, pair_model, args):
     K, Wm = args.batch_steps, max(1, args.warmup // 2)
     out = {"workload": "fitting_set_1024", "configurations": len(cfgs), "atoms": natoms,
            "placement_attempts": attempts, "generation_s": round(gen_s, 3), "unit": "Matom-evals/s"}
+    results = {}
     for prec, mode in (("mixed", _capi.PFW_MIXED), ("float64", _capi.PFW_FP64)):
         w = pfw.Workspace(precision=prec)
-        pfw.evaluate_batch(w, sys_pair, pair_model)
+        results[("pair", prec)] = pfw.evaluate_batch(w, sys_pair, pair_model)
         secs, stage = time_steps(lib, w.ctx.h, lib.pfw_pair_step, K, Wm, mode)
         out[f"pair_{prec}"] = {"value": round(natoms * K / secs / 1e6, 3), "ms_per_step": round(1e3 * secs / K, 4),
                                "stages_ms": {k: round(float(v), 5) for k, v in zip(STAGES, stage)}}
         calc = pfw.ACECalculator(model, ps, precision=prec)
-        calc.evaluate_batch(sys_ace)
+        results[("ace", prec)] = calc.evaluate_batch(sys_ace)
         secs, stage = time_steps(lib, calc.workspace.ctx.h, lib.pfw_ace_step, K, Wm, mode)
         out[f"ace_{prec}"] = {"value": round(natoms * K / secs / 1e6, 4), "ms_per_step": round(1e3 * secs / K, 4),
                               "stages_ms": {k: round(float(v), 5) for k, v in zip(STAGES, stage)}}
+    if not args.no_cpu_baseline:
+        # every 21st configuration (49 of 1,024) against the oracle, one at a time
+        from oracle.ace import ace_evaluate as oracle_ace
+        from oracle.ace import load_resolved as oracle_load
+        from oracle.core import KIND_SMOOTH, pair_evaluate
+
+        om = oracle_load(pfw.ace_basis.synthetic_model())
+        sample = list(range(0, len(cfgs), 21))
+        ref_pair = [pair_evaluate(cfgs[c][0], cfgs[c][3], cfgs[c][1], cfgs[c][2], KIND_SMOOTH,
+                                  pair_model.amplitudes, pair_model.cutoff()) for c in sample]
+        ref_ace = [oracle_ace(cfgs[c][0], cfgs[c][3], cfgs[c][1], cfgs[c][2], om)[:3] for c in sample]
+        out["accuracy_vs_fp64_oracle"] = {
+            "sample": f"{len(sample)} configurations (every 21st), {sum(len(cfgs[c][0]) for c in sample)} atoms",
+            **{f"{kind}_{prec}": pooled_accuracy([(results[(kind, prec)][c], ref[k]) for k, c in enumerate(sample)])
+               for kind, ref in (("pair", ref_pair), ("ace", ref_ace)) for prec in ("mixed", "float64")}}
     # adjoints: loss and gradient with respect to all ACE readout coefficients
     rng = np.random.default_rng(5)
     WBt = model.WB * (1 + 0.01 * rng.normal(size=model.WB.shape))
@@ -214,12 +346,15 @@ def bench_fitting_set(pfw, lib, _capi, model, ps, pair_model, args):
     return out
 
 
-def bench_verlet(pfw, lib, _capi, X, H, pbc, sp, model, labels, skin, prec, mode, steps=40, amp=0.005, ace=False):
+def bench_verlet(pfw, lib, _capi, X, H, pbc, sp, model, labels, skin, prec, mode, steps=40, amp=0.005, ace=False,
+                 check=None):
     """The reference's evolving-trajectory measurement (benchmark/gpu_pairs.jl:118-142):
     raw coordinates x_s = x_0 + s v with v[a, i] = amp sin(a + 0.73 i) (1-based a, i),
     40 steps on a list at cutoff + skin built at x_0; each step's positions are
     uploaded outside the timed region and every step is timed from the call to
-    its synchronised status read. Also the stage times of a reused step."""
+    its synchronised status read. Also the stage times of a reused step. With
+    `check(X) -> (E, F, W)` (the FP64 oracle), one more step at the last
+    positions, on the kept list, is audited and compared with it."""
     n = len(X)
     i1 = np.arange(1, n + 1)[:, None]
     a1 = np.arange(1, 4)[None, :]
@@ -261,7 +396,21 @@ def bench_verlet(pfw, lib, _capi, X, H, pbc, sp, model, labels, skin, prec, mode
         stage += buf
     _capi.check(h, lib.pfw_set_timing(h, 0))
     stage /= 10
+    checked = {}
+    if check is not None:
+        sys_s = pfw.System(Xs, H, pbc, labels)
+        before = w.rebuilds
+        held = {}
+
+        def run():
+            held["r"] = calc.evaluate(sys_s) if ace else pfw.evaluate_(w, sys_s, model)
+
+        audit = neighbour_audit(pfw, w, run, Xs, H, pbc, calc.cutoff() if ace else model.cutoff())
+        audit["list_kept"] = w.rebuilds == before
+        E, F, W = check(Xs)
+        checked = {"neighbour_audit": audit, "accuracy_vs_fp64_oracle": accuracy(held["r"], E, F, W, n)}
     return {
+        **checked,
         "atoms": n, "skin_A": skin, "precision": prec, "steps": steps, "step_amplitude_A": amp,
         "rebuild_steps": int(rebuilt.sum()), "amortized_ms": round(1e3 * float(times.mean()), 4),
         "value": round(n / float(times.mean()) / 1e6, 3), "unit": "Matom-evals/s",
@@ -301,9 +450,10 @@ def bench_may30(pfw, lib, _capi, args, reps=5):
            "unit": "Matom-evals/s",
            "reference_cpu_context": {"atoms": 1024, "prepared_ms": 74.629, "threads": 1,
                                      "source": "benchmark/cpu_ace/MAY30-REPORT.md:8-10,133-139 (DAG, bcc-1024)"}}
+    results = {}
     for prec, mode in (("mixed", _capi.PFW_MIXED), ("float64", _capi.PFW_FP64)):
         calc = pfw.ACECalculator(model, ps, precision=prec)
-        calc.evaluate(sys_)
+        results[prec] = calc.evaluate(sys_)
         secs, stage = time_steps(lib, calc.workspace.ctx.h, lib.pfw_ace_step, max(3, args.steps // 10),
                                  min(args.warmup, 2), mode)
         nsteps = max(3, args.steps // 10)
@@ -314,11 +464,26 @@ def bench_may30(pfw, lib, _capi, args, reps=5):
             flops, per_edge, per_site = ace_flops(model, edges, len(X))
             kms = float(stage[4])
             out["edges"] = edges
-            out["roofline"] = {"kernel": "k_ace_dens+k_ace_prod+k_ace_pull<float,6>", "bound": "mfma",
+            out["roofline"] = {"kernel": "k_ace_dens+k_ace_prod+k_ace_pull<float,6>", "bound": "valu",
                                "compute_unit": "VALU fp32", "achieved": round(flops / (kms * 1e-3) / 1e12, 3),
                                "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                                "frac": round(flops / (kms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4),
+                               "frac_of_unpacked_issue": round(flops / (kms * 1e-3) / 1e12 / PEAK_FP32_UNPACKED_TFLOPS,
+                                                               4),
                                "flops_per_edge": per_edge, "flops_per_site": per_site, "kernel_ms": round(kms, 5)}
+            out["mixed_calc"] = calc
+    if not args.no_cpu_baseline:
+        from oracle.ace import ace_evaluate as oracle_ace
+        from oracle.ace import load_resolved as oracle_load
+
+        om = oracle_load(pfw.ace.read_resolved_npz(os.path.join(gold, "may30_resolved.npz")))
+        sp = np.array([om["atomic_numbers"].index(z) for z in d["species"]] * len(shifts), np.int32)
+        run = lambda: out["mixed_calc"].evaluate(sys_)  # noqa: E731
+        out["neighbour_audit"] = neighbour_audit(pfw, out["mixed_calc"].workspace, run, X, H, (True,) * 3,
+                                                 model.cutoff)
+        E, F, W, _ = oracle_ace(X, sp, H, (True, True, True), om, threads=args.cpu_threads_used)
+        out["accuracy_vs_fp64_oracle"] = {prec: accuracy(results[prec], E, F, W, len(X)) for prec in results}
+    out.pop("mixed_calc", None)
     return out
 
 
@@ -365,8 +530,10 @@ def bench_features(pfw, args):
                        "ms_per_call": round(1e3 * vsecs, 3),
                        "call": "cotangents to the device, fresh cell list + pair export, per-atom reverse kernel, "
                                "position gather, species sums of the parameter gradients, gradients to host"}
+    # No CPU baseline: the feature oracle is a Python checker, not a compiled
+    # restatement whose speed would mean anything.
+    out["cpu_baseline"] = None
     if not args.no_cpu_baseline:
-        sys.path.insert(0, ROOT)
         from oracle.features import ace_features as oracle_features
         Xs, Hs, pbs, sps = pfw.workloads.bcc_binary(3, 3, 3, seed=3)
         T = dict(density=layer.density, products=layer.products, coupling=layer.coupling,
@@ -374,12 +541,65 @@ def bench_features(pfw, args):
                  radial_cutoff=layer.radial.cutoff, powers=layer.radial.powers)
         P = dict(mixing=[ps["mixing"][f"block_{b + 1}"] for b in range(len(layer.irreps))],
                  scale=ps["radial"]["scale"], rate=0.4, amplitude=1.3)
-        t0 = time.perf_counter()
-        oracle_features(Xs, np.asarray(sps), Hs, pbs, T, P)
-        cs = time.perf_counter() - t0
-        out["cpu_baseline"] = {"value": round(len(Xs) / cs / 1e6, 6), "unit": "Matom-features/s", "cores": 1,
-                               "kind": "port", "sample": "the oracle's Python restatement on 54 atoms of the "
-                                                        "same BCC lattice"}
+        from pfw_amd.ace import ELEMENTS
+        labs = ["Si" if v == 0 else "O" for v in sps]
+        chans = np.array([layer.atomic_numbers.index(ELEMENTS.index(x) + 1) for x in labs])
+        ref = oracle_features(Xs, chans, Hs, pbs, T, P)
+        got = pfw.ace_features(layer, pfw.System(Xs, Hs, pbs, labs), ps)
+        err = max(float(np.abs(np.asarray(g) - np.asarray(r)).max()) for g, r in zip(got, ref))
+        scale = max(float(np.abs(np.asarray(r)).max()) for r in ref)
+        out["accuracy_vs_fp64_oracle"] = {"sample": "54-atom cell of the same lattice (seed 3)",
+                                          "feature_max_abs": err, "feature_max_magnitude": scale}
+    return out
+
+
+def bench_argon_4000(pfw, lib, _capi, args, host):
+    """The reference's CPU comparison cell (benchmark/lennard_jones/common.jl:53-82,
+    RESULTS.md:25-28): 4,000 argon atoms, FCC a = 5.26 Å with the deterministic
+    0.04 Å rattle, cutoff 2.5 sigma = 8.5125 Å, eps 0.010323 eV, sigma 3.405 Å,
+    here with the framework's energy-shifted Lennard-Jones (lennard_jones.jl:5-22;
+    the benchmark's inverse-distance tilt costs the same per pair). One step =
+    fresh cell build + E+F+V + FP64 reductions + status read."""
+    X, H, pbc, sp = pfw.workloads.argon_fcc_rattled(10)
+    n = len(X)
+    eps, sigma = 0.010323, 3.405
+    rc = 2.5 * sigma
+    lj = pfw.lennard_jones(epsilon=eps, sigma=sigma, radius=rc)
+    sys_ = pfw.System(X, H, pbc, ["Ar"] * n)
+    steps = max(4 * args.steps, 200)
+    out = {"workload": "argon_fcc_4000_lj", "atoms": n, "cutoff_A": rc, "epsilon_eV": eps, "sigma_A": sigma,
+           "steps": steps, "unit": "Matom-evals/s",
+           "reference_cpu": {"threads_1_ms": 5.270, "threads_8_ms": 1.551, "cpu": "AMD Ryzen 9 9950X",
+                             "call": "complete public call rebuilding topology (W=4 phased executor)",
+                             "source": "benchmark/lennard_jones/RESULTS.md:25-28"}}
+    results, ws = {}, {}
+    for prec, mode in (("mixed", _capi.PFW_MIXED), ("float64", _capi.PFW_FP64)):
+        w = pfw.Workspace(precision=prec)
+        results[prec] = pfw.evaluate_(w, sys_, lj)
+        secs, stage = time_steps(lib, w.ctx.h, lib.pfw_pair_step, steps, args.warmup, mode)
+        ws[prec] = w
+        out[prec] = {"value": round(n * steps / secs / 1e6, 3), "ms_per_step": round(1e3 * secs / steps, 4),
+                     "stages_ms": {k: round(float(v), 5) for k, v in zip(STAGES, stage)}}
+    npairs = len(ws["mixed"].export_pairs()[0])
+    kms = out["mixed"]["stages_ms"]["interaction"]
+    flops = npairs * FLOPS_PER_PAIR["lj"]
+    out["roofline"] = {"kernel": "k_pair<float,LJ>", "bound": "valu", "compute_unit": "VALU fp32",
+                       "achieved": round(flops / (kms * 1e-3) / 1e12, 3), "peak": PEAK_FP32_TFLOPS,
+                       "unit": "TFLOP/s", "frac": round(flops / (kms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4),
+                       "directed_pairs": npairs, "flops_per_pair": FLOPS_PER_PAIR["lj"], "kernel_ms": kms,
+                       "cells": ws["mixed"].grid_info()["ncells"]}
+    if not args.no_cpu_baseline:
+        from oracle.core import KIND_LJ, pair_evaluate
+
+        def make(t):
+            return lambda: pair_evaluate(X, sp, H, pbc, KIND_LJ, (eps, sigma), rc, threads=t)
+
+        out["cpu_baseline"], (E, F, W) = cpu_baseline(
+            make, n, "full 4,000-atom E+F+V evaluations (cell build + phase-coloured half-list, FP64)",
+            args.cpu_threads_used, host, " over phase-coloured cell tasks")
+        out["accuracy_vs_fp64_oracle"] = {prec: accuracy(results[prec], E, F, W, n) for prec in results}
+        out["neighbour_audit"] = {prec: neighbour_audit(pfw, ws[prec], lambda: pfw.evaluate_(ws[prec], sys_, lj),
+                                                        X, H, pbc, rc) for prec in ws}
     return out
 
 
@@ -406,7 +626,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-verlet", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=10)
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=None,
+                    help="CPU baseline threads (default: physical cores of the affinity mask, at most 16)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a device copy of 256 MiB five times (HBM counter calibration)")
     args = ap.parse_args()
@@ -421,8 +642,13 @@ def main():
     pfw = load_package()
     from pfw_amd import _capi
 
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, ROOT)  # the oracle: CPU baseline and accuracy checks only
+
     lib = _capi.load()
-    threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
+    host = host_info()
+    threads = args.cpu_threads or baseline_threads(host)
+    args.cpu_threads_used = threads
 
     # ---------------------------------------------------------------- pair (headline)
     X, H, pbc, sp = pfw.workloads.b2_triclinic()
@@ -487,9 +713,11 @@ def main():
         "config": {"workload": "b2_smoothpair_256k_triclinic", "atoms": n, "cutoff_A": rc, "species": 2,
                    "cell": "B2 a=3.0 A, 40x40x80 cells, +/-5% seeded shear", "precision": "mixed",
                    "step": "cell build + E+F+V + FP64 reductions + status read"},
-        "roofline": {"kernel": "k_pair<float,SMOOTH>", "bound": "mfma", "compute_unit": "VALU fp32",
+        "roofline": {"kernel": "k_pair<float,SMOOTH>", "bound": "valu", "compute_unit": "VALU fp32",
                      "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": read_traffic("pair_b2_256k_traffic.json"),
+                     "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+                     "frac_of_unpacked_issue": round(achieved / PEAK_FP32_UNPACKED_TFLOPS, 4),
+                     "traffic": read_traffic("pair_b2_256k_traffic.json"),
                      "directed_pairs": npairs, "flops_per_pair": FLOPS_PER_PAIR["smooth"],
                      "kernel_ms": round(k_ms, 5)},
         "fp64": {"value": round(fp64_value, 3), "ms_per_step": round(1e3 * pair["float64"]["seconds"] / args.steps, 4),
@@ -506,10 +734,17 @@ def main():
         "cpu_baseline": None,
     }
 
+    # ---------------------------------------------------------------- argon 4,000 LJ (CPU reference cell)
+    line["argon_4000"] = bench_argon_4000(pfw, lib, _capi, args, host)
+
     # ---------------------------------------------------------------- Verlet reuse (skin 0.5 Å)
+    checks = not args.no_cpu_baseline
+    if checks:
+        from oracle.core import KIND_LJ, KIND_SMOOTH, pair_evaluate
     if not args.no_verlet:
         labels_b2 = ["A" if s == 0 else "B" for s in sp]
-        vb = {prec: bench_verlet(pfw, lib, _capi, X, H, pbc, sp, p, labels_b2, 0.5, prec, mode)
+        chk = (lambda Xs: pair_evaluate(Xs, sp, H, pbc, KIND_SMOOTH, A, rc, threads=threads)) if checks else None
+        vb = {prec: bench_verlet(pfw, lib, _capi, X, H, pbc, sp, p, labels_b2, 0.5, prec, mode, check=chk)
               for prec, mode in (("mixed", _capi.PFW_MIXED), ("float64", _capi.PFW_FP64))}
         kl = vb["mixed"]["reuse_stages_ms"]["interaction"]
         # HBM is the nearer roof of the list kernel: per list entry a 4 B index;
@@ -534,7 +769,9 @@ def main():
             "workload": "argon FCC 14^3 (10,976 atoms), LJ eps 0.0103 sigma 3.4 rc 8.5, the reference's "
                         "skin trajectory (benchmark/gpu_pairs.jl:118-142)",
             "mixed": bench_verlet(pfw, lib, _capi, Xr, Hr, pbr, spr, lj, ["Ar"] * len(Xr), 0.5, "mixed",
-                                  _capi.PFW_MIXED),
+                                  _capi.PFW_MIXED,
+                                  check=(lambda Xs: pair_evaluate(Xs, spr, Hr, pbr, KIND_LJ, (0.0103, 3.4), 8.5,
+                                                                  threads=threads)) if checks else None),
             "reference_rx6800xt_float32": {"amortized_ms": 0.256, "reuse_median_ms": 0.169, "rebuild_steps": 1,
                                            "source": "docs/implementation/GPU-PAIR-RESULTS.md:87-104"},
         }
@@ -568,20 +805,28 @@ def main():
             "unit": "Matom-evals/s",
             "ms_per_step": round(1e3 * ace["mixed"]["seconds"] / args.steps, 4),
             "fp64_ms_per_step": round(1e3 * ace["float64"]["seconds"] / args.steps, 4),
-            "roofline": {"kernel": "k_ace_dens+k_ace_prod+k_ace_pull<float,4>", "bound": "mfma",
+            "roofline": {"kernel": "k_ace_dens+k_ace_prod+k_ace_pull<float,4>", "bound": "valu",
                          "compute_unit": "VALU fp32",
                          "achieved": round(aflops / (ka * 1e-3) / 1e12, 3), "peak": PEAK_FP32_TFLOPS,
                          "unit": "TFLOP/s", "frac": round(aflops / (ka * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4),
+                         "frac_of_unpacked_issue": round(aflops / (ka * 1e-3) / 1e12 / PEAK_FP32_UNPACKED_TFLOPS, 4),
                          "flops_per_edge": per_edge, "flops_per_site": per_site, "kernel_ms": round(ka, 5),
                          "traffic": read_traffic("ace_bcc_32k_traffic.json")},
             "fp64_roofline_frac": round(aflops / (ka64 * 1e-3) / 1e12 / PEAK_FP64_TFLOPS, 4),
             "fp64_kernel_ms": round(ka64, 5),
             "stages_ms": {k: round(float(v), 5) for k, v in zip(STAGES, ace["mixed"]["stage_ms"])},
         }
+        if checks:
+            from oracle.ace import ace_evaluate as oracle_ace
+            from oracle.ace import load_resolved as oracle_load
+
+            om = oracle_load(D)
         if not args.no_verlet:
+            chk = (lambda Xs: oracle_ace(Xs, spa, Ha, pba, om, threads=threads)[:3]) if checks else None
             line["ace"]["verlet"] = {
                 "workload": "bcc_binary_ace_32k, trajectory x_s = x_0 + s v (environments at cutoff + skin)",
-                **{prec: bench_verlet(pfw, lib, _capi, Xa, Ha, pba, spa, (model, ps), labels, 0.5, prec, mode, ace=True)
+                **{prec: bench_verlet(pfw, lib, _capi, Xa, Ha, pba, spa, (model, ps), labels, 0.5, prec, mode,
+                                      ace=True, check=chk)
                    for prec, mode in (("mixed", _capi.PFW_MIXED), ("float64", _capi.PFW_FP64))}}
 
     # ---------------------------------------------------------------- May30 real model
@@ -597,28 +842,25 @@ def main():
         line["fitting_set"] = bench_fitting_set(pfw, lib, _capi, model, ps, p, args)
 
     # ---------------------------------------------------------------- CPU baseline + accuracy
-    if not args.no_cpu_baseline:
-        sys.path.insert(0, ROOT)
-        from oracle import KIND_SMOOTH, pair_evaluate
-
-        med, reps, (E, F, W) = cpu_baseline(lambda: pair_evaluate(X, sp, H, pbc, KIND_SMOOTH, A, rc, threads=threads))
-        line["cpu_baseline"] = {"value": round(n / med / 1e6, 4), "unit": "Matom-evals/s", "cores": threads,
-                                "kind": "port",
-                                "sample": f"median of {reps} full 256,000-atom E+F+V evaluations (cell build + "
-                                          f"phase-coloured half-list FP64, {threads} threads)"}
+    if checks:
+        line["cpu_baseline"], (E, F, W) = cpu_baseline(
+            lambda t: lambda: pair_evaluate(X, sp, H, pbc, KIND_SMOOTH, A, rc, threads=t), n,
+            "full 256,000-atom E+F+V evaluations (cell build + phase-coloured half-list, FP64)", threads, host,
+            " over phase-coloured cell tasks")
         line["accuracy_vs_fp64_oracle"] = {prec: accuracy(pair[prec]["result"], E, F, W, n) for prec in pair}
+        line["neighbour_audit"] = {
+            prec: neighbour_audit(pfw, pair[prec]["ws"], lambda: pfw.evaluate_(pair[prec]["ws"], sys_, p), X, H, pbc,
+                                  rc) for prec in pair}
         if "ace" in line:
-            from oracle.ace import ace_evaluate as oracle_ace
-            from oracle.ace import load_resolved as oracle_load
-
-            om = oracle_load(D)
-            med, reps, (Ea, Fa, Wa, _) = cpu_baseline(lambda: oracle_ace(Xa, spa, Ha, pba, om, threads=threads))
-            line["ace"]["cpu_baseline"] = {"value": round(na / med / 1e6, 5), "unit": "Matom-evals/s",
-                                           "cores": threads, "kind": "port",
-                                           "sample": f"median of {reps} full 32,768-atom E+F+V evaluations "
-                                                     f"(FP64 site pipeline, {threads} threads)"}
+            line["ace"]["cpu_baseline"], (Ea, Fa, Wa, _) = cpu_baseline(
+                lambda t: lambda: oracle_ace(Xa, spa, Ha, pba, om, threads=t, coloured=True), na,
+                "full 32,768-atom E+F+V evaluations (FP64 site pipeline)", threads, host,
+                " over greedily coloured site phases with one shared force array")
             line["ace"]["accuracy_vs_fp64_oracle"] = {prec: accuracy(ace[prec]["result"], Ea, Fa, Wa, na)
                                                       for prec in ace}
+            line["ace"]["neighbour_audit"] = {
+                prec: neighbour_audit(pfw, ace[prec]["calc"].workspace, lambda: ace[prec]["calc"].evaluate(sys_a), Xa,
+                                      Ha, pba, model.cutoff) for prec in ace}
     print(json.dumps(line))
 
 

@@ -3,6 +3,8 @@
 * ``argon_fcc``: FCC lattice a = 5.26 Å, m^3 conventional cells, 4 m^3 atoms in
   a cubic periodic box, Gaussian displacements (the 4,000-atom Lennard-Jones case
   at m = 10: eps 0.0104 eV, sigma 3.40 Å, cutoff 8.5 Å, energy shifted).
+* ``argon_fcc_rattled``: the reference's deterministic argon benchmark cell
+  (4,000 atoms at m = 10; its CPU timings are quoted for this geometry).
 * ``b2_triclinic``: two-species B2 lattice a = 3.0 Å (species 0 at the origin,
   1 at the body centre) on nx x ny x nz conventional cells, lattice vectors
   sheared by seeded uniform +/-5 % off-diagonal strains, Gaussian displacements
@@ -30,6 +32,23 @@ def argon_fcc(m=10, a=5.26, sigma=0.05, seed=1):
     X = frac * a + rng.normal(0.0, sigma, size=frac.shape)
     H = np.eye(3) * (m * a)
     return X, H, (True, True, True), np.zeros(len(X), np.int32)
+
+
+def argon_fcc_rattled(m=10, amplitude=0.04, a=5.26):
+    """The reference's deterministic argon benchmark fixture
+    (benchmark/lennard_jones/common.jl:65-82): FCC a = 5.26 Å with ix fastest,
+    then iy, iz, and the basis (0,0,0), (0,1/2,1/2), (1/2,0,1/2), (1/2,1/2,0);
+    atom i (1-based) is displaced by amplitude * sin(i (0.37 + k)) along axis
+    k = 1, 2, 3 and wrapped into the periodic box of side m a."""
+    basis = np.array([[0, 0, 0], [0, 0.5, 0.5], [0.5, 0, 0.5], [0.5, 0.5, 0]])
+    g = np.stack(np.meshgrid(np.arange(m), np.arange(m), np.arange(m), indexing="ij"), -1)
+    g = g.transpose(2, 1, 0, 3).reshape(-1, 3)  # (ix, iy, iz) with ix fastest
+    frac = (g[:, None, :] + basis[None, :, :]).reshape(-1, 3)
+    i = np.arange(1, len(frac) + 1, dtype=float)[:, None]
+    k = np.arange(1, 4, dtype=float)[None, :]
+    L = m * a
+    X = np.mod(a * frac + amplitude * np.sin(i * (0.37 + k)), L)
+    return X, np.eye(3) * L, (True, True, True), np.zeros(len(X), np.int32)
 
 
 def b2_triclinic(nx=40, ny=40, nz=80, a=3.0, shear=0.05, sigma=0.08, seed=2):
