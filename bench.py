@@ -603,6 +603,54 @@ def bench_argon_4000(pfw, lib, _capi, args, host):
     return out
 
 
+def profile_run(pfw, lib, _capi, which, prec, steps):
+    """A short run for profilers: `steps` device steps of one workload after
+    one set-up evaluation (pair: the 256k B2 SmoothPair step; ace: the 32k BCC
+    ACE step; may30: the 30,000-atom May30 step; argon: the 4,000-atom LJ
+    step), then one line naming what ran."""
+    mode = _capi.PFW_MIXED if prec == "mixed" else _capi.PFW_FP64
+    if which in ("pair", "argon"):
+        if which == "pair":
+            X, H, pbc, sp = pfw.workloads.b2_triclinic()
+            model = pfw.SmoothPair(["A", "B"], np.array([[1.0, 0.7], [0.7, 1.3]]), 6.0)
+            labels = ["A" if v == 0 else "B" for v in sp]
+        else:
+            X, H, pbc, sp = pfw.workloads.argon_fcc_rattled(10)
+            model = pfw.lennard_jones(epsilon=0.010323, sigma=3.405, radius=2.5 * 3.405)
+            labels = ["Ar"] * len(X)
+        w = pfw.Workspace(precision=prec)
+        pfw.evaluate_(w, pfw.System(X, H, pbc, labels), model)
+        h, step = w.ctx.h, lib.pfw_pair_step
+    else:
+        from pfw_amd.ace import ELEMENTS
+        if which == "ace":
+            model, ps, _ = pfw.load_resolved(pfw.ace_basis.synthetic_model())
+            X, H, pbc, sp = pfw.workloads.bcc_binary()
+            labels = [ELEMENTS[model.atomic_numbers[v] - 1] for v in sp]
+        else:
+            try:
+                import tomllib as toml
+            except ImportError:
+                import tomli as toml
+            gold = os.path.join(ROOT, "tests", "golden")
+            model, ps, _ = pfw.load_resolved(os.path.join(gold, "may30_resolved.npz"))
+            with open(os.path.join(gold, "may30_omega-240.toml"), "rb") as f:
+                d = toml.load(f)
+            n0 = len(d["species"])
+            H0 = np.asarray(d["cell"], float).reshape(3, 3).T
+            shifts = np.array([[a, b, c] for a in range(5) for b in range(5) for c in range(5)], float) @ H0.T
+            X = (np.asarray(d["positions"], float).reshape(n0, 3)[None] + shifts[:, None, :]).reshape(-1, 3)
+            H, pbc = 5 * H0, (True, True, True)
+            labels = [ELEMENTS[z - 1] for z in d["species"]] * len(shifts)
+        calc = pfw.ACECalculator(model, ps, precision=prec)
+        calc.evaluate(pfw.System(X, H, pbc, labels))
+        h, step = calc.workspace.ctx.h, lib.pfw_ace_step
+    for _ in range(steps):
+        _capi.check(h, step(h, 7, mode))
+    _capi.check(h, lib.pfw_synchronize(h))
+    print(json.dumps({"profile": which, "precision": prec, "atoms": len(X), "steps": steps}), flush=True)
+
+
 def calibrate_copy(nbytes=256 << 20, reps=5):
     """A plain device copy of known size: the reference point that scales the
     FETCH_SIZE / WRITE_SIZE counters into HBM bytes."""
@@ -630,6 +678,9 @@ def main():
                     help="CPU baseline threads (default: physical cores of the affinity mask, at most 16)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a device copy of 256 MiB five times (HBM counter calibration)")
+    ap.add_argument("--profile", choices=("pair", "ace", "may30", "argon"), default=None,
+                    help="only run --steps device steps of one workload (for rocprofv3)")
+    ap.add_argument("--precision", choices=("mixed", "float64"), default="mixed", help="precision of --profile")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
@@ -642,6 +693,9 @@ def main():
     pfw = load_package()
     from pfw_amd import _capi
 
+    if args.profile:
+        profile_run(pfw, _capi.load(), _capi, args.profile, args.precision, args.steps)
+        return
     if not args.no_cpu_baseline:
         sys.path.insert(0, ROOT)  # the oracle: CPU baseline and accuracy checks only
 
