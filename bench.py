@@ -94,6 +94,24 @@ def time_steps(lib, h, step, steps, warmup, mode):
     return t1 - t0, stage / reps
 
 
+def graph_steps(lib, _capi, w, step, steps, warmup, mode):
+    """Seconds of `steps` steps replayed from a captured graph (one eager step
+    sizes the buffers, the next captures); graphs are switched off after."""
+    h = w.ctx.h
+    w.set_graphs(True)
+    for _ in range(2 + warmup):
+        _capi.check(h, step(h, 7, mode))
+    _capi.check(h, lib.pfw_synchronize(h))
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        _capi.check(h, step(h, 7, mode))
+    _capi.check(h, lib.pfw_synchronize(h))
+    secs = time.perf_counter() - t0
+    info = w.graph_info()
+    w.set_graphs(False)
+    return secs, info
+
+
 def read_traffic(name):
     """HBM bytes per launch of a kernel from its committed rocprofv3 PMC summary
     (FETCH_SIZE + WRITE_SIZE, calibrated against a copy kernel), if present."""
@@ -256,11 +274,17 @@ def bench_fitting_set(pfw, lib, _capi, model, ps, pair_model, args):
         secs, stage = time_steps(lib, w.ctx.h, lib.pfw_pair_step, K, Wm, mode)
         out[f"pair_{prec}"] = {"value": round(natoms * K / secs / 1e6, 3), "ms_per_step": round(1e3 * secs / K, 4),
                                "stages_ms": {k: round(float(v), 5) for k, v in zip(STAGES, stage)}}
+        gsecs, _ = graph_steps(lib, _capi, w, lib.pfw_pair_step, K, Wm, mode)
+        out[f"pair_{prec}"]["graph"] = {"value": round(natoms * K / gsecs / 1e6, 3),
+                                        "ms_per_step": round(1e3 * gsecs / K, 4)}
         calc = pfw.ACECalculator(model, ps, precision=prec)
         results[("ace", prec)] = calc.evaluate_batch(sys_ace)
         secs, stage = time_steps(lib, calc.workspace.ctx.h, lib.pfw_ace_step, K, Wm, mode)
         out[f"ace_{prec}"] = {"value": round(natoms * K / secs / 1e6, 4), "ms_per_step": round(1e3 * secs / K, 4),
                               "stages_ms": {k: round(float(v), 5) for k, v in zip(STAGES, stage)}}
+        gsecs, _ = graph_steps(lib, _capi, calc.workspace, lib.pfw_ace_step, K, Wm, mode)
+        out[f"ace_{prec}"]["graph"] = {"value": round(natoms * K / gsecs / 1e6, 4),
+                                       "ms_per_step": round(1e3 * gsecs / K, 4)}
     if not args.no_cpu_baseline:
         # every 21st configuration (49 of 1,024) against the oracle, one at a time
         from oracle.ace import ace_evaluate as oracle_ace
@@ -580,6 +604,8 @@ def bench_argon_4000(pfw, lib, _capi, args, host):
         ws[prec] = w
         out[prec] = {"value": round(n * steps / secs / 1e6, 3), "ms_per_step": round(1e3 * secs / steps, 4),
                      "stages_ms": {k: round(float(v), 5) for k, v in zip(STAGES, stage)}}
+        gsecs, _ = graph_steps(lib, _capi, w, lib.pfw_pair_step, steps, args.warmup, mode)
+        out[prec]["graph"] = {"value": round(n * steps / gsecs / 1e6, 3), "ms_per_step": round(1e3 * gsecs / steps, 4)}
     npairs = len(ws["mixed"].export_pairs()[0])
     kms = out["mixed"]["stages_ms"]["interaction"]
     flops = npairs * FLOPS_PER_PAIR["lj"]
@@ -716,7 +742,8 @@ def main():
         w = pfw.Workspace(precision=prec)
         res = pfw.evaluate_(w, sys_, p)  # uploads geometry + model outside the timed region
         secs, stage = time_steps(lib, w.ctx.h, lib.pfw_pair_step, args.steps, args.warmup, mode)
-        pair[prec] = dict(seconds=secs, stage_ms=stage, result=res, ws=w)
+        gsecs, _ = graph_steps(lib, _capi, w, lib.pfw_pair_step, args.steps, args.warmup, mode)
+        pair[prec] = dict(seconds=secs, stage_ms=stage, result=res, ws=w, graph_seconds=gsecs)
     # host boundary: positions uploaded from and forces returned to pageable host memory every step
     wh = pair["mixed"]["ws"]
     Xh = np.ascontiguousarray(X)
@@ -778,6 +805,8 @@ def main():
                  "kernel_ms": round(k64_ms, 5),
                  "roofline_frac": round(pflops / (k64_ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS, 4)},
         "mixed_over_fp64": round(value / fp64_value, 3),
+        "graph": {prec: {"value": round(n * args.steps / pair[prec]["graph_seconds"] / 1e6, 3),
+                         "ms_per_step": round(1e3 * pair[prec]["graph_seconds"] / args.steps, 4)} for prec in pair},
         "host_boundary": {"value": round(n * args.steps / host_secs / 1e6, 3),
                           "ms_per_step": round(1e3 * host_secs / args.steps, 4),
                           "note": "positions host->device and forces device->host (pageable) inside every step",
@@ -844,7 +873,8 @@ def main():
             res = calc.evaluate(sys_a)
             h = calc.workspace.ctx.h
             secs, stage = time_steps(lib, h, lib.pfw_ace_step, args.steps, args.warmup, mode)
-            ace[prec] = dict(seconds=secs, stage_ms=stage, result=res, calc=calc)
+            gsecs, _ = graph_steps(lib, _capi, calc.workspace, lib.pfw_ace_step, args.steps, args.warmup, mode)
+            ace[prec] = dict(seconds=secs, stage_ms=stage, result=res, calc=calc, graph_seconds=gsecs)
         edges = len(ace["mixed"]["calc"].workspace.export_pairs()[0])
         na = len(Xa)
         aflops, per_edge, per_site = ace_flops(model, edges, na)
@@ -866,10 +896,29 @@ def main():
                          "frac_of_unpacked_issue": round(aflops / (ka * 1e-3) / 1e12 / PEAK_FP32_UNPACKED_TFLOPS, 4),
                          "flops_per_edge": per_edge, "flops_per_site": per_site, "kernel_ms": round(ka, 5),
                          "traffic": read_traffic("ace_bcc_32k_traffic.json")},
+            "graph": {prec: {"value": round(na * args.steps / ace[prec]["graph_seconds"] / 1e6, 4),
+                             "ms_per_step": round(1e3 * ace[prec]["graph_seconds"] / args.steps, 4)} for prec in ace},
             "fp64_roofline_frac": round(aflops / (ka64 * 1e-3) / 1e12 / PEAK_FP64_TFLOPS, 4),
             "fp64_kernel_ms": round(ka64, 5),
             "stages_ms": {k: round(float(v), 5) for k, v in zip(STAGES, ace["mixed"]["stage_ms"])},
         }
+        # the autotuner's sweep on this workload (a fresh calculator; knob and graph choices)
+        import tempfile
+        with tempfile.TemporaryDirectory() as td:
+            tcalc = pfw.ACECalculator(model, ps, precision="mixed")
+            rep = pfw.autotune(tcalc, sys_a, steps=10, warmup=2, cache=os.path.join(td, "tuning.json"))
+            tp = pfw.Workspace(precision="mixed")
+            repp = pfw.autotune(tp, sys_, p, steps=10, warmup=2, cache=os.path.join(td, "tuning.json"))
+        default = [c for c in rep["candidates"] if c[0] == {"ace_sites_per_wave": 2, "ace_radial_chunk": 40,
+                                                             "ace_product_waves": 16} and not c[1]]
+        line["autotune"] = {
+            "ace_bcc_32k_mixed": {"best": rep["best"], "graphs": rep["graphs"], "ms_per_step": round(rep["ms"], 4),
+                                  "built_in_ms": round(default[0][2], 4) if default else None,
+                                  "candidates": len(rep["candidates"])},
+            "b2_256k_mixed": {"best": repp["best"], "graphs": repp["graphs"], "ms_per_step": round(repp["ms"], 4),
+                              "candidates": [[c[0]["pair_lanes_per_atom"], c[1], round(c[2], 4)]
+                                             for c in repp["candidates"]]},
+            "timing": "median per-step wall time (each step ends with its status read), best of 3 runs of 10"}
         if checks:
             from oracle.ace import ace_evaluate as oracle_ace
             from oracle.ace import load_resolved as oracle_load

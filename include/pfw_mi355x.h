@@ -188,6 +188,32 @@ int pfw_pair_evaluate_noalloc(pfw_ctx* ctx, int kind, int nz, const double* para
 int pfw_set_audit(pfw_ctx* ctx, int enabled);
 int pfw_read_audit(pfw_ctx* ctx, int32_t* count, uint64_t* hash);
 
+/* Captured steps. With graphs enabled, the launches of a fresh pair or ACE
+ * step without skin (cell build, the interaction kernels, the FP64
+ * reductions) are captured once as a HIP graph and replayed while only the
+ * positions change (pfw_update_positions / pfw_update_positions_device); any
+ * other entry point that may change a buffer, shape or model drops the graph,
+ * and the step after that runs eagerly before the next capture. Timing and
+ * audit runs are never captured. Results are those of the eager step. The
+ * reference runs its steps as KernelAbstractions launches with no capture
+ * (src/submodules/NeighbourList/gpu.jl:236-402, ACE/gpu.jl:301-330); this is
+ * a launch-mode switch beside the GPUConfig knobs of gpu.jl:61-73.
+ * pfw_graph_info: out[0] captures, out[1] replays, out[2] 1 while a graph is held. */
+int pfw_set_graphs(pfw_ctx* ctx, int enabled);
+int pfw_graph_info(pfw_ctx* ctx, int64_t* out);
+
+/* Launch-shape knobs, the tunable counterparts of GPUConfig's strategy,
+ * workgroup and tile (src/submodules/NeighbourList/gpu.jl:61-73) and the ACE
+ * strategy (src/submodules/ACE/gpu.jl:301-303). knobs[0]: ACE sites per
+ * wavefront (0 built-in, 1 or 2); knobs[1]: ACE radial indices per density
+ * tile (0 built-in = 40, or 1..4096); knobs[2]: ACE product wavefronts per
+ * workgroup (0 built-in, 1..16); knobs[3]: most lanes per atom in the pair
+ * kernel (0 built-in = 4, 1..4). Every setting gives the same pair sets and
+ * results to rounding. nknobs <= 4 values are read or written from the
+ * first; PFW_INVALID_ARGUMENT for a value outside its range. */
+int pfw_set_tuning(pfw_ctx* ctx, const int32_t* knobs, int nknobs);
+int pfw_get_tuning(pfw_ctx* ctx, int32_t* knobs, int nknobs);
+
 /* Synchronise the context's stream. */
 int pfw_synchronize(pfw_ctx* ctx);
 

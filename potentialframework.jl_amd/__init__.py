@@ -47,6 +47,8 @@ from .adjoints import (  # noqa: F401
 from .fitting import design_matrix, training_rows, linear_parameters, install_readout  # noqa: F401
 from .precision import compare_precision, PrecisionReport  # noqa: F401
 from . import features  # noqa: F401
+from . import tune  # noqa: F401
+from .tune import autotune  # noqa: F401
 from .features import feature_vjp, ace_features_rrule  # noqa: F401
 from .features import ACEFeatureLayer, ace_features, feature_parameters, feature_basis_summary  # noqa: F401
 from .calculators import (  # noqa: F401

@@ -77,6 +77,10 @@ SIGNATURES = [
       ctypes.c_uint32, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     ("pfw_set_audit", ctypes.c_int, [_P, ctypes.c_int]),
     ("pfw_read_audit", ctypes.c_int, [_P, _I32, ctypes.POINTER(ctypes.c_uint64)]),
+    ("pfw_set_graphs", ctypes.c_int, [_P, ctypes.c_int]),
+    ("pfw_graph_info", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64)]),
+    ("pfw_set_tuning", ctypes.c_int, [_P, _I32, ctypes.c_int]),
+    ("pfw_get_tuning", ctypes.c_int, [_P, _I32, ctypes.c_int]),
     ("pfw_synchronize", ctypes.c_int, [_P]),
     ("pfw_host_alloc", ctypes.c_void_p, [ctypes.c_size_t]),
     ("pfw_host_free", None, [ctypes.c_void_p]),

@@ -1499,6 +1499,24 @@ void ace_set_parameters(pfw_ctx* ctx, const double* WB, const double* Wpair) {
   PFW_HIP(hipStreamSynchronize(st));
 }
 
+// Densities grouped by chunks of `rchunk` radial indices for the density
+// kernel's radial tile (0: the built-in 40, capped at the radial count).
+void ace_group_densities(pfw_ctx* ctx, int rchunk) {
+  AceDev& a = ctx->ace;
+  a.rchunk = std::max(1, std::min(a.radial.nr, rchunk > 0 ? rchunk : kRadialChunk));
+  std::vector<int32_t> dperm, dchunk(1, 0);
+  const int nd = (int)(a.h_dens.size() / 2);
+  for (int r0 = 0; r0 < std::max(a.radial.nr, 1); r0 += a.rchunk) {
+    for (int k = 0; k < nd; ++k)
+      if (a.h_dens[2 * k] >= r0 && a.h_dens[2 * k] < r0 + a.rchunk) dperm.push_back(k);
+    dchunk.push_back((int32_t)dperm.size());
+  }
+  dfree(a.dperm);
+  dfree(a.dchunk);
+  a.dperm = upload(dperm, ctx->stream);
+  a.dchunk = upload(dchunk, ctx->stream);
+}
+
 void ace_set_model(pfw_ctx* ctx, const pfw_ace_model* m) {
   ace_free(ctx);
   AceDev& a = ctx->ace;
@@ -1573,18 +1591,8 @@ void ace_set_model(pfw_ctx* ctx, const pfw_ace_model* m) {
     rlb[l + 1] = (int32_t)(rl.size() / 2);
   }
   a.dens = upload(dens, st);
-  // densities grouped by chunks of radial indices for the density kernel's tile
-  a.rchunk = std::max(1, std::min(a.radial.nr, kRadialChunk));
-  {
-    std::vector<int32_t> dperm, dchunk(1, 0);
-    for (int r0 = 0; r0 < std::max(a.radial.nr, 1); r0 += a.rchunk) {
-      for (int k = 0; k < m->ndens; ++k)
-        if (dens[2 * k] >= r0 && dens[2 * k] < r0 + a.rchunk) dperm.push_back(k);
-      dchunk.push_back((int32_t)dperm.size());
-    }
-    a.dperm = upload(dperm, st);
-    a.dchunk = upload(dchunk, st);
-  }
+  a.h_dens = dens;
+  ace_group_densities(ctx, ctx->tune[1]);
   a.rlb = upload(rlb, st);
   a.rl = upload(rl, st);
   // products (factor lists, -1 padded) and their conflict-free schedule
@@ -1744,7 +1752,8 @@ static void launch_sites(pfw_ctx* ctx, int grads) {
   const bool global_tables = tables + 4 * per_wave > kMaxLds;
   if (global_tables) tables = 0;
   if (tables + per_wave > kMaxLds) throw Error(PFW_INVALID_ARGUMENT, "ACE product accumulators too large for LDS");
-  const int waves = (int)std::min<size_t>(16, (kMaxLds - tables) / per_wave);
+  int waves = (int)std::min<size_t>(16, (kMaxLds - tables) / per_wave);
+  if (ctx->tune[2] > 0) waves = std::min(waves, ctx->tune[2]);
   const size_t lds_prod = tables + waves * per_wave;
   const int64_t want = (s.n + waves - 1) / waves;
   const unsigned pblocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>(want, ctx->num_cu));
@@ -1769,7 +1778,8 @@ static void launch_sites_spw(pfw_ctx* ctx, int grads) {
   // two sites per wavefront unless some site has more than 48 edges: a site
   // above 32 takes a second pass in its half-wavefront, which costs less than
   // halving the sites per wavefront for every site
-  if (ctx->ace.max_degree <= 48)
+  const int spw = ctx->tune[0] > 0 ? ctx->tune[0] : ctx->ace.max_degree <= 48 ? 2 : 1;
+  if (spw == 2)
     launch_sites<T, L, 2>(ctx, grads);
   else
     launch_sites<T, L, 1>(ctx, grads);
@@ -1814,7 +1824,7 @@ static void ace_environment(pfw_ctx* ctx, int rev) {
   }
   ace_rows_alloc(ctx, ad.ecap);
   const int64_t slots = n * (int64_t)ad.ecap;
-  PFW_HIP(hipMemsetAsync(ad.status, 0, 4 * sizeof(int32_t), st));
+  fill_i32(ad.status, 4, 0, st);  // a fill kernel: this sequence may be captured in a graph
   hipLaunchKernelGGL(k_ace_edges, dim3((unsigned)ctx->ncells), dim3(kWave * kEdgeWaves), 0, st, s.grids, s.cell_cfg, n, s.xs,
                      s.start, 1, ad.ecap, ad.ecount, ad.enb, ad.eent, ad.eR, ad.status, 0);
   PFW_HIP(hipGetLastError());
@@ -2022,6 +2032,23 @@ void ace_run(pfw_ctx* ctx, uint32_t mask, int precision) {
   const int grads = (mask & (PFW_FORCES | PFW_VIRIAL)) ? 1 : 0;
   if (ctx->verlet.skin > 0 && n > 0 && mask != 0) {
     ace_skin_step(ctx, grads, precision);
+  } else if (ctx->graphs && ctx->graph_warm && !ctx->timing && !ctx->audit && n > 0 && mask != 0 && ad.ecap > 0) {
+    struct Launch {
+      int grads, precision;
+    } arg{grads, precision};
+    graph_issue(ctx, 1, precision, mask, [](pfw_ctx* c, void* p) {
+      const Launch& a = *(const Launch*)p;
+      nl_build(c);
+      ace_environment(c, a.grads);
+      ace_evaluate_rows(c, a.grads, a.precision);
+      copy_outputs(c);
+    }, &arg);
+    PFW_HIP(hipStreamSynchronize(st));
+    if (ace_overflowed(ad)) {  // rows outgrew their capacity: drop the graph, repeat eagerly with wider rows
+      graph_reset(ctx);
+      ace_run(ctx, mask, precision);
+      return;
+    }
   } else {
     nl_build(ctx);
     record_stage(ctx, 3);
@@ -2070,6 +2097,7 @@ void ace_run(pfw_ctx* ctx, uint32_t mask, int precision) {
   ctx->last_mask = mask;
   ctx->have_results = true;
   if (ctx->h_out[10] != 0.0) throw Error(PFW_DOMAIN, "nonfinite ACE output (coincident atoms or singular envelope)");
+  ctx->graph_warm = true;
 }
 
 // ---------------------------------------------------------------- equivariant features

@@ -7,6 +7,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <utility>
 #include <vector>
 
 #include "pfw_device.h"
@@ -15,11 +16,15 @@ using pfw::Error;
 
 namespace {
 
-template <typename F>
+// Every entry point runs through here. Unless KEEP (steps, position updates
+// and read-outs), the captured step graph is dropped first: any other call may
+// replace a buffer, a shape or a model the graph's launches refer to.
+template <bool KEEP = false, typename F>
 int run(pfw_ctx* ctx, F&& f) {
   if (!ctx) return PFW_INVALID_ARGUMENT;
   try {
     PFW_HIP(hipSetDevice(ctx->device));
+    if (!KEEP) pfw::graph_reset(ctx);
     f();
     ctx->err.clear();
     return PFW_OK;
@@ -38,6 +43,11 @@ void check_positions(int64_t n, const double* X) {
     if (!std::isfinite(X[k])) throw Error(PFW_INVALID_ARGUMENT, "positions must be finite");
 }
 
+template <typename F>
+int run_keep(pfw_ctx* ctx, F&& f) {
+  return run<true>(ctx, std::forward<F>(f));
+}
+
 // Copy of caller device positions into the staging buffer, flagging any
 // nonfinite coordinate.
 __global__ void k_copy_positions(int64_t m, const double* __restrict__ src, double* __restrict__ dst,
@@ -50,6 +60,55 @@ __global__ void k_copy_positions(int64_t m, const double* __restrict__ src, doub
 }
 
 }  // namespace
+
+namespace pfw {
+
+void graph_reset(pfw_ctx* ctx) {
+  if (ctx->graph_exec) {
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipGraphExecDestroy(ctx->graph_exec);
+  }
+  ctx->graph_exec = nullptr;
+  ctx->graph_kind = -1;
+  ctx->graph_warm = false;
+}
+
+void graph_issue(pfw_ctx* ctx, int kind, int precision, uint32_t mask, void (*issue)(pfw_ctx*, void*), void* arg) {
+  if (ctx->graph_exec && ctx->graph_kind == kind && ctx->graph_precision == precision && ctx->graph_mask == mask) {
+    PFW_HIP(hipGraphLaunch(ctx->graph_exec, ctx->stream));
+    ++ctx->build_gen;  // the replay rebuilds the cell list
+    ++ctx->graph_replays;
+    return;
+  }
+  if (ctx->graph_exec) {  // another step kind, precision or mask: replace the graph
+    PFW_HIP(hipStreamSynchronize(ctx->stream));
+    (void)hipGraphExecDestroy(ctx->graph_exec);
+    ctx->graph_exec = nullptr;
+    ctx->graph_kind = -1;
+  }
+  hipGraph_t g = nullptr;
+  PFW_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  try {
+    issue(ctx, arg);
+  } catch (...) {
+    (void)hipStreamEndCapture(ctx->stream, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  PFW_HIP(hipStreamEndCapture(ctx->stream, &g));
+  hipGraphExec_t exec = nullptr;
+  const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  PFW_HIP(e);
+  ctx->graph_exec = exec;
+  ctx->graph_kind = kind;
+  ctx->graph_precision = precision;
+  ctx->graph_mask = mask;
+  ++ctx->graph_captures;
+  PFW_HIP(hipGraphLaunch(ctx->graph_exec, ctx->stream));
+}
+
+}  // namespace pfw
 
 static void realloc_doubles(double*& p, size_t count) {
   if (p) (void)hipFree(p);
@@ -97,6 +156,7 @@ void pfw_ctx_destroy(pfw_ctx* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  pfw::graph_reset(ctx);
   pfw::nl_free(ctx);
   pfw::ace_free(ctx);
   pfw::verlet_free(ctx);
@@ -177,7 +237,7 @@ int pfw_set_system(pfw_ctx* ctx, int64_t n, const double* X, const int32_t* spec
 }
 
 int pfw_update_positions_device(pfw_ctx* ctx, const double* X) {
-  return run(ctx, [&] {
+  return run_keep(ctx, [&] {
     if (!ctx->have_system) throw Error(PFW_STALE, "no system set");
     const int64_t n = ctx->sys.n;
     if (n > 0 && !X) throw Error(PFW_INVALID_ARGUMENT, "positions pointer is null");
@@ -191,13 +251,16 @@ int pfw_update_positions_device(pfw_ctx* ctx, const double* X) {
     PFW_HIP(hipMemcpyAsync(ctx->h_flag, ctx->d_flag, sizeof(int32_t), hipMemcpyDeviceToHost, st));
     PFW_HIP(hipStreamSynchronize(st));
     if (*ctx->h_flag) throw Error(PFW_INVALID_ARGUMENT, "positions must be finite");
-    std::swap(ctx->sys.X, ctx->d_Xstage);  // the previous positions stay untouched on failure
+    if (ctx->graph_exec)  // a captured step reads the positions buffer it was captured with
+      PFW_HIP(hipMemcpyAsync(ctx->sys.X, ctx->d_Xstage, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    else
+      std::swap(ctx->sys.X, ctx->d_Xstage);  // the previous positions stay untouched on failure
     ctx->have_results = false;
   });
 }
 
 int pfw_update_positions(pfw_ctx* ctx, const double* X) {
-  return run(ctx, [&] {
+  return run_keep(ctx, [&] {
     if (!ctx->have_system) throw Error(PFW_STALE, "no system set");
     check_positions(ctx->sys.n, X);
     if (ctx->sys.n > 0)
@@ -243,7 +306,7 @@ int pfw_set_pair_model(pfw_ctx* ctx, int kind, int nz, const double* params, dou
 }
 
 int pfw_pair_step(pfw_ctx* ctx, uint32_t mask, int precision) {
-  return run(ctx, [&] {
+  return run_keep(ctx, [&] {
     if (!ctx->have_system) throw Error(PFW_STALE, "no system set");
     if (precision != PFW_FP64 && precision != PFW_MIXED) throw Error(PFW_INVALID_ARGUMENT, "unknown precision");
     if (mask & ~7u) throw Error(PFW_INVALID_ARGUMENT, "unknown request bits");
@@ -253,7 +316,7 @@ int pfw_pair_step(pfw_ctx* ctx, uint32_t mask, int precision) {
 }
 
 int pfw_read_results(pfw_ctx* ctx, double* E, double* F, double* W) {
-  return run(ctx, [&] {
+  return run_keep(ctx, [&] {
     if (!ctx->have_results) throw Error(PFW_STALE, "no results: run a step first");
     if (E) *E = (ctx->last_mask & PFW_ENERGY) ? ctx->h_out[0] : 0.0;
     if (W)
@@ -270,7 +333,7 @@ int pfw_read_results(pfw_ctx* ctx, double* E, double* F, double* W) {
 }
 
 int pfw_read_batch_results(pfw_ctx* ctx, double* E, double* F, double* W) {
-  return run(ctx, [&] {
+  return run_keep(ctx, [&] {
     if (!ctx->have_results) throw Error(PFW_STALE, "no results: run a step first");
     const int ncfg = ctx->ncfg;
     for (int k = 0; k < ncfg; ++k) {
@@ -297,7 +360,7 @@ int pfw_pair_evaluate(pfw_ctx* ctx, uint32_t mask, int precision, double* E, dou
 }
 
 int pfw_export_pairs(pfw_ctx* ctx, int32_t* i, int32_t* j, int32_t* S, double* R, int64_t cap, int64_t* count) {
-  return run(ctx, [&] {
+  return run_keep(ctx, [&] {
     if (!ctx->have_system) throw Error(PFW_STALE, "no system set");
     if (!count) throw Error(PFW_INVALID_ARGUMENT, "null count");
     pfw::nl_export_pairs(ctx, i, j, S, R, cap, count);
@@ -305,7 +368,7 @@ int pfw_export_pairs(pfw_ctx* ctx, int32_t* i, int32_t* j, int32_t* S, double* R
 }
 
 int pfw_grid_info(pfw_ctx* ctx, int64_t* ncells, int32_t* dims, int32_t* minimum_image) {
-  return run(ctx, [&] {
+  return run_keep(ctx, [&] {
     if (!ctx->have_system) throw Error(PFW_STALE, "no system set");
     if (ncells) *ncells = ctx->ncells;
     if (dims)
@@ -332,7 +395,7 @@ int pfw_set_skin(pfw_ctx* ctx, double skin) {
 }
 
 int pfw_verlet_info(pfw_ctx* ctx, int64_t* info, double* max_displacement) {
-  return run(ctx, [&] {
+  return run_keep(ctx, [&] {
     const pfw::VerletDev& V = ctx->verlet;
     if (info) {
       info[0] = V.rebuilds;
@@ -374,7 +437,7 @@ int pfw_set_audit(pfw_ctx* ctx, int enabled) {
 }
 
 int pfw_read_audit(pfw_ctx* ctx, int32_t* count, uint64_t* hash) {
-  return run(ctx, [&] {
+  return run_keep(ctx, [&] {
     if (!ctx->audit || !ctx->audit_valid) throw Error(PFW_STALE, "no audited pair or ACE step since the audit was enabled");
     const int64_t n = ctx->sys.n;
     if (n == 0) return;
@@ -399,7 +462,7 @@ int pfw_set_ace_parameters(pfw_ctx* ctx, const double* WB, const double* Wpair) 
 }
 
 int pfw_ace_step(pfw_ctx* ctx, uint32_t mask, int precision) {
-  return run(ctx, [&] {
+  return run_keep(ctx, [&] {
     if (!ctx->have_system) throw Error(PFW_STALE, "no system set");
     if (precision != PFW_FP64 && precision != PFW_MIXED) throw Error(PFW_INVALID_ARGUMENT, "unknown precision");
     if (mask & ~7u) throw Error(PFW_INVALID_ARGUMENT, "unknown request bits");
@@ -415,7 +478,7 @@ int pfw_ace_evaluate(pfw_ctx* ctx, uint32_t mask, int precision, double* E, doub
 }
 
 int pfw_ace_site_energies(pfw_ctx* ctx, double* out) {
-  return run(ctx, [&] {
+  return run_keep(ctx, [&] {
     if (!ctx->have_results) throw Error(PFW_STALE, "no results: run an ACE step first");
     if (!out) throw Error(PFW_INVALID_ARGUMENT, "null output");
     pfw::ace_site_energies(ctx, out);
@@ -527,8 +590,44 @@ void pfw_host_free(void* p) {
   P.free[it->second].push_back(p);
 }
 
+int pfw_set_graphs(pfw_ctx* ctx, int enabled) {
+  return run(ctx, [&] { ctx->graphs = enabled != 0; });
+}
+
+int pfw_graph_info(pfw_ctx* ctx, int64_t* out) {
+  return run_keep(ctx, [&] {
+    if (!out) throw Error(PFW_INVALID_ARGUMENT, "null output");
+    out[0] = ctx->graph_captures;
+    out[1] = ctx->graph_replays;
+    out[2] = ctx->graph_exec ? 1 : 0;
+  });
+}
+
+int pfw_set_tuning(pfw_ctx* ctx, const int32_t* knobs, int nknobs) {
+  return run(ctx, [&] {
+    if (!knobs || nknobs < 0 || nknobs > 4) throw Error(PFW_INVALID_ARGUMENT, "knobs: at most 4 values");
+    int t[4] = {ctx->tune[0], ctx->tune[1], ctx->tune[2], ctx->tune[3]};
+    for (int k = 0; k < nknobs; ++k) t[k] = knobs[k];
+    if (t[0] < 0 || t[0] > 2) throw Error(PFW_INVALID_ARGUMENT, "ACE sites per wavefront must be 0 (built-in), 1 or 2");
+    if (t[1] < 0 || t[1] > 4096) throw Error(PFW_INVALID_ARGUMENT, "ACE radial chunk must be 0 (built-in) or 1..4096");
+    if (t[2] < 0 || t[2] > 16) throw Error(PFW_INVALID_ARGUMENT, "ACE product wavefronts must be 0 (built-in) or 1..16");
+    if (t[3] < 0 || t[3] > 4) throw Error(PFW_INVALID_ARGUMENT, "pair lanes per atom must be 0 (built-in) or 1..4");
+    PFW_HIP(hipStreamSynchronize(ctx->stream));
+    const bool regroup = t[1] != ctx->tune[1];
+    for (int k = 0; k < 4; ++k) ctx->tune[k] = t[k];
+    if (regroup && ctx->ace.ready) pfw::ace_group_densities(ctx, t[1]);
+  });
+}
+
+int pfw_get_tuning(pfw_ctx* ctx, int32_t* knobs, int nknobs) {
+  return run_keep(ctx, [&] {
+    if (!knobs || nknobs < 0 || nknobs > 4) throw Error(PFW_INVALID_ARGUMENT, "knobs: at most 4 values");
+    for (int k = 0; k < nknobs; ++k) knobs[k] = ctx->tune[k];
+  });
+}
+
 int pfw_synchronize(pfw_ctx* ctx) {
-  return run(ctx, [&] { PFW_HIP(hipStreamSynchronize(ctx->stream)); });
+  return run_keep(ctx, [&] { PFW_HIP(hipStreamSynchronize(ctx->stream)); });
 }
 
 }  // extern "C"

@@ -113,7 +113,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
                                              const int32_t* __restrict__ sps,
                                              const int32_t* __restrict__ perm, const double* __restrict__ params,
                                              int nz, LJConsts lj, uint32_t mask, double* __restrict__ F,
-                                             double* __restrict__ partials, int ncells, AuditBufs au) {
+                                             double* __restrict__ partials, int ncells, AuditBufs au,
+                                             int max_slices) {
   // candidates as (x, y, z, (1 - slack) |x|^2), then far sentinels up to the end
   // of the last slice's range; after the candidate loops of an atom group the
   // same bytes hold the per-slice force sums
@@ -163,7 +164,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
     const double rx = xs[b], ry = xs[n + b], rz = xs[2 * n + b];
     for (int ib = 0; ib < ncnt; ib += kWave) {
       const int nb = min(kWave, ncnt - ib);
-      const int S = min(kSlices, kWave / nb);
+      const int S = min(max_slices, kWave / nb);  // max_slices in 1..kSlices
       const int il = lane % nb, split = lane / nb;
       const bool on = split < S;
       const int is = b + ib + il;
@@ -404,16 +405,41 @@ static void launch_pair(pfw_ctx* ctx, uint32_t mask) {
   SystemBuffers& s = ctx->sys;
   const LJConsts lj = lj_consts(ctx->pair);
   const size_t dyn = KIND == PFW_PAIR_SMOOTH ? 2 * sizeof(T) * ctx->pair.nz * ctx->pair.nz : 0;
+  const int slices = ctx->tune[3] > 0 ? std::min(ctx->tune[3], kSlices) : kSlices;
   if (ctx->audit) {
     const AuditBufs au = audit_begin(ctx);
     hipLaunchKernelGGL((k_pair<T, KIND, true>), dim3(ctx->ncells), dim3(kWave), dyn, ctx->stream, s.grids,
                        s.cell_cfg, s.n, s.start, s.xs, s.xr, s.sps, s.perm, ctx->pair.d_params, ctx->pair.nz, lj,
-                       mask, ctx->d_F, ctx->d_partials, ctx->ncells, au);
+                       mask, ctx->d_F, ctx->d_partials, ctx->ncells, au, slices);
   } else {
     hipLaunchKernelGGL((k_pair<T, KIND, false>), dim3(ctx->ncells), dim3(kWave), dyn, ctx->stream, s.grids,
                        s.cell_cfg, s.n, s.start, s.xs, s.xr, s.sps, s.perm, ctx->pair.d_params, ctx->pair.nz, lj,
-                       mask, ctx->d_F, ctx->d_partials, ctx->ncells, AuditBufs{});
+                       mask, ctx->d_F, ctx->d_partials, ctx->ncells, AuditBufs{}, slices);
   }
+}
+
+// The pair kernel and the reductions of a fresh step (stream work only).
+static void pair_launches(pfw_ctx* ctx, uint32_t mask, int precision) {
+  const bool fp32 = precision == PFW_MIXED;
+  if (ctx->pair.kind == PFW_PAIR_SMOOTH) {
+    if (fp32)
+      launch_pair<float, PFW_PAIR_SMOOTH>(ctx, mask);
+    else
+      launch_pair<double, PFW_PAIR_SMOOTH>(ctx, mask);
+  } else {
+    if (fp32)
+      launch_pair<float, PFW_PAIR_LJ>(ctx, mask);
+    else
+      launch_pair<double, PFW_PAIR_LJ>(ctx, mask);
+  }
+  PFW_HIP(hipGetLastError());
+  record_stage(ctx, 5);
+  if (ctx->ncfg == 1) {
+    reduce_outputs(ctx, ctx->d_partials, ctx->ncells, 7, 0, nullptr);
+  } else {
+    batch_reduce(ctx, ctx->d_partials, 7, true, 0);
+  }
+  PFW_HIP(hipGetLastError());
 }
 
 void pair_run(pfw_ctx* ctx, uint32_t mask, int precision) {
@@ -429,31 +455,27 @@ void pair_run(pfw_ctx* ctx, uint32_t mask, int precision) {
     pair_finish(ctx, mask);
     return;
   }
+  hipStream_t st = ctx->stream;
+  if (ctx->graphs && ctx->graph_warm && !ctx->timing && !ctx->audit && mask != 0 && ctx->sys.n > 0) {
+    struct Launch {
+      uint32_t mask;
+      int precision;
+    } arg{mask, precision};
+    graph_issue(ctx, 0, precision, mask, [](pfw_ctx* c, void* p) {
+      const Launch& a = *(const Launch*)p;
+      nl_build(c);
+      pair_launches(c, a.mask, a.precision);
+      copy_outputs(c);
+    }, &arg);
+    PFW_HIP(hipStreamSynchronize(st));
+    pair_finish(ctx, mask);
+    return;
+  }
   nl_build(ctx);
   record_stage(ctx, 3);
   record_stage(ctx, 4);
-  hipStream_t st = ctx->stream;
   if (mask != 0 && ctx->sys.n > 0) {
-    bool fp32 = precision == PFW_MIXED;
-    if (ctx->pair.kind == PFW_PAIR_SMOOTH) {
-      if (fp32)
-        launch_pair<float, PFW_PAIR_SMOOTH>(ctx, mask);
-      else
-        launch_pair<double, PFW_PAIR_SMOOTH>(ctx, mask);
-    } else {
-      if (fp32)
-        launch_pair<float, PFW_PAIR_LJ>(ctx, mask);
-      else
-        launch_pair<double, PFW_PAIR_LJ>(ctx, mask);
-    }
-    PFW_HIP(hipGetLastError());
-    record_stage(ctx, 5);
-    if (ctx->ncfg == 1) {
-      reduce_outputs(ctx, ctx->d_partials, ctx->ncells, 7, 0, nullptr);
-    } else {
-      batch_reduce(ctx, ctx->d_partials, 7, true, 0);
-    }
-    PFW_HIP(hipGetLastError());
+    pair_launches(ctx, mask, precision);
   } else {
     if (ctx->audit) audit_begin(ctx);  // nothing accumulated: empty digests
     record_stage(ctx, 5);
@@ -465,6 +487,7 @@ void pair_run(pfw_ctx* ctx, uint32_t mask, int precision) {
   record_stage(ctx, 6);
   PFW_HIP(hipStreamSynchronize(st));
   pair_finish(ctx, mask);
+  ctx->graph_warm = true;
 }
 
 // After the step's outputs reached h_out: per-configuration rows, stage
@@ -590,7 +613,7 @@ static void launch_pair_static(const Grid* grid, const int32_t* cell_cfg, int64_
                                int ncells, hipStream_t st) {
   const size_t dyn = KIND == PFW_PAIR_SMOOTH ? 2 * sizeof(T) * nz * nz : 0;
   hipLaunchKernelGGL((k_pair<T, KIND, false>), dim3(ncells), dim3(kWave), dyn, st, grid, cell_cfg, n, start, xs, xr,
-                     sps, perm, params, nz, lj, mask, F, partials, ncells, AuditBufs{});
+                     sps, perm, params, nz, lj, mask, F, partials, ncells, AuditBufs{}, kSlices);
 }
 
 void pair_run_static(pfw_ctx* ctx, int kind, int nz, const double* params, double cutoff, int64_t n,

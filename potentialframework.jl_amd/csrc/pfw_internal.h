@@ -178,6 +178,7 @@ struct AceDev {
   int32_t* dens = nullptr;      // ndens x 2
   int32_t* dperm = nullptr;     // densities ordered by chunks of radial indices
   int32_t* dchunk = nullptr;    // chunk offsets into dperm
+  std::vector<int32_t> h_dens;  // host copy of dens: the density tiles are regrouped when rchunk changes
   int rchunk = 1;               // radial indices per chunk
   int32_t* rlb = nullptr;       // lmax + 2: radial indices grouped by l
   int32_t* rl = nullptr;        // (radial index, first density of its m-block)
@@ -305,9 +306,28 @@ struct pfw_ctx {
   bool timing = false;
   hipEvent_t ev[pfw::kStages + 1] = {};
   double stage_ms[pfw::kStages] = {};
+  // captured steps: the launches of one fresh (no-skin) pair or ACE step as an
+  // instantiated HIP graph, replayed while only positions change
+  bool graphs = false;
+  hipGraphExec_t graph_exec = nullptr;
+  int graph_kind = -1, graph_precision = -1;
+  uint32_t graph_mask = 0;
+  int64_t graph_captures = 0, graph_replays = 0;
+  bool graph_warm = false;  // an eager step has run since the last reset (buffers sized)
+  // launch-shape knobs (0 = the built-in choice): ACE sites per wavefront,
+  // ACE radial indices per density tile, ACE product wavefronts per
+  // workgroup, most lanes per atom in the pair kernel
+  int tune[4] = {0, 0, 0, 0};
 };
 
 namespace pfw {
+// capi.hip: drop the captured step (every entry point that may change a
+// buffer, a shape or a model calls it)
+void graph_reset(pfw_ctx* ctx);
+// Issue the launches of `issue` (stream work only: kernels and copies, no
+// host synchronisation, no memset) on ctx->stream; with graphs on, through
+// the graph captured for (kind, precision, mask), capturing it first.
+void graph_issue(pfw_ctx* ctx, int kind, int precision, uint32_t mask, void (*issue)(pfw_ctx*, void*), void* arg);
 // nl.hip
 void nl_configure(pfw_ctx* ctx, double cutoff, double skin);
 void batch_reduce(pfw_ctx* ctx, const double* records, int width, bool cells, int kind);
@@ -373,6 +393,7 @@ void verlet_free(pfw_ctx* ctx);
 void verlet_pair_run(pfw_ctx* ctx, uint32_t mask, int precision);
 // ace.hip
 void ace_set_model(pfw_ctx* ctx, const pfw_ace_model* m);
+void ace_group_densities(pfw_ctx* ctx, int rchunk);
 void ace_set_parameters(pfw_ctx* ctx, const double* WB, const double* Wpair);
 void ace_run(pfw_ctx* ctx, uint32_t mask, int precision);
 void ace_free(pfw_ctx* ctx);

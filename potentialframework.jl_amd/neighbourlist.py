@@ -147,6 +147,34 @@ class Workspace:
                       ptr(R), ctypes.c_int64(m), ptr(count, ctypes.c_int64))
         return i, j, S, R
 
+    # -- captured steps and launch knobs ----------------------------------------------
+    def set_graphs(self, enabled=True):
+        """Replay fresh pair/ACE steps from a captured HIP graph while only the
+        positions change (``pfw_set_graphs``)."""
+        self.ctx.call("pfw_set_graphs", int(bool(enabled)))
+
+    def graph_info(self):
+        out = np.zeros(3, np.int64)
+        self.ctx.call("pfw_graph_info", ptr(out, ctypes.c_int64))
+        return dict(captures=int(out[0]), replays=int(out[1]), held=bool(out[2]))
+
+    TUNING_KEYS = ("ace_sites_per_wave", "ace_radial_chunk", "ace_product_waves", "pair_lanes_per_atom")
+
+    def set_tuning(self, **knobs):
+        """Launch-shape knobs (``pfw_set_tuning``); 0 selects the built-in choice."""
+        cur = self.get_tuning()
+        for k, v in knobs.items():
+            if k not in self.TUNING_KEYS:
+                raise ArgumentError(f"unknown tuning knob {k!r}")
+            cur[k] = int(v)
+        vals = np.array([cur[k] for k in self.TUNING_KEYS], np.int32)
+        self.ctx.call("pfw_set_tuning", ptr(vals, ctypes.c_int32), len(vals))
+
+    def get_tuning(self):
+        vals = np.zeros(len(self.TUNING_KEYS), np.int32)
+        self.ctx.call("pfw_get_tuning", ptr(vals, ctypes.c_int32), len(vals))
+        return dict(zip(self.TUNING_KEYS, (int(v) for v in vals)))
+
     # -- pair-set audit ---------------------------------------------------------------
     def set_audit(self, enabled=True):
         """Record, for every later pair or ACE step, per-atom digests of the directed
