@@ -228,6 +228,18 @@ lib.oracle_ace_evaluate_scheduled.argtypes = [ctypes.c_int64, ctypes.c_void_p, c
 lib.oracle_ace_evaluate_scheduled.restype = ctypes.c_int
 
 
+lib.oracle_ace_set_input_rounding.argtypes = [ctypes.c_int]
+lib.oracle_ace_set_input_rounding.restype = None
+ROUNDING = {"none": 0, "bfloat16": 1, "float16": 2, "float32": 3}
+
+
+def set_input_rounding(kind):
+    """Precision study: round R, R', Y and grad Y to ``kind`` ("none",
+    "bfloat16", "float16", "float32") before the density contraction and the
+    pullback in every later evaluation (FP64 otherwise)."""
+    lib.oracle_ace_set_input_rounding(ROUNDING[kind])
+
+
 def ace_evaluate(X, species, H, pbc, model, threads=1, mask=7, coloured=False):
     """FP64 (E, F (n,3), W (3x3), site energies (n)) of the resolved model. With
     threads > 1 and ``coloured`` the sites run phase by phase over the greedy

@@ -11,6 +11,7 @@
 // Environments are the directed snapshot of the neighbour-list oracle grouped
 // by neighbour species in traversal order (src/submodules/NeighbourList/environments.jl:42-102).
 #include <algorithm>
+#include <cstdint>
 #include <array>
 #include <cmath>
 #include <cstring>
@@ -266,6 +267,33 @@ static double projected_product(const int* idx, int nu, const double* U, const d
 }
 
 // serial.jl:260-310 with cpu.jl:214-236 for the readout and gradients.
+// Precision study: 0 keeps R, R', Y and grad Y in FP64; 1, 2 and 3 round each
+// of them to bfloat16, binary16 or binary32 (nearest even) before the density
+// contraction and the pullback, as an MFMA with those input types would see
+// them; products, sums and everything else stay FP64.
+static int g_input_rounding = 0;
+
+static double round_to(double v, int mode) {
+  if (mode == 3) return (double)(float)v;
+  if (mode == 2) {  // binary16: 11 significant bits, exponent range ample for these values
+    if (v == 0 || !std::isfinite(v)) return v;
+    int ex;
+    std::frexp(v, &ex);
+    const double q = std::ldexp(1.0, ex - 11);
+    return std::nearbyint(v / q) * q;
+  }
+  if (mode == 1) {  // bfloat16: FP32 with the low 16 mantissa bits rounded off (nearest even)
+    float f = (float)v;
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    u &= 0xffff0000u;
+    std::memcpy(&f, &u, 4);
+    return (double)f;
+  }
+  return v;
+}
+
 static void site(const OracleAceModel& m, int iz, const std::vector<Edge>& edges, bool grads, SiteOut& out) {
   int ne = (int)edges.size(), nr = m.radial.nr, nq = m.pair.nr, ny = (m.lmax + 1) * (m.lmax + 1);
   std::vector<double> rs(ne), R(ne * nr), dR(ne * nr), Q(ne * nq), dQ(ne * nq), Y(ne * ny), dY(ne * ny * 3);
@@ -277,6 +305,15 @@ static void site(const OracleAceModel& m, int iz, const std::vector<Edge>& edges
     radial_ed(m.radial, ch, rs[e], &R[e * nr], &dR[e * nr]);
     if (nq) radial_ed(m.pair, ch, rs[e], &Q[e * nq], &dQ[e * nq]);
     angular(m.lmax, d, &Y[e * ny], &dY[e * ny * 3], m.solid);
+    if (g_input_rounding) {
+      const int md = g_input_rounding;
+      for (int k = 0; k < nr; ++k) {
+        R[e * nr + k] = round_to(R[e * nr + k], md);
+        dR[e * nr + k] = round_to(dR[e * nr + k], md);
+      }
+      for (int k = 0; k < ny; ++k) Y[e * ny + k] = round_to(Y[e * ny + k], md);
+      for (int k = 0; k < 3 * ny; ++k) dY[e * ny * 3 + k] = round_to(dY[e * ny * 3 + k], md);
+    }
   }
   // restricted orders: densities split by neighbour species into U (others)
   // and V (projected species); the pair term of an edge counts as order 1
@@ -341,6 +378,10 @@ static void site(const OracleAceModel& m, int iz, const std::vector<Edge>& edges
 }  // namespace oracle
 
 using namespace oracle;
+
+// Sets the precision-study rounding of site() (0 none, 1 bfloat16, 2 binary16,
+// 3 binary32); not thread-safe against a running evaluation.
+extern "C" void oracle_ace_set_input_rounding(int mode) { g_input_rounding = (mode >= 0 && mode <= 3) ? mode : 0; }
 
 static int ace_fail(char* err, int errlen, const char* msg, int code) {
   if (err && errlen > 0) {

@@ -347,3 +347,29 @@ def test_coloured_site_execution_matches_serial(synthetic):
     np.testing.assert_allclose(Fc, F, atol=1e-11)
     np.testing.assert_allclose(Wc, W, atol=1e-10 * abs(E))
     np.testing.assert_array_equal(sitec, site)
+
+
+def test_low_precision_input_study(synthetic):
+    """Precision study for MFMA input types: rounding R, R', Y and grad Y to
+    binary32 keeps the forces of the synthetic BCC cell within 1e-6 (RMS) of
+    FP64; bfloat16 inputs miss the 1e-5 eV/A force target by orders of
+    magnitude (and binary16 by two), so a BF16/FP16-input contraction cannot
+    meet the accuracy target. The rounding leaves the FP64 path untouched."""
+    from oracle.ace import set_input_rounding
+
+    _, m = synthetic
+    X, H, pbc, sp = small_bcc(seed=9)
+    E, F, W, _ = ace_evaluate(X, sp, H, pbc, m)
+    rms = {}
+    try:
+        for kind in ("float32", "float16", "bfloat16"):
+            set_input_rounding(kind)
+            _, Fk, _, _ = ace_evaluate(X, sp, H, pbc, m)
+            rms[kind] = float(np.sqrt(np.mean((Fk - F) ** 2)))
+    finally:
+        set_input_rounding("none")
+    assert 0 < rms["float32"] < 1e-6
+    assert rms["float16"] > 1e-4 and rms["bfloat16"] > 1e-3
+    assert rms["bfloat16"] > 4 * rms["float16"]  # 8 against 11 significant bits
+    E2, F2, _, _ = ace_evaluate(X, sp, H, pbc, m)
+    assert E2 == E and np.array_equal(F2, F)
