@@ -111,7 +111,7 @@ int pfw_pair_evaluate(pfw_ctx* ctx, uint32_t request_mask, int precision, double
 /* Canonical directed neighbour set of the current system, in original order:
  * for every directed image (i -> j, S) with R = x_j - x_i + H S strictly inside
  * the cutoff, excluding (i, i, 0). Replaces neighbour_list / materialize /
- * visit_directed_images! (src/submodules/NeighbourList/interface.jl:95-171).
+ * visit_directed_images! (src/submodules/NeighbourList/interface.jl:47-143).
  * i, j are 0-based; S is 3 ints per pair; R is 3 doubles per pair (either may
  * be NULL). If cap is too small, *count receives the size and PFW_CAPACITY is
  * returned. Pairs are ordered by (i, then traversal). */

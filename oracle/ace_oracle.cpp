@@ -111,7 +111,8 @@ static void envelope(int kind, const double* e, double r, double x, double dx, d
   }
 }
 
-// resolved.jl:228-250 for channel ch of family f.
+// radial_ed (serial.jl:19-42) over radial_values / envelope_derivative
+// (resolved.jl:184-209) for channel ch of family f.
 static void radial_ed(const Family& f, int ch, double r, double* v, double* d) {
   const double* t = f.tf + 8 * ch;
   int p = (int)t[0], q = (int)t[1];
@@ -266,7 +267,6 @@ static double projected_product(const int* idx, int nu, const double* U, const d
   return value;
 }
 
-// serial.jl:260-310 with cpu.jl:214-236 for the readout and gradients.
 // Precision study: 0 keeps R, R', Y and grad Y in FP64; 1, 2 and 3 round each
 // of them to bfloat16, binary16 or binary32 (nearest even) before the density
 // contraction and the pullback, as an MFMA with those input types would see
@@ -294,6 +294,8 @@ static double round_to(double v, int mode) {
   return v;
 }
 
+// site_data / site_reverse (serial.jl:51-101) with cpu.jl:214-236 for the
+// readout and gradients.
 static void site(const OracleAceModel& m, int iz, const std::vector<Edge>& edges, bool grads, SiteOut& out) {
   int ne = (int)edges.size(), nr = m.radial.nr, nq = m.pair.nr, ny = (m.lmax + 1) * (m.lmax + 1);
   std::vector<double> rs(ne), R(ne * nr), dR(ne * nr), Q(ne * nq), dQ(ne * nq), Y(ne * ny), dY(ne * ny * 3);

@@ -2,7 +2,7 @@
 (test infrastructure only).
 
 * ``min_image_oracle``: O(N^2) minimum-image pair sum with a deliberately different
-  wrapping implementation (test/fixtures.jl:70-94).
+  wrapping implementation (test/fixtures.jl:28-51).
 * ``image_oracle``: exhaustive image sum over a fixed oversized lattice cube, no
   reciprocal bounds, sorting or cell stencil (test/periodic_images.jl:3-28).
 

@@ -58,7 +58,8 @@ def _geom(X, H, pbc):
 
 def nl_pairs(X, H, pbc, cutoff, bruteforce_threshold=32):
     """Directed neighbour pairs (i, j, S, R) in original order, as
-    ``neighbour_list`` (src/submodules/NeighbourList/interface.jl:95-124) emits them.
+    ``neighbour_list`` (src/submodules/NeighbourList/interface.jl:47-83, through
+    ``visit_directed_images!`` :116-143) emits them.
     ``X`` is (n, 3) in Å; ``H`` is 3x3 with lattice vectors in COLUMNS."""
     X, Hc, pb = _geom(X, H, pbc)
     n = X.shape[0]

@@ -316,7 +316,7 @@ void prepare(Prepared& w, int64_t n, const double* X, const double* H, const boo
   if (do_schedule) schedule(w);
 }
 
-// interface.jl:128-142
+// image_windings! (interface.jl:100-114)
 static void windings(const Prepared& w, const double* X, const double* H,
                      std::vector<std::array<int64_t, 3>>& out) {
   double recip[9];

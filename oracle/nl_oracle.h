@@ -62,7 +62,7 @@ struct DirectedPair {
   int64_t S[3];
   double R[3];
 };
-// interface.jl:128-171: every directed image in original order.
+// interface.jl:116-143: every directed image in original order.
 void directed_pairs(const Prepared& w, const double* X, const double* H,
                     std::vector<DirectedPair>& out);
 

@@ -415,7 +415,7 @@ def _toml_table(d, path, out):
 
 
 def save_calculator(path, calc, parameters=None, state=None):
-    """``save_calculator(path, calc; parameters, state)`` (persistence.jl:147-172):
+    """``save_calculator(path, calc; parameters, state)`` (persistence.jl:147-167):
     a self-contained versioned TOML checkpoint of the resolved basis, explicit
     parameters and state; written to a temporary file and moved into place."""
     import os
@@ -586,7 +586,7 @@ ACECalculator.evaluate_batch = _evaluate_batch
 
 
 def ace_evaluate(X, model, ps, species, H, pbc, precision="float64", calculator=None):
-    """Numeric ``ace_evaluate(X, model, ps, env, H)`` (serial.jl:337-370) on the GPU:
+    """Numeric ``ace_evaluate(X, model, ps, env, H)`` (serial.jl:132-161) on the GPU:
     X (n, 3) Å, species labels, H columns = lattice vectors. Returns E, F (3 x n), W."""
     from .system import System
     calc = calculator or ACECalculator(model, ps, precision=precision)

@@ -4,9 +4,9 @@
 // force pullback. Neighbour forces are gathered over reverse edges, so no
 // global atomics are used anywhere.
 //
-// Reference: site_data / site_reverse (src/submodules/ACE/serial.jl:260-310),
+// Reference: site_data / site_reverse (src/submodules/ACE/serial.jl:51-101),
 // NI.evaluate_site! (src/submodules/ACE/cpu.jl:162-237), the radial spline
-// and envelopes (src/submodules/ACE/resolved.jl:156-250), the Y_lm recurrence
+// and envelopes (src/submodules/ACE/resolved.jl:156-209), the Y_lm recurrence
 // (src/submodules/ACE/angular_blocks.jl:25-81), unrestricted_products!
 // (src/submodules/ACE/projection.jl:39-56), reduce_site!
 // (src/submodules/NeighbourList/execution.jl:105-122); the KA kernels being
@@ -93,7 +93,7 @@ __device__ inline T ipow(T x, int n) {
 __host__ __device__ inline size_t align16(size_t b) { return (b + 15) & ~(size_t)15; }
 
 // Agnesi coordinate x(r), dx/dr, envelope and its r-derivative, spline
-// interval and local coordinate (resolved.jl:156-250).
+// interval and local coordinate (resolved.jl:156-209).
 template <typename T>
 struct RadialPoint {
   int k;
@@ -674,7 +674,7 @@ __global__ __launch_bounds__(64) void k_ace_pull(AceArgs<T> a) {
   }
 }
 
-// Directed environments (interface.jl:144-171 + environments.jl:357-391) in
+// Directed environments (interface.jl:116-143 + NeighbourList/environments.jl:42-95) in
 // fixed-capacity rows: site i's edges occupy slots [i ecap, i ecap + ecount[i]).
 // pass 0 only counts (to size ecap); pass 1 writes neighbour, stencil entry and
 // exact R, flags status[2] when a site has more than ecap edges, and records
@@ -1050,7 +1050,7 @@ __global__ void k_ace_scatter_sites(int64_t n, const int32_t* __restrict__ perm,
 // with Adot_a = sum_e [R'_ra (r^ . delta_e) Y_ma + R_ra (grad Y_ma . delta_e)],
 // the JVP of the site basis along the edge directions (the reference's
 // ace_vjp_impl / scalar_site_product, ext/PotentialFrameWorkForwardDiffExt.jl:63-109,
-// and site_parameter_gradient, src/submodules/ACE/serial.jl:312-330).
+// and site_parameter_gradient, src/submodules/ACE/serial.jl:103-121).
 // Environments (radials, Y_lm) are recomputed, nothing per edge is stored.
 // Each workgroup walks sites with a grid stride and accumulates its partial
 // gradient in LDS; a second kernel sums the workgroup partials in FP64.

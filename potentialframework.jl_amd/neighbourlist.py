@@ -5,7 +5,7 @@
   gpu.jl:61-116): reusable storage passed as ``neighbourlist=`` to the
   calculators. It owns one device context; do not share it between concurrent
   calls (README.md:217-219).
-* ``neighbour_list`` / ``neighbours`` mirror src/submodules/NeighbourList/interface.jl:29-179:
+* ``neighbour_list`` / ``neighbours`` mirror src/submodules/NeighbourList/interface.jl:1-151:
   a materialised directed snapshot in original atom order with integer shifts
   relative to the original (unwrapped) coordinates.
 """
@@ -227,7 +227,7 @@ def pair_set_digest(n, i, j, S):
 
 
 class NeighbourSnapshot:
-    """Materialised directed neighbour snapshot (interface.jl:29-36)."""
+    """Materialised directed neighbour snapshot (interface.jl:1-8)."""
 
     def __init__(self, n, i, j, S, R):
         order = np.lexsort((S[:, 2], S[:, 1], S[:, 0], j, i)) if len(i) else np.zeros(0, int)
@@ -239,7 +239,7 @@ class NeighbourSnapshot:
 
 
 def neighbours(snapshot, i):
-    """``(js, Rs, Ss)`` for atom ``i`` (0-based), as independent arrays (interface.jl:173-179)."""
+    """``(js, Rs, Ss)`` for atom ``i`` (0-based), as independent arrays (interface.jl:145-151)."""
     if not 0 <= i < len(snapshot.indices):
         raise IndexError(i)
     return snapshot.indices[i].copy(), snapshot.displacements[i].copy(), snapshot.shifts[i].copy()
@@ -249,7 +249,7 @@ def neighbour_list(X, cutoff, cell=None, pbc=None, *, workspace=None, device=0):
     """Directed neighbour snapshot on the GPU.
 
     ``neighbour_list(X, cutoff, cell_rows, pbc)`` takes a 3x3 cell with lattice
-    vectors in ROWS, like the reference's raw-array API (interface.jl:56-93);
+    vectors in ROWS, like the reference's raw-array API (interface.jl:28-65);
     ``neighbour_list(system, cutoff)`` takes a ``System``. For raw input
     ``Rs[k] = X[js[k]] - X[i] + cell.T @ Ss[k]``. All images strictly inside the
     cutoff are included, including both +/- self images; only the zero-shift

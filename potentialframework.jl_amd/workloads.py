@@ -17,7 +17,7 @@
   0.085 atoms/Å^3 with uniformly placed atoms rejected below 1.8 Å from any
   periodic image of an accepted atom, 2 species 50/50 (the 1,024-configuration
   fitting-set case).
-* ``fixture``: the reference tests' deterministic scatter (test/fixtures.jl:60-67).
+* ``fixture``: the reference tests' deterministic scatter (test/fixtures.jl:17-27).
 
 Positions are (n, 3) arrays in Å; cells have lattice vectors in columns.
 """

@@ -2,7 +2,7 @@
 
 Each check is independent of the oracle's own formulas: the exhaustive image
 sum and the O(N^2) minimum-image oracle restated from the reference's tests
-(test/periodic_images.jl:3-28, test/fixtures.jl:70-94), closed forms from the
+(test/periodic_images.jl:3-28, test/fixtures.jl:28-51), closed forms from the
 reference's expectations (test/runtests.jl:93-97, periodic_images.jl:96-108),
 finite differences and invariances.
 """
