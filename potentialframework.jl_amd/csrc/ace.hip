@@ -20,6 +20,9 @@
 // again; Y and grad Y live in registers (compile-time lmax), the radial rows
 // are recomputed, and g = r^ sum_r R'_r T_r + sum_m U_m grad Y_m.
 #include <algorithm>
+#include <cstring>
+#include <array>
+#include <map>
 #include <cmath>
 #include <vector>
 
@@ -28,7 +31,7 @@
 namespace pfw {
 
 constexpr int kStride = kWave + 1;
-constexpr int kRadialChunk = 40;  // radial indices per density-kernel tile
+constexpr int kRadialChunk = 20;  // radial indices per density-kernel tile
 
 template <typename T>
 struct FamArgs {
@@ -1610,6 +1613,7 @@ void ace_set_model(pfw_ctx* ctx, const pfw_ace_model* m) {
     }
   }
   a.prod = upload(prod, st);
+  a.h_prod = prod;
   std::vector<int32_t> copyb;
   std::vector<ProductSlot> slots = schedule_products(factors, m->ndens, copyb, a.order_step);
   a.nsteps = (int)(slots.size() / kWave);
@@ -2926,6 +2930,448 @@ void ace_parameter_sets(pfw_ctx* ctx, int precision, int64_t nsets, const double
   PFW_HIP(hipMemcpy(&h_bad, bad, sizeof(int32_t), hipMemcpyDeviceToHost));
   if (h_bad) throw Error(PFW_DOMAIN, "nonfinite ACE output (coincident atoms or singular envelope)");
   PFW_HIP(hipMemcpy(out, dout, (size_t)rows * nsets * sizeof(double), hipMemcpyDeviceToHost));
+}
+
+// ---- the linear design by per-site Jacobians (design_matrix_impl,
+// src/submodules/ACE/fitting.jl:18-63, forms each site's Jacobian once). For
+// the unprojected basis, the column of basis function k and centre species z
+// holds the energy, forces and virial of B_k summed over the sites of species
+// z. Per site: A (from k_ace_dens), G = dB/dA at the model's nonzero (k, d)
+// entries (each a sum of product-rule terms), B; per edge e and basis
+// function k, g_e^k = sum_d G_kd dA_d/dR_e with dA_d/dR_e = R'_r Y_m r^ +
+// R_r grad Y_m (r, m of density d). The forces gather g over each atom's
+// edges and their reverses, as the step's force gather does, into the column
+// block of the source site's species.
+
+template <typename T>
+struct DesignArgs {
+  int nb, nG;
+  const int32_t* row;   // nb + 1: entries of each basis row
+  const int32_t* d;     // nG: density of each entry
+  const int32_t* term;  // nG + 1: terms of each entry
+  const int32_t* f;     // 3 per term: the other factors (-1: none)
+  const T* c;           // per term: the coupling value (times the basis scale)
+  const int32_t* brow;  // nb + 1: products of each basis row
+  const int32_t* bprod;
+  const T* bc;
+  T* Bsite;  // n x nb
+  T* gbuf;   // (n ecap) x nb x 3
+};
+
+template <typename T>
+struct V3 {
+  T x, y, z;
+};
+
+// Byte offsets of k_design_site's LDS regions (CE edges per chunk).
+struct DesignLds {
+  size_t fl, A, G, yv, yg, rv, rd, eg, rp, ech, D, total;
+};
+template <typename T>
+__host__ __device__ inline DesignLds design_lds(int ny, int ndens, int nG, int nr, int ce) {
+  DesignLds o;
+  size_t b = 0;
+  o.fl = b;
+  b = align16(b + sizeof(T) * ny);
+  o.A = b;
+  b = align16(b + sizeof(T) * ndens);
+  o.G = b;
+  b = align16(b + sizeof(T) * nG);
+  o.yv = b;
+  b = align16(b + sizeof(T) * ce * ny);
+  o.yg = b;
+  b = align16(b + sizeof(T) * ce * 3 * ny);
+  o.rv = b;
+  b = align16(b + sizeof(T) * ce * nr);
+  o.rd = b;
+  b = align16(b + sizeof(T) * ce * nr);
+  o.eg = b;
+  b = align16(b + sizeof(T) * ce * 4);
+  o.rp = b;
+  b = align16(b + sizeof(RadialPoint<T>) * ce);
+  o.ech = b;
+  b = align16(b + sizeof(int) * ce);
+  o.D = b;
+  b = align16(b + sizeof(V3<T>) * ce * ndens);
+  o.total = b;
+  return o;
+}
+
+template <typename T, int L, int CE>
+__global__ __launch_bounds__(64) void k_design_site(AceArgs<T> a, DesignArgs<T> t) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int NY = (L + 1) * (L + 1);
+  const int nr = a.rad.nr;
+  const DesignLds o = design_lds<T>(NY, a.ndens, t.nG, nr, CE);
+  T* Fl = (T*)(smem + o.fl);
+  T* A = (T*)(smem + o.A);
+  T* G = (T*)(smem + o.G);
+  T* Yv = (T*)(smem + o.yv);
+  T* Yg = (T*)(smem + o.yg);
+  T* Rv = (T*)(smem + o.rv);
+  T* Rd = (T*)(smem + o.rd);
+  T* Eg = (T*)(smem + o.eg);
+  RadialPoint<T>* Rp = (RadialPoint<T>*)(smem + o.rp);
+  int* Ech = (int*)(smem + o.ech);
+  V3<T>* D = (V3<T>*)(smem + o.D);
+  const int lane = threadIdx.x;
+  const int64_t i = blockIdx.x;
+  fill_fl<T, L>(Fl);
+  for (int k = lane; k < a.ndens; k += kWave) A[k] = a.Abuf[i * a.dstride + k];
+  __syncthreads();
+  for (int g = lane; g < t.nG; g += kWave) {
+    T s = 0;
+    for (int u = t.term[g]; u < t.term[g + 1]; ++u) {
+      T v = t.c[u];
+      for (int q = 0; q < 3; ++q) {
+        const int f = t.f[3 * u + q];
+        if (f >= 0) v *= A[f];
+      }
+      s += v;
+    }
+    G[g] = s;
+  }
+  for (int k = lane; k < t.nb; k += kWave) {
+    T s = 0;
+    for (int u = t.brow[k]; u < t.brow[k + 1]; ++u) {
+      const int p = t.bprod[u];
+      T v = t.bc[u];
+      for (int q = 0; q < 4; ++q) {
+        const int f = a.prod[4 * p + q];
+        if (f >= 0) v *= A[f];
+      }
+      s += v;
+    }
+    t.Bsite[i * t.nb + k] = s;
+  }
+  __syncthreads();
+  const int iz = a.sps[i];
+  const int ne = a.ecount[i];
+  const int64_t e0 = i * (int64_t)a.ecap;
+  for (int c0 = 0; c0 < ne; c0 += CE) {
+    const int ce = min(CE, ne - c0);
+    if (lane < ce) {  // the edge's geometry, radial point and harmonics with gradients
+      const double* R = a.eR + 3 * (e0 + c0 + lane);
+      const T dx = (T)R[0], dy = (T)R[1], dz = (T)R[2];
+      const T r = (T)sqrt(R[0] * R[0] + R[1] * R[1] + R[2] * R[2]);
+      const T ri = (T)1 / r;
+      Eg[4 * lane] = dx * ri;
+      Eg[4 * lane + 1] = dy * ri;
+      Eg[4 * lane + 2] = dz * ri;
+      const int ch = iz + a.nz * a.sps[a.enb[e0 + c0 + lane]];
+      Ech[lane] = ch;
+      Rp[lane] = radial_point(a.rad, ch, r);
+      T* yv = Yv + lane * NY;
+      T* yg = Yg + lane * 3 * NY;
+      ylm_emit<T, L>(dx, dy, dz, Fl, a.solid, [&](int m, const Jet<T>& j) {
+        yv[m] = j.v;
+        yg[3 * m] = j.g0;
+        yg[3 * m + 1] = j.g1;
+        yg[3 * m + 2] = j.g2;
+      });
+    }
+    __syncthreads();
+    for (int u = lane; u < ce * nr; u += kWave) {
+      const int e = u / nr, n = u - e * nr;
+      T v, d;
+      radial_value(a.rad, Ech[e], Rp[e], n, v, d);
+      Rv[e * nr + n] = v;
+      Rd[e * nr + n] = d;
+    }
+    __syncthreads();
+    for (int u = lane; u < ce * a.ndens; u += kWave) {
+      const int e = u / a.ndens, k = u - e * a.ndens;
+      const int rr = a.dens[2 * k], mm = a.dens[2 * k + 1];
+      const T s = Rd[e * nr + rr] * Yv[e * NY + mm], v = Rv[e * nr + rr];
+      const T* yg = Yg + (e * NY + mm) * 3;
+      D[e * a.ndens + k] = V3<T>{s * Eg[4 * e] + v * yg[0], s * Eg[4 * e + 1] + v * yg[1], s * Eg[4 * e + 2] + v * yg[2]};
+    }
+    __syncthreads();
+    for (int k = lane; k < t.nb; k += kWave) {
+      T acc[CE][3];
+#pragma unroll
+      for (int e = 0; e < CE; ++e) acc[e][0] = acc[e][1] = acc[e][2] = 0;
+      for (int g = t.row[k]; g < t.row[k + 1]; ++g) {
+        const T gv = G[g];
+        const int d = t.d[g];
+#pragma unroll
+        for (int e = 0; e < CE; ++e) {  // edges past ce read in-bounds stale tiles; their sums are not written
+          const V3<T> q = D[e * a.ndens + d];
+          acc[e][0] += gv * q.x;
+          acc[e][1] += gv * q.y;
+          acc[e][2] += gv * q.z;
+        }
+      }
+#pragma unroll
+      for (int e = 0; e < CE; ++e)
+        if (e < ce) {
+          T* out = t.gbuf + ((e0 + c0 + e) * t.nb + k) * 3;
+          out[0] = acc[e][0];
+          out[1] = acc[e][1];
+          out[2] = acc[e][2];
+        }
+    }
+    __syncthreads();
+  }
+}
+
+// Force rows of every atom: one wavefront per (sorted) atom, lanes over basis
+// functions; F_a^{k, z_a} += sum_out g_e^k and, for each out-edge's reverse
+// (an edge of neighbour site j), F_a^{k, z_j} -= g_rev^k, in edge order
+// (deterministic). Also the atom's own-site virial -sum_out R_e (x) g_e^k.
+template <typename T>
+__global__ __launch_bounds__(64) void k_design_gather(AceArgs<T> a, DesignArgs<T> t, const int32_t* __restrict__ erev,
+                                                      const int32_t* __restrict__ atom_cfg, double* __restrict__ Wpart,
+                                                      double* __restrict__ out, int64_t ncols) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  double* acc = (double*)smem;  // nz x 3 per lane: acc[(lane * nz + z) * 3 + c]
+  const int lane = threadIdx.x;
+  const int64_t i = blockIdx.x;
+  const int nz = a.nz, nb = t.nb;
+  const int zi = a.sps[i];
+  const int ne = a.ecount[i];
+  const int64_t e0 = i * (int64_t)a.ecap;
+  const int64_t o = a.perm[i];
+  const int64_t row0 = 10 * (int64_t)atom_cfg[o] + 1 + 3 * o;
+  double* my = acc + (size_t)lane * nz * 3;
+  for (int k = lane; k < nb; k += kWave) {
+    for (int q = 0; q < 3 * nz; ++q) my[q] = 0;
+    double w[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double f0 = 0, f1 = 0, f2 = 0;
+    for (int e = 0; e < ne; ++e) {
+      const int64_t s = e0 + e;
+      const T* g = t.gbuf + (s * nb + k) * 3;
+      const double g0 = (double)g[0], g1 = (double)g[1], g2 = (double)g[2];
+      f0 += g0;
+      f1 += g1;
+      f2 += g2;
+      const double* R = a.eR + 3 * s;
+      for (int c = 0; c < 3; ++c) {
+        w[c] -= R[c] * g0;
+        w[c + 3] -= R[c] * g1;
+        w[c + 6] -= R[c] * g2;
+      }
+      const int r = erev[s];
+      if (r >= 0) {
+        const int zj = a.sps[a.enb[s]];
+        const T* h = t.gbuf + ((int64_t)r * nb + k) * 3;
+        my[3 * zj] -= (double)h[0];
+        my[3 * zj + 1] -= (double)h[1];
+        my[3 * zj + 2] -= (double)h[2];
+      }
+    }
+    my[3 * zi] += f0;
+    my[3 * zi + 1] += f1;
+    my[3 * zi + 2] += f2;
+    for (int z = 0; z < nz; ++z)
+      for (int c = 0; c < 3; ++c) out[(row0 + c) * ncols + k + (int64_t)nb * z] = my[3 * z + c];
+    for (int c = 0; c < 9; ++c) Wpart[(i * nb + k) * 9 + c] = w[c];
+  }
+}
+
+// Energy and virial rows of every configuration: one thread per (basis
+// function, configuration), the configuration's sorted sites in order.
+template <typename T>
+__global__ void k_design_rows(int nb, int nz, const int32_t* __restrict__ cfg_cell0, const int32_t* __restrict__ start,
+                              const int64_t* __restrict__ atom0, const int32_t* __restrict__ sps,
+                              const T* __restrict__ Bsite, const double* __restrict__ Wpart, double* __restrict__ out,
+                              int64_t ncols) {
+  const int c = blockIdx.x;
+  const int k = blockIdx.y * blockDim.x + threadIdx.x;
+  if (k >= nb) return;
+  const int lo = start[cfg_cell0[c]], hi = start[cfg_cell0[c + 1]];
+  const int64_t erow = 10 * (int64_t)c + 3 * atom0[c], wrow = 10 * (int64_t)c + 3 * atom0[c + 1] + 1;
+  for (int z = 0; z < nz; ++z) {
+    double E = 0, W[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int i = lo; i < hi; ++i) {
+      if (sps[i] != z) continue;
+      E += (double)Bsite[(int64_t)i * nb + k];
+      for (int q = 0; q < 9; ++q) W[q] += Wpart[((int64_t)i * nb + k) * 9 + q];
+    }
+    out[erow * ncols + k + (int64_t)nb * z] = E;
+    for (int q = 0; q < 9; ++q) out[(wrow + q) * ncols + k + (int64_t)nb * z] = W[q];
+  }
+}
+
+template <typename T, int L, int CE>
+static void launch_design_site(pfw_ctx* ctx, const DesignArgs<T>& t, size_t lds) {
+  AceArgs<T> a = make_args<T>(ctx, 0);
+  if (lds > 64 * 1024)
+    PFW_HIP(hipFuncSetAttribute((const void*)k_design_site<T, L, CE>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds));
+  hipLaunchKernelGGL((k_design_site<T, L, CE>), dim3((unsigned)ctx->sys.n), dim3(kWave), lds, ctx->stream, a, t);
+  PFW_HIP(hipGetLastError());
+}
+
+// The densities alone (k_ace_dens, one site per wavefront).
+template <typename T, int L>
+static void launch_dens_only(pfw_ctx* ctx) {
+  AceDev& ad = ctx->ace;
+  AceArgs<T> a = make_args<T>(ctx, 0);
+  constexpr int NY = (L + 1) * (L + 1);
+  const size_t lds = sizeof(T) * ((size_t)NY + (size_t)ad.ndens + (size_t)(ad.rchunk + NY) * kStride);
+  allow_lds(k_ace_dens<T, L, 1>, lds, "the density kernel");
+  hipLaunchKernelGGL((k_ace_dens<T, L, 1>), dim3((unsigned)ctx->sys.n), dim3(kWave), lds, ctx->stream, a);
+  PFW_HIP(hipGetLastError());
+}
+
+template <typename T, int L>
+static void design_run(pfw_ctx* ctx, const std::vector<int32_t>& row, const std::vector<int32_t>& dd,
+                       const std::vector<int32_t>& term, const std::vector<int32_t>& ff, const std::vector<double>& cc,
+                       const std::vector<int32_t>& brow, const std::vector<int32_t>& bprod,
+                       const std::vector<double>& bc, double* out) {
+  AceDev& ad = ctx->ace;
+  SystemBuffers& s = ctx->sys;
+  hipStream_t st = ctx->stream;
+  const int nb = ad.nbasis, nz = ad.nz;
+  const int64_t n = s.n, ncols = (int64_t)nb * nz, rows = 10 * (int64_t)ctx->ncfg + 3 * n;
+  constexpr int NY = (L + 1) * (L + 1);
+  const int nG = (int)dd.size();
+  // the largest edge chunk whose tiles leave room for a few workgroups per CU
+  int ce = 0;
+  size_t lds = 0;
+  for (int c : {8, 4, 2}) {
+    lds = design_lds<T>(NY, ad.ndens, nG, ad.radial.nr, c).total;
+    ce = c;
+    if (lds <= 64 * 1024) break;
+  }
+  if (lds > kMaxLds) throw Error(PFW_INVALID_ARGUMENT, "ACE model too large for the design kernel's LDS tiles");
+  const double gbytes = (double)n * ad.ecap * nb * 3 * sizeof(T);
+  if (gbytes > 64e9) throw Error(PFW_CAPACITY, "design: the per-edge basis gradients exceed 64 GB; use fewer configurations");
+  DeviceScratch dev;
+  auto up32 = [&](const std::vector<int32_t>& v) {
+    int32_t* p = dev.get<int32_t>(v.size());
+    if (!v.empty()) PFW_HIP(hipMemcpyAsync(p, v.data(), v.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    return p;
+  };
+  auto upT = [&](const std::vector<double>& v) {
+    std::vector<T> w(v.begin(), v.end());
+    T* p = dev.get<T>(w.size());
+    if (!w.empty()) PFW_HIP(hipMemcpyAsync(p, w.data(), w.size() * sizeof(T), hipMemcpyHostToDevice, st));
+    PFW_HIP(hipStreamSynchronize(st));  // w leaves scope
+    return p;
+  };
+  DesignArgs<T> t;
+  t.nb = nb;
+  t.nG = nG;
+  t.row = up32(row);
+  t.d = up32(dd);
+  t.term = up32(term);
+  t.f = up32(ff);
+  t.c = upT(cc);
+  t.brow = up32(brow);
+  t.bprod = up32(bprod);
+  t.bc = upT(bc);
+  t.Bsite = dev.get<T>((size_t)n * nb);
+  t.gbuf = dev.get<T>((size_t)n * ad.ecap * nb * 3);
+  double* Wpart = dev.get<double>((size_t)n * nb * 9);
+  double* dout = dev.get<double>((size_t)rows * ncols);
+  int64_t* atom0 = dev.get<int64_t>(ctx->atom0.size());
+  PFW_HIP(hipMemcpyAsync(atom0, ctx->atom0.data(), ctx->atom0.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
+  launch_dens_only<T, L>(ctx);
+  if (ce == 8)
+    launch_design_site<T, L, 8>(ctx, t, lds);
+  else if (ce == 4)
+    launch_design_site<T, L, 4>(ctx, t, lds);
+  else
+    launch_design_site<T, L, 2>(ctx, t, lds);
+  AceArgs<T> a = make_args<T>(ctx, 0);
+  const size_t glds = sizeof(double) * kWave * 3 * nz;
+  if (glds > 64 * 1024) throw Error(PFW_INVALID_ARGUMENT, "too many species for the design gather");
+  hipLaunchKernelGGL(k_design_gather<T>, dim3((unsigned)n), dim3(kWave), glds, st, a, t, (const int32_t*)ad.erev,
+                     (const int32_t*)s.atom_cfg, Wpart, dout, ncols);
+  PFW_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_design_rows<T>, dim3((unsigned)ctx->ncfg, (unsigned)((nb + 63) / 64)), dim3(64), 0, st, nb, nz,
+                     (const int32_t*)s.cfg_cell0, (const int32_t*)s.start, (const int64_t*)atom0, (const int32_t*)s.sps,
+                     (const T*)t.Bsite, (const double*)Wpart, dout, ncols);
+  PFW_HIP(hipGetLastError());
+  PFW_HIP(hipMemcpyAsync(out, dout, (size_t)rows * ncols * sizeof(double), hipMemcpyDeviceToHost, st));
+  PFW_HIP(hipStreamSynchronize(st));
+}
+
+// The basis columns of the linear design (rows as ace_parameter_sets', row-major
+// rows x nb nz, column k + nb z), by per-site Jacobians in one pass.
+void ace_design(pfw_ctx* ctx, int precision, const double* basis_scale, double* out) {
+  ctx->host_written = false;
+  ctx->have_results = false;
+  AceDev& ad = ctx->ace;
+  if (!ad.ready) throw Error(PFW_STALE, "no ACE model set");
+  if (ad.cutoff != ctx->cutoff)
+    throw Error(PFW_STALE, "the system's cell list was built for another cutoff than the ACE search cutoff");
+  if (ad.proj) throw Error(PFW_INVALID_ARGUMENT, "species-projected bases: use pfw_ace_parameter_sets");
+  if (precision != PFW_FP64 && precision != PFW_MIXED) throw Error(PFW_INVALID_ARGUMENT, "unknown precision");
+  if (!out) throw Error(PFW_INVALID_ARGUMENT, "null output");
+  const int nb = ad.nbasis;
+  SystemBuffers& s = ctx->sys;
+  const int64_t n = s.n, ncols = (int64_t)nb * ad.nz, rows = 10 * (int64_t)ctx->ncfg + 3 * n;
+  if (n == 0 || ncols == 0) {
+    std::memset(out, 0, sizeof(double) * (size_t)rows * ncols);
+    return;
+  }
+  // host tables: the entries (k, d) of dB/dA with their product-rule terms,
+  // and the products of each basis row, scaled by the basis scale (the mask)
+  std::vector<double> scale(nb, 1.0);
+  if (basis_scale)
+    for (int k = 0; k < nb; ++k) scale[k] = basis_scale[k];
+  std::map<std::pair<int, int>, std::vector<std::array<double, 4>>> ent;  // (k, d) -> (c, f0, f1, f2)
+  std::vector<std::vector<std::pair<int, double>>> bp(nb);
+  for (size_t u = 0; u < ad.c_vals.size(); ++u) {
+    const int k = ad.c_rows[u], p = ad.c_cols[u];
+    const double c = ad.c_vals[u] * scale[k];
+    bp[k].push_back({p, c});
+    int fac[4], nu = 0;
+    for (int q = 0; q < 4; ++q)
+      if (ad.h_prod[4 * (size_t)p + q] >= 0) fac[nu++] = ad.h_prod[4 * (size_t)p + q];
+    for (int tt = 0; tt < nu; ++tt) {
+      std::array<double, 4> term = {c, -1, -1, -1};
+      for (int q = 0, w = 1; q < nu; ++q)
+        if (q != tt) term[w++] = fac[q];
+      ent[{k, fac[tt]}].push_back(term);
+    }
+  }
+  std::vector<int32_t> row(nb + 1, 0), dd, term(1, 0), ff, brow(nb + 1, 0), bprod;
+  std::vector<double> cc, bc;
+  for (const auto& kv : ent) {
+    row[kv.first.first + 1]++;
+    dd.push_back(kv.first.second);
+    for (const auto& tm : kv.second) {
+      cc.push_back(tm[0]);
+      for (int q = 1; q < 4; ++q) ff.push_back((int32_t)tm[q]);
+    }
+    term.push_back((int32_t)cc.size());
+  }
+  for (int k = 0; k < nb; ++k) row[k + 1] += row[k];
+  for (int k = 0; k < nb; ++k) {
+    for (const auto& pc : bp[k]) {
+      bprod.push_back(pc.first);
+      bc.push_back(pc.second);
+    }
+    brow[k + 1] = (int32_t)bprod.size();
+  }
+  for (;;) {  // environments with the reverse index, as in a step
+    nl_build(ctx);
+    ace_environment(ctx, 1);
+    PFW_HIP(hipStreamSynchronize(ctx->stream));
+    if (!ace_overflowed(ad)) break;
+  }
+  const int L = ad.lmax <= 2 ? 2 : ad.lmax <= 4 ? 4 : 6;
+#define PFW_DESIGN(T, LL) design_run<T, LL>(ctx, row, dd, term, ff, cc, brow, bprod, bc, out)
+  if (precision == PFW_MIXED) {
+    if (L == 2)
+      PFW_DESIGN(float, 2);
+    else if (L == 4)
+      PFW_DESIGN(float, 4);
+    else
+      PFW_DESIGN(float, 6);
+  } else {
+    if (L == 2)
+      PFW_DESIGN(double, 2);
+    else if (L == 4)
+      PFW_DESIGN(double, 4);
+    else
+      PFW_DESIGN(double, 6);
+  }
+#undef PFW_DESIGN
 }
 
 template <typename T, int L>

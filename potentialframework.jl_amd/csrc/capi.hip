@@ -496,6 +496,13 @@ int pfw_ace_adjoint(pfw_ctx* ctx, int precision, const double* dE, const double*
   });
 }
 
+int pfw_ace_design(pfw_ctx* ctx, int precision, const double* basis_scale, double* out) {
+  return run(ctx, [&] {
+    if (!ctx->have_system) throw Error(PFW_STALE, "no system set");
+    pfw::ace_design(ctx, precision, basis_scale, out);
+  });
+}
+
 int pfw_ace_parameter_sets(pfw_ctx* ctx, int precision, int64_t nsets, const double* WB, const double* Wpair,
                            double* out) {
   return run(ctx, [&] {

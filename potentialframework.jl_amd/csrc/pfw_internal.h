@@ -167,6 +167,7 @@ struct AceDev {
   double* reference = nullptr;
   AceFamilyDev radial, pair;
   int32_t* prod = nullptr;      // nprod x 4, -1 padded
+  std::vector<int32_t> h_prod;  // host copy of prod
   int nsteps = 0;               // product schedule: nsteps x 64 slots
   int order_step[5] = {0, 0, 0, 0, 0};  // steps [order_step[nu - 1], order_step[nu]) hold products of nu factors
   uint4* sched = nullptr;       // per slot: 4 x u16 factors, 4 x u16 accumulator targets
@@ -394,6 +395,7 @@ void verlet_pair_run(pfw_ctx* ctx, uint32_t mask, int precision);
 // ace.hip
 void ace_set_model(pfw_ctx* ctx, const pfw_ace_model* m);
 void ace_group_densities(pfw_ctx* ctx, int rchunk);
+void ace_design(pfw_ctx* ctx, int precision, const double* basis_scale, double* out);
 void ace_set_parameters(pfw_ctx* ctx, const double* WB, const double* Wpair);
 void ace_run(pfw_ctx* ctx, uint32_t mask, int precision);
 void ace_free(pfw_ctx* ctx);

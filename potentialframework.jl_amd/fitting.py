@@ -68,24 +68,40 @@ def design_matrix(calc, systems, fit_pair=True, centers=None):
         w.prepare(systems[0], m.cutoff, chans[0])
     else:
         w.prepare_batch(systems, m.cutoff, chans)
-    ncols = nz * nb + (nz * nq if fit_pair else 0)
+    rows = sum(10 + 3 * len(s) for s in systems)
+    nbasis_cols = nz * nb
+    # unprojected bases: the vec(WB) columns in one pass of per-site Jacobians
+    # (pfw_ace_design); the parameter sets then give only the Wpair columns and
+    # the baseline. Projected bases take every column from parameter sets.
+    jac = m.orders is None
+    nsets_b = 0 if jac else nbasis_cols
+    npair = nz * nq if (fit_pair and nq > 0) else 0
+    nsets = nsets_b + npair + 1
     # parameter sets: the unit readouts (masked as the device model masks them), then the baseline set
-    WB = np.zeros((ncols + 1, nz, nb))   # set k: WB^T (nz x nbasis), i.e. vec(WB) column-major
-    WP = np.zeros((ncols + 1, nz, max(nq, 1)))
-    eye = np.arange(nz * nb)
-    WB.reshape(ncols + 1, -1)[eye, eye] = np.tile(m.mask.astype(float), nz)
-    if fit_pair and nq > 0:
-        k = nz * nb + np.arange(nz * nq)
+    WB = np.zeros((nsets, nz, nb))   # set k: WB^T (nz x nbasis), i.e. vec(WB) column-major
+    WP = np.zeros((nsets, nz, max(nq, 1)))
+    if nsets_b:
+        eye = np.arange(nbasis_cols)
+        WB.reshape(nsets, -1)[eye, eye] = np.tile(m.mask.astype(float), nz)
+    if npair:
+        k = nsets_b + np.arange(npair)
         WP[k, np.repeat(np.arange(nz), nq), np.tile(np.arange(nq), nz)] = 1.0 if m.pair_enabled else 0.0
     if not fit_pair and nq > 0 and m.pair_enabled:  # the current pair term in every set, so it cancels in A
         WP[:] = np.asarray(calc.parameters["Wpair"], float).T[None]
     WP = WP[:, :, :nq] if nq > 0 else None
-    rows = sum(10 + 3 * len(s) for s in systems)
-    out = np.zeros((ncols + 1, rows))  # column-major (rows x sets)
-    w.ctx.call("pfw_ace_parameter_sets", w.mode, ncols + 1, ptr(np.ascontiguousarray(WB)),
+    out = np.zeros((nsets, rows))  # column-major (rows x sets)
+    w.ctx.call("pfw_ace_parameter_sets", w.mode, nsets, ptr(np.ascontiguousarray(WB)),
                ptr(np.ascontiguousarray(WP)) if WP is not None else None, ptr(out))
-    base = out[ncols].copy()
-    A = (out[:ncols] - base[None, :]).T.copy()
+    base = out[nsets - 1].copy()
+    A = np.empty((rows, nbasis_cols + npair))
+    if jac:
+        Ab = np.empty((rows, nbasis_cols))  # row-major, column k + nb z
+        w.ctx.call("pfw_ace_design", w.mode, ptr(np.ascontiguousarray(m.mask, np.float64)), ptr(Ab))
+        A[:, :nbasis_cols] = Ab
+    else:
+        A[:, :nbasis_cols] = (out[:nbasis_cols] - base[None, :]).T
+    if npair:
+        A[:, nbasis_cols:] = (out[nsets_b:nsets_b + npair] - base[None, :]).T
     return {"matrix": A, "baseline": base}
 
 

@@ -152,3 +152,45 @@ def test_masked_invariant_gives_zero_column(pfw, tial, frame):
     for c in (2, 5, nb + 2, nb + 5):
         assert not np.any(d["matrix"][:, c])
     assert np.any(d["matrix"][:, 3])
+
+
+@pytest.mark.parametrize("prec", ["float64", "mixed"])
+def test_jacobian_design_equals_unit_readout_sets(pfw, prec):
+    """pfw_ace_design (one pass of per-site Jacobians) against the columns of
+    the unit-readout parameter sets (a full site pass per column, less the zero
+    set), on a five-configuration batch of the synthetic basis with two
+    invariants masked: the two device formulations share no kernel beyond the
+    environments and densities."""
+    from pfw_amd import _capi
+    from pfw_amd.ace import ELEMENTS
+
+    D = pfw.ace_basis.synthetic_model()
+    model, ps, _ = pfw.load_resolved(D)
+    lab = [ELEMENTS[z - 1] for z in model.atomic_numbers]
+    cfgs, _ = pfw.workloads.fitting_set(ncfg=5, nmin=8, nmax=90, seed=29)
+    systems = [pfw.System(X, H, pb, [lab[s] for s in sp]) for X, H, pb, sp in cfgs]
+    calc = pfw.ACECalculator(model, ps, precision=prec)
+    w = calc.workspace
+    w.prepare_batch(systems, model.cutoff, [model.channels(s.labels) for s in systems])
+    nb, nz, nq = model.nbasis, model.nz, model.pair.nr
+    scale = np.ones(nb)
+    scale[[3, 100]] = 0.0
+    rows = sum(10 + 3 * len(s) for s in systems)
+    got = np.empty((rows, nb * nz))
+    w.ctx.call("pfw_ace_design", w.mode, _capi.ptr(scale), _capi.ptr(got))
+    nsets = nb * nz + 1
+    WB = np.zeros((nsets, nz, nb))
+    WB.reshape(nsets, -1)[np.arange(nb * nz), np.arange(nb * nz)] = np.tile(scale, nz)
+    WP = np.zeros((nsets, nz, nq))
+    out = np.zeros((nsets, rows))
+    w.ctx.call("pfw_ace_parameter_sets", w.mode, nsets, _capi.ptr(WB), _capi.ptr(WP), _capi.ptr(out))
+    want = (out[:-1] - out[-1][None, :]).T
+    tol = 1e-10 if prec == "float64" else 2e-5
+    err = np.abs(got - want).max()
+    print(f"{prec}: max |jacobian - sets| = {err:.3e} of max {np.abs(want).max():.3e}")
+    np.testing.assert_allclose(got, want, atol=tol * max(1.0, np.abs(want).max()), rtol=0)
+    assert not got[:, [3, 100, nb + 3, nb + 100]].any()
+    with pytest.raises(pfw.ArgumentError):  # projected bases take the parameter-set path
+        pc = pfw.ACECalculator(pfw.project_species(model, lab[0], range(1, 3)), ps, precision=prec)
+        pc.workspace.prepare(systems[0], model.cutoff, model.channels(systems[0].labels))
+        pc.workspace.ctx.call("pfw_ace_design", pc.workspace.mode, None, _capi.ptr(np.empty((10 + 3 * len(systems[0]), nb * nz))))
