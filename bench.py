@@ -635,6 +635,18 @@ def profile_run(pfw, lib, _capi, which, prec, steps):
     ACE step; may30: the 30,000-atom May30 step; argon: the 4,000-atom LJ
     step), then one line naming what ran."""
     mode = _capi.PFW_MIXED if prec == "mixed" else _capi.PFW_FP64
+    if which == "design":  # the linear design of the first 64 fitting-set configurations
+        from pfw_amd.ace import ELEMENTS
+        model, ps, _ = pfw.load_resolved(pfw.ace_basis.synthetic_model())
+        lab = [ELEMENTS[z - 1] for z in model.atomic_numbers]
+        cfgs, _ = pfw.workloads.fitting_set()
+        sub = [pfw.System(X, H, pb, [lab[v] for v in sp]) for X, H, pb, sp in cfgs[:64]]
+        calc = pfw.ACECalculator(model, ps, precision=prec)
+        for _ in range(max(1, steps)):
+            d = pfw.design_matrix(calc, sub)
+        print(json.dumps({"profile": which, "precision": prec, "rows": d["matrix"].shape[0],
+                          "columns": d["matrix"].shape[1], "calls": max(1, steps)}), flush=True)
+        return
     if which in ("pair", "argon"):
         if which == "pair":
             X, H, pbc, sp = pfw.workloads.b2_triclinic()
@@ -704,7 +716,7 @@ def main():
                     help="CPU baseline threads (default: physical cores of the affinity mask, at most 16)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a device copy of 256 MiB five times (HBM counter calibration)")
-    ap.add_argument("--profile", choices=("pair", "ace", "may30", "argon"), default=None,
+    ap.add_argument("--profile", choices=("pair", "ace", "may30", "argon", "design"), default=None,
                     help="only run --steps device steps of one workload (for rocprofv3)")
     ap.add_argument("--precision", choices=("mixed", "float64"), default="mixed", help="precision of --profile")
     args = ap.parse_args()
@@ -909,7 +921,7 @@ def main():
             rep = pfw.autotune(tcalc, sys_a, steps=10, warmup=2, cache=os.path.join(td, "tuning.json"))
             tp = pfw.Workspace(precision="mixed")
             repp = pfw.autotune(tp, sys_, p, steps=10, warmup=2, cache=os.path.join(td, "tuning.json"))
-        default = [c for c in rep["candidates"] if c[0] == {"ace_sites_per_wave": 2, "ace_radial_chunk": 40,
+        default = [c for c in rep["candidates"] if c[0] == {"ace_sites_per_wave": 2, "ace_radial_chunk": 20,
                                                              "ace_product_waves": 16} and not c[1]]
         line["autotune"] = {
             "ace_bcc_32k_mixed": {"best": rep["best"], "graphs": rep["graphs"], "ms_per_step": round(rep["ms"], 4),

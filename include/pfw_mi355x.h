@@ -335,7 +335,9 @@ int pfw_ace_parameter_sets(pfw_ctx* ctx, int precision, int64_t nsets, const dou
 
 /* The basis columns of the linear design in one pass, by per-site Jacobians
  * (design_matrix_impl builds each site's Jacobian once, src/submodules/ACE/fitting.jl:18-63):
- * out (rows x nbasis nz, row-major; rows as for pfw_ace_parameter_sets: per
+ * out (rows x ld, row-major, ld >= nbasis nz: the first nbasis nz columns of a
+ * possibly wider matrix; the rest of each row is overwritten with unspecified
+ * values for the caller to fill; rows as for pfw_ace_parameter_sets: per
  * configuration E, vec(F), vec(W)) gets, in column k + nbasis z, the energy,
  * forces and virial of basis function k summed over the sites of centre
  * species z, times basis_scale[k] (the readout mask; NULL = all ones). These
@@ -344,7 +346,7 @@ int pfw_ace_parameter_sets(pfw_ctx* ctx, int precision, int64_t nsets, const dou
  * the Wpair columns). Species-projected bases return PFW_INVALID_ARGUMENT
  * (their columns come from pfw_ace_parameter_sets). precision: PFW_FP64, or
  * PFW_MIXED (FP32 site arithmetic, FP64 sums). */
-int pfw_ace_design(pfw_ctx* ctx, int precision, const double* basis_scale, double* out);
+int pfw_ace_design(pfw_ctx* ctx, int precision, const double* basis_scale, double* out, int64_t ld);
 
 /* ---- equivariant features (ace_features, src/submodules/ACE/features.jl:141-178) ---- */
 enum { PFW_RADIAL_SMOOTH_EXPONENTIAL = 1 };

@@ -449,34 +449,50 @@ end
 
 # ---------------------------------------------------------------- linear design (fitting.py: design_matrix)
 """`design_matrix(X, model, ps, env, H; fit_pair)` (ACE/fitting.jl:18-130) for one
-configuration on the device: rows [E; vec(F); vec(W)], columns vec(WB) then
-vec(Wpair); `matrix` excludes the baseline (the zero readout's outputs, which
-carry the reference energies and, without fit_pair, the current pair term)."""
+configuration on the device (fitting.py: design_matrix): rows [E; vec(F); vec(W)],
+columns vec(WB) then vec(Wpair); `matrix` excludes the baseline (the zero
+readout's outputs: the reference energies and, without fit_pair, the current pair
+term). For an unprojected basis the vec(WB) columns come from one pass of
+per-site Jacobians (pfw_ace_design); the Wpair columns and the baseline, and
+every column of a projected basis, from unit-readout parameter sets."""
 function design_matrix(X::AbstractMatrix, model, ps, env, H::AbstractMatrix; fit_pair=true,
                        c::MI355XACECalculator)
     set_system!(c.w, X, H, periodic_flags(H), Int32.(env.species .- 1), NIm.search_cutoff(model))
     nb, nz = size(ps.WB); nq = size(ps.Wpair, 1)
-    n = length(env.species); ncols = (nb + nq) * nz
-    mask = collect(Bool, model.policy.mask)
-    WB = zeros(nb, nz, ncols + 1); WP = zeros(nq, nz, ncols + 1)
-    for k in 1:nb * nz                                # vec(WB) columns, masked as effective_WB
-        i = mod1(k, nb); z = cld(k, nb)
-        WB[i, z, k] = mask[i] ? 1.0 : 0.0
+    n = length(env.species); rows = 1 + 3n + 9
+    mask = Float64.(collect(Bool, model.policy.mask))
+    jac = isnothing(model.policy.orders)
+    nsb = jac ? 0 : nb * nz                          # basis columns taken from parameter sets
+    npair = fit_pair ? nq * nz : 0
+    nsets = nsb + npair + 1
+    WB = zeros(nb, nz, nsets); WP = zeros(nq, nz, nsets)
+    for k in 1:nsb                                   # vec(WB) columns, masked as effective_WB
+        WB[mod1(k, nb), cld(k, nb), k] = mask[mod1(k, nb)]
     end
     if fit_pair && model.policy.pair
-        for k in 1:nq * nz
-            WP[mod1(k, nq), cld(k, nq), nb * nz + k] = 1.0
+        for k in 1:npair
+            WP[mod1(k, nq), cld(k, nq), nsb + k] = 1.0
         end
     elseif !fit_pair
         # the current pair term (off when the policy disables it), cancelling in the columns
         WP .= reshape(Matrix{Float64}(ps.Wpair) .* (model.policy.pair ? 1.0 : 0.0), nq, nz, 1)
     end
-    out = Matrix{Float64}(undef, 1 + 3n + 9, ncols + 1)
+    out = Matrix{Float64}(undef, rows, nsets)
     check(c.w.ctx, ccall((:pfw_ace_parameter_sets, LIB), Cint,
         (Ptr{Cvoid}, Cint, Int64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
-        c.w.ctx, c.w.mode, ncols + 1, WB, WP, out))
+        c.w.ctx, c.w.mode, nsets, WB, WP, out))
     baseline = out[:, end]
-    (; matrix=out[:, 1:ncols] .- baseline, baseline)
+    A = Matrix{Float64}(undef, rows, nb * nz + npair)
+    if jac
+        At = Matrix{Float64}(undef, nb * nz, rows)     # the row-major block of pfw_ace_design, transposed view
+        check(c.w.ctx, ccall((:pfw_ace_design, LIB), Cint, (Ptr{Cvoid}, Cint, Ptr{Float64}, Ptr{Float64}, Int64),
+            c.w.ctx, c.w.mode, mask, At, nb * nz))
+        A[:, 1:nb*nz] .= transpose(At)
+    else
+        A[:, 1:nb*nz] .= out[:, 1:nb*nz] .- baseline
+    end
+    npair > 0 && (A[:, nb*nz+1:end] .= out[:, nsb+1:nsb+npair] .- baseline)
+    (; matrix=A, baseline)
 end
 
 # ---------------------------------------------------------------- features (features.py)

@@ -92,7 +92,7 @@ SIGNATURES = [
     ("pfw_ace_adjoint", ctypes.c_int, [_P, ctypes.c_int, _D, _D, _D, _D, _D]),
     ("pfw_pair_adjoint", ctypes.c_int, [_P, _D, _D, _D, _D]),
     ("pfw_ace_parameter_sets", ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int64, _D, _D, _D]),
-    ("pfw_ace_design", ctypes.c_int, [_P, ctypes.c_int, _D, _D]),
+    ("pfw_ace_design", ctypes.c_int, [_P, ctypes.c_int, _D, _D, ctypes.c_int64]),
     ("pfw_features", ctypes.c_int, [_P, _P, _D]),
     ("pfw_features_vjp", ctypes.c_int, [_P, _P, _D, _D, _D, _D, _D]),
 ]

@@ -2943,6 +2943,8 @@ void ace_parameter_sets(pfw_ctx* ctx, int precision, int64_t nsets, const double
 // edges and their reverses, as the step's force gather does, into the column
 // block of the source site's species.
 
+constexpr int kDesignThreads = 256;  // four wavefronts per site
+
 template <typename T>
 struct DesignArgs {
   int nb, nG;
@@ -2965,7 +2967,7 @@ struct V3 {
 
 // Byte offsets of k_design_site's LDS regions (CE edges per chunk).
 struct DesignLds {
-  size_t fl, A, G, yv, yg, rv, rd, eg, rp, ech, D, total;
+  size_t fl, A, G, gd, yv, yg, rv, rd, eg, rp, ech, D, total;
 };
 template <typename T>
 __host__ __device__ inline DesignLds design_lds(int ny, int ndens, int nG, int nr, int ce) {
@@ -2977,6 +2979,8 @@ __host__ __device__ inline DesignLds design_lds(int ny, int ndens, int nG, int n
   b = align16(b + sizeof(T) * ndens);
   o.G = b;
   b = align16(b + sizeof(T) * nG);
+  o.gd = b;
+  b = align16(b + sizeof(uint16_t) * nG);
   o.yv = b;
   b = align16(b + sizeof(T) * ce * ny);
   o.yg = b;
@@ -2998,7 +3002,7 @@ __host__ __device__ inline DesignLds design_lds(int ny, int ndens, int nG, int n
 }
 
 template <typename T, int L, int CE>
-__global__ __launch_bounds__(64) void k_design_site(AceArgs<T> a, DesignArgs<T> t) {
+__global__ __launch_bounds__(kDesignThreads) void k_design_site(AceArgs<T> a, DesignArgs<T> t) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int NY = (L + 1) * (L + 1);
   const int nr = a.rad.nr;
@@ -3006,6 +3010,7 @@ __global__ __launch_bounds__(64) void k_design_site(AceArgs<T> a, DesignArgs<T> 
   T* Fl = (T*)(smem + o.fl);
   T* A = (T*)(smem + o.A);
   T* G = (T*)(smem + o.G);
+  uint16_t* Gd = (uint16_t*)(smem + o.gd);
   T* Yv = (T*)(smem + o.yv);
   T* Yg = (T*)(smem + o.yg);
   T* Rv = (T*)(smem + o.rv);
@@ -3017,9 +3022,9 @@ __global__ __launch_bounds__(64) void k_design_site(AceArgs<T> a, DesignArgs<T> 
   const int lane = threadIdx.x;
   const int64_t i = blockIdx.x;
   fill_fl<T, L>(Fl);
-  for (int k = lane; k < a.ndens; k += kWave) A[k] = a.Abuf[i * a.dstride + k];
+  for (int k = lane; k < a.ndens; k += kDesignThreads) A[k] = a.Abuf[i * a.dstride + k];
   __syncthreads();
-  for (int g = lane; g < t.nG; g += kWave) {
+  for (int g = lane; g < t.nG; g += kDesignThreads) {
     T s = 0;
     for (int u = t.term[g]; u < t.term[g + 1]; ++u) {
       T v = t.c[u];
@@ -3030,8 +3035,9 @@ __global__ __launch_bounds__(64) void k_design_site(AceArgs<T> a, DesignArgs<T> 
       s += v;
     }
     G[g] = s;
+    Gd[g] = (uint16_t)t.d[g];
   }
-  for (int k = lane; k < t.nb; k += kWave) {
+  for (int k = lane; k < t.nb; k += kDesignThreads) {
     T s = 0;
     for (int u = t.brow[k]; u < t.brow[k + 1]; ++u) {
       const int p = t.bprod[u];
@@ -3071,7 +3077,7 @@ __global__ __launch_bounds__(64) void k_design_site(AceArgs<T> a, DesignArgs<T> 
       });
     }
     __syncthreads();
-    for (int u = lane; u < ce * nr; u += kWave) {
+    for (int u = lane; u < ce * nr; u += kDesignThreads) {
       const int e = u / nr, n = u - e * nr;
       T v, d;
       radial_value(a.rad, Ech[e], Rp[e], n, v, d);
@@ -3079,7 +3085,7 @@ __global__ __launch_bounds__(64) void k_design_site(AceArgs<T> a, DesignArgs<T> 
       Rd[e * nr + n] = d;
     }
     __syncthreads();
-    for (int u = lane; u < ce * a.ndens; u += kWave) {
+    for (int u = lane; u < ce * a.ndens; u += kDesignThreads) {
       const int e = u / a.ndens, k = u - e * a.ndens;
       const int rr = a.dens[2 * k], mm = a.dens[2 * k + 1];
       const T s = Rd[e * nr + rr] * Yv[e * NY + mm], v = Rv[e * nr + rr];
@@ -3087,13 +3093,13 @@ __global__ __launch_bounds__(64) void k_design_site(AceArgs<T> a, DesignArgs<T> 
       D[e * a.ndens + k] = V3<T>{s * Eg[4 * e] + v * yg[0], s * Eg[4 * e + 1] + v * yg[1], s * Eg[4 * e + 2] + v * yg[2]};
     }
     __syncthreads();
-    for (int k = lane; k < t.nb; k += kWave) {
+    for (int k = lane; k < t.nb; k += kDesignThreads) {
       T acc[CE][3];
 #pragma unroll
       for (int e = 0; e < CE; ++e) acc[e][0] = acc[e][1] = acc[e][2] = 0;
       for (int g = t.row[k]; g < t.row[k + 1]; ++g) {
         const T gv = G[g];
-        const int d = t.d[g];
+        const int d = Gd[g];
 #pragma unroll
         for (int e = 0; e < CE; ++e) {  // edges past ce read in-bounds stale tiles; their sums are not written
           const V3<T> q = D[e * a.ndens + d];
@@ -3115,16 +3121,16 @@ __global__ __launch_bounds__(64) void k_design_site(AceArgs<T> a, DesignArgs<T> 
   }
 }
 
-// Force rows of every atom: one wavefront per (sorted) atom, lanes over basis
+// Force rows of every atom: one workgroup per (sorted) atom, threads over basis
 // functions; F_a^{k, z_a} += sum_out g_e^k and, for each out-edge's reverse
 // (an edge of neighbour site j), F_a^{k, z_j} -= g_rev^k, in edge order
 // (deterministic). Also the atom's own-site virial -sum_out R_e (x) g_e^k.
 template <typename T>
-__global__ __launch_bounds__(64) void k_design_gather(AceArgs<T> a, DesignArgs<T> t, const int32_t* __restrict__ erev,
+__global__ __launch_bounds__(kDesignThreads) void k_design_gather(AceArgs<T> a, DesignArgs<T> t, const int32_t* __restrict__ erev,
                                                       const int32_t* __restrict__ atom_cfg, double* __restrict__ Wpart,
-                                                      double* __restrict__ out, int64_t ncols) {
+                                                      double* __restrict__ out, int64_t ld) {
   extern __shared__ __align__(16) unsigned char smem[];
-  double* acc = (double*)smem;  // nz x 3 per lane: acc[(lane * nz + z) * 3 + c]
+  double* acc = (double*)smem;  // nz x 3 per thread: acc[(thread * nz + z) * 3 + c]
   const int lane = threadIdx.x;
   const int64_t i = blockIdx.x;
   const int nz = a.nz, nb = t.nb;
@@ -3134,7 +3140,7 @@ __global__ __launch_bounds__(64) void k_design_gather(AceArgs<T> a, DesignArgs<T
   const int64_t o = a.perm[i];
   const int64_t row0 = 10 * (int64_t)atom_cfg[o] + 1 + 3 * o;
   double* my = acc + (size_t)lane * nz * 3;
-  for (int k = lane; k < nb; k += kWave) {
+  for (int k = lane; k < nb; k += kDesignThreads) {
     for (int q = 0; q < 3 * nz; ++q) my[q] = 0;
     double w[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     double f0 = 0, f1 = 0, f2 = 0;
@@ -3164,34 +3170,51 @@ __global__ __launch_bounds__(64) void k_design_gather(AceArgs<T> a, DesignArgs<T
     my[3 * zi + 1] += f1;
     my[3 * zi + 2] += f2;
     for (int z = 0; z < nz; ++z)
-      for (int c = 0; c < 3; ++c) out[(row0 + c) * ncols + k + (int64_t)nb * z] = my[3 * z + c];
+      for (int c = 0; c < 3; ++c) out[(row0 + c) * ld + k + (int64_t)nb * z] = my[3 * z + c];
     for (int c = 0; c < 9; ++c) Wpart[(i * nb + k) * 9 + c] = w[c];
   }
 }
 
-// Energy and virial rows of every configuration: one thread per (basis
-// function, configuration), the configuration's sorted sites in order.
+// Energy and virial rows of every configuration: a block of ng x 64 threads per
+// (configuration, 64 basis functions); thread group q sums the sites q, q + ng,
+// ... of the configuration (sorted order) per centre species, and the groups'
+// sums are added in group order (deterministic).
 template <typename T>
-__global__ void k_design_rows(int nb, int nz, const int32_t* __restrict__ cfg_cell0, const int32_t* __restrict__ start,
-                              const int64_t* __restrict__ atom0, const int32_t* __restrict__ sps,
-                              const T* __restrict__ Bsite, const double* __restrict__ Wpart, double* __restrict__ out,
-                              int64_t ncols) {
+__global__ __launch_bounds__(256) void k_design_rows(int nb, int nz, const int32_t* __restrict__ cfg_cell0,
+                                                     const int32_t* __restrict__ start, const int64_t* __restrict__ atom0,
+                                                     const int32_t* __restrict__ sps, const T* __restrict__ Bsite,
+                                                     const double* __restrict__ Wpart, double* __restrict__ out,
+                                                     int64_t ld) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  double* acc = (double*)smem;  // ng groups x 64 lanes x nz x 10
   const int c = blockIdx.x;
-  const int k = blockIdx.y * blockDim.x + threadIdx.x;
-  if (k >= nb) return;
+  const int lane = threadIdx.x % 64, grp = threadIdx.x / 64, ng = blockDim.x / 64;
+  const int k = blockIdx.y * 64 + lane;
+  double* my = acc + ((size_t)grp * 64 + lane) * nz * 10;
+  for (int q = 0; q < nz * 10; ++q) my[q] = 0;
   const int lo = start[cfg_cell0[c]], hi = start[cfg_cell0[c + 1]];
+  if (k < nb)
+    for (int i = lo + grp; i < hi; i += ng) {
+      double* a = my + 10 * sps[i];
+      a[0] += (double)Bsite[(int64_t)i * nb + k];
+      const double* w = Wpart + ((int64_t)i * nb + k) * 9;
+      for (int q = 0; q < 9; ++q) a[1 + q] += w[q];
+    }
+  __syncthreads();
+  if (grp != 0 || k >= nb) return;
   const int64_t erow = 10 * (int64_t)c + 3 * atom0[c], wrow = 10 * (int64_t)c + 3 * atom0[c + 1] + 1;
   for (int z = 0; z < nz; ++z) {
-    double E = 0, W[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int i = lo; i < hi; ++i) {
-      if (sps[i] != z) continue;
-      E += (double)Bsite[(int64_t)i * nb + k];
-      for (int q = 0; q < 9; ++q) W[q] += Wpart[((int64_t)i * nb + k) * 9 + q];
+    double v[10];
+    for (int q = 0; q < 10; ++q) {
+      v[q] = 0;
+      for (int g = 0; g < ng; ++g) v[q] += acc[((size_t)g * 64 + lane) * nz * 10 + 10 * z + q];
     }
-    out[erow * ncols + k + (int64_t)nb * z] = E;
-    for (int q = 0; q < 9; ++q) out[(wrow + q) * ncols + k + (int64_t)nb * z] = W[q];
+    out[erow * ld + k + (int64_t)nb * z] = v[0];
+    for (int q = 0; q < 9; ++q) out[(wrow + q) * ld + k + (int64_t)nb * z] = v[1 + q];
   }
 }
+
+constexpr size_t kDesignLds = 40 * 1024;
 
 template <typename T, int L, int CE>
 static void launch_design_site(pfw_ctx* ctx, const DesignArgs<T>& t, size_t lds) {
@@ -3199,7 +3222,8 @@ static void launch_design_site(pfw_ctx* ctx, const DesignArgs<T>& t, size_t lds)
   if (lds > 64 * 1024)
     PFW_HIP(hipFuncSetAttribute((const void*)k_design_site<T, L, CE>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)lds));
-  hipLaunchKernelGGL((k_design_site<T, L, CE>), dim3((unsigned)ctx->sys.n), dim3(kWave), lds, ctx->stream, a, t);
+  hipLaunchKernelGGL((k_design_site<T, L, CE>), dim3((unsigned)ctx->sys.n), dim3(kDesignThreads), lds, ctx->stream, a,
+                     t);
   PFW_HIP(hipGetLastError());
 }
 
@@ -3219,7 +3243,7 @@ template <typename T, int L>
 static void design_run(pfw_ctx* ctx, const std::vector<int32_t>& row, const std::vector<int32_t>& dd,
                        const std::vector<int32_t>& term, const std::vector<int32_t>& ff, const std::vector<double>& cc,
                        const std::vector<int32_t>& brow, const std::vector<int32_t>& bprod,
-                       const std::vector<double>& bc, double* out) {
+                       const std::vector<double>& bc, double* out, int64_t ld) {
   AceDev& ad = ctx->ace;
   SystemBuffers& s = ctx->sys;
   hipStream_t st = ctx->stream;
@@ -3227,13 +3251,13 @@ static void design_run(pfw_ctx* ctx, const std::vector<int32_t>& row, const std:
   const int64_t n = s.n, ncols = (int64_t)nb * nz, rows = 10 * (int64_t)ctx->ncfg + 3 * n;
   constexpr int NY = (L + 1) * (L + 1);
   const int nG = (int)dd.size();
-  // the largest edge chunk whose tiles leave room for a few workgroups per CU
+  // the largest edge chunk whose tiles leave room for several workgroups per CU
   int ce = 0;
   size_t lds = 0;
   for (int c : {8, 4, 2}) {
     lds = design_lds<T>(NY, ad.ndens, nG, ad.radial.nr, c).total;
     ce = c;
-    if (lds <= 64 * 1024) break;
+    if (lds <= kDesignLds) break;
   }
   if (lds > kMaxLds) throw Error(PFW_INVALID_ARGUMENT, "ACE model too large for the design kernel's LDS tiles");
   const double gbytes = (double)n * ad.ecap * nb * 3 * sizeof(T);
@@ -3265,7 +3289,7 @@ static void design_run(pfw_ctx* ctx, const std::vector<int32_t>& row, const std:
   t.Bsite = dev.get<T>((size_t)n * nb);
   t.gbuf = dev.get<T>((size_t)n * ad.ecap * nb * 3);
   double* Wpart = dev.get<double>((size_t)n * nb * 9);
-  double* dout = dev.get<double>((size_t)rows * ncols);
+  double* dout = dev.get<double>((size_t)rows * ld);
   int64_t* atom0 = dev.get<int64_t>(ctx->atom0.size());
   PFW_HIP(hipMemcpyAsync(atom0, ctx->atom0.data(), ctx->atom0.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
   launch_dens_only<T, L>(ctx);
@@ -3276,22 +3300,27 @@ static void design_run(pfw_ctx* ctx, const std::vector<int32_t>& row, const std:
   else
     launch_design_site<T, L, 2>(ctx, t, lds);
   AceArgs<T> a = make_args<T>(ctx, 0);
-  const size_t glds = sizeof(double) * kWave * 3 * nz;
+  const size_t glds = sizeof(double) * kDesignThreads * 3 * nz;
   if (glds > 64 * 1024) throw Error(PFW_INVALID_ARGUMENT, "too many species for the design gather");
-  hipLaunchKernelGGL(k_design_gather<T>, dim3((unsigned)n), dim3(kWave), glds, st, a, t, (const int32_t*)ad.erev,
-                     (const int32_t*)s.atom_cfg, Wpart, dout, ncols);
+  hipLaunchKernelGGL(k_design_gather<T>, dim3((unsigned)n), dim3(kDesignThreads), glds, st, a, t, (const int32_t*)ad.erev,
+                     (const int32_t*)s.atom_cfg, Wpart, dout, ld);
   PFW_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_design_rows<T>, dim3((unsigned)ctx->ncfg, (unsigned)((nb + 63) / 64)), dim3(64), 0, st, nb, nz,
+  const size_t glds_rows = sizeof(double) * 64 * 10 * nz;  // per group of 64 threads
+  const int ng = (int)std::min<size_t>(4, (64 * 1024) / glds_rows);
+  if (ng < 1) throw Error(PFW_INVALID_ARGUMENT, "too many species for the design rows");
+  hipLaunchKernelGGL(k_design_rows<T>, dim3((unsigned)ctx->ncfg, (unsigned)((nb + 63) / 64)), dim3(64 * ng),
+                     ng * glds_rows, st, nb, nz,
                      (const int32_t*)s.cfg_cell0, (const int32_t*)s.start, (const int64_t*)atom0, (const int32_t*)s.sps,
-                     (const T*)t.Bsite, (const double*)Wpart, dout, ncols);
+                     (const T*)t.Bsite, (const double*)Wpart, dout, ld);
   PFW_HIP(hipGetLastError());
-  PFW_HIP(hipMemcpyAsync(out, dout, (size_t)rows * ncols * sizeof(double), hipMemcpyDeviceToHost, st));
+  // whole rows of the caller's width in one copy (the columns past ncols are the caller's to fill)
+  PFW_HIP(hipMemcpyAsync(out, dout, (size_t)rows * ld * sizeof(double), hipMemcpyDeviceToHost, st));
   PFW_HIP(hipStreamSynchronize(st));
 }
 
 // The basis columns of the linear design (rows as ace_parameter_sets', row-major
 // rows x nb nz, column k + nb z), by per-site Jacobians in one pass.
-void ace_design(pfw_ctx* ctx, int precision, const double* basis_scale, double* out) {
+void ace_design(pfw_ctx* ctx, int precision, const double* basis_scale, double* out, int64_t ld) {
   ctx->host_written = false;
   ctx->have_results = false;
   AceDev& ad = ctx->ace;
@@ -3304,8 +3333,9 @@ void ace_design(pfw_ctx* ctx, int precision, const double* basis_scale, double* 
   const int nb = ad.nbasis;
   SystemBuffers& s = ctx->sys;
   const int64_t n = s.n, ncols = (int64_t)nb * ad.nz, rows = 10 * (int64_t)ctx->ncfg + 3 * n;
+  if (ld < ncols) throw Error(PFW_INVALID_ARGUMENT, "design row stride below nbasis x nz");
   if (n == 0 || ncols == 0) {
-    std::memset(out, 0, sizeof(double) * (size_t)rows * ncols);
+    for (int64_t r = 0; r < rows; ++r) std::memset(out + r * ld, 0, sizeof(double) * (size_t)ncols);
     return;
   }
   // host tables: the entries (k, d) of dB/dA with their product-rule terms,
@@ -3355,7 +3385,7 @@ void ace_design(pfw_ctx* ctx, int precision, const double* basis_scale, double* 
     if (!ace_overflowed(ad)) break;
   }
   const int L = ad.lmax <= 2 ? 2 : ad.lmax <= 4 ? 4 : 6;
-#define PFW_DESIGN(T, LL) design_run<T, LL>(ctx, row, dd, term, ff, cc, brow, bprod, bc, out)
+#define PFW_DESIGN(T, LL) design_run<T, LL>(ctx, row, dd, term, ff, cc, brow, bprod, bc, out, ld)
   if (precision == PFW_MIXED) {
     if (L == 2)
       PFW_DESIGN(float, 2);

@@ -496,10 +496,10 @@ int pfw_ace_adjoint(pfw_ctx* ctx, int precision, const double* dE, const double*
   });
 }
 
-int pfw_ace_design(pfw_ctx* ctx, int precision, const double* basis_scale, double* out) {
+int pfw_ace_design(pfw_ctx* ctx, int precision, const double* basis_scale, double* out, int64_t ld) {
   return run(ctx, [&] {
     if (!ctx->have_system) throw Error(PFW_STALE, "no system set");
-    pfw::ace_design(ctx, precision, basis_scale, out);
+    pfw::ace_design(ctx, precision, basis_scale, out, ld);
   });
 }
 

@@ -395,7 +395,7 @@ void verlet_pair_run(pfw_ctx* ctx, uint32_t mask, int precision);
 // ace.hip
 void ace_set_model(pfw_ctx* ctx, const pfw_ace_model* m);
 void ace_group_densities(pfw_ctx* ctx, int rchunk);
-void ace_design(pfw_ctx* ctx, int precision, const double* basis_scale, double* out);
+void ace_design(pfw_ctx* ctx, int precision, const double* basis_scale, double* out, int64_t ld);
 void ace_set_parameters(pfw_ctx* ctx, const double* WB, const double* Wpair);
 void ace_run(pfw_ctx* ctx, uint32_t mask, int precision);
 void ace_free(pfw_ctx* ctx);

@@ -18,6 +18,8 @@ atom order), vec(W) (3 x 3, column-major); 1 + 3n + 9 rows. Columns: vec(WB)
 Masked invariants (basis policy) and a disabled pair term give zero columns,
 as the reference's masked B and zeroed Q do.
 """
+import ctypes
+
 import numpy as np
 
 from ._capi import ArgumentError, DimensionMismatch, ptr
@@ -94,10 +96,9 @@ def design_matrix(calc, systems, fit_pair=True, centers=None):
                ptr(np.ascontiguousarray(WP)) if WP is not None else None, ptr(out))
     base = out[nsets - 1].copy()
     A = np.empty((rows, nbasis_cols + npair))
-    if jac:
-        Ab = np.empty((rows, nbasis_cols))  # row-major, column k + nb z
-        w.ctx.call("pfw_ace_design", w.mode, ptr(np.ascontiguousarray(m.mask, np.float64)), ptr(Ab))
-        A[:, :nbasis_cols] = Ab
+    if jac:  # written straight into the first nb nz columns (row stride = A's width)
+        w.ctx.call("pfw_ace_design", w.mode, ptr(np.ascontiguousarray(m.mask, np.float64)), ptr(A),
+                   ctypes.c_int64(A.shape[1]))
     else:
         A[:, :nbasis_cols] = (out[:nbasis_cols] - base[None, :]).T
     if npair:

@@ -177,7 +177,7 @@ def test_jacobian_design_equals_unit_readout_sets(pfw, prec):
     scale[[3, 100]] = 0.0
     rows = sum(10 + 3 * len(s) for s in systems)
     got = np.empty((rows, nb * nz))
-    w.ctx.call("pfw_ace_design", w.mode, _capi.ptr(scale), _capi.ptr(got))
+    w.ctx.call("pfw_ace_design", w.mode, _capi.ptr(scale), _capi.ptr(got), nb * nz)
     nsets = nb * nz + 1
     WB = np.zeros((nsets, nz, nb))
     WB.reshape(nsets, -1)[np.arange(nb * nz), np.arange(nb * nz)] = np.tile(scale, nz)
@@ -193,4 +193,5 @@ def test_jacobian_design_equals_unit_readout_sets(pfw, prec):
     with pytest.raises(pfw.ArgumentError):  # projected bases take the parameter-set path
         pc = pfw.ACECalculator(pfw.project_species(model, lab[0], range(1, 3)), ps, precision=prec)
         pc.workspace.prepare(systems[0], model.cutoff, model.channels(systems[0].labels))
-        pc.workspace.ctx.call("pfw_ace_design", pc.workspace.mode, None, _capi.ptr(np.empty((10 + 3 * len(systems[0]), nb * nz))))
+        pc.workspace.ctx.call("pfw_ace_design", pc.workspace.mode, None,
+                              _capi.ptr(np.empty((10 + 3 * len(systems[0]), nb * nz))), nb * nz)
