@@ -101,7 +101,8 @@ static void dmalloc(T** p, size_t count) {
 void nl_free(pfw_ctx* ctx) {
   SystemBuffers& s = ctx->sys;
   void* ptrs[] = {s.X,     s.species, s.xw,     s.u,      s.cell_of,  s.count,    s.start,    s.rank,
-                  s.blocksum, s.perm, s.xs,    s.sps,    s.bounds, s.grids, s.atom_cfg, s.cell_cfg, s.cfg_cell0, s.xr};
+                  s.blocksum, s.perm, s.xs,    s.sps,    s.bounds, s.grids, s.atom_cfg, s.cell_cfg, s.cfg_cell0, s.xr,
+                  s.cbox};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   s = SystemBuffers();
@@ -241,6 +242,7 @@ void nl_configure(pfw_ctx* ctx, double c, double skin) {
   dmalloc(&s.xs, 3 * n);
   dmalloc(&s.sps, n);
   dmalloc(&s.xr, n);
+  dmalloc(&s.cbox, 2 * total);
   dmalloc(&s.count, total);
   dmalloc(&s.start, total + 1);
   dmalloc(&s.rank, n);
@@ -539,7 +541,7 @@ __global__ __launch_bounds__(64) void k_cell_sort_gather(int ncells, int64_t n, 
                                                          int32_t* __restrict__ perm, const double* __restrict__ xw,
                                                          const int32_t* __restrict__ species, double* __restrict__ xs,
                                                          int32_t* __restrict__ sps, float4* __restrict__ xr,
-                                                         int32_t* __restrict__ scratch) {
+                                                         float4* __restrict__ cbox, int32_t* __restrict__ scratch) {
   const int c = blockIdx.x;
   const int lane = threadIdx.x;
   const int b = start[c], e = start[c + 1], cnt = e - b;
@@ -577,9 +579,11 @@ __global__ __launch_bounds__(64) void k_cell_sort_gather(int ncells, int64_t n, 
     }
     __syncthreads();
   }
-  // xr: FP32 positions relative to the cell's first atom, species bits in w
+  // xr: FP32 positions relative to the cell's first atom, species bits in w;
+  // cbox: their box (the pair kernel's candidate cull)
   const int i0 = perm[b];
   const double ox = xw[i0], oy = xw[n + i0], oz = xw[2 * n + i0];
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
   for (int k = b + lane; k < e; k += kWave) {
     const int i = perm[k];
     const double x = xw[i], y = xw[n + i], z = xw[2 * n + i];
@@ -587,7 +591,23 @@ __global__ __launch_bounds__(64) void k_cell_sort_gather(int ncells, int64_t n, 
     xs[n + k] = y;
     xs[2 * n + k] = z;
     sps[k] = species[i];
-    xr[k] = make_float4((float)(x - ox), (float)(y - oy), (float)(z - oz), __int_as_float(species[i]));
+    const float4 r = make_float4((float)(x - ox), (float)(y - oy), (float)(z - oz), __int_as_float(species[i]));
+    xr[k] = r;
+    lo[0] = fminf(lo[0], r.x);
+    lo[1] = fminf(lo[1], r.y);
+    lo[2] = fminf(lo[2], r.z);
+    hi[0] = fmaxf(hi[0], r.x);
+    hi[1] = fmaxf(hi[1], r.y);
+    hi[2] = fmaxf(hi[2], r.z);
+  }
+  for (int off = kWave / 2; off > 0; off >>= 1)
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = fminf(lo[a], __shfl_xor(lo[a], off, kWave));
+      hi[a] = fmaxf(hi[a], __shfl_xor(hi[a], off, kWave));
+    }
+  if (lane == 0) {
+    cbox[2 * (int64_t)c] = make_float4(lo[0], lo[1], lo[2], 0.f);
+    cbox[2 * (int64_t)c + 1] = make_float4(hi[0], hi[1], hi[2], 0.f);
   }
 }
 
@@ -781,7 +801,7 @@ void nl_build_core(const BuildView& v, hipStream_t st, bool counts_zero, pfw_ctx
   if (n > 0) {
     hipLaunchKernelGGL(k_scatter, dim3(nblocks(n, 256)), dim3(256), 0, st, n, v.cell_of, v.start, v.rank, v.perm);
     hipLaunchKernelGGL(k_cell_sort_gather, dim3(ncells), dim3(kWave), 0, st, ncells, n, v.start, v.perm, v.xw,
-                       v.species, v.xs, v.sps, v.xr, v.rank);
+                       v.species, v.xs, v.sps, v.xr, v.cbox, v.rank);
   }
   PFW_HIP(hipGetLastError());
 }
@@ -791,7 +811,7 @@ void nl_build(pfw_ctx* ctx) {
   SystemBuffers& s = ctx->sys;
   BuildView v{s.n,    ctx->ncells, ctx->ncfg, ctx->open_any, s.X,     s.xw,      s.u,    s.cell_of, s.count,
               s.rank, s.bounds,    s.grids,   s.atom_cfg,    s.start, s.blocksum, s.perm, s.xs,      s.species,
-              s.sps,  s.xr};
+              s.sps,  s.xr,      s.cbox};
   nl_build_core(v, ctx->stream, ctx->ncells <= kScanOneMax, ctx);
 }
 

@@ -35,6 +35,31 @@ template <typename T>
 constexpr int kCandOf = 256;
 template <typename T>
 constexpr int kCandPadOf = kCandOf<T> + 9 * kSlices;
+static_assert(kCandOf<float> % 64 == 0 && kCandOf<double> % 64 == 0, "the fill walks a chunk in whole waves");
+// Candidate-to-entry table over a whole entry chunk (larger neighbourhoods
+// search the entry offsets instead).
+constexpr int kEofMax = 768;
+
+// Enumeration order of a neighbourhood's candidates. The entry-major index t
+// (stencil entry, then the cell's sorted atoms) is visited at position u with
+// t = u q mod total, q a prime that does not divide total (so the map is a
+// bijection). Consecutive positions then jump across the whole neighbourhood,
+// so every chunk and every 64-candidate block of a slice holds a spatially
+// even sample of it: the lanes' hit counts per block stay alike and the
+// divergent hit loop runs fewer rounds (on the B2 256k workload 30.5 instead
+// of 40.8 hit iterations per wavefront, counted on the host).
+__device__ inline uint32_t scramble_mul(int total) {
+  if (total % 37) return 37;
+  if (total % 41) return 41;
+  if (total % 43) return 43;
+  if (total % 47) return 47;
+  return 1;
+}
+// t = u q mod total for 0 <= u < total
+__device__ inline int scramble_of(uint32_t u, uint32_t q, int total) {
+  if (total < (1 << 25)) return (int)((u * q) % (uint32_t)total);  // u q < 2^31
+  return (int)(((uint64_t)u * q) % (uint64_t)total);
+}
 
 template <typename T>
 struct Band;
@@ -91,6 +116,17 @@ __device__ inline double prefilter_d(const Cand<double>& c, double xi, double yi
 __device__ inline uint32_t sign_word(float v) { return __float_as_uint(v); }
 __device__ inline uint32_t sign_word(double v) { return (uint32_t)__double2hiint(v); }
 
+// An entry's Cartesian image shift: the staged FP64 copy, or (FP32 kernel)
+// formed again from its integer lattice shift.
+template <typename T>
+__device__ inline void entry_shift(const Grid& g, const int* lat, const double* staged, double* sh) {
+  if constexpr (sizeof(T) == 8) {
+    for (int a = 0; a < 3; ++a) sh[a] = staged[a];
+  } else {
+    exact_shift(g, lat, sh);
+  }
+}
+
 template <typename T>
 struct Pair2;
 template <>
@@ -110,6 +146,7 @@ template <typename T, int KIND, bool AUDIT>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 5 : 4))) void k_pair(const Grid* __restrict__ grids, const int32_t* __restrict__ cell_cfg,
                                              int64_t n, const int32_t* __restrict__ start,
                                              const double* __restrict__ xs, const float4* __restrict__ xr,
+                                             const float4* __restrict__ cbox,
                                              const int32_t* __restrict__ sps,
                                              const int32_t* __restrict__ perm, const double* __restrict__ params,
                                              int nz, LJConsts lj, uint32_t mask, double* __restrict__ F,
@@ -125,11 +162,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
   double(*red)[3] = (double(*)[3])cbuf;
   __shared__ int cm[kCandPadOf<T>];  // per candidate position: (stencil entry of the chunk) << 8 | species
   __shared__ int e_pref[kEnt + 1], e_start[kEnt], e_lat[kEnt][3];
-  __shared__ uint8_t e_of[kCandOf<T>];  // stencil entry (of the entry chunk) of each candidate of the LDS chunk
-  __shared__ double e_sh[kEnt][3];
+  __shared__ uint8_t e_of[kEofMax];  // stencil entry of each entry-major candidate index of the entry chunk
+  __shared__ int selfpos[kWave];     // enumeration position of each home atom's self pair (atom group), or -1
+  // FP64: each entry's Cartesian image shift (FP32 forms it again in the rare
+  // exact-membership path, which keeps the LDS footprint at five wavefronts per SIMD)
+  __shared__ double e_sh[sizeof(T) == 8 ? kEnt : 1][3];
   // FP32: each entry's cell origin (its first atom) plus image shift, relative
   // to this cell's first atom; a candidate is its cell-relative position plus this
-  __shared__ __align__(16) float e_off[kEnt][4];
+  __shared__ __align__(16) float e_off[sizeof(T) == 4 ? kEnt : 1][4];
   extern __shared__ __align__(16) unsigned char dyn[];  // SmoothPair: per (z_i, z_j) the pair (A, -6 A / c^2)
   using T2 = typename Pair2<T>::type;
   const T2* sAA = (const T2*)dyn;
@@ -187,6 +227,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
       const T m2x = (T)-2 * xi, m2y = (T)-2 * yi, m2z = (T)-2 * zi;
       const T base = (xi * xi + yi * yi + zi * zi) * ((T)1 - DotSlack<T>::value) - c2hi;
       const int zsp = sps[is];
+      // the cell's box (its atoms' FP32 cell-relative positions) and the cull
+      // radius: those coordinates differ from the exact ones by about 1e-6 A,
+      // far inside the 1e-3 relative margin
+      const float4 blo = cbox[2 * (int64_t)cell], bhi = cbox[2 * (int64_t)cell + 1];
+      const T bx0 = (T)blo.x, bx1 = (T)bhi.x, by0 = (T)blo.y, by1 = (T)bhi.y, bz0 = (T)blo.z, bz1 = (T)bhi.z;
+      const T cullc2 = (T)(g.c2 * (1.0 + 1e-3));
       double fx = 0, fy = 0, fz = 0;
       for (int e0 = 0; e0 < g.nstencil; e0 += kEnt) {
         int cntE = 0;
@@ -204,7 +250,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
             double sh[3];
             exact_shift(g, lat, sh);
             for (int a = 0; a < 3; ++a) {
-              e_sh[lane][a] = sh[a];
+              if constexpr (sizeof(T) == 8) e_sh[lane][a] = sh[a];
               e_lat[lane][a] = lat[a];
             }
             if constexpr (sizeof(T) == 4) {
@@ -218,45 +264,103 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
         const int incl = wave_incl_scan(cntE);  // lanes past kEnt add nothing
         if (lane < kEnt) e_pref[lane + 1] = incl;
         if (lane == 0) e_pref[0] = 0;
-        const int total = __shfl(incl, kWave - 1, kWave);
+        const int total = __builtin_amdgcn_readfirstlane(__shfl(incl, kWave - 1, kWave));
+        const bool table = total <= kEofMax;
+        if (table && lane < kEnt)  // each entry marks its candidates
+          for (int t = incl - cntE; t < incl; ++t) e_of[t] = (uint8_t)lane;
+        selfpos[lane] = -1;
+        const uint32_t sq = __builtin_amdgcn_readfirstlane(scramble_mul(total));
+        const uint32_t tstep =
+            __builtin_amdgcn_readfirstlane(total > kWave ? (uint32_t)scramble_of((uint32_t)kWave, sq, total) : 0u);
         __syncthreads();
         const int ez = e_zero;
         const int ne = min(kEnt, g.nstencil - e0);  // entries of this chunk
+        // culling records a candidate's index in its cell in 19 bits of its metadata
+        const bool cull = total < (1 << 19);
+        // sorted index of a staged candidate from its metadata, entry el and u =
+        // chunk start + slot (without the cull the slot is its enumeration position)
+        auto cand_index = [&](int meta, int el, int u) -> int {
+          return e_start[el] + (cull ? (meta >> 13) : scramble_of((uint32_t)u, sq, total) - e_pref[el]);
+        };
         for (int t0 = 0; t0 < total; t0 += kCand) {
           const int tn = min(kCand, total - t0);
-          // slice s at positions [s (L + 1), s (L + 1) + L)
-          const int L = ((tn + S - 1) / S + 7) / 8 * 8;
-          const int stride = L + 1;
+          // slice s at positions [s (Lcap + 1), s (Lcap + 1) + Lcap): room for the
+          // whole chunk, of which only the candidates that pass the cull are kept
+          const int Lcap = ((tn + S - 1) / S + 7) / 8 * 8;
+          const int stride = Lcap + 1;
           const uint32_t invS = (65536u + S - 1) / S;  // t / S = (t invS) >> 16 for t < 4096
-          if (lane < ne) {  // each entry marks its candidates of this chunk
-            const int a0 = max(e_pref[lane], t0) - t0, a1 = min(e_pref[lane + 1], t0 + tn) - t0;
-            for (int t = a0; t < a1; ++t) e_of[t] = (uint8_t)lane;
-          }
-          __syncthreads();
-          for (int t = lane; t < tn; t += kWave) {
-            const int tt = t0 + t;
-            const int lo = e_of[t];
-            const int m = e_start[lo] + (tt - e_pref[lo]);
-            const int q = (int)(((uint32_t)t * invS) >> 16);
-            const int pos = (t - q * S) * stride + q;
-            if constexpr (sizeof(T) == 4) {
-              const float4 v = xr[m];
-              const float4 o = *(const float4*)e_off[lo];
-              const float x = v.x + o.x, y = v.y + o.y, z = v.z + o.z;
-              cand[pos] = Cand<T>{x, y, z, (x * x + y * y + z * z) * ((T)1 - DotSlack<T>::value)};
-              cm[pos] = (lo << 8) | __float_as_int(v.w);
-            } else {
-              const T x = (T)((xs[m] + e_sh[lo][0]) - rx);
-              const T y = (T)((xs[n + m] + e_sh[lo][1]) - ry);
-              const T z = (T)((xs[2 * n + m] + e_sh[lo][2]) - rz);
-              cand[pos] = Cand<T>{x, y, z};
-              cm[pos] = (lo << 8) | sps[m];
+          // entry-major index of this lane's fill position t0 + t
+          uint32_t tq = lane < tn ? (uint32_t)scramble_of((uint32_t)(t0 + lane), sq, total) : 0;
+          int kept = 0;  // candidates of this chunk kept so far (the same in every lane)
+          for (int p0 = 0; p0 < tn; p0 += kWave) {
+            const int t = p0 + lane;
+            const bool act = t < tn;
+            const int tt = (int)tq;  // the candidate at position t0 + t
+            tq += tstep;
+            if (tq >= (uint32_t)total) tq -= (uint32_t)total;
+            int lo = 0, m = 0;
+            T x = 0, y = 0, z = 0;
+            int spj = 0;
+            if (act) {
+              if (table) {
+                lo = e_of[tt];
+              } else {  // the last entry starting at or before tt (empty entries share its offset)
+                int lo2 = 0, hi2 = ne;
+                while (hi2 - lo2 > 1) {
+                  const int mid = (lo2 + hi2) >> 1;
+                  if (e_pref[mid] <= tt)
+                    lo2 = mid;
+                  else
+                    hi2 = mid;
+                }
+                lo = lo2;
+              }
+              m = e_start[lo] + (tt - e_pref[lo]);
+              if constexpr (sizeof(T) == 4) {
+                const float4 v = xr[m];
+                const float4 o = *(const float4*)e_off[lo];
+                x = v.x + o.x;
+                y = v.y + o.y;
+                z = v.z + o.z;
+                spj = __float_as_int(v.w);
+              } else {
+                x = (T)((xs[m] + e_sh[lo][0]) - rx);
+                y = (T)((xs[n + m] + e_sh[lo][1]) - ry);
+                z = (T)((xs[2 * n + m] + e_sh[lo][2]) - rz);
+                spj = sps[m];
+              }
             }
+            // cull: a candidate farther than the cutoff from the box around the
+            // atom group cannot pair with any of its atoms
+            const T gx = fmax(fmax(bx0 - x, x - bx1), (T)0), gy = fmax(fmax(by0 - y, y - by1), (T)0),
+                    gz = fmax(fmax(bz0 - z, z - bz1), (T)0);
+            const bool keep = act && (!cull || gx * gx + gy * gy + gz * gz < cullc2);
+            const uint64_t ball = __ballot(keep);
+            if (keep) {
+              const int slot =
+                  kept + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(ball >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)ball, 0));
+              if (lo == ez) {  // a home-cell atom: record where its self pair sits
+                const int j = m - b - ib;
+                if (j >= 0 && j < nb) selfpos[j] = t0 + slot;
+              }
+              const int q = (int)(((uint32_t)slot * invS) >> 16);
+              const int pos = (slot - q * S) * stride + q;
+              // culled chunks record each candidate's index in its cell for the
+              // exact-membership path; otherwise slot = t and the order gives it
+              const int meta = (cull ? (tt - e_pref[lo]) << 13 : 0) | (lo << 8) | spj;
+              if constexpr (sizeof(T) == 4)
+                cand[pos] = Cand<T>{x, y, z, (x * x + y * y + z * z) * ((T)1 - DotSlack<T>::value)};
+              else
+                cand[pos] = Cand<T>{x, y, z};
+              cm[pos] = meta;
+            }
+            kept += __builtin_popcountll(ball);
           }
-          // the gaps and the slices' tails past the chunk hold far sentinels
-          for (int k = max(0, tn / S - 1) + lane; k <= L; k += kWave)
+          const int L = ((kept + S - 1) / S + 7) / 8 * 8;  // <= Lcap
+          // the gaps and the slices' tails past the kept candidates hold far sentinels
+          for (int k = max(0, kept / S - 1) + lane; k <= L; k += kWave)
             for (int sl = 0; sl < S; ++sl)
-              if (k == L || k * S + sl >= tn) cand[sl * stride + k] = far;
+              if (k == L || k * S + sl >= kept) cand[sl * stride + k] = far;
           __syncthreads();
           if (on) {
             // cnt (8 .. 32, a multiple of 8) tests of candidates p[0 .. cnt): bit
@@ -273,11 +377,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
               }
               return h << (32 - cnt);
             };
-            // the lane's own atom among this chunk's candidates (the (i, i, 0) self
-            // pair): its place in this lane's slice, or -1
-            const int tself = ez >= 0 ? e_pref[ez] + (is - e_start[ez]) - t0 : -1;
+            // the lane's own atom among this chunk's candidates: its place in this
+            // lane's slice, or -1
+            const int us = selfpos[il];
+            const int tself = us >= 0 ? us - t0 : -1;
             int kself = -1;
-            if (tself >= 0 && tself < tn) {
+            if (tself >= 0 && tself < kept) {
               const int q = (int)(((uint32_t)tself * invS) >> 16);
               if (tself - q * S == split) kself = q;
             }
@@ -295,23 +400,24 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
                 const int k = r0 + 63 - __builtin_ctzll(hits);  // within the slice
                 hits &= hits - 1;
                 const Cand<T> c = mine[k];
+                const int meta = cmine[k];
                 const T dx = c.x - xi, dy = c.y - yi, dz = c.z - zi;
                 const T r2 = dx * dx + dy * dy + dz * dz;
-                const int meta = cmine[k], el = meta >> 8, zj = meta & 255;
+                const int el = (meta >> 8) & 31, zj = meta & 255;
                 if (!(r2 < c2lo)) {
-                  const int t = k * S + split;  // within the chunk
-                  const int m = e_start[el] + (t0 + t - e_pref[el]);
-                  double d[3];
+                  const int m = cand_index(meta, el, t0 + k * S + split);
+                  double sh[3], d[3];
                   int img[3];
+                  entry_shift<T>(g, e_lat[el], e_sh[sizeof(T) == 8 ? el : 0], sh);
                   if (!exact_member(g, xs[is], xs[n + is], xs[2 * n + is], xs[m], xs[n + m], xs[2 * n + m], e_lat[el],
-                                    e_sh[el], d, img))
+                                    sh, d, img))
                     continue;
                 }
                 if constexpr (AUDIT) {
-                  const int t = k * S + split;
-                  const int m = e_start[el] + (t0 + t - e_pref[el]);
-                  double d[3];
-                  for (int a = 0; a < 3; ++a) d[a] = (xs[a * n + m] + e_sh[el][a]) - xs[a * n + is];
+                  const int m = cand_index(meta, el, t0 + k * S + split);
+                  double sh[3], d[3];
+                  entry_shift<T>(g, e_lat[el], e_sh[sizeof(T) == 8 ? el : 0], sh);
+                  for (int a = 0; a < 3; ++a) d[a] = (xs[a * n + m] + sh[a]) - xs[a * n + is];
                   audit_add(au, g, d, perm[is], perm[m]);
                 }
                 T e, coef;
@@ -409,11 +515,11 @@ static void launch_pair(pfw_ctx* ctx, uint32_t mask) {
   if (ctx->audit) {
     const AuditBufs au = audit_begin(ctx);
     hipLaunchKernelGGL((k_pair<T, KIND, true>), dim3(ctx->ncells), dim3(kWave), dyn, ctx->stream, s.grids,
-                       s.cell_cfg, s.n, s.start, s.xs, s.xr, s.sps, s.perm, ctx->pair.d_params, ctx->pair.nz, lj,
+                       s.cell_cfg, s.n, s.start, s.xs, s.xr, s.cbox, s.sps, s.perm, ctx->pair.d_params, ctx->pair.nz, lj,
                        mask, ctx->d_F, ctx->d_partials, ctx->ncells, au, slices);
   } else {
     hipLaunchKernelGGL((k_pair<T, KIND, false>), dim3(ctx->ncells), dim3(kWave), dyn, ctx->stream, s.grids,
-                       s.cell_cfg, s.n, s.start, s.xs, s.xr, s.sps, s.perm, ctx->pair.d_params, ctx->pair.nz, lj,
+                       s.cell_cfg, s.n, s.start, s.xs, s.xr, s.cbox, s.sps, s.perm, ctx->pair.d_params, ctx->pair.nz, lj,
                        mask, ctx->d_F, ctx->d_partials, ctx->ncells, AuditBufs{}, slices);
   }
 }
@@ -530,7 +636,7 @@ constexpr size_t static_round(size_t b) { return (b + kStaticAlign - 1) / kStati
 // Byte offsets of the sub-buffers, in order.
 struct StaticLayout {
   size_t grid, params, flag, out, out2, red, cell_cfg, atom_cfg, count, start, blocksum, bounds, cell_of, rank, perm,
-      sps, xw, u, xs, xr, partials, total;
+      sps, xw, u, xs, xr, cbox, partials, total;
 };
 constexpr StaticLayout static_layout() {
   StaticLayout L{};
@@ -560,6 +666,7 @@ constexpr StaticLayout static_layout() {
   L.u = take(sizeof(double) * 3 * kStaticAtoms);
   L.xs = take(sizeof(double) * 3 * kStaticAtoms);
   L.xr = take(sizeof(float) * 4 * kStaticAtoms);
+  L.cbox = take(sizeof(float) * 8 * kStaticCells);
   L.partials = take(sizeof(double) * 7 * kStaticCells);
   L.total = o;
   return L;
@@ -608,11 +715,12 @@ __global__ void k_static_zero(double* __restrict__ out) {
 
 template <typename T, int KIND>
 static void launch_pair_static(const Grid* grid, const int32_t* cell_cfg, int64_t n, const int32_t* start,
-                               const double* xs, const float4* xr, const int32_t* sps, const int32_t* perm,
+                               const double* xs, const float4* xr, const float4* cbox, const int32_t* sps,
+                               const int32_t* perm,
                                const double* params, int nz, LJConsts lj, uint32_t mask, double* F, double* partials,
                                int ncells, hipStream_t st) {
   const size_t dyn = KIND == PFW_PAIR_SMOOTH ? 2 * sizeof(T) * nz * nz : 0;
-  hipLaunchKernelGGL((k_pair<T, KIND, false>), dim3(ncells), dim3(kWave), dyn, st, grid, cell_cfg, n, start, xs, xr,
+  hipLaunchKernelGGL((k_pair<T, KIND, false>), dim3(ncells), dim3(kWave), dyn, st, grid, cell_cfg, n, start, xs, xr, cbox,
                      sps, perm, params, nz, lj, mask, F, partials, ncells, AuditBufs{}, kSlices);
 }
 
@@ -695,14 +803,15 @@ void pair_run_static(pfw_ctx* ctx, int kind, int nz, const double* params, doubl
               (double*)(base + L.xs),
               species,
               (int32_t*)(base + L.sps),
-              (float4*)(base + L.xr)};
+              (float4*)(base + L.xr),
+              (float4*)(base + L.cbox)};
   nl_build_core(v, st, false, nullptr);
   if (mask != 0 && n > 0) {
     const bool fp32 = precision == PFW_MIXED;
     double* part = (double*)(base + L.partials);
 #define PFW_STATIC_LAUNCH(T, K)                                                                                  \
-  launch_pair_static<T, K>(d_grid, (const int32_t*)(base + L.cell_cfg), n, v.start, v.xs, v.xr, v.sps, v.perm,     \
-                           d_params, nz, lj, mask, F, part, ncells, st)
+  launch_pair_static<T, K>(d_grid, (const int32_t*)(base + L.cell_cfg), n, v.start, v.xs, v.xr, v.cbox, v.sps,    \
+                           v.perm, d_params, nz, lj, mask, F, part, ncells, st)
     if (kind == PFW_PAIR_SMOOTH) {
       if (fp32)
         PFW_STATIC_LAUNCH(float, PFW_PAIR_SMOOTH);

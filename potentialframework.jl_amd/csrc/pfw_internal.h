@@ -83,6 +83,7 @@ struct SystemBuffers {
   double* xs = nullptr;        // sorted wrapped coordinates SoA [3][n]
   int32_t* sps = nullptr;      // sorted species
   float4* xr = nullptr;        // sorted positions relative to their cell's first atom (FP32), species bits in w
+  float4* cbox = nullptr;      // 2 per cell: min and max of its atoms' xr (x, y, z)
   uint64_t* bounds = nullptr;  // 6 ordered-int min/max of open axes per configuration
   Grid* grids = nullptr;       // device copy, one per configuration
   int32_t* atom_cfg = nullptr;  // n: configuration of each atom (original order)
@@ -350,6 +351,7 @@ struct BuildView {
   const int32_t* species;
   int32_t* sps;
   float4* xr;
+  float4* cbox;
 };
 void nl_build_core(const BuildView& v, hipStream_t st, bool counts_zero, pfw_ctx* timing);
 void fill_i32(int32_t* p, int64_t m, int32_t v, hipStream_t st);
