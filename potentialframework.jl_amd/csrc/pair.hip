@@ -127,6 +127,15 @@ __device__ inline void entry_shift(const Grid& g, const int* lat, const double* 
   }
 }
 
+// Per entry chunk: candidate count, entry of the zero offset (or -1), entries
+// in the chunk, candidate-to-entry table in use, cull in use, enumeration
+// multiplier and its step per 64 positions.
+struct EntryInfo {
+  int total, ez, ne;
+  bool table, cull;
+  uint32_t sq, tstep;
+};
+
 template <typename T>
 struct Pair2;
 template <>
@@ -202,6 +211,58 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
     int key[3];
     cell_key(g, cell, key);
     const double rx = xs[b], ry = xs[n + b], rz = xs[2 * n + b];
+    // One chunk of up to kEnt stencil entries: each entry's cell range, lattice
+    // and offsets in LDS, the entry-major prefix of their candidates, the
+    // candidate-to-entry table and the enumeration multiplier. A radius-1
+    // stencil is one chunk, set up once per cell before the atom groups.
+    auto setup_entries = [&](int e0) -> EntryInfo {
+      int cntE = 0;
+      const int e = e0 + lane;
+      if (lane == 0) e_zero = -1;
+      __syncthreads();
+      if (lane < kEnt && e < g.nstencil) {
+        int lat[3];
+        bool zero;
+        int nc = stencil_entry(g, key, e, lat, &zero);
+        if (nc >= 0) {
+          int s0 = start[nc];
+          cntE = start[nc + 1] - s0;
+          e_start[lane] = s0;
+          double sh[3];
+          exact_shift(g, lat, sh);
+          for (int a = 0; a < 3; ++a) {
+            if constexpr (sizeof(T) == 8) e_sh[lane][a] = sh[a];
+            e_lat[lane][a] = lat[a];
+          }
+          if constexpr (sizeof(T) == 4) {
+            e_off[lane][0] = (float)((xs[s0] + sh[0]) - rx);
+            e_off[lane][1] = (float)((xs[n + s0] + sh[1]) - ry);
+            e_off[lane][2] = (float)((xs[2 * n + s0] + sh[2]) - rz);
+          }
+          if (zero) e_zero = lane;
+        }
+      }
+      const int incl = wave_incl_scan(cntE);  // lanes past kEnt add nothing
+      if (lane < kEnt) e_pref[lane + 1] = incl;
+      if (lane == 0) e_pref[0] = 0;
+      EntryInfo r;
+      r.total = __builtin_amdgcn_readfirstlane(__shfl(incl, kWave - 1, kWave));
+      r.table = r.total <= kEofMax;
+      if (r.table && lane < kEnt)  // each entry marks its candidates
+        for (int t = incl - cntE; t < incl; ++t) e_of[t] = (uint8_t)lane;
+      r.sq = __builtin_amdgcn_readfirstlane(scramble_mul(r.total));
+      r.tstep = __builtin_amdgcn_readfirstlane(r.total > kWave ? (uint32_t)scramble_of((uint32_t)kWave, r.sq, r.total)
+                                                                 : 0u);
+      // culling records a candidate's index in its cell in 19 bits of its metadata
+      r.cull = r.total < (1 << 19);
+      r.ne = min(kEnt, g.nstencil - e0);  // entries of this chunk
+      __syncthreads();
+      r.ez = __builtin_amdgcn_readfirstlane(e_zero);
+      return r;
+    };
+    const bool one_chunk = g.nstencil <= kEnt;
+    EntryInfo ei{};
+    if (one_chunk) ei = setup_entries(0);
     for (int ib = 0; ib < ncnt; ib += kWave) {
       const int nb = min(kWave, ncnt - ib);
       const int S = min(max_slices, kWave / nb);  // max_slices in 1..kSlices
@@ -235,48 +296,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
       const T cullc2 = (T)(g.c2 * (1.0 + 1e-3));
       double fx = 0, fy = 0, fz = 0;
       for (int e0 = 0; e0 < g.nstencil; e0 += kEnt) {
-        int cntE = 0;
-        const int e = e0 + lane;
-        if (lane == 0) e_zero = -1;
-        __syncthreads();
-        if (lane < kEnt && e < g.nstencil) {
-          int lat[3];
-          bool zero;
-          int nc = stencil_entry(g, key, e, lat, &zero);
-          if (nc >= 0) {
-            int s0 = start[nc];
-            cntE = start[nc + 1] - s0;
-            e_start[lane] = s0;
-            double sh[3];
-            exact_shift(g, lat, sh);
-            for (int a = 0; a < 3; ++a) {
-              if constexpr (sizeof(T) == 8) e_sh[lane][a] = sh[a];
-              e_lat[lane][a] = lat[a];
-            }
-            if constexpr (sizeof(T) == 4) {
-              e_off[lane][0] = (float)((xs[s0] + sh[0]) - rx);
-              e_off[lane][1] = (float)((xs[n + s0] + sh[1]) - ry);
-              e_off[lane][2] = (float)((xs[2 * n + s0] + sh[2]) - rz);
-            }
-            if (zero) e_zero = lane;
-          }
-        }
-        const int incl = wave_incl_scan(cntE);  // lanes past kEnt add nothing
-        if (lane < kEnt) e_pref[lane + 1] = incl;
-        if (lane == 0) e_pref[0] = 0;
-        const int total = __builtin_amdgcn_readfirstlane(__shfl(incl, kWave - 1, kWave));
-        const bool table = total <= kEofMax;
-        if (table && lane < kEnt)  // each entry marks its candidates
-          for (int t = incl - cntE; t < incl; ++t) e_of[t] = (uint8_t)lane;
+        if (!one_chunk) ei = setup_entries(e0);
+        const int total = ei.total, ez = ei.ez, ne = ei.ne;
+        const bool table = ei.table, cull = ei.cull;
+        const uint32_t sq = ei.sq, tstep = ei.tstep;
         selfpos[lane] = -1;
-        const uint32_t sq = __builtin_amdgcn_readfirstlane(scramble_mul(total));
-        const uint32_t tstep =
-            __builtin_amdgcn_readfirstlane(total > kWave ? (uint32_t)scramble_of((uint32_t)kWave, sq, total) : 0u);
         __syncthreads();
-        const int ez = e_zero;
-        const int ne = min(kEnt, g.nstencil - e0);  // entries of this chunk
-        // culling records a candidate's index in its cell in 19 bits of its metadata
-        const bool cull = total < (1 << 19);
         // sorted index of a staged candidate from its metadata, entry el and u =
         // chunk start + slot (without the cull the slot is its enumeration position)
         auto cand_index = [&](int meta, int el, int u) -> int {
