@@ -690,12 +690,14 @@ __global__ __launch_bounds__(64) void k_ace_pull(AceArgs<T> a) {
 // for a share of the centres, were measured twice as slow: the staging, not
 // the centre loop, is the larger part.)
 constexpr int kEdgeWaves = 1;
+constexpr int kSegEntries = 27;  // the radius-1 stencil, for the direct reverse index
 __global__ __launch_bounds__(kWave * kEdgeWaves) void k_ace_edges(const Grid* __restrict__ grids, const int32_t* __restrict__ cell_cfg,
                                                   int64_t n, const double* __restrict__ xs,
                                                   const int32_t* __restrict__ start, int pass, int ecap,
                                                   int32_t* __restrict__ ecount, int32_t* __restrict__ enb,
                                                   int32_t* __restrict__ eent, double* __restrict__ eR,
-                                                  int32_t* __restrict__ status, int skin_rows) {
+                                                  int32_t* __restrict__ status, int skin_rows,
+                                                  int32_t* __restrict__ erank, uint8_t* __restrict__ eseg) {
   __shared__ int s_pref[kWave + 1], s_start[kWave], s_lat[kWave][3];
   __shared__ double s_sh[kWave][3];
   __shared__ float s_ax[kWave], s_ay[kWave], s_az[kWave];  // the centres, relative to the cell's first atom
@@ -757,6 +759,12 @@ __global__ __launch_bounds__(kWave * kEdgeWaves) void k_ace_edges(const Grid* __
       __syncthreads();
       const int total = s_pref[kWave];
       const int ez = s_zero;
+      // direct reverse index (k_ace_rev): each centre's segment starts, when the
+      // stencil has radius 1 and a row fits 8-bit slots, and each edge's rank
+      // among this cell's centres that reach its neighbour through the same
+      // entry, when the cell is also one atom group of one wavefront
+      const bool seg_ok = pass && eseg && g.nstencil == kSegEntries && ecap <= 255;
+      const bool direct = seg_ok && cnt <= kWave && kEdgeWaves == 1;
       for (int t0 = 0; t0 < total; t0 += kWave) {
         const int t = t0 + lane;
         const bool cand = t < total;
@@ -780,6 +788,8 @@ __global__ __launch_bounds__(kWave * kEdgeWaves) void k_ace_edges(const Grid* __
           fy = (float)((yj + s_sh[lo][1]) - ry);
           fz = (float)((zj + s_sh[lo][2]) - rz);
         }
+        int hitc = 0;  // centres so far that reach this candidate (ascending sorted index)
+        const bool seg_first = cand && t == s_pref[lo];  // the first candidate of its entry
         for (int a = wid; a < na; a += kEdgeWaves) {
           const int64_t k = b + ag + a;
           bool hit = false;
@@ -812,7 +822,10 @@ __global__ __launch_bounds__(kWave * kEdgeWaves) void k_ace_edges(const Grid* __
             eR[3 * q] = d[0];
             eR[3 * q + 1] = d[1];
             eR[3 * q + 2] = d[2];
+            if (erank) erank[q] = direct ? hitc : -1;
           }
+          if (seg_ok && seg_first && slot < ecap) eseg[k * kSegEntries + lo] = (uint8_t)slot;
+          hitc += hit ? 1 : 0;
           if (lane == a) {
             pos += h;
             deg += h;
@@ -847,22 +860,38 @@ __global__ void k_ace_rev(int64_t nslots, int ecap, const Grid* __restrict__ gri
                           const int32_t* __restrict__ atom_cfg, const int32_t* __restrict__ perm,
                           const int32_t* __restrict__ ecount, const int32_t* __restrict__ enb,
                           const int32_t* __restrict__ eent, const double* __restrict__ eR,
-                          int32_t* __restrict__ erev, int32_t* __restrict__ status) {
+                          int32_t* __restrict__ erev, int32_t* __restrict__ status, const uint8_t* __restrict__ eseg,
+                          int nstencil) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nslots) return;
   const int i = (int)(e / ecap);
   if ((int)(e - (int64_t)i * ecap) >= ecount[i]) return;  // an unused slot
   const int j = enb[e];
-  const int want = grids[atom_cfg[perm[i]]].nstencil - 1 - eent[e];
+  // nstencil > 0: every configuration's stencil size
+  const int want = (nstencil > 0 ? nstencil : grids[atom_cfg[perm[i]]].nstencil) - 1 - eent[e];
+  const int rank = erev[e];  // from k_ace_edges: the direct index, or -1
   int64_t a = (int64_t)j * ecap, b = a + ecount[j];  // first f in [a, b) with (eent, enb)[f] >= (want, i)
   const int64_t end = b;
-  while (a < b) {
-    const int64_t mid = (a + b) >> 1;
-    const int em = eent[mid];
-    if (em < want || (em == want && enb[mid] < i))
-      a = mid + 1;
-    else
-      b = mid;
+  if (rank >= 0) {
+    // j's segment of entry `want` lists the centres of i's cell that reach j in
+    // ascending order; i is the rank-th of them. The displacement needs no
+    // check here: both directions form it as fl(fl(x_j - x_i) + H s) with the
+    // shift of the mirrored entry exactly -H s (or with minimum-image
+    // rounding, odd in its argument), so R(rev) = -R bitwise.
+    a += (int64_t)eseg[(int64_t)j * kSegEntries + want] + rank;
+    const int found = (a < end && eent[a] == want && enb[a] == i) ? (int)a : -1;
+    erev[e] = found;
+    if (found < 0) atomicOr(status, 1);
+    return;
+  } else {
+    while (a < b) {
+      const int64_t mid = (a + b) >> 1;
+      const int em = eent[mid];
+      if (em < want || (em == want && enb[mid] < i))
+        a = mid + 1;
+      else
+        b = mid;
+    }
   }
   int found = -1;
   if (a < end && eent[a] == want && enb[a] == i && eR[3 * a] == -eR[3 * e] && eR[3 * a + 1] == -eR[3 * e + 1] &&
@@ -1217,6 +1246,7 @@ void ace_free_edges(pfw_ctx* ctx) {
   dfree(a.enb);
   dfree(a.eR);
   dfree(a.erev);
+  dfree(a.eseg);
   dfree(a.eent);
   dfree(a.eg);
   dfree(a.siteE);
@@ -1231,6 +1261,7 @@ void ace_free_edges(pfw_ctx* ctx) {
   dfree(a.enb_s);
   dfree(a.eent_s);
   dfree(a.erev_s);
+  dfree(a.eseg_s);
   dfree(a.eR_s);
   dfree(a.cslot);
   dfree(a.Xref);
@@ -1810,6 +1841,16 @@ static void dispatch_sites(pfw_ctx* ctx, int grads) {
 // found in status[1]).
 static void ace_rows_alloc(pfw_ctx* ctx, int ecap);
 
+// The stencil size shared by every configuration's grid, or 0.
+static int uniform_stencil(const pfw_ctx* ctx) {
+  int ns = -1;
+  for (const Grid& g : ctx->grids) {
+    if (ns >= 0 && g.nstencil != ns) return 0;
+    ns = g.nstencil;
+  }
+  return ns > 0 ? ns : 0;
+}
+
 static void ace_environment(pfw_ctx* ctx, int rev) {
   AceDev& ad = ctx->ace;
   SystemBuffers& s = ctx->sys;
@@ -1819,7 +1860,8 @@ static void ace_environment(pfw_ctx* ctx, int rev) {
   if (ad.ecap == 0) {  // size the rows: one counting pass
     PFW_HIP(hipMemsetAsync(ad.status, 0, 4 * sizeof(int32_t), st));
     hipLaunchKernelGGL(k_ace_edges, dim3((unsigned)ctx->ncells), dim3(kWave * kEdgeWaves), 0, st, s.grids, s.cell_cfg, n, s.xs,
-                       s.start, 0, 0, ad.ecount, (int32_t*)nullptr, (int32_t*)nullptr, (double*)nullptr, ad.status, 0);
+                       s.start, 0, 0, ad.ecount, (int32_t*)nullptr, (int32_t*)nullptr, (double*)nullptr, ad.status, 0,
+                       (int32_t*)nullptr, (uint8_t*)nullptr);
     PFW_HIP(hipGetLastError());
     PFW_HIP(hipMemcpyAsync(ad.h_status, ad.status, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
     PFW_HIP(hipStreamSynchronize(st));
@@ -1830,11 +1872,13 @@ static void ace_environment(pfw_ctx* ctx, int rev) {
   const int64_t slots = n * (int64_t)ad.ecap;
   fill_i32(ad.status, 4, 0, st);  // a fill kernel: this sequence may be captured in a graph
   hipLaunchKernelGGL(k_ace_edges, dim3((unsigned)ctx->ncells), dim3(kWave * kEdgeWaves), 0, st, s.grids, s.cell_cfg, n, s.xs,
-                     s.start, 1, ad.ecap, ad.ecount, ad.enb, ad.eent, ad.eR, ad.status, 0);
+                     s.start, 1, ad.ecap, ad.ecount, ad.enb, ad.eent, ad.eR, ad.status, 0, rev ? ad.erev : nullptr,
+                     rev ? ad.eseg : nullptr);
   PFW_HIP(hipGetLastError());
   if (rev)
     hipLaunchKernelGGL(k_ace_rev, dim3(nblk(slots, 256)), dim3(256), 0, st, slots, ad.ecap, s.grids, s.atom_cfg,
-                       s.perm, ad.ecount, ad.enb, ad.eent, ad.eR, ad.erev, ad.status);
+                       s.perm, ad.ecount, ad.enb, ad.eent, ad.eR, ad.erev, ad.status, (const uint8_t*)ad.eseg,
+                       uniform_stencil(ctx));
   PFW_HIP(hipMemcpyAsync(ad.h_status, ad.status, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
 }
 
@@ -1886,6 +1930,7 @@ static void ace_rows_alloc(pfw_ctx* ctx, int ecap) {
     PFW_HIP(hipMalloc(&ad.rec, 10 * n * sizeof(double)));
     PFW_HIP(hipMalloc(&ad.status, 4 * sizeof(int32_t)));
     PFW_HIP(hipHostMalloc(&ad.h_status, 4 * sizeof(int32_t), hipHostMallocDefault));
+    PFW_HIP(hipMalloc(&ad.eseg, std::max<int64_t>(n, 1) * kSegEntries));
   }
   const int64_t slots = n * (int64_t)ecap;
   if (slots > ad.edge_cap || !ad.enb) {
@@ -1917,6 +1962,7 @@ static void ace_skin_rebuild(pfw_ctx* ctx) {
   ace_rows_alloc(ctx, std::max(ad.ecap, 1));
   if (!ad.ecount_s) {
     PFW_HIP(hipMalloc(&ad.ecount_s, n * sizeof(int32_t)));
+    PFW_HIP(hipMalloc(&ad.eseg_s, std::max<int64_t>(n, 1) * kSegEntries));
     PFW_HIP(hipMalloc(&ad.Xref, 3 * n * sizeof(double)));
     PFW_HIP(hipMalloc(&ad.dX, 3 * n * sizeof(double)));
     PFW_HIP(hipMalloc(&ad.maxd2, 2 * sizeof(unsigned long long)));
@@ -1927,12 +1973,13 @@ static void ace_skin_rebuild(pfw_ctx* ctx) {
   auto build = [&](int pass) {
     PFW_HIP(hipMemsetAsync(ad.status, 0, 4 * sizeof(int32_t), st));
     hipLaunchKernelGGL(k_ace_edges, dim3((unsigned)ctx->ncells), dim3(kWave * kEdgeWaves), 0, st, s.grids, s.cell_cfg, n, s.xs,
-                       s.start, pass, pass ? ad.ecap_s : 0, ad.ecount_s, ad.enb_s, ad.eent_s, ad.eR_s, ad.status, 1);
+                       s.start, pass, pass ? ad.ecap_s : 0, ad.ecount_s, ad.enb_s, ad.eent_s, ad.eR_s, ad.status, 1,
+                       pass ? ad.erev_s : nullptr, pass ? ad.eseg_s : nullptr);
     PFW_HIP(hipGetLastError());
     if (pass)
       hipLaunchKernelGGL(k_ace_rev, dim3(nblk(n * (int64_t)ad.ecap_s, 256)), dim3(256), 0, st, n * (int64_t)ad.ecap_s,
                          ad.ecap_s, s.grids, s.atom_cfg, s.perm, ad.ecount_s, ad.enb_s, ad.eent_s, ad.eR_s, ad.erev_s,
-                         ad.status);
+                         ad.status, (const uint8_t*)ad.eseg_s, uniform_stencil(ctx));
     PFW_HIP(hipGetLastError());
     PFW_HIP(hipMemcpyAsync(ad.h_status, ad.status, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
     PFW_HIP(hipStreamSynchronize(st));

@@ -196,6 +196,7 @@ struct AceDev {
   int32_t* enb = nullptr;     // n x ecap: neighbour (sorted index)
   double* eR = nullptr;       // 3 x slots
   int32_t* erev = nullptr;    // slot of the reverse edge
+  uint8_t* eseg = nullptr;    // n x 27: first slot of each stencil entry's edges in a site's row (radius-1 stencils)
   int32_t* eent = nullptr;    // stencil entry of each edge (a site's edges are sorted by entry, neighbour)
   double* eg = nullptr;       // 3 x slots edge gradients
   int32_t* h_status = nullptr;  // pinned copy of status after each environment build
@@ -216,6 +217,7 @@ struct AceDev {
   int32_t* enb_s = nullptr;     // n x ecap_s
   int32_t* eent_s = nullptr;
   int32_t* erev_s = nullptr;
+  uint8_t* eseg_s = nullptr;    // n x 27
   double* eR_s = nullptr;       // 3 x slots, at the build
   int32_t* cslot = nullptr;     // skin slot -> compacted slot (-1: outside the cutoff now)
   int32_t* corig = nullptr;     // compacted slot -> skin slot
