@@ -235,7 +235,7 @@ void nl_configure(pfw_ctx* ctx, double c, double skin) {
     for (int32_t q = cell0[k]; q < cell0[k + 1]; ++q) cell_cfg[q] = k;
 
   SystemBuffers& s = ctx->sys;
-  dmalloc(&s.xw, 3 * n);
+  dmalloc(&s.xw, 4 * n);  // wrap records: x, y, z, species bits
   dmalloc(&s.u, 3 * n);
   dmalloc(&s.cell_of, n);
   dmalloc(&s.perm, n);
@@ -304,14 +304,18 @@ __device__ inline int wave_rank_count(int32_t* count, int c) {
 }
 
 // Wrap atom i with its configuration's basis; when no configuration has an
-// open axis also bin and count it.
+// open axis also bin and count it. The wrapped position goes out as one
+// 32-byte record (x, y, z, species bits), so that the cell sort gathers one
+// record per atom.
 __device__ inline void wrap_one(const Grid& g, int cfg, int64_t i, int64_t n, const double* __restrict__ X,
-                                double* __restrict__ xw, double* __restrict__ u, int32_t* __restrict__ cell_of,
-                                int32_t* __restrict__ count, int32_t* __restrict__ rank,
-                                unsigned long long* __restrict__ bounds, int open_any) {
+                                const int32_t* __restrict__ species, double* __restrict__ xw,
+                                double* __restrict__ u, int32_t* __restrict__ cell_of, int32_t* __restrict__ count,
+                                int32_t* __restrict__ rank, unsigned long long* __restrict__ bounds, int open_any) {
   double x[3] = {X[3 * i], X[3 * i + 1], X[3 * i + 2]}, w[3], uu[3];
   wrap_atom(g, x, w, uu);
-  for (int a = 0; a < 3; ++a) xw[a * n + i] = w[a];
+  double2* rec = (double2*)(xw + 4 * i);
+  rec[0] = make_double2(w[0], w[1]);
+  rec[1] = make_double2(w[2], __longlong_as_double((long long)species[i]));
   if (!open_any) {  // bin coordinates are needed later only for open axes
     const int c = bin_atom(g, uu);
     cell_of[i] = c;
@@ -331,16 +335,16 @@ __device__ inline void wrap_one(const Grid& g, int cfg, int64_t i, int64_t n, co
 // single: one configuration, whose grid is then read with scalar loads (no
 // dependence on a per-atom load).
 __global__ void k_wrap(const Grid* __restrict__ grids, const int32_t* __restrict__ atom_cfg, int64_t n,
-                       const double* __restrict__ X, double* __restrict__ xw, double* __restrict__ u,
-                       int32_t* __restrict__ cell_of, int32_t* __restrict__ count, int32_t* __restrict__ rank,
-                       unsigned long long* __restrict__ bounds, int open_any, int single) {
+                       const double* __restrict__ X, const int32_t* __restrict__ species, double* __restrict__ xw,
+                       double* __restrict__ u, int32_t* __restrict__ cell_of, int32_t* __restrict__ count,
+                       int32_t* __restrict__ rank, unsigned long long* __restrict__ bounds, int open_any, int single) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   if (single) {
-    wrap_one(grids[0], 0, i, n, X, xw, u, cell_of, count, rank, bounds, open_any);
+    wrap_one(grids[0], 0, i, n, X, species, xw, u, cell_of, count, rank, bounds, open_any);
   } else {
     const int cfg = atom_cfg[i];
-    wrap_one(grids[cfg], cfg, i, n, X, xw, u, cell_of, count, rank, bounds, open_any);
+    wrap_one(grids[cfg], cfg, i, n, X, species, xw, u, cell_of, count, rank, bounds, open_any);
   }
 }
 
@@ -407,31 +411,29 @@ __device__ inline int block_exclusive_scan(int v, int* lds, int* total) {
   return woff + incl - v;
 }
 
-// One-workgroup exclusive scan for up to kScanOneMax counts, read as rows of
-// 1024 (coalesced, all in flight at once). All rows are scanned together:
-// wave scans in registers, one pass over the wave totals of every row, one
-// pass over the row totals. With zero_count it also zeroes the
-// counts themselves (ready for the next step's binning).
+// One-workgroup exclusive scan of count[0, m) for m <= 1024 ROWS, read as
+// rows of 1024 (coalesced, all in flight at once). All rows are scanned
+// together: wave scans in registers, one pass over the wave totals of every
+// row, one pass over the row totals. emit(k, s) receives the exclusive prefix
+// s of every k < m (after every count has been read); returns the total.
 constexpr int kScanOneThreads = 1024;
-constexpr int kScanOneRows = 32;
-constexpr int kScanOneMax = kScanOneThreads * kScanOneRows;
-__global__ __launch_bounds__(1024) void k_scan_one(int32_t* __restrict__ count, int m, int32_t* __restrict__ start,
-                                                   int64_t n, int zero_count) {
+template <int ROWS, typename Emit>
+__device__ inline int scan_rows(const int32_t* __restrict__ count, int m, Emit emit) {
   constexpr int NW = kScanOneThreads / kWave;
-  __shared__ int wt[kScanOneRows][NW];    // wave totals of each row
-  __shared__ int woff[kScanOneRows][NW];  // exclusive wave offsets within each row
-  __shared__ int rt[kScanOneRows];        // row totals
-  __shared__ int roff[kScanOneRows + 1];  // exclusive row offsets; roff[rows] = total
+  __shared__ int wt[ROWS][NW];    // wave totals of each row
+  __shared__ int woff[ROWS][NW];  // exclusive wave offsets within each row
+  __shared__ int rt[ROWS];        // row totals
+  __shared__ int roff[ROWS + 1];  // exclusive row offsets; roff[rows] = total
   const int tid = threadIdx.x, lane = tid % kWave, wid = tid / kWave;
   const int rows = (m + kScanOneThreads - 1) / kScanOneThreads;
-  int v[kScanOneRows], incl[kScanOneRows];
+  int v[ROWS], incl[ROWS];
 #pragma unroll
-  for (int r = 0; r < kScanOneRows; ++r) {
+  for (int r = 0; r < ROWS; ++r) {
     const int k = r * kScanOneThreads + tid;
     v[r] = (r < rows && k < m) ? count[k] : 0;
   }
 #pragma unroll
-  for (int r = 0; r < kScanOneRows; ++r) {
+  for (int r = 0; r < ROWS; ++r) {
     int x = v[r];
     if (r < rows) {  // uniform over the workgroup
       x = wave_incl_scan(x);
@@ -458,14 +460,56 @@ __global__ __launch_bounds__(1024) void k_scan_one(int32_t* __restrict__ count, 
   }
   __syncthreads();
 #pragma unroll
-  for (int r = 0; r < kScanOneRows; ++r) {
+  for (int r = 0; r < ROWS; ++r) {
     const int k = r * kScanOneThreads + tid;
-    if (r < rows && k < m) {
-      start[k] = roff[r] + woff[r][wid] + incl[r] - v[r];
-      if (zero_count) count[k] = 0;
-    }
+    if (r < rows && k < m) emit(k, roff[r] + woff[r][wid] + incl[r] - v[r]);
   }
-  if (tid == 0) start[m] = n >= 0 ? (int32_t)n : roff[rows];
+  return roff[rows];
+}
+
+// The scan of up to kScanOneMax counts as one workgroup. With zero_count it
+// also zeroes the counts themselves (ready for the next step's binning).
+constexpr int kScanOneRows = 32;
+constexpr int kScanOneMax = kScanOneThreads * kScanOneRows;
+__global__ __launch_bounds__(1024) void k_scan_one(int32_t* __restrict__ count, int m, int32_t* __restrict__ start,
+                                                   int64_t n, int zero_count) {
+  const int total = scan_rows<kScanOneRows>(count, m, [&](int k, int s) {
+    start[k] = s;
+    if (zero_count) count[k] = 0;
+  });
+  if (threadIdx.x == 0) start[m] = n >= 0 ? (int32_t)n : total;
+}
+
+// Scan and count-sort placement in one launch, for grids of at most
+// kFusedScanMax cells: every workgroup scans the whole count array (a few
+// tens of KB, read from L2) into LDS, then places its 1024 atoms at
+// start[c] + rank; workgroup 0 also writes start. This replaces the
+// one-workgroup scan and the placement kernel (one launch and the scan's
+// serial tail fewer). The counts are zeroed later, by the cell sort.
+constexpr int kFusedScanRows = 15;
+constexpr int kFusedScanMax = kScanOneThreads * kFusedScanRows;  // (m + 1) starts: at most 60 KB of LDS
+__global__ __launch_bounds__(1024) void k_scan_scatter(const int32_t* __restrict__ count, int m,
+                                                       int32_t* __restrict__ start, int64_t n,
+                                                       const int32_t* __restrict__ cell_of,
+                                                       const int32_t* __restrict__ rank, int32_t* __restrict__ perm) {
+  extern __shared__ int st[];  // m + 1 exclusive starts
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool on = i < n;
+  const int c = on ? cell_of[i] : 0, rk = on ? rank[i] : 0;  // in flight during the scan
+  const bool first = blockIdx.x == 0;
+  const int total = scan_rows<kFusedScanRows>(count, m, [&](int k, int s) {
+    st[k] = s;
+    if (first) start[k] = s;
+  });
+  if (threadIdx.x == 0) {
+    st[m] = total;
+    if (first) start[m] = (int32_t)n;
+  }
+  __syncthreads();
+  if (on) {
+    const int pos = st[c] + rk;
+    if (pos < st[c + 1]) perm[pos] = (int32_t)i;  // ranks are 0 .. count - 1 by construction
+  }
 }
 
 __global__ void k_scan_blocks(const int32_t* __restrict__ count, int ncells, int32_t* __restrict__ blocksum) {
@@ -531,34 +575,70 @@ __global__ void k_scatter(int64_t n, const int32_t* __restrict__ cell_of, const 
 
 // Deterministic order inside each cell: ascending original index (the stable
 // sortperm of storage.jl:199), then the gather into sorted SoA coordinates.
-// One wavefront per cell: cells of up to 64 atoms are sorted with a register
-// bitonic network over lane shuffles (of the smallest power of two that holds
-// the cell), cells of up to kRankSort atoms by rank counting over an LDS copy
-// (indices are distinct), larger cells (dense clusters only) by rank counting
-// over a copy in `scratch` (n entries, free at this point).
+// One wavefront per pair of consecutive cells. When both hold at most 32 atoms
+// (the common case) each half-wave sorts one of them with a register bitonic
+// network over lane shuffles (of the smallest power of two that holds the
+// larger cell), so one generation of wavefronts covers twice the cells.
+// Otherwise the wavefront takes the two cells in turn: cells of up to 64 atoms
+// by the same network over the whole wave, cells of up to kRankSort atoms by
+// rank counting over an LDS copy (indices are distinct), larger cells (dense
+// clusters only) by rank counting over a copy in `scratch` (n entries, free at
+// this point). With `count` given, the cells' counts are zeroed (ready for the
+// next binning; the fused scan does not zero them).
 constexpr int kRankSort = 256;
-__global__ __launch_bounds__(64) void k_cell_sort_gather(int ncells, int64_t n, const int32_t* __restrict__ start,
-                                                         int32_t* __restrict__ perm, const double* __restrict__ xw,
-                                                         const int32_t* __restrict__ species, double* __restrict__ xs,
-                                                         int32_t* __restrict__ sps, float4* __restrict__ xr,
-                                                         float4* __restrict__ cbox, int32_t* __restrict__ scratch) {
-  const int c = blockIdx.x;
-  const int lane = threadIdx.x;
-  const int b = start[c], e = start[c + 1], cnt = e - b;
+
+// Bitonic sort of v within each aligned group of `size` lanes (size a power
+// of two, at most 64): every group ends ascending. The last merge (k = size)
+// is ascending in every group; taking its direction from lane bit k instead
+// would leave the odd groups descending.
+__device__ inline int bitonic_lanes(int v, int lane, int size) {
+  for (int k = 2; k <= size; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      const int o = __shfl_xor(v, j, kWave);
+      const bool up = k == size || (lane & k) == 0;
+      const bool lower = (lane & j) == 0;
+      v = (lower == up) ? min(v, o) : max(v, o);
+    }
+  return v;
+}
+
+// Gather of one sorted atom (its sorted slot k, original index i) from its
+// wrap record into the SoA arrays; xr relative to the cell's first atom
+// (ox, oy, oz); extends the lane's box.
+__device__ inline void gather_sorted(int64_t n, int k, int i, double ox, double oy, double oz,
+                                     const double* __restrict__ xw, double* __restrict__ xs,
+                                     int32_t* __restrict__ sps, float4* __restrict__ xr, float* lo, float* hi) {
+  const double2* rec = (const double2*)(xw + 4 * (int64_t)i);
+  const double2 a = rec[0], b = rec[1];
+  const double x = a.x, y = a.y, z = b.x;
+  const int sp = (int)__double_as_longlong(b.y);
+  xs[k] = x;
+  xs[n + k] = y;
+  xs[2 * n + k] = z;
+  sps[k] = sp;
+  const float4 r = make_float4((float)(x - ox), (float)(y - oy), (float)(z - oz), __int_as_float(sp));
+  xr[k] = r;
+  lo[0] = fminf(lo[0], r.x);
+  lo[1] = fminf(lo[1], r.y);
+  lo[2] = fminf(lo[2], r.z);
+  hi[0] = fmaxf(hi[0], r.x);
+  hi[1] = fmaxf(hi[1], r.y);
+  hi[2] = fmaxf(hi[2], r.z);
+}
+
+// One cell [b, e) by the whole wavefront.
+__device__ inline void sort_gather_cell(int c, int b, int e, int64_t n, int lane, int* buf,
+                                        int32_t* __restrict__ perm, const double* __restrict__ xw,
+                                        double* __restrict__ xs,
+                                        int32_t* __restrict__ sps, float4* __restrict__ xr,
+                                        float4* __restrict__ cbox, int32_t* __restrict__ scratch) {
+  const int cnt = e - b;
   if (cnt == 0) return;
   if (cnt <= kWave) {
     int v = lane < cnt ? perm[b + lane] : 0x7fffffff;
-    const int size = cnt <= 16 ? 16 : (cnt <= 32 ? 32 : kWave);  // lanes [0, size) end ascending
-    for (int k = 2; k <= size; k <<= 1)
-      for (int j = k >> 1; j > 0; j >>= 1) {
-        const int o = __shfl_xor(v, j, kWave);
-        const bool up = (lane & k) == 0;
-        const bool lower = (lane & j) == 0;
-        v = (lower == up) ? min(v, o) : max(v, o);
-      }
+    v = bitonic_lanes(v, lane, cnt <= 16 ? 16 : (cnt <= 32 ? 32 : kWave));
     if (lane < cnt) perm[b + lane] = v;
   } else if (cnt <= kRankSort) {
-    __shared__ int buf[kRankSort];
     for (int k = lane; k < cnt; k += kWave) buf[k] = perm[b + k];
     __syncthreads();
     for (int k = lane; k < cnt; k += kWave) {
@@ -581,25 +661,10 @@ __global__ __launch_bounds__(64) void k_cell_sort_gather(int ncells, int64_t n, 
   }
   // xr: FP32 positions relative to the cell's first atom, species bits in w;
   // cbox: their box (the pair kernel's candidate cull)
-  const int i0 = perm[b];
-  const double ox = xw[i0], oy = xw[n + i0], oz = xw[2 * n + i0];
+  const int64_t i0 = perm[b];
+  const double ox = xw[4 * i0], oy = xw[4 * i0 + 1], oz = xw[4 * i0 + 2];
   float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-  for (int k = b + lane; k < e; k += kWave) {
-    const int i = perm[k];
-    const double x = xw[i], y = xw[n + i], z = xw[2 * n + i];
-    xs[k] = x;
-    xs[n + k] = y;
-    xs[2 * n + k] = z;
-    sps[k] = species[i];
-    const float4 r = make_float4((float)(x - ox), (float)(y - oy), (float)(z - oz), __int_as_float(species[i]));
-    xr[k] = r;
-    lo[0] = fminf(lo[0], r.x);
-    lo[1] = fminf(lo[1], r.y);
-    lo[2] = fminf(lo[2], r.z);
-    hi[0] = fmaxf(hi[0], r.x);
-    hi[1] = fmaxf(hi[1], r.y);
-    hi[2] = fmaxf(hi[2], r.z);
-  }
+  for (int k = b + lane; k < e; k += kWave) gather_sorted(n, k, perm[k], ox, oy, oz, xw, xs, sps, xr, lo, hi);
   for (int off = kWave / 2; off > 0; off >>= 1)
     for (int a = 0; a < 3; ++a) {
       lo[a] = fminf(lo[a], __shfl_xor(lo[a], off, kWave));
@@ -608,6 +673,46 @@ __global__ __launch_bounds__(64) void k_cell_sort_gather(int ncells, int64_t n, 
   if (lane == 0) {
     cbox[2 * (int64_t)c] = make_float4(lo[0], lo[1], lo[2], 0.f);
     cbox[2 * (int64_t)c + 1] = make_float4(hi[0], hi[1], hi[2], 0.f);
+  }
+}
+
+__global__ __launch_bounds__(64) void k_cell_sort_gather(int ncells, int64_t n, const int32_t* __restrict__ start,
+                                                         int32_t* __restrict__ perm, const double* __restrict__ xw,
+                                                         double* __restrict__ xs, int32_t* __restrict__ sps,
+                                                         float4* __restrict__ xr, float4* __restrict__ cbox,
+                                                         int32_t* __restrict__ scratch, int32_t* __restrict__ count) {
+  __shared__ int buf[kRankSort];
+  const int lane = threadIdx.x;
+  const int c0 = 2 * blockIdx.x;
+  const bool two = c0 + 1 < ncells;
+  const int b0 = start[c0], e0 = start[c0 + 1], e1 = two ? start[c0 + 2] : e0;
+  if (count && lane < (two ? 2 : 1)) count[c0 + lane] = 0;
+  const int n0 = e0 - b0, n1 = e1 - e0;
+  if (n0 > 32 || n1 > 32) {
+    sort_gather_cell(c0, b0, e0, n, lane, buf, perm, xw, xs, sps, xr, cbox, scratch);
+    if (two) sort_gather_cell(c0 + 1, e0, e1, n, lane, buf, perm, xw, xs, sps, xr, cbox, scratch);
+    return;
+  }
+  // half-wave h sorts cell c0 + h
+  const int h = lane >> 5, hl = lane & 31;
+  const int b = h ? e0 : b0, cnt = h ? n1 : n0;
+  int v = hl < cnt ? perm[b + hl] : 0x7fffffff;
+  v = bitonic_lanes(v, lane, max(n0, n1) <= 16 ? 16 : 32);  // shuffles stay inside each half
+  const int64_t i0 = __shfl(v, lane & 32, kWave);
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  if (hl < cnt) {
+    perm[b + hl] = v;
+    gather_sorted(n, b + hl, v, xw[4 * i0], xw[4 * i0 + 1], xw[4 * i0 + 2], xw, xs, sps, xr, lo, hi);
+  }
+  for (int off = 16; off > 0; off >>= 1)
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = fminf(lo[a], __shfl_xor(lo[a], off, kWave));
+      hi[a] = fmaxf(hi[a], __shfl_xor(hi[a], off, kWave));
+    }
+  if (hl == 0 && cnt > 0) {
+    const int64_t c = c0 + h;
+    cbox[2 * c] = make_float4(lo[0], lo[1], lo[2], 0.f);
+    cbox[2 * c + 1] = make_float4(hi[0], hi[1], hi[2], 0.f);
   }
 }
 
@@ -776,7 +881,7 @@ void nl_build_core(const BuildView& v, hipStream_t st, bool counts_zero, pfw_ctx
     hipLaunchKernelGGL(k_bounds_init, dim3(nblocks(6 * (int64_t)v.ncfg, 64)), dim3(64), 0, st,
                        (unsigned long long*)v.bounds, 6 * v.ncfg);
   if (n > 0)
-    hipLaunchKernelGGL(k_wrap, dim3(nblocks(n, 256)), dim3(256), 0, st, v.grids, v.atom_cfg, n, v.X, v.xw, v.u,
+    hipLaunchKernelGGL(k_wrap, dim3(nblocks(n, 256)), dim3(256), 0, st, v.grids, v.atom_cfg, n, v.X, v.species, v.xw, v.u,
                        v.cell_of, v.count, v.rank, (unsigned long long*)v.bounds, (int)v.open_any,
                        (int)(v.ncfg == 1));
   if (v.open_any) {
@@ -788,7 +893,12 @@ void nl_build_core(const BuildView& v, hipStream_t st, bool counts_zero, pfw_ctx
   }
   PFW_HIP(hipGetLastError());
   stage(1);
-  if (one) {
+  // counts are zeroed by the one-workgroup scan, or, after the fused scan, by the cell sort
+  const bool fused = n > 0 && ncells <= kFusedScanMax;
+  if (fused) {
+    hipLaunchKernelGGL(k_scan_scatter, dim3(nblocks(n, kScanOneThreads)), dim3(kScanOneThreads),
+                       (ncells + 1) * sizeof(int), st, v.count, ncells, v.start, n, v.cell_of, v.rank, v.perm);
+  } else if (one) {
     hipLaunchKernelGGL(k_scan_one, dim3(1), dim3(kScanOneThreads), 0, st, v.count, ncells, v.start, n, 1);
   } else {
     int nb = (int)nblocks(ncells, kScanBlock);
@@ -799,9 +909,10 @@ void nl_build_core(const BuildView& v, hipStream_t st, bool counts_zero, pfw_ctx
   PFW_HIP(hipGetLastError());
   stage(2);
   if (n > 0) {
-    hipLaunchKernelGGL(k_scatter, dim3(nblocks(n, 256)), dim3(256), 0, st, n, v.cell_of, v.start, v.rank, v.perm);
-    hipLaunchKernelGGL(k_cell_sort_gather, dim3(ncells), dim3(kWave), 0, st, ncells, n, v.start, v.perm, v.xw,
-                       v.species, v.xs, v.sps, v.xr, v.cbox, v.rank);
+    if (!fused)
+      hipLaunchKernelGGL(k_scatter, dim3(nblocks(n, 256)), dim3(256), 0, st, n, v.cell_of, v.start, v.rank, v.perm);
+    hipLaunchKernelGGL(k_cell_sort_gather, dim3(nblocks(ncells, 2)), dim3(kWave), 0, st, ncells, n, v.start, v.perm,
+                       v.xw, v.xs, v.sps, v.xr, v.cbox, v.rank, fused ? v.count : nullptr);
   }
   PFW_HIP(hipGetLastError());
 }

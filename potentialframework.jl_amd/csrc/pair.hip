@@ -687,7 +687,7 @@ constexpr StaticLayout static_layout() {
   L.rank = take(sizeof(int32_t) * kStaticAtoms);
   L.perm = take(sizeof(int32_t) * kStaticAtoms);
   L.sps = take(sizeof(int32_t) * kStaticAtoms);
-  L.xw = take(sizeof(double) * 3 * kStaticAtoms);
+  L.xw = take(sizeof(double) * 4 * kStaticAtoms);
   L.u = take(sizeof(double) * 3 * kStaticAtoms);
   L.xs = take(sizeof(double) * 3 * kStaticAtoms);
   L.xr = take(sizeof(float) * 4 * kStaticAtoms);
@@ -696,7 +696,7 @@ constexpr StaticLayout static_layout() {
   L.total = o;
   return L;
 }
-constexpr StaticLayout kStatic = static_layout();  // about 46 MB
+constexpr StaticLayout kStatic = static_layout();  // about 48 MB
 
 static __device__ __align__(256) unsigned char scratch[kStatic.total];
 

@@ -72,7 +72,7 @@ struct SystemBuffers {
   int64_t cap = 0;
   double* X = nullptr;        // 3 x n original order (AoS)
   int32_t* species = nullptr;  // n original order
-  double* xw = nullptr;       // wrapped coordinates, SoA [3][n], original order
+  double* xw = nullptr;       // wrap records (x, y, z, species bits), [n][4], original order
   double* u = nullptr;        // bin coordinates, SoA [3][n], original order
   int32_t* cell_of = nullptr;  // n
   int32_t* count = nullptr;    // ncells
