@@ -681,33 +681,44 @@ __global__ __launch_bounds__(64) void k_ace_pull(AceArgs<T> a) {
 // fixed-capacity rows: site i's edges occupy slots [i ecap, i ecap + ecount[i]).
 // pass 0 only counts (to size ecap); pass 1 writes neighbour, stencil entry and
 // exact R, flags status[2] when a site has more than ecap edges, and records
-// the largest degree in status[1]. One wavefront per cell: the stencil entries are staged in LDS,
-// lanes hold 64 candidates at a time (in stencil-entry, cell order) and test
-// them against each centre of the cell in turn; a ballot compacts the hits, so
-// every centre's edges come out sorted by (entry, neighbour) without atomics.
-// A centre's write cursor lives in the lane of the same index within its group
-// of 64 centres. (Several wavefronts per cell, each staging the candidates
-// for a share of the centres, were measured twice as slow: the staging, not
-// the centre loop, is the larger part.)
-constexpr int kEdgeWaves = 1;
+// the largest degree in status[1].
+// One workgroup of kEdgeWaves wavefronts per cell. The stencil entries are
+// staged in LDS, and the cell's candidates (in stencil-entry, cell order) are
+// taken 64 at a time: chunk q of a round by wavefront q mod kEdgeWaves, which
+// stages it and tests it against each centre of the cell in turn (the FP32
+// prefilter, then the exact FP64 test), keeping every (chunk, centre) hit
+// ballot in LDS. A prefix over the round's chunks, in chunk order, then gives
+// each centre's first slot in every chunk, and each wavefront writes its
+// chunks' edges compacted by the ballots: every centre's edges come out sorted
+// by (entry, neighbour) without atomics. The staging (entry search and
+// neighbour loads) is thereby spread over the wavefronts; splitting the
+// centres instead, each wavefront staging every candidate, was measured twice
+// as slow as one wavefront per cell.
+constexpr int kEdgeWaves = 4;
+constexpr int kEdgeRound = 1;                         // chunks of one wavefront held at once
+constexpr int kEdgeChunks = kEdgeWaves * kEdgeRound;  // chunks of one round
 constexpr int kSegEntries = 27;  // the radius-1 stencil, for the direct reverse index
-__global__ __launch_bounds__(kWave * kEdgeWaves) void k_ace_edges(const Grid* __restrict__ grids, const int32_t* __restrict__ cell_cfg,
+__global__ __launch_bounds__(kWave * kEdgeWaves) __attribute__((amdgpu_waves_per_eu(8))) void k_ace_edges(const Grid* __restrict__ grids, const int32_t* __restrict__ cell_cfg,
                                                   int64_t n, const double* __restrict__ xs,
                                                   const int32_t* __restrict__ start, int pass, int ecap,
                                                   int32_t* __restrict__ ecount, int32_t* __restrict__ enb,
                                                   int32_t* __restrict__ eent, double* __restrict__ eR,
                                                   int32_t* __restrict__ status, int skin_rows,
-                                                  int32_t* __restrict__ erank, uint8_t* __restrict__ eseg) {
+                                                  int32_t* __restrict__ erank, uint8_t* __restrict__ eseg,
+                                                  int single) {
   __shared__ int s_pref[kWave + 1], s_start[kWave], s_lat[kWave][3];
   __shared__ double s_sh[kWave][3];
   __shared__ float s_ax[kWave], s_ay[kWave], s_az[kWave];  // the centres, relative to the cell's first atom
   __shared__ double s_cx[kWave], s_cy[kWave], s_cz[kWave];  // and exactly
   __shared__ int s_zero;
+  __shared__ unsigned long long s_mask[kEdgeChunks][kWave];  // hit ballot of each (chunk of the round, centre)
+  __shared__ int s_off[kEdgeChunks][kWave];                  // each centre's first slot in each chunk
+  __shared__ int s_pos[kWave], s_cut[kWave];                  // each centre's hits so far (and inside the cutoff)
   const int cell = blockIdx.x;
   const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
   const int b = start[cell], cnt = start[cell + 1] - b;
   if (cnt == 0) return;
-  const Grid& g = grids[cell_cfg[cell]];
+  const Grid& g = grids[single ? 0 : cell_cfg[cell]];  // one configuration: no dependent load
   int key[3];
   cell_key(g, cell, key);
   const double rx = xs[b], ry = xs[n + b], rz = xs[2 * n + b];
@@ -717,18 +728,27 @@ __global__ __launch_bounds__(kWave * kEdgeWaves) void k_ace_edges(const Grid* __
   // (so that the reverse edge's R is exactly -R on every path)
   const double r2max = skin_rows ? g.cs2 : g.c2;
   const float c2hi = (float)(r2max * (1.0 + 1e-4));
+  // direct reverse index (k_ace_rev): each centre's segment starts, when the
+  // stencil has radius 1 and a row fits 8-bit slots, and each edge's rank
+  // among this cell's centres that reach its neighbour through the same
+  // entry, when the cell is also one group of centres
+  const bool seg_ok = pass && eseg && g.nstencil == kSegEntries && ecap <= 255;
+  const bool direct = seg_ok && cnt <= kWave;
+  const uint64_t below = (1ull << lane) - 1;
   for (int ag = 0; ag < cnt; ag += kWave) {
     const int na = min(kWave, cnt - ag);
     const int64_t mine = b + ag + lane;
-    int pos = 0;  // the centre's next slot (lane of the same index)
-    int deg = 0, cut = 0;
-    if (wid == 0 && lane < na) {
-      s_cx[lane] = xs[mine];
-      s_cy[lane] = xs[n + mine];
-      s_cz[lane] = xs[2 * n + mine];
-      s_ax[lane] = (float)(xs[mine] - rx);
-      s_ay[lane] = (float)(xs[n + mine] - ry);
-      s_az[lane] = (float)(xs[2 * n + mine] - rz);
+    if (wid == 0) {
+      if (lane < na) {
+        s_cx[lane] = xs[mine];
+        s_cy[lane] = xs[n + mine];
+        s_cz[lane] = xs[2 * n + mine];
+        s_ax[lane] = (float)(xs[mine] - rx);
+        s_ay[lane] = (float)(xs[n + mine] - ry);
+        s_az[lane] = (float)(xs[2 * n + mine] - rz);
+      }
+      s_pos[lane] = 0;
+      s_cut[lane] = 0;
     }
     for (int e0 = 0; e0 < g.nstencil; e0 += kWave) {
       const int e = e0 + lane;
@@ -759,93 +779,140 @@ __global__ __launch_bounds__(kWave * kEdgeWaves) void k_ace_edges(const Grid* __
       __syncthreads();
       const int total = s_pref[kWave];
       const int ez = s_zero;
-      // direct reverse index (k_ace_rev): each centre's segment starts, when the
-      // stencil has radius 1 and a row fits 8-bit slots, and each edge's rank
-      // among this cell's centres that reach its neighbour through the same
-      // entry, when the cell is also one atom group of one wavefront
-      const bool seg_ok = pass && eseg && g.nstencil == kSegEntries && ecap <= 255;
-      const bool direct = seg_ok && cnt <= kWave && kEdgeWaves == 1;
-      for (int t0 = 0; t0 < total; t0 += kWave) {
-        const int t = t0 + lane;
-        const bool cand = t < total;
-        int m = 0, lo = 0;
-        double xj = 0, yj = 0, zj = 0;
-        float fx = 1e30f, fy = 1e30f, fz = 1e30f;
-        if (cand) {
-          int hi = kWave;
-          while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (s_pref[mid] <= t)
-              lo = mid;
-            else
-              hi = mid;
+      const int nchunks = (total + kWave - 1) / kWave;
+      for (int r0 = 0; r0 < nchunks; r0 += kEdgeChunks) {
+        const int nq = min(kEdgeChunks, nchunks - r0);
+        // this wavefront's chunks q = wid + kEdgeWaves j of the round: stage
+        // each (kept in registers for the writes) and ballot its hits per centre
+        int cm[kEdgeRound], clo[kEdgeRound];
+        double cxj[kEdgeRound], cyj[kEdgeRound], czj[kEdgeRound];
+        bool ccand[kEdgeRound];
+#pragma unroll
+        for (int j = 0; j < kEdgeRound; ++j) {
+          const int q = wid + kEdgeWaves * j;
+          const int t = (r0 + q) * kWave + lane;
+          const bool cand = q < nq && t < total;
+          int m = 0, lo = 0;
+          double xj = 0, yj = 0, zj = 0;
+          float fx = 1e30f, fy = 1e30f, fz = 1e30f;
+          if (cand) {
+            int hi = kWave;
+            while (hi - lo > 1) {
+              const int mid = (lo + hi) >> 1;
+              if (s_pref[mid] <= t)
+                lo = mid;
+              else
+                hi = mid;
+            }
+            m = s_start[lo] + (t - s_pref[lo]);
+            xj = xs[m];
+            yj = xs[n + m];
+            zj = xs[2 * n + m];
+            fx = (float)((xj + s_sh[lo][0]) - rx);
+            fy = (float)((yj + s_sh[lo][1]) - ry);
+            fz = (float)((zj + s_sh[lo][2]) - rz);
           }
-          m = s_start[lo] + (t - s_pref[lo]);
-          xj = xs[m];
-          yj = xs[n + m];
-          zj = xs[2 * n + m];
-          fx = (float)((xj + s_sh[lo][0]) - rx);
-          fy = (float)((yj + s_sh[lo][1]) - ry);
-          fz = (float)((zj + s_sh[lo][2]) - rz);
-        }
-        int hitc = 0;  // centres so far that reach this candidate (ascending sorted index)
-        const bool seg_first = cand && t == s_pref[lo];  // the first candidate of its entry
-        for (int a = wid; a < na; a += kEdgeWaves) {
-          const int64_t k = b + ag + a;
-          bool hit = false;
-          double d[3] = {0, 0, 0};
-          const float ux = fx - s_ax[a], uy = fy - s_ay[a], uz = fz - s_az[a];
-          if (ux * ux + uy * uy + uz * uz < c2hi && !(lo == ez && m == (int)k)) {
-            if (skin_rows) {
+          ccand[j] = cand;
+          cm[j] = m;
+          clo[j] = lo;
+          cxj[j] = xj;
+          cyj[j] = yj;
+          czj[j] = zj;
+          if (q < nq) {  // uniform over the wavefront
+            for (int a = 0; a < na; ++a) {
+              bool hit = false;
+              const float ux = fx - s_ax[a], uy = fy - s_ay[a], uz = fz - s_az[a];
+              if (ux * ux + uy * uy + uz * uz < c2hi && !(lo == ez && m == (int)(b + ag + a))) {
+                if (skin_rows) {
 #pragma clang fp contract(off)
-              d[0] = (xj - s_cx[a]) + s_sh[lo][0];
-              d[1] = (yj - s_cy[a]) + s_sh[lo][1];
-              d[2] = (zj - s_cz[a]) + s_sh[lo][2];
-              hit = d[0] * d[0] + d[1] * d[1] + d[2] * d[2] < r2max;
-            } else {
-              int img[3];
-              hit = exact_member(g, s_cx[a], s_cy[a], s_cz[a], xj, yj, zj, s_lat[lo], s_sh[lo], d, img);
+                  const double d0 = (xj - s_cx[a]) + s_sh[lo][0];
+                  const double d1 = (yj - s_cy[a]) + s_sh[lo][1];
+                  const double d2 = (zj - s_cz[a]) + s_sh[lo][2];
+                  hit = d0 * d0 + d1 * d1 + d2 * d2 < r2max;
+                } else {
+                  double d[3];
+                  int img[3];
+                  hit = exact_member(g, s_cx[a], s_cy[a], s_cz[a], xj, yj, zj, s_lat[lo], s_sh[lo], d, img);
+                }
+              }
+              const uint64_t mask = __ballot(hit);
+              if (lane == 0) s_mask[q][a] = mask;
             }
           }
-          const uint64_t mask = __ballot(hit);
-          const int h = __popcll(mask);
-          if (skin_rows) {  // of which inside the cutoff itself (a launch-shape hint)
-            const int hc = __popcll(__ballot(hit && d[0] * d[0] + d[1] * d[1] + d[2] * d[2] < g.c2));
-            if (lane == a) cut += hc;
+        }
+        __syncthreads();
+        if (wid == 0 && lane < na) {  // each centre's first slot in each chunk, in chunk order
+          int acc = s_pos[lane];
+          for (int q = 0; q < nq; ++q) {
+            s_off[q][lane] = acc;
+            acc += __popcll(s_mask[q][lane]);
           }
-          const int base = __shfl(pos, a, kWave);  // outside the branch: the source lane must be active
-          const int slot = base + __popcll(mask & ((1ull << lane) - 1));
-          if (pass && hit && slot < ecap) {
-            const int64_t q = k * ecap + slot;
-            enb[q] = m;
-            eent[q] = e0 + lo;
-            eR[3 * q] = d[0];
-            eR[3 * q + 1] = d[1];
-            eR[3 * q + 2] = d[2];
-            if (erank) erank[q] = direct ? hitc : -1;
-          }
-          if (seg_ok && seg_first && slot < ecap) eseg[k * kSegEntries + lo] = (uint8_t)slot;
-          hitc += hit ? 1 : 0;
-          if (lane == a) {
-            pos += h;
-            deg += h;
+          s_pos[lane] = acc;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kEdgeRound; ++j) {
+          const int q = wid + kEdgeWaves * j;
+          if (q >= nq) continue;  // uniform over the wavefront
+          const int m = cm[j], lo = clo[j];
+          const bool seg_first = ccand[j] && (r0 + q) * kWave + lane == s_pref[lo];  // the first candidate of its entry
+          int hitc = 0;  // centres so far that reach this candidate (ascending sorted index)
+          for (int a = 0; a < na; ++a) {
+            const int64_t k = b + ag + a;
+            const uint64_t mask = s_mask[q][a];
+            const bool hit = (mask >> lane) & 1ull;
+            const int slot = s_off[q][a] + __popcll(mask & below);
+            double d[3] = {0, 0, 0};
+            if (hit) {  // R again, by the same operations as the test
+              if (skin_rows) {
+#pragma clang fp contract(off)
+                d[0] = (cxj[j] - s_cx[a]) + s_sh[lo][0];
+                d[1] = (cyj[j] - s_cy[a]) + s_sh[lo][1];
+                d[2] = (czj[j] - s_cz[a]) + s_sh[lo][2];
+              } else {
+                member_disp(g, s_cx[a], s_cy[a], s_cz[a], cxj[j], cyj[j], czj[j], s_lat[lo], s_sh[lo], d);
+              }
+            }
+            if (skin_rows) {  // of which inside the cutoff itself (a launch-shape hint)
+              const int hc = __popcll(__ballot(hit && d[0] * d[0] + d[1] * d[1] + d[2] * d[2] < g.c2));
+              if (lane == 0 && hc) atomicAdd(&s_cut[a], hc);
+            }
+            if (pass && hit && slot < ecap) {
+              const int64_t qq = k * ecap + slot;
+              enb[qq] = m;
+              eent[qq] = e0 + lo;
+              eR[3 * qq] = d[0];
+              eR[3 * qq + 1] = d[1];
+              eR[3 * qq + 2] = d[2];
+              if (erank) erank[qq] = direct ? hitc : -1;
+            }
+            if (seg_ok && seg_first && slot < ecap) eseg[k * kSegEntries + lo] = (uint8_t)slot;
+            hitc += hit ? 1 : 0;
           }
         }
+        __syncthreads();  // the ballots and offsets are rewritten by the next round
       }
-      __syncthreads();
     }
-    if (lane < na && lane % kEdgeWaves == wid) ecount[mine] = min(deg, ecap);
-    // one atomic per wavefront and word (same-address atomics serialise across the device)
-    int dmax = deg, cmax = cut;
-    for (int off = kWave / 2; off > 0; off >>= 1) {
-      dmax = max(dmax, __shfl_xor(dmax, off, kWave));
-      cmax = max(cmax, __shfl_xor(cmax, off, kWave));
+    if (wid == 0) {
+      const int deg = lane < na ? s_pos[lane] : 0, cut = lane < na ? s_cut[lane] : 0;
+      if (lane < na) ecount[mine] = min(deg, ecap);
+      // at most one atomic per workgroup and word, and none when a coherent
+      // read already shows as large a value (the words only grow during the
+      // launch): same-address atomics serialise across the device, about
+      // 11 ns each
+      int dmax = deg, cmax = cut;
+      for (int off = kWave / 2; off > 0; off >>= 1) {
+        dmax = max(dmax, __shfl_xor(dmax, off, kWave));
+        cmax = max(cmax, __shfl_xor(cmax, off, kWave));
+      }
+      if (lane == 0) {
+        if (dmax > __hip_atomic_load(status + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(status + 1, dmax);
+        if (dmax > ecap) atomicOr(status + 2, 1);  // capacity too small: the host repeats the step
+        if (skin_rows && cmax > __hip_atomic_load(status + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+          atomicMax(status + 3, cmax);
+      }
     }
-    if (lane == 0) {
-      atomicMax(status + 1, dmax);
-      if (dmax > ecap) atomicOr(status + 2, 1);  // capacity too small: the host repeats the step
-      if (skin_rows) atomicMax(status + 3, cmax);
-    }
+    __syncthreads();  // the centres' staging and counters are rewritten by the next group
   }
 }
 
@@ -961,7 +1028,11 @@ __global__ __launch_bounds__(256) void k_ace_delta(int64_t n, const int32_t* __r
   if (threadIdx.x == 0) {
     double m = wmax[0];
     for (int w = 1; w < (int)(blockDim.x / kWave); ++w) m = fmax(m, wmax[w]);
-    if (m > 0) atomicMax(&words[parity], (unsigned long long)__double_as_longlong(m));
+    // no atomic when a coherent read already shows as large a value (the word
+    // only grows during the launch; same-address atomics serialise)
+    const unsigned long long mb = (unsigned long long)__double_as_longlong(m);
+    if (m > 0 && mb > __hip_atomic_load(&words[parity], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      atomicMax(&words[parity], mb);
     if (blockIdx.x == 0) words[1 - parity] = 0;
   }
 }
@@ -1861,7 +1932,7 @@ static void ace_environment(pfw_ctx* ctx, int rev) {
     PFW_HIP(hipMemsetAsync(ad.status, 0, 4 * sizeof(int32_t), st));
     hipLaunchKernelGGL(k_ace_edges, dim3((unsigned)ctx->ncells), dim3(kWave * kEdgeWaves), 0, st, s.grids, s.cell_cfg, n, s.xs,
                        s.start, 0, 0, ad.ecount, (int32_t*)nullptr, (int32_t*)nullptr, (double*)nullptr, ad.status, 0,
-                       (int32_t*)nullptr, (uint8_t*)nullptr);
+                       (int32_t*)nullptr, (uint8_t*)nullptr, (int)(ctx->ncfg == 1));
     PFW_HIP(hipGetLastError());
     PFW_HIP(hipMemcpyAsync(ad.h_status, ad.status, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
     PFW_HIP(hipStreamSynchronize(st));
@@ -1873,7 +1944,7 @@ static void ace_environment(pfw_ctx* ctx, int rev) {
   fill_i32(ad.status, 4, 0, st);  // a fill kernel: this sequence may be captured in a graph
   hipLaunchKernelGGL(k_ace_edges, dim3((unsigned)ctx->ncells), dim3(kWave * kEdgeWaves), 0, st, s.grids, s.cell_cfg, n, s.xs,
                      s.start, 1, ad.ecap, ad.ecount, ad.enb, ad.eent, ad.eR, ad.status, 0, rev ? ad.erev : nullptr,
-                     rev ? ad.eseg : nullptr);
+                     rev ? ad.eseg : nullptr, (int)(ctx->ncfg == 1));
   PFW_HIP(hipGetLastError());
   if (rev)
     hipLaunchKernelGGL(k_ace_rev, dim3(nblk(slots, 256)), dim3(256), 0, st, slots, ad.ecap, s.grids, s.atom_cfg,
@@ -1974,7 +2045,7 @@ static void ace_skin_rebuild(pfw_ctx* ctx) {
     PFW_HIP(hipMemsetAsync(ad.status, 0, 4 * sizeof(int32_t), st));
     hipLaunchKernelGGL(k_ace_edges, dim3((unsigned)ctx->ncells), dim3(kWave * kEdgeWaves), 0, st, s.grids, s.cell_cfg, n, s.xs,
                        s.start, pass, pass ? ad.ecap_s : 0, ad.ecount_s, ad.enb_s, ad.eent_s, ad.eR_s, ad.status, 1,
-                       pass ? ad.erev_s : nullptr, pass ? ad.eseg_s : nullptr);
+                       pass ? ad.erev_s : nullptr, pass ? ad.eseg_s : nullptr, (int)(ctx->ncfg == 1));
     PFW_HIP(hipGetLastError());
     if (pass)
       hipLaunchKernelGGL(k_ace_rev, dim3(nblk(n * (int64_t)ad.ecap_s, 256)), dim3(256), 0, st, n * (int64_t)ad.ecap_s,

@@ -130,6 +130,7 @@ Grid nl_configure_one(const double* H, const int* pb, int64_t n, double c, doubl
   for (int a = 0; a < 3; ++a) {
     g.pb[a] = pb[a];
     g.L[a] = L[a];
+    g.invL[a] = 1.0 / L[a];
   }
   if (mi) {
     for (int k = 0; k < 9; ++k) g.basis[k] = g.recip[k] = (k % 4 == 0) ? 1.0 : 0.0;

@@ -69,7 +69,11 @@ __device__ inline bool exact_member(const Grid& g, double xi, double yi, double 
   if (g.minimum_image) {
     for (int a = 0; a < 3; ++a) {
       if (g.pb[a]) {
-        double m = rint(raw[a] / g.L[a]);
+        // m = rint(raw / L). raw (1/L) differs from raw / L by a few ulp, so it
+        // decides the rounding unless it lies within 1e-9 of a half-integer;
+        // there the division decides
+        double m = rint(raw[a] * g.invL[a]);
+        if (fabs(fabs(raw[a] * g.invL[a] - m) - 0.5) < 1e-9) m = rint(raw[a] / g.L[a]);
         d[a] = raw[a] - m * g.L[a];
         img[a] = -(int)m;
         if (img[a] != lat[a]) return false;
@@ -85,6 +89,25 @@ __device__ inline bool exact_member(const Grid& g, double xi, double yi, double 
     }
   }
   return d[0] * d[0] + d[1] * d[1] + d[2] * d[2] < g.c2;
+}
+
+// d of a directed pair that exact_member has accepted through the stencil
+// entry with lattice `lat` and shift `sh`, by the same operations (on the
+// minimum-image path an accepted pair has m = -lat).
+__device__ inline void member_disp(const Grid& g, double xi, double yi, double zi, double xj, double yj, double zj,
+                                   const int* lat, const double* sh, double* d) {
+#pragma clang fp contract(off)
+  const double raw[3] = {xj - xi, yj - yi, zj - zi};
+  for (int a = 0; a < 3; ++a) {
+    if (!g.minimum_image) {
+      d[a] = raw[a] + sh[a];
+    } else if (g.pb[a]) {
+      const double m = -(double)lat[a];
+      d[a] = raw[a] - m * g.L[a];
+    } else {
+      d[a] = raw[a];
+    }
+  }
 }
 
 // Global cell index <-> key of the configuration's grid.

@@ -54,6 +54,7 @@ struct Grid {
   double basis[9];  // column-major periodic basis (storage.jl:83-108), identity if minimum image
   double recip[9];  // its inverse
   double L[3];      // diag(H) for the minimum-image path
+  double invL[3];   // 1 / L (the minimum-image rounding's fast path)
   double c, c2;
   double cs2;         // squared radius of the Verlet list (cutoff + skin, padded); c2 without a skin
   double reach[3];    // cell reach of the search radius (cutoff + skin)
