@@ -100,7 +100,8 @@ __host__ __device__ inline size_t align16(size_t b) { return (b + 15) & ~(size_t
 template <typename T>
 struct RadialPoint {
   int k;
-  T u, e, de, dx, invstep;
+  T u, e, de;
+  T ed;  // e dx/dr / (interval length): the envelope times du/dr
 };
 
 template <typename T>
@@ -123,7 +124,6 @@ __device__ inline RadialPoint<T> radial_point(const FamArgs<T>& f, int ch, T r) 
     dx = (T)-2 * a * du / (t * t * (ycut - yin));
   }
   RadialPoint<T> o;
-  o.dx = dx;
   const int kind = f.env_kind[ch];
   const int e0 = f.env_pow[2 * ch], e1 = f.env_pow[2 * ch + 1];
   if (kind == PFW_ENV_POLY2SX) {
@@ -174,9 +174,10 @@ __device__ inline RadialPoint<T> radial_point(const FamArgs<T>& f, int ch, T r) 
     k = lo > f.nint - 1 ? f.nint - 1 : lo;
   }
   const T step = nodes[k + 1] - nodes[k];
+  const T invstep = (T)1 / step;
   o.k = k;
-  o.invstep = (T)1 / step;
-  o.u = (x - nodes[k]) * o.invstep;
+  o.u = (x - nodes[k]) * invstep;
+  o.ed = o.e * dx * invstep;
   return o;
 }
 
@@ -184,10 +185,13 @@ template <typename T>
 __device__ inline void radial_value(const FamArgs<T>& f, int ch, const RadialPoint<T>& P, int n, T& v, T& d) {
   const T* c = f.coef + (((int64_t)ch * f.nint + P.k) * f.nr + n) * 4;
   const T c0 = c[0], c1 = c[1], c2 = c[2], c3 = c[3];
-  const T poly = ((c3 * P.u + c2) * P.u + c1) * P.u + c0;
-  const T dpoly = (((T)3 * c3 * P.u + (T)2 * c2) * P.u + c1) * P.invstep;
+  // Horner for the cubic and its u-derivative, sharing p1 = c3 u + c2:
+  // 3 c3 u^2 + 2 c2 u + c1 = (c3 u + 2 p1) u + c1
+  const T p1 = c3 * P.u + c2;
+  const T poly = (p1 * P.u + c1) * P.u + c0;
+  const T dpoly = (c3 * P.u + (p1 + p1)) * P.u + c1;
   v = P.e * poly;
-  d = P.de * poly + P.e * dpoly * P.dx;
+  d = P.de * poly + P.ed * dpoly;
 }
 
 template <typename T>
@@ -209,6 +213,11 @@ __device__ inline Jet<T> jadd(const Jet<T>& a, const Jet<T>& b) {
 template <typename T>
 __device__ inline Jet<T> jsub(const Jet<T>& a, const Jet<T>& b) {
   return {a.v - b.v, a.g0 - b.g0, a.g1 - b.g1, a.g2 - b.g2};
+}
+// s a - t b
+template <typename T>
+__device__ inline Jet<T> jlin(T s, const Jet<T>& a, T t, const Jet<T>& b) {
+  return {s * a.v - t * b.v, s * a.g0 - t * b.g0, s * a.g1 - t * b.g1, s * a.g2 - t * b.g2};
 }
 
 // Real L2 Y_lm (no Condon-Shortley phase, index l*l+l+m) of the direction of
@@ -255,8 +264,8 @@ __device__ inline void ylm_emit(T dx, T dy, T dz, const T* Fl, int solid, Emit e
         q = diag;
       else if (l == m + 1)
         q = js((T)(2 * l - 1), jm(z, q1));
-      else
-        q = js((T)1 / (T)(l - m), jsub(js((T)(2 * l - 1), jm(z, q1)), js((T)(l + m - 1), jm(r2, q2))));
+      else  // the constants (2l-1)/(l-m) and (l+m-1)/(l-m) folded: one multiply and one FMA per component
+        q = jlin((T)(2 * l - 1) / (T)(l - m), jm(z, q1), (T)(l + m - 1) / (T)(l - m), jm(r2, q2));
       const T F = Fl[l * (L + 1) + m];
       emit(l * l + l + m, js(F, jm(q, cm)));
       if (m > 0) emit(l * l + l - m, js(F, jm(q, sgn)));

@@ -186,7 +186,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
 
   const int lane = threadIdx.x;
   const int cell = xcd_remap(blockIdx.x, gridDim.x);
-  const Grid& g = grids[cell_cfg[cell]];
+  const Grid& g = grids[cell_cfg ? cell_cfg[cell] : 0];  // no table for one configuration: no dependent load
   const T c2 = (T)g.c2;
   const T c2lo = (T)(g.c2 * (1.0 - Band<T>::rel)), c2hi = (T)(g.c2 * (1.0 + Band<T>::rel));
   const T invc2 = (T)(1.0 / g.c2);
@@ -537,14 +537,15 @@ static void launch_pair(pfw_ctx* ctx, uint32_t mask) {
   const LJConsts lj = lj_consts(ctx->pair);
   const size_t dyn = KIND == PFW_PAIR_SMOOTH ? 2 * sizeof(T) * ctx->pair.nz * ctx->pair.nz : 0;
   const int slices = ctx->tune[3] > 0 ? std::min(ctx->tune[3], kSlices) : kSlices;
+  const int32_t* cfgs = ctx->ncfg == 1 ? nullptr : s.cell_cfg;  // one configuration: grid 0 for every cell
   if (ctx->audit) {
     const AuditBufs au = audit_begin(ctx);
     hipLaunchKernelGGL((k_pair<T, KIND, true>), dim3(ctx->ncells), dim3(kWave), dyn, ctx->stream, s.grids,
-                       s.cell_cfg, s.n, s.start, s.xs, s.xr, s.cbox, s.sps, s.perm, ctx->pair.d_params, ctx->pair.nz, lj,
+                       cfgs, s.n, s.start, s.xs, s.xr, s.cbox, s.sps, s.perm, ctx->pair.d_params, ctx->pair.nz, lj,
                        mask, ctx->d_F, ctx->d_partials, ctx->ncells, au, slices);
   } else {
     hipLaunchKernelGGL((k_pair<T, KIND, false>), dim3(ctx->ncells), dim3(kWave), dyn, ctx->stream, s.grids,
-                       s.cell_cfg, s.n, s.start, s.xs, s.xr, s.cbox, s.sps, s.perm, ctx->pair.d_params, ctx->pair.nz, lj,
+                       cfgs, s.n, s.start, s.xs, s.xr, s.cbox, s.sps, s.perm, ctx->pair.d_params, ctx->pair.nz, lj,
                        mask, ctx->d_F, ctx->d_partials, ctx->ncells, AuditBufs{}, slices);
   }
 }
@@ -660,7 +661,7 @@ constexpr size_t static_round(size_t b) { return (b + kStaticAlign - 1) / kStati
 
 // Byte offsets of the sub-buffers, in order.
 struct StaticLayout {
-  size_t grid, params, flag, out, out2, red, cell_cfg, atom_cfg, count, start, blocksum, bounds, cell_of, rank, perm,
+  size_t grid, params, flag, out, out2, red, atom_cfg, count, start, blocksum, bounds, cell_of, rank, perm,
       sps, xw, u, xs, xr, cbox, partials, total;
 };
 constexpr StaticLayout static_layout() {
@@ -677,7 +678,6 @@ constexpr StaticLayout static_layout() {
   L.out = take(sizeof(double) * 16);
   L.out2 = take(sizeof(double) * 16);
   L.red = take(sizeof(double) * 64 * 16);
-  L.cell_cfg = take(sizeof(int32_t) * kStaticCells);
   L.atom_cfg = take(sizeof(int32_t) * kStaticAtoms);
   L.count = take(sizeof(int32_t) * kStaticCells);
   L.start = take(sizeof(int32_t) * (kStaticCells + 1));
@@ -807,7 +807,6 @@ void pair_run_static(pfw_ctx* ctx, int kind, int nz, const double* params, doubl
     PFW_HIP(hipStreamSynchronize(st));
     if (h_flag[0]) throw Error(PFW_INVALID_ARGUMENT, "positions must be finite and species channels in range");
   }
-  fill_i32((int32_t*)(base + L.cell_cfg), ncells, 0, st);
   fill_i32((int32_t*)(base + L.atom_cfg), n, 0, st);
   BuildView v{n,
               ncells,
@@ -835,7 +834,7 @@ void pair_run_static(pfw_ctx* ctx, int kind, int nz, const double* params, doubl
     const bool fp32 = precision == PFW_MIXED;
     double* part = (double*)(base + L.partials);
 #define PFW_STATIC_LAUNCH(T, K)                                                                                  \
-  launch_pair_static<T, K>(d_grid, (const int32_t*)(base + L.cell_cfg), n, v.start, v.xs, v.xr, v.cbox, v.sps,    \
+  launch_pair_static<T, K>(d_grid, (const int32_t*)nullptr, n, v.start, v.xs, v.xr, v.cbox, v.sps,              \
                            v.perm, d_params, nz, lj, mask, F, part, ncells, st)
     if (kind == PFW_PAIR_SMOOTH) {
       if (fp32)
