@@ -598,8 +598,11 @@ __global__ __launch_bounds__(1024) void k_ace_prod(AceArgs<T> a, int grads) {
 // Force pullback g_e = r^ sum_r R'_r T_r + sum_m U_m grad Y_m with
 // T_r = sum_m dE/dA[r, m] Y_m and U_m = sum_r dE/dA[r, m] R_r (serial.jl:85-101;
 // cpu.jl:226-236), plus the pair radials; edge gradients and site virials.
+// FP32 with lmax <= 4 fits 128 VGPRs without spilling: four wavefronts per SIMD
+// instead of three (larger lmax and FP64 keep the compiler's choice)
 template <typename T, int L, int SPW>
-__global__ __launch_bounds__(64) void k_ace_pull(AceArgs<T> a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && L <= 4 ? 4 : 1))) void k_ace_pull(
+    AceArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int NY = (L + 1) * (L + 1);
   constexpr int LPS = kWave / SPW;
