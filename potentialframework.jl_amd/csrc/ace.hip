@@ -1474,7 +1474,9 @@ static std::vector<ProductSlot> schedule_products(const std::vector<std::vector<
   for (const auto& f : factors)
     for (int d : f) ++occ[d];
   copyb.assign(ndens + 1, 0);
-  for (int d = 0; d < ndens; ++d) copyb[d + 1] = copyb[d] + std::max(1, (occ[d] + ideal - 1) / ideal);
+  // at least two accumulator copies per density, so that a step can usually
+  // pick one whose LDS bank is free in its half-wave (at most 64 are chosen from)
+  for (int d = 0; d < ndens; ++d) copyb[d + 1] = copyb[d] + std::max(2, (occ[d] + ideal - 1) / ideal);
   std::vector<ProductSlot> out;
   order_step[0] = 0;
   for (int nu = 1; nu <= 4; ++nu) {
@@ -1495,9 +1497,10 @@ static std::vector<ProductSlot> schedule_class(const std::vector<std::vector<int
                                                int ndens, int ntarget, int nu_class) {
   std::vector<ProductSlot> out;
   constexpr size_t kWindow = 512;
-  std::vector<std::vector<int>> cnt(4, std::vector<int>(ndens));
+  // per factor slot and density: the accumulator copies this step has taken
+  std::vector<std::vector<uint64_t>> used(4, std::vector<uint64_t>(ndens));
   while (!remaining.empty()) {
-    for (auto& c : cnt) std::fill(c.begin(), c.end(), 0);
+    for (auto& c : used) std::fill(c.begin(), c.end(), 0ull);
     int abank[2][4][32];
     for (auto& h : abank)
       for (auto& t : h)
@@ -1522,15 +1525,26 @@ static std::vector<ProductSlot> schedule_class(const std::vector<std::vector<int
           cand.k = remaining[r];
           for (int t = 0; t < nu && fits; ++t) {
             const int d = f[perm[t]];
-            const int rank = cnt[t][d];
-            if (copyb[d] + rank >= copyb[d + 1]) {
+            // a copy of d not yet taken in this step and slot, preferably in a
+            // bank no lane of this half-wave writes in this slot
+            const int nc = std::min(64, copyb[d + 1] - copyb[d]);
+            int tgt = -1, tcost = 2;
+            for (int c = 0; c < nc && tcost > 0; ++c) {
+              if ((used[t][d] >> c) & 1ull) continue;
+              const int cand_t = copyb[d] + c;
+              const int cc = ((tbank[half][t] >> (cand_t & 31)) & 1u) ? 1 : 0;
+              if (cc < tcost) {
+                tcost = cc;
+                tgt = cand_t;
+              }
+            }
+            if (tgt < 0) {
               fits = false;
               break;
             }
-            const int tgt = copyb[d] + rank;
             const int ab = abank[half][t][d & 31];
             cost += (ab >= 0 && ab != d) ? 1 : 0;
-            cost += ((tbank[half][t] >> (tgt & 31)) & 1u) ? 1 : 0;
+            cost += tcost;
             cand.f[t] = d;
             cand.t[t] = tgt;
           }
@@ -1545,7 +1559,7 @@ static std::vector<ProductSlot> schedule_class(const std::vector<std::vector<int
       if (best_r < remaining.size()) {
         for (int t = 0; t < 4 && best.f[t] >= 0; ++t) {
           const int d = best.f[t];
-          ++cnt[t][d];
+          used[t][d] |= 1ull << (best.t[t] - copyb[d]);
           if (abank[half][t][d & 31] < 0) abank[half][t][d & 31] = d;
           tbank[half][t] |= 1u << (best.t[t] & 31);
         }

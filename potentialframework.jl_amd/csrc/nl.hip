@@ -784,7 +784,9 @@ __global__ void k_finish(const double* __restrict__ part, int nb, int width, int
 
 // E, W and the status word of one configuration: k_colsum, then k_finish.
 // (A single launch whose last block finishes the sum needs a device-scope
-// fence in every block, which writes back the L2 and was measured slower.)
+// fence in every block, which writes back the L2 and was measured slower; one
+// 1,024-thread workgroup doing both stages took 30 us against 9 us for the
+// 256k pair step's 14,440 rows.)
 void reduce_core(const double* rec, int64_t m, int width, int kind, const int32_t* status, double* red, double* out,
                  double* out2, double* out3, hipStream_t st) {
   hipLaunchKernelGGL(k_colsum, dim3(kRedBlocks), dim3(256), 0, st, rec, m, width, red);
