@@ -154,7 +154,7 @@ int pfw_get_timings(pfw_ctx* ctx, double* ms, int n);
  * harnesses that forbid allocation). No counterpart in the reference: it is
  * pfw_set_system + pfw_set_pair_model + pfw_pair_evaluate in one call whose
  * intermediate buffers (cell grid, sorted positions, partial sums) live in a
- * static device area of about 42 MB that each device holds from the first
+ * static device area of about 56 MB that each device holds from the first
  * context created on it until the process ends; nothing is allocated or
  * freed. Inputs X_device (3 x n doubles, column-major, original order) and
  * species_device (n channels) and outputs E_device (1), F_device (3 x n,
