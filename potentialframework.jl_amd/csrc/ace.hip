@@ -908,20 +908,19 @@ __global__ __launch_bounds__(kWave * kEdgeWaves) __attribute__((amdgpu_waves_per
     if (wid == 0) {
       const int deg = lane < na ? s_pos[lane] : 0, cut = lane < na ? s_cut[lane] : 0;
       if (lane < na) ecount[mine] = min(deg, ecap);
-      // at most one atomic per workgroup and word, and none when a coherent
-      // read already shows as large a value (the words only grow during the
-      // launch): same-address atomics serialise across the device, about
-      // 11 ns each
+      // one atomic per workgroup and word (same-address atomics serialise
+      // across the device). Skipping the atomic when a coherent read already
+      // showed as large a value under-reported the maximum in the Verlet list
+      // build (an extra build pass), so every workgroup issues it.
       int dmax = deg, cmax = cut;
       for (int off = kWave / 2; off > 0; off >>= 1) {
         dmax = max(dmax, __shfl_xor(dmax, off, kWave));
         cmax = max(cmax, __shfl_xor(cmax, off, kWave));
       }
       if (lane == 0) {
-        if (dmax > __hip_atomic_load(status + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(status + 1, dmax);
+        atomicMax(status + 1, dmax);
         if (dmax > ecap) atomicOr(status + 2, 1);  // capacity too small: the host repeats the step
-        if (skin_rows && cmax > __hip_atomic_load(status + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-          atomicMax(status + 3, cmax);
+        if (skin_rows) atomicMax(status + 3, cmax);
       }
     }
     __syncthreads();  // the centres' staging and counters are rewritten by the next group
@@ -1040,11 +1039,7 @@ __global__ __launch_bounds__(256) void k_ace_delta(int64_t n, const int32_t* __r
   if (threadIdx.x == 0) {
     double m = wmax[0];
     for (int w = 1; w < (int)(blockDim.x / kWave); ++w) m = fmax(m, wmax[w]);
-    // no atomic when a coherent read already shows as large a value (the word
-    // only grows during the launch; same-address atomics serialise)
-    const unsigned long long mb = (unsigned long long)__double_as_longlong(m);
-    if (m > 0 && mb > __hip_atomic_load(&words[parity], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-      atomicMax(&words[parity], mb);
+    if (m > 0) atomicMax(&words[parity], (unsigned long long)__double_as_longlong(m));
     if (blockIdx.x == 0) words[1 - parity] = 0;
   }
 }

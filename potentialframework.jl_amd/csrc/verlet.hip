@@ -245,7 +245,7 @@ __global__ __launch_bounds__(64) void k_verlet_build(const Grid* __restrict__ gr
       dsum += __shfl_xor(dsum, off, kWave);
     }
     if (lane == 0) {
-      if (dmax > __hip_atomic_load(status + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(status + 1, dmax);
+      atomicMax(status + 1, dmax);
       if (dmax > ecap) atomicOr(status + 2, 1);  // capacity too small: the host grows it and builds again
       if (pass) atomicAdd(reinterpret_cast<unsigned long long*>(status + 4), (unsigned long long)dsum);
     }
@@ -282,11 +282,7 @@ __global__ __launch_bounds__(256) void k_verlet_refresh(int64_t n, int64_t n_ext
   if (threadIdx.x == 0) {
     double m = wmax[0];
     for (int w = 1; w < (int)(blockDim.x / kWave); ++w) m = fmax(m, wmax[w]);
-    // no atomic when a coherent read already shows as large a value (the word
-    // only grows during the launch; same-address atomics serialise)
-    const unsigned long long mb = (unsigned long long)__double_as_longlong(m);
-    if (m > 0 && mb > __hip_atomic_load(&words[parity], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-      atomicMax(&words[parity], mb);
+    if (m > 0) atomicMax(&words[parity], (unsigned long long)__double_as_longlong(m));
     if (blockIdx.x == 0) words[1 - parity] = 0;
   }
 }
