@@ -629,11 +629,12 @@ def bench_argon_4000(pfw, lib, _capi, args, host):
     return out
 
 
-def profile_run(pfw, lib, _capi, which, prec, steps):
+def profile_run(pfw, lib, _capi, which, prec, steps, density=0):
     """A short run for profilers: `steps` device steps of one workload after
     one set-up evaluation (pair: the 256k B2 SmoothPair step; ace: the 32k BCC
     ACE step; may30: the 30,000-atom May30 step; argon: the 4,000-atom LJ
-    step), then one line naming what ran."""
+    step), then one line naming what ran. `density` selects the ACE density
+    contraction (the ace_density_contraction knob)."""
     mode = _capi.PFW_MIXED if prec == "mixed" else _capi.PFW_FP64
     if which == "design":  # the linear design of the first 64 fitting-set configurations
         from pfw_amd.ace import ELEMENTS
@@ -681,12 +682,14 @@ def profile_run(pfw, lib, _capi, which, prec, steps):
             H, pbc = 5 * H0, (True, True, True)
             labels = [ELEMENTS[z - 1] for z in d["species"]] * len(shifts)
         calc = pfw.ACECalculator(model, ps, precision=prec)
+        calc.workspace.set_tuning(ace_density_contraction=density)
         calc.evaluate(pfw.System(X, H, pbc, labels))
         h, step = calc.workspace.ctx.h, lib.pfw_ace_step
     for _ in range(steps):
         _capi.check(h, step(h, 7, mode))
     _capi.check(h, lib.pfw_synchronize(h))
-    print(json.dumps({"profile": which, "precision": prec, "atoms": len(X), "steps": steps}), flush=True)
+    print(json.dumps({"profile": which, "precision": prec, "atoms": len(X), "steps": steps,
+                      "ace_density_contraction": density}), flush=True)
 
 
 def calibrate_copy(nbytes=256 << 20, reps=5):
@@ -719,6 +722,8 @@ def main():
     ap.add_argument("--profile", choices=("pair", "ace", "may30", "argon", "design"), default=None,
                     help="only run --steps device steps of one workload (for rocprofv3)")
     ap.add_argument("--precision", choices=("mixed", "float64"), default="mixed", help="precision of --profile")
+    ap.add_argument("--ace-density-contraction", type=int, choices=(0, 1, 2), default=0,
+                    help="ACE density contraction of --profile: 0 built-in, 1 VALU, 2 matrix cores")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
@@ -732,7 +737,7 @@ def main():
     from pfw_amd import _capi
 
     if args.profile:
-        profile_run(pfw, _capi.load(), _capi, args.profile, args.precision, args.steps)
+        profile_run(pfw, _capi.load(), _capi, args.profile, args.precision, args.steps, args.ace_density_contraction)
         return
     if not args.no_cpu_baseline:
         sys.path.insert(0, ROOT)  # the oracle: CPU baseline and accuracy checks only
@@ -887,6 +892,14 @@ def main():
             secs, stage = time_steps(lib, h, lib.pfw_ace_step, args.steps, args.warmup, mode)
             gsecs, _ = graph_steps(lib, _capi, calc.workspace, lib.pfw_ace_step, args.steps, args.warmup, mode)
             ace[prec] = dict(seconds=secs, stage_ms=stage, result=res, calc=calc, graph_seconds=gsecs)
+        # the density contraction on the VALU and on the matrix cores, same calculator
+        dcw = ace["mixed"]["calc"].workspace
+        dens_ms = {}
+        for name, dc in (("valu", 1), ("mfma", 2)):
+            dcw.set_tuning(ace_density_contraction=dc)
+            dsecs, dstage = time_steps(lib, dcw.ctx.h, lib.pfw_ace_step, args.steps, args.warmup, _capi.PFW_MIXED)
+            dens_ms[name] = {"ms_per_step": round(1e3 * dsecs / args.steps, 4), "interaction_ms": round(float(dstage[4]), 5)}
+        dcw.set_tuning(ace_density_contraction=0)
         edges = len(ace["mixed"]["calc"].workspace.export_pairs()[0])
         na = len(Xa)
         aflops, per_edge, per_site = ace_flops(model, edges, na)
@@ -901,8 +914,8 @@ def main():
             "unit": "Matom-evals/s",
             "ms_per_step": round(1e3 * ace["mixed"]["seconds"] / args.steps, 4),
             "fp64_ms_per_step": round(1e3 * ace["float64"]["seconds"] / args.steps, 4),
-            "roofline": {"kernel": "k_ace_dens+k_ace_prod+k_ace_pull<float,4>", "bound": "valu",
-                         "compute_unit": "VALU fp32",
+            "roofline": {"kernel": "k_ace_dens_mfma<4,2>+k_ace_prod+k_ace_pull<float,4>", "bound": "valu",
+                         "compute_unit": "VALU fp32 (the density contraction on FP32 MFMA)",
                          "achieved": round(aflops / (ka * 1e-3) / 1e12, 3), "peak": PEAK_FP32_TFLOPS,
                          "unit": "TFLOP/s", "frac": round(aflops / (ka * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4),
                          "frac_of_unpacked_issue": round(aflops / (ka * 1e-3) / 1e12 / PEAK_FP32_UNPACKED_TFLOPS, 4),
@@ -913,6 +926,7 @@ def main():
             "fp64_roofline_frac": round(aflops / (ka64 * 1e-3) / 1e12 / PEAK_FP64_TFLOPS, 4),
             "fp64_kernel_ms": round(ka64, 5),
             "stages_ms": {k: round(float(v), 5) for k, v in zip(STAGES, ace["mixed"]["stage_ms"])},
+            "density_contraction": dens_ms,
         }
         # the autotuner's sweep on this workload (a fresh calculator; knob and graph choices)
         import tempfile
@@ -922,7 +936,8 @@ def main():
             tp = pfw.Workspace(precision="mixed")
             repp = pfw.autotune(tp, sys_, p, steps=10, warmup=2, cache=os.path.join(td, "tuning.json"))
         default = [c for c in rep["candidates"] if c[0] == {"ace_sites_per_wave": 2, "ace_radial_chunk": 20,
-                                                             "ace_product_waves": 16} and not c[1]]
+                                                             "ace_product_waves": 16, "ace_density_contraction": 2}
+                   and not c[1]]
         line["autotune"] = {
             "ace_bcc_32k_mixed": {"best": rep["best"], "graphs": rep["graphs"], "ms_per_step": round(rep["ms"], 4),
                                   "built_in_ms": round(default[0][2], 4) if default else None,

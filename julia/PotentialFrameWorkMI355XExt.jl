@@ -80,13 +80,15 @@ function rebuilds(w::MI355XWorkspace)
     info[1]
 end
 
-"""Launch knobs of GPUConfig's strategy/workgroup/tile (gpu.jl:61-73); 0 keeps the
-built-in choice: ACE sites per wavefront, ACE radial tile, ACE product wavefronts,
-pair lanes per atom (pfw_set_tuning; tune.py: autotune sweeps them)."""
+"""Launch knobs of GPUConfig's strategy/workgroup/tile (gpu.jl:61-73) and the ACE
+strategy (ACE/gpu.jl:301-303); 0 keeps the built-in choice: ACE sites per
+wavefront, ACE radial tile, ACE product wavefronts, pair lanes per atom, ACE
+density contraction (1 VALU, 2 matrix cores) (pfw_set_tuning; tune.py: autotune
+sweeps them)."""
 function set_tuning!(w::MI355XWorkspace; ace_sites_per_wave=0, ace_radial_chunk=0, ace_product_waves=0,
-                     pair_lanes_per_atom=0)
-    k = Int32[ace_sites_per_wave, ace_radial_chunk, ace_product_waves, pair_lanes_per_atom]
-    check(w.ctx, ccall((:pfw_set_tuning, LIB), Cint, (Ptr{Cvoid}, Ptr{Int32}, Cint), w.ctx, k, 4))
+                     pair_lanes_per_atom=0, ace_density_contraction=0)
+    k = Int32[ace_sites_per_wave, ace_radial_chunk, ace_product_waves, pair_lanes_per_atom, ace_density_contraction]
+    check(w.ctx, ccall((:pfw_set_tuning, LIB), Cint, (Ptr{Cvoid}, Ptr{Int32}, Cint), w.ctx, k, 5))
 end
 
 """Replay fresh steps from a captured HIP graph while only positions change (pfw_set_graphs)."""

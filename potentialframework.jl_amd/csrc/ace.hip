@@ -32,6 +32,9 @@ namespace pfw {
 
 constexpr int kStride = kWave + 1;
 constexpr int kRadialChunk = 20;  // radial indices per density-kernel tile
+// the built-in density contraction: the matrix cores where the model's tiles
+// fit (FP32, unprojected); 32k-atom BCC, lmax 4: 127.9 -> 124.0 us per launch
+constexpr bool kMfmaDensDefault = true;
 
 template <typename T>
 struct FamArgs {
@@ -415,6 +418,157 @@ __global__ __launch_bounds__(64) void k_ace_dens(AceArgs<T> a) {
     for (int off = LPS / 2; off > 0; off >>= 1) epair += __shfl_xor(epair, off, kWave);
     if (ll == 0) a.siteE[site] = epair;
   }
+}
+
+// The density contraction on the matrix cores (FP32, unprojected models).
+// The densities of one l form the block A_l = sum_e R_l(e)^T Y_l(e) of radial
+// indices of degree l by the 2l+1 harmonics of l. The host cuts each block
+// (per radial chunk) into tiles of 4 radial indices x 4 harmonics and packs
+// 16 tiles into a group: one v_mfma_f32_4x4x1_16b_f32 per group and edge
+// adds the 16 outer products of the edge's radial rows and harmonic columns
+// to the group's accumulators. Lane 4b+i supplies row i and column i of tile
+// b; after the last edge, register i of lane 4b+j holds the tile's entry
+// (i, j). Partial tiles read a zero row or column. Staging (radials and Y_lm
+// per edge in LDS) is k_ace_dens's; the accumulators stay in registers over
+// every chunk and are written to Abuf once.
+constexpr int kMfmaGroups = 4;  // tile groups a site keeps in registers (4 x 4 VGPRs per site)
+typedef float mfma_v4 __attribute__((ext_vector_type(4)));
+struct MfmaTiles {
+  const int32_t* tab;  // [groups][6][64]: row, column, densities of registers 0..3
+  int groups;
+  int chunk[kMfmaGroups];
+};
+
+template <int L, int SPW>
+__global__ __launch_bounds__(64) void k_ace_dens_mfma(AceArgs<float> a, MfmaTiles t) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int NY = (L + 1) * (L + 1);
+  constexpr int LPS = kWave / SPW;
+  float* Fl = (float*)smem;                     // (L+1)^2
+  float* Rb = Fl + NY;                          // (rchunk + 1) x kStride: radial tile, then a zero row
+  float* Yb = Rb + (a.rchunk + 1) * kStride;    // (NY + 1) x kStride: harmonics, then a zero row
+  const int lane = threadIdx.x;
+  const int sl = lane / LPS, ll = lane % LPS;
+  const int64_t site = (int64_t)blockIdx.x * SPW + sl;
+  const bool valid = site < a.n;
+  fill_fl<float, L>(Fl);
+  for (int k = lane; k < kStride; k += kWave) {
+    Rb[a.rchunk * kStride + k] = 0;
+    Yb[NY * kStride + k] = 0;
+  }
+  int row[kMfmaGroups], col[kMfmaGroups];
+  mfma_v4 acc[SPW][kMfmaGroups];
+#pragma unroll
+  for (int g = 0; g < kMfmaGroups; ++g) {
+    row[g] = g < t.groups ? t.tab[(6 * g) * kWave + lane] : a.rchunk;
+    col[g] = g < t.groups ? t.tab[(6 * g + 1) * kWave + lane] : NY;
+#pragma unroll
+    for (int s = 0; s < SPW; ++s) acc[s][g] = mfma_v4{0, 0, 0, 0};
+  }
+  const int iz = valid ? a.sps[site] : 0;
+  const int64_t e0 = valid ? site * a.ecap : 0;
+  const int ne = valid ? a.ecount[site] : 0;
+  int nemax = ne;
+  for (int off = LPS; off < kWave; off <<= 1) nemax = max(nemax, __shfl_xor(nemax, off, kWave));
+  const int nq = a.pair.nr;
+  double epair = 0;
+  __syncthreads();
+  for (int c0 = 0; c0 < nemax; c0 += LPS) {
+    const int e = c0 + ll;
+    const bool on = e < ne;
+    int ch = 0;
+    RadialPoint<float> P{};
+    if (on) {
+      const double* R = a.eR + 3 * (int64_t)(e0 + e);
+      const float dx = (float)R[0], dy = (float)R[1], dz = (float)R[2];
+      const float r = (float)sqrt(R[0] * R[0] + R[1] * R[1] + R[2] * R[2]);
+      const int jz = a.sps[a.enb[e0 + e]];
+      ch = iz + a.nz * jz;
+      P = radial_point(a.rad, ch, r);
+      if (nq && ((a.pair_gate >> jz) & 1u)) {
+        const RadialPoint<float> PQ = radial_point(a.pair, ch, r);
+        float sp = 0;
+        for (int n = 0; n < nq; ++n) {
+          float v, d;
+          radial_value(a.pair, ch, PQ, n, v, d);
+          sp += v * a.wp[iz * nq + n];
+        }
+        epair += (double)sp;
+      }
+      float Y[NY], G[3 * NY];
+      ylm<float, L>(dx, dy, dz, Fl, Y, G, a.solid);
+#pragma unroll
+      for (int m = 0; m < NY; ++m) Yb[m * kStride + lane] = Y[m];
+    }
+    // edges of each half in this chunk (wave-uniform)
+    int cns[SPW];
+    int emax = 0;
+#pragma unroll
+    for (int s = 0; s < SPW; ++s) {
+      cns[s] = __builtin_amdgcn_readfirstlane(max(0, min(LPS, __shfl(ne, s * LPS, kWave) - c0)));
+      emax = max(emax, cns[s]);
+    }
+    for (int r0 = 0, q = 0; r0 < a.rad.nr; r0 += a.rchunk, ++q) {
+      if (on) {
+        const int r1 = min(a.rad.nr, r0 + a.rchunk);
+#pragma unroll 8
+        for (int n = r0; n < r1; ++n) {
+          float v, d;
+          radial_value(a.rad, ch, P, n, v, d);
+          Rb[(n - r0) * kStride + lane] = v;
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int g = 0; g < kMfmaGroups; ++g) {
+        if (g < t.groups && t.chunk[g] == q) {
+          const float* rr = Rb + row[g] * kStride;
+          const float* yy = Yb + col[g] * kStride;
+          // four edges at a time, the operand loads ahead of the MFMAs; a
+          // half past its last edge multiplies zeros (the stale columns
+          // beyond it may hold anything, so both operands are replaced).
+          // Columns stay below (s + 1) LPS since LPS is a multiple of 4.
+          for (int ee = 0; ee < emax; ee += 4) {
+            float x[SPW][4], y[SPW][4];
+#pragma unroll
+            for (int s = 0; s < SPW; ++s)
+#pragma unroll
+              for (int u = 0; u < 4; ++u) {
+                x[s][u] = rr[s * LPS + ee + u];
+                y[s][u] = yy[s * LPS + ee + u];
+              }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+              for (int s = 0; s < SPW; ++s) {
+                const bool in = ee + u < cns[s];
+                acc[s][g] = __builtin_amdgcn_mfma_f32_4x4x1f32(in ? x[s][u] : 0.0f, in ? y[s][u] : 0.0f, acc[s][g], 0, 0,
+                                                               0);
+              }
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < SPW; ++s) {
+    const int64_t ss = (int64_t)blockIdx.x * SPW + s;
+    if (ss < a.n) {
+#pragma unroll
+      for (int g = 0; g < kMfmaGroups; ++g) {
+        if (g < t.groups) {
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int k = t.tab[(6 * g + 2 + i) * kWave + lane];
+            if (k >= 0) a.Abuf[ss * a.dstride + k] = acc[s][g][i];
+          }
+        }
+      }
+    }
+  }
+  for (int off = LPS / 2; off > 0; off >>= 1) epair += __shfl_xor(epair, off, kWave);
+  if (valid && ll == 0) a.siteE[site] = epair;
 }
 
 // Products P_k = prod A[f], E = ref + pair + sum_k w_k P_k, and dE/dA
@@ -1368,6 +1522,8 @@ void ace_free(pfw_ctx* ctx) {
   dfree(a.rlb);
   dfree(a.dperm);
   dfree(a.dchunk);
+  dfree(a.mtab);
+  a.mgroups = 0;
   dfree(a.rl);
   dfree(a.w64);
   dfree(a.w32);
@@ -1627,6 +1783,67 @@ void ace_set_parameters(pfw_ctx* ctx, const double* WB, const double* Wpair) {
 
 // Densities grouped by chunks of `rchunk` radial indices for the density
 // kernel's radial tile (0: the built-in 40, capped at the radial count).
+// The matrix-core density tiles of k_ace_dens_mfma for the current radial
+// chunk: per chunk and l, the chunk's radial indices of degree l in groups of
+// 4 (rows) by the 2l+1 harmonics in groups of 4 (columns); 16 tiles to a
+// group, a new group for each chunk. mgroups = 0 when the tiles need more
+// than kMfmaGroups groups or the model is species-projected.
+static void ace_mfma_tiles(pfw_ctx* ctx) {
+  AceDev& a = ctx->ace;
+  dfree(a.mtab);
+  a.mgroups = 0;
+  const int nd = (int)(a.h_dens.size() / 2);
+  if (a.proj || nd == 0) return;
+  const int Lk = a.lmax <= 2 ? 2 : a.lmax <= 4 ? 4 : 6;  // the kernel's harmonic rows (dispatch_sites)
+  const int zcol = (Lk + 1) * (Lk + 1), zrow = a.rchunk;
+  // first density and degree of each radial index (m-blocks are contiguous and complete)
+  std::vector<int> first(a.radial.nr, -1), deg(a.radial.nr, -1);
+  for (int k = 0; k < nd; ++k) {
+    const int r = a.h_dens[2 * k], h = a.h_dens[2 * k + 1];
+    int l = 0;
+    while ((l + 1) * (l + 1) <= h) ++l;
+    if (h == l * l) first[r] = k;
+    deg[r] = l;
+  }
+  std::vector<int32_t> tab;
+  int groups = 0, in_group = 16;
+  int gch[kMfmaGroups] = {0, 0, 0, 0};
+  for (int r0 = 0, q = 0; r0 < a.radial.nr; r0 += a.rchunk, ++q) {
+    in_group = 16;  // every chunk starts a group
+    for (int l = 0; l <= a.lmax; ++l) {
+      std::vector<int> rs;
+      for (int r = r0; r < std::min(a.radial.nr, r0 + a.rchunk); ++r)
+        if (first[r] >= 0 && deg[r] == l) rs.push_back(r);
+      for (size_t rg = 0; rg < rs.size(); rg += 4)
+        for (int mg = 0; mg < 2 * l + 1; mg += 4) {
+          if (in_group == 16) {
+            if (groups == kMfmaGroups) return;
+            gch[groups++] = q;
+            in_group = 0;
+            const size_t base = tab.size();
+            tab.resize(base + 6 * kWave, -1);
+            for (int x = 0; x < kWave; ++x) {
+              tab[base + x] = zrow;
+              tab[base + kWave + x] = zcol;
+            }
+          }
+          int32_t* T = tab.data() + tab.size() - 6 * kWave;
+          const int b = in_group++;
+          for (int i = 0; i < 4; ++i) {
+            const bool ri = rg + i < rs.size(), ci = mg + i <= 2 * l;
+            T[4 * b + i] = ri ? rs[rg + i] - r0 : zrow;
+            T[kWave + 4 * b + i] = ci ? l * l + mg + i : zcol;
+            for (int j = 0; j < 4; ++j)  // register i of lane 4b + j: entry (i, j)
+              T[(2 + i) * kWave + 4 * b + j] = ri && mg + j <= 2 * l ? first[rs[rg + i]] + mg + j : -1;
+          }
+        }
+    }
+  }
+  a.mtab = upload(tab, ctx->stream);
+  a.mgroups = groups;
+  for (int g = 0; g < kMfmaGroups; ++g) a.mgchunk[g] = gch[g];
+}
+
 void ace_group_densities(pfw_ctx* ctx, int rchunk) {
   AceDev& a = ctx->ace;
   a.rchunk = std::max(1, std::min(a.radial.nr, rchunk > 0 ? rchunk : kRadialChunk));
@@ -1641,6 +1858,7 @@ void ace_group_densities(pfw_ctx* ctx, int rchunk) {
   dfree(a.dchunk);
   a.dperm = upload(dperm, ctx->stream);
   a.dchunk = upload(dchunk, ctx->stream);
+  ace_mfma_tiles(ctx);
 }
 
 void ace_set_model(pfw_ctx* ctx, const pfw_ace_model* m) {
@@ -1870,8 +2088,20 @@ static void launch_sites(pfw_ctx* ctx, int grads) {
   const size_t dstride = ad.proj ? 2 * (size_t)ad.ndens : (size_t)ad.ndens;
   const size_t lds_dens =
       sizeof(T) * ((size_t)NY + SPW * dstride + (size_t)(ad.rchunk + (ad.proj ? 2 : 1) * NY) * kStride);
-  allow_lds(k_ace_dens<T, L, SPW>, lds_dens, "the density kernel");
-  hipLaunchKernelGGL((k_ace_dens<T, L, SPW>), dim3(blocks), dim3(kWave), lds_dens, st, a);
+  bool mfma = false;
+  if constexpr (sizeof(T) == 4) {
+    mfma = ad.mgroups > 0 && (ctx->tune[4] == 2 || (ctx->tune[4] == 0 && kMfmaDensDefault));
+    if (mfma) {
+      MfmaTiles t{ad.mtab, ad.mgroups, {ad.mgchunk[0], ad.mgchunk[1], ad.mgchunk[2], ad.mgchunk[3]}};
+      const size_t lds_m = sizeof(float) * ((size_t)NY + (size_t)(ad.rchunk + 1 + NY + 1) * kStride);
+      allow_lds(k_ace_dens_mfma<L, SPW>, lds_m, "the density kernel");
+      hipLaunchKernelGGL((k_ace_dens_mfma<L, SPW>), dim3(blocks), dim3(kWave), lds_m, st, a, t);
+    }
+  }
+  if (!mfma) {
+    allow_lds(k_ace_dens<T, L, SPW>, lds_dens, "the density kernel");
+    hipLaunchKernelGGL((k_ace_dens<T, L, SPW>), dim3(blocks), dim3(kWave), lds_dens, st, a);
+  }
   PFW_HIP(hipGetLastError());
   size_t tables, per_wave;
   prod_lds<T>(ad, tables, per_wave);

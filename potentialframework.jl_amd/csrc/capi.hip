@@ -612,23 +612,25 @@ int pfw_graph_info(pfw_ctx* ctx, int64_t* out) {
 
 int pfw_set_tuning(pfw_ctx* ctx, const int32_t* knobs, int nknobs) {
   return run(ctx, [&] {
-    if (!knobs || nknobs < 0 || nknobs > 4) throw Error(PFW_INVALID_ARGUMENT, "knobs: at most 4 values");
-    int t[4] = {ctx->tune[0], ctx->tune[1], ctx->tune[2], ctx->tune[3]};
+    if (!knobs || nknobs < 0 || nknobs > 5) throw Error(PFW_INVALID_ARGUMENT, "knobs: at most 5 values");
+    int t[5] = {ctx->tune[0], ctx->tune[1], ctx->tune[2], ctx->tune[3], ctx->tune[4]};
     for (int k = 0; k < nknobs; ++k) t[k] = knobs[k];
     if (t[0] < 0 || t[0] > 2) throw Error(PFW_INVALID_ARGUMENT, "ACE sites per wavefront must be 0 (built-in), 1 or 2");
     if (t[1] < 0 || t[1] > 4096) throw Error(PFW_INVALID_ARGUMENT, "ACE radial chunk must be 0 (built-in) or 1..4096");
     if (t[2] < 0 || t[2] > 16) throw Error(PFW_INVALID_ARGUMENT, "ACE product wavefronts must be 0 (built-in) or 1..16");
     if (t[3] < 0 || t[3] > 4) throw Error(PFW_INVALID_ARGUMENT, "pair lanes per atom must be 0 (built-in) or 1..4");
+    if (t[4] < 0 || t[4] > 2)
+      throw Error(PFW_INVALID_ARGUMENT, "ACE density contraction must be 0 (built-in), 1 (VALU) or 2 (matrix cores)");
     PFW_HIP(hipStreamSynchronize(ctx->stream));
     const bool regroup = t[1] != ctx->tune[1];
-    for (int k = 0; k < 4; ++k) ctx->tune[k] = t[k];
+    for (int k = 0; k < 5; ++k) ctx->tune[k] = t[k];
     if (regroup && ctx->ace.ready) pfw::ace_group_densities(ctx, t[1]);
   });
 }
 
 int pfw_get_tuning(pfw_ctx* ctx, int32_t* knobs, int nknobs) {
   return run_keep(ctx, [&] {
-    if (!knobs || nknobs < 0 || nknobs > 4) throw Error(PFW_INVALID_ARGUMENT, "knobs: at most 4 values");
+    if (!knobs || nknobs < 0 || nknobs > 5) throw Error(PFW_INVALID_ARGUMENT, "knobs: at most 5 values");
     for (int k = 0; k < nknobs; ++k) knobs[k] = ctx->tune[k];
   });
 }

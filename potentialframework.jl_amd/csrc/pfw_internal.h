@@ -185,6 +185,13 @@ struct AceDev {
   int rchunk = 1;               // radial indices per chunk
   int32_t* rlb = nullptr;       // lmax + 2: radial indices grouped by l
   int32_t* rl = nullptr;        // (radial index, first density of its m-block)
+  // matrix-core density tiles (FP32, unprojected models): groups of 16 tiles
+  // of 4 radial indices x 4 harmonics, one group per MFMA; per group and lane
+  // the radial row, the harmonic column and the density of each of the 4
+  // accumulator registers (-1 none), [mgroups][6][64]
+  int32_t* mtab = nullptr;
+  int mgroups = 0;              // 0 when the model's tiles exceed the kernel's accumulators
+  int mgchunk[4] = {0, 0, 0, 0};  // radial chunk of each group
   int max_degree = 0;           // largest edge count of the last environment build
   double* w64 = nullptr;        // nz x nprod precontracted readout
   float* w32 = nullptr;
@@ -321,8 +328,9 @@ struct pfw_ctx {
   bool graph_warm = false;  // an eager step has run since the last reset (buffers sized)
   // launch-shape knobs (0 = the built-in choice): ACE sites per wavefront,
   // ACE radial indices per density tile, ACE product wavefronts per
-  // workgroup, most lanes per atom in the pair kernel
-  int tune[4] = {0, 0, 0, 0};
+  // workgroup, most lanes per atom in the pair kernel, ACE density
+  // contraction (1 VALU, 2 matrix cores)
+  int tune[5] = {0, 0, 0, 0, 0};
 };
 
 namespace pfw {

@@ -158,7 +158,8 @@ class Workspace:
         self.ctx.call("pfw_graph_info", ptr(out, ctypes.c_int64))
         return dict(captures=int(out[0]), replays=int(out[1]), held=bool(out[2]))
 
-    TUNING_KEYS = ("ace_sites_per_wave", "ace_radial_chunk", "ace_product_waves", "pair_lanes_per_atom")
+    TUNING_KEYS = ("ace_sites_per_wave", "ace_radial_chunk", "ace_product_waves", "pair_lanes_per_atom",
+                   "ace_density_contraction")
 
     def set_tuning(self, **knobs):
         """Launch-shape knobs (``pfw_set_tuning``); 0 selects the built-in choice."""
