@@ -722,8 +722,9 @@ def main():
     ap.add_argument("--profile", choices=("pair", "ace", "may30", "argon", "design"), default=None,
                     help="only run --steps device steps of one workload (for rocprofv3)")
     ap.add_argument("--precision", choices=("mixed", "float64"), default="mixed", help="precision of --profile")
-    ap.add_argument("--ace-density-contraction", type=int, choices=(0, 1, 2), default=0,
-                    help="ACE density contraction of --profile: 0 built-in, 1 VALU, 2 matrix cores")
+    ap.add_argument("--ace-density-contraction", type=int, choices=(0, 1, 2, 3), default=0,
+                    help="ACE contractions of --profile: 0 built-in, 1 VALU, 2 densities on matrix cores, "
+                         "3 densities and pullback on matrix cores")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
@@ -895,7 +896,7 @@ def main():
         # the density contraction on the VALU and on the matrix cores, same calculator
         dcw = ace["mixed"]["calc"].workspace
         dens_ms = {}
-        for name, dc in (("valu", 1), ("mfma", 2)):
+        for name, dc in (("valu", 1), ("mfma_densities", 2), ("mfma_densities_pullback", 3)):
             dcw.set_tuning(ace_density_contraction=dc)
             dsecs, dstage = time_steps(lib, dcw.ctx.h, lib.pfw_ace_step, args.steps, args.warmup, _capi.PFW_MIXED)
             dens_ms[name] = {"ms_per_step": round(1e3 * dsecs / args.steps, 4), "interaction_ms": round(float(dstage[4]), 5)}

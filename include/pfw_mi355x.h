@@ -208,9 +208,10 @@ int pfw_graph_info(pfw_ctx* ctx, int64_t* out);
  * wavefront (0 built-in, 1 or 2); knobs[1]: ACE radial indices per density
  * tile (0 built-in = 20, or 1..4096); knobs[2]: ACE product wavefronts per
  * workgroup (0 built-in, 1..16); knobs[3]: most lanes per atom in the pair
- * kernel (0 built-in = 4, 1..4); knobs[4]: ACE density contraction (0
- * built-in, 1 VALU, 2 matrix cores: FP32 steps of unprojected models whose
- * 4 x 4 density tiles fit four groups of 16, else VALU). Every setting gives
+ * kernel (0 built-in = 4, 1..4); knobs[4]: ACE contractions (0 built-in,
+ * 1 VALU, 2 densities on the matrix cores, 3 densities and pullback on the
+ * matrix cores; the matrix cores serve FP32 steps of unprojected models, the
+ * densities only when their 4 x 4 tiles fit four groups of 16). Every setting gives
  * the same pair sets and results to rounding. nknobs <= 5 values are read or
  * written from the first; PFW_INVALID_ARGUMENT for a value outside its
  * range. */

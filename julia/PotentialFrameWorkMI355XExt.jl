@@ -83,8 +83,8 @@ end
 """Launch knobs of GPUConfig's strategy/workgroup/tile (gpu.jl:61-73) and the ACE
 strategy (ACE/gpu.jl:301-303); 0 keeps the built-in choice: ACE sites per
 wavefront, ACE radial tile, ACE product wavefronts, pair lanes per atom, ACE
-density contraction (1 VALU, 2 matrix cores) (pfw_set_tuning; tune.py: autotune
-sweeps them)."""
+contractions (1 VALU, 2 densities on matrix cores, 3 densities and pullback)
+(pfw_set_tuning; tune.py: autotune sweeps them)."""
 function set_tuning!(w::MI355XWorkspace; ace_sites_per_wave=0, ace_radial_chunk=0, ace_product_waves=0,
                      pair_lanes_per_atom=0, ace_density_contraction=0)
     k = Int32[ace_sites_per_wave, ace_radial_chunk, ace_product_waves, pair_lanes_per_atom, ace_density_contraction]

@@ -35,6 +35,7 @@ constexpr int kRadialChunk = 20;  // radial indices per density-kernel tile
 // the built-in density contraction: the matrix cores where the model's tiles
 // fit (FP32, unprojected); 32k-atom BCC, lmax 4: 127.9 -> 124.0 us per launch
 constexpr bool kMfmaDensDefault = true;
+constexpr bool kMfmaPullDefault = false;  // the built-in pullback contraction: VALU
 
 template <typename T>
 struct FamArgs {
@@ -829,6 +830,133 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
     });
     const T g0 = gs * dx * ri + h0, g1 = gs * dy * ri + h1, g2 = gs * dz * ri + h2;
     T* go = a.eg + 3 * (int64_t)(e0 + e);
+    go[0] = g0;
+    go[1] = g1;
+    go[2] = g2;
+    const double gg[3] = {(double)g0, (double)g1, (double)g2};
+    for (int bb = 0; bb < 3; ++bb)
+      for (int aa = 0; aa < 3; ++aa) wv[aa + 3 * bb] -= R[aa] * gg[bb];
+  }
+  for (int k = 0; k < 9; ++k) {
+    double v = wv[k];
+    for (int off = LPS / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    if (valid && ll == 0) a.siteW[9 * site + k] = v;
+  }
+}
+
+// The pullback's contractions U_m = sum_r dE/dA[r, m] R_r and V_m = sum_r
+// dE/dA[r, m] R'_r on the matrix cores (FP32, unprojected models). For each
+// l, each group of 4 harmonics m0..m0+3 and each radial index r of degree l,
+// one v_mfma_f32_4x4x1_16b_f32 takes A = the site's dE/dA[r, m0 + i] from
+// lane 4b+i and B = R_r of the edge of lane 4b+j, so register i of every lane
+// accumulates U_{m0+i} of the lane's own edge (no transposes); a second MFMA
+// with B = R'_r gives V. The four lanes of a block share a site (LPS is a
+// multiple of 4), so they share the weights. Lanes past their site's edges
+// supply zero radials. The rest (pair radials, Y_lm jets, gradients and
+// virials) is k_ace_pull's.
+__host__ __device__ constexpr int mgroups_below(int l) { return l <= 0 ? 0 : mgroups_below(l - 1) + (2 * (l - 1) + 1 + 3) / 4; }
+
+template <int L, int SPW>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(L <= 4 ? 3 : 1))) void k_ace_pull_mfma(AceArgs<float> a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int NY = (L + 1) * (L + 1);
+  constexpr int LPS = kWave / SPW;
+  constexpr int NG = mgroups_below(L + 1);
+  float* Fl = (float*)smem;   // (L+1)^2
+  float* dA = Fl + NY;        // SPW x dstride
+  const int lane = threadIdx.x;
+  const int sl = lane / LPS, ll = lane % LPS;
+  const int64_t site = (int64_t)blockIdx.x * SPW + sl;
+  const bool valid = site < a.n;
+  fill_fl<float, L>(Fl);
+  float* dAs = dA + sl * a.dstride;
+  if (valid)
+    for (int k = ll; k < a.dstride; k += LPS) dAs[k] = a.Abuf[site * a.dstride + k];
+  const int iz = valid ? a.sps[site] : 0;
+  const int64_t e0 = valid ? site * a.ecap : 0;
+  const int ne = valid ? a.ecount[site] : 0;
+  int nemax = ne;
+  for (int off = LPS; off < kWave; off <<= 1) nemax = max(nemax, __shfl_xor(nemax, off, kWave));
+  const int nq = a.pair.nr;
+  const int li = lane & 3;
+  __syncthreads();
+  double wv[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int c0 = 0; c0 < nemax; c0 += LPS) {
+    const int e = c0 + ll;
+    const bool on = e < ne;
+    const double* R = a.eR + 3 * (int64_t)(e0 + (on ? e : 0));
+    float dx = 0, dy = 0, dz = 0, r = 1;
+    int jz = 0, ch = 0;
+    RadialPoint<float> P{};
+    if (on) {
+      dx = (float)R[0];
+      dy = (float)R[1];
+      dz = (float)R[2];
+      r = (float)sqrt(R[0] * R[0] + R[1] * R[1] + R[2] * R[2]);
+      jz = a.sps[a.enb[e0 + e]];
+      ch = iz + a.nz * jz;
+      P = radial_point(a.rad, ch, r);
+    }
+    mfma_v4 aU[NG], aV[NG];
+#pragma unroll
+    for (int g = 0; g < NG; ++g) aU[g] = aV[g] = mfma_v4{0, 0, 0, 0};
+    // four radial indices at a time (unrolled by hand: the compiler does not
+    // unroll loops around the MFMA builtin), their coefficient loads ahead of
+    // the MFMAs; a lane past its edges has a zero envelope (P = {}), so its
+    // radials are zero, and an index past the degree's last repeats the last
+    // with zero weights
+#pragma unroll
+    for (int l = 0; l <= L; ++l) {
+      const int q1 = a.rlb[l + 1];
+      for (int q = a.rlb[l]; q < q1; q += 4) {
+        float v[4], d[4];
+        int a0s[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int qq = min(q + u, q1 - 1);
+          a0s[u] = a.rl[2 * qq + 1];
+          radial_value(a.rad, ch, P, a.rl[2 * qq], v[u], d[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const bool ok = q + u < q1;
+#pragma unroll
+          for (int mg = 0; mg < (2 * l + 1 + 3) / 4; ++mg) {
+            const float w = ok && 4 * mg + li <= 2 * l ? dAs[a0s[u] + 4 * mg + li] : 0.0f;
+            aU[mgroups_below(l) + mg] = __builtin_amdgcn_mfma_f32_4x4x1f32(w, v[u], aU[mgroups_below(l) + mg], 0, 0, 0);
+            aV[mgroups_below(l) + mg] = __builtin_amdgcn_mfma_f32_4x4x1f32(w, d[u], aV[mgroups_below(l) + mg], 0, 0, 0);
+          }
+        }
+      }
+    }
+    if (!on) continue;
+    float U[NY], V[NY];
+#pragma unroll
+    for (int l = 0; l <= L; ++l)
+#pragma unroll
+      for (int mi = 0; mi <= 2 * l; ++mi) {
+        U[l * l + mi] = aU[mgroups_below(l) + mi / 4][mi % 4];
+        V[l * l + mi] = aV[mgroups_below(l) + mi / 4][mi % 4];
+      }
+    float gs = 0;
+    if (nq && ((a.pair_gate >> jz) & 1u)) {
+      const RadialPoint<float> PQ = radial_point(a.pair, ch, r);
+      for (int n = 0; n < nq; ++n) {
+        float v, d;
+        radial_value(a.pair, ch, PQ, n, v, d);
+        gs += a.wp[iz * nq + n] * d;
+      }
+    }
+    const float ri = 1.0f / r;
+    float h0 = 0, h1 = 0, h2 = 0;
+    ylm_emit<float, L>(dx, dy, dz, Fl, a.solid, [&](int i, const Jet<float>& j) {
+      gs += V[i] * j.v;
+      h0 += U[i] * j.g0;
+      h1 += U[i] * j.g1;
+      h2 += U[i] * j.g2;
+    });
+    const float g0 = gs * dx * ri + h0, g1 = gs * dy * ri + h1, g2 = gs * dz * ri + h2;
+    float* go = a.eg + 3 * (int64_t)(e0 + e);
     go[0] = g0;
     go[1] = g1;
     go[2] = g2;
@@ -2090,7 +2218,7 @@ static void launch_sites(pfw_ctx* ctx, int grads) {
       sizeof(T) * ((size_t)NY + SPW * dstride + (size_t)(ad.rchunk + (ad.proj ? 2 : 1) * NY) * kStride);
   bool mfma = false;
   if constexpr (sizeof(T) == 4) {
-    mfma = ad.mgroups > 0 && (ctx->tune[4] == 2 || (ctx->tune[4] == 0 && kMfmaDensDefault));
+    mfma = ad.mgroups > 0 && (ctx->tune[4] >= 2 || (ctx->tune[4] == 0 && kMfmaDensDefault));
     if (mfma) {
       MfmaTiles t{ad.mtab, ad.mgroups, {ad.mgchunk[0], ad.mgchunk[1], ad.mgchunk[2], ad.mgchunk[3]}};
       const size_t lds_m = sizeof(float) * ((size_t)NY + (size_t)(ad.rchunk + 1 + NY + 1) * kStride);
@@ -2124,8 +2252,18 @@ static void launch_sites(pfw_ctx* ctx, int grads) {
   PFW_HIP(hipGetLastError());
   if (grads) {
     const size_t lds_pull = sizeof(T) * ((size_t)NY + SPW * dstride);
-    allow_lds(k_ace_pull<T, L, SPW>, lds_pull, "the pullback kernel");
-    hipLaunchKernelGGL((k_ace_pull<T, L, SPW>), dim3(blocks), dim3(kWave), lds_pull, st, a);
+    bool mpull = false;
+    if constexpr (sizeof(T) == 4) {
+      mpull = !ad.proj && (ctx->tune[4] == 3 || (ctx->tune[4] == 0 && kMfmaPullDefault));
+      if (mpull) {
+        allow_lds(k_ace_pull_mfma<L, SPW>, lds_pull, "the pullback kernel");
+        hipLaunchKernelGGL((k_ace_pull_mfma<L, SPW>), dim3(blocks), dim3(kWave), lds_pull, st, a);
+      }
+    }
+    if (!mpull) {
+      allow_lds(k_ace_pull<T, L, SPW>, lds_pull, "the pullback kernel");
+      hipLaunchKernelGGL((k_ace_pull<T, L, SPW>), dim3(blocks), dim3(kWave), lds_pull, st, a);
+    }
     PFW_HIP(hipGetLastError());
   }
 }

@@ -619,8 +619,10 @@ int pfw_set_tuning(pfw_ctx* ctx, const int32_t* knobs, int nknobs) {
     if (t[1] < 0 || t[1] > 4096) throw Error(PFW_INVALID_ARGUMENT, "ACE radial chunk must be 0 (built-in) or 1..4096");
     if (t[2] < 0 || t[2] > 16) throw Error(PFW_INVALID_ARGUMENT, "ACE product wavefronts must be 0 (built-in) or 1..16");
     if (t[3] < 0 || t[3] > 4) throw Error(PFW_INVALID_ARGUMENT, "pair lanes per atom must be 0 (built-in) or 1..4");
-    if (t[4] < 0 || t[4] > 2)
-      throw Error(PFW_INVALID_ARGUMENT, "ACE density contraction must be 0 (built-in), 1 (VALU) or 2 (matrix cores)");
+    if (t[4] < 0 || t[4] > 3)
+      throw Error(PFW_INVALID_ARGUMENT,
+                  "ACE contractions must be 0 (built-in), 1 (VALU), 2 (densities on matrix cores) or 3 (densities and "
+                  "pullback on matrix cores)");
     PFW_HIP(hipStreamSynchronize(ctx->stream));
     const bool regroup = t[1] != ctx->tune[1];
     for (int k = 0; k < 5; ++k) ctx->tune[k] = t[k];

@@ -328,8 +328,8 @@ struct pfw_ctx {
   bool graph_warm = false;  // an eager step has run since the last reset (buffers sized)
   // launch-shape knobs (0 = the built-in choice): ACE sites per wavefront,
   // ACE radial indices per density tile, ACE product wavefronts per
-  // workgroup, most lanes per atom in the pair kernel, ACE density
-  // contraction (1 VALU, 2 matrix cores)
+  // workgroup, most lanes per atom in the pair kernel, ACE contractions (1
+  // VALU, 2 densities on matrix cores, 3 densities and pullback)
   int tune[5] = {0, 0, 0, 0, 0};
 };
 

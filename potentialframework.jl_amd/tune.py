@@ -29,7 +29,7 @@ from .neighbourlist import Workspace
 
 PAIR_SPACE = {"pair_lanes_per_atom": (4, 3, 2)}
 ACE_SPACE = {"ace_sites_per_wave": (2, 1), "ace_radial_chunk": (40, 20, 10), "ace_product_waves": (16, 8, 4),
-             "ace_density_contraction": (1, 2)}
+             "ace_density_contraction": (1, 2, 3)}
 
 
 def _cache_path(cache):

@@ -173,11 +173,12 @@ def test_every_knob_setting_matches_oracle(pfw, synthetic, prec):
     s = pfw.System(X, H, pbc, ace_labels(pfw, model, sp))
     calc = pfw.ACECalculator(model, ps, precision=prec)
     # radial chunks 40 and 13 give 2 and 4 groups of matrix-core density
-    # tiles; 7 and 1 exceed the kernel's 4 groups and fall back to the VALU
+    # tiles; 7 and 1 exceed the kernel's 4 groups and fall back to the VALU;
+    # dc 3 also runs the pullback on the matrix cores
     for spw in (1, 2):
         for rchunk in (40, 13, 7, 1):
             for waves in (16, 3, 1):
-                for dc in (1, 2):
+                for dc in (1, 2, 3):
                     calc.workspace.set_tuning(ace_sites_per_wave=spw, ace_radial_chunk=rchunk, ace_product_waves=waves,
                                               ace_density_contraction=dc)
                     r = calc.evaluate(s)
@@ -205,7 +206,7 @@ def test_every_knob_setting_matches_oracle(pfw, synthetic, prec):
     with pytest.raises(pfw.ArgumentError):
         w.set_tuning(ace_sites_per_wave=3)
     with pytest.raises(pfw.ArgumentError):
-        w.set_tuning(ace_density_contraction=3)
+        w.set_tuning(ace_density_contraction=4)
     with pytest.raises(pfw.ArgumentError):
         w.set_tuning(bogus=1)
 
@@ -217,7 +218,7 @@ def test_autotune_sweeps_and_caches(pfw, synthetic, tmp_path):
     cache = str(tmp_path / "tuning.json")
     calc = pfw.ACECalculator(model, ps, precision="mixed")
     rep = pfw.autotune(calc, s, steps=3, warmup=1, cache=cache)
-    assert not rep["cached"] and len(rep["candidates"]) == 2 * 2 * 3 * 3 * 2
+    assert not rep["cached"] and len(rep["candidates"]) == 2 * 2 * 3 * 3 * 3
     print("ACE autotune", rep["best"], rep["graphs"], f"{rep['ms']:.3f} ms")
     assert json.load(open(cache))[rep["key"]]["knobs"] == rep["best"]
     E, F, W, _ = oracle_ace(X, sp, H, pbc, om)
