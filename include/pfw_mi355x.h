@@ -211,7 +211,9 @@ int pfw_graph_info(pfw_ctx* ctx, int64_t* out);
  * kernel (0 built-in = 4, 1..4); knobs[4]: ACE contractions (0 built-in,
  * 1 VALU, 2 densities on the matrix cores, 3 densities and pullback on the
  * matrix cores; the matrix cores serve FP32 steps of unprojected models, the
- * densities only when their 4 x 4 tiles fit four groups of 16). Every setting gives
+ * densities only when their 4 x 4 tiles fit four groups of 16; the built-in
+ * takes them for the densities when at least half of the tile entries are
+ * densities). Every setting gives
  * the same pair sets and results to rounding. nknobs <= 5 values are read or
  * written from the first; PFW_INVALID_ARGUMENT for a value outside its
  * range. */

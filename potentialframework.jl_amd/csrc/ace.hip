@@ -33,8 +33,10 @@ namespace pfw {
 constexpr int kStride = kWave + 1;
 constexpr int kRadialChunk = 20;  // radial indices per density-kernel tile
 // the built-in density contraction: the matrix cores where the model's tiles
-// fit (FP32, unprojected); 32k-atom BCC, lmax 4: 127.9 -> 124.0 us per launch
-constexpr bool kMfmaDensDefault = true;
+// fit (FP32, unprojected) and at least half of the tile entries are
+// densities. 32k-atom BCC, lmax 4 (69 % useful): 127.9 -> 124.0 us per
+// launch; the May30 model (40 % useful) ran slower on the matrix cores
+// (interaction 1.17 -> 1.22 ms per step)
 constexpr bool kMfmaPullDefault = false;  // the built-in pullback contraction: VALU
 
 template <typename T>
@@ -1920,6 +1922,7 @@ static void ace_mfma_tiles(pfw_ctx* ctx) {
   AceDev& a = ctx->ace;
   dfree(a.mtab);
   a.mgroups = 0;
+  a.mdefault = false;
   const int nd = (int)(a.h_dens.size() / 2);
   if (a.proj || nd == 0) return;
   const int Lk = a.lmax <= 2 ? 2 : a.lmax <= 4 ? 4 : 6;  // the kernel's harmonic rows (dispatch_sites)
@@ -1934,7 +1937,7 @@ static void ace_mfma_tiles(pfw_ctx* ctx) {
     deg[r] = l;
   }
   std::vector<int32_t> tab;
-  int groups = 0, in_group = 16;
+  int groups = 0, in_group = 16, tiles = 0;
   int gch[kMfmaGroups] = {0, 0, 0, 0};
   for (int r0 = 0, q = 0; r0 < a.radial.nr; r0 += a.rchunk, ++q) {
     in_group = 16;  // every chunk starts a group
@@ -1957,6 +1960,7 @@ static void ace_mfma_tiles(pfw_ctx* ctx) {
           }
           int32_t* T = tab.data() + tab.size() - 6 * kWave;
           const int b = in_group++;
+          ++tiles;
           for (int i = 0; i < 4; ++i) {
             const bool ri = rg + i < rs.size(), ci = mg + i <= 2 * l;
             T[4 * b + i] = ri ? rs[rg + i] - r0 : zrow;
@@ -1969,6 +1973,7 @@ static void ace_mfma_tiles(pfw_ctx* ctx) {
   }
   a.mtab = upload(tab, ctx->stream);
   a.mgroups = groups;
+  a.mdefault = 2 * nd >= 16 * tiles;
   for (int g = 0; g < kMfmaGroups; ++g) a.mgchunk[g] = gch[g];
 }
 
@@ -2218,7 +2223,7 @@ static void launch_sites(pfw_ctx* ctx, int grads) {
       sizeof(T) * ((size_t)NY + SPW * dstride + (size_t)(ad.rchunk + (ad.proj ? 2 : 1) * NY) * kStride);
   bool mfma = false;
   if constexpr (sizeof(T) == 4) {
-    mfma = ad.mgroups > 0 && (ctx->tune[4] >= 2 || (ctx->tune[4] == 0 && kMfmaDensDefault));
+    mfma = ad.mgroups > 0 && (ctx->tune[4] >= 2 || (ctx->tune[4] == 0 && ad.mdefault));
     if (mfma) {
       MfmaTiles t{ad.mtab, ad.mgroups, {ad.mgchunk[0], ad.mgchunk[1], ad.mgchunk[2], ad.mgchunk[3]}};
       const size_t lds_m = sizeof(float) * ((size_t)NY + (size_t)(ad.rchunk + 1 + NY + 1) * kStride);

@@ -191,6 +191,7 @@ struct AceDev {
   // accumulator registers (-1 none), [mgroups][6][64]
   int32_t* mtab = nullptr;
   int mgroups = 0;              // 0 when the model's tiles exceed the kernel's accumulators
+  bool mdefault = false;        // tiles at least half useful: the built-in density contraction
   int mgchunk[4] = {0, 0, 0, 0};  // radial chunk of each group
   int max_degree = 0;           // largest edge count of the last environment build
   double* w64 = nullptr;        // nz x nprod precontracted readout
