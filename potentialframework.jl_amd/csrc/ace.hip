@@ -990,6 +990,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(L <= 4 ? 3 :
 // neighbour loads) is thereby spread over the wavefronts; splitting the
 // centres instead, each wavefront staging every candidate, was measured twice
 // as slow as one wavefront per cell.
+// k_ace_edges's status: words 0..3 as the host reads them (0 failure flags,
+// 1 largest degree, 2 capacity overflow, 3 largest in-cutoff degree of skin
+// rows), then kStatusStripes stripes of (1, 2, 3) that the kernel's atomics
+// go to and fold_status() folds into words 1..3 on the host
+constexpr int kStatusStripes = 32;
+constexpr int kStatusWords = 4 + 3 * kStatusStripes;
 constexpr int kEdgeWaves = 4;
 constexpr int kEdgeRound = 1;                         // chunks of one wavefront held at once
 constexpr int kEdgeChunks = kEdgeWaves * kEdgeRound;  // chunks of one round
@@ -1202,9 +1208,12 @@ __global__ __launch_bounds__(kWave * kEdgeWaves) __attribute__((amdgpu_waves_per
         cmax = max(cmax, __shfl_xor(cmax, off, kWave));
       }
       if (lane == 0) {
-        atomicMax(status + 1, dmax);
-        if (dmax > ecap) atomicOr(status + 2, 1);  // capacity too small: the host repeats the step
-        if (skin_rows) atomicMax(status + 3, cmax);
+        // into one of kStatusStripes stripes (the host folds them): same-address
+        // device atomics serialize, and every workgroup issues these
+        int32_t* sw = status + 4 + 3 * (blockIdx.x % kStatusStripes);
+        atomicMax(sw, dmax);
+        if (dmax > ecap) atomicOr(sw + 1, 1);  // capacity too small: the host repeats the step
+        if (skin_rows) atomicMax(sw + 2, cmax);
       }
     }
     __syncthreads();  // the centres' staging and counters are rewritten by the next group
@@ -2316,6 +2325,16 @@ static int uniform_stencil(const pfw_ctx* ctx) {
   return ns > 0 ? ns : 0;
 }
 
+// Words 1..3 of k_ace_edges's status from its stripes (host copy).
+static void fold_status(int32_t* h) {
+  for (int k = 0; k < kStatusStripes; ++k) {
+    const int32_t* w = h + 4 + 3 * k;
+    h[1] = std::max(h[1], w[0]);
+    h[2] |= w[1];
+    h[3] = std::max(h[3], w[2]);
+  }
+}
+
 static void ace_environment(pfw_ctx* ctx, int rev) {
   AceDev& ad = ctx->ace;
   SystemBuffers& s = ctx->sys;
@@ -2323,19 +2342,20 @@ static void ace_environment(pfw_ctx* ctx, int rev) {
   const int64_t n = s.n;
   ace_rows_alloc(ctx, std::max(ad.ecap, 1));
   if (ad.ecap == 0) {  // size the rows: one counting pass
-    PFW_HIP(hipMemsetAsync(ad.status, 0, 4 * sizeof(int32_t), st));
+    PFW_HIP(hipMemsetAsync(ad.status, 0, kStatusWords * sizeof(int32_t), st));
     hipLaunchKernelGGL(k_ace_edges, dim3((unsigned)ctx->ncells), dim3(kWave * kEdgeWaves), 0, st, s.grids, s.cell_cfg, n, s.xs,
                        s.start, 0, 0, ad.ecount, (int32_t*)nullptr, (int32_t*)nullptr, (double*)nullptr, ad.status, 0,
                        (int32_t*)nullptr, (uint8_t*)nullptr, (int)(ctx->ncfg == 1));
     PFW_HIP(hipGetLastError());
-    PFW_HIP(hipMemcpyAsync(ad.h_status, ad.status, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    PFW_HIP(hipMemcpyAsync(ad.h_status, ad.status, kStatusWords * sizeof(int32_t), hipMemcpyDeviceToHost, st));
     PFW_HIP(hipStreamSynchronize(st));
+    fold_status(ad.h_status);
     ad.max_degree = ad.h_status[1];
     ad.ecap = std::max(1, ad.max_degree + std::max(4, ad.max_degree / 8));
   }
   ace_rows_alloc(ctx, ad.ecap);
   const int64_t slots = n * (int64_t)ad.ecap;
-  fill_i32(ad.status, 4, 0, st);  // a fill kernel: this sequence may be captured in a graph
+  fill_i32(ad.status, kStatusWords, 0, st);  // a fill kernel: this sequence may be captured in a graph
   hipLaunchKernelGGL(k_ace_edges, dim3((unsigned)ctx->ncells), dim3(kWave * kEdgeWaves), 0, st, s.grids, s.cell_cfg, n, s.xs,
                      s.start, 1, ad.ecap, ad.ecount, ad.enb, ad.eent, ad.eR, ad.status, 0, rev ? ad.erev : nullptr,
                      rev ? ad.eseg : nullptr, (int)(ctx->ncfg == 1));
@@ -2344,12 +2364,13 @@ static void ace_environment(pfw_ctx* ctx, int rev) {
     hipLaunchKernelGGL(k_ace_rev, dim3(nblk(slots, 256)), dim3(256), 0, st, slots, ad.ecap, s.grids, s.atom_cfg,
                        s.perm, ad.ecount, ad.enb, ad.eent, ad.eR, ad.erev, ad.status, (const uint8_t*)ad.eseg,
                        uniform_stencil(ctx));
-  PFW_HIP(hipMemcpyAsync(ad.h_status, ad.status, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  PFW_HIP(hipMemcpyAsync(ad.h_status, ad.status, kStatusWords * sizeof(int32_t), hipMemcpyDeviceToHost, st));
 }
 
 // After the step's synchronisation: true when some site had more edges than
 // its row holds; ecap then grows to the degree found, and the step is repeated.
 static bool ace_overflowed(AceDev& ad) {
+  fold_status(ad.h_status);
   ad.max_degree = ad.h_status[1];
   if (!ad.h_status[2]) return false;
   ad.ecap = ad.max_degree + std::max(4, ad.max_degree / 8);
@@ -2393,8 +2414,8 @@ static void ace_rows_alloc(pfw_ctx* ctx, int ecap) {
     PFW_HIP(hipMalloc(&ad.Abuf, std::max<size_t>(1, (size_t)n * ad.ndens * (ad.proj ? 2 : 1)) * sizeof(double)));
     PFW_HIP(hipMalloc(&ad.siteW, 9 * n * sizeof(double)));
     PFW_HIP(hipMalloc(&ad.rec, 10 * n * sizeof(double)));
-    PFW_HIP(hipMalloc(&ad.status, 4 * sizeof(int32_t)));
-    PFW_HIP(hipHostMalloc(&ad.h_status, 4 * sizeof(int32_t), hipHostMallocDefault));
+    PFW_HIP(hipMalloc(&ad.status, kStatusWords * sizeof(int32_t)));
+    PFW_HIP(hipHostMalloc(&ad.h_status, kStatusWords * sizeof(int32_t), hipHostMallocDefault));
     PFW_HIP(hipMalloc(&ad.eseg, std::max<int64_t>(n, 1) * kSegEntries));
   }
   const int64_t slots = n * (int64_t)ecap;
@@ -2436,7 +2457,7 @@ static void ace_skin_rebuild(pfw_ctx* ctx) {
     ad.parity = 0;
   }
   auto build = [&](int pass) {
-    PFW_HIP(hipMemsetAsync(ad.status, 0, 4 * sizeof(int32_t), st));
+    PFW_HIP(hipMemsetAsync(ad.status, 0, kStatusWords * sizeof(int32_t), st));
     hipLaunchKernelGGL(k_ace_edges, dim3((unsigned)ctx->ncells), dim3(kWave * kEdgeWaves), 0, st, s.grids, s.cell_cfg, n, s.xs,
                        s.start, pass, pass ? ad.ecap_s : 0, ad.ecount_s, ad.enb_s, ad.eent_s, ad.eR_s, ad.status, 1,
                        pass ? ad.erev_s : nullptr, pass ? ad.eseg_s : nullptr, (int)(ctx->ncfg == 1));
@@ -2446,8 +2467,9 @@ static void ace_skin_rebuild(pfw_ctx* ctx) {
                          ad.ecap_s, s.grids, s.atom_cfg, s.perm, ad.ecount_s, ad.enb_s, ad.eent_s, ad.eR_s, ad.erev_s,
                          ad.status, (const uint8_t*)ad.eseg_s, uniform_stencil(ctx));
     PFW_HIP(hipGetLastError());
-    PFW_HIP(hipMemcpyAsync(ad.h_status, ad.status, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    PFW_HIP(hipMemcpyAsync(ad.h_status, ad.status, kStatusWords * sizeof(int32_t), hipMemcpyDeviceToHost, st));
     PFW_HIP(hipStreamSynchronize(st));
+    fold_status(ad.h_status);
   };
   auto size_rows = [&](int degree) {
     ad.ecap_s = std::max(ad.ecap_s, degree + std::max(4, degree / 8));
@@ -2508,7 +2530,7 @@ static void ace_skin_step(pfw_ctx* ctx, int grads, int precision) {
     // sites per wavefront follow the largest in-cutoff degree at the build (any
     // degree is correct; a smaller one only packs two sites per wavefront)
     ad.max_degree = ad.compact_degree;
-    PFW_HIP(hipMemsetAsync(ad.status, 0, 4 * sizeof(int32_t), st));
+    PFW_HIP(hipMemsetAsync(ad.status, 0, kStatusWords * sizeof(int32_t), st));
     hipLaunchKernelGGL(k_ace_delta, dim3(nblk(n, 256)), dim3(256), 0, st, n, s.perm, s.X, ad.Xref, ad.dX, ad.maxd2,
                        ad.parity);
     hipLaunchKernelGGL(k_ace_compact, dim3((unsigned)n), dim3(kWave), 0, st, n, ad.ecap, ad.ecount_s, ad.enb_s,
